@@ -1,0 +1,940 @@
+// Host side of libcubbs_bbs_hip.so: the C ABI v2 entry points, the
+// device-resident state (generator comb tables, prepared issuer public keys,
+// prepared -G2) and the batch planner that turns ABI items into the flat
+// job arrays the kernels consume.
+//
+// Nothing here computes on the CPU: every curve, field and hash operation runs
+// in the kernels of bbs_kernels.hip.  When no device is usable the entry points
+// return CUBBS_BBS_GPU_ERR_INTERNAL (there is no CPU fallback).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/cubbs_bbs_gpu_abi.h"
+#include "../../include/cubbs_hip_ext.h"
+#include "bbs_internal.h"
+
+namespace cubbs {
+
+struct GpuError {
+  hipError_t err;
+};
+static void check(hipError_t e) {
+  if (e != hipSuccess) throw GpuError{e};
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static uint32_t env_u32(const char* name, uint32_t def) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return def;
+  char* end = nullptr;
+  unsigned long x = std::strtoul(v, &end, 10);
+  return end == v ? def : (uint32_t)x;
+}
+
+// ------------------------------------------------------------------ device context
+struct Context {
+  std::mutex mu;
+  bool ready = false;
+  bool failed = false;
+  hipStream_t setup = nullptr;
+  // generators: slot 0 = P1, slot 1 = Q1, slot 1 + j = H_j
+  uint32_t gen_slots = 0, gen_cap = 0;
+  uint32_t* gen_affine = nullptr;
+  uint32_t* gen_tables = nullptr;
+  uint8_t* gen_comp = nullptr;
+  uint32_t* gen_vstate = nullptr;
+  // prepared public keys
+  std::unordered_map<std::string, uint32_t> pk_slots;
+  uint32_t pk_count = 0, pk_cap = 0;
+  uint8_t* pk_bytes = nullptr;
+  uint8_t* pk_status = nullptr;
+  uint32_t* pk_lines = nullptr;
+  uint32_t* neg_g2_lines = nullptr;
+  // buffers replaced by a growth step stay allocated: batches that snapshot
+  // them may still be in flight on other streams
+  std::vector<void*> retired;
+};
+
+static Context& ctx() {
+  static Context* c = new Context();
+  return *c;
+}
+
+// Must be called with ctx().mu held.
+static void init_locked(Context& c) {
+  if (c.ready) return;
+  if (c.failed) throw GpuError{hipErrorNoDevice};
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    c.failed = true;
+    throw GpuError{hipErrorNoDevice};
+  }
+  check(hipStreamCreateWithFlags(&c.setup, hipStreamNonBlocking));
+  check(hipMalloc(&c.gen_vstate, 64));
+  check(hipMalloc(&c.neg_g2_lines, PK_LINES_WORDS * 4));
+  launch_prepare_neg_g2(c.setup, c.neg_g2_lines);
+  check(hipGetLastError());
+  check(hipStreamSynchronize(c.setup));
+  c.ready = true;
+}
+
+// Ensure generator slots [0, slots) exist (tables included).  ctx().mu held.
+static void ensure_generators_locked(Context& c, uint32_t slots) {
+  if (slots <= c.gen_slots) return;
+  if (slots > c.gen_cap) {
+    uint32_t cap = std::max<uint32_t>({slots, c.gen_cap * 2, 64u});
+    uint32_t* aff = nullptr;
+    uint32_t* tab = nullptr;
+    uint8_t* comp = nullptr;
+    check(hipMalloc(&aff, (size_t)cap * G1A_WORDS * 4));
+    check(hipMalloc(&tab, (size_t)cap * COMB_TABLE_WORDS * 4));
+    check(hipMalloc(&comp, (size_t)cap * 48));
+    if (c.gen_slots) {
+      check(hipMemcpyAsync(aff, c.gen_affine, (size_t)c.gen_slots * G1A_WORDS * 4,
+                           hipMemcpyDeviceToDevice, c.setup));
+      check(hipMemcpyAsync(tab, c.gen_tables, (size_t)c.gen_slots * COMB_TABLE_WORDS * 4,
+                           hipMemcpyDeviceToDevice, c.setup));
+      check(hipMemcpyAsync(comp, c.gen_comp, (size_t)c.gen_slots * 48, hipMemcpyDeviceToDevice,
+                           c.setup));
+    }
+    if (c.gen_affine) {
+      c.retired.push_back(c.gen_affine);
+      c.retired.push_back(c.gen_tables);
+      c.retired.push_back(c.gen_comp);
+    }
+    c.gen_affine = aff;
+    c.gen_tables = tab;
+    c.gen_comp = comp;
+    c.gen_cap = cap;
+  }
+  uint32_t first_chain = c.gen_slots == 0 ? 1 : c.gen_slots;
+  uint32_t count = slots - first_chain;
+  if (count) {
+    uint32_t* seeds = nullptr;
+    check(hipMalloc(&seeds, (size_t)count * 12 * 4));
+    launch_gen_seeds(c.setup, first_chain, count, c.gen_vstate, seeds);
+    launch_gen_points(c.setup, first_chain, count, seeds, c.gen_affine, c.gen_comp);
+    check(hipGetLastError());
+    check(hipStreamSynchronize(c.setup));
+    check(hipFree(seeds));
+  }
+  uint32_t first_table = c.gen_slots;
+  uint32_t tcount = slots - first_table;
+  // build tables in slices to bound the scratch buffer
+  const uint32_t slice = 64;
+  uint32_t* scratch = nullptr;
+  check(hipMalloc(&scratch, (size_t)slice * COMB_WINDOWS * COMB_ENTRIES * 24 * 4));
+  for (uint32_t s = 0; s < tcount; s += slice) {
+    uint32_t k = std::min(slice, tcount - s);
+    launch_comb_build(c.setup, first_table + s, k, c.gen_affine, c.gen_tables, scratch);
+    check(hipGetLastError());
+  }
+  check(hipStreamSynchronize(c.setup));
+  check(hipFree(scratch));
+  c.gen_slots = slots;
+}
+
+// Map public keys to cache slots, preparing unseen ones (decompression,
+// subgroup check, Miller-loop lines) on the device.  ctx().mu held.
+static void lookup_pks_locked(Context& c, const std::vector<const uint8_t*>& pks,
+                              std::vector<uint32_t>& slots_out) {
+  std::vector<uint32_t> fresh;
+  std::vector<const uint8_t*> fresh_bytes;
+  slots_out.resize(pks.size());
+  for (size_t i = 0; i < pks.size(); i++) {
+    std::string key(reinterpret_cast<const char*>(pks[i]), 96);
+    auto it = c.pk_slots.find(key);
+    if (it != c.pk_slots.end()) {
+      slots_out[i] = it->second;
+      continue;
+    }
+    uint32_t slot = c.pk_count++;
+    c.pk_slots.emplace(std::move(key), slot);
+    slots_out[i] = slot;
+    fresh.push_back(slot);
+    fresh_bytes.push_back(pks[i]);
+  }
+  if (fresh.empty()) return;
+  if (c.pk_count > c.pk_cap) {
+    uint32_t cap = std::max<uint32_t>({c.pk_count, c.pk_cap * 2, 64u});
+    uint8_t* b = nullptr;
+    uint8_t* st = nullptr;
+    uint32_t* ln = nullptr;
+    check(hipMalloc(&b, (size_t)cap * 96));
+    check(hipMalloc(&st, cap));
+    check(hipMalloc(&ln, (size_t)cap * PK_LINES_WORDS * 4));
+    uint32_t old = c.pk_count - (uint32_t)fresh.size();
+    if (old) {
+      check(hipMemcpyAsync(b, c.pk_bytes, (size_t)old * 96, hipMemcpyDeviceToDevice, c.setup));
+      check(hipMemcpyAsync(st, c.pk_status, old, hipMemcpyDeviceToDevice, c.setup));
+      check(hipMemcpyAsync(ln, c.pk_lines, (size_t)old * PK_LINES_WORDS * 4,
+                           hipMemcpyDeviceToDevice, c.setup));
+    }
+    if (c.pk_bytes) {
+      c.retired.push_back(c.pk_bytes);
+      c.retired.push_back(c.pk_status);
+      c.retired.push_back(c.pk_lines);
+    }
+    c.pk_bytes = b;
+    c.pk_status = st;
+    c.pk_lines = ln;
+    c.pk_cap = cap;
+  }
+  std::vector<uint8_t> hb(fresh.size() * 96);
+  for (size_t i = 0; i < fresh.size(); i++) std::memcpy(hb.data() + 96 * i, fresh_bytes[i], 96);
+  uint32_t* d_slots = nullptr;
+  uint8_t* d_b = nullptr;
+  check(hipMalloc(&d_slots, fresh.size() * 4));
+  check(hipMalloc(&d_b, hb.size()));
+  check(hipMemcpyAsync(d_slots, fresh.data(), fresh.size() * 4, hipMemcpyHostToDevice, c.setup));
+  check(hipMemcpyAsync(d_b, hb.data(), hb.size(), hipMemcpyHostToDevice, c.setup));
+  // scatter the bytes into their slots, then prepare
+  for (size_t i = 0; i < fresh.size(); i++)
+    check(hipMemcpyAsync(c.pk_bytes + (size_t)fresh[i] * 96, d_b + 96 * i, 96,
+                         hipMemcpyDeviceToDevice, c.setup));
+  launch_prepare_pk(c.setup, d_slots, (uint32_t)fresh.size(), c.pk_bytes, c.pk_status, c.pk_lines);
+  check(hipGetLastError());
+  check(hipStreamSynchronize(c.setup));
+  check(hipFree(d_slots));
+  check(hipFree(d_b));
+}
+
+// ------------------------------------------------------------------ batch
+struct Layout {
+  size_t size = 0;
+  size_t add(size_t bytes) {
+    size_t off = size;
+    size = align256(size + bytes);
+    return off;
+  }
+};
+
+struct KernelTimer {
+  bool on = false;
+  std::vector<std::string> names;
+  std::vector<hipEvent_t> starts, stops;
+  std::vector<double> total_ms;
+  std::vector<int> count;
+  std::vector<size_t> names_of_event;
+};
+
+struct Batch {
+  int kind = 0;
+  hipStream_t stream = nullptr;
+  size_t n_total = 0;                 // items passed in
+  std::vector<int32_t> pre_code;      // per item: -1 = on device, else final code
+  std::vector<uint32_t> dev_index;    // per item: index among device items
+  uint32_t nv = 0;                    // items on the device
+  // staged inputs (same offsets host and device)
+  size_t in_bytes = 0;
+  uint8_t* h_in = nullptr;
+  uint8_t* d_in = nullptr;
+  uint8_t* d_work = nullptr;
+  size_t work_bytes = 0;
+  // counts
+  uint32_t n_msgs = 0, n_terms = 0, n_chunks = 0, n_partials = 0, n_groups = 0;
+  uint64_t comb_madds = 0;
+  // device pointers
+  VerifyItem* items = nullptr;
+  H2sJob* h2s = nullptr;
+  DomainJob* dom = nullptr;
+  CombChunk* chunks = nullptr;
+  ReduceGroup* groups = nullptr;
+  uint32_t* term_gen = nullptr;
+  uint8_t* sig_blob = nullptr;
+  uint8_t* hdr_blob = nullptr;
+  uint8_t* msg_blob = nullptr;
+  uint32_t* term_scalar = nullptr;
+  uint32_t* partials = nullptr;
+  uint32_t* a_aff = nullptr;
+  uint8_t* a_inf = nullptr;
+  uint32_t* b_aff = nullptr;
+  uint8_t* b_inf = nullptr;
+  uint8_t* status = nullptr;
+  uint8_t* valid = nullptr;
+  // snapshot of resident state
+  const uint32_t* tables = nullptr;
+  const uint8_t* gen_comp = nullptr;
+  const uint8_t* pk_bytes = nullptr;
+  const uint8_t* pk_status = nullptr;
+  const uint32_t* pk_lines = nullptr;
+  const uint32_t* neg_g2_lines = nullptr;
+  // results
+  std::vector<uint8_t> h_status, h_valid;
+  KernelTimer timer;
+
+  ~Batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (size_t i = 0; i < timer.starts.size(); i++) {
+      (void)hipEventDestroy(timer.starts[i]);
+      (void)hipEventDestroy(timer.stops[i]);
+    }
+    if (h_in) (void)hipHostFree(h_in);
+    if (d_in) (void)hipFree(d_in);
+    if (d_work) (void)hipFree(d_work);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// Bracket a kernel launch with events when profiling.
+struct TimedLaunch {
+  Batch& b;
+  size_t idx;
+  TimedLaunch(Batch& bb, const char* name) : b(bb), idx(0) {
+    if (!b.timer.on) return;
+    KernelTimer& t = b.timer;
+    auto it = std::find(t.names.begin(), t.names.end(), name);
+    if (it == t.names.end()) {
+      t.names.push_back(name);
+      t.total_ms.push_back(0);
+      t.count.push_back(0);
+      idx = t.names.size() - 1;
+    } else {
+      idx = it - t.names.begin();
+    }
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0));
+    check(hipEventCreate(&e1));
+    t.starts.push_back(e0);
+    t.stops.push_back(e1);
+    check(hipEventRecord(e0, b.stream));
+  }
+  ~TimedLaunch() {
+    if (!b.timer.on) return;
+    (void)hipEventRecord(b.timer.stops.back(), b.stream);
+    b.timer.count[idx]++;
+    b.timer.names_of_event.push_back(idx);
+  }
+};
+
+static uint32_t comb_chunk_terms() { return std::max<uint32_t>(1, env_u32("CUBBS_HIP_COMB_CHUNK", 4)); }
+
+static bool slice_ok(const CuBbsBbsGpuByteSlice& s) { return s.len == 0 || s.data != nullptr; }
+
+// ---------------------------------------------------------------- Verify planner
+static std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n) {
+  auto b = std::make_unique<Batch>();
+  b->kind = CUBBS_HIP_BATCH_VERIFY;
+  b->n_total = n;
+  b->pre_code.assign(n, -1);
+  b->dev_index.assign(n, 0);
+  std::vector<size_t> dev_items;
+  uint64_t hdr_total = 0, msg_total = 0, msgs = 0;
+  uint32_t maxL = 0;
+  for (size_t i = 0; i < n; i++) {
+    const auto& it = items[i];
+    bool ok = it.public_key.len == 96 && it.public_key.data && it.signature.len == 80 &&
+              it.signature.data && slice_ok(it.header) &&
+              (it.message_count == 0 || it.messages != nullptr);
+    if (ok) {
+      for (size_t m = 0; m < it.message_count && ok; m++) ok = slice_ok(it.messages[m]);
+    }
+    if (!ok || it.message_count > (1u << 20)) {
+      b->pre_code[i] = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      continue;
+    }
+    b->dev_index[i] = (uint32_t)dev_items.size();
+    dev_items.push_back(i);
+    hdr_total += it.header.len;
+    msgs += it.message_count;
+    for (size_t m = 0; m < it.message_count; m++) msg_total += it.messages[m].len;
+    maxL = std::max<uint32_t>(maxL, (uint32_t)it.message_count);
+  }
+  const uint32_t nv = (uint32_t)dev_items.size();
+  b->nv = nv;
+  if (nv == 0) return b;
+  Context& c = ctx();
+  std::vector<uint32_t> pk_slot;
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    init_locked(c);
+    ensure_generators_locked(c, maxL + 2);
+    std::vector<const uint8_t*> pks(nv);
+    for (uint32_t k = 0; k < nv; k++) pks[k] = items[dev_items[k]].public_key.data;
+    lookup_pks_locked(c, pks, pk_slot);
+    b->tables = c.gen_tables;
+    b->gen_comp = c.gen_comp;
+    b->pk_bytes = c.pk_bytes;
+    b->pk_status = c.pk_status;
+    b->pk_lines = c.pk_lines;
+    b->neg_g2_lines = c.neg_g2_lines;
+  }
+  const uint32_t CH = comb_chunk_terms();
+  uint64_t terms = 0, chunks = 0;
+  for (uint32_t k = 0; k < nv; k++) {
+    uint64_t t = items[dev_items[k]].message_count + 2;
+    terms += t;
+    chunks += (t + CH - 1) / CH;
+  }
+  b->n_msgs = (uint32_t)msgs;
+  b->n_terms = (uint32_t)terms;
+  b->n_chunks = (uint32_t)chunks;
+  b->n_partials = (uint32_t)(chunks + nv);
+  b->n_groups = nv;
+  // staged input layout
+  Layout in;
+  size_t o_items = in.add(sizeof(VerifyItem) * nv);
+  size_t o_h2s = in.add(sizeof(H2sJob) * std::max<uint64_t>(msgs, 1));
+  size_t o_dom = in.add(sizeof(DomainJob) * nv);
+  size_t o_chunks = in.add(sizeof(CombChunk) * chunks);
+  size_t o_groups = in.add(sizeof(ReduceGroup) * nv);
+  size_t o_gen = in.add(4 * terms);
+  size_t o_sig = in.add(80 * (size_t)nv);
+  size_t o_hdr = in.add(std::max<uint64_t>(hdr_total, 1));
+  size_t o_msg = in.add(std::max<uint64_t>(msg_total, 1));
+  b->in_bytes = in.size;
+  check(hipHostMalloc(&b->h_in, in.size, hipHostMallocDefault));
+  uint8_t* H = b->h_in;
+  auto* vi = reinterpret_cast<VerifyItem*>(H + o_items);
+  auto* hj = reinterpret_cast<H2sJob*>(H + o_h2s);
+  auto* dj = reinterpret_cast<DomainJob*>(H + o_dom);
+  auto* cj = reinterpret_cast<CombChunk*>(H + o_chunks);
+  auto* gj = reinterpret_cast<ReduceGroup*>(H + o_groups);
+  auto* tg = reinterpret_cast<uint32_t*>(H + o_gen);
+  uint64_t term = 0, chunk = 0, partial = 0, hoff = 0, moff = 0, msg = 0;
+  for (uint32_t k = 0; k < nv; k++) {
+    const auto& it = items[dev_items[k]];
+    const uint32_t L = (uint32_t)it.message_count;
+    std::memcpy(H + o_sig + 80 * (size_t)k, it.signature.data, 80);
+    if (it.header.len) std::memcpy(H + o_hdr + hoff, it.header.data, it.header.len);
+    dj[k] = DomainJob{hoff, (uint32_t)it.header.len, pk_slot[k], L, (uint32_t)(term + 1)};
+    hoff += it.header.len;
+    for (uint32_t m = 0; m < L; m++) {
+      const auto& ms = it.messages[m];
+      if (ms.len) std::memcpy(H + o_msg + moff, ms.data, ms.len);
+      hj[msg++] = H2sJob{moff, (uint32_t)ms.len, (uint32_t)(term + 2 + m)};
+      moff += ms.len;
+    }
+    for (uint32_t t = 0; t < L + 2; t++) tg[term + t] = t;  // P1, Q1, H_1..H_L
+    vi[k] = VerifyItem{80ull * k, pk_slot[k], (uint32_t)term, (uint32_t)partial, 1u};
+    uint32_t first_partial = (uint32_t)partial;
+    partial++;  // variable partial (-e) A
+    for (uint32_t t = 0; t < L + 2; t += CH) {
+      uint32_t cnt = std::min(CH, L + 2 - t);
+      cj[chunk++] = CombChunk{(uint32_t)(term + t), cnt, (uint32_t)partial++};
+    }
+    gj[k] = ReduceGroup{first_partial, (uint32_t)(partial - first_partial)};
+    term += L + 2;
+  }
+  b->comb_madds = 32ull * terms;  // upper bound: one mixed addition per non-zero digit
+  // work layout
+  Layout w;
+  size_t o_ts = w.add(4ull * FR_WORDS * terms);
+  size_t o_part = w.add(4ull * G1J_WORDS * b->n_partials);
+  size_t o_aa = w.add(4ull * G1A_WORDS * nv);
+  size_t o_ai = w.add(nv);
+  size_t o_ba = w.add(4ull * G1A_WORDS * nv);
+  size_t o_bi = w.add(nv);
+  size_t o_st = w.add(nv);
+  size_t o_va = w.add(nv);
+  b->work_bytes = w.size;
+  check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  check(hipMalloc(&b->d_in, in.size));
+  check(hipMalloc(&b->d_work, w.size));
+  uint8_t* D = b->d_in;
+  b->items = reinterpret_cast<VerifyItem*>(D + o_items);
+  b->h2s = reinterpret_cast<H2sJob*>(D + o_h2s);
+  b->dom = reinterpret_cast<DomainJob*>(D + o_dom);
+  b->chunks = reinterpret_cast<CombChunk*>(D + o_chunks);
+  b->groups = reinterpret_cast<ReduceGroup*>(D + o_groups);
+  b->term_gen = reinterpret_cast<uint32_t*>(D + o_gen);
+  b->sig_blob = D + o_sig;
+  b->hdr_blob = D + o_hdr;
+  b->msg_blob = D + o_msg;
+  uint8_t* W = b->d_work;
+  b->term_scalar = reinterpret_cast<uint32_t*>(W + o_ts);
+  b->partials = reinterpret_cast<uint32_t*>(W + o_part);
+  b->a_aff = reinterpret_cast<uint32_t*>(W + o_aa);
+  b->a_inf = W + o_ai;
+  b->b_aff = reinterpret_cast<uint32_t*>(W + o_ba);
+  b->b_inf = W + o_bi;
+  b->status = W + o_st;
+  b->valid = W + o_va;
+  check(hipMemcpyAsync(b->d_in, b->h_in, in.size, hipMemcpyHostToDevice, b->stream));
+  return b;
+}
+
+static void launch_verify(Batch& b) {
+  if (b.nv == 0) return;
+  {
+    TimedLaunch t(b, "h2s_messages");
+    launch_h2s(b.stream, b.h2s, b.n_msgs, b.msg_blob, 0, b.term_scalar);
+  }
+  {
+    TimedLaunch t(b, "domain");
+    launch_domain(b.stream, b.dom, b.nv, b.hdr_blob, b.pk_bytes, b.gen_comp, b.term_scalar);
+  }
+  {
+    TimedLaunch t(b, "verify_sig_glv");
+    launch_verify_sig(b.stream, b.items, b.nv, b.sig_blob, b.term_scalar, b.a_aff, b.a_inf,
+                      b.status, b.partials);
+  }
+  {
+    TimedLaunch t(b, "comb_msm");
+    launch_comb_msm(b.stream, b.chunks, b.n_chunks, b.term_gen, b.term_scalar, b.tables,
+                    b.partials);
+  }
+  {
+    TimedLaunch t(b, "reduce_affine");
+    launch_reduce_affine(b.stream, b.groups, b.n_groups, b.partials, b.b_aff, b.b_inf);
+  }
+  {
+    TimedLaunch t(b, "verify_pairing");
+    launch_verify_pairing(b.stream, b.items, b.nv, b.a_aff, b.a_inf, b.b_aff, b.b_inf,
+                          b.pk_status, b.pk_lines, b.neg_g2_lines, b.status, b.valid);
+  }
+  check(hipGetLastError());
+}
+
+static void collect_verify(Batch& b, CuBbsBbsGpuResult* results) {
+  b.h_status.resize(b.nv);
+  b.h_valid.resize(b.nv);
+  if (b.nv) {
+    check(hipMemcpyAsync(b.h_status.data(), b.status, b.nv, hipMemcpyDeviceToHost, b.stream));
+    check(hipMemcpyAsync(b.h_valid.data(), b.valid, b.nv, hipMemcpyDeviceToHost, b.stream));
+    check(hipStreamSynchronize(b.stream));
+  }
+  for (size_t i = 0; i < b.n_total; i++) {
+    results[i].bytes_written = 0;
+    if (b.pre_code[i] >= 0) {
+      results[i].code = b.pre_code[i];
+      results[i].value = 0;
+      continue;
+    }
+    uint32_t k = b.dev_index[i];
+    results[i].code = b.h_status[k] == 0 ? CUBBS_BBS_GPU_SUCCESS : CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+    results[i].value = b.h_status[k] == 0 ? b.h_valid[k] : 0;
+  }
+}
+
+static void launch_batch(Batch& b) {
+  if (b.kind == CUBBS_HIP_BATCH_VERIFY) launch_verify(b);
+}
+
+}  // namespace cubbs
+
+using namespace cubbs;
+
+// ==================================================================== C ABI
+extern "C" {
+
+int32_t cubbs_bbs_gpu_backend_info_v2(uint32_t abi_version, CuBbsBbsGpuBackendInfo* info_out) {
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION || !info_out) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  info_out->backend_kind = CUBBS_BBS_GPU_BACKEND_KIND_CUDA;
+  info_out->hardware_acceleration = 0;
+  try {
+    Context& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    init_locked(c);
+    info_out->hardware_acceleration = 1;
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (const GpuError&) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
+
+int32_t cubbs_bbs_gpu_verify_batch_v2(uint32_t abi_version, const CuBbsBbsGpuVerifyItem* items,
+                                      size_t item_count, CuBbsBbsGpuResult* results) {
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  if (item_count == 0) return CUBBS_BBS_GPU_SUCCESS;
+  if (!items || !results) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  try {
+    auto b = plan_verify(items, item_count);
+    launch_verify(*b);
+    collect_verify(*b, results);
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (const GpuError&) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
+
+void* cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void* items, size_t item_count,
+                              int32_t* status_out) {
+  int32_t dummy;
+  int32_t& st = status_out ? *status_out : dummy;
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION || (item_count && !items)) {
+    st = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+    return nullptr;
+  }
+  try {
+    std::unique_ptr<Batch> b;
+    if (kind == CUBBS_HIP_BATCH_VERIFY) {
+      b = plan_verify(static_cast<const CuBbsBbsGpuVerifyItem*>(items), item_count);
+    } else {
+      st = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      return nullptr;
+    }
+    check(hipStreamSynchronize(b->stream ? b->stream : 0));
+    st = CUBBS_BBS_GPU_SUCCESS;
+    return b.release();
+  } catch (const GpuError&) {
+    st = CUBBS_BBS_GPU_ERR_INTERNAL;
+    return nullptr;
+  }
+}
+
+int32_t cubbs_hip_batch_launch(void* batch) {
+  if (!batch) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  try {
+    launch_batch(*static_cast<Batch*>(batch));
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (const GpuError&) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
+
+int32_t cubbs_hip_batch_sync(void* batch) {
+  if (!batch) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  Batch* b = static_cast<Batch*>(batch);
+  if (!b->stream) return CUBBS_BBS_GPU_SUCCESS;
+  return hipStreamSynchronize(b->stream) == hipSuccess ? CUBBS_BBS_GPU_SUCCESS
+                                                       : CUBBS_BBS_GPU_ERR_INTERNAL;
+}
+
+int32_t cubbs_hip_batch_results(void* batch, CuBbsBbsGpuResult* results) {
+  if (!batch || !results) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  try {
+    Batch* b = static_cast<Batch*>(batch);
+    if (b->kind == CUBBS_HIP_BATCH_VERIFY) collect_verify(*b, results);
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (const GpuError&) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
+
+void cubbs_hip_batch_free(void* batch) { delete static_cast<Batch*>(batch); }
+
+int32_t cubbs_hip_batch_shape(void* batch, uint64_t* items, uint64_t* msm_terms,
+                              uint64_t* comb_madds, uint64_t* pairings) {
+  if (!batch) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  Batch* b = static_cast<Batch*>(batch);
+  if (items) *items = b->nv;
+  if (msm_terms) *msm_terms = b->n_terms;
+  if (comb_madds) *comb_madds = b->comb_madds;
+  if (pairings) *pairings = 2ull * b->nv;
+  return CUBBS_BBS_GPU_SUCCESS;
+}
+
+int32_t cubbs_hip_batch_profile(void* batch, int iters, char* names_out, size_t names_cap,
+                                double* ms_out, int max_kernels, double* total_ms_out) {
+  if (!batch || iters <= 0) return -CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  Batch* b = static_cast<Batch*>(batch);
+  try {
+    KernelTimer& t = b->timer;
+    t = KernelTimer();
+    t.on = true;
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0));
+    check(hipEventCreate(&e1));
+    check(hipEventRecord(e0, b->stream));
+    for (int i = 0; i < iters; i++) launch_batch(*b);
+    check(hipEventRecord(e1, b->stream));
+    check(hipStreamSynchronize(b->stream));
+    float total = 0;
+    check(hipEventElapsedTime(&total, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (total_ms_out) *total_ms_out = total / iters;
+    std::vector<double> sum(t.names.size(), 0.0);
+    for (size_t k = 0; k < t.starts.size(); k++) {
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, t.starts[k], t.stops[k]));
+      sum[t.names_of_event[k]] += ms;
+    }
+    int nk = (int)std::min<size_t>(t.names.size(), (size_t)max_kernels);
+    std::string joined;
+    for (int k = 0; k < nk; k++) {
+      if (ms_out) ms_out[k] = sum[k] / std::max(1, t.count[k]);
+      joined += t.names[k];
+      joined += "\n";
+    }
+    if (names_out && names_cap) {
+      size_t m = std::min(names_cap - 1, joined.size());
+      std::memcpy(names_out, joined.data(), m);
+      names_out[m] = 0;
+    }
+    t.on = false;
+    return nk;
+  } catch (const GpuError&) {
+    return -CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
+
+// ---------------------------------------------------------------- hashing
+// Shared implementation of the packed hash entries: kind 0 = SHA-256 digest,
+// 1 = expand_message_xmd(.., 48), 2 = hash_to_scalar.
+static int32_t packed_hash(int kind, const uint8_t* packed, const size_t* offsets,
+                           const size_t* lengths, size_t n, const uint8_t* dst, size_t dst_len,
+                           uint8_t* out) {
+  if (n == 0) return CUBBS_BBS_GPU_SUCCESS;
+  if (!offsets || !lengths || !out) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  if (kind != 0 && (dst_len > 255 || (dst_len && !dst))) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  size_t total = 0;
+  for (size_t i = 0; i < n; i++) {
+    if (lengths[i] && !packed) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+    total = std::max(total, offsets[i] + lengths[i]);
+  }
+  try {
+    {
+      Context& c = ctx();
+      std::lock_guard<std::mutex> g(c.mu);
+      init_locked(c);
+    }
+    const size_t out_w = kind == 0 ? 32 : (kind == 1 ? 48 : 32);
+    Layout L;
+    size_t o_jobs = L.add(sizeof(H2sJob) * n);
+    size_t o_dst = L.add(std::max<size_t>(dst_len, 1));
+    size_t o_blob = L.add(std::max<size_t>(total, 1));
+    size_t o_out = L.add(out_w * n);
+    std::vector<uint8_t> h(L.size);
+    auto* jobs = reinterpret_cast<H2sJob*>(h.data() + o_jobs);
+    for (size_t i = 0; i < n; i++) jobs[i] = H2sJob{offsets[i], (uint32_t)lengths[i], (uint32_t)i};
+    if (dst_len) std::memcpy(h.data() + o_dst, dst, dst_len);
+    if (total) std::memcpy(h.data() + o_blob, packed, total);
+    hipStream_t s;
+    check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    uint8_t* d = nullptr;
+    check(hipMalloc(&d, L.size));
+    check(hipMemcpyAsync(d, h.data(), o_out, hipMemcpyHostToDevice, s));
+    launch_hash_jobs(s, kind, reinterpret_cast<H2sJob*>(d + o_jobs), (uint32_t)n, d + o_blob,
+                     d + o_dst, (uint32_t)dst_len, d + o_out);
+    check(hipGetLastError());
+    check(hipMemcpyAsync(out, d + o_out, out_w * n, hipMemcpyDeviceToHost, s));
+    check(hipStreamSynchronize(s));
+    check(hipFree(d));
+    check(hipStreamDestroy(s));
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (const GpuError&) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
+
+int32_t cubbs_bbs_gpu_sha256_batch_for_test_v2(uint32_t abi_version, const uint8_t* packed,
+                                               const size_t* offsets, const size_t* lengths,
+                                               size_t item_count, uint8_t* digests_out) {
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  return packed_hash(0, packed, offsets, lengths, item_count, nullptr, 0, digests_out);
+}
+
+int32_t cubbs_bbs_gpu_expand_xmd_sha256_48_batch_for_test_v2(
+    uint32_t abi_version, const uint8_t* packed, const size_t* offsets, const size_t* lengths,
+    size_t item_count, const uint8_t* dst, size_t dst_len, uint8_t* out48) {
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  return packed_hash(1, packed, offsets, lengths, item_count, dst, dst_len, out48);
+}
+
+int32_t cubbs_bbs_gpu_hash_to_scalar_sha256_batch_for_test_v2(
+    uint32_t abi_version, const uint8_t* packed, const size_t* offsets, const size_t* lengths,
+    size_t item_count, const uint8_t* dst, size_t dst_len, uint8_t* scalars_out) {
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  return packed_hash(2, packed, offsets, lengths, item_count, dst, dst_len, scalars_out);
+}
+
+// crates/cubbs-bbs/src/gpu_runtime.rs:763 ; reference semantics
+// native/cuda/src/cubbs_bbs_cuda.cu:353-447 (one DST per call, 32-byte outputs).
+int32_t cubbs_bbs_gpu_messages_to_scalar_batch_v2(uint32_t abi_version,
+                                                  CuBbsBbsGpuMessagesToScalarItem* items,
+                                                  size_t item_count, CuBbsBbsGpuResult* results) {
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION || (item_count && (!items || !results)))
+    return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  if (item_count == 0) return CUBBS_BBS_GPU_SUCCESS;
+  size_t total = 0;
+  for (size_t i = 0; i < item_count; i++) {
+    const auto& it = items[i];
+    if (!slice_ok(it.message)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+    if (it.dst.len > 255 || !slice_ok(it.dst) || !it.scalar_out.data || it.scalar_out.capacity < 32)
+      return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+    total += it.message.len;
+  }
+  const auto& d0 = items[0].dst;
+  for (size_t i = 1; i < item_count; i++) {
+    if (items[i].dst.len != d0.len ||
+        (d0.len && std::memcmp(items[i].dst.data, d0.data, d0.len) != 0))
+      return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  }
+  std::vector<uint8_t> packed(std::max<size_t>(total, 1));
+  std::vector<size_t> offs(item_count), lens(item_count);
+  size_t cur = 0;
+  for (size_t i = 0; i < item_count; i++) {
+    offs[i] = cur;
+    lens[i] = items[i].message.len;
+    if (lens[i]) std::memcpy(packed.data() + cur, items[i].message.data, lens[i]);
+    cur += lens[i];
+  }
+  std::vector<uint8_t> out(32 * item_count);
+  int32_t st = packed_hash(2, packed.data(), offs.data(), lens.data(), item_count, d0.data, d0.len,
+                           out.data());
+  if (st != CUBBS_BBS_GPU_SUCCESS) return st;
+  for (size_t i = 0; i < item_count; i++) {
+    std::memcpy(items[i].scalar_out.data, out.data() + 32 * i, 32);
+    items[i].scalar_out.len = 32;
+    results[i].code = CUBBS_BBS_GPU_SUCCESS;
+    results[i].value = 0;
+    results[i].bytes_written = 32;
+  }
+  return CUBBS_BBS_GPU_SUCCESS;
+}
+
+// ---------------------------------------------------------------- probes
+}  // extern "C"
+template <class F>
+static int32_t run_probe(size_t in_bytes, const void* in, size_t out_bytes, void* out, size_t ok_bytes,
+                         uint8_t* ok, F launch) {
+  try {
+    {
+      Context& c = ctx();
+      std::lock_guard<std::mutex> g(c.mu);
+      init_locked(c);
+    }
+    hipStream_t s;
+    check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    uint8_t* d = nullptr;
+    size_t total = align256(in_bytes) + align256(out_bytes) + align256(ok_bytes) + 256;
+    check(hipMalloc(&d, total));
+    uint8_t* din = d;
+    uint8_t* dout = d + align256(in_bytes);
+    uint8_t* dok = dout + align256(out_bytes);
+    check(hipMemsetAsync(d, 0, total, s));
+    if (in_bytes) check(hipMemcpyAsync(din, in, in_bytes, hipMemcpyHostToDevice, s));
+    launch(s, din, dout, dok);
+    check(hipGetLastError());
+    if (out_bytes) check(hipMemcpyAsync(out, dout, out_bytes, hipMemcpyDeviceToHost, s));
+    if (ok_bytes) check(hipMemcpyAsync(ok, dok, ok_bytes, hipMemcpyDeviceToHost, s));
+    check(hipStreamSynchronize(s));
+    check(hipFree(d));
+    check(hipStreamDestroy(s));
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (const GpuError&) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
+
+extern "C" {
+int32_t cubbs_hip_test_g1_mul(const uint8_t* points48, const uint8_t* scalars32, size_t n,
+                              uint8_t* out48, uint8_t* ok) {
+  if (n == 0) return CUBBS_BBS_GPU_SUCCESS;
+  if (!points48 || !scalars32 || !out48 || !ok) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  std::vector<uint8_t> in(80 * n);
+  std::memcpy(in.data(), points48, 48 * n);
+  std::memcpy(in.data() + 48 * n, scalars32, 32 * n);
+  return run_probe(in.size(), in.data(), 48 * n, out48, n, ok,
+                   [&](hipStream_t s, uint8_t* din, uint8_t* dout, uint8_t* dok) {
+                     launch_test_g1_mul(s, din, din + 48 * n, (uint32_t)n, dout, dok);
+                   });
+}
+
+int32_t cubbs_hip_test_pairing_gt(const uint8_t* g1_48, const uint8_t* g2_96, size_t n,
+                                  uint8_t* out576, uint8_t* ok) {
+  if (n == 0) return CUBBS_BBS_GPU_SUCCESS;
+  if (!g1_48 || !g2_96 || !out576 || !ok) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  std::vector<uint8_t> in(144 * n);
+  std::memcpy(in.data(), g1_48, 48 * n);
+  std::memcpy(in.data() + 48 * n, g2_96, 96 * n);
+  return run_probe(in.size(), in.data(), 576 * n, out576, n, ok,
+                   [&](hipStream_t s, uint8_t* din, uint8_t* dout, uint8_t* dok) {
+                     launch_test_pairing_gt(s, din, din + 48 * n, (uint32_t)n, dout, dok);
+                   });
+}
+
+int32_t cubbs_hip_test_g2_roundtrip(const uint8_t* g2_96, size_t n, uint8_t* out96, uint8_t* ok) {
+  if (n == 0) return CUBBS_BBS_GPU_SUCCESS;
+  if (!g2_96 || !out96 || !ok) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  return run_probe(96 * n, g2_96, 96 * n, out96, n, ok,
+                   [&](hipStream_t s, uint8_t* din, uint8_t* dout, uint8_t* dok) {
+                     launch_test_g2_decompress(s, din, (uint32_t)n, dout, dok);
+                   });
+}
+
+int32_t cubbs_hip_generators(size_t count, uint8_t* out48) {
+  if (count == 0) return CUBBS_BBS_GPU_SUCCESS;
+  if (!out48) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  try {
+    Context& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    init_locked(c);
+    ensure_generators_locked(c, (uint32_t)count);
+    check(hipMemcpy(out48, c.gen_comp, 48 * count, hipMemcpyDeviceToHost));
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (const GpuError&) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
+
+int32_t cubbs_hip_fp_mul_bench(uint32_t lanes, uint32_t iters, double* ms) {
+  if (!lanes || !ms) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  try {
+    {
+      Context& c = ctx();
+      std::lock_guard<std::mutex> g(c.mu);
+      init_locked(c);
+    }
+    hipStream_t s;
+    check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    uint32_t* d = nullptr;
+    check(hipMalloc(&d, (size_t)lanes * 24 * 4));
+    // any values below p work: small non-zero limbs
+    std::vector<uint32_t> h((size_t)lanes * 24);
+    for (size_t i = 0; i < h.size(); i++) h[i] = (uint32_t)(i * 2654435761u) & 0x0fffffffu;
+    check(hipMemcpyAsync(d, h.data(), h.size() * 4, hipMemcpyHostToDevice, s));
+    launch_fp_mul_bench(s, d, lanes, 1);  // warm-up
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0));
+    check(hipEventCreate(&e1));
+    check(hipEventRecord(e0, s));
+    launch_fp_mul_bench(s, d, lanes, iters);
+    check(hipEventRecord(e1, s));
+    check(hipStreamSynchronize(s));
+    float t = 0;
+    check(hipEventElapsedTime(&t, e0, e1));
+    *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    check(hipFree(d));
+    check(hipStreamDestroy(s));
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (const GpuError&) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
+
+// ---------------------------------------------------------------- not yet batched on the device
+// Mandatory symbols of the loader (gpu_runtime.rs:303-327).  Until their
+// device pipelines land they mark every item INTERNAL, which the Rust engine
+// treats as "use another route".
+static int32_t mark_internal(uint32_t abi_version, const void* items, size_t n,
+                             CuBbsBbsGpuResult* results) {
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  if (n == 0) return CUBBS_BBS_GPU_SUCCESS;
+  if (!items || !results) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  for (size_t i = 0; i < n; i++) {
+    results[i].code = CUBBS_BBS_GPU_ERR_INTERNAL;
+    results[i].value = 0;
+    results[i].bytes_written = 0;
+  }
+  return CUBBS_BBS_GPU_SUCCESS;
+}
+int32_t cubbs_bbs_gpu_sign_batch_v2(uint32_t abi_version, CuBbsBbsGpuSignItem* items,
+                                    size_t item_count, CuBbsBbsGpuResult* results) {
+  return mark_internal(abi_version, items, item_count, results);
+}
+int32_t cubbs_bbs_gpu_proof_gen_batch_v2(uint32_t abi_version, CuBbsBbsGpuProofGenItem* items,
+                                         size_t item_count, CuBbsBbsGpuResult* results) {
+  return mark_internal(abi_version, items, item_count, results);
+}
+int32_t cubbs_bbs_gpu_proof_verify_batch_v2(uint32_t abi_version,
+                                            const CuBbsBbsGpuProofVerifyItem* items,
+                                            size_t item_count, CuBbsBbsGpuResult* results) {
+  return mark_internal(abi_version, items, item_count, results);
+}
+
+}  // extern "C"

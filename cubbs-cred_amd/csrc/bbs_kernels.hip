@@ -1,0 +1,647 @@
+// Device kernels of the BBS hot path (gfx950).  One lane owns one job
+// (message, term chunk, signature, pairing check); every kernel is bounds-
+// checked against its job count and exits on all lanes.
+#include "bbs_internal.h"
+#include "h2c.hpp"
+#include "pairing.hpp"
+
+namespace cubbs {
+
+#define GRID(n, b) dim3(((n) + (b) - 1) / (b)), dim3(b)
+
+DEV void load_fr(Fr& r, const uint32_t* p) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) r.v[k] = p[k];
+}
+DEV void store_fr(uint32_t* p, const Fr& r) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) p[k] = r.v[k];
+}
+DEV void load_g1a(G1A& r, const uint32_t* p) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    r.x.v[k] = p[k];
+    r.y.v[k] = p[12 + k];
+  }
+}
+DEV void store_g1a(uint32_t* p, const G1A& r) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    p[k] = r.x.v[k];
+    p[12 + k] = r.y.v[k];
+  }
+}
+DEV void load_g1j(G1J& r, const uint32_t* p) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    r.x.v[k] = p[k];
+    r.y.v[k] = p[12 + k];
+    r.z.v[k] = p[24 + k];
+  }
+}
+DEV void store_g1j(uint32_t* p, const G1J& r) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    p[k] = r.x.v[k];
+    p[12 + k] = r.y.v[k];
+    p[24 + k] = r.z.v[k];
+  }
+}
+DEV void load_fp2(Fp2& r, const uint32_t* p) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    r.c0.v[k] = p[k];
+    r.c1.v[k] = p[12 + k];
+  }
+}
+DEV void store_fp2(uint32_t* p, const Fp2& r) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    p[k] = r.c0.v[k];
+    p[12 + k] = r.c1.v[k];
+  }
+}
+
+// ------------------------------------------------------------------ hash to scalar
+__global__ __launch_bounds__(256) void k_h2s(const H2sJob* __restrict__ jobs, uint32_t n,
+                                            const uint8_t* __restrict__ blob, int dst_kind,
+                                            uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  H2sJob j = jobs[i];
+  Fr s;
+  if (dst_kind == 0)
+    hash_to_scalar_global(s, blob + j.off, j.len, DST_MAP_MSG, DST_MAP_MSG_LEN);
+  else
+    hash_to_scalar_global(s, blob + j.off, j.len, DST_H2S, DST_H2S_LEN);
+  store_fr(out + (size_t)j.out_index * FR_WORDS, s);
+}
+void launch_h2s(hipStream_t s, const H2sJob* jobs, uint32_t n, const uint8_t* blob, int dst_kind,
+                uint32_t* out) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_h2s, GRID(n, 256), 0, s, jobs, n, blob, dst_kind, out);
+}
+
+// Packed hash jobs with a caller DST in device memory: kind 0 = SHA-256
+// digest (32 bytes), 1 = expand_message_xmd(msg, dst, 48), 2 = hash_to_scalar.
+__global__ __launch_bounds__(256) void k_hash_jobs(int kind, const H2sJob* __restrict__ jobs,
+                                                  uint32_t n, const uint8_t* __restrict__ blob,
+                                                  const uint8_t* __restrict__ dst, uint32_t dst_len,
+                                                  uint8_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  H2sJob j = jobs[i];
+  if (kind == 0) {
+    Sha256 s;
+    sha_init(s);
+    sha_bytes_global(s, blob + j.off, j.len);
+    sha_final(s);
+    for (int k = 0; k < 8; k++) {
+      uint32_t w = s.h[k];
+      uint8_t* o = out + (size_t)32 * j.out_index + 4 * k;
+      o[0] = w >> 24; o[1] = w >> 16; o[2] = w >> 8; o[3] = w;
+    }
+    return;
+  }
+  Sha256 s;
+  sha_init_after_zpad(s);
+  sha_bytes_global(s, blob + j.off, j.len);
+  xmd_b0_suffix(s, 48, dst, dst_len);
+  uint32_t okm[12];
+  xmd48_from_b0(okm, s.h, dst, dst_len);
+  if (kind == 1) {
+    for (int k = 0; k < 12; k++) {
+      uint32_t w = okm[k];
+      uint8_t* o = out + (size_t)48 * j.out_index + 4 * k;
+      o[0] = w >> 24; o[1] = w >> 16; o[2] = w >> 8; o[3] = w;
+    }
+    return;
+  }
+  Fr sc;
+  scalar_from_okm48(sc, okm);
+  fr_to_be_bytes(out + (size_t)32 * j.out_index, sc);
+}
+void launch_hash_jobs(hipStream_t s, int kind, const H2sJob* jobs, uint32_t n, const uint8_t* blob,
+                      const uint8_t* dst, uint32_t dst_len, uint8_t* out) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_hash_jobs, GRID(n, 256), 0, s, kind, jobs, n, blob, dst, dst_len, out);
+}
+
+// domain = hash_to_scalar(PK || I2OSP(L, 8) || Q1 || H_1..H_L || api_id ||
+//                         I2OSP(len(header), 8) || header, api_id || "H2S_")
+__global__ __launch_bounds__(256) void k_domain(const DomainJob* __restrict__ jobs, uint32_t n,
+                                               const uint8_t* __restrict__ hdr_blob,
+                                               const uint8_t* __restrict__ pk_bytes,
+                                               const uint8_t* __restrict__ gen_comp,
+                                               uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  DomainJob j = jobs[i];
+  Sha256 s;
+  sha_init_after_zpad(s);
+  sha_bytes_global(s, pk_bytes + (size_t)j.pk_slot * 96, 96);
+  sha_u64_be(s, j.L);
+  sha_bytes_global(s, gen_comp + 48 * (size_t)GEN_Q1, 48u * (j.L + 1));
+  sha_bytes_const(s, API_ID_BYTES, API_ID_LEN);
+  sha_u64_be(s, j.hdr_len);
+  sha_bytes_global(s, hdr_blob + j.hdr_off, j.hdr_len);
+  xmd_b0_suffix(s, 48, DST_H2S, DST_H2S_LEN);
+  uint32_t okm[12];
+  xmd48_from_b0(okm, s.h, DST_H2S, DST_H2S_LEN);
+  Fr d;
+  scalar_from_okm48(d, okm);
+  store_fr(out + (size_t)j.out_index * FR_WORDS, d);
+}
+void launch_domain(hipStream_t s, const DomainJob* jobs, uint32_t n, const uint8_t* hdr_blob,
+                   const uint8_t* pk_bytes, const uint8_t* gen_comp, uint32_t* out) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_domain, GRID(n, 256), 0, s, jobs, n, hdr_blob, pk_bytes, gen_comp, out);
+}
+
+// ------------------------------------------------------------------ generators
+// Seed chain (generators.rs:119-148): v = expand(api_id || "MESSAGE_GENERATOR_SEED",
+// seed_dst, 48); v_i = expand(v_{i-1} || I2OSP(i, 8), seed_dst, 48).  Sequential:
+// one lane.  v_state holds the last v (12 words) and is updated.
+__global__ void k_gen_seeds(uint32_t first, uint32_t count, uint32_t* v_state, uint32_t* seeds) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  uint32_t v[12];
+  if (first == 1) {
+    Sha256 s;
+    sha_init_after_zpad(s);
+    sha_bytes_const(s, API_ID_BYTES, API_ID_LEN);
+    const char* tag = "MESSAGE_GENERATOR_SEED";
+    for (int k = 0; k < 22; k++) sha_byte(s, (uint8_t)tag[k]);
+    xmd_b0_suffix(s, 48, DST_GEN_SEED, DST_GEN_SEED_LEN);
+    xmd48_from_b0(v, s.h, DST_GEN_SEED, DST_GEN_SEED_LEN);
+  } else {
+    for (int k = 0; k < 12; k++) v[k] = v_state[k];
+  }
+  for (uint32_t i = first; i < first + count; i++) {
+    Sha256 s;
+    sha_init_after_zpad(s);
+    for (int k = 0; k < 12; k++) sha_word(s, v[k]);
+    sha_u64_be(s, i);
+    xmd_b0_suffix(s, 48, DST_GEN_SEED, DST_GEN_SEED_LEN);
+    xmd48_from_b0(v, s.h, DST_GEN_SEED, DST_GEN_SEED_LEN);
+    for (int k = 0; k < 12; k++) seeds[(size_t)(i - first) * 12 + k] = v[k];
+  }
+  for (int k = 0; k < 12; k++) v_state[k] = v[k];
+}
+void launch_gen_seeds(hipStream_t s, uint32_t first, uint32_t count, uint32_t* v_state,
+                      uint32_t* seeds) {
+  hipLaunchKernelGGL(k_gen_seeds, dim3(1), dim3(1), 0, s, first, count, v_state, seeds);
+}
+
+// Generator from chain seed v_i (i >= 1) = hash_to_curve(v_i, api_id ||
+// "SIG_GENERATOR_DST_"); it occupies slot i (slot 1 = Q1, slot 0 = P1).
+__global__ __launch_bounds__(64) void k_gen_points(uint32_t first, uint32_t count,
+                                                  const uint32_t* __restrict__ seeds,
+                                                  uint32_t* __restrict__ gen_affine,
+                                                  uint8_t* __restrict__ gen_comp) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  uint32_t v[12];
+  for (int k = 0; k < 12; k++) v[k] = seeds[(size_t)t * 12 + k];
+  G1J p;
+  hash_to_curve_g1_words(p, v, 12, DST_GEN, DST_GEN_LEN);
+  G1A a;
+  pt_to_affine(a, p);
+  uint32_t slot = first + t;
+  store_g1a(gen_affine + (size_t)slot * G1A_WORDS, a);
+  uint8_t c[48];
+  g1_compress_affine(c, a);
+  for (int k = 0; k < 48; k++) gen_comp[(size_t)slot * 48 + k] = c[k];
+  if (first == 1 && t == 0) {
+    // slot 0: P1 (ciphersuite constant)
+    G1A p1;
+    g1_load_affine_const(p1, bls::BBS_P1);
+    store_g1a(gen_affine, p1);
+    g1_compress_affine(c, p1);
+    for (int k = 0; k < 48; k++) gen_comp[k] = c[k];
+  }
+}
+void launch_gen_points(hipStream_t s, uint32_t first, uint32_t count, const uint32_t* seeds,
+                       uint32_t* gen_affine, uint8_t* gen_comp) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_gen_points, GRID(count, 64), 0, s, first, count, seeds, gen_affine, gen_comp);
+}
+
+// Comb tables: one lane per (generator, window): base = 2^(8w) G, entries
+// d * base for d = 1..128 built with mixed additions in Jacobian form, then
+// normalised to affine with one inversion (Montgomery's batch trick).
+// `scratch` holds 24 words per entry and lane (z_d and the prefix product).
+__global__ __launch_bounds__(64) void k_comb_build(uint32_t first, uint32_t count,
+                                                  const uint32_t* __restrict__ gen_affine,
+                                                  uint32_t* __restrict__ tables,
+                                                  uint32_t* __restrict__ scratch) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count * COMB_WINDOWS) return;
+  uint32_t g = first + t / COMB_WINDOWS;
+  uint32_t w = t % COMB_WINDOWS;
+  G1A ga;
+  load_g1a(ga, gen_affine + (size_t)g * G1A_WORDS);
+  G1J base;
+  pt_from_affine(base, ga);
+  for (uint32_t k = 0; k < 8 * w; k++) pt_dbl(base, base);
+  G1A baseA;
+  pt_to_affine(baseA, base);
+  uint32_t* out = tables + ((size_t)g * COMB_WINDOWS + w) * COMB_ENTRIES * COMB_ENTRY_WORDS;
+  uint32_t* sc = scratch + (size_t)t * COMB_ENTRIES * 24;
+  G1J acc;
+  pt_from_affine(acc, baseA);
+  Fp prod;
+  fp_one(prod);
+  for (int d = 1; d <= COMB_ENTRIES; d++) {
+    fp_mul(prod, prod, acc.z);
+    uint32_t* e = out + (size_t)(d - 1) * COMB_ENTRY_WORDS;
+    uint32_t* z = sc + (size_t)(d - 1) * 24;
+    for (int k = 0; k < 12; k++) {
+      e[k] = acc.x.v[k];
+      e[12 + k] = acc.y.v[k];
+      z[k] = acc.z.v[k];
+      z[12 + k] = prod.v[k];
+    }
+    if (d < COMB_ENTRIES) pt_madd(acc, acc, baseA);
+  }
+  Fp inv;
+  fp_inv(inv, prod);  // 1 / (z_1 ... z_128)
+  for (int d = COMB_ENTRIES; d >= 1; d--) {
+    uint32_t* e = out + (size_t)(d - 1) * COMB_ENTRY_WORDS;
+    const uint32_t* z = sc + (size_t)(d - 1) * 24;
+    Fp zd, pre, zi;
+    for (int k = 0; k < 12; k++) zd.v[k] = z[k];
+    if (d > 1) {
+      for (int k = 0; k < 12; k++) pre.v[k] = sc[(size_t)(d - 2) * 24 + 12 + k];
+      fp_mul(zi, inv, pre);
+    } else {
+      zi = inv;
+    }
+    fp_mul(inv, inv, zd);
+    Fp X, Y, zi2, zi3;
+    for (int k = 0; k < 12; k++) {
+      X.v[k] = e[k];
+      Y.v[k] = e[12 + k];
+    }
+    fp_sqr(zi2, zi);
+    fp_mul(zi3, zi2, zi);
+    G1A a;
+    fp_mul(a.x, X, zi2);
+    fp_mul(a.y, Y, zi3);
+    store_g1a(e, a);
+  }
+}
+void launch_comb_build(hipStream_t s, uint32_t first, uint32_t count, const uint32_t* gen_affine,
+                       uint32_t* tables, uint32_t* scratch) {
+  if (!count) return;
+  uint32_t n = count * COMB_WINDOWS;
+  hipLaunchKernelGGL(k_comb_build, GRID(n, 64), 0, s, first, count, gen_affine, tables, scratch);
+}
+
+// ------------------------------------------------------------------ fixed-base comb MSM
+// Each lane sums up to term_count terms of one chunk into one Jacobian partial.
+// Scalar k (canonical, < r < 2^255) is recoded into 32 signed 8-bit digits.
+__global__ __launch_bounds__(128) void k_comb_msm(const CombChunk* __restrict__ chunks, uint32_t n,
+                                                 const uint32_t* __restrict__ term_gen,
+                                                 const uint32_t* __restrict__ term_scalar,
+                                                 const uint32_t* __restrict__ tables,
+                                                 uint32_t* __restrict__ partials) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  CombChunk c = chunks[i];
+  G1J acc;
+  pt_set_inf(acc);
+  for (uint32_t t = c.term_begin; t < c.term_begin + c.term_count; t++) {
+    const uint32_t g = term_gen[t];
+    uint32_t k[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) k[q] = term_scalar[(size_t)t * FR_WORDS + q];
+    const uint32_t* tab = tables + (size_t)g * COMB_TABLE_WORDS;
+    uint32_t carry = 0;
+#pragma unroll 1
+    for (int w = 0; w < COMB_WINDOWS; w++) {
+      uint32_t b = ((k[w >> 2] >> ((w & 3) * 8)) & 0xff) + carry;
+      int d;
+      if (b > 128) {
+        d = (int)b - 256;
+        carry = 1;
+      } else {
+        d = (int)b;
+        carry = 0;
+      }
+      if (d != 0) {
+        uint32_t ad = d < 0 ? (uint32_t)(-d) : (uint32_t)d;
+        const uint4* e4 = reinterpret_cast<const uint4*>(
+            tab + ((size_t)w * COMB_ENTRIES + (ad - 1)) * COMB_ENTRY_WORDS);
+        G1A e;
+        uint32_t ew[24];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+          uint4 v = e4[q];
+          ew[4 * q] = v.x;
+          ew[4 * q + 1] = v.y;
+          ew[4 * q + 2] = v.z;
+          ew[4 * q + 3] = v.w;
+        }
+        load_g1a(e, ew);
+        if (d < 0) fp_neg(e.y, e.y);
+        pt_madd(acc, acc, e);
+      }
+    }
+  }
+  store_g1j(partials + (size_t)c.partial * G1J_WORDS, acc);
+}
+void launch_comb_msm(hipStream_t s, const CombChunk* chunks, uint32_t n, const uint32_t* term_gen,
+                     const uint32_t* term_scalar, const uint32_t* tables, uint32_t* partials) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_comb_msm, GRID(n, 128), 0, s, chunks, n, term_gen, term_scalar, tables,
+                     partials);
+}
+
+// Sum partials of each group; write affine (Montgomery) and an identity flag.
+__global__ __launch_bounds__(64) void k_reduce_affine(const ReduceGroup* __restrict__ groups,
+                                                     uint32_t n, const uint32_t* __restrict__ partials,
+                                                     uint32_t* __restrict__ out_affine,
+                                                     uint8_t* __restrict__ out_inf) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ReduceGroup g = groups[i];
+  G1J acc;
+  pt_set_inf(acc);
+  for (uint32_t k = 0; k < g.count; k++) {
+    G1J p;
+    load_g1j(p, partials + (size_t)(g.begin + k) * G1J_WORDS);
+    pt_add(acc, acc, p);
+  }
+  if (pt_is_inf(acc)) {
+    out_inf[i] = 1;
+    G1A z;
+    fp_zero(z.x);
+    fp_zero(z.y);
+    store_g1a(out_affine + (size_t)i * G1A_WORDS, z);
+  } else {
+    out_inf[i] = 0;
+    G1A a;
+    pt_to_affine(a, acc);
+    store_g1a(out_affine + (size_t)i * G1A_WORDS, a);
+  }
+}
+void launch_reduce_affine(hipStream_t s, const ReduceGroup* groups, uint32_t n,
+                          const uint32_t* partials, uint32_t* out_affine, uint8_t* out_inf) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_reduce_affine, GRID(n, 64), 0, s, groups, n, partials, out_affine, out_inf);
+}
+
+// ------------------------------------------------------------------ public keys
+// pk_status: 0 invalid, 1 valid, 2 valid identity
+__global__ __launch_bounds__(64) void k_prepare_pk(const uint32_t* __restrict__ slots, uint32_t n,
+                                                  const uint8_t* __restrict__ pk_bytes,
+                                                  uint8_t* __restrict__ pk_status,
+                                                  uint32_t* __restrict__ pk_lines) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t slot = slots[i];
+  G2J q;
+  bool ok = g2_decompress(q, pk_bytes + (size_t)slot * 96);
+  if (!ok) {
+    pk_status[slot] = 0;
+    return;
+  }
+  if (pt_is_inf(q)) {
+    pk_status[slot] = 2;
+    return;
+  }
+  G2A a;
+  pt_to_affine(a, q);
+  Line* lines = reinterpret_cast<Line*>(pk_lines + (size_t)slot * PK_LINES_WORDS);
+  g2_prepare_lines(lines, a);
+  pk_status[slot] = 1;
+}
+void launch_prepare_pk(hipStream_t s, const uint32_t* slots, uint32_t n, const uint8_t* pk_bytes,
+                       uint8_t* pk_status, uint32_t* pk_lines) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_prepare_pk, GRID(n, 64), 0, s, slots, n, pk_bytes, pk_status, pk_lines);
+}
+
+__global__ void k_prepare_neg_g2(uint32_t* lines) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  G2A q;
+  g2_load_affine_const(q, bls::G2_GEN_NEG);
+  g2_prepare_lines(reinterpret_cast<Line*>(lines), q);
+}
+void launch_prepare_neg_g2(hipStream_t s, uint32_t* lines) {
+  hipLaunchKernelGGL(k_prepare_neg_g2, dim3(1), dim3(1), 0, s, lines);
+}
+
+// ------------------------------------------------------------------ Verify
+// status per item: 0 ok, 1 invalid input (signature encoding)
+// Parses A (compressed G1, subgroup checked) and e (canonical), writes the P1
+// term scalar (= 1), A in affine, and the variable partial (r - e) * A.
+__global__ __launch_bounds__(64) void k_verify_sig(const VerifyItem* __restrict__ items, uint32_t n,
+                                                  const uint8_t* __restrict__ sig_blob,
+                                                  uint32_t* __restrict__ term_scalar,
+                                                  uint32_t* __restrict__ a_affine,
+                                                  uint8_t* __restrict__ a_inf,
+                                                  uint8_t* __restrict__ status,
+                                                  uint32_t* __restrict__ partials) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  VerifyItem it = items[i];
+  G1J negeA;
+  pt_set_inf(negeA);
+  if (!(it.flags & 1)) {
+    status[i] = 1;
+    store_g1j(partials + (size_t)it.var_partial * G1J_WORDS, negeA);
+    return;
+  }
+  const uint8_t* sig = sig_blob + it.sig_off;
+  G1J A;
+  bool ok = g1_decompress(A, sig);
+  Fr e;
+  fr_from_be_bytes_raw(e, sig + 48);
+  ok = ok && fr_canonical(e);
+  Fr one;
+  fr_zero(one);
+  one.v[0] = 1;
+  store_fr(term_scalar + (size_t)it.term_begin * FR_WORDS, one);
+  status[i] = ok ? 0 : 1;
+  if (ok && !pt_is_inf(A)) {
+    G1A Aa;
+    Aa.x = A.x;
+    Aa.y = A.y;  // z = 1 after decompression
+    store_g1a(a_affine + (size_t)i * G1A_WORDS, Aa);
+    a_inf[i] = 0;
+    Fr ne;
+    fr_neg(ne, e);  // canonical r - e (0 stays 0)
+    g1_mul_glv(negeA, Aa, ne.v);
+  } else {
+    a_inf[i] = 1;
+  }
+  store_g1j(partials + (size_t)it.var_partial * G1J_WORDS, negeA);
+}
+void launch_verify_sig(hipStream_t s, const VerifyItem* items, uint32_t n, const uint8_t* sig_blob,
+                       uint32_t* term_scalar, uint32_t* a_affine, uint8_t* a_inf, uint8_t* status,
+                       uint32_t* partials) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_verify_sig, GRID(n, 64), 0, s, items, n, sig_blob, term_scalar, a_affine,
+                     a_inf, status, partials);
+}
+
+// e(A, PK) * e(B - eA, -G2) == 1  <=>  e(A, PK + e G2) * e(B, -G2) == 1
+// (crates/cubbs-bbs-protocol/src/bbsplus/signature.rs:708-728)
+__global__ __launch_bounds__(64) void k_verify_pairing(
+    const VerifyItem* __restrict__ items, uint32_t n, const uint32_t* __restrict__ a_affine,
+    const uint8_t* __restrict__ a_inf, const uint32_t* __restrict__ b_affine,
+    const uint8_t* __restrict__ b_inf, const uint8_t* __restrict__ pk_status,
+    const uint32_t* __restrict__ pk_lines, const uint32_t* __restrict__ neg_g2_lines,
+    uint8_t* __restrict__ status, uint8_t* __restrict__ valid) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  VerifyItem it = items[i];
+  if (status[i] != 0) {
+    valid[i] = 0;
+    return;
+  }
+  const uint8_t pks = pk_status[it.pk_slot];
+  if (pks == 0) {
+    status[i] = 1;
+    valid[i] = 0;
+    return;
+  }
+  G1A ps[2];
+  bool skip[2];
+  load_g1a(ps[0], a_affine + (size_t)i * G1A_WORDS);
+  load_g1a(ps[1], b_affine + (size_t)i * G1A_WORDS);
+  skip[0] = a_inf[i] || pks == 2;
+  skip[1] = b_inf[i];
+  const Line* lines[2] = {reinterpret_cast<const Line*>(pk_lines + (size_t)it.pk_slot * PK_LINES_WORDS),
+                          reinterpret_cast<const Line*>(neg_g2_lines)};
+  Fp12 f, r;
+  miller_loop_prepared<2>(f, ps, skip, lines);
+  final_exponentiation(r, f);
+  valid[i] = fp12_is_one(r) ? 1 : 0;
+}
+void launch_verify_pairing(hipStream_t s, const VerifyItem* items, uint32_t n,
+                           const uint32_t* a_affine, const uint8_t* a_inf, const uint32_t* b_affine,
+                           const uint8_t* b_inf, const uint8_t* pk_status, const uint32_t* pk_lines,
+                           const uint32_t* neg_g2_lines, uint8_t* status, uint8_t* valid) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_verify_pairing, GRID(n, 64), 0, s, items, n, a_affine, a_inf, b_affine,
+                     b_inf, pk_status, pk_lines, neg_g2_lines, status, valid);
+}
+
+// ------------------------------------------------------------------ test-only primitives
+__global__ __launch_bounds__(64) void k_test_g1_mul(const uint8_t* pts, const uint8_t* scalars,
+                                                   uint32_t n, uint8_t* out, uint8_t* ok) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  G1J p;
+  bool good = g1_decompress(p, pts + (size_t)48 * i);
+  Fr k;
+  fr_from_be_bytes_raw(k, scalars + (size_t)32 * i);
+  good = good && fr_canonical(k);
+  ok[i] = good;
+  if (!good) return;
+  G1J r;
+  if (pt_is_inf(p)) {
+    pt_set_inf(r);
+  } else {
+    G1A a;
+    a.x = p.x;
+    a.y = p.y;
+    g1_mul_glv(r, a, k.v);
+  }
+  uint8_t c[48];
+  g1_compress(c, r);
+  for (int q = 0; q < 48; q++) out[(size_t)48 * i + q] = c[q];
+}
+void launch_test_g1_mul(hipStream_t s, const uint8_t* pts, const uint8_t* scalars, uint32_t n,
+                        uint8_t* out, uint8_t* ok) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_test_g1_mul, GRID(n, 64), 0, s, pts, scalars, n, out, ok);
+}
+
+__global__ __launch_bounds__(64) void k_test_pairing_gt(const uint8_t* g1, const uint8_t* g2,
+                                                       uint32_t n, uint8_t* out, uint8_t* ok,
+                                                       uint32_t* lines_scratch) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  G1J p;
+  G2J q;
+  bool good = g1_decompress(p, g1 + (size_t)48 * i) && g2_decompress(q, g2 + (size_t)96 * i);
+  good = good && !pt_is_inf(p) && !pt_is_inf(q);
+  ok[i] = good;
+  if (!good) return;
+  G2A qa;
+  pt_to_affine(qa, q);
+  Line* lines = reinterpret_cast<Line*>(lines_scratch + (size_t)i * PK_LINES_WORDS);
+  g2_prepare_lines(lines, qa);
+  G1A pa;
+  pa.x = p.x;
+  pa.y = p.y;
+  bool skip = false;
+  const Line* lp[1] = {lines};
+  Fp12 f, r;
+  miller_loop_prepared<1>(f, &pa, &skip, lp);
+  final_exponentiation(r, f);
+  const Fp2* cs[6] = {&r.c0.c0, &r.c0.c1, &r.c0.c2, &r.c1.c0, &r.c1.c1, &r.c1.c2};
+  for (int k = 0; k < 6; k++) {
+    Fp c;
+    fp_from_mont(c, cs[k]->c0);
+    fp_to_be_bytes(out + (size_t)576 * i + 96 * k, c);
+    fp_from_mont(c, cs[k]->c1);
+    fp_to_be_bytes(out + (size_t)576 * i + 96 * k + 48, c);
+  }
+}
+void launch_test_pairing_gt(hipStream_t s, const uint8_t* g1, const uint8_t* g2, uint32_t n,
+                            uint8_t* out, uint8_t* ok) {
+  if (!n) return;
+  uint32_t* scratch = nullptr;
+  if (hipMalloc(&scratch, (size_t)n * PK_LINES_WORDS * 4) != hipSuccess) return;
+  hipLaunchKernelGGL(k_test_pairing_gt, GRID(n, 64), 0, s, g1, g2, n, out, ok, scratch);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(scratch);
+}
+
+__global__ __launch_bounds__(64) void k_test_g2_decompress(const uint8_t* g2, uint32_t n,
+                                                          uint8_t* out, uint8_t* ok) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  G2J q;
+  bool good = g2_decompress(q, g2 + (size_t)96 * i);
+  ok[i] = good;
+  if (!good) return;
+  uint8_t c[96];
+  g2_compress(c, q);
+  for (int k = 0; k < 96; k++) out[(size_t)96 * i + k] = c[k];
+}
+void launch_test_g2_decompress(hipStream_t s, const uint8_t* g2, uint32_t n, uint8_t* out,
+                               uint8_t* ok) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_test_g2_decompress, GRID(n, 64), 0, s, g2, n, out, ok);
+}
+
+// Throughput probe: `iters` dependent Fp multiplications per lane.
+__global__ __launch_bounds__(256) void k_fp_mul_bench(uint32_t* data, uint32_t n, uint32_t iters) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Fp a, b, c, d;
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    a.v[k] = data[(size_t)i * 24 + k];
+    b.v[k] = data[(size_t)i * 24 + 12 + k];
+  }
+  c = a;
+  d = b;
+  for (uint32_t t = 0; t < iters; t++) {
+    fp_mul(a, a, b);
+    fp_mul(c, c, d);
+  }
+  fp_add(a, a, c);
+#pragma unroll
+  for (int k = 0; k < 12; k++) data[(size_t)i * 24 + k] = a.v[k];
+}
+void launch_fp_mul_bench(hipStream_t s, uint32_t* data, uint32_t n, uint32_t iters) {
+  hipLaunchKernelGGL(k_fp_mul_bench, GRID(n, 256), 0, s, data, n, iters);
+}
+
+}  // namespace cubbs

@@ -1,0 +1,496 @@
+// G1 (E: y^2 = x^3 + 4 over Fp) and G2 (E': y^2 = x^3 + 4(1+u) over Fp2)
+// point arithmetic in Jacobian coordinates, ZCash point codec, subgroup
+// checks and scalar multiplication (plain windowed and GLV for G1).
+//
+// Jacobian (X, Y, Z) stands for (X/Z^2, Y/Z^3); Z = 0 is the identity.
+#pragma once
+#include "field.hpp"
+
+// ------------------------------------------------------------------ field-generic shims
+DEV void f_mul(Fp& r, const Fp& a, const Fp& b) { fp_mul(r, a, b); }
+DEV void f_sqr(Fp& r, const Fp& a) { fp_sqr(r, a); }
+DEV void f_add(Fp& r, const Fp& a, const Fp& b) { fp_add(r, a, b); }
+DEV void f_sub(Fp& r, const Fp& a, const Fp& b) { fp_sub(r, a, b); }
+DEV void f_dbl(Fp& r, const Fp& a) { fp_dbl(r, a); }
+DEV void f_neg(Fp& r, const Fp& a) { fp_neg(r, a); }
+DEV bool f_is_zero(const Fp& a) { return fp_is_zero(a); }
+DEV bool f_eq(const Fp& a, const Fp& b) { return fp_eq(a, b); }
+DEV void f_one(Fp& r) { fp_one(r); }
+DEV void f_zero(Fp& r) { fp_zero(r); }
+DEVI void f_inv(Fp& r, const Fp& a) { fp_inv(r, a); }
+DEVI void f_mul(Fp2& r, const Fp2& a, const Fp2& b) { fp2_mul(r, a, b); }
+DEVI void f_sqr(Fp2& r, const Fp2& a) { fp2_sqr(r, a); }
+DEV void f_add(Fp2& r, const Fp2& a, const Fp2& b) { fp2_add(r, a, b); }
+DEV void f_sub(Fp2& r, const Fp2& a, const Fp2& b) { fp2_sub(r, a, b); }
+DEV void f_dbl(Fp2& r, const Fp2& a) { fp2_dbl(r, a); }
+DEV void f_neg(Fp2& r, const Fp2& a) { fp2_neg(r, a); }
+DEV bool f_is_zero(const Fp2& a) { return fp2_is_zero(a); }
+DEV bool f_eq(const Fp2& a, const Fp2& b) { return fp2_eq(a, b); }
+DEV void f_one(Fp2& r) { fp2_one(r); }
+DEV void f_zero(Fp2& r) { fp2_zero(r); }
+DEVI void f_inv(Fp2& r, const Fp2& a) { fp2_inv(r, a); }
+
+template <class F>
+struct PtJ { F x, y, z; };
+template <class F>
+struct PtA { F x, y; };
+
+using G1J = PtJ<Fp>;
+using G1A = PtA<Fp>;
+using G2J = PtJ<Fp2>;
+using G2A = PtA<Fp2>;
+
+template <class F>
+DEV bool pt_is_inf(const PtJ<F>& p) { return f_is_zero(p.z); }
+template <class F>
+DEV void pt_set_inf(PtJ<F>& p) {
+  f_one(p.x);
+  f_one(p.y);
+  f_zero(p.z);
+}
+template <class F>
+DEV void pt_from_affine(PtJ<F>& r, const PtA<F>& a) {
+  r.x = a.x;
+  r.y = a.y;
+  f_one(r.z);
+}
+
+// dbl-2009-l (a = 0): 2M + 5S
+template <class F>
+DEVN void pt_dbl(PtJ<F>& r, const PtJ<F>& p) {
+  F A, B, C, D, E, G, t;
+  f_sqr(A, p.x);
+  f_sqr(B, p.y);
+  f_sqr(C, B);
+  f_add(t, p.x, B);
+  f_sqr(t, t);
+  f_sub(t, t, A);
+  f_sub(t, t, C);
+  f_dbl(D, t);
+  f_dbl(E, A);
+  f_add(E, E, A);
+  f_sqr(G, E);
+  F X3, Y3, Z3;
+  f_sub(X3, G, D);
+  f_sub(X3, X3, D);
+  f_mul(Z3, p.y, p.z);
+  f_dbl(Z3, Z3);
+  f_sub(t, D, X3);
+  f_mul(Y3, E, t);
+  f_dbl(C, C);
+  f_dbl(C, C);
+  f_dbl(C, C);
+  f_sub(r.y, Y3, C);
+  r.x = X3;
+  r.z = Z3;  // identity (z = 0) maps to z = 0
+}
+
+// add-2007-bl: 11M + 5S, with the exceptional cases handled.
+template <class F>
+DEVN void pt_add(PtJ<F>& r, const PtJ<F>& p, const PtJ<F>& q) {
+  if (pt_is_inf(p)) {
+    r = q;
+    return;
+  }
+  if (pt_is_inf(q)) {
+    r = p;
+    return;
+  }
+  F Z1Z1, Z2Z2, U1, U2, S1, S2, H, I, J, rr, V, t;
+  f_sqr(Z1Z1, p.z);
+  f_sqr(Z2Z2, q.z);
+  f_mul(U1, p.x, Z2Z2);
+  f_mul(U2, q.x, Z1Z1);
+  f_mul(S1, p.y, q.z);
+  f_mul(S1, S1, Z2Z2);
+  f_mul(S2, q.y, p.z);
+  f_mul(S2, S2, Z1Z1);
+  f_sub(H, U2, U1);
+  f_sub(rr, S2, S1);
+  if (f_is_zero(H)) {
+    if (f_is_zero(rr)) {
+      pt_dbl(r, p);
+    } else {
+      pt_set_inf(r);
+    }
+    return;
+  }
+  f_dbl(I, H);
+  f_sqr(I, I);
+  f_mul(J, H, I);
+  f_dbl(rr, rr);
+  f_mul(V, U1, I);
+  F X3, Y3, Z3;
+  f_sqr(X3, rr);
+  f_sub(X3, X3, J);
+  f_sub(X3, X3, V);
+  f_sub(X3, X3, V);
+  f_sub(t, V, X3);
+  f_mul(Y3, rr, t);
+  f_mul(t, S1, J);
+  f_dbl(t, t);
+  f_sub(Y3, Y3, t);
+  f_add(t, p.z, q.z);
+  f_sqr(t, t);
+  f_sub(t, t, Z1Z1);
+  f_sub(t, t, Z2Z2);
+  f_mul(Z3, t, H);
+  r.x = X3;
+  r.y = Y3;
+  r.z = Z3;
+}
+
+// madd-2007-bl (q affine, not the identity): 7M + 4S.
+template <class F>
+DEVN void pt_madd(PtJ<F>& r, const PtJ<F>& p, const PtA<F>& q) {
+  if (pt_is_inf(p)) {
+    pt_from_affine(r, q);
+    return;
+  }
+  F Z1Z1, U2, S2, H, HH, I, J, rr, V, t;
+  f_sqr(Z1Z1, p.z);
+  f_mul(U2, q.x, Z1Z1);
+  f_mul(S2, q.y, p.z);
+  f_mul(S2, S2, Z1Z1);
+  f_sub(H, U2, p.x);
+  f_sub(rr, S2, p.y);
+  if (f_is_zero(H)) {
+    if (f_is_zero(rr)) {
+      PtJ<F> qj;
+      pt_from_affine(qj, q);
+      pt_dbl(r, qj);
+    } else {
+      pt_set_inf(r);
+    }
+    return;
+  }
+  f_sqr(HH, H);
+  f_dbl(I, HH);
+  f_dbl(I, I);
+  f_mul(J, H, I);
+  f_dbl(rr, rr);
+  f_mul(V, p.x, I);
+  F X3, Y3, Z3;
+  f_sqr(X3, rr);
+  f_sub(X3, X3, J);
+  f_sub(X3, X3, V);
+  f_sub(X3, X3, V);
+  f_sub(t, V, X3);
+  f_mul(Y3, rr, t);
+  f_mul(t, p.y, J);
+  f_dbl(t, t);
+  f_sub(Y3, Y3, t);
+  f_add(t, p.z, H);
+  f_sqr(t, t);
+  f_sub(t, t, Z1Z1);
+  f_sub(Z3, t, HH);
+  r.x = X3;
+  r.y = Y3;
+  r.z = Z3;
+}
+
+template <class F>
+DEV void pt_neg(PtJ<F>& r, const PtJ<F>& p) {
+  r.x = p.x;
+  f_neg(r.y, p.y);
+  r.z = p.z;
+}
+
+// Jacobian -> affine (p must not be the identity)
+template <class F>
+DEVN void pt_to_affine(PtA<F>& r, const PtJ<F>& p) {
+  F zi, zi2, zi3;
+  f_inv(zi, p.z);
+  f_sqr(zi2, zi);
+  f_mul(zi3, zi2, zi);
+  f_mul(r.x, p.x, zi2);
+  f_mul(r.y, p.y, zi3);
+}
+
+// P == (x, y) affine?  (p may be the identity -> false)
+template <class F>
+DEVN bool pt_eq_affine(const PtJ<F>& p, const PtA<F>& q) {
+  if (pt_is_inf(p)) return false;
+  F z2, z3, t;
+  f_sqr(z2, p.z);
+  f_mul(t, q.x, z2);
+  if (!f_eq(t, p.x)) return false;
+  f_mul(z3, z2, p.z);
+  f_mul(t, q.y, z3);
+  return f_eq(t, p.y);
+}
+
+// r = [k] p, k a small (<= 64 bit) constant, left-to-right
+template <class F>
+DEVN void pt_mul_u64(PtJ<F>& r, const PtJ<F>& p, uint64_t k) {
+  PtJ<F> acc;
+  pt_set_inf(acc);
+#pragma unroll 1
+  for (int i = 63; i >= 0; i--) {
+    pt_dbl(acc, acc);
+    if ((k >> i) & 1) pt_add(acc, acc, p);
+  }
+  r = acc;
+}
+
+// r = [k] q for a canonical scalar k given as N little-endian 32-bit limbs
+// and an affine base (binary left-to-right with mixed additions).
+template <class F, int N>
+DEVN void pt_mul_limbs_affine(PtJ<F>& r, const PtA<F>& q, const uint32_t* k) {
+  PtJ<F> acc;
+  pt_set_inf(acc);
+#pragma unroll 1
+  for (int i = N * 32 - 1; i >= 0; i--) {
+    pt_dbl(acc, acc);
+    if ((k[i >> 5] >> (i & 31)) & 1) pt_madd(acc, acc, q);
+  }
+  r = acc;
+}
+
+// ------------------------------------------------------------------ G1 specifics
+DEV void g1_load_affine_const(G1A& r, const uint32_t* c) {
+  fp_load_const(r.x, c);
+  fp_load_const(r.y, c + 12);
+}
+
+// phi(x, y) = (beta x, y) acts on G1 as multiplication by lambda = x^2 - 1.
+DEV void g1_endo(G1J& r, const G1J& p) {
+  Fp beta;
+  fp_load_const(beta, bls::GLV_BETA);
+  fp_mul(r.x, p.x, beta);
+  r.y = p.y;
+  r.z = p.z;
+}
+
+// Subgroup test for points of E(Fp) (Bowe, "Faster subgroup checks for
+// BLS12-381", 2019): P in G1 iff phi'(P) = [-x^2] P where phi' = phi^2 acts
+// as lambda^2 = -x^2.  Here: [x^2] P == (beta^2 x, -y).
+DEVN bool g1_in_subgroup(const G1J& p) {
+  if (pt_is_inf(p)) return true;
+  G1J t;
+  pt_mul_u64(t, p, 0xd201000000010000ull);
+  pt_mul_u64(t, t, 0xd201000000010000ull);
+  G1A q, pa;
+  pt_to_affine(pa, p);
+  Fp beta;
+  fp_load_const(beta, bls::GLV_BETA);
+  fp_mul(q.x, pa.x, beta);
+  fp_mul(q.x, q.x, beta);
+  fp_neg(q.y, pa.y);
+  return pt_eq_affine(t, q);
+}
+
+// Decode a 48-byte compressed G1 point (bls12_381 G1Affine::from_compressed
+// semantics): compression flag set; infinity encoding must be 0xc0 00..00;
+// x < p; on curve; in the prime-order subgroup.  Returns false on any failure.
+DEVN bool g1_decompress(G1J& r, const uint8_t* in) {
+  uint8_t b[48];
+#pragma unroll
+  for (int i = 0; i < 48; i++) b[i] = in[i];
+  const uint32_t flags = b[0];
+  const bool comp = (flags >> 7) & 1, inf = (flags >> 6) & 1, sign = (flags >> 5) & 1;
+  if (!comp) return false;
+  b[0] &= 0x1f;
+  if (inf) {
+    uint32_t o = sign;
+#pragma unroll
+    for (int i = 0; i < 48; i++) o |= b[i];
+    pt_set_inf(r);
+    return o == 0;
+  }
+  Fp x, y, rhs, bb;
+  if (!fp_from_be_bytes(x, b)) return false;
+  fp_to_mont(x, x);
+  fp_sqr(rhs, x);
+  fp_mul(rhs, rhs, x);
+  fp_load_const(bb, bls::FP_B);
+  fp_add(rhs, rhs, bb);
+  if (!fp_sqrt(y, rhs)) return false;
+  if (fp_lex_largest(y) != sign) fp_neg(y, y);
+  r.x = x;
+  r.y = y;
+  fp_one(r.z);
+  return g1_in_subgroup(r);
+}
+
+DEVN void g1_compress_affine(uint8_t* out, const G1A& a) {
+  Fp c;
+  fp_from_mont(c, a.x);
+  fp_to_be_bytes(out, c);
+  uint8_t f = 0x80;
+  if (fp_lex_largest(a.y)) f |= 0x20;
+  out[0] |= f;
+}
+DEVN void g1_compress(uint8_t* out, const G1J& p) {
+  if (pt_is_inf(p)) {
+    out[0] = 0xc0;
+    for (int i = 1; i < 48; i++) out[i] = 0;
+    return;
+  }
+  G1A a;
+  pt_to_affine(a, p);
+  g1_compress_affine(out, a);
+}
+
+// ------------------------------------------------------------------ G2 specifics
+DEV void g2_load_affine_const(G2A& r, const uint32_t* c) {
+  fp_load_const(r.x.c0, c);
+  fp_load_const(r.x.c1, c + 12);
+  fp_load_const(r.y.c0, c + 24);
+  fp_load_const(r.y.c1, c + 36);
+}
+
+DEVN bool g2_in_subgroup(const G2J& q) {
+  if (pt_is_inf(q)) return true;
+  G2A a;
+  pt_to_affine(a, q);
+  G2J t;
+  pt_mul_limbs_affine<Fp2, 8>(t, a, bls::R_MOD);
+  return pt_is_inf(t);
+}
+
+// 96-byte compressed G2 (x.c1 || x.c0, flags in the first byte).
+DEVN bool g2_decompress(G2J& r, const uint8_t* in) {
+  uint8_t b[96];
+  for (int i = 0; i < 96; i++) b[i] = in[i];
+  const uint32_t flags = b[0];
+  const bool comp = (flags >> 7) & 1, inf = (flags >> 6) & 1, sign = (flags >> 5) & 1;
+  if (!comp) return false;
+  b[0] &= 0x1f;
+  if (inf) {
+    uint32_t o = sign;
+    for (int i = 0; i < 96; i++) o |= b[i];
+    pt_set_inf(r);
+    return o == 0;
+  }
+  Fp2 x, y, rhs, bb;
+  if (!fp_from_be_bytes(x.c1, b) || !fp_from_be_bytes(x.c0, b + 48)) return false;
+  fp_to_mont(x.c0, x.c0);
+  fp_to_mont(x.c1, x.c1);
+  fp2_sqr(rhs, x);
+  fp2_mul(rhs, rhs, x);
+  fp_load_const(bb.c0, bls::FP_B);
+  bb.c1 = bb.c0;
+  fp2_add(rhs, rhs, bb);
+  if (!fp2_sqrt(y, rhs)) return false;
+  if (fp2_lex_largest(y) != sign) fp2_neg(y, y);
+  r.x = x;
+  r.y = y;
+  fp2_one(r.z);
+  return g2_in_subgroup(r);
+}
+
+DEVN void g2_compress(uint8_t* out, const G2J& p) {
+  if (pt_is_inf(p)) {
+    out[0] = 0xc0;
+    for (int i = 1; i < 96; i++) out[i] = 0;
+    return;
+  }
+  G2A a;
+  pt_to_affine(a, p);
+  Fp c;
+  fp_from_mont(c, a.x.c1);
+  fp_to_be_bytes(out, c);
+  fp_from_mont(c, a.x.c0);
+  fp_to_be_bytes(out + 48, c);
+  uint8_t f = 0x80;
+  if (fp2_lex_largest(a.y)) f |= 0x20;
+  out[0] |= f;
+}
+
+// ------------------------------------------------------------------ GLV scalar split
+// k = k1 + k2 * lambda with lambda = x^2 - 1 (127.4 bits): k2 = floor(k / lambda),
+// k1 = k mod lambda; both fit in 128 bits and are non-negative.
+DEVN void glv_split(uint32_t k1[4], uint32_t k2[4], const uint32_t k[8]) {
+  // q = floor(k * mu / 2^256), mu = floor(2^256 / lambda) (129 bits) ; q <= true quotient
+  uint32_t prod[17];
+#pragma unroll
+  for (int i = 0; i < 17; i++) prod[i] = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    uint64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      c += (uint64_t)k[i] * bls::GLV_MU[j] + prod[i + j];
+      prod[i + j] = (uint32_t)c;
+      c >>= 32;
+    }
+    prod[i + 8] = (uint32_t)c;
+  }
+  uint32_t q[5];
+#pragma unroll
+  for (int i = 0; i < 5; i++) q[i] = prod[8 + i];
+  // rem = k - q * lambda
+  uint32_t ql[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) ql[i] = 0;
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    uint64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (i + j < 9) {
+        c += (uint64_t)q[i] * bls::GLV_LAMBDA[j] + ql[i + j];
+        ql[i + j] = (uint32_t)c;
+        c >>= 32;
+      }
+    }
+    if (i + 4 < 9) ql[i + 4] = (uint32_t)c;
+  }
+  uint32_t rem[8];
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    uint64_t x = (uint64_t)k[i] - ql[i] - borrow;
+    rem[i] = (uint32_t)x;
+    borrow = (uint32_t)(x >> 63);
+  }
+  // at most a few corrections: while rem >= lambda
+#pragma unroll 1
+  for (int it = 0; it < 4; it++) {
+    uint32_t d[8];
+    uint32_t b2 = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      uint64_t x = (uint64_t)rem[i] - (i < 4 ? bls::GLV_LAMBDA[i] : 0u) - b2;
+      d[i] = (uint32_t)x;
+      b2 = (uint32_t)(x >> 63);
+    }
+    if (b2) break;
+#pragma unroll
+    for (int i = 0; i < 8; i++) rem[i] = d[i];
+    uint32_t c = 1;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      uint64_t x = (uint64_t)q[i] + c;
+      q[i] = (uint32_t)x;
+      c = (uint32_t)(x >> 32);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    k1[i] = rem[i];
+    k2[i] = q[i];
+  }
+}
+
+// [k] P on G1 (P affine, not the identity) with the GLV split
+// k = k1 + k2 lambda: 128 shared doublings, mixed additions of P and phi(P).
+DEVN void g1_mul_glv(G1J& r, const G1A& p, const uint32_t k[8]) {
+  uint32_t k1[4], k2[4];
+  glv_split(k1, k2, k);
+  G1A pe;
+  Fp beta;
+  fp_load_const(beta, bls::GLV_BETA);
+  fp_mul(pe.x, p.x, beta);
+  pe.y = p.y;
+  G1J acc;
+  pt_set_inf(acc);
+#pragma unroll 1
+  for (int i = 127; i >= 0; i--) {
+    pt_dbl(acc, acc);
+    if ((k1[i >> 5] >> (i & 31)) & 1) pt_madd(acc, acc, p);
+    if ((k2[i >> 5] >> (i & 31)) & 1) pt_madd(acc, acc, pe);
+  }
+  r = acc;
+}

@@ -1,0 +1,630 @@
+// Finite-field arithmetic for BLS12-381 on gfx950: Fp (381-bit, 12 x 32-bit
+// limbs) and Fr (255-bit, 8 x 32-bit limbs) in Montgomery form, and the
+// Fp2 / Fp6 / Fp12 tower used by the pairing.
+//
+// One lane owns one field element.  Multiplication is CIOS Montgomery with the
+// "no final carry" variant (both moduli leave a spare top bit), so every inner
+// step is one v_mad_u64_u32 plus a 32-bit carry add; values are kept fully
+// reduced in [0, m).
+//
+// Tower: Fp2 = Fp[u]/(u^2+1), Fp6 = Fp2[v]/(v^3 - xi), xi = 1+u,
+//        Fp12 = Fp6[w]/(w^2 - v).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bls12_381_consts.hpp"
+
+#define DEV __device__ __forceinline__
+#define DEVI __device__ inline
+#define DEVN __device__ __noinline__
+
+struct Fp { uint32_t v[12]; };
+struct Fr { uint32_t v[8]; };
+struct Fp2 { Fp c0, c1; };
+struct Fp6 { Fp2 c0, c1, c2; };
+struct Fp12 { Fp6 c0, c1; };
+
+// ------------------------------------------------------------------ generic limb helpers
+template <int N>
+DEV void mont_mul_n(uint32_t* __restrict__ r, const uint32_t* a, const uint32_t* b,
+                    const uint32_t* M, uint32_t minv) {
+  uint32_t t[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) t[k] = 0;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    const uint32_t bi = b[i];
+    uint64_t x = (uint64_t)a[0] * bi + t[0];
+    uint32_t A = (uint32_t)(x >> 32);
+    const uint32_t t0 = (uint32_t)x;
+    const uint32_t m = t0 * minv;
+    uint64_t y = (uint64_t)m * M[0] + t0;
+    uint32_t C = (uint32_t)(y >> 32);
+#pragma unroll
+    for (int j = 1; j < N; j++) {
+      x = (uint64_t)a[j] * bi + t[j] + A;
+      A = (uint32_t)(x >> 32);
+      y = (uint64_t)m * M[j] + (uint32_t)x + C;
+      C = (uint32_t)(y >> 32);
+      t[j - 1] = (uint32_t)y;
+    }
+    t[N - 1] = C + A;
+  }
+  // conditional subtraction of M
+  uint32_t d[N];
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    uint64_t s = (uint64_t)t[k] - M[k] - borrow;
+    d[k] = (uint32_t)s;
+    borrow = (uint32_t)(s >> 63);
+  }
+#pragma unroll
+  for (int k = 0; k < N; k++) r[k] = borrow ? t[k] : d[k];
+}
+
+template <int N>
+DEV void mod_add_n(uint32_t* r, const uint32_t* a, const uint32_t* b, const uint32_t* M) {
+  uint32_t s[N], d[N];
+  uint32_t c = 0;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    uint64_t x = (uint64_t)a[k] + b[k] + c;
+    s[k] = (uint32_t)x;
+    c = (uint32_t)(x >> 32);
+  }
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    uint64_t x = (uint64_t)s[k] - M[k] - borrow;
+    d[k] = (uint32_t)x;
+    borrow = (uint32_t)(x >> 63);
+  }
+#pragma unroll
+  for (int k = 0; k < N; k++) r[k] = borrow ? s[k] : d[k];
+}
+
+template <int N>
+DEV void mod_sub_n(uint32_t* r, const uint32_t* a, const uint32_t* b, const uint32_t* M) {
+  uint32_t d[N];
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    uint64_t x = (uint64_t)a[k] - b[k] - borrow;
+    d[k] = (uint32_t)x;
+    borrow = (uint32_t)(x >> 63);
+  }
+  const uint32_t mask = 0u - borrow;
+  uint32_t c = 0;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    uint64_t x = (uint64_t)d[k] + (M[k] & mask) + c;
+    r[k] = (uint32_t)x;
+    c = (uint32_t)(x >> 32);
+  }
+}
+
+template <int N>
+DEV bool limbs_is_zero(const uint32_t* a) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int k = 0; k < N; k++) o |= a[k];
+  return o == 0;
+}
+
+template <int N>
+DEV bool limbs_eq(const uint32_t* a, const uint32_t* b) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int k = 0; k < N; k++) o |= a[k] ^ b[k];
+  return o == 0;
+}
+
+// a < b (unsigned multi-precision)
+template <int N>
+DEV bool limbs_lt(const uint32_t* a, const uint32_t* b) {
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    uint64_t x = (uint64_t)a[k] - b[k] - borrow;
+    borrow = (uint32_t)(x >> 63);
+  }
+  return borrow != 0;
+}
+
+// ------------------------------------------------------------------ Fp
+DEV void fp_mul(Fp& r, const Fp& a, const Fp& b) {
+  mont_mul_n<12>(r.v, a.v, b.v, bls::P_MOD, bls::P_INV);
+}
+DEV void fp_sqr(Fp& r, const Fp& a) { fp_mul(r, a, a); }
+DEV void fp_add(Fp& r, const Fp& a, const Fp& b) { mod_add_n<12>(r.v, a.v, b.v, bls::P_MOD); }
+DEV void fp_sub(Fp& r, const Fp& a, const Fp& b) { mod_sub_n<12>(r.v, a.v, b.v, bls::P_MOD); }
+DEV void fp_dbl(Fp& r, const Fp& a) { fp_add(r, a, a); }
+DEV bool fp_is_zero(const Fp& a) { return limbs_is_zero<12>(a.v); }
+DEV bool fp_eq(const Fp& a, const Fp& b) { return limbs_eq<12>(a.v, b.v); }
+DEV void fp_zero(Fp& r) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) r.v[k] = 0;
+}
+DEV void fp_one(Fp& r) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) r.v[k] = bls::P_ONE[k];
+}
+DEV void fp_neg(Fp& r, const Fp& a) {
+  Fp z;
+  fp_zero(z);
+  fp_sub(r, z, a);
+}
+DEV void fp_load_const(Fp& r, const uint32_t* c) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) r.v[k] = c[k];
+}
+// canonical -> Montgomery
+DEV void fp_to_mont(Fp& r, const Fp& a) {
+  Fp r2;
+  fp_load_const(r2, bls::P_R2);
+  fp_mul(r, a, r2);
+}
+// Montgomery -> canonical
+DEV void fp_from_mont(Fp& r, const Fp& a) {
+  Fp one;
+  fp_zero(one);
+  one.v[0] = 1;
+  fp_mul(r, a, one);
+}
+// r = a^e, e given as 12 little-endian limbs (compile-time constant array)
+DEVN void fp_pow_const(Fp& r, const Fp& a, const uint32_t* e) {
+  Fp acc;
+  fp_one(acc);
+  bool started = false;
+#pragma unroll 1
+  for (int i = 12 * 32 - 1; i >= 0; i--) {
+    if (started) fp_sqr(acc, acc);
+    if ((e[i >> 5] >> (i & 31)) & 1) {
+      if (started)
+        fp_mul(acc, acc, a);
+      else
+        acc = a;
+      started = true;
+    }
+  }
+  r = acc;
+}
+DEVI void fp_inv(Fp& r, const Fp& a) { fp_pow_const(r, a, bls::P_MINUS_2); }
+// returns false if a is not a square
+DEVI bool fp_sqrt(Fp& r, const Fp& a) {
+  Fp s, t;
+  fp_pow_const(s, a, bls::P_SQRT_EXP);
+  fp_sqr(t, s);
+  r = s;
+  return fp_eq(t, a);
+}
+// lexicographically largest (canonical value > (p-1)/2)
+DEV bool fp_lex_largest(const Fp& a) {
+  Fp c;
+  fp_from_mont(c, a);
+  return limbs_lt<12>(bls::P_HALF, c.v);
+}
+DEV uint32_t fp_sgn0(const Fp& a) {
+  Fp c;
+  fp_from_mont(c, a);
+  return c.v[0] & 1;
+}
+
+// ------------------------------------------------------------------ Fr
+DEV void fr_mul(Fr& r, const Fr& a, const Fr& b) {
+  mont_mul_n<8>(r.v, a.v, b.v, bls::R_MOD, bls::R_INV);
+}
+DEV void fr_add(Fr& r, const Fr& a, const Fr& b) { mod_add_n<8>(r.v, a.v, b.v, bls::R_MOD); }
+DEV void fr_sub(Fr& r, const Fr& a, const Fr& b) { mod_sub_n<8>(r.v, a.v, b.v, bls::R_MOD); }
+DEV bool fr_is_zero(const Fr& a) { return limbs_is_zero<8>(a.v); }
+DEV void fr_zero(Fr& r) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) r.v[k] = 0;
+}
+DEV void fr_neg(Fr& r, const Fr& a) {
+  Fr z;
+  fr_zero(z);
+  fr_sub(r, z, a);
+}
+DEV void fr_to_mont(Fr& r, const Fr& a) {
+  Fr r2;
+#pragma unroll
+  for (int k = 0; k < 8; k++) r2.v[k] = bls::R_R2[k];
+  fr_mul(r, a, r2);
+}
+DEV void fr_from_mont(Fr& r, const Fr& a) {
+  Fr one;
+  fr_zero(one);
+  one.v[0] = 1;
+  fr_mul(r, a, one);
+}
+DEVN void fr_inv(Fr& r, const Fr& a) {  // Montgomery in, Montgomery out
+  // r - 2 has its top set bit at position 254
+  Fr acc = a;
+#pragma unroll 1
+  for (int i = 253; i >= 0; i--) {
+    fr_mul(acc, acc, acc);
+    if ((bls::R_MINUS_2[i >> 5] >> (i & 31)) & 1) fr_mul(acc, acc, a);
+  }
+  r = acc;
+}
+DEV bool fr_canonical(const Fr& a) { return limbs_lt<8>(a.v, bls::R_MOD); }
+
+// Reduce a 512-bit little-endian value (16 limbs) mod r, result canonical
+// (not Montgomery).  v = hi * 2^256 + lo with hi, lo < 2^256 < 3r.
+DEV void fr_sub_if_geq(Fr& a) {
+  uint32_t d[8];
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    uint64_t x = (uint64_t)a.v[k] - bls::R_MOD[k] - borrow;
+    d[k] = (uint32_t)x;
+    borrow = (uint32_t)(x >> 63);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; k++) a.v[k] = borrow ? a.v[k] : d[k];
+}
+DEV void fr_reduce_wide(Fr& out, const uint32_t* v16) {
+  Fr hi, lo, r2, t;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    lo.v[k] = v16[k];
+    hi.v[k] = v16[8 + k];
+    r2.v[k] = bls::R_R2[k];
+  }
+  fr_sub_if_geq(hi);
+  fr_sub_if_geq(hi);
+  fr_sub_if_geq(lo);
+  fr_sub_if_geq(lo);
+  // mont(hi, R2) = hi * 2^256 mod r, canonical
+  fr_mul(t, hi, r2);
+  fr_add(out, t, lo);
+}
+
+// Reduce a 512-bit little-endian value mod p (canonical -> Montgomery form).
+// v = hi * 2^384 + lo, hi < 2^128, lo < 2^384 < 10 p.
+DEV void fp_sub_if_geq(Fp& a) {
+  uint32_t d[12];
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    uint64_t x = (uint64_t)a.v[k] - bls::P_MOD[k] - borrow;
+    d[k] = (uint32_t)x;
+    borrow = (uint32_t)(x >> 63);
+  }
+#pragma unroll
+  for (int k = 0; k < 12; k++) a.v[k] = borrow ? a.v[k] : d[k];
+}
+DEVN void fp_reduce_wide_to_mont(Fp& out, const uint32_t* v16) {
+  Fp lo, hi, r2, t;
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    lo.v[k] = v16[k];
+    hi.v[k] = k < 4 ? v16[12 + k] : 0u;
+    r2.v[k] = bls::P_R2[k];
+  }
+#pragma unroll 1
+  for (int it = 0; it < 9; it++) fp_sub_if_geq(lo);
+  fp_mul(t, hi, r2);       // hi * 2^384 mod p
+  fp_add(t, t, lo);
+  fp_mul(out, t, r2);      // to Montgomery
+}
+
+// ------------------------------------------------------------------ Fp2
+DEV void fp2_add(Fp2& r, const Fp2& a, const Fp2& b) { fp_add(r.c0, a.c0, b.c0); fp_add(r.c1, a.c1, b.c1); }
+DEV void fp2_sub(Fp2& r, const Fp2& a, const Fp2& b) { fp_sub(r.c0, a.c0, b.c0); fp_sub(r.c1, a.c1, b.c1); }
+DEV void fp2_dbl(Fp2& r, const Fp2& a) { fp_dbl(r.c0, a.c0); fp_dbl(r.c1, a.c1); }
+DEV void fp2_neg(Fp2& r, const Fp2& a) { fp_neg(r.c0, a.c0); fp_neg(r.c1, a.c1); }
+DEV void fp2_conj(Fp2& r, const Fp2& a) { r.c0 = a.c0; fp_neg(r.c1, a.c1); }
+DEV bool fp2_is_zero(const Fp2& a) { return fp_is_zero(a.c0) && fp_is_zero(a.c1); }
+DEV bool fp2_eq(const Fp2& a, const Fp2& b) { return fp_eq(a.c0, b.c0) && fp_eq(a.c1, b.c1); }
+DEV void fp2_zero(Fp2& r) { fp_zero(r.c0); fp_zero(r.c1); }
+DEV void fp2_one(Fp2& r) { fp_one(r.c0); fp_zero(r.c1); }
+DEVN void fp2_mul(Fp2& r, const Fp2& a, const Fp2& b) {
+  Fp v0, v1, s0, s1;
+  fp_mul(v0, a.c0, b.c0);
+  fp_mul(v1, a.c1, b.c1);
+  fp_add(s0, a.c0, a.c1);
+  fp_add(s1, b.c0, b.c1);
+  fp_mul(s0, s0, s1);
+  fp_sub(r.c0, v0, v1);
+  fp_sub(s0, s0, v0);
+  fp_sub(r.c1, s0, v1);
+}
+DEVN void fp2_sqr(Fp2& r, const Fp2& a) {
+  Fp s, d, m;
+  fp_add(s, a.c0, a.c1);
+  fp_sub(d, a.c0, a.c1);
+  fp_mul(m, a.c0, a.c1);
+  fp_mul(r.c0, s, d);
+  fp_dbl(r.c1, m);
+}
+DEV void fp2_mul_fp(Fp2& r, const Fp2& a, const Fp& b) { fp_mul(r.c0, a.c0, b); fp_mul(r.c1, a.c1, b); }
+// multiply by xi = 1 + u
+DEV void fp2_mul_xi(Fp2& r, const Fp2& a) {
+  Fp t0, t1;
+  fp_sub(t0, a.c0, a.c1);
+  fp_add(t1, a.c0, a.c1);
+  r.c0 = t0;
+  r.c1 = t1;
+}
+DEVN void fp2_inv(Fp2& r, const Fp2& a) {
+  Fp n, t;
+  fp_sqr(n, a.c0);
+  fp_sqr(t, a.c1);
+  fp_add(n, n, t);
+  fp_inv(n, n);
+  fp_mul(r.c0, a.c0, n);
+  fp_mul(t, a.c1, n);
+  fp_neg(r.c1, t);
+}
+DEVN void fp2_pow_const(Fp2& r, const Fp2& a, const uint32_t* e) {
+  Fp2 acc;
+  fp2_one(acc);
+#pragma unroll 1
+  for (int i = 12 * 32 - 1; i >= 0; i--) {
+    fp2_sqr(acc, acc);
+    if ((e[i >> 5] >> (i & 31)) & 1) fp2_mul(acc, acc, a);
+  }
+  r = acc;
+}
+DEV void fp2_load_const(Fp2& r, const uint32_t* c) {
+  fp_load_const(r.c0, c);
+  fp_load_const(r.c1, c + 12);
+}
+// Square root in Fp2, p = 3 mod 4 (Adj & Rodriguez-Henriquez, Algorithm 9).
+DEVN bool fp2_sqrt(Fp2& r, const Fp2& a) {
+  if (fp2_is_zero(a)) {
+    r = a;
+    return true;
+  }
+  Fp2 a1, alpha, x0, t, one, minus_one;
+  fp2_one(one);
+  fp2_neg(minus_one, one);
+  fp2_pow_const(a1, a, bls::P_MINUS3_DIV4);
+  fp2_mul(x0, a1, a);
+  fp2_mul(alpha, a1, x0);
+  if (fp2_eq(alpha, minus_one)) {
+    // x = u * x0
+    fp_neg(t.c0, x0.c1);
+    t.c1 = x0.c0;
+  } else {
+    Fp2 b;
+    fp2_add(b, alpha, one);
+    fp2_pow_const(b, b, bls::P_LEGENDRE_EXP);
+    fp2_mul(t, b, x0);
+  }
+  Fp2 chk;
+  fp2_sqr(chk, t);
+  r = t;
+  return fp2_eq(chk, a);
+}
+DEV bool fp2_lex_largest(const Fp2& a) {
+  if (!fp_is_zero(a.c1)) return fp_lex_largest(a.c1);
+  return fp_lex_largest(a.c0);
+}
+
+// ------------------------------------------------------------------ Fp6
+DEV void fp6_add(Fp6& r, const Fp6& a, const Fp6& b) { fp2_add(r.c0, a.c0, b.c0); fp2_add(r.c1, a.c1, b.c1); fp2_add(r.c2, a.c2, b.c2); }
+DEV void fp6_sub(Fp6& r, const Fp6& a, const Fp6& b) { fp2_sub(r.c0, a.c0, b.c0); fp2_sub(r.c1, a.c1, b.c1); fp2_sub(r.c2, a.c2, b.c2); }
+DEV void fp6_neg(Fp6& r, const Fp6& a) { fp2_neg(r.c0, a.c0); fp2_neg(r.c1, a.c1); fp2_neg(r.c2, a.c2); }
+DEV void fp6_zero(Fp6& r) { fp2_zero(r.c0); fp2_zero(r.c1); fp2_zero(r.c2); }
+DEVN void fp6_mul(Fp6& r, const Fp6& a, const Fp6& b) {
+  Fp2 t0, t1, t2, s0, s1, c0, c1, c2;
+  fp2_mul(t0, a.c0, b.c0);
+  fp2_mul(t1, a.c1, b.c1);
+  fp2_mul(t2, a.c2, b.c2);
+  fp2_add(s0, a.c1, a.c2);
+  fp2_add(s1, b.c1, b.c2);
+  fp2_mul(c0, s0, s1);
+  fp2_sub(c0, c0, t1);
+  fp2_sub(c0, c0, t2);
+  fp2_mul_xi(c0, c0);
+  fp2_add(c0, c0, t0);
+  fp2_add(s0, a.c0, a.c1);
+  fp2_add(s1, b.c0, b.c1);
+  fp2_mul(c1, s0, s1);
+  fp2_sub(c1, c1, t0);
+  fp2_sub(c1, c1, t1);
+  fp2_mul_xi(s0, t2);
+  fp2_add(c1, c1, s0);
+  fp2_add(s0, a.c0, a.c2);
+  fp2_add(s1, b.c0, b.c2);
+  fp2_mul(c2, s0, s1);
+  fp2_sub(c2, c2, t0);
+  fp2_sub(c2, c2, t2);
+  fp2_add(r.c2, c2, t1);
+  r.c0 = c0;
+  r.c1 = c1;
+}
+// Chung-Hasan SQR2
+DEVN void fp6_sqr(Fp6& r, const Fp6& a) {
+  Fp2 s0, s1, s2, s3, s4, t;
+  fp2_sqr(s0, a.c0);
+  fp2_mul(s1, a.c0, a.c1);
+  fp2_dbl(s1, s1);
+  fp2_sub(t, a.c0, a.c1);
+  fp2_add(t, t, a.c2);
+  fp2_sqr(s2, t);
+  fp2_mul(s3, a.c1, a.c2);
+  fp2_dbl(s3, s3);
+  fp2_sqr(s4, a.c2);
+  // c0 = s0 + xi s3 ; c1 = s1 + xi s4 ; c2 = s1 + s2 + s3 - s0 - s4
+  Fp2 c0, c1, c2;
+  fp2_mul_xi(t, s3);
+  fp2_add(c0, s0, t);
+  fp2_mul_xi(t, s4);
+  fp2_add(c1, s1, t);
+  fp2_add(c2, s1, s2);
+  fp2_add(c2, c2, s3);
+  fp2_sub(c2, c2, s0);
+  fp2_sub(c2, c2, s4);
+  r.c0 = c0;
+  r.c1 = c1;
+  r.c2 = c2;
+}
+DEV void fp6_mul_v(Fp6& r, const Fp6& a) {
+  Fp2 t;
+  fp2_mul_xi(t, a.c2);
+  r.c2 = a.c1;
+  r.c1 = a.c0;
+  r.c0 = t;
+}
+DEVN void fp6_inv(Fp6& r, const Fp6& a) {
+  Fp2 t0, t1, t2, tmp, d;
+  fp2_sqr(t0, a.c0);
+  fp2_mul(tmp, a.c1, a.c2);
+  fp2_mul_xi(tmp, tmp);
+  fp2_sub(t0, t0, tmp);
+  fp2_sqr(t1, a.c2);
+  fp2_mul_xi(t1, t1);
+  fp2_mul(tmp, a.c0, a.c1);
+  fp2_sub(t1, t1, tmp);
+  fp2_sqr(t2, a.c1);
+  fp2_mul(tmp, a.c0, a.c2);
+  fp2_sub(t2, t2, tmp);
+  fp2_mul(d, a.c2, t1);
+  fp2_mul(tmp, a.c1, t2);
+  fp2_add(d, d, tmp);
+  fp2_mul_xi(d, d);
+  fp2_mul(tmp, a.c0, t0);
+  fp2_add(d, d, tmp);
+  fp2_inv(d, d);
+  fp2_mul(r.c0, t0, d);
+  fp2_mul(r.c1, t1, d);
+  fp2_mul(r.c2, t2, d);
+}
+
+// ------------------------------------------------------------------ Fp12
+DEV void fp12_one(Fp12& r) {
+  fp6_zero(r.c0);
+  fp6_zero(r.c1);
+  fp_one(r.c0.c0.c0);
+}
+DEVN bool fp12_is_one(const Fp12& a) {
+  Fp one;
+  fp_one(one);
+  bool ok = fp_eq(a.c0.c0.c0, one) && fp_is_zero(a.c0.c0.c1);
+  ok = ok && fp2_is_zero(a.c0.c1) && fp2_is_zero(a.c0.c2);
+  ok = ok && fp2_is_zero(a.c1.c0) && fp2_is_zero(a.c1.c1) && fp2_is_zero(a.c1.c2);
+  return ok;
+}
+DEVN void fp12_mul(Fp12& r, const Fp12& a, const Fp12& b) {
+  Fp6 t0, t1, s0, s1;
+  fp6_mul(t0, a.c0, b.c0);
+  fp6_mul(t1, a.c1, b.c1);
+  fp6_add(s0, a.c0, a.c1);
+  fp6_add(s1, b.c0, b.c1);
+  fp6_mul(s0, s0, s1);
+  fp6_sub(s0, s0, t0);
+  fp6_sub(r.c1, s0, t1);
+  fp6_mul_v(t1, t1);
+  fp6_add(r.c0, t0, t1);
+}
+// complex squaring: (a0 + a1 w)^2 = (a0^2 + v a1^2) + 2 a0 a1 w
+DEVN void fp12_sqr(Fp12& r, const Fp12& a) {
+  Fp6 t0, t1, s;
+  fp6_mul(t0, a.c0, a.c1);
+  fp6_add(s, a.c0, a.c1);
+  fp6_mul_v(t1, a.c1);
+  fp6_add(t1, a.c0, t1);
+  fp6_mul(s, s, t1);          // (a0 + a1)(a0 + v a1) = a0^2 + v a1^2 + (1+v) a0 a1
+  fp6_sub(s, s, t0);
+  fp6_mul_v(t1, t0);
+  fp6_sub(r.c0, s, t1);
+  fp6_add(r.c1, t0, t0);
+}
+DEV void fp12_conj(Fp12& r, const Fp12& a) {
+  r.c0 = a.c0;
+  fp6_neg(r.c1, a.c1);
+}
+DEVN void fp12_inv(Fp12& r, const Fp12& a) {
+  Fp6 t0, t1;
+  fp6_sqr(t0, a.c0);
+  fp6_sqr(t1, a.c1);
+  fp6_mul_v(t1, t1);
+  fp6_sub(t0, t0, t1);
+  fp6_inv(t0, t0);
+  fp6_mul(r.c0, a.c0, t0);
+  fp6_mul(t1, a.c1, t0);
+  fp6_neg(r.c1, t1);
+}
+// Frobenius x -> x^p.  Coefficient of w^k (k = 2i + j for c_j.c_i) is
+// conjugated and multiplied by gamma1[k] = xi^(k (p-1)/6).
+DEVN void fp12_frob(Fp12& r, const Fp12& a) {
+  Fp2 g;
+  fp2_conj(r.c0.c0, a.c0.c0);
+  fp2_conj(r.c0.c1, a.c0.c1);
+  fp2_load_const(g, bls::FROB1 + 24 * 2);
+  fp2_mul(r.c0.c1, r.c0.c1, g);
+  fp2_conj(r.c0.c2, a.c0.c2);
+  fp2_load_const(g, bls::FROB1 + 24 * 4);
+  fp2_mul(r.c0.c2, r.c0.c2, g);
+  fp2_conj(r.c1.c0, a.c1.c0);
+  fp2_load_const(g, bls::FROB1 + 24 * 1);
+  fp2_mul(r.c1.c0, r.c1.c0, g);
+  fp2_conj(r.c1.c1, a.c1.c1);
+  fp2_load_const(g, bls::FROB1 + 24 * 3);
+  fp2_mul(r.c1.c1, r.c1.c1, g);
+  fp2_conj(r.c1.c2, a.c1.c2);
+  fp2_load_const(g, bls::FROB1 + 24 * 5);
+  fp2_mul(r.c1.c2, r.c1.c2, g);
+}
+// x -> x^(p^2): coefficients stay, scaled by gamma2[k] in Fp.
+DEVN void fp12_frob2(Fp12& r, const Fp12& a) {
+  Fp g;
+  r.c0.c0 = a.c0.c0;
+  fp_load_const(g, bls::FROB2 + 12 * 2);
+  fp2_mul_fp(r.c0.c1, a.c0.c1, g);
+  fp_load_const(g, bls::FROB2 + 12 * 4);
+  fp2_mul_fp(r.c0.c2, a.c0.c2, g);
+  fp_load_const(g, bls::FROB2 + 12 * 1);
+  fp2_mul_fp(r.c1.c0, a.c1.c0, g);
+  fp_load_const(g, bls::FROB2 + 12 * 3);
+  fp2_mul_fp(r.c1.c1, a.c1.c1, g);
+  fp_load_const(g, bls::FROB2 + 12 * 5);
+  fp2_mul_fp(r.c1.c2, a.c1.c2, g);
+}
+
+// ------------------------------------------------------------------ byte codecs
+// 48-byte big-endian -> canonical limbs; false when >= p (flags must be cleared
+// by the caller).
+DEV bool fp_from_be_bytes(Fp& r, const uint8_t* be) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    const uint8_t* q = be + 44 - 4 * k;
+    r.v[k] = ((uint32_t)q[0] << 24) | ((uint32_t)q[1] << 16) | ((uint32_t)q[2] << 8) | q[3];
+  }
+  return limbs_lt<12>(r.v, bls::P_MOD);
+}
+DEV void fp_to_be_bytes(uint8_t* be, const Fp& canonical) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    uint8_t* q = be + 44 - 4 * k;
+    uint32_t w = canonical.v[k];
+    q[0] = (uint8_t)(w >> 24);
+    q[1] = (uint8_t)(w >> 16);
+    q[2] = (uint8_t)(w >> 8);
+    q[3] = (uint8_t)w;
+  }
+}
+DEV void fr_from_be_bytes_raw(Fr& r, const uint8_t* be) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint8_t* q = be + 28 - 4 * k;
+    r.v[k] = ((uint32_t)q[0] << 24) | ((uint32_t)q[1] << 16) | ((uint32_t)q[2] << 8) | q[3];
+  }
+}
+DEV void fr_to_be_bytes(uint8_t* be, const Fr& canonical) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    uint8_t* q = be + 28 - 4 * k;
+    uint32_t w = canonical.v[k];
+    q[0] = (uint8_t)(w >> 24);
+    q[1] = (uint8_t)(w >> 16);
+    q[2] = (uint8_t)(w >> 8);
+    q[3] = (uint8_t)w;
+  }
+}

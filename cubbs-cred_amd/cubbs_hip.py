@@ -1,0 +1,330 @@
+"""ctypes binding of libcubbs_bbs_hip.so (the C ABI v2 of cuBBS-Cred plus the
+extensions in include/cubbs_hip_ext.h).
+
+This is the Python-side mirror of the reference's FFI layer
+(crates/cubbs-bbs/src/gpu_abi.rs for the struct layouts and
+crates/cubbs-bbs/src/gpu_runtime.rs for how batches are marshalled): the same
+items, the same result codes, the same per-item error behaviour.  Loading
+fails loudly when the shared library is missing; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("CUBBS_BBS_GPU_LIBRARY") or os.path.join(HERE, "libcubbs_bbs_hip.so")
+
+ABI_VERSION = 2
+SUCCESS, ERR_INVALID_INPUT, ERR_INSUFFICIENT_OUTPUT, ERR_INTERNAL = 0, 1, 2, 3
+BATCH_VERIFY, BATCH_PROOF_VERIFY = 0, 1
+
+API_ID = b"BBS_BLS12381G1_XMD:SHA-256_SSWU_RO_H2G_HM2S_"
+MAP_DST = API_ID + b"MAP_MSG_TO_SCALAR_AS_HASH_"
+H2S_DST = API_ID + b"H2S_"
+
+# Every exported symbol declared in include/cubbs_bbs_gpu_abi.h and
+# include/cubbs_hip_ext.h (checked by the CPU test suite).
+ABI_SYMBOLS = [
+    "cubbs_bbs_gpu_backend_info_v2",
+    "cubbs_bbs_gpu_sign_batch_v2",
+    "cubbs_bbs_gpu_verify_batch_v2",
+    "cubbs_bbs_gpu_proof_gen_batch_v2",
+    "cubbs_bbs_gpu_proof_verify_batch_v2",
+    "cubbs_bbs_gpu_messages_to_scalar_batch_v2",
+    "cubbs_bbs_gpu_sha256_batch_for_test_v2",
+    "cubbs_bbs_gpu_expand_xmd_sha256_48_batch_for_test_v2",
+    "cubbs_bbs_gpu_hash_to_scalar_sha256_batch_for_test_v2",
+]
+EXT_SYMBOLS = [
+    "cubbs_hip_batch_prepare",
+    "cubbs_hip_batch_launch",
+    "cubbs_hip_batch_sync",
+    "cubbs_hip_batch_results",
+    "cubbs_hip_batch_free",
+    "cubbs_hip_batch_profile",
+    "cubbs_hip_batch_shape",
+    "cubbs_hip_test_g1_mul",
+    "cubbs_hip_test_pairing_gt",
+    "cubbs_hip_test_g2_roundtrip",
+    "cubbs_hip_generators",
+    "cubbs_hip_fp_mul_bench",
+]
+
+
+class ByteSlice(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("len", ctypes.c_size_t)]
+
+
+class ByteBuffer(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("capacity", ctypes.c_size_t), ("len", ctypes.c_size_t)]
+
+
+class IndexSlice(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("len", ctypes.c_size_t)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_int32), ("value", ctypes.c_uint8), ("bytes_written", ctypes.c_size_t)]
+
+
+class BackendInfo(ctypes.Structure):
+    _fields_ = [("backend_kind", ctypes.c_uint32), ("hardware_acceleration", ctypes.c_uint8)]
+
+
+class SignItem(ctypes.Structure):
+    _fields_ = [("secret_key", ByteSlice), ("public_key", ByteSlice), ("header", ByteSlice),
+                ("messages", ctypes.c_void_p), ("message_count", ctypes.c_size_t),
+                ("signature_out", ByteBuffer)]
+
+
+class VerifyItem(ctypes.Structure):
+    _fields_ = [("public_key", ByteSlice), ("signature", ByteSlice), ("header", ByteSlice),
+                ("messages", ctypes.c_void_p), ("message_count", ctypes.c_size_t)]
+
+
+class ProofGenItem(ctypes.Structure):
+    _fields_ = [("public_key", ByteSlice), ("signature", ByteSlice), ("header", ByteSlice),
+                ("presentation_header", ByteSlice), ("messages", ctypes.c_void_p),
+                ("message_count", ctypes.c_size_t), ("disclosed_indexes", IndexSlice),
+                ("proof_out", ByteBuffer)]
+
+
+class ProofVerifyItem(ctypes.Structure):
+    _fields_ = [("public_key", ByteSlice), ("proof", ByteSlice), ("header", ByteSlice),
+                ("presentation_header", ByteSlice), ("disclosed_messages", ctypes.c_void_p),
+                ("disclosed_count", ctypes.c_size_t), ("disclosed_indexes", IndexSlice),
+                ("total_message_count", ctypes.c_size_t)]
+
+
+class MessagesToScalarItem(ctypes.Structure):
+    _fields_ = [("message", ByteSlice), ("dst", ByteSlice), ("scalar_out", ByteBuffer)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libcubbs_bbs_hip.so not built (%s): run __graft_entry__.build()" % LIB_PATH)
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.cubbs_hip_batch_prepare.restype = ctypes.c_void_p
+        _lib.cubbs_hip_batch_prepare.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.c_size_t, ctypes.POINTER(ctypes.c_int32)]
+        for name in ("cubbs_hip_batch_launch", "cubbs_hip_batch_sync"):
+            getattr(_lib, name).argtypes = [ctypes.c_void_p]
+        _lib.cubbs_hip_batch_results.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _lib.cubbs_hip_batch_free.argtypes = [ctypes.c_void_p]
+        _lib.cubbs_hip_batch_free.restype = None
+        _lib.cubbs_hip_batch_profile.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p,
+                                                 ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.c_void_p]
+        _lib.cubbs_hip_batch_shape.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+    return _lib
+
+
+class _Keep:
+    """Holds ctypes buffers alive for the duration of a call."""
+
+    def __init__(self):
+        self.refs = []
+
+    def slice(self, b):
+        b = bytes(b)
+        if not b:
+            return ByteSlice(None, 0)
+        buf = ctypes.create_string_buffer(b, len(b))
+        self.refs.append(buf)
+        return ByteSlice(ctypes.cast(buf, ctypes.c_void_p), len(b))
+
+    def slices(self, items):
+        arr = (ByteSlice * max(1, len(items)))(*[self.slice(m) for m in items])
+        self.refs.append(arr)
+        return ctypes.cast(arr, ctypes.c_void_p) if items else None
+
+    def indexes(self, idx):
+        arr = (ctypes.c_size_t * max(1, len(idx)))(*idx)
+        self.refs.append(arr)
+        return IndexSlice(ctypes.cast(arr, ctypes.c_void_p) if idx else None, len(idx))
+
+    def out(self, cap):
+        buf = ctypes.create_string_buffer(max(1, cap))
+        self.refs.append(buf)
+        return buf, ByteBuffer(ctypes.cast(buf, ctypes.c_void_p), cap, 0)
+
+
+def backend_info():
+    info = BackendInfo()
+    rc = lib().cubbs_bbs_gpu_backend_info_v2(ABI_VERSION, ctypes.byref(info))
+    return rc, info.backend_kind, info.hardware_acceleration
+
+
+def build_verify_items(reqs, keep):
+    """reqs: list of (pk, sig, header, [messages])."""
+    items = (VerifyItem * max(1, len(reqs)))()
+    for i, (pk, sig, hdr, msgs) in enumerate(reqs):
+        items[i].public_key = keep.slice(pk)
+        items[i].signature = keep.slice(sig)
+        items[i].header = keep.slice(hdr)
+        items[i].messages = keep.slices(msgs)
+        items[i].message_count = len(msgs)
+    keep.refs.append(items)
+    return items
+
+
+def verify_batch(reqs):
+    """Returns (status, [(code, valid)])."""
+    keep = _Keep()
+    items = build_verify_items(reqs, keep)
+    res = (Result * max(1, len(reqs)))()
+    rc = lib().cubbs_bbs_gpu_verify_batch_v2(ABI_VERSION, items, ctypes.c_size_t(len(reqs)), res)
+    return rc, [(res[i].code, bool(res[i].value)) for i in range(len(reqs))]
+
+
+def messages_to_scalars(msgs, dst=MAP_DST):
+    keep = _Keep()
+    items = (MessagesToScalarItem * max(1, len(msgs)))()
+    outs = []
+    for i, m in enumerate(msgs):
+        items[i].message = keep.slice(m)
+        items[i].dst = keep.slice(dst)
+        buf, bb = keep.out(32)
+        items[i].scalar_out = bb
+        outs.append(buf)
+    res = (Result * max(1, len(msgs)))()
+    rc = lib().cubbs_bbs_gpu_messages_to_scalar_batch_v2(ABI_VERSION, items, ctypes.c_size_t(len(msgs)), res)
+    return rc, [o.raw[:32] for o in outs]
+
+
+def _packed(msgs):
+    data = b"".join(msgs)
+    offs, cur = [], 0
+    for m in msgs:
+        offs.append(cur)
+        cur += len(m)
+    n = max(1, len(msgs))
+    return (ctypes.create_string_buffer(data, max(1, len(data))),
+            (ctypes.c_size_t * n)(*offs), (ctypes.c_size_t * n)(*[len(m) for m in msgs]))
+
+
+def sha256_batch(msgs):
+    data, offs, lens = _packed(msgs)
+    out = ctypes.create_string_buffer(32 * max(1, len(msgs)))
+    rc = lib().cubbs_bbs_gpu_sha256_batch_for_test_v2(ABI_VERSION, data, offs, lens, ctypes.c_size_t(len(msgs)), out)
+    return rc, [out.raw[32 * i:32 * i + 32] for i in range(len(msgs))]
+
+
+def expand_xmd48_batch(msgs, dst):
+    data, offs, lens = _packed(msgs)
+    out = ctypes.create_string_buffer(48 * max(1, len(msgs)))
+    d = ctypes.create_string_buffer(dst, max(1, len(dst)))
+    rc = lib().cubbs_bbs_gpu_expand_xmd_sha256_48_batch_for_test_v2(
+        ABI_VERSION, data, offs, lens, ctypes.c_size_t(len(msgs)), d, ctypes.c_size_t(len(dst)), out)
+    return rc, [out.raw[48 * i:48 * i + 48] for i in range(len(msgs))]
+
+
+def hash_to_scalar_batch(msgs, dst):
+    data, offs, lens = _packed(msgs)
+    out = ctypes.create_string_buffer(32 * max(1, len(msgs)))
+    d = ctypes.create_string_buffer(dst, max(1, len(dst)))
+    rc = lib().cubbs_bbs_gpu_hash_to_scalar_sha256_batch_for_test_v2(
+        ABI_VERSION, data, offs, lens, ctypes.c_size_t(len(msgs)), d, ctypes.c_size_t(len(dst)), out)
+    return rc, [out.raw[32 * i:32 * i + 32] for i in range(len(msgs))]
+
+
+def g1_mul(points, scalars):
+    n = len(points)
+    out = ctypes.create_string_buffer(48 * max(1, n))
+    ok = ctypes.create_string_buffer(max(1, n))
+    rc = lib().cubbs_hip_test_g1_mul(b"".join(points), b"".join(scalars), ctypes.c_size_t(n), out, ok)
+    return rc, [(out.raw[48 * i:48 * i + 48] if ok.raw[i] else None) for i in range(n)]
+
+
+def pairing_gt(g1s, g2s):
+    n = len(g1s)
+    out = ctypes.create_string_buffer(576 * max(1, n))
+    ok = ctypes.create_string_buffer(max(1, n))
+    rc = lib().cubbs_hip_test_pairing_gt(b"".join(g1s), b"".join(g2s), ctypes.c_size_t(n), out, ok)
+    return rc, [(out.raw[576 * i:576 * i + 576] if ok.raw[i] else None) for i in range(n)]
+
+
+def g2_roundtrip(g2s):
+    n = len(g2s)
+    out = ctypes.create_string_buffer(96 * max(1, n))
+    ok = ctypes.create_string_buffer(max(1, n))
+    rc = lib().cubbs_hip_test_g2_roundtrip(b"".join(g2s), ctypes.c_size_t(n), out, ok)
+    return rc, [(out.raw[96 * i:96 * i + 96] if ok.raw[i] else None) for i in range(n)]
+
+
+def generators(count):
+    out = ctypes.create_string_buffer(48 * count)
+    rc = lib().cubbs_hip_generators(ctypes.c_size_t(count), out)
+    return rc, [out.raw[48 * i:48 * i + 48] for i in range(count)]
+
+
+def fp_mul_bench(lanes, iters):
+    ms = ctypes.c_double()
+    rc = lib().cubbs_hip_fp_mul_bench(ctypes.c_uint32(lanes), ctypes.c_uint32(iters), ctypes.byref(ms))
+    return rc, ms.value
+
+
+class StagedBatch:
+    """A batch whose inputs stay resident on the device (prepare once, launch many)."""
+
+    def __init__(self, kind, reqs):
+        self.keep = _Keep()
+        if kind == BATCH_VERIFY:
+            items = build_verify_items(reqs, self.keep)
+        else:
+            raise ValueError("unsupported batch kind")
+        st = ctypes.c_int32(0)
+        self.n = len(reqs)
+        self.h = lib().cubbs_hip_batch_prepare(ABI_VERSION, kind, ctypes.cast(items, ctypes.c_void_p),
+                                               ctypes.c_size_t(len(reqs)), ctypes.byref(st))
+        self.keep = None  # items are only read during prepare
+        if not self.h:
+            raise RuntimeError("batch prepare failed with status %d" % st.value)
+
+    def launch(self):
+        rc = lib().cubbs_hip_batch_launch(self.h)
+        if rc:
+            raise RuntimeError("launch failed %d" % rc)
+
+    def sync(self):
+        rc = lib().cubbs_hip_batch_sync(self.h)
+        if rc:
+            raise RuntimeError("sync failed %d" % rc)
+
+    def results(self):
+        res = (Result * max(1, self.n))()
+        rc = lib().cubbs_hip_batch_results(self.h, res)
+        if rc:
+            raise RuntimeError("results failed %d" % rc)
+        return [(res[i].code, bool(res[i].value)) for i in range(self.n)]
+
+    def profile(self, iters):
+        names = ctypes.create_string_buffer(4096)
+        ms = (ctypes.c_double * 64)()
+        total = ctypes.c_double()
+        k = lib().cubbs_hip_batch_profile(self.h, iters, names, 4096, ms, 64, ctypes.byref(total))
+        if k < 0:
+            raise RuntimeError("profile failed %d" % k)
+        nm = names.value.decode().split("\n")[:k]
+        return total.value, dict(zip(nm, [ms[i] for i in range(k)]))
+
+    def shape(self):
+        v = [ctypes.c_uint64() for _ in range(4)]
+        lib().cubbs_hip_batch_shape(self.h, *[ctypes.byref(x) for x in v])
+        return dict(zip(["items", "msm_terms", "comb_madds", "pairings"], [x.value for x in v]))
+
+    def free(self):
+        if self.h:
+            lib().cubbs_hip_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

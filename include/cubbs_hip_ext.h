@@ -1,0 +1,82 @@
+/*
+ * Extensions of libcubbs_bbs_hip.so beyond the reference ABI v2.
+ *
+ * None of these has a counterpart in the reference header
+ * (native/cuda/include/cubbs_bbs_gpu_abi.h); they serve callers the reference
+ * has no entry for:
+ *   - reproducible ProofGen (seeded blinding, or explicitly injected scalars
+ *     as in the CFRG fixtures' traces);
+ *   - staged batches, so a benchmark or a serving loop can keep inputs
+ *     resident on the device and launch the pipeline repeatedly;
+ *   - small primitive probes used by the parity tests.
+ * Same conventions as the ABI: caller-owned memory, status codes
+ * CUBBS_BBS_GPU_*, abi_version must be CUBBS_BBS_GPU_ABI_VERSION.
+ */
+#ifndef CUBBS_HIP_EXT_H
+#define CUBBS_HIP_EXT_H
+
+#include "cubbs_bbs_gpu_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ProofGen whose blinding scalars come from ChaCha20 (RFC 8439) keyed by a
+ * 32-byte seed per item, zero nonce, block counter k for scalar k; each
+ * 64-byte block read as a little-endian integer and reduced mod r.  Scalar
+ * order: r1, r2, e~, r1~, r3~, m~_1..m~_U (U = undisclosed count). */
+int32_t cubbs_hip_proof_gen_seeded_batch(uint32_t abi_version, CuBbsBbsGpuProofGenItem *items,
+                                         size_t item_count, const uint8_t *seeds32,
+                                         CuBbsBbsGpuResult *results);
+
+/* ProofGen with the blinding scalars given explicitly: item i reads
+ * (5 + U_i) big-endian 32-byte scalars starting at scalars_be + 32 * offsets[i]. */
+int32_t cubbs_hip_proof_gen_with_scalars_batch(uint32_t abi_version,
+                                               CuBbsBbsGpuProofGenItem *items, size_t item_count,
+                                               const uint8_t *scalars_be, const size_t *offsets,
+                                               CuBbsBbsGpuResult *results);
+
+/* ---- staged batches: prepare (host packing + upload), launch (enqueue the
+ * device pipeline on the batch's own stream), sync, read results, free.
+ * kind: 0 = Verify, 1 = ProofVerify.  Items are read only during prepare. */
+#define CUBBS_HIP_BATCH_VERIFY 0
+#define CUBBS_HIP_BATCH_PROOF_VERIFY 1
+void *cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void *items, size_t item_count,
+                              int32_t *status_out);
+int32_t cubbs_hip_batch_launch(void *batch);
+int32_t cubbs_hip_batch_sync(void *batch);
+int32_t cubbs_hip_batch_results(void *batch, CuBbsBbsGpuResult *results);
+void cubbs_hip_batch_free(void *batch);
+/* Kernel timing with HIP events recorded on the batch's stream around every
+ * launch of `iters` back-to-back pipeline runs.  names_out receives a
+ * newline-separated list of kernel names (capacity names_cap bytes); ms_out
+ * receives the average milliseconds per launch of each name, in that order.
+ * Returns the number of kernels (<= max_kernels), negative on error. */
+int32_t cubbs_hip_batch_profile(void *batch, int iters, char *names_out, size_t names_cap,
+                                double *ms_out, int max_kernels, double *total_ms_out);
+/* Work units of the batch for the roofline: number of items, total terms of
+ * the fixed-base MSM, pairings. */
+int32_t cubbs_hip_batch_shape(void *batch, uint64_t *items, uint64_t *msm_terms,
+                              uint64_t *comb_madds, uint64_t *pairings);
+
+/* ---- primitive probes (parity tests) */
+/* out = k * P for n compressed points and 32-byte big-endian scalars; ok[i] = 0
+ * when the point or scalar does not decode. */
+int32_t cubbs_hip_test_g1_mul(const uint8_t *points48, const uint8_t *scalars32, size_t n,
+                              uint8_t *out48, uint8_t *ok);
+/* e(P, Q) after final exponentiation (the pairing cubed, see pairing.hpp),
+ * 12 Fp coefficients of 48 bytes, order c0.c0.c0, c0.c0.c1, c0.c1.c0 ... */
+int32_t cubbs_hip_test_pairing_gt(const uint8_t *g1_48, const uint8_t *g2_96, size_t n,
+                                  uint8_t *out576, uint8_t *ok);
+/* decompress + subgroup check + recompress of G2 points */
+int32_t cubbs_hip_test_g2_roundtrip(const uint8_t *g2_96, size_t n, uint8_t *out96, uint8_t *ok);
+/* compressed generators [P1, Q1, H_1 .. H_{count-2}] as built on the device */
+int32_t cubbs_hip_generators(size_t count, uint8_t *out48);
+/* Fp multiplication throughput probe: lanes x iters x 2 dependent chains;
+ * returns elapsed milliseconds of the kernel (HIP events) in *ms. */
+int32_t cubbs_hip_fp_mul_bench(uint32_t lanes, uint32_t iters, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
