@@ -1,0 +1,225 @@
+#!/usr/bin/env python3
+"""Benchmark: batched Verify + ProofVerify ops/s on one MI355X.
+
+Workload (one "step" = one pass of the device pipeline over both batches,
+inputs resident in HBM):
+  * 8192 Verify: signatures from 256 seeded issuer keys, message count
+    log-uniform in [1, 128], messages 32..256 random bytes, a 64-byte random
+    header per signature, ~1% tampered (a message replaced);
+  * 2048 ProofVerify: proofs over L = 32 messages of 64 random bytes, a seeded
+    random 25% disclosed, 64-byte presentation header, blinding scalars from
+    seeded ChaCha20.
+All inputs (keys, signatures, proofs) are produced by the library itself
+(keygen, Sign and ProofGen run on the device) from fixed seeds; the timed
+region only launches the staged Verify and ProofVerify pipelines on their two
+streams and waits for both.
+
+Prints one JSON line with the metric, a roofline object for the dominant
+kernel (VALU v_mad_u64_u32 bound), and the CPU baseline (the
+oracle's restatement of the reference CPU path, timed on the host's cores on a
+bounded sample of the same workload).
+"""
+import argparse
+import concurrent.futures as cf
+import hashlib
+import json
+import math
+import os
+import random
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cubbs-cred_amd"))
+
+N_VERIFY = 8192
+N_ISSUERS = 256
+N_PROOFS = 2048
+PROOF_L = 32
+# algorithmic work model (Fp multiplications) of one 2-pairing check with
+# prepared G2 lines: two Miller loops (~5,200 FpM each) and one final
+# exponentiation (~11,000 FpM); one FpM = 300 v_mad_u64_u32 (12x32-bit CIOS).
+FPM_PER_PAIRING_CHECK = 2 * 5200 + 11000
+MADS_PER_FPM = 300
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def log_uniform_L(rng):
+    return max(1, min(128, int(math.exp(rng.uniform(0, math.log(129))))))
+
+
+def build_workload(g, seed=b"cubbs-hip/bench"):
+    rng = random.Random(hashlib.sha256(seed).digest())
+    rc, kps = g.keygen([rng.randbytes(32) for _ in range(N_ISSUERS)])
+    assert rc == 0
+    # ---- Verify batch
+    sreqs = []
+    for i in range(N_VERIFY):
+        sk, pk = kps[rng.randrange(N_ISSUERS)]
+        L = log_uniform_L(rng)
+        msgs = [rng.randbytes(rng.randint(32, 256)) for _ in range(L)]
+        sreqs.append((sk, pk, rng.randbytes(64), msgs))
+    t = time.time()
+    rc, sigs = g.sign_batch(sreqs)
+    assert rc == 0 and all(c == 0 for c, _ in sigs)
+    log("signed %d in %.2fs" % (N_VERIFY, time.time() - t))
+    vreqs, expect_v = [], []
+    for (sk, pk, hdr, msgs), (_, sig) in zip(sreqs, sigs):
+        ok = True
+        if rng.random() < 0.01:
+            msgs = list(msgs)
+            msgs[rng.randrange(len(msgs))] = rng.randbytes(32)
+            ok = False
+        vreqs.append((pk, sig, hdr, msgs))
+        expect_v.append(ok)
+    # ---- ProofVerify batch
+    preqs = []
+    for i in range(N_PROOFS):
+        sk, pk = kps[rng.randrange(N_ISSUERS)]
+        msgs = [rng.randbytes(64) for _ in range(PROOF_L)]
+        preqs.append((sk, pk, rng.randbytes(64), msgs))
+    rc, psigs = g.sign_batch(preqs)
+    assert rc == 0
+    greqs, seeds = [], []
+    for (sk, pk, hdr, msgs), (_, sig) in zip(preqs, psigs):
+        idx = sorted(rng.sample(range(PROOF_L), PROOF_L // 4))
+        greqs.append((pk, sig, hdr, rng.randbytes(64), msgs, idx))
+        seeds.append(rng.randbytes(32))
+    t = time.time()
+    rc, proofs = g.proof_gen_batch(greqs, seeds=seeds)
+    assert rc == 0 and all(c == 0 for c, _ in proofs)
+    log("proofs generated %d in %.2fs" % (N_PROOFS, time.time() - t))
+    pvreqs = [(pk, proof, hdr, ph, [msgs[i] for i in idx], idx, len(msgs))
+              for (pk, sig, hdr, ph, msgs, idx), (_, proof) in zip(greqs, proofs)]
+    return vreqs, expect_v, pvreqs, kps
+
+
+def cpu_baseline(vreqs, pvreqs, budget_s=20.0):
+    """The oracle (C restatement of the reference CPU path) on the host cores,
+    on a bounded random sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as o
+    o.lib()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    rng = random.Random(1)
+    # estimate per-op cost with a few single-thread calls, then size the sample
+    t = time.time()
+    for r in rng.sample(vreqs, 4):
+        o.verify(*r)
+    for r in rng.sample(pvreqs, 2):
+        o.proof_verify(*r)
+    per = (time.time() - t) / 6
+    n_total = max(threads * 4, int(budget_s / max(per, 1e-4)))
+    n_v = int(n_total * N_VERIFY / (N_VERIFY + N_PROOFS))
+    n_p = max(1, n_total - n_v)
+    sample = [("v", r) for r in rng.sample(vreqs, min(n_v, len(vreqs)))] + \
+             [("p", r) for r in rng.sample(pvreqs, min(n_p, len(pvreqs)))]
+
+    def run(job):
+        kind, r = job
+        return o.verify(*r) if kind == "v" else o.proof_verify(*r)
+
+    t = time.time()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(run, sample))
+    el = time.time() - t
+    nv = sum(1 for k, _ in sample if k == "v")
+    return {"value": round(len(sample) / el, 2), "unit": "ops/s", "cores": threads, "kind": "port",
+            "sample": "%d Verify + %d ProofVerify drawn from the benchmark batches (oracle C code, "
+                      "%d threads)" % (nv, len(sample) - nv, threads)}
+
+
+def per_request_gpu(g, vreqs, n=16):
+    """Plain per-request dispatch of the same kernels: batch = 1, synchronous."""
+    t = time.time()
+    for r in vreqs[:n]:
+        rc, _ = g.verify_batch([r])
+        assert rc == 0
+    return n / (time.time() - t)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-only", action="store_true", help="a few launches, no baselines")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(independent requests shard by issuer; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import cubbs_hip as g
+    rc, kind, hw = g.backend_info()
+    if rc != 0 or not hw:
+        raise SystemExit("no usable HIP device (backend_info rc=%d)" % rc)
+    t0 = time.time()
+    vreqs, expect_v, pvreqs, kps = build_workload(g)
+    log("workload built in %.1fs" % (time.time() - t0))
+    vb = g.StagedBatch(g.BATCH_VERIFY, vreqs)
+    pb = g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs)
+    log("staged")
+    for _ in range(args.warmup):
+        vb.launch()
+        pb.launch()
+        vb.sync()
+        pb.sync()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        vb.launch()
+        pb.launch()
+        vb.sync()
+        pb.sync()
+    el = time.perf_counter() - t
+    ms_per_step = el * 1e3 / args.steps
+    ops = N_VERIFY + N_PROOFS
+    value = ops * args.steps / el
+    rv = vb.results()
+    rp = pb.results()
+    assert [v for _, v in rv] == expect_v, "verify results differ from expectation"
+    assert all(c == 0 and v for c, v in rp), "proof verify results differ from expectation"
+    log("timed %d steps: %.2f ms/step, %.0f ops/s" % (args.steps, ms_per_step, value))
+    # per-kernel timing with HIP events on each batch's stream
+    vt, vk = vb.profile(2)
+    pt, pk = pb.profile(2)
+    log("verify kernels (ms): %s" % {k: round(v, 3) for k, v in vk.items()})
+    log("proof-verify kernels (ms): %s" % {k: round(v, 3) for k, v in pk.items()})
+    # dominant kernel: pairing (per launch, both batches)
+    pairing_ms = vk.get("pairing", 0.0)
+    checks = N_VERIFY
+    rc, mad_ms = g.mad_peak_bench(1 << 20, 256)
+    peak_mad_s = (1 << 20) * 256 * 64 / (mad_ms * 1e-3)
+    achieved_mad_s = checks * FPM_PER_PAIRING_CHECK * MADS_PER_FPM / (pairing_ms * 1e-3)
+    roofline = {"bound": "valu", "kernel": "k_pairing (Verify batch)",
+                "achieved": round(achieved_mad_s / 1e12, 3), "peak": round(peak_mad_s / 1e12, 3),
+                "unit": "Tmad/s (v_mad_u64_u32)", "frac": round(achieved_mad_s / peak_mad_s, 4),
+                "traffic": None, "launch_ms": round(pairing_ms, 3),
+                "work_per_launch": "%d pairing checks x %d FpM x %d mads" % (checks, FPM_PER_PAIRING_CHECK, MADS_PER_FPM)}
+    out = {"metric": "Verify+ProofVerify ops/s on 1 MI355X; p50/p99 latency @ fixed load",
+           "value": round(value, 1), "unit": "ops/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "u32 (exact Fp/Fr)",
+           "data": "synthetic, seeded; keys/signatures/proofs made by the library",
+           "config": {"workload": "8192 batched uneven Verify (256 issuers, L log-uniform 1..128, "
+                                  "32-256 B messages, 64 B header, ~1% tampered) + 2048 ProofVerify "
+                                  "(L=32 x 64 B, 25% disclosed, 64 B ph)",
+                      "verify_items": N_VERIFY, "proof_verify_items": N_PROOFS,
+                      "msm_terms": vb.shape()["msm_terms"] + pb.shape()["msm_terms"]},
+           "roofline": roofline,
+           "kernel_ms": {"verify": {k: round(v, 3) for k, v in vk.items()},
+                         "proof_verify": {k: round(v, 3) for k, v in pk.items()}}}
+    if not args.profile_only:
+        out["per_request_gpu_ops_s"] = round(per_request_gpu(g, vreqs), 2)
+        if not args.no_cpu_baseline:
+            out["cpu_baseline"] = cpu_baseline(vreqs, pvreqs)
+    vb.free()
+    pb.free()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -7,6 +7,7 @@
 // in the kernels of bbs_kernels.hip.  When no device is usable the entry points
 // return CUBBS_BBS_GPU_ERR_INTERNAL (there is no CPU fallback).
 #include <hip/hip_runtime.h>
+#include <sys/random.h>
 
 #include <algorithm>
 #include <array>
@@ -20,58 +21,39 @@
 
 #include "../../include/cubbs_bbs_gpu_abi.h"
 #include "../../include/cubbs_hip_ext.h"
-#include "bbs_internal.h"
+#include "bbs_batch.h"
+#include "host_ctx.h"
 
 namespace cubbs {
 
-struct GpuError {
-  hipError_t err;
+static bool slice_ok(const CuBbsBbsGpuByteSlice& s) { return s.len == 0 || s.data != nullptr; }
+
+struct Layout {
+  size_t size = 0;
+  size_t add(size_t bytes) {
+    size_t off = size;
+    size = align256(size + bytes);
+    return off;
+  }
 };
-static void check(hipError_t e) {
-  if (e != hipSuccess) throw GpuError{e};
-}
 
-static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-static uint32_t env_u32(const char* name, uint32_t def) {
-  const char* v = std::getenv(name);
-  if (!v || !*v) return def;
-  char* end = nullptr;
-  unsigned long x = std::strtoul(v, &end, 10);
-  return end == v ? def : (uint32_t)x;
+void fill_os_random(uint8_t* out, size_t n) {
+  size_t got = 0;
+  while (got < n) {
+    ssize_t r = getrandom(out + got, n - got, 0);
+    if (r <= 0) throw GpuError{hipErrorUnknown};
+    got += (size_t)r;
+  }
 }
 
 // ------------------------------------------------------------------ device context
-struct Context {
-  std::mutex mu;
-  bool ready = false;
-  bool failed = false;
-  hipStream_t setup = nullptr;
-  // generators: slot 0 = P1, slot 1 = Q1, slot 1 + j = H_j
-  uint32_t gen_slots = 0, gen_cap = 0;
-  uint32_t* gen_affine = nullptr;
-  uint32_t* gen_tables = nullptr;
-  uint8_t* gen_comp = nullptr;
-  uint32_t* gen_vstate = nullptr;
-  // prepared public keys
-  std::unordered_map<std::string, uint32_t> pk_slots;
-  uint32_t pk_count = 0, pk_cap = 0;
-  uint8_t* pk_bytes = nullptr;
-  uint8_t* pk_status = nullptr;
-  uint32_t* pk_lines = nullptr;
-  uint32_t* neg_g2_lines = nullptr;
-  // buffers replaced by a growth step stay allocated: batches that snapshot
-  // them may still be in flight on other streams
-  std::vector<void*> retired;
-};
-
-static Context& ctx() {
+Context& ctx() {
   static Context* c = new Context();
   return *c;
 }
 
 // Must be called with ctx().mu held.
-static void init_locked(Context& c) {
+void init_locked(Context& c) {
   if (c.ready) return;
   if (c.failed) throw GpuError{hipErrorNoDevice};
   int n = 0;
@@ -89,7 +71,7 @@ static void init_locked(Context& c) {
 }
 
 // Ensure generator slots [0, slots) exist (tables included).  ctx().mu held.
-static void ensure_generators_locked(Context& c, uint32_t slots) {
+void ensure_generators_locked(Context& c, uint32_t slots) {
   if (slots <= c.gen_slots) return;
   if (slots > c.gen_cap) {
     uint32_t cap = std::max<uint32_t>({slots, c.gen_cap * 2, 64u});
@@ -146,8 +128,8 @@ static void ensure_generators_locked(Context& c, uint32_t slots) {
 
 // Map public keys to cache slots, preparing unseen ones (decompression,
 // subgroup check, Miller-loop lines) on the device.  ctx().mu held.
-static void lookup_pks_locked(Context& c, const std::vector<const uint8_t*>& pks,
-                              std::vector<uint32_t>& slots_out) {
+void lookup_pks_locked(Context& c, const std::vector<const uint8_t*>& pks,
+                       std::vector<uint32_t>& slots_out) {
   std::vector<uint32_t> fresh;
   std::vector<const uint8_t*> fresh_bytes;
   slots_out.resize(pks.size());
@@ -209,318 +191,6 @@ static void lookup_pks_locked(Context& c, const std::vector<const uint8_t*>& pks
   check(hipFree(d_b));
 }
 
-// ------------------------------------------------------------------ batch
-struct Layout {
-  size_t size = 0;
-  size_t add(size_t bytes) {
-    size_t off = size;
-    size = align256(size + bytes);
-    return off;
-  }
-};
-
-struct KernelTimer {
-  bool on = false;
-  std::vector<std::string> names;
-  std::vector<hipEvent_t> starts, stops;
-  std::vector<double> total_ms;
-  std::vector<int> count;
-  std::vector<size_t> names_of_event;
-};
-
-struct Batch {
-  int kind = 0;
-  hipStream_t stream = nullptr;
-  size_t n_total = 0;                 // items passed in
-  std::vector<int32_t> pre_code;      // per item: -1 = on device, else final code
-  std::vector<uint32_t> dev_index;    // per item: index among device items
-  uint32_t nv = 0;                    // items on the device
-  // staged inputs (same offsets host and device)
-  size_t in_bytes = 0;
-  uint8_t* h_in = nullptr;
-  uint8_t* d_in = nullptr;
-  uint8_t* d_work = nullptr;
-  size_t work_bytes = 0;
-  // counts
-  uint32_t n_msgs = 0, n_terms = 0, n_chunks = 0, n_partials = 0, n_groups = 0;
-  uint64_t comb_madds = 0;
-  // device pointers
-  VerifyItem* items = nullptr;
-  H2sJob* h2s = nullptr;
-  DomainJob* dom = nullptr;
-  CombChunk* chunks = nullptr;
-  ReduceGroup* groups = nullptr;
-  uint32_t* term_gen = nullptr;
-  uint8_t* sig_blob = nullptr;
-  uint8_t* hdr_blob = nullptr;
-  uint8_t* msg_blob = nullptr;
-  uint32_t* term_scalar = nullptr;
-  uint32_t* partials = nullptr;
-  uint32_t* a_aff = nullptr;
-  uint8_t* a_inf = nullptr;
-  uint32_t* b_aff = nullptr;
-  uint8_t* b_inf = nullptr;
-  uint8_t* status = nullptr;
-  uint8_t* valid = nullptr;
-  // snapshot of resident state
-  const uint32_t* tables = nullptr;
-  const uint8_t* gen_comp = nullptr;
-  const uint8_t* pk_bytes = nullptr;
-  const uint8_t* pk_status = nullptr;
-  const uint32_t* pk_lines = nullptr;
-  const uint32_t* neg_g2_lines = nullptr;
-  // results
-  std::vector<uint8_t> h_status, h_valid;
-  KernelTimer timer;
-
-  ~Batch() {
-    if (stream) (void)hipStreamSynchronize(stream);
-    for (size_t i = 0; i < timer.starts.size(); i++) {
-      (void)hipEventDestroy(timer.starts[i]);
-      (void)hipEventDestroy(timer.stops[i]);
-    }
-    if (h_in) (void)hipHostFree(h_in);
-    if (d_in) (void)hipFree(d_in);
-    if (d_work) (void)hipFree(d_work);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
-
-// Bracket a kernel launch with events when profiling.
-struct TimedLaunch {
-  Batch& b;
-  size_t idx;
-  TimedLaunch(Batch& bb, const char* name) : b(bb), idx(0) {
-    if (!b.timer.on) return;
-    KernelTimer& t = b.timer;
-    auto it = std::find(t.names.begin(), t.names.end(), name);
-    if (it == t.names.end()) {
-      t.names.push_back(name);
-      t.total_ms.push_back(0);
-      t.count.push_back(0);
-      idx = t.names.size() - 1;
-    } else {
-      idx = it - t.names.begin();
-    }
-    hipEvent_t e0, e1;
-    check(hipEventCreate(&e0));
-    check(hipEventCreate(&e1));
-    t.starts.push_back(e0);
-    t.stops.push_back(e1);
-    check(hipEventRecord(e0, b.stream));
-  }
-  ~TimedLaunch() {
-    if (!b.timer.on) return;
-    (void)hipEventRecord(b.timer.stops.back(), b.stream);
-    b.timer.count[idx]++;
-    b.timer.names_of_event.push_back(idx);
-  }
-};
-
-static uint32_t comb_chunk_terms() { return std::max<uint32_t>(1, env_u32("CUBBS_HIP_COMB_CHUNK", 4)); }
-
-static bool slice_ok(const CuBbsBbsGpuByteSlice& s) { return s.len == 0 || s.data != nullptr; }
-
-// ---------------------------------------------------------------- Verify planner
-static std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n) {
-  auto b = std::make_unique<Batch>();
-  b->kind = CUBBS_HIP_BATCH_VERIFY;
-  b->n_total = n;
-  b->pre_code.assign(n, -1);
-  b->dev_index.assign(n, 0);
-  std::vector<size_t> dev_items;
-  uint64_t hdr_total = 0, msg_total = 0, msgs = 0;
-  uint32_t maxL = 0;
-  for (size_t i = 0; i < n; i++) {
-    const auto& it = items[i];
-    bool ok = it.public_key.len == 96 && it.public_key.data && it.signature.len == 80 &&
-              it.signature.data && slice_ok(it.header) &&
-              (it.message_count == 0 || it.messages != nullptr);
-    if (ok) {
-      for (size_t m = 0; m < it.message_count && ok; m++) ok = slice_ok(it.messages[m]);
-    }
-    if (!ok || it.message_count > (1u << 20)) {
-      b->pre_code[i] = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
-      continue;
-    }
-    b->dev_index[i] = (uint32_t)dev_items.size();
-    dev_items.push_back(i);
-    hdr_total += it.header.len;
-    msgs += it.message_count;
-    for (size_t m = 0; m < it.message_count; m++) msg_total += it.messages[m].len;
-    maxL = std::max<uint32_t>(maxL, (uint32_t)it.message_count);
-  }
-  const uint32_t nv = (uint32_t)dev_items.size();
-  b->nv = nv;
-  if (nv == 0) return b;
-  Context& c = ctx();
-  std::vector<uint32_t> pk_slot;
-  {
-    std::lock_guard<std::mutex> g(c.mu);
-    init_locked(c);
-    ensure_generators_locked(c, maxL + 2);
-    std::vector<const uint8_t*> pks(nv);
-    for (uint32_t k = 0; k < nv; k++) pks[k] = items[dev_items[k]].public_key.data;
-    lookup_pks_locked(c, pks, pk_slot);
-    b->tables = c.gen_tables;
-    b->gen_comp = c.gen_comp;
-    b->pk_bytes = c.pk_bytes;
-    b->pk_status = c.pk_status;
-    b->pk_lines = c.pk_lines;
-    b->neg_g2_lines = c.neg_g2_lines;
-  }
-  const uint32_t CH = comb_chunk_terms();
-  uint64_t terms = 0, chunks = 0;
-  for (uint32_t k = 0; k < nv; k++) {
-    uint64_t t = items[dev_items[k]].message_count + 2;
-    terms += t;
-    chunks += (t + CH - 1) / CH;
-  }
-  b->n_msgs = (uint32_t)msgs;
-  b->n_terms = (uint32_t)terms;
-  b->n_chunks = (uint32_t)chunks;
-  b->n_partials = (uint32_t)(chunks + nv);
-  b->n_groups = nv;
-  // staged input layout
-  Layout in;
-  size_t o_items = in.add(sizeof(VerifyItem) * nv);
-  size_t o_h2s = in.add(sizeof(H2sJob) * std::max<uint64_t>(msgs, 1));
-  size_t o_dom = in.add(sizeof(DomainJob) * nv);
-  size_t o_chunks = in.add(sizeof(CombChunk) * chunks);
-  size_t o_groups = in.add(sizeof(ReduceGroup) * nv);
-  size_t o_gen = in.add(4 * terms);
-  size_t o_sig = in.add(80 * (size_t)nv);
-  size_t o_hdr = in.add(std::max<uint64_t>(hdr_total, 1));
-  size_t o_msg = in.add(std::max<uint64_t>(msg_total, 1));
-  b->in_bytes = in.size;
-  check(hipHostMalloc(&b->h_in, in.size, hipHostMallocDefault));
-  uint8_t* H = b->h_in;
-  auto* vi = reinterpret_cast<VerifyItem*>(H + o_items);
-  auto* hj = reinterpret_cast<H2sJob*>(H + o_h2s);
-  auto* dj = reinterpret_cast<DomainJob*>(H + o_dom);
-  auto* cj = reinterpret_cast<CombChunk*>(H + o_chunks);
-  auto* gj = reinterpret_cast<ReduceGroup*>(H + o_groups);
-  auto* tg = reinterpret_cast<uint32_t*>(H + o_gen);
-  uint64_t term = 0, chunk = 0, partial = 0, hoff = 0, moff = 0, msg = 0;
-  for (uint32_t k = 0; k < nv; k++) {
-    const auto& it = items[dev_items[k]];
-    const uint32_t L = (uint32_t)it.message_count;
-    std::memcpy(H + o_sig + 80 * (size_t)k, it.signature.data, 80);
-    if (it.header.len) std::memcpy(H + o_hdr + hoff, it.header.data, it.header.len);
-    dj[k] = DomainJob{hoff, (uint32_t)it.header.len, pk_slot[k], L, (uint32_t)(term + 1)};
-    hoff += it.header.len;
-    for (uint32_t m = 0; m < L; m++) {
-      const auto& ms = it.messages[m];
-      if (ms.len) std::memcpy(H + o_msg + moff, ms.data, ms.len);
-      hj[msg++] = H2sJob{moff, (uint32_t)ms.len, (uint32_t)(term + 2 + m)};
-      moff += ms.len;
-    }
-    for (uint32_t t = 0; t < L + 2; t++) tg[term + t] = t;  // P1, Q1, H_1..H_L
-    vi[k] = VerifyItem{80ull * k, pk_slot[k], (uint32_t)term, (uint32_t)partial, 1u};
-    uint32_t first_partial = (uint32_t)partial;
-    partial++;  // variable partial (-e) A
-    for (uint32_t t = 0; t < L + 2; t += CH) {
-      uint32_t cnt = std::min(CH, L + 2 - t);
-      cj[chunk++] = CombChunk{(uint32_t)(term + t), cnt, (uint32_t)partial++};
-    }
-    gj[k] = ReduceGroup{first_partial, (uint32_t)(partial - first_partial)};
-    term += L + 2;
-  }
-  b->comb_madds = 32ull * terms;  // upper bound: one mixed addition per non-zero digit
-  // work layout
-  Layout w;
-  size_t o_ts = w.add(4ull * FR_WORDS * terms);
-  size_t o_part = w.add(4ull * G1J_WORDS * b->n_partials);
-  size_t o_aa = w.add(4ull * G1A_WORDS * nv);
-  size_t o_ai = w.add(nv);
-  size_t o_ba = w.add(4ull * G1A_WORDS * nv);
-  size_t o_bi = w.add(nv);
-  size_t o_st = w.add(nv);
-  size_t o_va = w.add(nv);
-  b->work_bytes = w.size;
-  check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  check(hipMalloc(&b->d_in, in.size));
-  check(hipMalloc(&b->d_work, w.size));
-  uint8_t* D = b->d_in;
-  b->items = reinterpret_cast<VerifyItem*>(D + o_items);
-  b->h2s = reinterpret_cast<H2sJob*>(D + o_h2s);
-  b->dom = reinterpret_cast<DomainJob*>(D + o_dom);
-  b->chunks = reinterpret_cast<CombChunk*>(D + o_chunks);
-  b->groups = reinterpret_cast<ReduceGroup*>(D + o_groups);
-  b->term_gen = reinterpret_cast<uint32_t*>(D + o_gen);
-  b->sig_blob = D + o_sig;
-  b->hdr_blob = D + o_hdr;
-  b->msg_blob = D + o_msg;
-  uint8_t* W = b->d_work;
-  b->term_scalar = reinterpret_cast<uint32_t*>(W + o_ts);
-  b->partials = reinterpret_cast<uint32_t*>(W + o_part);
-  b->a_aff = reinterpret_cast<uint32_t*>(W + o_aa);
-  b->a_inf = W + o_ai;
-  b->b_aff = reinterpret_cast<uint32_t*>(W + o_ba);
-  b->b_inf = W + o_bi;
-  b->status = W + o_st;
-  b->valid = W + o_va;
-  check(hipMemcpyAsync(b->d_in, b->h_in, in.size, hipMemcpyHostToDevice, b->stream));
-  return b;
-}
-
-static void launch_verify(Batch& b) {
-  if (b.nv == 0) return;
-  {
-    TimedLaunch t(b, "h2s_messages");
-    launch_h2s(b.stream, b.h2s, b.n_msgs, b.msg_blob, 0, b.term_scalar);
-  }
-  {
-    TimedLaunch t(b, "domain");
-    launch_domain(b.stream, b.dom, b.nv, b.hdr_blob, b.pk_bytes, b.gen_comp, b.term_scalar);
-  }
-  {
-    TimedLaunch t(b, "verify_sig_glv");
-    launch_verify_sig(b.stream, b.items, b.nv, b.sig_blob, b.term_scalar, b.a_aff, b.a_inf,
-                      b.status, b.partials);
-  }
-  {
-    TimedLaunch t(b, "comb_msm");
-    launch_comb_msm(b.stream, b.chunks, b.n_chunks, b.term_gen, b.term_scalar, b.tables,
-                    b.partials);
-  }
-  {
-    TimedLaunch t(b, "reduce_affine");
-    launch_reduce_affine(b.stream, b.groups, b.n_groups, b.partials, b.b_aff, b.b_inf);
-  }
-  {
-    TimedLaunch t(b, "verify_pairing");
-    launch_verify_pairing(b.stream, b.items, b.nv, b.a_aff, b.a_inf, b.b_aff, b.b_inf,
-                          b.pk_status, b.pk_lines, b.neg_g2_lines, b.status, b.valid);
-  }
-  check(hipGetLastError());
-}
-
-static void collect_verify(Batch& b, CuBbsBbsGpuResult* results) {
-  b.h_status.resize(b.nv);
-  b.h_valid.resize(b.nv);
-  if (b.nv) {
-    check(hipMemcpyAsync(b.h_status.data(), b.status, b.nv, hipMemcpyDeviceToHost, b.stream));
-    check(hipMemcpyAsync(b.h_valid.data(), b.valid, b.nv, hipMemcpyDeviceToHost, b.stream));
-    check(hipStreamSynchronize(b.stream));
-  }
-  for (size_t i = 0; i < b.n_total; i++) {
-    results[i].bytes_written = 0;
-    if (b.pre_code[i] >= 0) {
-      results[i].code = b.pre_code[i];
-      results[i].value = 0;
-      continue;
-    }
-    uint32_t k = b.dev_index[i];
-    results[i].code = b.h_status[k] == 0 ? CUBBS_BBS_GPU_SUCCESS : CUBBS_BBS_GPU_ERR_INVALID_INPUT;
-    results[i].value = b.h_status[k] == 0 ? b.h_valid[k] : 0;
-  }
-}
-
-static void launch_batch(Batch& b) {
-  if (b.kind == CUBBS_HIP_BATCH_VERIFY) launch_verify(b);
-}
-
 }  // namespace cubbs
 
 using namespace cubbs;
@@ -550,12 +220,91 @@ int32_t cubbs_bbs_gpu_verify_batch_v2(uint32_t abi_version, const CuBbsBbsGpuVer
   if (!items || !results) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   try {
     auto b = plan_verify(items, item_count);
-    launch_verify(*b);
-    collect_verify(*b, results);
+    launch_batch(*b);
+    collect_batch(*b, results, nullptr, nullptr);
     return CUBBS_BBS_GPU_SUCCESS;
   } catch (const GpuError&) {
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
+}
+
+int32_t cubbs_bbs_gpu_proof_verify_batch_v2(uint32_t abi_version,
+                                            const CuBbsBbsGpuProofVerifyItem* items,
+                                            size_t item_count, CuBbsBbsGpuResult* results) {
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  if (item_count == 0) return CUBBS_BBS_GPU_SUCCESS;
+  if (!items || !results) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  try {
+    auto b = plan_proof_verify(items, item_count);
+    launch_batch(*b);
+    collect_batch(*b, results, nullptr, nullptr);
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (const GpuError&) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
+
+int32_t cubbs_bbs_gpu_sign_batch_v2(uint32_t abi_version, CuBbsBbsGpuSignItem* items,
+                                    size_t item_count, CuBbsBbsGpuResult* results) {
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  if (item_count == 0) return CUBBS_BBS_GPU_SUCCESS;
+  if (!items || !results) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  try {
+    auto b = plan_sign(items, item_count);
+    launch_batch(*b);
+    std::vector<uint8_t*> outs(item_count);
+    std::vector<size_t> lens(item_count, 0);
+    for (size_t i = 0; i < item_count; i++) outs[i] = items[i].signature_out.data;
+    collect_batch(*b, results, outs.data(), lens.data());
+    for (size_t i = 0; i < item_count; i++)
+      if (results[i].code == CUBBS_BBS_GPU_SUCCESS) items[i].signature_out.len = lens[i];
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (const GpuError&) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
+
+static int32_t proof_gen_common(uint32_t abi_version, CuBbsBbsGpuProofGenItem* items,
+                                size_t item_count, int mode, const uint8_t* seeds,
+                                const uint8_t* scalars, const size_t* offsets,
+                                CuBbsBbsGpuResult* results) {
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  if (item_count == 0) return CUBBS_BBS_GPU_SUCCESS;
+  if (!items || !results) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  try {
+    auto b = plan_proof_gen(items, item_count, mode, seeds, scalars, offsets);
+    launch_batch(*b);
+    std::vector<uint8_t*> outs(item_count);
+    std::vector<size_t> lens(item_count, 0);
+    for (size_t i = 0; i < item_count; i++) outs[i] = items[i].proof_out.data;
+    collect_batch(*b, results, outs.data(), lens.data());
+    for (size_t i = 0; i < item_count; i++)
+      if (results[i].code == CUBBS_BBS_GPU_SUCCESS) items[i].proof_out.len = lens[i];
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (const GpuError&) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
+
+int32_t cubbs_bbs_gpu_proof_gen_batch_v2(uint32_t abi_version, CuBbsBbsGpuProofGenItem* items,
+                                         size_t item_count, CuBbsBbsGpuResult* results) {
+  return proof_gen_common(abi_version, items, item_count, 0, nullptr, nullptr, nullptr, results);
+}
+
+int32_t cubbs_hip_proof_gen_seeded_batch(uint32_t abi_version, CuBbsBbsGpuProofGenItem* items,
+                                         size_t item_count, const uint8_t* seeds32,
+                                         CuBbsBbsGpuResult* results) {
+  if (item_count && !seeds32) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  return proof_gen_common(abi_version, items, item_count, 1, seeds32, nullptr, nullptr, results);
+}
+
+int32_t cubbs_hip_proof_gen_with_scalars_batch(uint32_t abi_version,
+                                               CuBbsBbsGpuProofGenItem* items, size_t item_count,
+                                               const uint8_t* scalars_be, const size_t* offsets,
+                                               CuBbsBbsGpuResult* results) {
+  if (item_count && (!scalars_be || !offsets)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  return proof_gen_common(abi_version, items, item_count, 2, nullptr, scalars_be, offsets,
+                          results);
 }
 
 void* cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void* items, size_t item_count,
@@ -570,11 +319,13 @@ void* cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void* items,
     std::unique_ptr<Batch> b;
     if (kind == CUBBS_HIP_BATCH_VERIFY) {
       b = plan_verify(static_cast<const CuBbsBbsGpuVerifyItem*>(items), item_count);
+    } else if (kind == CUBBS_HIP_BATCH_PROOF_VERIFY) {
+      b = plan_proof_verify(static_cast<const CuBbsBbsGpuProofVerifyItem*>(items), item_count);
     } else {
       st = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
       return nullptr;
     }
-    check(hipStreamSynchronize(b->stream ? b->stream : 0));
+    if (b->stream) check(hipStreamSynchronize(b->stream));
     st = CUBBS_BBS_GPU_SUCCESS;
     return b.release();
   } catch (const GpuError&) {
@@ -604,8 +355,7 @@ int32_t cubbs_hip_batch_sync(void* batch) {
 int32_t cubbs_hip_batch_results(void* batch, CuBbsBbsGpuResult* results) {
   if (!batch || !results) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   try {
-    Batch* b = static_cast<Batch*>(batch);
-    if (b->kind == CUBBS_HIP_BATCH_VERIFY) collect_verify(*b, results);
+    collect_batch(*static_cast<Batch*>(batch), results, nullptr, nullptr);
     return CUBBS_BBS_GPU_SUCCESS;
   } catch (const GpuError&) {
     return CUBBS_BBS_GPU_ERR_INTERNAL;
@@ -619,9 +369,9 @@ int32_t cubbs_hip_batch_shape(void* batch, uint64_t* items, uint64_t* msm_terms,
   if (!batch) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   Batch* b = static_cast<Batch*>(batch);
   if (items) *items = b->nv;
-  if (msm_terms) *msm_terms = b->n_terms;
-  if (comb_madds) *comb_madds = b->comb_madds;
-  if (pairings) *pairings = 2ull * b->nv;
+  if (msm_terms) *msm_terms = b->comb_terms;
+  if (comb_madds) *comb_madds = 32ull * b->comb_terms;
+  if (pairings) *pairings = 2ull * b->n_pairs;
   return CUBBS_BBS_GPU_SUCCESS;
 }
 
@@ -631,6 +381,10 @@ int32_t cubbs_hip_batch_profile(void* batch, int iters, char* names_out, size_t 
   Batch* b = static_cast<Batch*>(batch);
   try {
     KernelTimer& t = b->timer;
+    for (size_t i = 0; i < t.starts.size(); i++) {
+      (void)hipEventDestroy(t.starts[i]);
+      (void)hipEventDestroy(t.stops[i]);
+    }
     t = KernelTimer();
     t.on = true;
     hipEvent_t e0, e1;
@@ -649,7 +403,7 @@ int32_t cubbs_hip_batch_profile(void* batch, int iters, char* names_out, size_t 
     for (size_t k = 0; k < t.starts.size(); k++) {
       float ms = 0;
       check(hipEventElapsedTime(&ms, t.starts[k], t.stops[k]));
-      sum[t.names_of_event[k]] += ms;
+      sum[t.event_name[k]] += ms;
     }
     int nk = (int)std::min<size_t>(t.names.size(), (size_t)max_kernels);
     std::string joined;
@@ -870,7 +624,8 @@ int32_t cubbs_hip_generators(size_t count, uint8_t* out48) {
   }
 }
 
-int32_t cubbs_hip_fp_mul_bench(uint32_t lanes, uint32_t iters, double* ms) {
+static int32_t time_probe(uint32_t lanes, uint32_t iters, double* ms,
+                          void (*launch)(hipStream_t, uint32_t*, uint32_t, uint32_t)) {
   if (!lanes || !ms) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   try {
     {
@@ -882,16 +637,16 @@ int32_t cubbs_hip_fp_mul_bench(uint32_t lanes, uint32_t iters, double* ms) {
     check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     uint32_t* d = nullptr;
     check(hipMalloc(&d, (size_t)lanes * 24 * 4));
-    // any values below p work: small non-zero limbs
+    // operands below p: small non-zero limbs
     std::vector<uint32_t> h((size_t)lanes * 24);
     for (size_t i = 0; i < h.size(); i++) h[i] = (uint32_t)(i * 2654435761u) & 0x0fffffffu;
     check(hipMemcpyAsync(d, h.data(), h.size() * 4, hipMemcpyHostToDevice, s));
-    launch_fp_mul_bench(s, d, lanes, 1);  // warm-up
+    launch(s, d, lanes, 1);  // warm-up
     hipEvent_t e0, e1;
     check(hipEventCreate(&e0));
     check(hipEventCreate(&e1));
     check(hipEventRecord(e0, s));
-    launch_fp_mul_bench(s, d, lanes, iters);
+    launch(s, d, lanes, iters);
     check(hipEventRecord(e1, s));
     check(hipStreamSynchronize(s));
     float t = 0;
@@ -907,34 +662,21 @@ int32_t cubbs_hip_fp_mul_bench(uint32_t lanes, uint32_t iters, double* ms) {
   }
 }
 
-// ---------------------------------------------------------------- not yet batched on the device
-// Mandatory symbols of the loader (gpu_runtime.rs:303-327).  Until their
-// device pipelines land they mark every item INTERNAL, which the Rust engine
-// treats as "use another route".
-static int32_t mark_internal(uint32_t abi_version, const void* items, size_t n,
-                             CuBbsBbsGpuResult* results) {
-  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+int32_t cubbs_hip_fp_mul_bench(uint32_t lanes, uint32_t iters, double* ms) {
+  return time_probe(lanes, iters, ms, launch_fp_mul_bench);
+}
+
+int32_t cubbs_hip_mad_peak_bench(uint32_t lanes, uint32_t iters, double* ms) {
+  return time_probe(lanes, iters, ms, launch_mad_peak);
+}
+
+int32_t cubbs_hip_sk_to_pk(const uint8_t* sk32, size_t n, uint8_t* pk96, uint8_t* ok) {
   if (n == 0) return CUBBS_BBS_GPU_SUCCESS;
-  if (!items || !results) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
-  for (size_t i = 0; i < n; i++) {
-    results[i].code = CUBBS_BBS_GPU_ERR_INTERNAL;
-    results[i].value = 0;
-    results[i].bytes_written = 0;
-  }
-  return CUBBS_BBS_GPU_SUCCESS;
-}
-int32_t cubbs_bbs_gpu_sign_batch_v2(uint32_t abi_version, CuBbsBbsGpuSignItem* items,
-                                    size_t item_count, CuBbsBbsGpuResult* results) {
-  return mark_internal(abi_version, items, item_count, results);
-}
-int32_t cubbs_bbs_gpu_proof_gen_batch_v2(uint32_t abi_version, CuBbsBbsGpuProofGenItem* items,
-                                         size_t item_count, CuBbsBbsGpuResult* results) {
-  return mark_internal(abi_version, items, item_count, results);
-}
-int32_t cubbs_bbs_gpu_proof_verify_batch_v2(uint32_t abi_version,
-                                            const CuBbsBbsGpuProofVerifyItem* items,
-                                            size_t item_count, CuBbsBbsGpuResult* results) {
-  return mark_internal(abi_version, items, item_count, results);
+  if (!sk32 || !pk96 || !ok) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  return run_probe(32 * n, sk32, 96 * n, pk96, n, ok,
+                   [&](hipStream_t s, uint8_t* din, uint8_t* dout, uint8_t* dok) {
+                     launch_g2_mul_gen(s, din, (uint32_t)n, dout, dok);
+                   });
 }
 
 }  // extern "C"

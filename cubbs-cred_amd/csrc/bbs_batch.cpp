@@ -1,0 +1,822 @@
+// Batch planner: turns Sign / Verify / ProofGen / ProofVerify items into the
+// flat job arrays of the device pipeline (bbs_pipeline.hip), stages them in
+// one pinned upload, enqueues the stages on the batch's own stream and reads
+// back per-item results.  A batch can be launched repeatedly with its inputs
+// resident on the device (used by the benchmark and the serving loop).
+//
+// Semantics per operation follow the reference engine entry points
+// (crates/cubbs-bbs/src/protocol.rs): sign :28-40, verify :318-326,
+// proof_gen :423-437, proof_verify :772-793.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/cubbs_bbs_gpu_abi.h"
+#include "../../include/cubbs_hip_ext.h"
+#include "bbs_batch.h"
+#include "host_ctx.h"
+
+namespace cubbs {
+
+static uint32_t env_u32(const char* name, uint32_t def) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return def;
+  char* end = nullptr;
+  unsigned long x = std::strtoul(v, &end, 10);
+  return end == v ? def : (uint32_t)x;
+}
+static bool slice_ok(const CuBbsBbsGpuByteSlice& s) { return s.len == 0 || s.data != nullptr; }
+
+static const char* kApiId = "BBS_BLS12381G1_XMD:SHA-256_SSWU_RO_H2G_HM2S_";
+
+uint32_t comb_chunk_terms() { return std::max<uint32_t>(1, env_u32("CUBBS_HIP_COMB_CHUNK", 4)); }
+
+// ------------------------------------------------------------------ planning helpers
+template <class T>
+static size_t put(std::vector<uint8_t>& blob, const T* p, size_t n) {
+  size_t off = blob.size();
+  blob.resize(off + n * sizeof(T));
+  if (n) std::memcpy(blob.data() + off, p, n * sizeof(T));
+  return off;
+}
+
+struct Plan {
+  // host-side job lists
+  std::vector<H2sJob> h2s;
+  std::vector<DomainJob> dom;
+  std::vector<DecodeG1Job> dec_g1;
+  std::vector<DecodeFrJob> dec_fr;
+  std::vector<TermFill> fills;
+  std::vector<VarJob> var;
+  std::vector<CombChunk> chunks;
+  std::vector<uint32_t> term_gen;
+  std::vector<ReduceGroup> groups1, groups2;
+  std::vector<uint32_t> out1, out2;
+  std::vector<PairJob> pairs;
+  std::vector<SignJob> sign;
+  std::vector<PvChallengeJob> pvch;
+  std::vector<PgJob> pg;
+  std::vector<uint32_t> disc_index, disc_scalar, und_scalar;
+  std::vector<uint8_t> blob;       // signatures, proofs, headers, messages, keys, seeds
+  std::vector<uint8_t> map_dst;    // DST of message scalars
+  uint32_t n_scalars = 0, n_points = 0, n_partials = 0;
+  size_t out_bytes = 0;            // signature / proof output area
+  bool seeded = false;
+  size_t seeds_off = 0;
+
+  uint32_t scalars(uint32_t k) { uint32_t s = n_scalars; n_scalars += k; return s; }
+  uint32_t points(uint32_t k) { uint32_t s = n_points; n_points += k; return s; }
+  uint32_t partials(uint32_t k) { uint32_t s = n_partials; n_partials += k; return s; }
+  size_t bytes(const void* p, size_t n) {
+    size_t off = blob.size();
+    blob.resize(off + n);
+    if (n) std::memcpy(blob.data() + off, p, n);
+    return off;
+  }
+  // Fixed-base terms [term, term + count) in chunks, partials appended to the
+  // caller's running partial range.
+  void comb(uint32_t term, uint32_t count, uint32_t chunk) {
+    for (uint32_t t = 0; t < count; t += chunk) {
+      uint32_t c = std::min(chunk, count - t);
+      chunks.push_back(CombChunk{term + t, c, partials(1)});
+    }
+  }
+};
+
+// Copy the plan into one pinned buffer + device mirror; allocate the work area.
+static void materialize(Batch& b, Plan& p) {
+  std::vector<uint8_t> H;
+  H.reserve(p.blob.size() + 4096);
+  auto place = [&](const void* src, size_t bytes) {
+    size_t off = align256(H.size());
+    H.resize(off + bytes);
+    if (bytes) std::memcpy(H.data() + off, src, bytes);
+    return off;
+  };
+  size_t o_h2s = place(p.h2s.data(), p.h2s.size() * sizeof(H2sJob));
+  size_t o_dom = place(p.dom.data(), p.dom.size() * sizeof(DomainJob));
+  size_t o_dg1 = place(p.dec_g1.data(), p.dec_g1.size() * sizeof(DecodeG1Job));
+  size_t o_dfr = place(p.dec_fr.data(), p.dec_fr.size() * sizeof(DecodeFrJob));
+  size_t o_fill = place(p.fills.data(), p.fills.size() * sizeof(TermFill));
+  size_t o_var = place(p.var.data(), p.var.size() * sizeof(VarJob));
+  size_t o_chk = place(p.chunks.data(), p.chunks.size() * sizeof(CombChunk));
+  size_t o_tg = place(p.term_gen.data(), p.term_gen.size() * 4);
+  size_t o_g1 = place(p.groups1.data(), p.groups1.size() * sizeof(ReduceGroup));
+  size_t o_o1 = place(p.out1.data(), p.out1.size() * 4);
+  size_t o_g2 = place(p.groups2.data(), p.groups2.size() * sizeof(ReduceGroup));
+  size_t o_o2 = place(p.out2.data(), p.out2.size() * 4);
+  size_t o_pair = place(p.pairs.data(), p.pairs.size() * sizeof(PairJob));
+  size_t o_sign = place(p.sign.data(), p.sign.size() * sizeof(SignJob));
+  size_t o_pvch = place(p.pvch.data(), p.pvch.size() * sizeof(PvChallengeJob));
+  size_t o_pg = place(p.pg.data(), p.pg.size() * sizeof(PgJob));
+  size_t o_di = place(p.disc_index.data(), p.disc_index.size() * 4);
+  size_t o_ds = place(p.disc_scalar.data(), p.disc_scalar.size() * 4);
+  size_t o_us = place(p.und_scalar.data(), p.und_scalar.size() * 4);
+  size_t o_dst = place(p.map_dst.data(), p.map_dst.size());
+  size_t o_blob = place(p.blob.data(), std::max<size_t>(p.blob.size(), 1));
+  b.in_bytes = align256(H.size());
+  check(hipHostMalloc(&b.h_in, b.in_bytes, hipHostMallocDefault));
+  std::memcpy(b.h_in, H.data(), H.size());
+  check(hipMalloc(&b.d_in, b.in_bytes));
+  uint8_t* D = b.d_in;
+  b.h2s = reinterpret_cast<H2sJob*>(D + o_h2s);
+  b.n_h2s = (uint32_t)p.h2s.size();
+  b.dom = reinterpret_cast<DomainJob*>(D + o_dom);
+  b.n_dom = (uint32_t)p.dom.size();
+  b.dec_g1 = reinterpret_cast<DecodeG1Job*>(D + o_dg1);
+  b.n_dec_g1 = (uint32_t)p.dec_g1.size();
+  b.dec_fr = reinterpret_cast<DecodeFrJob*>(D + o_dfr);
+  b.n_dec_fr = (uint32_t)p.dec_fr.size();
+  b.fills = reinterpret_cast<TermFill*>(D + o_fill);
+  b.n_fills = (uint32_t)p.fills.size();
+  b.var = reinterpret_cast<VarJob*>(D + o_var);
+  b.n_var = (uint32_t)p.var.size();
+  b.chunks = reinterpret_cast<CombChunk*>(D + o_chk);
+  b.n_chunks = (uint32_t)p.chunks.size();
+  b.term_gen = reinterpret_cast<uint32_t*>(D + o_tg);
+  b.n_terms = (uint32_t)p.term_gen.size();
+  b.groups1 = reinterpret_cast<ReduceGroup*>(D + o_g1);
+  b.out1 = reinterpret_cast<uint32_t*>(D + o_o1);
+  b.n_groups1 = (uint32_t)p.groups1.size();
+  b.groups2 = reinterpret_cast<ReduceGroup*>(D + o_g2);
+  b.out2 = reinterpret_cast<uint32_t*>(D + o_o2);
+  b.n_groups2 = (uint32_t)p.groups2.size();
+  b.pairs = reinterpret_cast<PairJob*>(D + o_pair);
+  b.n_pairs = (uint32_t)p.pairs.size();
+  b.sign = reinterpret_cast<SignJob*>(D + o_sign);
+  b.n_sign = (uint32_t)p.sign.size();
+  b.pvch = reinterpret_cast<PvChallengeJob*>(D + o_pvch);
+  b.n_pvch = (uint32_t)p.pvch.size();
+  b.pg = reinterpret_cast<PgJob*>(D + o_pg);
+  b.n_pg = (uint32_t)p.pg.size();
+  b.disc_index = reinterpret_cast<uint32_t*>(D + o_di);
+  b.disc_scalar = reinterpret_cast<uint32_t*>(D + o_ds);
+  b.und_scalar = reinterpret_cast<uint32_t*>(D + o_us);
+  b.map_dst = D + o_dst;
+  b.map_dst_len = (uint32_t)p.map_dst.size();
+  b.blob = D + o_blob;
+  b.seeded = p.seeded;
+  b.seeds = D + o_blob + p.seeds_off;
+  for (const auto& c : p.chunks) b.comb_terms += c.term_count;
+  // work area
+  size_t w = 0;
+  auto take = [&](size_t bytes) {
+    size_t off = w;
+    w = align256(w + bytes);
+    return off;
+  };
+  size_t o_sc = take(4ull * FR_WORDS * std::max<uint32_t>(p.n_scalars, 1));
+  size_t o_pt = take(4ull * G1A_WORDS * std::max<uint32_t>(p.n_points, 1));
+  size_t o_pi = take(std::max<uint32_t>(p.n_points, 1));
+  size_t o_pa = take(4ull * G1J_WORDS * std::max<uint32_t>(p.n_partials, 1));
+  size_t o_st = take(std::max<uint32_t>(b.nv, 1));
+  size_t o_ga = take(std::max<uint32_t>(b.nv, 1));
+  size_t o_va = take(std::max<uint32_t>(b.nv, 1));
+  size_t o_out = take(std::max<size_t>(p.out_bytes, 1));
+  b.work_bytes = w;
+  check(hipMalloc(&b.d_work, w));
+  uint8_t* W = b.d_work;
+  b.scalars = reinterpret_cast<uint32_t*>(W + o_sc);
+  b.points = reinterpret_cast<uint32_t*>(W + o_pt);
+  b.pinf = W + o_pi;
+  b.partials = reinterpret_cast<uint32_t*>(W + o_pa);
+  b.status = W + o_st;
+  b.gate = W + o_ga;
+  b.valid = W + o_va;
+  b.out = W + o_out;
+  b.out_bytes = p.out_bytes;
+  b.n_scalars = p.n_scalars;
+  b.n_points = p.n_points;
+  b.n_partials = p.n_partials;
+  check(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+  check(hipMemcpyAsync(b.d_in, b.h_in, b.in_bytes, hipMemcpyHostToDevice, b.stream));
+}
+
+// Lock the context, make generators/public keys resident, snapshot pointers.
+static void attach(Batch& b, uint32_t max_gen_slots, const std::vector<const uint8_t*>& pks,
+                   std::vector<uint32_t>& pk_slot) {
+  Context& c = ctx();
+  std::lock_guard<std::mutex> g(c.mu);
+  init_locked(c);
+  ensure_generators_locked(c, std::max<uint32_t>(max_gen_slots, 2));
+  lookup_pks_locked(c, pks, pk_slot);
+  b.tables = c.gen_tables;
+  b.gen_comp = c.gen_comp;
+  b.pk_bytes = c.pk_bytes;
+  b.pk_status = c.pk_status;
+  b.pk_lines = c.pk_lines;
+  b.neg_g2_lines = c.neg_g2_lines;
+}
+
+static std::string map_dst() { return std::string(kApiId) + "MAP_MSG_TO_SCALAR_AS_HASH_"; }
+
+// ------------------------------------------------------------------ Verify
+std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n) {
+  auto b = std::make_unique<Batch>();
+  b->kind = CUBBS_HIP_BATCH_VERIFY;
+  b->n_total = n;
+  b->pre_code.assign(n, -1);
+  b->pre_value.assign(n, 0);
+  b->dev_index.assign(n, 0);
+  std::vector<size_t> dev;
+  uint32_t maxL = 0;
+  for (size_t i = 0; i < n; i++) {
+    const auto& it = items[i];
+    bool ok = it.public_key.len == 96 && it.public_key.data && it.signature.len == 80 &&
+              it.signature.data && slice_ok(it.header) &&
+              (it.message_count == 0 || it.messages != nullptr) && it.message_count < (1u << 24);
+    for (size_t m = 0; ok && m < it.message_count; m++) ok = slice_ok(it.messages[m]);
+    if (!ok) {
+      b->pre_code[i] = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      continue;
+    }
+    b->dev_index[i] = (uint32_t)dev.size();
+    dev.push_back(i);
+    maxL = std::max<uint32_t>(maxL, (uint32_t)it.message_count);
+  }
+  b->nv = (uint32_t)dev.size();
+  if (!b->nv) return b;
+  std::vector<const uint8_t*> pks(b->nv);
+  for (uint32_t k = 0; k < b->nv; k++) pks[k] = items[dev[k]].public_key.data;
+  std::vector<uint32_t> pk_slot;
+  attach(*b, maxL + 2, pks, pk_slot);
+  Plan p;
+  std::string d = map_dst();
+  p.map_dst.assign(d.begin(), d.end());
+  const uint32_t CH = comb_chunk_terms();
+  // terms first: [0, T)
+  uint32_t T = 0;
+  for (uint32_t k = 0; k < b->nv; k++) T += (uint32_t)items[dev[k]].message_count + 2;
+  p.scalars(T);
+  p.term_gen.resize(T);
+  uint32_t term = 0;
+  for (uint32_t k = 0; k < b->nv; k++) {
+    const auto& it = items[dev[k]];
+    const uint32_t L = (uint32_t)it.message_count;
+    size_t sig = p.bytes(it.signature.data, 80);
+    size_t hdr = p.bytes(it.header.data, it.header.len);
+    p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, term + 1});
+    for (uint32_t m = 0; m < L; m++) {
+      size_t mo = p.bytes(it.messages[m].data, it.messages[m].len);
+      p.h2s.push_back(H2sJob{mo, (uint32_t)it.messages[m].len, term + 2 + m});
+    }
+    for (uint32_t t = 0; t < L + 2; t++) p.term_gen[term + t] = t;
+    p.fills.push_back(TermFill{term, 0, 0, 2});  // P1 scalar = 1
+    uint32_t pa = p.points(1), pb = p.points(1), se = p.scalars(1);
+    p.dec_g1.push_back(DecodeG1Job{sig, pa, k});
+    p.dec_fr.push_back(DecodeFrJob{sig + 48, se, k, 1u, 0u});  // store r - e
+    uint32_t first = p.n_partials;
+    p.var.push_back(VarJob{pa, se, p.partials(1)});         // (-e) A
+    p.comb(term, L + 2, CH);                                  // B
+    p.groups1.push_back(ReduceGroup{first, p.n_partials - first});
+    p.out1.push_back(pb);                                     // B - e A
+    p.pairs.push_back(PairJob{k, pa, pb, pk_slot[k]});
+    term += L + 2;
+  }
+  materialize(*b, p);
+  return b;
+}
+
+// ------------------------------------------------------------------ ProofVerify
+static bool indexes_strictly_increasing(const CuBbsBbsGpuIndexSlice& s, size_t bound) {
+  if (s.len && !s.data) return false;
+  for (size_t i = 0; i < s.len; i++) {
+    if (s.data[i] >= bound) return false;
+    if (i && s.data[i] <= s.data[i - 1]) return false;
+  }
+  return true;
+}
+
+std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items, size_t n) {
+  auto b = std::make_unique<Batch>();
+  b->kind = CUBBS_HIP_BATCH_PROOF_VERIFY;
+  b->n_total = n;
+  b->pre_code.assign(n, -1);
+  b->pre_value.assign(n, 0);
+  b->dev_index.assign(n, 0);
+  std::vector<size_t> dev;
+  uint32_t maxL = 0;
+  for (size_t i = 0; i < n; i++) {
+    const auto& it = items[i];
+    // protocol.rs:774-779: bad disclosed indexes or count mismatch -> Ok(false)
+    if (!indexes_strictly_increasing(it.disclosed_indexes, it.total_message_count) ||
+        it.disclosed_count != it.disclosed_indexes.len ||
+        (it.disclosed_count && !it.disclosed_messages)) {
+      b->pre_code[i] = CUBBS_BBS_GPU_SUCCESS;
+      continue;
+    }
+    bool ok = it.public_key.len == 96 && it.public_key.data && slice_ok(it.header) &&
+              slice_ok(it.presentation_header) && it.proof.data && it.proof.len >= 272 &&
+              it.proof.len < (1ull << 30);
+    for (size_t m = 0; ok && m < it.disclosed_count; m++) ok = slice_ok(it.disclosed_messages[m]);
+    if (!ok) {
+      b->pre_code[i] = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      continue;
+    }
+    const size_t U = (it.proof.len - 240) / 32 - 1;
+    const size_t R = it.disclosed_count;
+    const size_t L = U + R;
+    // proof.rs:3027-3033: an index beyond L - 1 fails verification
+    bool idx_ok = true;
+    for (size_t r = 0; r < R; r++) idx_ok = idx_ok && it.disclosed_indexes.data[r] < L;
+    if (!idx_ok) {
+      b->pre_code[i] = CUBBS_BBS_GPU_SUCCESS;
+      continue;
+    }
+    b->dev_index[i] = (uint32_t)dev.size();
+    dev.push_back(i);
+    maxL = std::max<uint32_t>(maxL, (uint32_t)L);
+  }
+  b->nv = (uint32_t)dev.size();
+  if (!b->nv) return b;
+  std::vector<const uint8_t*> pks(b->nv);
+  for (uint32_t k = 0; k < b->nv; k++) pks[k] = items[dev[k]].public_key.data;
+  std::vector<uint32_t> pk_slot;
+  attach(*b, maxL + 2, pks, pk_slot);
+  Plan p;
+  std::string d = map_dst();
+  p.map_dst.assign(d.begin(), d.end());
+  const uint32_t CH = comb_chunk_terms();
+  uint32_t T = 0;
+  for (uint32_t k = 0; k < b->nv; k++) {
+    const auto& it = items[dev[k]];
+    T += (uint32_t)((it.proof.len - 240) / 32 - 1 + it.disclosed_count) + 2;
+  }
+  p.scalars(T);
+  p.term_gen.resize(T);
+  uint32_t term = 0;
+  for (uint32_t k = 0; k < b->nv; k++) {
+    const auto& it = items[dev[k]];
+    const uint32_t U = (uint32_t)((it.proof.len - 240) / 32 - 1);
+    const uint32_t R = (uint32_t)it.disclosed_count;
+    const uint32_t L = U + R;
+    size_t pr = p.bytes(it.proof.data, 240 + 32 * (size_t)(U + 1));
+    size_t hdr = p.bytes(it.header.data, it.header.len);
+    size_t ph = p.bytes(it.presentation_header.data, it.presentation_header.len);
+    uint32_t pab = p.points(1), pbb = p.points(1), pd = p.points(1), pt1 = p.points(1),
+             pt2 = p.points(1);
+    p.dec_g1.push_back(DecodeG1Job{pr + 0, pab, k});
+    p.dec_g1.push_back(DecodeG1Job{pr + 48, pbb, k});
+    p.dec_g1.push_back(DecodeG1Job{pr + 96, pd, k});
+    uint32_t sc = p.scalars(4 + U);  // e^, r1^, r3^, m^_1..m^_U, then c
+    uint32_t s_ehat = sc, s_r1 = sc + 1, s_r3 = sc + 2, s_mhat = sc + 3, s_c = sc + 3 + U;
+    p.dec_fr.push_back(DecodeFrJob{pr + 144, s_ehat, k, 0, 0});
+    p.dec_fr.push_back(DecodeFrJob{pr + 176, s_r1, k, 0, 0});
+    p.dec_fr.push_back(DecodeFrJob{pr + 208, s_r3, k, 0, 0});
+    for (uint32_t u = 0; u <= U; u++)
+      p.dec_fr.push_back(DecodeFrJob{pr + 240 + 32ull * u, s_mhat + u, k, 0, 0});
+    uint32_t s_dom = p.scalars(1);
+    p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, s_dom});
+    uint32_t s_m = p.scalars(R);
+    uint32_t disc_begin = (uint32_t)p.disc_index.size();
+    for (uint32_t r = 0; r < R; r++) {
+      size_t mo = p.bytes(it.disclosed_messages[r].data, it.disclosed_messages[r].len);
+      p.h2s.push_back(H2sJob{mo, (uint32_t)it.disclosed_messages[r].len, s_m + r});
+      p.disc_index.push_back((uint32_t)it.disclosed_indexes.data[r]);
+      p.disc_scalar.push_back(s_m + r);
+    }
+    // T2 = c P1 + (c dom) Q1 + sum_R (c m_i) H_i + sum_U m^_j H_j + r3^ D   (proof.rs:3048-3066)
+    for (uint32_t t = 0; t < L + 2; t++) p.term_gen[term + t] = t;
+    p.fills.push_back(TermFill{term, s_c, 0, 0});
+    p.fills.push_back(TermFill{term + 1, s_dom, s_c, 1});
+    uint32_t r = 0, u = 0;
+    for (uint32_t i = 0; i < L; i++) {
+      if (r < R && it.disclosed_indexes.data[r] == i) {
+        p.fills.push_back(TermFill{term + 2 + i, s_m + r, s_c, 1});
+        r++;
+      } else {
+        p.fills.push_back(TermFill{term + 2 + i, s_mhat + u, 0, 0});
+        u++;
+      }
+    }
+    // T1 = c Bbar + e^ Abar + r1^ D
+    uint32_t f1 = p.n_partials;
+    p.var.push_back(VarJob{pbb, s_c, p.partials(1)});
+    p.var.push_back(VarJob{pab, s_ehat, p.partials(1)});
+    p.var.push_back(VarJob{pd, s_r1, p.partials(1)});
+    p.groups1.push_back(ReduceGroup{f1, 3});
+    p.out1.push_back(pt1);
+    uint32_t f2 = p.n_partials;
+    p.var.push_back(VarJob{pd, s_r3, p.partials(1)});
+    p.comb(term, L + 2, CH);
+    p.groups1.push_back(ReduceGroup{f2, p.n_partials - f2});
+    p.out1.push_back(pt2);
+    p.pvch.push_back(PvChallengeJob{ph, (uint32_t)it.presentation_header.len, k, R, disc_begin,
+                                    pab, pbb, pd, pt1, pt2, s_dom, s_c});
+    p.pairs.push_back(PairJob{k, pab, pbb, pk_slot[k]});
+    term += L + 2;
+  }
+  materialize(*b, p);
+  return b;
+}
+
+// ------------------------------------------------------------------ Sign
+std::unique_ptr<Batch> plan_sign(const CuBbsBbsGpuSignItem* items, size_t n) {
+  auto b = std::make_unique<Batch>();
+  b->kind = KIND_SIGN;
+  b->n_total = n;
+  b->pre_code.assign(n, -1);
+  b->pre_value.assign(n, 0);
+  b->dev_index.assign(n, 0);
+  std::vector<size_t> dev;
+  uint32_t maxL = 0;
+  for (size_t i = 0; i < n; i++) {
+    const auto& it = items[i];
+    bool ok = it.secret_key.len == 32 && it.secret_key.data && it.public_key.len == 96 &&
+              it.public_key.data && slice_ok(it.header) &&
+              (it.message_count == 0 || it.messages != nullptr) && it.message_count < (1u << 24);
+    for (size_t m = 0; ok && m < it.message_count; m++) ok = slice_ok(it.messages[m]);
+    if (!ok) {
+      b->pre_code[i] = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      continue;
+    }
+    if (!it.signature_out.data || it.signature_out.capacity < 80) {
+      b->pre_code[i] = CUBBS_BBS_GPU_ERR_INSUFFICIENT_OUTPUT;
+      continue;
+    }
+    b->dev_index[i] = (uint32_t)dev.size();
+    dev.push_back(i);
+    maxL = std::max<uint32_t>(maxL, (uint32_t)it.message_count);
+  }
+  b->nv = (uint32_t)dev.size();
+  if (!b->nv) return b;
+  std::vector<const uint8_t*> pks(b->nv);
+  for (uint32_t k = 0; k < b->nv; k++) pks[k] = items[dev[k]].public_key.data;
+  std::vector<uint32_t> pk_slot;
+  attach(*b, maxL + 2, pks, pk_slot);
+  Plan p;
+  std::string d = map_dst();
+  p.map_dst.assign(d.begin(), d.end());
+  const uint32_t CH = comb_chunk_terms();
+  uint32_t T = 0;
+  for (uint32_t k = 0; k < b->nv; k++) T += (uint32_t)items[dev[k]].message_count + 2;
+  p.scalars(T);
+  p.term_gen.resize(T);
+  uint32_t term = 0;
+  for (uint32_t k = 0; k < b->nv; k++) {
+    const auto& it = items[dev[k]];
+    const uint32_t L = (uint32_t)it.message_count;
+    size_t sk = p.bytes(it.secret_key.data, 32);
+    size_t hdr = p.bytes(it.header.data, it.header.len);
+    p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, term + 1});
+    for (uint32_t m = 0; m < L; m++) {
+      size_t mo = p.bytes(it.messages[m].data, it.messages[m].len);
+      p.h2s.push_back(H2sJob{mo, (uint32_t)it.messages[m].len, term + 2 + m});
+    }
+    for (uint32_t t = 0; t < L + 2; t++) p.term_gen[term + t] = t;
+    uint32_t se = p.scalars(1), pa = p.points(1);
+    uint32_t first = p.n_partials;
+    p.comb(term, L + 2, CH);
+    p.groups1.push_back(ReduceGroup{first, p.n_partials - first});
+    p.out1.push_back(pa);
+    p.sign.push_back(SignJob{sk, k, term, L, se, pa, pk_slot[k]});
+    term += L + 2;
+  }
+  p.out_bytes = 80ull * b->nv;
+  materialize(*b, p);
+  return b;
+}
+
+// ------------------------------------------------------------------ ProofGen
+// scalars_mode: 0 = OS randomness (seeds drawn here), 1 = seeds given,
+// 2 = explicit scalars.
+std::unique_ptr<Batch> plan_proof_gen(const CuBbsBbsGpuProofGenItem* items, size_t n, int mode,
+                                      const uint8_t* seeds32, const uint8_t* scalars_be,
+                                      const size_t* scalar_offsets) {
+  auto b = std::make_unique<Batch>();
+  b->kind = KIND_PROOF_GEN;
+  b->n_total = n;
+  b->pre_code.assign(n, -1);
+  b->pre_value.assign(n, 0);
+  b->dev_index.assign(n, 0);
+  std::vector<size_t> dev;
+  uint32_t maxL = 0;
+  for (size_t i = 0; i < n; i++) {
+    const auto& it = items[i];
+    bool ok = it.public_key.len == 96 && it.public_key.data && it.signature.len == 80 &&
+              it.signature.data && slice_ok(it.header) && slice_ok(it.presentation_header) &&
+              (it.message_count == 0 || it.messages != nullptr) && it.message_count < (1u << 24) &&
+              indexes_strictly_increasing(it.disclosed_indexes, it.message_count);
+    for (size_t m = 0; ok && m < it.message_count; m++) ok = slice_ok(it.messages[m]);
+    if (!ok) {
+      b->pre_code[i] = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      continue;
+    }
+    size_t U = it.message_count - it.disclosed_indexes.len;
+    if (!it.proof_out.data || it.proof_out.capacity < 272 + 32 * U) {
+      b->pre_code[i] = CUBBS_BBS_GPU_ERR_INSUFFICIENT_OUTPUT;
+      continue;
+    }
+    b->dev_index[i] = (uint32_t)dev.size();
+    dev.push_back(i);
+    maxL = std::max<uint32_t>(maxL, (uint32_t)it.message_count);
+  }
+  b->nv = (uint32_t)dev.size();
+  if (!b->nv) return b;
+  std::vector<const uint8_t*> pks(b->nv);
+  for (uint32_t k = 0; k < b->nv; k++) pks[k] = items[dev[k]].public_key.data;
+  std::vector<uint32_t> pk_slot;
+  attach(*b, maxL + 2, pks, pk_slot);
+  Plan p;
+  std::string d = map_dst();
+  p.map_dst.assign(d.begin(), d.end());
+  const uint32_t CH = comb_chunk_terms();
+  uint32_t T = 0;
+  for (uint32_t k = 0; k < b->nv; k++) {
+    const auto& it = items[dev[k]];
+    T += (uint32_t)it.message_count + 2 + (uint32_t)(it.message_count - it.disclosed_indexes.len);
+  }
+  p.scalars(T);
+  p.term_gen.resize(T);
+  std::vector<uint8_t> seeds;
+  if (mode != 2) {
+    seeds.resize(32ull * b->nv);
+    if (mode == 1) {
+      for (uint32_t k = 0; k < b->nv; k++) std::memcpy(seeds.data() + 32ull * k, seeds32 + 32 * dev[k], 32);
+    } else {
+      fill_os_random(seeds.data(), seeds.size());
+    }
+  }
+  uint32_t term = 0;
+  size_t out = 0;
+  for (uint32_t k = 0; k < b->nv; k++) {
+    const auto& it = items[dev[k]];
+    const uint32_t L = (uint32_t)it.message_count;
+    const uint32_t R = (uint32_t)it.disclosed_indexes.len;
+    const uint32_t U = L - R;
+    size_t sig = p.bytes(it.signature.data, 80);
+    size_t hdr = p.bytes(it.header.data, it.header.len);
+    size_t ph = p.bytes(it.presentation_header.data, it.presentation_header.len);
+    uint32_t pa = p.points(1), pb = p.points(1);
+    uint32_t pd = p.points(1), pab = p.points(1), pbb = p.points(1), pt1 = p.points(1),
+             pt2 = p.points(1);
+    uint32_t s_e = p.scalars(1), s_dom = p.scalars(1), s_m = p.scalars(L), rs = p.scalars(5 + U),
+             kk = p.scalars(7);
+    p.dec_g1.push_back(DecodeG1Job{sig, pa, k});
+    p.dec_fr.push_back(DecodeFrJob{sig + 48, s_e, k, 0, 0});
+    p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, s_dom});
+    for (uint32_t m = 0; m < L; m++) {
+      size_t mo = p.bytes(it.messages[m].data, it.messages[m].len);
+      p.h2s.push_back(H2sJob{mo, (uint32_t)it.messages[m].len, s_m + m});
+    }
+    if (mode == 2) {
+      size_t so = p.bytes(scalars_be + 32 * scalar_offsets[dev[k]], 32ull * (5 + U));
+      for (uint32_t t = 0; t < 5 + U; t++) p.dec_fr.push_back(DecodeFrJob{so + 32ull * t, rs + t, k, 0, 0});
+    }
+    // B = P1 + Q1 dom + sum H_i m_i
+    for (uint32_t t = 0; t < L + 2; t++) p.term_gen[term + t] = t;
+    p.fills.push_back(TermFill{term, 0, 0, 2});
+    p.fills.push_back(TermFill{term + 1, s_dom, 0, 0});
+    for (uint32_t m = 0; m < L; m++) p.fills.push_back(TermFill{term + 2 + m, s_m + m, 0, 0});
+    uint32_t fb = p.n_partials;
+    p.comb(term, L + 2, CH);
+    p.groups1.push_back(ReduceGroup{fb, p.n_partials - fb});
+    p.out1.push_back(pb);
+    term += L + 2;
+    // sum_U H_j m~_j (part of T2)
+    uint32_t disc_begin = (uint32_t)p.disc_index.size();
+    uint32_t und_begin = (uint32_t)p.und_scalar.size();
+    uint32_t r = 0, u = 0;
+    for (uint32_t i = 0; i < L; i++) {
+      if (r < R && it.disclosed_indexes.data[r] == i) {
+        p.disc_index.push_back(i);
+        p.disc_scalar.push_back(s_m + i);
+        r++;
+      } else {
+        p.term_gen[term + u] = 2 + i;
+        p.fills.push_back(TermFill{term + u, rs + 5 + u, 0, 0});
+        p.und_scalar.push_back(s_m + i);
+        u++;
+      }
+    }
+    uint32_t q = p.partials(7);
+    p.var.push_back(VarJob{pb, kk + 0, q + 0});   // D = r2 B
+    p.var.push_back(VarJob{pa, kk + 1, q + 1});   // Abar = r1 r2 A
+    p.var.push_back(VarJob{pb, kk + 2, q + 2});   // Bbar = r1 r2 B
+    p.var.push_back(VarJob{pa, kk + 3, q + 3});   //       - r1 r2 e A
+    p.var.push_back(VarJob{pa, kk + 4, q + 4});   // T1 = r1 r2 e~ A
+    p.var.push_back(VarJob{pb, kk + 5, q + 5});   //      + r2 r1~ B
+    p.var.push_back(VarJob{pb, kk + 6, q + 6});   // T2 = r2 r3~ B + sum
+    uint32_t ft2 = p.n_partials;
+    p.comb(term, U, CH);
+    term += U;
+    p.groups2.push_back(ReduceGroup{q + 0, 1});
+    p.out2.push_back(pd);
+    p.groups2.push_back(ReduceGroup{q + 1, 1});
+    p.out2.push_back(pab);
+    p.groups2.push_back(ReduceGroup{q + 2, 2});
+    p.out2.push_back(pbb);
+    p.groups2.push_back(ReduceGroup{q + 4, 2});
+    p.out2.push_back(pt1);
+    // T2 partials: q + 6 then the comb chunks [ft2, n_partials) -- contiguous
+    // only if ft2 == q + 7, which holds because partials(7) was taken just before.
+    p.groups2.push_back(ReduceGroup{q + 6, 1 + (p.n_partials - ft2)});
+    p.out2.push_back(pt2);
+    PgJob job{};
+    job.ph_off = ph;
+    job.proof_off = out;
+    job.ph_len = (uint32_t)it.presentation_header.len;
+    job.item = k;
+    job.L = L;
+    job.R = R;
+    job.U = U;
+    job.disc_begin = disc_begin;
+    job.und_begin = und_begin;
+    job.rs = rs;
+    job.e_index = s_e;
+    job.dom_index = s_dom;
+    job.k_index = kk;
+    job.a = pa;
+    job.b = pb;
+    job.d = pd;
+    job.abar = pab;
+    job.bbar = pbb;
+    job.t1 = pt1;
+    job.t2 = pt2;
+    p.pg.push_back(job);
+    b->out_off.push_back(out);
+    b->out_len.push_back(272 + 32ull * U);
+    out += 272 + 32ull * U;
+  }
+  p.out_bytes = out;
+  if (mode != 2) {
+    p.seeded = true;
+    p.seeds_off = p.bytes(seeds.data(), seeds.size());
+  }
+  materialize(*b, p);
+  return b;
+}
+
+// ------------------------------------------------------------------ launching
+struct Timed {
+  Batch& b;
+  size_t idx = 0;
+  Timed(Batch& bb, const char* name) : b(bb) {
+    KernelTimer& t = b.timer;
+    if (!t.on) return;
+    auto it = std::find(t.names.begin(), t.names.end(), name);
+    if (it == t.names.end()) {
+      t.names.push_back(name);
+      t.count.push_back(0);
+      idx = t.names.size() - 1;
+    } else {
+      idx = it - t.names.begin();
+    }
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0));
+    check(hipEventCreate(&e1));
+    t.starts.push_back(e0);
+    t.stops.push_back(e1);
+    t.event_name.push_back(idx);
+    check(hipEventRecord(e0, b.stream));
+  }
+  ~Timed() {
+    if (!b.timer.on) return;
+    (void)hipEventRecord(b.timer.stops.back(), b.stream);
+    b.timer.count[idx]++;
+  }
+};
+
+void launch_batch(Batch& b) {
+  if (b.nv == 0) return;
+  hipStream_t s = b.stream;
+  {
+    Timed t(b, "fill_status");
+    launch_fill_status(s, b.nv, b.status, b.gate, 1);
+  }
+  if (b.n_dec_g1) {
+    Timed t(b, "decode_g1");
+    launch_decode_g1(s, b.dec_g1, b.n_dec_g1, b.blob, b.points, b.pinf, b.status);
+  }
+  if (b.n_dec_fr) {
+    Timed t(b, "decode_fr");
+    launch_decode_fr(s, b.dec_fr, b.n_dec_fr, b.blob, b.scalars, b.status);
+  }
+  if (b.n_h2s) {
+    Timed t(b, "h2s_messages");
+    launch_h2s(s, b.h2s, b.n_h2s, b.blob, b.map_dst, b.map_dst_len, b.scalars);
+  }
+  if (b.n_dom) {
+    Timed t(b, "domain");
+    launch_domain(s, b.dom, b.n_dom, b.blob, b.pk_bytes, b.gen_comp, b.scalars);
+  }
+  if (b.kind == KIND_PROOF_GEN && b.seeded) {
+    Timed t(b, "pg_chacha");
+    launch_pg_chacha(s, b.pg, b.n_pg, b.seeds, b.scalars);
+  }
+  if (b.n_fills) {
+    Timed t(b, "term_fill");
+    launch_term_fill(s, b.fills, b.n_fills, b.scalars);
+  }
+  if (b.kind == KIND_SIGN) {
+    Timed t(b, "sign_e");
+    launch_sign_e(s, b.sign, b.n_sign, b.blob, b.pk_status, b.scalars, b.status);
+  }
+  if (b.kind != KIND_PROOF_GEN && b.n_var) {
+    Timed t(b, "var_mul_glv");
+    launch_var_mul(s, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials);
+  }
+  if (b.n_chunks) {
+    Timed t(b, "comb_msm");
+    launch_comb_msm(s, b.chunks, b.n_chunks, b.term_gen, b.scalars, b.tables, b.partials);
+  }
+  if (b.n_groups1) {
+    Timed t(b, "reduce");
+    launch_reduce(s, b.groups1, b.out1, b.n_groups1, b.partials, b.points, b.pinf);
+  }
+  if (b.kind == KIND_PROOF_GEN) {
+    {
+      Timed t(b, "pg_scalars");
+      launch_pg_scalars(s, b.pg, b.n_pg, b.scalars, b.status);
+    }
+    {
+      Timed t(b, "var_mul_glv");
+      launch_var_mul(s, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials);
+    }
+    {
+      Timed t(b, "reduce2");
+      launch_reduce(s, b.groups2, b.out2, b.n_groups2, b.partials, b.points, b.pinf);
+    }
+    {
+      Timed t(b, "pg_finalize");
+      launch_pg_finalize(s, b.pg, b.n_pg, b.blob, b.disc_index, b.disc_scalar, b.und_scalar,
+                         b.points, b.pinf, b.scalars, b.status, b.out);
+    }
+  }
+  if (b.kind == KIND_SIGN) {
+    Timed t(b, "sign_finalize");
+    launch_sign_finalize(s, b.sign, b.n_sign, b.points, b.pinf, b.scalars, b.status, b.out);
+  }
+  if (b.n_pvch) {
+    Timed t(b, "pv_challenge");
+    launch_pv_challenge(s, b.pvch, b.n_pvch, b.blob, b.disc_index, b.disc_scalar, b.points,
+                        b.pinf, b.scalars, b.gate);
+  }
+  if (b.n_pairs) {
+    Timed t(b, "pairing");
+    launch_pairing(s, b.pairs, b.n_pairs, b.points, b.pinf, b.pk_status, b.pk_lines,
+                   b.neg_g2_lines, b.status, b.gate, b.valid);
+  }
+  check(hipGetLastError());
+}
+
+// Copy results back and fill the caller's result array.  For Sign and
+// ProofGen, outputs are written into the items' buffers (out_ptrs).
+void collect_batch(Batch& b, CuBbsBbsGpuResult* results, uint8_t* const* out_ptrs,
+                   size_t* out_lens) {
+  b.h_status.resize(b.nv);
+  b.h_valid.resize(b.nv);
+  std::vector<uint8_t> hout;
+  if (b.nv) {
+    check(hipMemcpyAsync(b.h_status.data(), b.status, b.nv, hipMemcpyDeviceToHost, b.stream));
+    check(hipMemcpyAsync(b.h_valid.data(), b.valid, b.nv, hipMemcpyDeviceToHost, b.stream));
+    if (b.out_bytes) {
+      hout.resize(b.out_bytes);
+      check(hipMemcpyAsync(hout.data(), b.out, b.out_bytes, hipMemcpyDeviceToHost, b.stream));
+    }
+    check(hipStreamSynchronize(b.stream));
+  }
+  for (size_t i = 0; i < b.n_total; i++) {
+    results[i].bytes_written = 0;
+    results[i].value = 0;
+    if (b.pre_code[i] >= 0) {
+      results[i].code = b.pre_code[i];
+      results[i].value = b.pre_value[i];
+      continue;
+    }
+    uint32_t k = b.dev_index[i];
+    if (b.h_status[k] != 0) {
+      results[i].code = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      continue;
+    }
+    results[i].code = CUBBS_BBS_GPU_SUCCESS;
+    if (b.kind == CUBBS_HIP_BATCH_VERIFY || b.kind == CUBBS_HIP_BATCH_PROOF_VERIFY) {
+      results[i].value = b.h_valid[k];
+    } else if (b.kind == KIND_SIGN) {
+      if (out_ptrs) std::memcpy(out_ptrs[i], hout.data() + 80ull * k, 80);
+      if (out_lens) out_lens[i] = 80;
+      results[i].bytes_written = 80;
+    } else if (b.kind == KIND_PROOF_GEN) {
+      if (out_ptrs) std::memcpy(out_ptrs[i], hout.data() + b.out_off[k], b.out_len[k]);
+      if (out_lens) out_lens[i] = b.out_len[k];
+      results[i].bytes_written = b.out_len[k];
+    }
+  }
+}
+
+Batch::~Batch() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (size_t i = 0; i < timer.starts.size(); i++) {
+    (void)hipEventDestroy(timer.starts[i]);
+    (void)hipEventDestroy(timer.stops[i]);
+  }
+  if (h_in) (void)hipHostFree(h_in);
+  if (d_in) (void)hipFree(d_in);
+  if (d_work) (void)hipFree(d_work);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+}  // namespace cubbs

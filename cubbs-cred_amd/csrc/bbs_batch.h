@@ -1,0 +1,107 @@
+// Batches of whole BBS operations staged on the device (see bbs_batch.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/cubbs_bbs_gpu_abi.h"
+#include "bbs_internal.h"
+
+namespace cubbs {
+
+// Batch kinds beyond the two public staged kinds of cubbs_hip_ext.h.
+constexpr int KIND_SIGN = 2;
+constexpr int KIND_PROOF_GEN = 3;
+
+struct KernelTimer {
+  bool on = false;
+  std::vector<std::string> names;
+  std::vector<int> count;
+  std::vector<hipEvent_t> starts, stops;
+  std::vector<size_t> event_name;
+};
+
+struct Batch {
+  int kind = 0;
+  hipStream_t stream = nullptr;
+  size_t n_total = 0;                // items passed in
+  std::vector<int32_t> pre_code;     // per item: -1 = decided on the device
+  std::vector<uint8_t> pre_value;    // value for items decided on the host
+  std::vector<uint32_t> dev_index;   // per item: index among device items
+  uint32_t nv = 0;                   // items on the device
+  std::vector<size_t> out_off, out_len;  // ProofGen output slices per device item
+  // staged inputs
+  size_t in_bytes = 0, work_bytes = 0, out_bytes = 0;
+  uint8_t* h_in = nullptr;
+  uint8_t* d_in = nullptr;
+  uint8_t* d_work = nullptr;
+  // job arrays (device)
+  H2sJob* h2s = nullptr;
+  DomainJob* dom = nullptr;
+  DecodeG1Job* dec_g1 = nullptr;
+  DecodeFrJob* dec_fr = nullptr;
+  TermFill* fills = nullptr;
+  VarJob* var = nullptr;
+  CombChunk* chunks = nullptr;
+  uint32_t* term_gen = nullptr;
+  ReduceGroup* groups1 = nullptr;
+  uint32_t* out1 = nullptr;
+  ReduceGroup* groups2 = nullptr;
+  uint32_t* out2 = nullptr;
+  PairJob* pairs = nullptr;
+  SignJob* sign = nullptr;
+  PvChallengeJob* pvch = nullptr;
+  PgJob* pg = nullptr;
+  uint32_t* disc_index = nullptr;
+  uint32_t* disc_scalar = nullptr;
+  uint32_t* und_scalar = nullptr;
+  uint8_t* map_dst = nullptr;
+  uint32_t map_dst_len = 0;
+  uint8_t* blob = nullptr;
+  uint8_t* seeds = nullptr;
+  bool seeded = false;
+  uint32_t n_h2s = 0, n_dom = 0, n_dec_g1 = 0, n_dec_fr = 0, n_fills = 0, n_var = 0, n_chunks = 0,
+           n_terms = 0, n_groups1 = 0, n_groups2 = 0, n_pairs = 0, n_sign = 0, n_pvch = 0,
+           n_pg = 0;
+  uint64_t comb_terms = 0;
+  // work arrays (device)
+  uint32_t* scalars = nullptr;
+  uint32_t* points = nullptr;
+  uint8_t* pinf = nullptr;
+  uint32_t* partials = nullptr;
+  uint8_t* status = nullptr;
+  uint8_t* gate = nullptr;
+  uint8_t* valid = nullptr;
+  uint8_t* out = nullptr;
+  uint32_t n_scalars = 0, n_points = 0, n_partials = 0;
+  // snapshot of resident state
+  const uint32_t* tables = nullptr;
+  const uint8_t* gen_comp = nullptr;
+  const uint8_t* pk_bytes = nullptr;
+  const uint8_t* pk_status = nullptr;
+  const uint32_t* pk_lines = nullptr;
+  const uint32_t* neg_g2_lines = nullptr;
+  // host results
+  std::vector<uint8_t> h_status, h_valid;
+  KernelTimer timer;
+  ~Batch();
+};
+
+uint32_t comb_chunk_terms();
+void fill_os_random(uint8_t* out, size_t n);
+
+std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n);
+std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items, size_t n);
+std::unique_ptr<Batch> plan_sign(const CuBbsBbsGpuSignItem* items, size_t n);
+// mode 0: blinding from the OS CSPRNG, 1: per-item 32-byte seeds, 2: explicit scalars
+std::unique_ptr<Batch> plan_proof_gen(const CuBbsBbsGpuProofGenItem* items, size_t n, int mode,
+                                      const uint8_t* seeds32, const uint8_t* scalars_be,
+                                      const size_t* scalar_offsets);
+void launch_batch(Batch& b);
+void collect_batch(Batch& b, CuBbsBbsGpuResult* results, uint8_t* const* out_ptrs,
+                   size_t* out_lens);
+
+}  // namespace cubbs

@@ -58,63 +58,94 @@ struct ReduceGroup {
   uint32_t count;
 };
 
-// Per-item record of a batched Verify.
-struct VerifyItem {
-  uint64_t sig_off;      // 80-byte signature in the signature blob
-  uint32_t pk_slot;
-  uint32_t term_begin;   // L + 2 terms: P1, Q1, H_1..H_L
-  uint32_t var_partial;  // partial slot receiving (-e) * A
-  uint32_t flags;        // bit0: structurally valid (lengths)
+// Decode a compressed G1 point from the input blob into points[out]; a
+// failure marks status[item] = 1 (invalid input).
+struct DecodeG1Job {
+  uint64_t off;
+  uint32_t out;
+  uint32_t item;
 };
 
-// Per-item record of a batched ProofVerify.
-struct ProofVerifyItem {
-  uint64_t proof_off;    // proof bytes in the proof blob
-  uint64_t ph_off;       // presentation header in the header blob
-  uint32_t ph_len;
-  uint32_t pk_slot;
-  uint32_t R;            // disclosed count
-  uint32_t U;            // undisclosed count
-  uint32_t disc_begin;   // index of this item's first disclosed index / scalar
-  uint32_t term_begin;   // T2 terms: P1, Q1, then H_1..H_L in index order
-  uint32_t t2_var_partial;  // r3^ * D
-  uint32_t t1_partial;      // 3 consecutive partials: c*Bbar, e^*Abar, r1^*D
-  uint32_t dom_index;    // scalar slot holding the domain
-  uint32_t flags;        // bit0: structurally valid
-};
-
-// Per-item record of a batched Sign.
-struct SignItem {
-  uint32_t term_begin;   // L + 2 terms
-  uint32_t L;
+// Decode a 32-byte big-endian scalar (canonical check) into scalars[out];
+// flags bit0: store r - s instead of s.
+struct DecodeFrJob {
+  uint64_t off;
+  uint32_t out;
+  uint32_t item;
   uint32_t flags;
   uint32_t pad;
-  uint8_t sk[32];
 };
 
-// Per-item record of a batched ProofGen.
-struct ProofGenItem {
-  uint64_t sig_off;
+// partials[partial] = scalars[scalar] * points[point] (identity-aware).
+struct VarJob {
+  uint32_t point;
+  uint32_t scalar;
+  uint32_t partial;
+};
+
+// Fill one fixed-base term scalar: mode 0: scalars[dst] = scalars[a];
+// mode 1: scalars[dst] = scalars[a] * scalars[b] mod r; mode 2: scalars[dst] = 1.
+struct TermFill {
+  uint32_t dst;
+  uint32_t a;
+  uint32_t b;
+  uint32_t mode;
+};
+
+// e(points[a], PK) * e(points[b], -G2) == 1, and gate[item] must be set.
+struct PairJob {
+  uint32_t item;
+  uint32_t a;
+  uint32_t b;
+  uint32_t pk_slot;
+};
+
+// Sign: e = H2S(sk || m_1..m_L || domain); A = B / (sk + e) computed as the
+// comb MSM over scalars scaled by 1 / (sk + e).  Terms [term, term + L + 2).
+struct SignJob {
+  uint64_t sk_off;
+  uint32_t item;
+  uint32_t term;
+  uint32_t L;
+  uint32_t e_index;
+  uint32_t a_point;   // reduce output holding A
+  uint32_t pk_slot;
+};
+
+// ProofVerify challenge (proof.rs:2155-2297): compares H2S(c_arr) with c.
+struct PvChallengeJob {
   uint64_t ph_off;
   uint32_t ph_len;
-  uint32_t pk_slot;
-  uint32_t L, R, U;
-  uint32_t disc_begin;     // disclosed indexes for this item
-  uint32_t b_term_begin;   // L + 2 terms of B
-  uint32_t t2_term_begin;  // U terms of sum H_j m~_j
-  uint32_t b_group;        // reduce group index of B
-  uint32_t t2_group;       // reduce group index of the H part of T2
-  uint32_t rs_begin;       // 5 + U random scalars (canonical)
-  uint32_t msg_scalar_begin;  // L message scalars
+  uint32_t item;
+  uint32_t R;
+  uint32_t disc_begin;   // into disc_index[] / disc_scalar[]
+  uint32_t abar, bbar, d, t1, t2;  // point indexes
   uint32_t dom_index;
-  uint32_t flags;
-  uint64_t proof_off;      // output offset in the proof output blob
+  uint32_t c_index;
+};
+
+// ProofGen per item (proof.rs:1568-1645, 1754-1815, 2317-2377).
+struct PgJob {
+  uint64_t ph_off;
+  uint64_t proof_off;     // output offset
+  uint32_t ph_len;
+  uint32_t item;
+  uint32_t L, R, U;
+  uint32_t disc_begin;    // disclosed indexes / scalar indexes
+  uint32_t und_begin;     // undisclosed message scalar indexes
+  uint32_t rs;            // first of 5 + U random scalars
+  uint32_t e_index;
+  uint32_t dom_index;
+  uint32_t k_index;       // first of 7 derived scalars
+  uint32_t a, b;          // points: A (signature), B
+  uint32_t d, abar, bbar, t1, t2;  // reduce outputs
 };
 
 // ------------------------------------------------------------------ launchers
 // (all asynchronous on the given stream)
-void launch_h2s(hipStream_t s, const H2sJob* jobs, uint32_t n, const uint8_t* blob, int dst_kind,
-                uint32_t* out_scalars);
+void launch_fill_status(hipStream_t s, uint32_t n, uint8_t* status, uint8_t* gate, uint8_t gate_value);
+void launch_h2s(hipStream_t s, const H2sJob* jobs, uint32_t n, const uint8_t* blob,
+                const uint8_t* dst, uint32_t dst_len, uint32_t* out_scalars);
 void launch_hash_jobs(hipStream_t s, int kind, const H2sJob* jobs, uint32_t n, const uint8_t* blob,
                       const uint8_t* dst, uint32_t dst_len, uint8_t* out);
 void launch_domain(hipStream_t s, const DomainJob* jobs, uint32_t n, const uint8_t* hdr_blob,
@@ -127,19 +158,42 @@ void launch_gen_points(hipStream_t s, uint32_t first, uint32_t count, const uint
 void launch_comb_build(hipStream_t s, uint32_t first, uint32_t count, const uint32_t* gen_affine,
                        uint32_t* tables, uint32_t* scratch);
 void launch_comb_msm(hipStream_t s, const CombChunk* chunks, uint32_t n, const uint32_t* term_gen,
-                     const uint32_t* term_scalar, const uint32_t* tables, uint32_t* partials);
-void launch_reduce_affine(hipStream_t s, const ReduceGroup* groups, uint32_t n,
-                          const uint32_t* partials, uint32_t* out_affine, uint8_t* out_inf);
+                     const uint32_t* scalars, const uint32_t* tables, uint32_t* partials);
+// groups[i] sums into points[out_point[i]]
+void launch_reduce(hipStream_t s, const ReduceGroup* groups, const uint32_t* out_point, uint32_t n,
+                   const uint32_t* partials, uint32_t* points, uint8_t* pinf);
 void launch_prepare_pk(hipStream_t s, const uint32_t* slots, uint32_t n, const uint8_t* pk_bytes,
                        uint8_t* pk_status, uint32_t* pk_lines);
 void launch_prepare_neg_g2(hipStream_t s, uint32_t* lines);
-void launch_verify_sig(hipStream_t s, const VerifyItem* items, uint32_t n, const uint8_t* sig_blob,
-                       uint32_t* term_scalar, uint32_t* a_affine, uint8_t* a_inf, uint8_t* status,
-                       uint32_t* partials);
-void launch_verify_pairing(hipStream_t s, const VerifyItem* items, uint32_t n,
-                           const uint32_t* a_affine, const uint8_t* a_inf, const uint32_t* b_affine,
-                           const uint8_t* b_inf, const uint8_t* pk_status, const uint32_t* pk_lines,
-                           const uint32_t* neg_g2_lines, uint8_t* status, uint8_t* valid);
+void launch_decode_g1(hipStream_t s, const DecodeG1Job* jobs, uint32_t n, const uint8_t* blob,
+                      uint32_t* points, uint8_t* pinf, uint8_t* status);
+void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const uint8_t* blob,
+                      uint32_t* scalars, uint8_t* status);
+void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_t* points,
+                    const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials);
+void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t* scalars);
+void launch_pairing(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
+                    const uint8_t* pinf, const uint8_t* pk_status, const uint32_t* pk_lines,
+                    const uint32_t* neg_g2_lines, uint8_t* status, const uint8_t* gate,
+                    uint8_t* valid);
+void launch_sign_e(hipStream_t s, const SignJob* jobs, uint32_t n, const uint8_t* blob,
+                   const uint8_t* pk_status, uint32_t* scalars, uint8_t* status);
+void launch_sign_finalize(hipStream_t s, const SignJob* jobs, uint32_t n, const uint32_t* points,
+                          const uint8_t* pinf, const uint32_t* scalars, uint8_t* status,
+                          uint8_t* sig_out);
+void launch_pv_challenge(hipStream_t s, const PvChallengeJob* jobs, uint32_t n,
+                         const uint8_t* blob, const uint32_t* disc_index,
+                         const uint32_t* disc_scalar, const uint32_t* points, const uint8_t* pinf,
+                         const uint32_t* scalars, uint8_t* gate);
+void launch_pg_chacha(hipStream_t s, const PgJob* jobs, uint32_t n, const uint8_t* seeds,
+                      uint32_t* scalars);
+void launch_pg_scalars(hipStream_t s, const PgJob* jobs, uint32_t n, uint32_t* scalars,
+                       uint8_t* status);
+void launch_pg_finalize(hipStream_t s, const PgJob* jobs, uint32_t n, const uint8_t* blob,
+                        const uint32_t* disc_index, const uint32_t* disc_scalar,
+                        const uint32_t* und_scalar, const uint32_t* points, const uint8_t* pinf,
+                        uint32_t* scalars, uint8_t* status, uint8_t* proof_out);
+void launch_g2_mul_gen(hipStream_t s, const uint8_t* sk, uint32_t n, uint8_t* pk_out, uint8_t* ok);
 
 // test-only primitives
 void launch_test_g1_mul(hipStream_t s, const uint8_t* pts, const uint8_t* scalars, uint32_t n,
@@ -149,5 +203,6 @@ void launch_test_pairing_gt(hipStream_t s, const uint8_t* g1, const uint8_t* g2,
 void launch_test_g2_decompress(hipStream_t s, const uint8_t* g2, uint32_t n, uint8_t* out,
                                uint8_t* ok);
 void launch_fp_mul_bench(hipStream_t s, uint32_t* data, uint32_t n, uint32_t iters);
+void launch_mad_peak(hipStream_t s, uint32_t* data, uint32_t n, uint32_t iters);
 
 }  // namespace cubbs

@@ -2,84 +2,29 @@
 // (message, term chunk, signature, pairing check); every kernel is bounds-
 // checked against its job count and exits on all lanes.
 #include "bbs_internal.h"
+#include "dev_io.hpp"
 #include "h2c.hpp"
-#include "pairing.hpp"
 
 namespace cubbs {
 
 #define GRID(n, b) dim3(((n) + (b) - 1) / (b)), dim3(b)
 
-DEV void load_fr(Fr& r, const uint32_t* p) {
-#pragma unroll
-  for (int k = 0; k < 8; k++) r.v[k] = p[k];
-}
-DEV void store_fr(uint32_t* p, const Fr& r) {
-#pragma unroll
-  for (int k = 0; k < 8; k++) p[k] = r.v[k];
-}
-DEV void load_g1a(G1A& r, const uint32_t* p) {
-#pragma unroll
-  for (int k = 0; k < 12; k++) {
-    r.x.v[k] = p[k];
-    r.y.v[k] = p[12 + k];
-  }
-}
-DEV void store_g1a(uint32_t* p, const G1A& r) {
-#pragma unroll
-  for (int k = 0; k < 12; k++) {
-    p[k] = r.x.v[k];
-    p[12 + k] = r.y.v[k];
-  }
-}
-DEV void load_g1j(G1J& r, const uint32_t* p) {
-#pragma unroll
-  for (int k = 0; k < 12; k++) {
-    r.x.v[k] = p[k];
-    r.y.v[k] = p[12 + k];
-    r.z.v[k] = p[24 + k];
-  }
-}
-DEV void store_g1j(uint32_t* p, const G1J& r) {
-#pragma unroll
-  for (int k = 0; k < 12; k++) {
-    p[k] = r.x.v[k];
-    p[12 + k] = r.y.v[k];
-    p[24 + k] = r.z.v[k];
-  }
-}
-DEV void load_fp2(Fp2& r, const uint32_t* p) {
-#pragma unroll
-  for (int k = 0; k < 12; k++) {
-    r.c0.v[k] = p[k];
-    r.c1.v[k] = p[12 + k];
-  }
-}
-DEV void store_fp2(uint32_t* p, const Fp2& r) {
-#pragma unroll
-  for (int k = 0; k < 12; k++) {
-    p[k] = r.c0.v[k];
-    p[12 + k] = r.c1.v[k];
-  }
-}
-
 // ------------------------------------------------------------------ hash to scalar
 __global__ __launch_bounds__(256) void k_h2s(const H2sJob* __restrict__ jobs, uint32_t n,
-                                            const uint8_t* __restrict__ blob, int dst_kind,
+                                            const uint8_t* __restrict__ blob,
+                                            const uint8_t* __restrict__ dst, uint32_t dst_len,
                                             uint32_t* __restrict__ out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   H2sJob j = jobs[i];
   Fr s;
-  if (dst_kind == 0)
-    hash_to_scalar_global(s, blob + j.off, j.len, DST_MAP_MSG, DST_MAP_MSG_LEN);
-  else
-    hash_to_scalar_global(s, blob + j.off, j.len, DST_H2S, DST_H2S_LEN);
+  hash_to_scalar_global(s, blob + j.off, j.len, dst, dst_len);
   store_fr(out + (size_t)j.out_index * FR_WORDS, s);
 }
-void launch_h2s(hipStream_t s, const H2sJob* jobs, uint32_t n, const uint8_t* blob, int dst_kind,
-                uint32_t* out) {
+void launch_h2s(hipStream_t s, const H2sJob* jobs, uint32_t n, const uint8_t* blob,
+                const uint8_t* dst, uint32_t dst_len, uint32_t* out) {
   if (!n) return;
-  hipLaunchKernelGGL(k_h2s, GRID(n, 256), 0, s, jobs, n, blob, dst_kind, out);
+  hipLaunchKernelGGL(k_h2s, GRID(n, 256), 0, s, jobs, n, blob, dst, dst_len, out);
 }
 
 // Packed hash jobs with a caller DST in device memory: kind 0 = SHA-256
@@ -357,40 +302,6 @@ void launch_comb_msm(hipStream_t s, const CombChunk* chunks, uint32_t n, const u
                      partials);
 }
 
-// Sum partials of each group; write affine (Montgomery) and an identity flag.
-__global__ __launch_bounds__(64) void k_reduce_affine(const ReduceGroup* __restrict__ groups,
-                                                     uint32_t n, const uint32_t* __restrict__ partials,
-                                                     uint32_t* __restrict__ out_affine,
-                                                     uint8_t* __restrict__ out_inf) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  ReduceGroup g = groups[i];
-  G1J acc;
-  pt_set_inf(acc);
-  for (uint32_t k = 0; k < g.count; k++) {
-    G1J p;
-    load_g1j(p, partials + (size_t)(g.begin + k) * G1J_WORDS);
-    pt_add(acc, acc, p);
-  }
-  if (pt_is_inf(acc)) {
-    out_inf[i] = 1;
-    G1A z;
-    fp_zero(z.x);
-    fp_zero(z.y);
-    store_g1a(out_affine + (size_t)i * G1A_WORDS, z);
-  } else {
-    out_inf[i] = 0;
-    G1A a;
-    pt_to_affine(a, acc);
-    store_g1a(out_affine + (size_t)i * G1A_WORDS, a);
-  }
-}
-void launch_reduce_affine(hipStream_t s, const ReduceGroup* groups, uint32_t n,
-                          const uint32_t* partials, uint32_t* out_affine, uint8_t* out_inf) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_reduce_affine, GRID(n, 64), 0, s, groups, n, partials, out_affine, out_inf);
-}
-
 // ------------------------------------------------------------------ public keys
 // pk_status: 0 invalid, 1 valid, 2 valid identity
 __global__ __launch_bounds__(64) void k_prepare_pk(const uint32_t* __restrict__ slots, uint32_t n,
@@ -430,103 +341,6 @@ __global__ void k_prepare_neg_g2(uint32_t* lines) {
 }
 void launch_prepare_neg_g2(hipStream_t s, uint32_t* lines) {
   hipLaunchKernelGGL(k_prepare_neg_g2, dim3(1), dim3(1), 0, s, lines);
-}
-
-// ------------------------------------------------------------------ Verify
-// status per item: 0 ok, 1 invalid input (signature encoding)
-// Parses A (compressed G1, subgroup checked) and e (canonical), writes the P1
-// term scalar (= 1), A in affine, and the variable partial (r - e) * A.
-__global__ __launch_bounds__(64) void k_verify_sig(const VerifyItem* __restrict__ items, uint32_t n,
-                                                  const uint8_t* __restrict__ sig_blob,
-                                                  uint32_t* __restrict__ term_scalar,
-                                                  uint32_t* __restrict__ a_affine,
-                                                  uint8_t* __restrict__ a_inf,
-                                                  uint8_t* __restrict__ status,
-                                                  uint32_t* __restrict__ partials) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  VerifyItem it = items[i];
-  G1J negeA;
-  pt_set_inf(negeA);
-  if (!(it.flags & 1)) {
-    status[i] = 1;
-    store_g1j(partials + (size_t)it.var_partial * G1J_WORDS, negeA);
-    return;
-  }
-  const uint8_t* sig = sig_blob + it.sig_off;
-  G1J A;
-  bool ok = g1_decompress(A, sig);
-  Fr e;
-  fr_from_be_bytes_raw(e, sig + 48);
-  ok = ok && fr_canonical(e);
-  Fr one;
-  fr_zero(one);
-  one.v[0] = 1;
-  store_fr(term_scalar + (size_t)it.term_begin * FR_WORDS, one);
-  status[i] = ok ? 0 : 1;
-  if (ok && !pt_is_inf(A)) {
-    G1A Aa;
-    Aa.x = A.x;
-    Aa.y = A.y;  // z = 1 after decompression
-    store_g1a(a_affine + (size_t)i * G1A_WORDS, Aa);
-    a_inf[i] = 0;
-    Fr ne;
-    fr_neg(ne, e);  // canonical r - e (0 stays 0)
-    g1_mul_glv(negeA, Aa, ne.v);
-  } else {
-    a_inf[i] = 1;
-  }
-  store_g1j(partials + (size_t)it.var_partial * G1J_WORDS, negeA);
-}
-void launch_verify_sig(hipStream_t s, const VerifyItem* items, uint32_t n, const uint8_t* sig_blob,
-                       uint32_t* term_scalar, uint32_t* a_affine, uint8_t* a_inf, uint8_t* status,
-                       uint32_t* partials) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_verify_sig, GRID(n, 64), 0, s, items, n, sig_blob, term_scalar, a_affine,
-                     a_inf, status, partials);
-}
-
-// e(A, PK) * e(B - eA, -G2) == 1  <=>  e(A, PK + e G2) * e(B, -G2) == 1
-// (crates/cubbs-bbs-protocol/src/bbsplus/signature.rs:708-728)
-__global__ __launch_bounds__(64) void k_verify_pairing(
-    const VerifyItem* __restrict__ items, uint32_t n, const uint32_t* __restrict__ a_affine,
-    const uint8_t* __restrict__ a_inf, const uint32_t* __restrict__ b_affine,
-    const uint8_t* __restrict__ b_inf, const uint8_t* __restrict__ pk_status,
-    const uint32_t* __restrict__ pk_lines, const uint32_t* __restrict__ neg_g2_lines,
-    uint8_t* __restrict__ status, uint8_t* __restrict__ valid) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  VerifyItem it = items[i];
-  if (status[i] != 0) {
-    valid[i] = 0;
-    return;
-  }
-  const uint8_t pks = pk_status[it.pk_slot];
-  if (pks == 0) {
-    status[i] = 1;
-    valid[i] = 0;
-    return;
-  }
-  G1A ps[2];
-  bool skip[2];
-  load_g1a(ps[0], a_affine + (size_t)i * G1A_WORDS);
-  load_g1a(ps[1], b_affine + (size_t)i * G1A_WORDS);
-  skip[0] = a_inf[i] || pks == 2;
-  skip[1] = b_inf[i];
-  const Line* lines[2] = {reinterpret_cast<const Line*>(pk_lines + (size_t)it.pk_slot * PK_LINES_WORDS),
-                          reinterpret_cast<const Line*>(neg_g2_lines)};
-  Fp12 f, r;
-  miller_loop_prepared<2>(f, ps, skip, lines);
-  final_exponentiation(r, f);
-  valid[i] = fp12_is_one(r) ? 1 : 0;
-}
-void launch_verify_pairing(hipStream_t s, const VerifyItem* items, uint32_t n,
-                           const uint32_t* a_affine, const uint8_t* a_inf, const uint32_t* b_affine,
-                           const uint8_t* b_inf, const uint8_t* pk_status, const uint32_t* pk_lines,
-                           const uint32_t* neg_g2_lines, uint8_t* status, uint8_t* valid) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_verify_pairing, GRID(n, 64), 0, s, items, n, a_affine, a_inf, b_affine,
-                     b_inf, pk_status, pk_lines, neg_g2_lines, status, valid);
 }
 
 // ------------------------------------------------------------------ test-only primitives
@@ -642,6 +456,34 @@ __global__ __launch_bounds__(256) void k_fp_mul_bench(uint32_t* data, uint32_t n
 }
 void launch_fp_mul_bench(hipStream_t s, uint32_t* data, uint32_t n, uint32_t iters) {
   hipLaunchKernelGGL(k_fp_mul_bench, GRID(n, 256), 0, s, data, n, iters);
+}
+
+// Peak v_mad_u64_u32 throughput: 8 independent 64-bit accumulator chains per
+// lane, `iters` rounds; 8 * iters multiply-adds per lane.
+__global__ __launch_bounds__(256) void k_mad_peak(uint32_t* data, uint32_t n, uint32_t iters) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t x = data[i] | 1u;
+  uint64_t a0 = x, a1 = x + 1, a2 = x + 2, a3 = x + 3, a4 = x + 4, a5 = x + 5, a6 = x + 6, a7 = x + 7;
+  uint32_t y = x * 2654435761u;
+#pragma unroll 1
+  for (uint32_t t = 0; t < iters; t++) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      a0 = (uint64_t)(uint32_t)a0 * y + (a0 >> 32);
+      a1 = (uint64_t)(uint32_t)a1 * y + (a1 >> 32);
+      a2 = (uint64_t)(uint32_t)a2 * y + (a2 >> 32);
+      a3 = (uint64_t)(uint32_t)a3 * y + (a3 >> 32);
+      a4 = (uint64_t)(uint32_t)a4 * y + (a4 >> 32);
+      a5 = (uint64_t)(uint32_t)a5 * y + (a5 >> 32);
+      a6 = (uint64_t)(uint32_t)a6 * y + (a6 >> 32);
+      a7 = (uint64_t)(uint32_t)a7 * y + (a7 >> 32);
+    }
+  }
+  data[i] = (uint32_t)(a0 ^ a1 ^ a2 ^ a3 ^ a4 ^ a5 ^ a6 ^ a7);
+}
+void launch_mad_peak(hipStream_t s, uint32_t* data, uint32_t n, uint32_t iters) {
+  hipLaunchKernelGGL(k_mad_peak, GRID(n, 256), 0, s, data, n, iters);
 }
 
 }  // namespace cubbs

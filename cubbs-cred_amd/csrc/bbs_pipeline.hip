@@ -1,0 +1,577 @@
+// Batched whole-operation pipelines (Sign, Verify, ProofGen, ProofVerify)
+// built from generic stages: decode, term fill, variable-base GLV
+// multiplication, reduction of partial sums, pairing checks, and the
+// operation-specific hash/finalise steps.  Every kernel maps one lane to one
+// job and is bounds-checked against its job count.
+//
+// Arrays shared by all stages of a batch:
+//   scalars[] : canonical Fr (8 words); [0, T) are the fixed-base term scalars
+//   points[]  : affine G1 (Montgomery, 24 words) with pinf[] identity flags
+//   partials[]: Jacobian G1 partial sums
+//   status[]  : per item, 0 ok, 1 invalid input / crypto failure
+//   gate[]    : per item, extra condition of the pairing check (challenge)
+#include "bbs_internal.h"
+#include "dev_io.hpp"
+
+namespace cubbs {
+
+#define GRID(n, b) dim3(((n) + (b) - 1) / (b)), dim3(b)
+
+__global__ void k_fill_status(uint32_t n, uint8_t* status, uint8_t* gate, uint8_t gv) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  status[i] = 0;
+  gate[i] = gv;
+}
+void launch_fill_status(hipStream_t s, uint32_t n, uint8_t* status, uint8_t* gate, uint8_t gv) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_status, GRID(n, 256), 0, s, n, status, gate, gv);
+}
+
+// ------------------------------------------------------------------ decoding
+__global__ __launch_bounds__(64) void k_decode_g1(const DecodeG1Job* __restrict__ jobs, uint32_t n,
+                                                 const uint8_t* __restrict__ blob,
+                                                 uint32_t* __restrict__ points,
+                                                 uint8_t* __restrict__ pinf,
+                                                 uint8_t* __restrict__ status) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  DecodeG1Job j = jobs[i];
+  G1J p;
+  bool ok = g1_decompress(p, blob + j.off);
+  G1A a;
+  if (ok && !pt_is_inf(p)) {
+    a.x = p.x;
+    a.y = p.y;
+    pinf[j.out] = 0;
+  } else {
+    fp_zero(a.x);
+    fp_zero(a.y);
+    pinf[j.out] = 1;
+  }
+  store_g1a(points + (size_t)j.out * G1A_WORDS, a);
+  if (!ok) status[j.item] = 1;
+}
+void launch_decode_g1(hipStream_t s, const DecodeG1Job* jobs, uint32_t n, const uint8_t* blob,
+                      uint32_t* points, uint8_t* pinf, uint8_t* status) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_decode_g1, GRID(n, 64), 0, s, jobs, n, blob, points, pinf, status);
+}
+
+__global__ __launch_bounds__(256) void k_decode_fr(const DecodeFrJob* __restrict__ jobs, uint32_t n,
+                                                  const uint8_t* __restrict__ blob,
+                                                  uint32_t* __restrict__ scalars,
+                                                  uint8_t* __restrict__ status) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  DecodeFrJob j = jobs[i];
+  Fr s;
+  fr_from_be_bytes_raw(s, blob + j.off);
+  if (!fr_canonical(s)) {
+    status[j.item] = 1;
+    fr_zero(s);
+  }
+  if (j.flags & 1) fr_neg(s, s);
+  store_fr(scalars + (size_t)j.out * FR_WORDS, s);
+}
+void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const uint8_t* blob,
+                      uint32_t* scalars, uint8_t* status) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_decode_fr, GRID(n, 256), 0, s, jobs, n, blob, scalars, status);
+}
+
+// ------------------------------------------------------------------ term scalars
+__global__ __launch_bounds__(256) void k_term_fill(const TermFill* __restrict__ jobs, uint32_t n,
+                                                  uint32_t* __restrict__ scalars) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  TermFill j = jobs[i];
+  Fr r;
+  if (j.mode == 2) {
+    fr_zero(r);
+    r.v[0] = 1;
+  } else {
+    load_fr(r, scalars + (size_t)j.a * FR_WORDS);
+    if (j.mode == 1) {
+      Fr b;
+      load_fr(b, scalars + (size_t)j.b * FR_WORDS);
+      fr_to_mont(b, b);
+      fr_mul(r, r, b);  // canonical * Montgomery -> canonical product
+    }
+  }
+  store_fr(scalars + (size_t)j.dst * FR_WORDS, r);
+}
+void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t* scalars) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_term_fill, GRID(n, 256), 0, s, jobs, n, scalars);
+}
+
+// ------------------------------------------------------------------ variable-base multiplication
+__global__ __launch_bounds__(64) void k_var_mul(const VarJob* __restrict__ jobs, uint32_t n,
+                                               const uint32_t* __restrict__ points,
+                                               const uint8_t* __restrict__ pinf,
+                                               const uint32_t* __restrict__ scalars,
+                                               uint32_t* __restrict__ partials) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  VarJob j = jobs[i];
+  G1J r;
+  pt_set_inf(r);
+  if (!pinf[j.point]) {
+    G1A p;
+    load_g1a(p, points + (size_t)j.point * G1A_WORDS);
+    Fr k;
+    load_fr(k, scalars + (size_t)j.scalar * FR_WORDS);
+    g1_mul_glv(r, p, k.v);
+  }
+  store_g1j(partials + (size_t)j.partial * G1J_WORDS, r);
+}
+void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_t* points,
+                    const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_var_mul, GRID(n, 64), 0, s, jobs, n, points, pinf, scalars, partials);
+}
+
+// ------------------------------------------------------------------ reduction
+__global__ __launch_bounds__(64) void k_reduce(const ReduceGroup* __restrict__ groups,
+                                              const uint32_t* __restrict__ out_point, uint32_t n,
+                                              const uint32_t* __restrict__ partials,
+                                              uint32_t* __restrict__ points,
+                                              uint8_t* __restrict__ pinf) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ReduceGroup g = groups[i];
+  G1J acc;
+  pt_set_inf(acc);
+  for (uint32_t k = 0; k < g.count; k++) {
+    G1J p;
+    load_g1j(p, partials + (size_t)(g.begin + k) * G1J_WORDS);
+    pt_add(acc, acc, p);
+  }
+  const uint32_t o = out_point[i];
+  G1A a;
+  if (pt_is_inf(acc)) {
+    pinf[o] = 1;
+    fp_zero(a.x);
+    fp_zero(a.y);
+  } else {
+    pinf[o] = 0;
+    pt_to_affine(a, acc);
+  }
+  store_g1a(points + (size_t)o * G1A_WORDS, a);
+}
+void launch_reduce(hipStream_t s, const ReduceGroup* groups, const uint32_t* out_point, uint32_t n,
+                   const uint32_t* partials, uint32_t* points, uint8_t* pinf) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_reduce, GRID(n, 64), 0, s, groups, out_point, n, partials, points, pinf);
+}
+
+// ------------------------------------------------------------------ pairing check
+// e(a, PK) * e(b, -G2) == 1 (identity arguments contribute 1).
+__global__ __launch_bounds__(64) void k_pairing(const PairJob* __restrict__ jobs, uint32_t n,
+                                               const uint32_t* __restrict__ points,
+                                               const uint8_t* __restrict__ pinf,
+                                               const uint8_t* __restrict__ pk_status,
+                                               const uint32_t* __restrict__ pk_lines,
+                                               const uint32_t* __restrict__ neg_g2_lines,
+                                               uint8_t* __restrict__ status,
+                                               const uint8_t* __restrict__ gate,
+                                               uint8_t* __restrict__ valid) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  PairJob j = jobs[i];
+  const uint8_t pks = pk_status[j.pk_slot];
+  if (pks == 0) status[j.item] = 1;
+  if (status[j.item] != 0 || !gate[j.item]) {
+    valid[j.item] = 0;
+    return;
+  }
+  G1A ps[2];
+  bool skip[2];
+  load_g1a(ps[0], points + (size_t)j.a * G1A_WORDS);
+  load_g1a(ps[1], points + (size_t)j.b * G1A_WORDS);
+  skip[0] = pinf[j.a] || pks == 2;
+  skip[1] = pinf[j.b];
+  const Line* lines[2] = {reinterpret_cast<const Line*>(pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS),
+                          reinterpret_cast<const Line*>(neg_g2_lines)};
+  Fp12 f, r;
+  miller_loop_prepared<2>(f, ps, skip, lines);
+  final_exponentiation(r, f);
+  valid[j.item] = fp12_is_one(r) ? 1 : 0;
+}
+void launch_pairing(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
+                    const uint8_t* pinf, const uint8_t* pk_status, const uint32_t* pk_lines,
+                    const uint32_t* neg_g2_lines, uint8_t* status, const uint8_t* gate,
+                    uint8_t* valid) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_pairing, GRID(n, 64), 0, s, jobs, n, points, pinf, pk_status, pk_lines,
+                     neg_g2_lines, status, gate, valid);
+}
+
+// ------------------------------------------------------------------ Sign
+// core_sign (crates/cubbs-bbs-protocol/src/bbsplus/signature.rs:544-580):
+// e = hash_to_scalar(sk || m_1 .. m_L || domain, api_id || "H2S_"); the term
+// scalars [1, domain, m_1 .. m_L] are rescaled by 1 / (sk + e) so that the
+// comb MSM yields A = B / (sk + e) directly (signature.rs:642-674).
+__global__ __launch_bounds__(64) void k_sign_e(const SignJob* __restrict__ jobs, uint32_t n,
+                                              const uint8_t* __restrict__ blob,
+                                              const uint8_t* __restrict__ pk_status,
+                                              uint32_t* __restrict__ scalars,
+                                              uint8_t* __restrict__ status) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  SignJob j = jobs[i];
+  Fr sk;
+  fr_from_be_bytes_raw(sk, blob + j.sk_off);
+  if (!fr_canonical(sk) || pk_status[j.pk_slot] == 0) {
+    status[j.item] = 1;
+    return;
+  }
+  Sha256 s;
+  sha_init_after_zpad(s);
+  sha_scalar(s, sk);
+  for (uint32_t k = 0; k < j.L; k++) {
+    Fr m;
+    load_fr(m, scalars + (size_t)(j.term + 2 + k) * FR_WORDS);
+    sha_scalar(s, m);
+  }
+  Fr dom;
+  load_fr(dom, scalars + (size_t)(j.term + 1) * FR_WORDS);
+  sha_scalar(s, dom);
+  xmd_b0_suffix(s, 48, DST_H2S, DST_H2S_LEN);
+  uint32_t okm[12];
+  xmd48_from_b0(okm, s.h, DST_H2S, DST_H2S_LEN);
+  Fr e;
+  scalar_from_okm48(e, okm);
+  store_fr(scalars + (size_t)j.e_index * FR_WORDS, e);
+  Fr se;
+  fr_add(se, sk, e);
+  if (fr_is_zero(se)) {
+    status[j.item] = 1;
+    return;
+  }
+  Fr inv;
+  fr_to_mont(se, se);
+  fr_inv(inv, se);  // Montgomery
+  Fr t;
+  fr_from_mont(t, inv);
+  store_fr(scalars + (size_t)j.term * FR_WORDS, t);  // P1 term: 1 / (sk + e)
+  for (uint32_t k = 1; k < j.L + 2; k++) {
+    Fr x;
+    load_fr(x, scalars + (size_t)(j.term + k) * FR_WORDS);
+    fr_mul(x, x, inv);
+    store_fr(scalars + (size_t)(j.term + k) * FR_WORDS, x);
+  }
+}
+void launch_sign_e(hipStream_t s, const SignJob* jobs, uint32_t n, const uint8_t* blob,
+                   const uint8_t* pk_status, uint32_t* scalars, uint8_t* status) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_sign_e, GRID(n, 64), 0, s, jobs, n, blob, pk_status, scalars, status);
+}
+
+__global__ __launch_bounds__(64) void k_sign_finalize(const SignJob* __restrict__ jobs, uint32_t n,
+                                                     const uint32_t* __restrict__ points,
+                                                     const uint8_t* __restrict__ pinf,
+                                                     const uint32_t* __restrict__ scalars,
+                                                     uint8_t* __restrict__ status,
+                                                     uint8_t* __restrict__ sig_out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  SignJob j = jobs[i];
+  if (status[j.item]) return;
+  if (pinf[j.a_point]) {  // A == identity is rejected (signature.rs:664-668)
+    status[j.item] = 1;
+    return;
+  }
+  G1A a;
+  load_g1a(a, points + (size_t)j.a_point * G1A_WORDS);
+  uint8_t c[48];
+  g1_compress_affine(c, a);
+  uint8_t* o = sig_out + (size_t)80 * j.item;
+  for (int k = 0; k < 48; k++) o[k] = c[k];
+  Fr e;
+  load_fr(e, scalars + (size_t)j.e_index * FR_WORDS);
+  fr_to_be_bytes(o + 48, e);
+}
+void launch_sign_finalize(hipStream_t s, const SignJob* jobs, uint32_t n, const uint32_t* points,
+                          const uint8_t* pinf, const uint32_t* scalars, uint8_t* status,
+                          uint8_t* sig_out) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_sign_finalize, GRID(n, 64), 0, s, jobs, n, points, pinf, scalars, status,
+                     sig_out);
+}
+
+// ------------------------------------------------------------------ challenge hashing
+// c_arr = I2OSP(R, 8) || (I2OSP(i, 8) || m_i)* || Abar || Bbar || D || T1 || T2 ||
+//         domain || I2OSP(len(ph), 8) || ph   (proof.rs:2231-2297)
+DEVN void challenge_scalar(Fr& out, uint32_t R, const uint32_t* disc_index,
+                           const uint32_t* disc_scalar, const uint32_t* scalars,
+                           const uint8_t* const pts[5], const Fr& dom, const uint8_t* ph,
+                           uint32_t ph_len) {
+  Sha256 s;
+  sha_init_after_zpad(s);
+  sha_u64_be(s, R);
+  for (uint32_t r = 0; r < R; r++) {
+    sha_u64_be(s, disc_index[r]);
+    Fr m;
+    load_fr(m, scalars + (size_t)disc_scalar[r] * FR_WORDS);
+    sha_scalar(s, m);
+  }
+  for (int k = 0; k < 5; k++) sha_bytes48(s, pts[k]);
+  sha_scalar(s, dom);
+  sha_u64_be(s, ph_len);
+  sha_bytes_global(s, ph, ph_len);
+  xmd_b0_suffix(s, 48, DST_H2S, DST_H2S_LEN);
+  uint32_t okm[12];
+  xmd48_from_b0(okm, s.h, DST_H2S, DST_H2S_LEN);
+  scalar_from_okm48(out, okm);
+}
+
+DEVN void compress_point(uint8_t* out, const uint32_t* points, const uint8_t* pinf, uint32_t idx) {
+  if (pinf[idx]) {
+    out[0] = 0xc0;
+    for (int k = 1; k < 48; k++) out[k] = 0;
+    return;
+  }
+  G1A a;
+  load_g1a(a, points + (size_t)idx * G1A_WORDS);
+  g1_compress_affine(out, a);
+}
+
+// ProofVerify: gate[item] = (challenge recomputed from T1, T2 == proof's c)
+__global__ __launch_bounds__(64) void k_pv_challenge(const PvChallengeJob* __restrict__ jobs,
+                                                    uint32_t n, const uint8_t* __restrict__ blob,
+                                                    const uint32_t* __restrict__ disc_index,
+                                                    const uint32_t* __restrict__ disc_scalar,
+                                                    const uint32_t* __restrict__ points,
+                                                    const uint8_t* __restrict__ pinf,
+                                                    const uint32_t* __restrict__ scalars,
+                                                    uint8_t* __restrict__ gate) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  PvChallengeJob j = jobs[i];
+  uint8_t c[5][48];
+  compress_point(c[0], points, pinf, j.abar);
+  compress_point(c[1], points, pinf, j.bbar);
+  compress_point(c[2], points, pinf, j.d);
+  compress_point(c[3], points, pinf, j.t1);
+  compress_point(c[4], points, pinf, j.t2);
+  const uint8_t* pts[5] = {c[0], c[1], c[2], c[3], c[4]};
+  Fr dom, chal, want;
+  load_fr(dom, scalars + (size_t)j.dom_index * FR_WORDS);
+  challenge_scalar(chal, j.R, disc_index + j.disc_begin, disc_scalar + j.disc_begin, scalars, pts,
+                   dom, blob + j.ph_off, j.ph_len);
+  load_fr(want, scalars + (size_t)j.c_index * FR_WORDS);
+  gate[j.item] = limbs_eq<8>(chal.v, want.v) ? 1 : 0;
+}
+void launch_pv_challenge(hipStream_t s, const PvChallengeJob* jobs, uint32_t n,
+                         const uint8_t* blob, const uint32_t* disc_index,
+                         const uint32_t* disc_scalar, const uint32_t* points, const uint8_t* pinf,
+                         const uint32_t* scalars, uint8_t* gate) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_pv_challenge, GRID(n, 64), 0, s, jobs, n, blob, disc_index, disc_scalar,
+                     points, pinf, scalars, gate);
+}
+
+// ------------------------------------------------------------------ ProofGen
+DEV uint32_t rotl32(uint32_t x, int n) { return (x << n) | (x >> (32 - n)); }
+#define CHACHA_QR(a, b, c, d)                     \
+  a += b; d ^= a; d = rotl32(d, 16);              \
+  c += d; b ^= c; b = rotl32(b, 12);              \
+  a += b; d ^= a; d = rotl32(d, 8);               \
+  c += d; b ^= c; b = rotl32(b, 7);
+
+// RFC 8439 block with a zero nonce; out[16] little-endian keystream words.
+DEVN void chacha20_block_words(uint32_t out[16], const uint32_t key[8], uint32_t counter) {
+  uint32_t st[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u,
+                     key[0], key[1], key[2], key[3], key[4], key[5], key[6], key[7],
+                     counter, 0u, 0u, 0u};
+  uint32_t x[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) x[k] = st[k];
+#pragma unroll 1
+  for (int r = 0; r < 10; r++) {
+    CHACHA_QR(x[0], x[4], x[8], x[12]);
+    CHACHA_QR(x[1], x[5], x[9], x[13]);
+    CHACHA_QR(x[2], x[6], x[10], x[14]);
+    CHACHA_QR(x[3], x[7], x[11], x[15]);
+    CHACHA_QR(x[0], x[5], x[10], x[15]);
+    CHACHA_QR(x[1], x[6], x[11], x[12]);
+    CHACHA_QR(x[2], x[7], x[8], x[13]);
+    CHACHA_QR(x[3], x[4], x[9], x[14]);
+  }
+#pragma unroll
+  for (int k = 0; k < 16; k++) out[k] = x[k] + st[k];
+}
+
+// Blinding scalars from the item's 32-byte seed: scalar k = LE(block k) mod r.
+__global__ __launch_bounds__(64) void k_pg_chacha(const PgJob* __restrict__ jobs, uint32_t n,
+                                                 const uint8_t* __restrict__ seeds,
+                                                 uint32_t* __restrict__ scalars) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  PgJob j = jobs[i];
+  uint32_t key[8];
+  const uint8_t* sd = seeds + (size_t)32 * i;
+  for (int k = 0; k < 8; k++)
+    key[k] = (uint32_t)sd[4 * k] | ((uint32_t)sd[4 * k + 1] << 8) | ((uint32_t)sd[4 * k + 2] << 16) |
+             ((uint32_t)sd[4 * k + 3] << 24);
+  for (uint32_t t = 0; t < 5 + j.U; t++) {
+    uint32_t w[16];
+    chacha20_block_words(w, key, t);
+    Fr r;
+    fr_reduce_wide(r, w);
+    store_fr(scalars + (size_t)(j.rs + t) * FR_WORDS, r);
+  }
+}
+void launch_pg_chacha(hipStream_t s, const PgJob* jobs, uint32_t n, const uint8_t* seeds,
+                      uint32_t* scalars) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_pg_chacha, GRID(n, 64), 0, s, jobs, n, seeds, scalars);
+}
+
+// The seven multipliers of A and B:
+//   k0 = r2 (D), k1 = r1 r2 (Abar), k2 = r1 r2, k3 = -r1 r2 e (Bbar),
+//   k4 = r1 r2 e~, k5 = r2 r1~ (T1), k6 = r2 r3~ (T2 part)
+__global__ __launch_bounds__(64) void k_pg_scalars(const PgJob* __restrict__ jobs, uint32_t n,
+                                                  uint32_t* __restrict__ scalars,
+                                                  uint8_t* __restrict__ status) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  PgJob j = jobs[i];
+  Fr r1, r2, et, r1t, r3t, e;
+  load_fr(r1, scalars + (size_t)(j.rs + 0) * FR_WORDS);
+  load_fr(r2, scalars + (size_t)(j.rs + 1) * FR_WORDS);
+  load_fr(et, scalars + (size_t)(j.rs + 2) * FR_WORDS);
+  load_fr(r1t, scalars + (size_t)(j.rs + 3) * FR_WORDS);
+  load_fr(r3t, scalars + (size_t)(j.rs + 4) * FR_WORDS);
+  load_fr(e, scalars + (size_t)j.e_index * FR_WORDS);
+  Fr r2m, r1r2, r1r2m, k3, k4, k5, k6, t;
+  fr_to_mont(r2m, r2);
+  fr_mul(r1r2, r1, r2m);        // canonical r1 r2
+  fr_to_mont(r1r2m, r1r2);
+  fr_mul(t, e, r1r2m);          // r1 r2 e
+  fr_neg(k3, t);
+  fr_mul(k4, et, r1r2m);
+  fr_mul(k5, r1t, r2m);
+  fr_mul(k6, r3t, r2m);
+  uint32_t* o = scalars + (size_t)j.k_index * FR_WORDS;
+  store_fr(o + 0 * FR_WORDS, r2);
+  store_fr(o + 1 * FR_WORDS, r1r2);
+  store_fr(o + 2 * FR_WORDS, r1r2);
+  store_fr(o + 3 * FR_WORDS, k3);
+  store_fr(o + 4 * FR_WORDS, k4);
+  store_fr(o + 5 * FR_WORDS, k5);
+  store_fr(o + 6 * FR_WORDS, k6);
+  if (fr_is_zero(r2)) status[j.item] = 1;  // r2 must be invertible (proof.rs:2336-2337)
+}
+void launch_pg_scalars(hipStream_t s, const PgJob* jobs, uint32_t n, uint32_t* scalars,
+                       uint8_t* status) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_pg_scalars, GRID(n, 64), 0, s, jobs, n, scalars, status);
+}
+
+// Challenge and proof_finalize_parts (proof.rs:2317-2377); writes
+// Abar || Bbar || D || e^ || r1^ || r3^ || m^_1..m^_U || c.
+__global__ __launch_bounds__(64) void k_pg_finalize(const PgJob* __restrict__ jobs, uint32_t n,
+                                                   const uint8_t* __restrict__ blob,
+                                                   const uint32_t* __restrict__ disc_index,
+                                                   const uint32_t* __restrict__ disc_scalar,
+                                                   const uint32_t* __restrict__ und_scalar,
+                                                   const uint32_t* __restrict__ points,
+                                                   const uint8_t* __restrict__ pinf,
+                                                   uint32_t* __restrict__ scalars,
+                                                   uint8_t* __restrict__ status,
+                                                   uint8_t* __restrict__ proof_out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  PgJob j = jobs[i];
+  if (status[j.item]) return;
+  uint8_t c[5][48];
+  compress_point(c[0], points, pinf, j.abar);
+  compress_point(c[1], points, pinf, j.bbar);
+  compress_point(c[2], points, pinf, j.d);
+  compress_point(c[3], points, pinf, j.t1);
+  compress_point(c[4], points, pinf, j.t2);
+  const uint8_t* pts[5] = {c[0], c[1], c[2], c[3], c[4]};
+  Fr dom, ch;
+  load_fr(dom, scalars + (size_t)j.dom_index * FR_WORDS);
+  challenge_scalar(ch, j.R, disc_index + j.disc_begin, disc_scalar + j.disc_begin, scalars, pts,
+                   dom, blob + j.ph_off, j.ph_len);
+  Fr chm, r1, r2, et, r1t, r3t, e, t, x;
+  fr_to_mont(chm, ch);
+  load_fr(r1, scalars + (size_t)(j.rs + 0) * FR_WORDS);
+  load_fr(r2, scalars + (size_t)(j.rs + 1) * FR_WORDS);
+  load_fr(et, scalars + (size_t)(j.rs + 2) * FR_WORDS);
+  load_fr(r1t, scalars + (size_t)(j.rs + 3) * FR_WORDS);
+  load_fr(r3t, scalars + (size_t)(j.rs + 4) * FR_WORDS);
+  load_fr(e, scalars + (size_t)j.e_index * FR_WORDS);
+  uint8_t* o = proof_out + j.proof_off;
+  for (int k = 0; k < 48; k++) {
+    o[k] = c[0][k];
+    o[48 + k] = c[1][k];
+    o[96 + k] = c[2][k];
+  }
+  // e^ = e~ + e c
+  fr_mul(t, e, chm);
+  fr_add(x, et, t);
+  fr_to_be_bytes(o + 144, x);
+  // r1^ = r1~ - r1 c
+  fr_mul(t, r1, chm);
+  fr_sub(x, r1t, t);
+  fr_to_be_bytes(o + 176, x);
+  // r3^ = r3~ - r2^-1 c
+  Fr r2m, r3;
+  fr_to_mont(r2m, r2);
+  fr_inv(r3, r2m);         // Montgomery r2^-1
+  fr_mul(t, chm, r3);      // Montgomery (c r2^-1) -> from_mont
+  fr_from_mont(t, t);
+  fr_sub(x, r3t, t);
+  fr_to_be_bytes(o + 208, x);
+  // m^_j = m~_j + m_j c
+  for (uint32_t u = 0; u < j.U; u++) {
+    Fr mt, m;
+    load_fr(mt, scalars + (size_t)(j.rs + 5 + u) * FR_WORDS);
+    load_fr(m, scalars + (size_t)und_scalar[j.und_begin + u] * FR_WORDS);
+    fr_mul(t, m, chm);
+    fr_add(x, mt, t);
+    fr_to_be_bytes(o + 240 + 32 * u, x);
+  }
+  fr_to_be_bytes(o + 240 + 32 * j.U, ch);
+}
+void launch_pg_finalize(hipStream_t s, const PgJob* jobs, uint32_t n, const uint8_t* blob,
+                        const uint32_t* disc_index, const uint32_t* disc_scalar,
+                        const uint32_t* und_scalar, const uint32_t* points, const uint8_t* pinf,
+                        uint32_t* scalars, uint8_t* status, uint8_t* proof_out) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_pg_finalize, GRID(n, 64), 0, s, jobs, n, blob, disc_index, disc_scalar,
+                     und_scalar, points, pinf, scalars, status, proof_out);
+}
+
+// ------------------------------------------------------------------ keys
+// PK = sk * G2 (crates/cubbs-bbs-protocol/src/bbsplus/keys.rs:256-260)
+__global__ __launch_bounds__(64) void k_g2_mul_gen(const uint8_t* sk, uint32_t n, uint8_t* pk_out,
+                                                  uint8_t* ok) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Fr k;
+  fr_from_be_bytes_raw(k, sk + (size_t)32 * i);
+  if (!fr_canonical(k)) {
+    ok[i] = 0;
+    return;
+  }
+  G2A g;
+  g2_load_affine_const(g, bls::G2_GEN);
+  G2J r;
+  pt_mul_limbs_affine<Fp2, 8>(r, g, k.v);
+  uint8_t c[96];
+  g2_compress(c, r);
+  for (int q = 0; q < 96; q++) pk_out[(size_t)96 * i + q] = c[q];
+  ok[i] = 1;
+}
+void launch_g2_mul_gen(hipStream_t s, const uint8_t* sk, uint32_t n, uint8_t* pk_out, uint8_t* ok) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_g2_mul_gen, GRID(n, 64), 0, s, sk, n, pk_out, ok);
+}
+
+}  // namespace cubbs

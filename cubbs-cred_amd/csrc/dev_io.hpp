@@ -1,0 +1,55 @@
+// Load/store helpers between flat device arrays (32-bit words) and the
+// field / point types.
+#pragma once
+#include "pairing.hpp"
+#include "sha.hpp"
+
+DEV void load_fr(Fr& r, const uint32_t* p) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) r.v[k] = p[k];
+}
+DEV void store_fr(uint32_t* p, const Fr& r) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) p[k] = r.v[k];
+}
+DEV void load_g1a(G1A& r, const uint32_t* p) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    r.x.v[k] = p[k];
+    r.y.v[k] = p[12 + k];
+  }
+}
+DEV void store_g1a(uint32_t* p, const G1A& r) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    p[k] = r.x.v[k];
+    p[12 + k] = r.y.v[k];
+  }
+}
+DEV void load_g1j(G1J& r, const uint32_t* p) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    r.x.v[k] = p[k];
+    r.y.v[k] = p[12 + k];
+    r.z.v[k] = p[24 + k];
+  }
+}
+DEV void store_g1j(uint32_t* p, const G1J& r) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    p[k] = r.x.v[k];
+    p[12 + k] = r.y.v[k];
+    p[24 + k] = r.z.v[k];
+  }
+}
+// 32-byte big-endian scalar as 8 SHA words (most significant first)
+DEV void sha_scalar(Sha256& s, const Fr& canonical) {
+#pragma unroll
+  for (int k = 7; k >= 0; k--) sha_word(s, canonical.v[k]);
+}
+DEV void sha_bytes48(Sha256& s, const uint8_t* c) {
+#pragma unroll
+  for (int k = 0; k < 12; k++)
+    sha_word(s, ((uint32_t)c[4 * k] << 24) | ((uint32_t)c[4 * k + 1] << 16) |
+                    ((uint32_t)c[4 * k + 2] << 8) | c[4 * k + 3]);
+}

@@ -1,0 +1,54 @@
+// Device-resident state shared by every entry point of the library:
+// generator comb tables, prepared issuer public keys and -G2 lines.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bbs_internal.h"
+
+namespace cubbs {
+
+struct GpuError {
+  hipError_t err;
+};
+inline void check(hipError_t e) {
+  if (e != hipSuccess) throw GpuError{e};
+}
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Context {
+  std::mutex mu;
+  bool ready = false;
+  bool failed = false;
+  hipStream_t setup = nullptr;
+  // generators: slot 0 = P1, slot 1 = Q1, slot 1 + j = H_j
+  uint32_t gen_slots = 0, gen_cap = 0;
+  uint32_t* gen_affine = nullptr;
+  uint32_t* gen_tables = nullptr;
+  uint8_t* gen_comp = nullptr;
+  uint32_t* gen_vstate = nullptr;
+  // prepared public keys (status: 0 invalid, 1 valid, 2 identity)
+  std::unordered_map<std::string, uint32_t> pk_slots;
+  uint32_t pk_count = 0, pk_cap = 0;
+  uint8_t* pk_bytes = nullptr;
+  uint8_t* pk_status = nullptr;
+  uint32_t* pk_lines = nullptr;
+  uint32_t* neg_g2_lines = nullptr;
+  // buffers replaced by a growth step stay allocated: batches that snapshot
+  // them may still be in flight on other streams
+  std::vector<void*> retired;
+};
+
+Context& ctx();
+// The *_locked functions require ctx().mu to be held.
+void init_locked(Context& c);
+void ensure_generators_locked(Context& c, uint32_t slots);
+void lookup_pks_locked(Context& c, const std::vector<const uint8_t*>& pks,
+                       std::vector<uint32_t>& slots_out);
+
+}  // namespace cubbs

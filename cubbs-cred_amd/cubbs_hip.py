@@ -46,7 +46,11 @@ EXT_SYMBOLS = [
     "cubbs_hip_test_pairing_gt",
     "cubbs_hip_test_g2_roundtrip",
     "cubbs_hip_generators",
+    "cubbs_hip_sk_to_pk",
     "cubbs_hip_fp_mul_bench",
+    "cubbs_hip_mad_peak_bench",
+    "cubbs_hip_proof_gen_seeded_batch",
+    "cubbs_hip_proof_gen_with_scalars_batch",
 ]
 
 
@@ -182,6 +186,111 @@ def verify_batch(reqs):
     return rc, [(res[i].code, bool(res[i].value)) for i in range(len(reqs))]
 
 
+def build_proof_verify_items(reqs, keep):
+    """reqs: list of (pk, proof, header, ph, disclosed_msgs, disclosed_idx, total)."""
+    items = (ProofVerifyItem * max(1, len(reqs)))()
+    for i, (pk, proof, hdr, ph, dmsgs, didx, total) in enumerate(reqs):
+        items[i].public_key = keep.slice(pk)
+        items[i].proof = keep.slice(proof)
+        items[i].header = keep.slice(hdr)
+        items[i].presentation_header = keep.slice(ph)
+        items[i].disclosed_messages = keep.slices(dmsgs)
+        items[i].disclosed_count = len(dmsgs)
+        items[i].disclosed_indexes = keep.indexes(didx)
+        items[i].total_message_count = total
+    keep.refs.append(items)
+    return items
+
+
+def proof_verify_batch(reqs):
+    """Returns (status, [(code, valid)])."""
+    keep = _Keep()
+    items = build_proof_verify_items(reqs, keep)
+    res = (Result * max(1, len(reqs)))()
+    rc = lib().cubbs_bbs_gpu_proof_verify_batch_v2(ABI_VERSION, items, ctypes.c_size_t(len(reqs)), res)
+    return rc, [(res[i].code, bool(res[i].value)) for i in range(len(reqs))]
+
+
+def sign_batch(reqs):
+    """reqs: list of (sk, pk, header, [messages]) -> (status, [(code, sig or None)])."""
+    keep = _Keep()
+    items = (SignItem * max(1, len(reqs)))()
+    outs = []
+    for i, (sk, pk, hdr, msgs) in enumerate(reqs):
+        items[i].secret_key = keep.slice(sk)
+        items[i].public_key = keep.slice(pk)
+        items[i].header = keep.slice(hdr)
+        items[i].messages = keep.slices(msgs)
+        items[i].message_count = len(msgs)
+        buf, bb = keep.out(80)
+        items[i].signature_out = bb
+        outs.append(buf)
+    res = (Result * max(1, len(reqs)))()
+    rc = lib().cubbs_bbs_gpu_sign_batch_v2(ABI_VERSION, items, ctypes.c_size_t(len(reqs)), res)
+    return rc, [(res[i].code, outs[i].raw[:items[i].signature_out.len] if res[i].code == 0 else None)
+                for i in range(len(reqs))]
+
+
+def proof_gen_batch(reqs, seeds=None, scalars=None):
+    """reqs: list of (pk, sig, header, ph, [messages], disclosed_idx).
+
+    seeds: list of 32-byte seeds (ChaCha20 blinding) or None; scalars: list of
+    lists of 32-byte blinding scalars (explicit) or None (OS randomness).
+    Returns (status, [(code, proof or None)])."""
+    keep = _Keep()
+    items = (ProofGenItem * max(1, len(reqs)))()
+    outs = []
+    for i, (pk, sig, hdr, ph, msgs, didx) in enumerate(reqs):
+        items[i].public_key = keep.slice(pk)
+        items[i].signature = keep.slice(sig)
+        items[i].header = keep.slice(hdr)
+        items[i].presentation_header = keep.slice(ph)
+        items[i].messages = keep.slices(msgs)
+        items[i].message_count = len(msgs)
+        items[i].disclosed_indexes = keep.indexes(didx)
+        cap = 272 + 32 * max(0, len(msgs) - len(didx))
+        buf, bb = keep.out(cap)
+        items[i].proof_out = bb
+        outs.append(buf)
+    res = (Result * max(1, len(reqs)))()
+    n = ctypes.c_size_t(len(reqs))
+    if scalars is not None:
+        flat = b"".join(b"".join(x) for x in scalars)
+        offs, cur = [], 0
+        for x in scalars:
+            offs.append(cur)
+            cur += len(x)
+        offarr = (ctypes.c_size_t * max(1, len(offs)))(*offs)
+        sbuf = ctypes.create_string_buffer(flat, max(1, len(flat)))
+        rc = lib().cubbs_hip_proof_gen_with_scalars_batch(ABI_VERSION, items, n, sbuf, offarr, res)
+    elif seeds is not None:
+        sbuf = ctypes.create_string_buffer(b"".join(seeds), max(1, 32 * len(seeds)))
+        rc = lib().cubbs_hip_proof_gen_seeded_batch(ABI_VERSION, items, n, sbuf, res)
+    else:
+        rc = lib().cubbs_bbs_gpu_proof_gen_batch_v2(ABI_VERSION, items, n, res)
+    return rc, [(res[i].code, outs[i].raw[:items[i].proof_out.len] if res[i].code == 0 else None)
+                for i in range(len(reqs))]
+
+
+def sk_to_pk(sks):
+    n = len(sks)
+    out = ctypes.create_string_buffer(96 * max(1, n))
+    ok = ctypes.create_string_buffer(max(1, n))
+    rc = lib().cubbs_hip_sk_to_pk(b"".join(sks), ctypes.c_size_t(n), out, ok)
+    return rc, [(out.raw[96 * i:96 * i + 96] if ok.raw[i] else None) for i in range(n)]
+
+
+def keygen(ikms, info=b""):
+    """SK = hash_to_scalar(IKM || I2OSP(len(info), 2) || info, api_id || "KEYGEN_DST_"),
+    PK = SK * G2 (crates/cubbs-bbs-protocol/src/bbsplus/keys.rs:214-260), on the device."""
+    inputs = [ikm + len(info).to_bytes(2, "big") + info for ikm in ikms]
+    rc, sks = hash_to_scalar_batch(inputs, API_ID + b"KEYGEN_DST_")
+    if rc:
+        return rc, []
+    rc, pks = sk_to_pk(sks)
+    return rc, list(zip(sks, pks))
+
+
 def messages_to_scalars(msgs, dst=MAP_DST):
     keep = _Keep()
     items = (MessagesToScalarItem * max(1, len(msgs)))()
@@ -269,6 +378,13 @@ def fp_mul_bench(lanes, iters):
     return rc, ms.value
 
 
+def mad_peak_bench(lanes, iters):
+    """Elapsed ms of lanes * iters * 64 independent v_mad_u64_u32."""
+    ms = ctypes.c_double()
+    rc = lib().cubbs_hip_mad_peak_bench(ctypes.c_uint32(lanes), ctypes.c_uint32(iters), ctypes.byref(ms))
+    return rc, ms.value
+
+
 class StagedBatch:
     """A batch whose inputs stay resident on the device (prepare once, launch many)."""
 
@@ -276,6 +392,8 @@ class StagedBatch:
         self.keep = _Keep()
         if kind == BATCH_VERIFY:
             items = build_verify_items(reqs, self.keep)
+        elif kind == BATCH_PROOF_VERIFY:
+            items = build_proof_verify_items(reqs, self.keep)
         else:
             raise ValueError("unsupported batch kind")
         st = ctypes.c_int32(0)

@@ -72,9 +72,15 @@ int32_t cubbs_hip_test_pairing_gt(const uint8_t *g1_48, const uint8_t *g2_96, si
 int32_t cubbs_hip_test_g2_roundtrip(const uint8_t *g2_96, size_t n, uint8_t *out96, uint8_t *ok);
 /* compressed generators [P1, Q1, H_1 .. H_{count-2}] as built on the device */
 int32_t cubbs_hip_generators(size_t count, uint8_t *out48);
+/* PK = sk * G2 for n 32-byte big-endian secret keys (keys.rs:256-260 of the
+ * reference protocol crate); ok[i] = 0 for a non-canonical key. */
+int32_t cubbs_hip_sk_to_pk(const uint8_t *sk32, size_t n, uint8_t *pk96, uint8_t *ok);
 /* Fp multiplication throughput probe: lanes x iters x 2 dependent chains;
  * returns elapsed milliseconds of the kernel (HIP events) in *ms. */
 int32_t cubbs_hip_fp_mul_bench(uint32_t lanes, uint32_t iters, double *ms);
+/* Peak v_mad_u64_u32 rate probe: lanes x iters x 64 independent multiply-adds;
+ * elapsed kernel milliseconds in *ms. */
+int32_t cubbs_hip_mad_peak_bench(uint32_t lanes, uint32_t iters, double *ms);
 
 #ifdef __cplusplus
 }

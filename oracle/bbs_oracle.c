@@ -1307,6 +1307,25 @@ int orc_pairing_gt(const uint8_t p[48], const uint8_t q[96], int naive, uint8_t 
   return ORC_OK;
 }
 
+/* Self-check of the final exponentiation: the x-chain result must equal the
+ * cube of the plain (p^12 - 1)/r power (3 (p^4 - p^2 + 1)/r =
+ * (x-1)^2 (x+p) (x^2+p^2-1) + 3).  Returns 1 when it holds. */
+int orc_check_final_exp(const uint8_t p[48], const uint8_t q[96]) {
+  orc_init();
+  g1j a;
+  g2j b;
+  if (!g1_decompress(&a, p) || !g2_decompress(&b, q)) return ORC_ERR_INPUT;
+  g2_prepared prep;
+  g2_prepare(&prep, &b);
+  fp12 f, fast, naive, cube;
+  miller_multi(&f, &a, &prep, 1);
+  final_exp(&fast, &f);
+  fp12_pow_limbs(&naive, &f, FE_NAIVE_EXP, FE_NAIVE_N);
+  fp12_sqr(&cube, &naive);
+  fp12_mul(&cube, &cube, &naive);
+  return fp12_eq(&fast, &cube) && !fp12_is_one(&fast);
+}
+
 int orc_fr_mul(const uint8_t a[32], const uint8_t b[32], uint8_t out[32]) {
   orc_init();
   fr x, y;

@@ -65,6 +65,8 @@ int orc_pairing_eq(const uint8_t a1[48], const uint8_t a2[96], const uint8_t b1[
  * each 48-byte big-endian.  `naive` selects the plain (p^12-1)/r exponent. */
 int orc_pairing_gt(const uint8_t p[48], const uint8_t q[96], int naive, uint8_t out[576]);
 
+int orc_check_final_exp(const uint8_t p[48], const uint8_t q[96]);
+
 /* scalar field helpers (canonical check, arithmetic) */
 int orc_fr_mul(const uint8_t a[32], const uint8_t b[32], uint8_t out[32]);
 int orc_fr_add(const uint8_t a[32], const uint8_t b[32], uint8_t out[32]);

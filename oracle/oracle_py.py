@@ -258,3 +258,7 @@ def proof_verify(pk, proof, header, ph, disclosed_msgs, disclosed, total):
                                 data, lens, ctypes.c_size_t(len(disclosed_msgs)), _sizes(disclosed),
                                 ctypes.c_size_t(len(disclosed)), ctypes.c_size_t(total))
     return None if rc < 0 else bool(rc)
+
+
+def check_final_exp(p, q):
+    return lib().orc_check_final_exp(_buf(p), _buf(q)) == 1
