@@ -4,6 +4,7 @@
 #include "bbs_internal.h"
 #include "dev_io.hpp"
 #include "h2c.hpp"
+#include "pairing6.hpp"
 
 namespace cubbs {
 
@@ -374,36 +375,53 @@ void launch_test_g1_mul(hipStream_t s, const uint8_t* pts, const uint8_t* scalar
   hipLaunchKernelGGL(k_test_g1_mul, GRID(n, 64), 0, s, pts, scalars, n, out, ok);
 }
 
+// e(P, Q) as a GT value through the same 6-lane Miller loop / final
+// exponentiation as the pairing checks (one 8-lane group per pair).
 __global__ __launch_bounds__(64) void k_test_pairing_gt(const uint8_t* g1, const uint8_t* g2,
                                                        uint32_t n, uint8_t* out, uint8_t* ok,
                                                        uint32_t* lines_scratch) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  G1J p;
-  G2J q;
-  bool good = g1_decompress(p, g1 + (size_t)48 * i) && g2_decompress(q, g2 + (size_t)96 * i);
-  good = good && !pt_is_inf(p) && !pt_is_inf(q);
-  ok[i] = good;
-  if (!good) return;
-  G2A qa;
-  pt_to_affine(qa, q);
-  Line* lines = reinterpret_cast<Line*>(lines_scratch + (size_t)i * PK_LINES_WORDS);
-  g2_prepare_lines(lines, qa);
+  __shared__ uint32_t lds_all[8 * G6_LDS_WORDS];
+  const int k = threadIdx.x & 7;
+  const int grp = threadIdx.x >> 3;
+  uint32_t* lds = lds_all + grp * G6_LDS_WORDS;
+  const uint32_t i = blockIdx.x * 8 + grp;
+  bool good = false;
   G1A pa;
-  pa.x = p.x;
-  pa.y = p.y;
-  bool skip = false;
-  const Line* lp[1] = {lines};
-  Fp12 f, r;
-  miller_loop_prepared<1>(f, &pa, &skip, lp);
-  final_exponentiation(r, f);
-  const Fp2* cs[6] = {&r.c0.c0, &r.c0.c1, &r.c0.c2, &r.c1.c0, &r.c1.c1, &r.c1.c2};
-  for (int k = 0; k < 6; k++) {
+  fp_zero(pa.x);
+  fp_zero(pa.y);
+  Line* lines = reinterpret_cast<Line*>(lines_scratch + (size_t)(i < n ? i : 0) * PK_LINES_WORDS);
+  if (i < n) {
+    G1J p;
+    G2J q;
+    good = g1_decompress(p, g1 + (size_t)48 * i) && g2_decompress(q, g2 + (size_t)96 * i);
+    good = good && !pt_is_inf(p) && !pt_is_inf(q);
+    if (good) {
+      pa.x = p.x;
+      pa.y = p.y;
+      if (k == 0) {
+        G2A qa;
+        pt_to_affine(qa, q);
+        g2_prepare_lines(lines, qa);
+      }
+    }
+  }
+  __syncthreads();
+  __threadfence_block();
+  G1A ps[2] = {pa, pa};
+  bool skip[2] = {!good, true};
+  const Line* lp[2] = {lines, lines};
+  Fp2 f, r;
+  g6_miller2(f, k, lds, ps, skip, lp);
+  g6_final_exp(r, f, k, lds);
+  if (i < n && k < 6) {
+    if (k == 0) ok[i] = good;
+    // slot k is tower coordinate c_{k mod 2}.c_{k div 2}; output order c0.c0, c0.c1, c0.c2, c1.c0, ..
+    const int pos = (k & 1) * 3 + (k >> 1);
     Fp c;
-    fp_from_mont(c, cs[k]->c0);
-    fp_to_be_bytes(out + (size_t)576 * i + 96 * k, c);
-    fp_from_mont(c, cs[k]->c1);
-    fp_to_be_bytes(out + (size_t)576 * i + 96 * k + 48, c);
+    fp_from_mont(c, r.c0);
+    fp_to_be_bytes(out + (size_t)576 * i + 96 * pos, c);
+    fp_from_mont(c, r.c1);
+    fp_to_be_bytes(out + (size_t)576 * i + 96 * pos + 48, c);
   }
 }
 void launch_test_pairing_gt(hipStream_t s, const uint8_t* g1, const uint8_t* g2, uint32_t n,
@@ -411,7 +429,8 @@ void launch_test_pairing_gt(hipStream_t s, const uint8_t* g1, const uint8_t* g2,
   if (!n) return;
   uint32_t* scratch = nullptr;
   if (hipMalloc(&scratch, (size_t)n * PK_LINES_WORDS * 4) != hipSuccess) return;
-  hipLaunchKernelGGL(k_test_pairing_gt, GRID(n, 64), 0, s, g1, g2, n, out, ok, scratch);
+  hipLaunchKernelGGL(k_test_pairing_gt, dim3((n + 7) / 8), dim3(64), 0, s, g1, g2, n, out, ok,
+                     scratch);
   (void)hipStreamSynchronize(s);
   (void)hipFree(scratch);
 }

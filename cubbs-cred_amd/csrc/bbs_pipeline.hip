@@ -12,6 +12,7 @@
 //   gate[]    : per item, extra condition of the pairing check (challenge)
 #include "bbs_internal.h"
 #include "dev_io.hpp"
+#include "pairing6.hpp"
 
 namespace cubbs {
 
@@ -167,7 +168,8 @@ void launch_reduce(hipStream_t s, const ReduceGroup* groups, const uint32_t* out
 }
 
 // ------------------------------------------------------------------ pairing check
-// e(a, PK) * e(b, -G2) == 1 (identity arguments contribute 1).
+// e(a, PK) * e(b, -G2) == 1 (identity arguments contribute 1), one check per
+// 8-lane group (6 active lanes, see pairing6.hpp), 8 checks per 64-thread block.
 __global__ __launch_bounds__(64) void k_pairing(const PairJob* __restrict__ jobs, uint32_t n,
                                                const uint32_t* __restrict__ points,
                                                const uint8_t* __restrict__ pinf,
@@ -177,35 +179,57 @@ __global__ __launch_bounds__(64) void k_pairing(const PairJob* __restrict__ jobs
                                                uint8_t* __restrict__ status,
                                                const uint8_t* __restrict__ gate,
                                                uint8_t* __restrict__ valid) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  PairJob j = jobs[i];
-  const uint8_t pks = pk_status[j.pk_slot];
-  if (pks == 0) status[j.item] = 1;
-  if (status[j.item] != 0 || !gate[j.item]) {
-    valid[j.item] = 0;
+  __shared__ uint32_t lds_all[8 * G6_LDS_WORDS];
+  const int k = threadIdx.x & 7;
+  const int grp = threadIdx.x >> 3;
+  uint32_t* lds = lds_all + grp * G6_LDS_WORDS;
+  const uint32_t job = blockIdx.x * 8 + grp;
+  bool live = job < n;
+  PairJob j{};
+  uint8_t pks = 1;
+  if (live) {
+    j = jobs[job];
+    pks = pk_status[j.pk_slot];
+    live = status[j.item] == 0 && pks != 0 && gate[j.item];
+  }
+  // blocks with no live check leave together (uniform exit, no barrier skipped)
+  if (!__syncthreads_or(live)) {
+    if (job < n && k == 0) {
+      if (pks == 0) status[j.item] = 1;
+      valid[j.item] = 0;
+    }
     return;
   }
   G1A ps[2];
-  bool skip[2];
-  load_g1a(ps[0], points + (size_t)j.a * G1A_WORDS);
-  load_g1a(ps[1], points + (size_t)j.b * G1A_WORDS);
-  skip[0] = pinf[j.a] || pks == 2;
-  skip[1] = pinf[j.b];
-  const Line* lines[2] = {reinterpret_cast<const Line*>(pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS),
+  bool skip[2] = {true, true};
+  const Line* lines[2] = {reinterpret_cast<const Line*>(neg_g2_lines),
                           reinterpret_cast<const Line*>(neg_g2_lines)};
-  Fp12 f, r;
-  miller_loop_prepared<2>(f, ps, skip, lines);
-  final_exponentiation(r, f);
-  valid[j.item] = fp12_is_one(r) ? 1 : 0;
+  fp_zero(ps[0].x);
+  fp_zero(ps[0].y);
+  ps[1] = ps[0];
+  if (live) {
+    load_g1a(ps[0], points + (size_t)j.a * G1A_WORDS);
+    load_g1a(ps[1], points + (size_t)j.b * G1A_WORDS);
+    skip[0] = pinf[j.a] || pks == 2;
+    skip[1] = pinf[j.b];
+    lines[0] = reinterpret_cast<const Line*>(pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS);
+  }
+  Fp2 f, r;
+  g6_miller2(f, k, lds, ps, skip, lines);
+  g6_final_exp(r, f, k, lds);
+  const bool one = g6_is_one(r, k, lds);
+  if (job < n && k == 0) {
+    if (pks == 0) status[j.item] = 1;
+    valid[j.item] = (live && one) ? 1 : 0;
+  }
 }
 void launch_pairing(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
                     const uint8_t* pinf, const uint8_t* pk_status, const uint32_t* pk_lines,
                     const uint32_t* neg_g2_lines, uint8_t* status, const uint8_t* gate,
                     uint8_t* valid) {
   if (!n) return;
-  hipLaunchKernelGGL(k_pairing, GRID(n, 64), 0, s, jobs, n, points, pinf, pk_status, pk_lines,
-                     neg_g2_lines, status, gate, valid);
+  hipLaunchKernelGGL(k_pairing, dim3((n + 7) / 8), dim3(64), 0, s, jobs, n, points, pinf,
+                     pk_status, pk_lines, neg_g2_lines, status, gate, valid);
 }
 
 // ------------------------------------------------------------------ Sign
