@@ -115,7 +115,7 @@ void ensure_generators_locked(Context& c, uint32_t slots) {
   // build tables in slices to bound the scratch buffer
   const uint32_t slice = 64;
   uint32_t* scratch = nullptr;
-  check(hipMalloc(&scratch, (size_t)slice * COMB_WINDOWS * COMB_ENTRIES * 24 * 4));
+  check(hipMalloc(&scratch, (size_t)slice * COMB_WINDOWS * COMB_ENTRIES * COMB_ENTRY_WORDS * 4));
   for (uint32_t s = 0; s < tcount; s += slice) {
     uint32_t k = std::min(slice, tcount - s);
     launch_comb_build(c.setup, first_table + s, k, c.gen_affine, c.gen_tables, scratch);
@@ -636,10 +636,11 @@ static int32_t time_probe(uint32_t lanes, uint32_t iters, double* ms,
     hipStream_t s;
     check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     uint32_t* d = nullptr;
-    check(hipMalloc(&d, (size_t)lanes * 24 * 4));
-    // operands below p: small non-zero limbs
-    std::vector<uint32_t> h((size_t)lanes * 24);
-    for (size_t i = 0; i < h.size(); i++) h[i] = (uint32_t)(i * 2654435761u) & 0x0fffffffu;
+    // two Fp operands per lane (2 x 14 limbs of 28 bits), values below 2^380 < p
+    check(hipMalloc(&d, (size_t)lanes * 28 * 4));
+    std::vector<uint32_t> h((size_t)lanes * 28);
+    for (size_t i = 0; i < h.size(); i++)
+      h[i] = (uint32_t)(i * 2654435761u) & (i % 14 == 13 ? 0xffffu : 0x0fffffffu);
     check(hipMemcpyAsync(d, h.data(), h.size() * 4, hipMemcpyHostToDevice, s));
     launch(s, d, lanes, 1);  // warm-up
     hipEvent_t e0, e1;

@@ -18,15 +18,15 @@ constexpr uint32_t GEN_H0 = 2;
 // affine table entries d * 2^(8w) * G for d = 1..128.
 constexpr int COMB_WINDOWS = 32;
 constexpr int COMB_ENTRIES = 128;
-constexpr size_t COMB_ENTRY_WORDS = 24;  // affine x, y (Montgomery), 96 bytes
+constexpr size_t COMB_ENTRY_WORDS = 28;  // affine x, y: 2 x 14 limbs of 28 bits (Montgomery), 112 bytes
 constexpr size_t COMB_TABLE_WORDS = (size_t)COMB_WINDOWS * COMB_ENTRIES * COMB_ENTRY_WORDS;
 
-constexpr int MILLER_LINE_WORDS = 48;   // (lambda, c) in Fp2: 192 bytes
+constexpr int MILLER_LINE_WORDS = 56;   // (lambda, c) in Fp2: 4 x 14 limbs, 224 bytes
 constexpr int MILLER_LINE_COUNT = 68;
 constexpr size_t PK_LINES_WORDS = (size_t)MILLER_LINE_WORDS * MILLER_LINE_COUNT;
 
-constexpr int G1J_WORDS = 36;  // Jacobian G1 point
-constexpr int G1A_WORDS = 24;  // affine G1 point
+constexpr int G1J_WORDS = 42;  // Jacobian G1 point, 3 x 14 limbs
+constexpr int G1A_WORDS = 28;  // affine G1 point, 2 x 14 limbs
 constexpr int FR_WORDS = 8;
 
 // hash-to-scalar job: message bytes -> scalar written to out[out_index]
@@ -154,7 +154,7 @@ void launch_gen_seeds(hipStream_t s, uint32_t first, uint32_t count, uint32_t* v
                       uint32_t* seeds_out);
 void launch_gen_points(hipStream_t s, uint32_t first, uint32_t count, const uint32_t* seeds,
                        uint32_t* gen_affine, uint8_t* gen_comp);
-// scratch: count * COMB_WINDOWS * COMB_ENTRIES * 24 words
+// scratch: count * COMB_WINDOWS * COMB_ENTRIES * COMB_ENTRY_WORDS words
 void launch_comb_build(hipStream_t s, uint32_t first, uint32_t count, const uint32_t* gen_affine,
                        uint32_t* tables, uint32_t* scratch);
 void launch_comb_msm(hipStream_t s, const CombChunk* chunks, uint32_t n, const uint32_t* term_gen,

@@ -192,7 +192,7 @@ __global__ __launch_bounds__(64) void k_comb_build(uint32_t first, uint32_t coun
   G1A baseA;
   pt_to_affine(baseA, base);
   uint32_t* out = tables + ((size_t)g * COMB_WINDOWS + w) * COMB_ENTRIES * COMB_ENTRY_WORDS;
-  uint32_t* sc = scratch + (size_t)t * COMB_ENTRIES * 24;
+  uint32_t* sc = scratch + (size_t)t * COMB_ENTRIES * COMB_ENTRY_WORDS;
   G1J acc;
   pt_from_affine(acc, baseA);
   Fp prod;
@@ -200,12 +200,12 @@ __global__ __launch_bounds__(64) void k_comb_build(uint32_t first, uint32_t coun
   for (int d = 1; d <= COMB_ENTRIES; d++) {
     fp_mul(prod, prod, acc.z);
     uint32_t* e = out + (size_t)(d - 1) * COMB_ENTRY_WORDS;
-    uint32_t* z = sc + (size_t)(d - 1) * 24;
-    for (int k = 0; k < 12; k++) {
+    uint32_t* z = sc + (size_t)(d - 1) * COMB_ENTRY_WORDS;
+    for (int k = 0; k < FPW; k++) {
       e[k] = acc.x.v[k];
-      e[12 + k] = acc.y.v[k];
+      e[FPW + k] = acc.y.v[k];
       z[k] = acc.z.v[k];
-      z[12 + k] = prod.v[k];
+      z[FPW + k] = prod.v[k];
     }
     if (d < COMB_ENTRIES) pt_madd(acc, acc, baseA);
   }
@@ -213,20 +213,20 @@ __global__ __launch_bounds__(64) void k_comb_build(uint32_t first, uint32_t coun
   fp_inv(inv, prod);  // 1 / (z_1 ... z_128)
   for (int d = COMB_ENTRIES; d >= 1; d--) {
     uint32_t* e = out + (size_t)(d - 1) * COMB_ENTRY_WORDS;
-    const uint32_t* z = sc + (size_t)(d - 1) * 24;
+    const uint32_t* z = sc + (size_t)(d - 1) * COMB_ENTRY_WORDS;
     Fp zd, pre, zi;
-    for (int k = 0; k < 12; k++) zd.v[k] = z[k];
+    for (int k = 0; k < FPW; k++) zd.v[k] = z[k];
     if (d > 1) {
-      for (int k = 0; k < 12; k++) pre.v[k] = sc[(size_t)(d - 2) * 24 + 12 + k];
+      for (int k = 0; k < FPW; k++) pre.v[k] = sc[(size_t)(d - 2) * COMB_ENTRY_WORDS + FPW + k];
       fp_mul(zi, inv, pre);
     } else {
       zi = inv;
     }
     fp_mul(inv, inv, zd);
     Fp X, Y, zi2, zi3;
-    for (int k = 0; k < 12; k++) {
+    for (int k = 0; k < FPW; k++) {
       X.v[k] = e[k];
-      Y.v[k] = e[12 + k];
+      Y.v[k] = e[FPW + k];
     }
     fp_sqr(zi2, zi);
     fp_mul(zi3, zi2, zi);
@@ -279,9 +279,9 @@ __global__ __launch_bounds__(128) void k_comb_msm(const CombChunk* __restrict__ 
         const uint4* e4 = reinterpret_cast<const uint4*>(
             tab + ((size_t)w * COMB_ENTRIES + (ad - 1)) * COMB_ENTRY_WORDS);
         G1A e;
-        uint32_t ew[24];
+        uint32_t ew[COMB_ENTRY_WORDS];
 #pragma unroll
-        for (int q = 0; q < 6; q++) {
+        for (int q = 0; q < (int)COMB_ENTRY_WORDS / 4; q++) {
           uint4 v = e4[q];
           ew[4 * q] = v.x;
           ew[4 * q + 1] = v.y;
@@ -459,9 +459,9 @@ __global__ __launch_bounds__(256) void k_fp_mul_bench(uint32_t* data, uint32_t n
   if (i >= n) return;
   Fp a, b, c, d;
 #pragma unroll
-  for (int k = 0; k < 12; k++) {
-    a.v[k] = data[(size_t)i * 24 + k];
-    b.v[k] = data[(size_t)i * 24 + 12 + k];
+  for (int k = 0; k < FPW; k++) {
+    a.v[k] = data[(size_t)i * 2 * FPW + k];
+    b.v[k] = data[(size_t)i * 2 * FPW + FPW + k];
   }
   c = a;
   d = b;
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(256) void k_fp_mul_bench(uint32_t* data, uint32_t n
   }
   fp_add(a, a, c);
 #pragma unroll
-  for (int k = 0; k < 12; k++) data[(size_t)i * 24 + k] = a.v[k];
+  for (int k = 0; k < FPW; k++) data[(size_t)i * 2 * FPW + k] = a.v[k];
 }
 void launch_fp_mul_bench(hipStream_t s, uint32_t* data, uint32_t n, uint32_t iters) {
   hipLaunchKernelGGL(k_fp_mul_bench, GRID(n, 256), 0, s, data, n, iters);

@@ -250,7 +250,7 @@ DEVN void pt_mul_limbs_affine(PtJ<F>& r, const PtA<F>& q, const uint32_t* k) {
 // ------------------------------------------------------------------ G1 specifics
 DEV void g1_load_affine_const(G1A& r, const uint32_t* c) {
   fp_load_const(r.x, c);
-  fp_load_const(r.y, c + 12);
+  fp_load_const(r.y, c + FPW);
 }
 
 // phi(x, y) = (beta x, y) acts on G1 as multiplication by lambda = x^2 - 1.
@@ -335,9 +335,9 @@ DEVN void g1_compress(uint8_t* out, const G1J& p) {
 // ------------------------------------------------------------------ G2 specifics
 DEV void g2_load_affine_const(G2A& r, const uint32_t* c) {
   fp_load_const(r.x.c0, c);
-  fp_load_const(r.x.c1, c + 12);
-  fp_load_const(r.y.c0, c + 24);
-  fp_load_const(r.y.c1, c + 36);
+  fp_load_const(r.x.c1, c + FPW);
+  fp_load_const(r.y.c0, c + 2 * FPW);
+  fp_load_const(r.y.c1, c + 3 * FPW);
 }
 
 DEVN bool g2_in_subgroup(const G2J& q) {

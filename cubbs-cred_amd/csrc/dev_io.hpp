@@ -14,32 +14,32 @@ DEV void store_fr(uint32_t* p, const Fr& r) {
 }
 DEV void load_g1a(G1A& r, const uint32_t* p) {
 #pragma unroll
-  for (int k = 0; k < 12; k++) {
+  for (int k = 0; k < FPW; k++) {
     r.x.v[k] = p[k];
-    r.y.v[k] = p[12 + k];
+    r.y.v[k] = p[FPW + k];
   }
 }
 DEV void store_g1a(uint32_t* p, const G1A& r) {
 #pragma unroll
-  for (int k = 0; k < 12; k++) {
+  for (int k = 0; k < FPW; k++) {
     p[k] = r.x.v[k];
-    p[12 + k] = r.y.v[k];
+    p[FPW + k] = r.y.v[k];
   }
 }
 DEV void load_g1j(G1J& r, const uint32_t* p) {
 #pragma unroll
-  for (int k = 0; k < 12; k++) {
+  for (int k = 0; k < FPW; k++) {
     r.x.v[k] = p[k];
-    r.y.v[k] = p[12 + k];
-    r.z.v[k] = p[24 + k];
+    r.y.v[k] = p[FPW + k];
+    r.z.v[k] = p[2 * FPW + k];
   }
 }
 DEV void store_g1j(uint32_t* p, const G1J& r) {
 #pragma unroll
-  for (int k = 0; k < 12; k++) {
+  for (int k = 0; k < FPW; k++) {
     p[k] = r.x.v[k];
-    p[12 + k] = r.y.v[k];
-    p[24 + k] = r.z.v[k];
+    p[FPW + k] = r.y.v[k];
+    p[2 * FPW + k] = r.z.v[k];
   }
 }
 // 32-byte big-endian scalar as 8 SHA words (most significant first)

@@ -1,11 +1,12 @@
-// Finite-field arithmetic for BLS12-381 on gfx950: Fp (381-bit, 12 x 32-bit
+// Finite-field arithmetic for BLS12-381 on gfx950: Fp (381-bit, 14 x 28-bit
 // limbs) and Fr (255-bit, 8 x 32-bit limbs) in Montgomery form, and the
 // Fp2 / Fp6 / Fp12 tower used by the pairing.
 //
-// One lane owns one field element.  Multiplication is CIOS Montgomery with the
-// "no final carry" variant (both moduli leave a spare top bit), so every inner
-// step is one v_mad_u64_u32 plus a 32-bit carry add; values are kept fully
-// reduced in [0, m).
+// One lane owns one field element.  Fp multiplication is product scanning on
+// 28-bit limbs with 64-bit column accumulators (see the Fp section); Fr uses
+// CIOS on 32-bit limbs with the "no final carry" variant (r leaves a spare top
+// bit), every inner step one v_mad_u64_u32 plus a carry add, values fully
+// reduced in [0, r).
 //
 // Tower: Fp2 = Fp[u]/(u^2+1), Fp6 = Fp2[v]/(v^3 - xi), xi = 1+u,
 //        Fp12 = Fp6[w]/(w^2 - v).
@@ -19,7 +20,7 @@
 #define DEVI __device__ inline
 #define DEVN __device__ __noinline__
 
-struct Fp { uint32_t v[12]; };
+struct Fp { uint32_t v[14]; };
 struct Fr { uint32_t v[8]; };
 struct Fp2 { Fp c0, c1; };
 struct Fp6 { Fp2 c0, c1, c2; };
@@ -134,22 +135,151 @@ DEV bool limbs_lt(const uint32_t* a, const uint32_t* b) {
 }
 
 // ------------------------------------------------------------------ Fp
+// Fp elements are 14 limbs of 28 bits (little-endian), Montgomery radix
+// R = 2^392.  Invariant: every limb < 2^28 and the value < 2p.  Since 4p < R/2^9,
+// a Montgomery product of two values below 2p lands below 2p without a final
+// subtraction, and each column of the product (at most 28 products of < 2^56
+// plus a < 2^37 carry) fits a 64-bit accumulator: one v_mad_u64_u32 per
+// partial product and no carry instructions.  Values are made canonical
+// (< p) only where they are compared or encoded.
+static constexpr int FPW = 14;             // 32-bit words per Fp element
+static constexpr uint32_t FP_MASK = (1u << 28) - 1;
+
 DEV void fp_mul(Fp& r, const Fp& a, const Fp& b) {
-  mont_mul_n<12>(r.v, a.v, b.v, bls::P_MOD, bls::P_INV);
+  const uint32_t* M = bls::P28;
+  uint32_t m[14];
+  uint64_t carry = 0;
+#pragma unroll
+  for (int s = 0; s < 14; s++) {
+    uint64_t x = carry, y = 0;
+#pragma unroll
+    for (int i = 0; i <= s; i++) x += (uint64_t)a.v[i] * b.v[s - i];
+#pragma unroll
+    for (int i = 0; i < s; i++) y += (uint64_t)m[i] * M[s - i];
+    x += y;
+    m[s] = ((uint32_t)x * bls::P28_INV) & FP_MASK;
+    x += (uint64_t)m[s] * M[0];
+    carry = x >> 28;
+  }
+#pragma unroll
+  for (int s = 14; s < 27; s++) {
+    uint64_t x = carry, y = 0;
+#pragma unroll
+    for (int i = s - 13; i < 14; i++) {
+      x += (uint64_t)a.v[i] * b.v[s - i];
+      y += (uint64_t)m[i] * M[s - i];
+    }
+    x += y;
+    r.v[s - 14] = (uint32_t)x & FP_MASK;
+    carry = x >> 28;
+  }
+  r.v[13] = (uint32_t)carry;
 }
-DEV void fp_sqr(Fp& r, const Fp& a) { fp_mul(r, a, a); }
-DEV void fp_add(Fp& r, const Fp& a, const Fp& b) { mod_add_n<12>(r.v, a.v, b.v, bls::P_MOD); }
-DEV void fp_sub(Fp& r, const Fp& a, const Fp& b) { mod_sub_n<12>(r.v, a.v, b.v, bls::P_MOD); }
+// Squaring: the off-diagonal products are formed once and doubled.
+DEV void fp_sqr(Fp& r, const Fp& a) {
+  const uint32_t* M = bls::P28;
+  uint32_t m[14];
+  uint64_t carry = 0;
+#pragma unroll
+  for (int s = 0; s < 14; s++) {
+    uint64_t c = 0, y = 0;
+#pragma unroll
+    for (int i = 0; 2 * i < s; i++) c += (uint64_t)a.v[i] * a.v[s - i];
+    uint64_t x = carry + (c << 1);
+    if ((s & 1) == 0) x += (uint64_t)a.v[s / 2] * a.v[s / 2];
+#pragma unroll
+    for (int i = 0; i < s; i++) y += (uint64_t)m[i] * M[s - i];
+    x += y;
+    m[s] = ((uint32_t)x * bls::P28_INV) & FP_MASK;
+    x += (uint64_t)m[s] * M[0];
+    carry = x >> 28;
+  }
+#pragma unroll
+  for (int s = 14; s < 27; s++) {
+    uint64_t c = 0, y = 0;
+#pragma unroll
+    for (int i = s - 13; 2 * i < s; i++) c += (uint64_t)a.v[i] * a.v[s - i];
+#pragma unroll
+    for (int i = s - 13; i < 14; i++) y += (uint64_t)m[i] * M[s - i];
+    uint64_t x = carry + (c << 1) + y;
+    if ((s & 1) == 0) x += (uint64_t)a.v[s / 2] * a.v[s / 2];
+    r.v[s - 14] = (uint32_t)x & FP_MASK;
+    carry = x >> 28;
+  }
+  r.v[13] = (uint32_t)carry;
+}
+// r = a + b, reduced below 2p
+DEV void fp_add(Fp& r, const Fp& a, const Fp& b) {
+  uint32_t s[14], d[14];
+  uint32_t c = 0;
+  int32_t br = 0;
+#pragma unroll
+  for (int k = 0; k < 14; k++) {
+    const uint32_t t = a.v[k] + b.v[k] + c;
+    c = t >> 28;
+    s[k] = t & FP_MASK;
+    const int32_t u = (int32_t)s[k] - (int32_t)bls::P2_28[k] + br;
+    br = u >> 28;
+    d[k] = (uint32_t)u & FP_MASK;
+  }
+#pragma unroll
+  for (int k = 0; k < 14; k++) r.v[k] = br ? s[k] : d[k];
+}
+// r = a - b, reduced below 2p (2p is added back when a < b)
+DEV void fp_sub(Fp& r, const Fp& a, const Fp& b) {
+  uint32_t d[14];
+  int32_t br = 0;
+#pragma unroll
+  for (int k = 0; k < 14; k++) {
+    const int32_t u = (int32_t)a.v[k] - (int32_t)b.v[k] + br;
+    br = u >> 28;
+    d[k] = (uint32_t)u & FP_MASK;
+  }
+  const uint32_t msk = (uint32_t)br;
+  uint32_t c = 0;
+#pragma unroll
+  for (int k = 0; k < 14; k++) {
+    const uint32_t t = d[k] + (bls::P2_28[k] & msk) + c;
+    c = t >> 28;
+    r.v[k] = t & FP_MASK;
+  }
+}
 DEV void fp_dbl(Fp& r, const Fp& a) { fp_add(r, a, a); }
-DEV bool fp_is_zero(const Fp& a) { return limbs_is_zero<12>(a.v); }
-DEV bool fp_eq(const Fp& a, const Fp& b) { return limbs_eq<12>(a.v, b.v); }
+// a - p if a >= p: the canonical representative of a value below 2p
+DEV void fp_canon(Fp& r, const Fp& a) {
+  uint32_t d[14];
+  int32_t br = 0;
+#pragma unroll
+  for (int k = 0; k < 14; k++) {
+    const int32_t u = (int32_t)a.v[k] - (int32_t)bls::P28[k] + br;
+    br = u >> 28;
+    d[k] = (uint32_t)u & FP_MASK;
+  }
+#pragma unroll
+  for (int k = 0; k < 14; k++) r.v[k] = br ? a.v[k] : d[k];
+}
+// zero is represented by 0 or by p
+DEV bool fp_is_zero(const Fp& a) {
+  uint32_t z = 0, q = 0;
+#pragma unroll
+  for (int k = 0; k < 14; k++) {
+    z |= a.v[k];
+    q |= a.v[k] ^ bls::P28[k];
+  }
+  return z == 0 || q == 0;
+}
+DEV bool fp_eq(const Fp& a, const Fp& b) {
+  Fp d;
+  fp_sub(d, a, b);
+  return fp_is_zero(d);
+}
 DEV void fp_zero(Fp& r) {
 #pragma unroll
-  for (int k = 0; k < 12; k++) r.v[k] = 0;
+  for (int k = 0; k < 14; k++) r.v[k] = 0;
 }
 DEV void fp_one(Fp& r) {
 #pragma unroll
-  for (int k = 0; k < 12; k++) r.v[k] = bls::P_ONE[k];
+  for (int k = 0; k < 14; k++) r.v[k] = bls::P_ONE[k];
 }
 DEV void fp_neg(Fp& r, const Fp& a) {
   Fp z;
@@ -158,7 +288,7 @@ DEV void fp_neg(Fp& r, const Fp& a) {
 }
 DEV void fp_load_const(Fp& r, const uint32_t* c) {
 #pragma unroll
-  for (int k = 0; k < 12; k++) r.v[k] = c[k];
+  for (int k = 0; k < 14; k++) r.v[k] = c[k];
 }
 // canonical -> Montgomery
 DEV void fp_to_mont(Fp& r, const Fp& a) {
@@ -166,14 +296,15 @@ DEV void fp_to_mont(Fp& r, const Fp& a) {
   fp_load_const(r2, bls::P_R2);
   fp_mul(r, a, r2);
 }
-// Montgomery -> canonical
+// Montgomery -> canonical (fully reduced below p)
 DEV void fp_from_mont(Fp& r, const Fp& a) {
   Fp one;
   fp_zero(one);
   one.v[0] = 1;
   fp_mul(r, a, one);
+  fp_canon(r, r);
 }
-// r = a^e, e given as 12 little-endian limbs (compile-time constant array)
+// r = a^e, e given as 12 little-endian 32-bit words (a constant array)
 DEVN void fp_pow_const(Fp& r, const Fp& a, const uint32_t* e) {
   Fp acc;
   fp_one(acc);
@@ -204,12 +335,32 @@ DEVI bool fp_sqrt(Fp& r, const Fp& a) {
 DEV bool fp_lex_largest(const Fp& a) {
   Fp c;
   fp_from_mont(c, a);
-  return limbs_lt<12>(bls::P_HALF, c.v);
+  return limbs_lt<14>(bls::P_HALF, c.v);
 }
 DEV uint32_t fp_sgn0(const Fp& a) {
   Fp c;
   fp_from_mont(c, a);
   return c.v[0] & 1;
+}
+// 384-bit little-endian words (12 x 32) <-> 28-bit limbs
+DEV void fp_limbs_from_words(uint32_t* r14, const uint32_t* w12) {
+#pragma unroll
+  for (int i = 0; i < 14; i++) {
+    const int bit = 28 * i, k = bit >> 5, sh = bit & 31;
+    uint32_t v = k < 12 ? (w12[k] >> sh) : 0u;
+    if (sh > 4 && k + 1 < 12) v |= w12[k + 1] << (32 - sh);
+    r14[i] = v & FP_MASK;
+  }
+}
+DEV void fp_words_from_limbs(uint32_t* w12, const uint32_t* r14) {
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    const int bit = 32 * k, i = bit / 28, sh = bit % 28;
+    uint32_t v = r14[i] >> sh;
+    v |= r14[i + 1] << (28 - sh);
+    if (sh > 24 && i + 2 < 14) v |= r14[i + 2] << (56 - sh);
+    w12[k] = v;
+  }
 }
 
 // ------------------------------------------------------------------ Fr
@@ -283,33 +434,21 @@ DEV void fr_reduce_wide(Fr& out, const uint32_t* v16) {
   fr_add(out, t, lo);
 }
 
-// Reduce a 512-bit little-endian value mod p (canonical -> Montgomery form).
-// v = hi * 2^384 + lo, hi < 2^128, lo < 2^384 < 10 p.
-DEV void fp_sub_if_geq(Fp& a) {
-  uint32_t d[12];
-  uint32_t borrow = 0;
-#pragma unroll
-  for (int k = 0; k < 12; k++) {
-    uint64_t x = (uint64_t)a.v[k] - bls::P_MOD[k] - borrow;
-    d[k] = (uint32_t)x;
-    borrow = (uint32_t)(x >> 63);
-  }
-#pragma unroll
-  for (int k = 0; k < 12; k++) a.v[k] = borrow ? a.v[k] : d[k];
-}
+// Reduce a 512-bit little-endian value (16 x 32-bit words) mod p, result in
+// Montgomery form: v = hi * 2^384 + lo with lo < 2^384 (below 10p, which the
+// Montgomery product accepts) and hi < 2^128.
 DEVN void fp_reduce_wide_to_mont(Fp& out, const uint32_t* v16) {
-  Fp lo, hi, r2, t;
+  Fp lo, hi, c, t;
+  fp_limbs_from_words(lo.v, v16);
+  uint32_t w[12];
 #pragma unroll
-  for (int k = 0; k < 12; k++) {
-    lo.v[k] = v16[k];
-    hi.v[k] = k < 4 ? v16[12 + k] : 0u;
-    r2.v[k] = bls::P_R2[k];
-  }
-#pragma unroll 1
-  for (int it = 0; it < 9; it++) fp_sub_if_geq(lo);
-  fp_mul(t, hi, r2);       // hi * 2^384 mod p
-  fp_add(t, t, lo);
-  fp_mul(out, t, r2);      // to Montgomery
+  for (int k = 0; k < 12; k++) w[k] = k < 4 ? v16[12 + k] : 0u;
+  fp_limbs_from_words(hi.v, w);
+  fp_load_const(c, bls::P_R2);
+  fp_mul(t, lo, c);                 // lo * R mod p
+  fp_load_const(c, bls::P_WIDE_HI);
+  fp_mul(out, hi, c);               // hi * 2^384 * R mod p
+  fp_add(out, out, t);
 }
 
 // ------------------------------------------------------------------ Fp2
@@ -372,7 +511,7 @@ DEVN void fp2_pow_const(Fp2& r, const Fp2& a, const uint32_t* e) {
 }
 DEV void fp2_load_const(Fp2& r, const uint32_t* c) {
   fp_load_const(r.c0, c);
-  fp_load_const(r.c1, c + 12);
+  fp_load_const(r.c1, c + FPW);
 }
 // Square root in Fp2, p = 3 mod 4 (Adj & Rodriguez-Henriquez, Algorithm 9).
 DEVN bool fp2_sqrt(Fp2& r, const Fp2& a) {
@@ -557,34 +696,34 @@ DEVN void fp12_frob(Fp12& r, const Fp12& a) {
   Fp2 g;
   fp2_conj(r.c0.c0, a.c0.c0);
   fp2_conj(r.c0.c1, a.c0.c1);
-  fp2_load_const(g, bls::FROB1 + 24 * 2);
+  fp2_load_const(g, bls::FROB1 + 2 * FPW * 2);
   fp2_mul(r.c0.c1, r.c0.c1, g);
   fp2_conj(r.c0.c2, a.c0.c2);
-  fp2_load_const(g, bls::FROB1 + 24 * 4);
+  fp2_load_const(g, bls::FROB1 + 2 * FPW * 4);
   fp2_mul(r.c0.c2, r.c0.c2, g);
   fp2_conj(r.c1.c0, a.c1.c0);
-  fp2_load_const(g, bls::FROB1 + 24 * 1);
+  fp2_load_const(g, bls::FROB1 + 2 * FPW * 1);
   fp2_mul(r.c1.c0, r.c1.c0, g);
   fp2_conj(r.c1.c1, a.c1.c1);
-  fp2_load_const(g, bls::FROB1 + 24 * 3);
+  fp2_load_const(g, bls::FROB1 + 2 * FPW * 3);
   fp2_mul(r.c1.c1, r.c1.c1, g);
   fp2_conj(r.c1.c2, a.c1.c2);
-  fp2_load_const(g, bls::FROB1 + 24 * 5);
+  fp2_load_const(g, bls::FROB1 + 2 * FPW * 5);
   fp2_mul(r.c1.c2, r.c1.c2, g);
 }
 // x -> x^(p^2): coefficients stay, scaled by gamma2[k] in Fp.
 DEVN void fp12_frob2(Fp12& r, const Fp12& a) {
   Fp g;
   r.c0.c0 = a.c0.c0;
-  fp_load_const(g, bls::FROB2 + 12 * 2);
+  fp_load_const(g, bls::FROB2 + FPW * 2);
   fp2_mul_fp(r.c0.c1, a.c0.c1, g);
-  fp_load_const(g, bls::FROB2 + 12 * 4);
+  fp_load_const(g, bls::FROB2 + FPW * 4);
   fp2_mul_fp(r.c0.c2, a.c0.c2, g);
-  fp_load_const(g, bls::FROB2 + 12 * 1);
+  fp_load_const(g, bls::FROB2 + FPW * 1);
   fp2_mul_fp(r.c1.c0, a.c1.c0, g);
-  fp_load_const(g, bls::FROB2 + 12 * 3);
+  fp_load_const(g, bls::FROB2 + FPW * 3);
   fp2_mul_fp(r.c1.c1, a.c1.c1, g);
-  fp_load_const(g, bls::FROB2 + 12 * 5);
+  fp_load_const(g, bls::FROB2 + FPW * 5);
   fp2_mul_fp(r.c1.c2, a.c1.c2, g);
 }
 
@@ -592,22 +731,26 @@ DEVN void fp12_frob2(Fp12& r, const Fp12& a) {
 // 48-byte big-endian -> canonical limbs; false when >= p (flags must be cleared
 // by the caller).
 DEV bool fp_from_be_bytes(Fp& r, const uint8_t* be) {
+  uint32_t w[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) {
     const uint8_t* q = be + 44 - 4 * k;
-    r.v[k] = ((uint32_t)q[0] << 24) | ((uint32_t)q[1] << 16) | ((uint32_t)q[2] << 8) | q[3];
+    w[k] = ((uint32_t)q[0] << 24) | ((uint32_t)q[1] << 16) | ((uint32_t)q[2] << 8) | q[3];
   }
-  return limbs_lt<12>(r.v, bls::P_MOD);
+  fp_limbs_from_words(r.v, w);
+  return limbs_lt<14>(r.v, bls::P28);
 }
+// canonical (below p) -> 48-byte big-endian
 DEV void fp_to_be_bytes(uint8_t* be, const Fp& canonical) {
+  uint32_t w[12];
+  fp_words_from_limbs(w, canonical.v);
 #pragma unroll
   for (int k = 0; k < 12; k++) {
     uint8_t* q = be + 44 - 4 * k;
-    uint32_t w = canonical.v[k];
-    q[0] = (uint8_t)(w >> 24);
-    q[1] = (uint8_t)(w >> 16);
-    q[2] = (uint8_t)(w >> 8);
-    q[3] = (uint8_t)w;
+    q[0] = (uint8_t)(w[k] >> 24);
+    q[1] = (uint8_t)(w[k] >> 16);
+    q[2] = (uint8_t)(w[k] >> 8);
+    q[3] = (uint8_t)w[k];
   }
 }
 DEV void fr_from_be_bytes_raw(Fr& r, const uint8_t* be) {

@@ -2,8 +2,9 @@
 """Generate bls12_381_consts.hpp: every arithmetic constant the HIP library
 uses, derived from the BLS parameter x = -0xd201000000010000.
 
-Constants are emitted as 32-bit little-endian limbs, field elements already in
-Montgomery form (R = 2^384 for Fp, 2^256 for Fr).  The 11-isogeny
+Fp elements are emitted as 14 little-endian limbs of 28 bits in Montgomery
+form with R = 2^392; Fr elements as 8 little-endian 32-bit limbs in Montgomery
+form with R = 2^256.  Exponents stay 12 x 32-bit words.  The 11-isogeny
 coefficients come from h2c_iso_consts.h (derived by the oracle-side script and
 pinned there against the CFRG generator vectors); this script converts them.
 
@@ -16,12 +17,19 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 X = -0xD201000000010000
 P = ((X - 1) ** 2 * (X ** 4 - X ** 2 + 1)) // 3 + X
 R = X ** 4 - X ** 2 + 1
-RP = 1 << 384
+NL = 14          # Fp limbs
+LB = 28          # bits per Fp limb
+RP = 1 << (NL * LB)   # Montgomery radix of Fp: 2^392
 RR = 1 << 256
 
 
 def limbs(v, n):
     return [(v >> (32 * i)) & 0xFFFFFFFF for i in range(n)]
+
+
+def limbs28(v):
+    assert 0 <= v < RP
+    return [(v >> (LB * i)) & ((1 << LB) - 1) for i in range(NL)]
 
 
 def mont(v):
@@ -92,31 +100,38 @@ def main():
     def arr(name, vals, n, typ="uint32_t"):
         flat = []
         for v in vals:
-            flat += limbs(v, n)
+            flat += limbs28(v) if n == "fp" else limbs(v, n)
         body = ", ".join("0x%08xu" % w for w in flat)
         emit("static constexpr %s %s[%d] = {%s};" % (typ, name, len(flat), body))
 
     pinv = (-pow(P, -1, 1 << 32)) % (1 << 32)
+    pinv28 = (-pow(P, -1, 1 << LB)) % (1 << LB)
     rinv = (-pow(R, -1, 1 << 32)) % (1 << 32)
     emit("// Generated by gen_consts.py from x = -0xd201000000010000. Do not edit.")
     emit("#pragma once")
     emit("#include <stdint.h>")
     emit("namespace bls {")
-    arr("P_MOD", [P], 12)
-    emit("static constexpr uint32_t P_INV = 0x%08xu;" % pinv)
-    arr("P_R2", [RP * RP % P], 12)
-    arr("P_ONE", [mont(1)], 12)
+    emit("// Fp elements: %d limbs of %d bits, Montgomery radix R = 2^%d" % (NL, LB, NL * LB))
+    emit("static constexpr int FP_LIMBS = %d;" % NL)
+    emit("static constexpr int FP_LIMB_BITS = %d;" % LB)
+    arr("P28", [P], "fp")
+    arr("P2_28", [2 * P], "fp")
+    emit("static constexpr uint32_t P28_INV = 0x%08xu;" % pinv28)
+    arr("P_R2", [RP * RP % P], "fp")
+    arr("P_ONE", [mont(1)], "fp")
+    # 2^384 R^2 mod p: mont_mul(hi, P_WIDE_HI) = Montgomery form of hi * 2^384
+    arr("P_WIDE_HI", [(1 << 384) * RP * RP % P], "fp")
     arr("P_MINUS_2", [P - 2], 12)
     arr("P_SQRT_EXP", [(P + 1) // 4], 12)
     arr("P_LEGENDRE_EXP", [(P - 1) // 2], 12)
     arr("P_MINUS3_DIV4", [(P - 3) // 4], 12)
-    arr("P_HALF", [(P - 1) // 2], 12)  # canonical, for lexicographic sign
+    arr("P_HALF", [(P - 1) // 2], "fp")  # canonical (not Montgomery), lexicographic sign
     arr("R_MOD", [R], 8)
     emit("static constexpr uint32_t R_INV = 0x%08xu;" % rinv)
     arr("R_R2", [RR * RR % R], 8)
     arr("R_ONE", [montr(1)], 8)
     arr("R_MINUS_2", [R - 2], 8)
-    arr("FP_B", [mont(4)], 12)
+    arr("FP_B", [mont(4)], "fp")
     # Frobenius: gamma1[k] = xi^(k(p-1)/6) in Fp2 ; gamma2[k] = xi^(k(p^2-1)/6) in Fp
     xi = (1, 1)
     g = f2pow(xi, (P - 1) // 6)
@@ -124,14 +139,14 @@ def main():
     for k in range(1, 6):
         gam.append(f2mul(gam[-1], g))
     emit("// gamma1[k] = xi^(k (p-1)/6) as (c0, c1), Montgomery")
-    arr("FROB1", [mont(c) for gk in gam for c in gk], 12)
+    arr("FROB1", [mont(c) for gk in gam for c in gk], "fp")
     g2 = f2pow(xi, (P * P - 1) // 6)
     gam2 = [(1, 0)]
     for k in range(1, 6):
         gam2.append(f2mul(gam2[-1], g2))
     assert all(c[1] == 0 for c in gam2)
     emit("// gamma2[k] = xi^(k (p^2-1)/6), an element of Fp")
-    arr("FROB2", [mont(c[0]) for c in gam2], 12)
+    arr("FROB2", [mont(c[0]) for c in gam2], "fp")
     # G2 generator (affine, Montgomery): x = x0 + x1 u
     g2x0 = 0x024AA2B2F08F0A91260805272DC51051C6E47AD4FA403B02B4510B647AE3D1770BAC0326A805BBEFD48056C8C121BDB8
     g2x1 = 0x13E02B6052719F607DACD3A088274F65596BD0D09920B61AB5DA61BBDC7F5049334CF11213945D57E5AC7D055D042B7E
@@ -144,14 +159,14 @@ def main():
     rhs = f2mul(x, f2mul(x, x))
     rhs = ((rhs[0] + 4) % P, (rhs[1] + 4) % P)
     assert lhs == rhs
-    arr("G2_GEN", [mont(g2x0), mont(g2x1), mont(g2y0), mont(g2y1)], 12)
-    arr("G2_GEN_NEG", [mont(g2x0), mont(g2x1), mont((-g2y0) % P), mont((-g2y1) % P)], 12)
+    arr("G2_GEN", [mont(g2x0), mont(g2x1), mont(g2y0), mont(g2y1)], "fp")
+    arr("G2_GEN_NEG", [mont(g2x0), mont(g2x1), mont((-g2y0) % P), mont((-g2y1) % P)], "fp")
     # G1 generator and BBS P1
     g1 = decompress_g1("97f1d3a73197d7942695638c4fa9ac0fc3688c4f9774b905a14e3a3f171bac586c55e83ff97a1aeffb3af00adb22c6bb")
     assert ec_mul(g1, R) is None
-    arr("G1_GEN", [mont(g1[0]), mont(g1[1])], 12)
+    arr("G1_GEN", [mont(g1[0]), mont(g1[1])], "fp")
     p1 = decompress_g1("a8ce256102840821a3e94ea9025e4662b205762f9776b3a766c872b948f1fd225e7c59698588e70d11406d161b4e28c9")
-    arr("BBS_P1", [mont(p1[0]), mont(p1[1])], 12)
+    arr("BBS_P1", [mont(p1[0]), mont(p1[1])], "fp")
     # GLV: beta (cube root of unity in Fp) with (beta x, y) = lambda P, lambda = x^2 - 1
     lam = (X * X - 1) % R
     assert (lam * lam + lam + 1) % R == 0
@@ -164,7 +179,7 @@ def main():
                     beta = cand
             break
     assert beta is not None
-    arr("GLV_BETA", [mont(beta)], 12)
+    arr("GLV_BETA", [mont(beta)], "fp")
     arr("GLV_LAMBDA", [lam], 8)
     # Barrett constant for k div lambda, k < 2^256: mu = floor(2^256 / lambda)
     mu = (1 << 256) // lam
@@ -174,16 +189,16 @@ def main():
     iso = open(os.path.join(HERE, "h2c_iso_consts.h")).read()
     A_iso = int(re.search(r'H2C_ISO_A "([0-9a-f]+)"', iso).group(1), 16)
     B_iso = int(re.search(r'H2C_ISO_B "([0-9a-f]+)"', iso).group(1), 16)
-    arr("SSWU_A", [mont(A_iso)], 12)
-    arr("SSWU_B", [mont(B_iso)], 12)
-    arr("SSWU_Z", [mont(11)], 12)
+    arr("SSWU_A", [mont(A_iso)], "fp")
+    arr("SSWU_B", [mont(B_iso)], "fp")
+    arr("SSWU_Z", [mont(11)], "fp")
     # -B/A and B/(Z A)
-    arr("SSWU_MINUS_B_OVER_A", [mont((-B_iso) * pow(A_iso, P - 2, P) % P)], 12)
-    arr("SSWU_B_OVER_ZA", [mont(B_iso * pow(11 * A_iso, P - 2, P) % P)], 12)
+    arr("SSWU_MINUS_B_OVER_A", [mont((-B_iso) * pow(A_iso, P - 2, P) % P)], "fp")
+    arr("SSWU_B_OVER_ZA", [mont(B_iso * pow(11 * A_iso, P - 2, P) % P)], "fp")
     for name in ("XNUM", "XDEN", "YNUM", "YDEN"):
         block = re.search(r"H2C_ISO_%s\[(\d+)\] = \{([^}]*)\}" % name, iso, re.S)
         vals = [int(v, 16) for v in re.findall(r'"([0-9a-f]+)"', block.group(2))]
-        arr("ISO_" + name, [mont(v) for v in vals], 12)
+        arr("ISO_" + name, [mont(v) for v in vals], "fp")
         emit("static constexpr int ISO_%s_LEN = %d;" % (name, len(vals)))
     emit("}  // namespace bls")
     path = os.path.join(HERE, "bls12_381_consts.hpp")

@@ -36,7 +36,7 @@ DEVN void fp_horner_const(Fp& r, const uint32_t* coeffs, int n, const Fp& x) {
 #pragma unroll 1
   for (int i = n - 1; i >= 0; i--) {
     fp_mul(acc, acc, x);
-    fp_load_const(c, coeffs + 12 * i);
+    fp_load_const(c, coeffs + FPW * i);
     fp_add(acc, acc, c);
   }
   r = acc;

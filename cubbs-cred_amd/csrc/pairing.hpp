@@ -19,7 +19,7 @@
 #define MILLER_LINES 68
 static constexpr uint64_t BLS_X_ABS = 0xd201000000010000ull;
 
-struct Line { Fp2 lambda, c; };   // 48 u32 = 192 bytes
+struct Line { Fp2 lambda, c; };   // 4 x 14 limbs = 224 bytes
 
 // Compute the lines of a G2 point (affine, not the identity).
 DEVN void g2_prepare_lines(Line* out, const G2A& q) {

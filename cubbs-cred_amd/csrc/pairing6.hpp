@@ -17,32 +17,34 @@
 #include "pairing.hpp"
 
 #define G6_LANES 8
-// LDS words per group: two Fp12 operands of 6 x 24 words, plus flags
-#define G6_LDS_WORDS (2 * 6 * 24 + 8)
+// words per Fp2 slot in LDS
+#define G6_F2 (2 * FPW)
+// LDS words per group: two Fp12 operands of 6 Fp2 slots, plus flags
+#define G6_LDS_WORDS (2 * 6 * 2 * FPW + 8)
 
 DEV void g6_put(uint32_t* dst, const Fp2& v) {
 #pragma unroll
-  for (int i = 0; i < 12; i++) {
+  for (int i = 0; i < FPW; i++) {
     dst[i] = v.c0.v[i];
-    dst[12 + i] = v.c1.v[i];
+    dst[FPW + i] = v.c1.v[i];
   }
 }
 DEV void g6_get(Fp2& v, const uint32_t* src) {
 #pragma unroll
-  for (int i = 0; i < 12; i++) {
+  for (int i = 0; i < FPW; i++) {
     v.c0.v[i] = src[i];
-    v.c1.v[i] = src[12 + i];
+    v.c1.v[i] = src[FPW + i];
   }
 }
 
 // f <- f * g
 DEVN void g6_mul(Fp2& f, const Fp2& g, int k, uint32_t* lds) {
   uint32_t* A = lds;
-  uint32_t* B = lds + 6 * 24;
+  uint32_t* B = lds + 6 * G6_F2;
   __syncthreads();
   if (k < 6) {
-    g6_put(A + 24 * k, f);
-    g6_put(B + 24 * k, g);
+    g6_put(A + G6_F2 * k, f);
+    g6_put(B + G6_F2 * k, g);
   }
   __syncthreads();
   if (k < 6) {
@@ -50,8 +52,8 @@ DEVN void g6_mul(Fp2& f, const Fp2& g, int k, uint32_t* lds) {
     fp2_zero(acc);
     for (int i = 0; i < 6; i++) {
       const int j = (k - i + 6) % 6;
-      g6_get(x, A + 24 * i);
-      g6_get(y, B + 24 * j);
+      g6_get(x, A + G6_F2 * i);
+      g6_get(y, B + G6_F2 * j);
       fp2_mul(p, x, y);
       if (i > k) fp2_mul_xi(p, p);
       fp2_add(acc, acc, p);
@@ -73,7 +75,7 @@ __constant__ static const int8_t G6_SQ_TERMS[6][4][4] = {
 DEVN void g6_sqr(Fp2& f, int k, uint32_t* lds) {
   uint32_t* A = lds;
   __syncthreads();
-  if (k < 6) g6_put(A + 24 * k, f);
+  if (k < 6) g6_put(A + G6_F2 * k, f);
   __syncthreads();
   if (k < 6) {
     Fp2 acc, x, y, p;
@@ -82,8 +84,8 @@ DEVN void g6_sqr(Fp2& f, int k, uint32_t* lds) {
       const int i = G6_SQ_TERMS[k][t][0];
       if (i < 0) continue;
       const int j = G6_SQ_TERMS[k][t][1];
-      g6_get(x, A + 24 * i);
-      g6_get(y, A + 24 * j);
+      g6_get(x, A + G6_F2 * i);
+      g6_get(y, A + G6_F2 * j);
       fp2_mul(p, x, y);
       if (G6_SQ_TERMS[k][t][2]) fp2_dbl(p, p);
       if (G6_SQ_TERMS[k][t][3]) fp2_mul_xi(p, p);
@@ -97,16 +99,16 @@ DEVN void g6_sqr(Fp2& f, int k, uint32_t* lds) {
 DEVN void g6_mul_line(Fp2& f, const Fp2& l0, const Fp2& l2, const Fp& l3, int k, uint32_t* lds) {
   uint32_t* A = lds;
   __syncthreads();
-  if (k < 6) g6_put(A + 24 * k, f);
+  if (k < 6) g6_put(A + G6_F2 * k, f);
   __syncthreads();
   if (k < 6) {
     Fp2 acc, x, p;
     fp2_mul(acc, f, l0);
-    g6_get(x, A + 24 * ((k + 4) % 6));
+    g6_get(x, A + G6_F2 * ((k + 4) % 6));
     fp2_mul(p, x, l2);
     if (k < 2) fp2_mul_xi(p, p);
     fp2_add(acc, acc, p);
-    g6_get(x, A + 24 * ((k + 3) % 6));
+    g6_get(x, A + G6_F2 * ((k + 3) % 6));
     fp2_mul_fp(p, x, l3);
     if (k < 3) fp2_mul_xi(p, p);
     fp2_add(f, acc, p);
@@ -121,14 +123,14 @@ DEV void g6_conj(Fp2& f, int k) {
 DEVN void g6_frob(Fp2& f, int k) {
   if (k >= 6) return;
   Fp2 g, t;
-  fp2_load_const(g, bls::FROB1 + 24 * k);
+  fp2_load_const(g, bls::FROB1 + 2 * FPW * k);
   fp2_conj(t, f);
   fp2_mul(f, t, g);
 }
 DEVN void g6_frob2(Fp2& f, int k) {
   if (k >= 6) return;
   Fp g;
-  fp_load_const(g, bls::FROB2 + 12 * k);
+  fp_load_const(g, bls::FROB2 + FPW * k);
   fp2_mul_fp(f, f, g);
 }
 
@@ -139,17 +141,17 @@ DEVN void g6_frob2(Fp2& f, int k) {
 //   w1' = 3 xi s21 + 2 w1, w4' = 3 s20 - 2 w4  (pair 2)
 DEVN void g6_cyc_sqr(Fp2& f, int k, uint32_t* lds) {
   uint32_t* A = lds;            // inputs
-  uint32_t* S = lds + 6 * 24;   // pair results: s_p0 at 2p, s_p1 at 2p + 1
+  uint32_t* S = lds + 6 * G6_F2;   // pair results: s_p0 at 2p, s_p1 at 2p + 1
   __syncthreads();
-  if (k < 6) g6_put(A + 24 * k, f);
+  if (k < 6) g6_put(A + G6_F2 * k, f);
   __syncthreads();
   if (k < 3) {
     Fp2 a, b, s0, s1;
-    g6_get(a, A + 24 * k);
-    g6_get(b, A + 24 * (k + 3));
+    g6_get(a, A + G6_F2 * k);
+    g6_get(b, A + G6_F2 * (k + 3));
     fp4_sqr(s0, s1, a, b);
-    g6_put(S + 24 * (2 * k), s0);
-    g6_put(S + 24 * (2 * k + 1), s1);
+    g6_put(S + G6_F2 * (2 * k), s0);
+    g6_put(S + G6_F2 * (2 * k + 1), s1);
   }
   __syncthreads();
   if (k < 6) {
@@ -157,7 +159,7 @@ DEVN void g6_cyc_sqr(Fp2& f, int k, uint32_t* lds) {
     const int pair = (k == 0 || k == 3) ? 0 : ((k == 2 || k == 5) ? 1 : 2);
     const int comp = (k == 3 || k == 5 || k == 1) ? 1 : 0;
     Fp2 s, t;
-    g6_get(s, S + 24 * (2 * pair + comp));
+    g6_get(s, S + G6_F2 * (2 * pair + comp));
     if (k == 1) fp2_mul_xi(s, s);
     fp2_dbl(t, s);
     fp2_add(t, t, s);  // 3 s
@@ -174,26 +176,26 @@ DEVN void g6_cyc_sqr(Fp2& f, int k, uint32_t* lds) {
 DEVN void g6_inv(Fp2& f, int k, uint32_t* lds) {
   uint32_t* A = lds;
   __syncthreads();
-  if (k < 6) g6_put(A + 24 * k, f);
+  if (k < 6) g6_put(A + G6_F2 * k, f);
   __syncthreads();
   if (k == 0) {
     Fp12 x, y;
     g6_get(x.c0.c0, A + 0);
-    g6_get(x.c1.c0, A + 24);
-    g6_get(x.c0.c1, A + 48);
-    g6_get(x.c1.c1, A + 72);
-    g6_get(x.c0.c2, A + 96);
-    g6_get(x.c1.c2, A + 120);
+    g6_get(x.c1.c0, A + G6_F2);
+    g6_get(x.c0.c1, A + 2 * G6_F2);
+    g6_get(x.c1.c1, A + 3 * G6_F2);
+    g6_get(x.c0.c2, A + 4 * G6_F2);
+    g6_get(x.c1.c2, A + 5 * G6_F2);
     fp12_inv(y, x);
     g6_put(A + 0, y.c0.c0);
-    g6_put(A + 24, y.c1.c0);
-    g6_put(A + 48, y.c0.c1);
-    g6_put(A + 72, y.c1.c1);
-    g6_put(A + 96, y.c0.c2);
-    g6_put(A + 120, y.c1.c2);
+    g6_put(A + G6_F2, y.c1.c0);
+    g6_put(A + 2 * G6_F2, y.c0.c1);
+    g6_put(A + 3 * G6_F2, y.c1.c1);
+    g6_put(A + 4 * G6_F2, y.c0.c2);
+    g6_put(A + 5 * G6_F2, y.c1.c2);
   }
   __syncthreads();
-  if (k < 6) g6_get(f, A + 24 * k);
+  if (k < 6) g6_get(f, A + G6_F2 * k);
 }
 
 // a^x (x = -|x|) for a cyclotomic element
@@ -285,7 +287,7 @@ DEVN void g6_miller2(Fp2& f, int k, uint32_t* lds, const G1A* ps, const bool* sk
 
 // 1 when every lane's slot matches the identity (result valid on all lanes)
 DEVN bool g6_is_one(const Fp2& f, int k, uint32_t* lds) {
-  uint32_t* F = lds + 2 * 6 * 24;
+  uint32_t* F = lds + 2 * 6 * G6_F2;
   __syncthreads();
   if (k < 6) {
     bool ok;
