@@ -14,10 +14,15 @@ All inputs (keys, signatures, proofs) are produced by the library itself
 region only launches the staged Verify and ProofVerify pipelines on their two
 streams and waits for both.
 
+After the timed steps, a serving run pushes an open-loop Poisson stream
+(20k req/s, 40% Verify / 30% ProofVerify / 20% Sign / 10% ProofGen, 64 issuers,
+L log-uniform in [1, 128], 60-200 byte statements) through the library's
+continuous-batching server and reports p50/p99 latency at that fixed load.
+
 Prints one JSON line with the metric, a roofline object for the dominant
-kernel (VALU v_mad_u64_u32 bound), and the CPU baseline (the
-oracle's restatement of the reference CPU path, timed on the host's cores on a
-bounded sample of the same workload).
+kernel (VALU v_mad_u64_u32 bound), the latency object of the serving run, and
+the CPU baseline (the oracle's restatement of the reference CPU path, timed on
+the host's cores on a bounded sample of the same workload).
 """
 import argparse
 import concurrent.futures as cf
@@ -38,7 +43,8 @@ N_PROOFS = 2048
 PROOF_L = 32
 # algorithmic work model (Fp multiplications) of one 2-pairing check with
 # prepared G2 lines: two Miller loops (~5,200 FpM each) and one final
-# exponentiation (~11,000 FpM); one FpM = 300 v_mad_u64_u32 (12x32-bit CIOS).
+# exponentiation (~11,000 FpM); one FpM is valued at 300 v_mad_u64_u32 (the
+# canonical 12x32-bit CIOS count; the library's 14x28-bit product uses 392).
 FPM_PER_PAIRING_CHECK = 2 * 5200 + 11000
 MADS_PER_FPM = 300
 
@@ -132,6 +138,85 @@ def cpu_baseline(vreqs, pvreqs, budget_s=20.0):
                       "%d threads)" % (nv, len(sample) - nv, threads)}
 
 
+def serving_run(g, rate=20000.0, warm_s=3.0, steady_s=15.0, seed=b"cubbs-hip/serving"):
+    """Open-loop serving load through the library's continuous-batching server:
+    Poisson arrivals at `rate` req/s, mix 40% Verify / 30% ProofVerify / 20% Sign
+    / 10% ProofGen, 64 issuer keys, message count log-uniform in [1, 128] with
+    60-200 byte statements.  Requests are drawn from a pool of distinct
+    prepared requests; latency is arrival -> completion, percentiles over the
+    steady window (after warm_s)."""
+    rng = random.Random(hashlib.sha256(seed).digest())
+    rc, kps = g.keygen([rng.randbytes(32) for _ in range(64)])
+    assert rc == 0
+
+    def msgs():
+        return [rng.randbytes(rng.randint(60, 200)) for _ in range(log_uniform_L(rng))]
+
+    n_pool = {"v": 2048, "pv": 1024, "s": 1024, "pg": 512}
+    base = [(kps[rng.randrange(64)], rng.randbytes(64), msgs())
+            for _ in range(n_pool["v"] + n_pool["pv"] + n_pool["pg"])]
+    rc, sigs = g.sign_batch([(sk, pk, h, m) for (sk, pk), h, m in base])
+    assert rc == 0 and all(c == 0 for c, _ in sigs)
+    pools = {"v": [], "pv": [], "s": [], "pg": []}
+    for i in range(n_pool["v"]):
+        (sk, pk), h, m = base[i]
+        pools["v"].append((g.REQ_VERIFY, (pk, sigs[i][1], h, m), None))
+    off = n_pool["v"]
+    greqs, gseeds = [], []
+    for i in range(off, off + n_pool["pv"]):
+        (sk, pk), h, m = base[i]
+        idx = sorted(rng.sample(range(len(m)), len(m) // 4))
+        greqs.append((pk, sigs[i][1], h, rng.randbytes(64), m, idx))
+        gseeds.append(rng.randbytes(32))
+    rc, proofs = g.proof_gen_batch(greqs, seeds=gseeds)
+    assert rc == 0 and all(c == 0 for c, _ in proofs)
+    for (pk, sig, h, ph, m, idx), (_, proof) in zip(greqs, proofs):
+        pools["pv"].append((g.REQ_PROOF_VERIFY, (pk, proof, h, ph, [m[i] for i in idx], idx, len(m)), None))
+    for _ in range(n_pool["s"]):
+        sk, pk = kps[rng.randrange(64)]
+        pools["s"].append((g.REQ_SIGN, (sk, pk, rng.randbytes(64), msgs()), None))
+    off += n_pool["pv"]
+    for i in range(off, off + n_pool["pg"]):
+        (sk, pk), h, m = base[i]
+        idx = sorted(rng.sample(range(len(m)), len(m) // 4))
+        pools["pg"].append((g.REQ_PROOF_GEN, (pk, sigs[i][1], h, rng.randbytes(64), m, idx), rng.randbytes(32)))
+    pool = pools["v"] + pools["pv"] + pools["s"] + pools["pg"]
+    starts = {"v": 0, "pv": n_pool["v"], "s": n_pool["v"] + n_pool["pv"],
+              "pg": n_pool["v"] + n_pool["pv"] + n_pool["s"]}
+    kinds = ["v", "pv", "s", "pg"]
+    weights = [0.4, 0.3, 0.2, 0.1]
+    n = int(rate * (warm_s + steady_s))
+    t, arrivals, order = 0.0, [], []
+    for _ in range(n):
+        t += rng.expovariate(rate)
+        arrivals.append(int(t * 1e9))
+        k = rng.choices(kinds, weights)[0]
+        order.append(starts[k] + rng.randrange(n_pool[k]))
+    srv = g.Server()
+    try:
+        # warm the server (generator tables up to L = 128, issuer keys resident)
+        srv.run_open_loop(pool, list(range(0, len(pool), 7)), [0] * len(range(0, len(pool), 7)))
+        t0 = time.perf_counter()
+        rc, res, lat, _ = srv.run_open_loop(pool, order, arrivals)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert rc == 0, "serving run failed with status %d" % rc
+    bad = sum(1 for (c, v), k in zip(res, order)
+              if c != 0 or (pool[k][0] in (g.REQ_VERIFY, g.REQ_PROOF_VERIFY) and v != 1))
+    assert bad == 0, "%d serving results differ from expectation" % bad
+    steady = sorted(l for l, a in zip(lat, arrivals) if a >= warm_s * 1e9)
+    pct = lambda q: steady[min(len(steady) - 1, int(q * len(steady)))] / 1e6
+    vps = sum(1 for k, a in zip(order, arrivals) if a >= warm_s * 1e9 and pool[k][0] in (0, 1))
+    return {"offered_rps": rate, "mix": "40% Verify, 30% ProofVerify, 20% Sign, 10% ProofGen",
+            "issuers": 64, "requests": n, "warmup_s": warm_s, "window_s": steady_s,
+            "p50_ms": round(pct(0.50), 2), "p99_ms": round(pct(0.99), 2), "max_ms": round(steady[-1] / 1e6, 2),
+            "achieved_rps": round(n / wall, 1), "verify_proofverify_rps": round(vps / steady_s, 1),
+            "batches": st["batches"], "avg_batch": round(st["requests"] / max(1, st["batches"]), 1),
+            "max_concurrent_batches": st["max_concurrent_batches"]}
+
+
 def per_request_gpu(g, vreqs, n=16):
     """Plain per-request dispatch of the same kernels: batch = 1, synchronous."""
     t = time.time()
@@ -147,6 +232,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency run")
+    ap.add_argument("--serving-rate", type=float, default=20000.0)
+    ap.add_argument("--serving-seconds", type=float, default=15.0)
     ap.add_argument("--profile-only", action="store_true", help="a few launches, no baselines")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (counter calibration)")
@@ -226,6 +314,10 @@ def main():
                          "proof_verify": {k: round(v, 3) for k, v in pk.items()}}}
     if not args.profile_only:
         out["per_request_gpu_ops_s"] = round(per_request_gpu(g, vreqs), 2)
+        if not args.no_serving:
+            t = time.time()
+            out["latency"] = serving_run(g, rate=args.serving_rate, steady_s=args.serving_seconds)
+            log("serving run %.1fs: %s" % (time.time() - t, out["latency"]))
         if not args.no_cpu_baseline:
             out["cpu_baseline"] = cpu_baseline(vreqs, pvreqs)
     vb.free()

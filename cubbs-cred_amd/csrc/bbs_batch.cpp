@@ -34,6 +34,47 @@ static const char* kApiId = "BBS_BLS12381G1_XMD:SHA-256_SSWU_RO_H2G_HM2S_";
 
 uint32_t comb_chunk_terms() { return std::max<uint32_t>(1, env_u32("CUBBS_HIP_COMB_CHUNK", 4)); }
 
+// ------------------------------------------------------------------ arenas
+static thread_local Arena* tl_arena = nullptr;
+void set_thread_arena(Arena* a) { tl_arena = a; }
+
+static uint8_t* arena_take(std::vector<Arena::Block>& blocks, size_t& held, size_t bytes,
+                           size_t min_block, bool host) {
+  bytes = align256(std::max<size_t>(bytes, 1));
+  for (auto& bl : blocks) {
+    if (bl.cap - bl.used >= bytes) {
+      uint8_t* p = bl.p + bl.used;
+      bl.used += bytes;
+      return p;
+    }
+  }
+  const size_t cap = std::max(bytes, min_block);
+  uint8_t* p = nullptr;
+  if (host)
+    check(hipHostMalloc(&p, cap, hipHostMallocDefault));
+  else
+    check(hipMalloc(&p, cap));
+  blocks.push_back(Arena::Block{p, cap, bytes});
+  held += cap;
+  return p;
+}
+uint8_t* Arena::alloc_dev(size_t bytes) { return arena_take(dev, dev_bytes, bytes, min_block, false); }
+uint8_t* Arena::alloc_host(size_t bytes) {
+  return arena_take(host, host_bytes, bytes, min_block / 4, true);
+}
+void Arena::reset() {
+  for (auto& bl : dev) bl.used = 0;
+  for (auto& bl : host) bl.used = 0;
+}
+Arena::~Arena() {
+  for (hipStream_t s : streams)
+    if (s) (void)hipStreamSynchronize(s);
+  for (auto& bl : dev) (void)hipFree(bl.p);
+  for (auto& bl : host) (void)hipHostFree(bl.p);
+  for (hipStream_t s : streams)
+    if (s) (void)hipStreamDestroy(s);
+}
+
 // ------------------------------------------------------------------ planning helpers
 template <class T>
 static size_t put(std::vector<uint8_t>& blob, const T* p, size_t n) {
@@ -118,9 +159,16 @@ static void materialize(Batch& b, Plan& p) {
   size_t o_dst = place(p.map_dst.data(), p.map_dst.size());
   size_t o_blob = place(p.blob.data(), std::max<size_t>(p.blob.size(), 1));
   b.in_bytes = align256(H.size());
-  check(hipHostMalloc(&b.h_in, b.in_bytes, hipHostMallocDefault));
+  Arena* ar = tl_arena;
+  b.from_arena = ar != nullptr;
+  if (ar) {
+    b.h_in = ar->alloc_host(b.in_bytes);
+    b.d_in = ar->alloc_dev(b.in_bytes);
+  } else {
+    check(hipHostMalloc(&b.h_in, b.in_bytes, hipHostMallocDefault));
+    check(hipMalloc(&b.d_in, b.in_bytes));
+  }
   std::memcpy(b.h_in, H.data(), H.size());
-  check(hipMalloc(&b.d_in, b.in_bytes));
   uint8_t* D = b.d_in;
   b.h2s = reinterpret_cast<H2sJob*>(D + o_h2s);
   b.n_h2s = (uint32_t)p.h2s.size();
@@ -177,7 +225,10 @@ static void materialize(Batch& b, Plan& p) {
   size_t o_va = take(std::max<uint32_t>(b.nv, 1));
   size_t o_out = take(std::max<size_t>(p.out_bytes, 1));
   b.work_bytes = w;
-  check(hipMalloc(&b.d_work, w));
+  if (ar)
+    b.d_work = ar->alloc_dev(w);
+  else
+    check(hipMalloc(&b.d_work, w));
   uint8_t* W = b.d_work;
   b.scalars = reinterpret_cast<uint32_t*>(W + o_sc);
   b.points = reinterpret_cast<uint32_t*>(W + o_pt);
@@ -191,7 +242,10 @@ static void materialize(Batch& b, Plan& p) {
   b.n_scalars = p.n_scalars;
   b.n_points = p.n_points;
   b.n_partials = p.n_partials;
-  check(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+  if (ar)
+    b.stream = ar->streams[ar->cur];
+  else
+    check(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
   check(hipMemcpyAsync(b.d_in, b.h_in, b.in_bytes, hipMemcpyHostToDevice, b.stream));
 }
 
@@ -813,6 +867,7 @@ Batch::~Batch() {
     (void)hipEventDestroy(timer.starts[i]);
     (void)hipEventDestroy(timer.stops[i]);
   }
+  if (from_arena) return;  // the arena owns memory and stream
   if (h_in) (void)hipHostFree(h_in);
   if (d_in) (void)hipFree(d_in);
   if (d_work) (void)hipFree(d_work);

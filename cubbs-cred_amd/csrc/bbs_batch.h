@@ -24,9 +24,33 @@ struct KernelTimer {
   std::vector<size_t> event_name;
 };
 
+// Reusable staging memory for repeated batches on one stream (a serving slot).
+// Device and pinned host blocks are kept across batches; reset() recycles
+// them once the previous batch on the stream has completed, so steady-state
+// batches allocate nothing.
+struct Arena {
+  struct Block {
+    uint8_t* p;
+    size_t cap, used;
+  };
+  hipStream_t streams[4] = {};  // one per operation kind, so sub-batches overlap
+  int cur = 0;                   // stream used by the next planned batch
+  std::vector<Block> dev, host;
+  size_t dev_bytes = 0, host_bytes = 0;  // held in blocks
+  size_t min_block = 64ull << 20;
+  uint8_t* alloc_dev(size_t bytes);
+  uint8_t* alloc_host(size_t bytes);
+  void reset();
+  ~Arena();
+};
+// While set on a thread, batches planned on that thread take their memory and
+// stream from the arena instead of allocating their own.
+void set_thread_arena(Arena* a);
+
 struct Batch {
   int kind = 0;
   hipStream_t stream = nullptr;
+  bool from_arena = false;           // memory and stream belong to an Arena
   size_t n_total = 0;                // items passed in
   std::vector<int32_t> pre_code;     // per item: -1 = decided on the device
   std::vector<uint8_t> pre_value;    // value for items decided on the host

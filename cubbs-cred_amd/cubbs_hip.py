@@ -51,6 +51,12 @@ EXT_SYMBOLS = [
     "cubbs_hip_mad_peak_bench",
     "cubbs_hip_proof_gen_seeded_batch",
     "cubbs_hip_proof_gen_with_scalars_batch",
+    "cubbs_hip_server_create",
+    "cubbs_hip_server_submit",
+    "cubbs_hip_server_wait",
+    "cubbs_hip_server_run_open_loop",
+    "cubbs_hip_server_stats",
+    "cubbs_hip_server_destroy",
 ]
 
 
@@ -103,6 +109,18 @@ class MessagesToScalarItem(ctypes.Structure):
     _fields_ = [("message", ByteSlice), ("dst", ByteSlice), ("scalar_out", ByteBuffer)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
+                ("hbm_budget_bytes", ctypes.c_uint64)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("item", ctypes.c_void_p), ("seed32", ctypes.c_void_p)]
+
+
+REQ_VERIFY, REQ_PROOF_VERIFY, REQ_SIGN, REQ_PROOF_GEN = 0, 1, 2, 3
+
+
 _lib = None
 
 
@@ -125,6 +143,18 @@ def lib():
                                                  ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int,
                                                  ctypes.c_void_p]
         _lib.cubbs_hip_batch_shape.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+        _lib.cubbs_hip_server_create.restype = ctypes.c_void_p
+        _lib.cubbs_hip_server_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+        _lib.cubbs_hip_server_submit.restype = ctypes.c_int64
+        _lib.cubbs_hip_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _lib.cubbs_hip_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+        _lib.cubbs_hip_server_run_open_loop.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                        ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                                                        ctypes.c_void_p, ctypes.c_void_p]
+        _lib.cubbs_hip_server_stats.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+        _lib.cubbs_hip_server_destroy.argtypes = [ctypes.c_void_p]
+        _lib.cubbs_hip_server_destroy.restype = None
     return _lib
 
 
@@ -444,5 +474,139 @@ class StagedBatch:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+def _request_item(kind, payload, keep):
+    """One ABI item for a server request (output buffers are not needed)."""
+    if kind == REQ_VERIFY:
+        return build_verify_items([payload], keep)
+    if kind == REQ_PROOF_VERIFY:
+        return build_proof_verify_items([payload], keep)
+    if kind == REQ_SIGN:
+        sk, pk, hdr, msgs = payload
+        it = (SignItem * 1)()
+        it[0].secret_key = keep.slice(sk)
+        it[0].public_key = keep.slice(pk)
+        it[0].header = keep.slice(hdr)
+        it[0].messages = keep.slices(msgs)
+        it[0].message_count = len(msgs)
+        keep.refs.append(it)
+        return it
+    if kind == REQ_PROOF_GEN:
+        pk, sig, hdr, ph, msgs, didx = payload
+        it = (ProofGenItem * 1)()
+        it[0].public_key = keep.slice(pk)
+        it[0].signature = keep.slice(sig)
+        it[0].header = keep.slice(hdr)
+        it[0].presentation_header = keep.slice(ph)
+        it[0].messages = keep.slices(msgs)
+        it[0].message_count = len(msgs)
+        it[0].disclosed_indexes = keep.indexes(didx)
+        keep.refs.append(it)
+        return it
+    raise ValueError("unknown request kind %r" % kind)
+
+
+def _out_cap(kind, payload):
+    if kind == REQ_SIGN:
+        return 80
+    if kind == REQ_PROOF_GEN:
+        return 272 + 32 * max(0, len(payload[4]) - len(payload[5]))
+    return 0
+
+
+class Server:
+    """The library's serving loop (continuous batching over device slots).
+
+    Requests are (kind, payload, seed) with payloads shaped as for
+    verify_batch / proof_verify_batch / sign_batch / proof_gen_batch and seed a
+    32-byte ProofGen blinding seed or None."""
+
+    def __init__(self, slots=0, max_batch=0, hbm_budget=0):
+        cfg = ServerConfig(slots, max_batch, hbm_budget)
+        st = ctypes.c_int32(0)
+        self.h = lib().cubbs_hip_server_create(ctypes.byref(cfg), ctypes.byref(st))
+        if not self.h:
+            raise RuntimeError("server create failed with status %d" % st.value)
+
+    def _requests(self, pool, keep):
+        reqs = (Request * max(1, len(pool)))()
+        for i, (kind, payload, seed) in enumerate(pool):
+            it = _request_item(kind, payload, keep)
+            sd = None
+            if seed is not None:
+                buf = ctypes.create_string_buffer(bytes(seed), 32)
+                keep.refs.append(buf)
+                sd = ctypes.cast(buf, ctypes.c_void_p)
+            reqs[i] = Request(kind, ctypes.cast(it, ctypes.c_void_p), sd)
+        keep.refs.append(reqs)
+        return reqs
+
+    def submit(self, kind, payload, seed=None):
+        keep = _Keep()
+        reqs = self._requests([(kind, payload, seed)], keep)
+        t = lib().cubbs_hip_server_submit(self.h, ctypes.byref(reqs[0]))
+        if t < 0:
+            raise RuntimeError("submit rejected with code %d" % -t)
+        return t, _out_cap(kind, payload)
+
+    def wait(self, ticket):
+        """ticket as returned by submit -> (code, value, output bytes, latency_ns)."""
+        t, cap = ticket
+        res = Result()
+        out = ctypes.create_string_buffer(max(1, cap))
+        lat = ctypes.c_uint64()
+        rc = lib().cubbs_hip_server_wait(self.h, t, ctypes.byref(res), out, cap, ctypes.byref(lat))
+        if rc:
+            raise RuntimeError("wait failed %d" % rc)
+        return res.code, res.value, out.raw[:res.bytes_written], lat.value
+
+    def run_open_loop(self, pool, order, arrivals_ns, outputs=False):
+        """Submit pool[order[i]] at arrivals_ns[i]; returns (status, results,
+        latencies_ns, outputs) with results [(code, value)]."""
+        keep = _Keep()
+        preqs = self._requests(pool, keep)
+        n = len(order)
+        reqs = (Request * max(1, n))()
+        for i, k in enumerate(order):
+            reqs[i] = preqs[k]
+        arr = (ctypes.c_uint64 * max(1, n))(*arrivals_ns)
+        res = (Result * max(1, n))()
+        lat = (ctypes.c_uint64 * max(1, n))()
+        outs_p = caps_p = None
+        bufs = []
+        if outputs:
+            outs_p = (ctypes.c_void_p * max(1, n))()
+            caps_p = (ctypes.c_size_t * max(1, n))()
+            for i, k in enumerate(order):
+                cap = _out_cap(pool[k][0], pool[k][1])
+                b = ctypes.create_string_buffer(max(1, cap))
+                bufs.append(b)
+                outs_p[i] = ctypes.cast(b, ctypes.c_void_p)
+                caps_p[i] = cap
+        rc = lib().cubbs_hip_server_run_open_loop(self.h, reqs, arr, ctypes.c_size_t(n), res, lat,
+                                                  outs_p, caps_p)
+        results = [(res[i].code, res[i].value) for i in range(n)]
+        outv = [bufs[i].raw[:res[i].bytes_written] for i in range(n)] if outputs else None
+        return rc, results, [lat[i] for i in range(n)], outv
+
+    def stats(self):
+        v = [ctypes.c_uint64() for _ in range(4)]
+        rc = lib().cubbs_hip_server_stats(self.h, *[ctypes.byref(x) for x in v])
+        d = dict(zip(["batches", "requests", "max_concurrent_batches", "peak_slot_bytes"],
+                     [x.value for x in v]))
+        d["fault"] = rc
+        return d
+
+    def close(self):
+        if self.h:
+            lib().cubbs_hip_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

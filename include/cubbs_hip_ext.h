@@ -59,6 +59,55 @@ int32_t cubbs_hip_batch_profile(void *batch, int iters, char *names_out, size_t 
 int32_t cubbs_hip_batch_shape(void *batch, uint64_t *items, uint64_t *msm_terms,
                               uint64_t *comb_madds, uint64_t *pairings);
 
+/* ---- serving loop: continuous batching of mixed requests.
+ * `slots` worker slots, each with its own stream and reusable device/pinned
+ * memory; a free slot takes every pending request up to max_batch and its
+ * share of the HBM budget (hbm_budget_bytes / slots), runs one sub-batch per
+ * operation kind and completes them, while other slots' batches run on the
+ * device.  Zero fields take the defaults (3 slots, 4096, 4 GiB). */
+typedef struct CubbsHipServerConfig {
+  uint32_t slots;
+  uint32_t max_batch;
+  uint64_t hbm_budget_bytes;
+} CubbsHipServerConfig;
+
+#define CUBBS_HIP_REQ_VERIFY 0
+#define CUBBS_HIP_REQ_PROOF_VERIFY 1
+#define CUBBS_HIP_REQ_SIGN 2
+#define CUBBS_HIP_REQ_PROOF_GEN 3
+
+/* One request: `item` points to the ABI item of the kind (VerifyItem,
+ * ProofVerifyItem, SignItem, ProofGenItem); output buffers inside the item are
+ * ignored (the server keeps the output until it is waited for).  ProofGen
+ * blinding: ChaCha20 with seed32 as in cubbs_hip_proof_gen_seeded_batch, or
+ * the OS CSPRNG when seed32 is NULL. */
+typedef struct CubbsHipRequest {
+  int32_t kind;
+  const void *item;
+  const uint8_t *seed32;
+} CubbsHipRequest;
+
+void *cubbs_hip_server_create(const CubbsHipServerConfig *cfg, int32_t *status_out);
+/* Copies the request and queues it.  Returns a ticket >= 0, or -code. */
+int64_t cubbs_hip_server_submit(void *server, const CubbsHipRequest *request);
+/* Blocks until the ticket completes; writes its result, its output bytes
+ * (signature or proof, result->bytes_written of them) and the latency from
+ * submission to completion.  A ticket can be waited for once. */
+int32_t cubbs_hip_server_wait(void *server, int64_t ticket, CuBbsBbsGpuResult *result,
+                              uint8_t *out, size_t out_cap, uint64_t *latency_ns);
+/* Open-loop load: submits request i at arrival_ns[i] (non-decreasing, from the
+ * call's start) on the calling thread, then waits for all.  Latency is measured
+ * from the scheduled arrival.  outs/out_caps may be NULL (outputs dropped). */
+int32_t cubbs_hip_server_run_open_loop(void *server, const CubbsHipRequest *requests,
+                                       const uint64_t *arrival_ns, size_t n,
+                                       CuBbsBbsGpuResult *results, uint64_t *latency_ns,
+                                       uint8_t *const *outs, const size_t *out_caps);
+/* Counters since creation; returns the server's fault code (0 = healthy). */
+int32_t cubbs_hip_server_stats(void *server, uint64_t *batches, uint64_t *requests,
+                               uint64_t *max_concurrent_batches, uint64_t *peak_slot_bytes);
+/* Drains queued requests, joins the workers and frees the server. */
+void cubbs_hip_server_destroy(void *server);
+
 /* ---- primitive probes (parity tests) */
 /* out = k * P for n compressed points and 32-byte big-endian scalars; ok[i] = 0
  * when the point or scalar does not decode. */
