@@ -224,6 +224,7 @@ static void materialize(Batch& b, Plan& p) {
   size_t o_ga = take(std::max<uint32_t>(b.nv, 1));
   size_t o_va = take(std::max<uint32_t>(b.nv, 1));
   size_t o_out = take(std::max<size_t>(p.out_bytes, 1));
+  size_t o_l2 = take(4ull * PAIR_L2_WORDS * std::max<size_t>(p.pairs.size(), 1));
   b.work_bytes = w;
   if (ar)
     b.d_work = ar->alloc_dev(w);
@@ -238,6 +239,7 @@ static void materialize(Batch& b, Plan& p) {
   b.gate = W + o_ga;
   b.valid = W + o_va;
   b.out = W + o_out;
+  b.l2buf = reinterpret_cast<uint32_t*>(W + o_l2);
   b.out_bytes = p.out_bytes;
   b.n_scalars = p.n_scalars;
   b.n_points = p.n_points;
@@ -810,9 +812,13 @@ void launch_batch(Batch& b) {
                         b.pinf, b.scalars, b.gate);
   }
   if (b.n_pairs) {
+    {
+      Timed t(b, "pair_lines");
+      launch_pair_lines(s, b.pairs, b.n_pairs, b.points, b.pk_lines, b.neg_g2_lines, b.l2buf);
+    }
     Timed t(b, "pairing");
     launch_pairing(s, b.pairs, b.n_pairs, b.points, b.pinf, b.pk_status, b.pk_lines,
-                   b.neg_g2_lines, b.status, b.gate, b.valid);
+                   b.neg_g2_lines, b.l2buf, b.status, b.gate, b.valid);
   }
   check(hipGetLastError());
 }

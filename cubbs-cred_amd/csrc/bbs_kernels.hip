@@ -4,7 +4,7 @@
 #include "bbs_internal.h"
 #include "dev_io.hpp"
 #include "h2c.hpp"
-#include "pairing6.hpp"
+#include "pairing12.hpp"
 
 namespace cubbs {
 
@@ -375,22 +375,27 @@ void launch_test_g1_mul(hipStream_t s, const uint8_t* pts, const uint8_t* scalar
   hipLaunchKernelGGL(k_test_g1_mul, GRID(n, 64), 0, s, pts, scalars, n, out, ok);
 }
 
-// e(P, Q) as a GT value through the same 6-lane Miller loop / final
-// exponentiation as the pairing checks (one 8-lane group per pair).
+// e(P, Q) as a GT value through the same 12-lane Miller loop / final
+// exponentiation as the pairing checks (pairing12.hpp, 5 pairs per block).
+// The group's lanes share the line work: lane 0 prepares Q's lines, then
+// every lane evaluates -lambda x_P for its share of the 68 lines.
 __global__ __launch_bounds__(64) void k_test_pairing_gt(const uint8_t* g1, const uint8_t* g2,
                                                        uint32_t n, uint8_t* out, uint8_t* ok,
-                                                       uint32_t* lines_scratch) {
-  __shared__ uint32_t lds_all[8 * G6_LDS_WORDS];
-  const int k = threadIdx.x & 7;
-  const int grp = threadIdx.x >> 3;
-  uint32_t* lds = lds_all + grp * G6_LDS_WORDS;
-  const uint32_t i = blockIdx.x * 8 + grp;
+                                                       uint32_t* scratch) {
+  __shared__ uint32_t lds_all[p12::GROUPS * p12::LDS_WORDS];
+  const int lane = threadIdx.x;
+  const int grp = lane / 12, gl = lane % 12;
+  const p12::Lane L = p12::lane_setup(lds_all, lane);
+  const uint32_t i = blockIdx.x * p12::CHECKS_PER_BLOCK + grp;
+  const bool mine = grp < p12::CHECKS_PER_BLOCK && i < n;
+  const size_t per = PK_LINES_WORDS + p12::L2_WORDS;
+  uint32_t* lines = scratch + (size_t)(mine ? i : 0) * per;
+  uint32_t* l2 = lines + PK_LINES_WORDS;
   bool good = false;
   G1A pa;
   fp_zero(pa.x);
   fp_zero(pa.y);
-  Line* lines = reinterpret_cast<Line*>(lines_scratch + (size_t)(i < n ? i : 0) * PK_LINES_WORDS);
-  if (i < n) {
+  if (mine) {
     G1J p;
     G2J q;
     good = g1_decompress(p, g1 + (size_t)48 * i) && g2_decompress(q, g2 + (size_t)96 * i);
@@ -398,39 +403,49 @@ __global__ __launch_bounds__(64) void k_test_pairing_gt(const uint8_t* g1, const
     if (good) {
       pa.x = p.x;
       pa.y = p.y;
-      if (k == 0) {
+      if (gl == 0) {
         G2A qa;
         pt_to_affine(qa, q);
-        g2_prepare_lines(lines, qa);
+        g2_prepare_lines(reinterpret_cast<Line*>(lines), qa);
       }
     }
   }
   __syncthreads();
   __threadfence_block();
-  G1A ps[2] = {pa, pa};
-  bool skip[2] = {!good, true};
-  const Line* lp[2] = {lines, lines};
-  Fp2 f, r;
-  g6_miller2(f, k, lds, ps, skip, lp);
-  g6_final_exp(r, f, k, lds);
-  if (i < n && k < 6) {
-    if (k == 0) ok[i] = good;
+  if (mine && good) {
+    for (int e = gl; e < MILLER_LINES * 2; e += 12) {
+      const int idx = e >> 1, comp = e & 1;
+      Fp lam, v;
+      fp_load_const(lam, lines + (size_t)idx * MILLER_LINE_WORDS + comp * FPW);
+      fp_mul(v, lam, pa.x);
+      fp_neg(v, v);
+      p12::st(l2 + ((size_t)idx * 2) * 2 * FPW + comp * FPW, v);
+    }
+  }
+  __syncthreads();
+  __threadfence_block();
+  Fp f, r, zero;
+  fp_zero(zero);
+  p12::miller(f, lines, lines, l2, pa.y, zero, !good, true, L);
+  p12::final_exp(r, f, L);
+  if (mine) {
+    if (gl == 0) ok[i] = good;
     // slot k is tower coordinate c_{k mod 2}.c_{k div 2}; output order c0.c0, c0.c1, c0.c2, c1.c0, ..
-    const int pos = (k & 1) * 3 + (k >> 1);
+    const int pos = (L.k & 1) * 3 + (L.k >> 1);
     Fp c;
-    fp_from_mont(c, r.c0);
-    fp_to_be_bytes(out + (size_t)576 * i + 96 * pos, c);
-    fp_from_mont(c, r.c1);
-    fp_to_be_bytes(out + (size_t)576 * i + 96 * pos + 48, c);
+    fp_from_mont(c, r);
+    fp_to_be_bytes(out + (size_t)576 * i + 96 * pos + 48 * L.c, c);
   }
 }
 void launch_test_pairing_gt(hipStream_t s, const uint8_t* g1, const uint8_t* g2, uint32_t n,
                             uint8_t* out, uint8_t* ok) {
   if (!n) return;
   uint32_t* scratch = nullptr;
-  if (hipMalloc(&scratch, (size_t)n * PK_LINES_WORDS * 4) != hipSuccess) return;
-  hipLaunchKernelGGL(k_test_pairing_gt, dim3((n + 7) / 8), dim3(64), 0, s, g1, g2, n, out, ok,
-                     scratch);
+  const size_t per = PK_LINES_WORDS + p12::L2_WORDS;
+  if (hipMalloc(&scratch, (size_t)n * per * 4) != hipSuccess) return;
+  hipLaunchKernelGGL(k_test_pairing_gt,
+                     dim3((n + p12::CHECKS_PER_BLOCK - 1) / p12::CHECKS_PER_BLOCK), dim3(64), 0, s,
+                     g1, g2, n, out, ok, scratch);
   (void)hipStreamSynchronize(s);
   (void)hipFree(scratch);
 }

@@ -6,13 +6,13 @@
 //
 // Arrays shared by all stages of a batch:
 //   scalars[] : canonical Fr (8 words); [0, T) are the fixed-base term scalars
-//   points[]  : affine G1 (Montgomery, 24 words) with pinf[] identity flags
+//   points[]  : affine G1 (Montgomery, 2 x 14 words) with pinf[] identity flags
 //   partials[]: Jacobian G1 partial sums
 //   status[]  : per item, 0 ok, 1 invalid input / crypto failure
 //   gate[]    : per item, extra condition of the pairing check (challenge)
 #include "bbs_internal.h"
 #include "dev_io.hpp"
-#include "pairing6.hpp"
+#include "pairing12.hpp"
 
 namespace cubbs {
 
@@ -168,68 +168,104 @@ void launch_reduce(hipStream_t s, const ReduceGroup* groups, const uint32_t* out
 }
 
 // ------------------------------------------------------------------ pairing check
-// e(a, PK) * e(b, -G2) == 1 (identity arguments contribute 1), one check per
-// 8-lane group (6 active lanes, see pairing6.hpp), 8 checks per 64-thread block.
+// e(a, PK) * e(b, -G2) == 1 (identity arguments contribute 1).
+//
+// k_pair_lines evaluates the item-dependent part of every Miller-loop line,
+// l2 = -lambda x_P, one lane per (check, line, pairing, component); the
+// pairing kernel then runs each check on 12 lanes (pairing12.hpp), 5 checks
+// per 64-thread block.
+__global__ __launch_bounds__(256) void k_pair_lines(const PairJob* __restrict__ jobs, uint32_t n,
+                                                   const uint32_t* __restrict__ points,
+                                                   const uint32_t* __restrict__ pk_lines,
+                                                   const uint32_t* __restrict__ neg_g2_lines,
+                                                   uint32_t* __restrict__ l2buf) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * (uint32_t)(MILLER_LINES * 4)) return;
+  const uint32_t job = t / (MILLER_LINES * 4);
+  const uint32_t r = t % (MILLER_LINES * 4);
+  const uint32_t idx = r >> 2, q = (r >> 1) & 1, comp = r & 1;
+  const PairJob j = jobs[job];
+  const uint32_t* line = (q ? neg_g2_lines : pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS) +
+                         (size_t)idx * MILLER_LINE_WORDS;
+  Fp lam, x, v;
+  fp_load_const(lam, line + comp * FPW);
+  fp_load_const(x, points + (size_t)(q ? j.b : j.a) * G1A_WORDS);
+  fp_mul(v, lam, x);
+  fp_neg(v, v);
+  p12::st(l2buf + (size_t)job * p12::L2_WORDS + ((size_t)idx * 2 + q) * 2 * FPW + comp * FPW, v);
+}
+
+void launch_pair_lines(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
+                       const uint32_t* pk_lines, const uint32_t* neg_g2_lines, uint32_t* l2buf) {
+  if (!n) return;
+  const uint32_t nl = n * (uint32_t)(MILLER_LINES * 4);
+  hipLaunchKernelGGL(k_pair_lines, GRID(nl, 256), 0, s, jobs, n, points, pk_lines, neg_g2_lines,
+                     l2buf);
+}
+
 __global__ __launch_bounds__(64) void k_pairing(const PairJob* __restrict__ jobs, uint32_t n,
                                                const uint32_t* __restrict__ points,
                                                const uint8_t* __restrict__ pinf,
                                                const uint8_t* __restrict__ pk_status,
                                                const uint32_t* __restrict__ pk_lines,
                                                const uint32_t* __restrict__ neg_g2_lines,
+                                               const uint32_t* __restrict__ l2buf,
                                                uint8_t* __restrict__ status,
                                                const uint8_t* __restrict__ gate,
                                                uint8_t* __restrict__ valid) {
-  __shared__ uint32_t lds_all[8 * G6_LDS_WORDS];
-  const int k = threadIdx.x & 7;
-  const int grp = threadIdx.x >> 3;
-  uint32_t* lds = lds_all + grp * G6_LDS_WORDS;
-  const uint32_t job = blockIdx.x * 8 + grp;
-  bool live = job < n;
+  __shared__ uint32_t lds_all[p12::GROUPS * p12::LDS_WORDS];
+  const int lane = threadIdx.x;
+  const int grp = lane / 12;
+  const p12::Lane L = p12::lane_setup(lds_all, lane);
+  const uint32_t job = blockIdx.x * p12::CHECKS_PER_BLOCK + grp;
+  const bool mine = grp < p12::CHECKS_PER_BLOCK && job < n;
+  bool live = mine;
   PairJob j{};
   uint8_t pks = 1;
-  if (live) {
+  if (mine) {
     j = jobs[job];
     pks = pk_status[j.pk_slot];
     live = status[j.item] == 0 && pks != 0 && gate[j.item];
   }
   // blocks with no live check leave together (uniform exit, no barrier skipped)
   if (!__syncthreads_or(live)) {
-    if (job < n && k == 0) {
+    if (mine && lane % 12 == 0) {
       if (pks == 0) status[j.item] = 1;
       valid[j.item] = 0;
     }
     return;
   }
-  G1A ps[2];
-  bool skip[2] = {true, true};
-  const Line* lines[2] = {reinterpret_cast<const Line*>(neg_g2_lines),
-                          reinterpret_cast<const Line*>(neg_g2_lines)};
-  fp_zero(ps[0].x);
-  fp_zero(ps[0].y);
-  ps[1] = ps[0];
+  Fp yp0, yp1;
+  fp_zero(yp0);
+  fp_zero(yp1);
+  bool skip0 = true, skip1 = true;
+  const uint32_t* lines0 = neg_g2_lines;
+  const uint32_t* l2 = l2buf;
   if (live) {
-    load_g1a(ps[0], points + (size_t)j.a * G1A_WORDS);
-    load_g1a(ps[1], points + (size_t)j.b * G1A_WORDS);
-    skip[0] = pinf[j.a] || pks == 2;
-    skip[1] = pinf[j.b];
-    lines[0] = reinterpret_cast<const Line*>(pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS);
+    fp_load_const(yp0, points + (size_t)j.a * G1A_WORDS + FPW);
+    fp_load_const(yp1, points + (size_t)j.b * G1A_WORDS + FPW);
+    skip0 = pinf[j.a] || pks == 2;
+    skip1 = pinf[j.b];
+    lines0 = pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS;
+    l2 = l2buf + (size_t)job * p12::L2_WORDS;
   }
-  Fp2 f, r;
-  g6_miller2(f, k, lds, ps, skip, lines);
-  g6_final_exp(r, f, k, lds);
-  const bool one = g6_is_one(r, k, lds);
-  if (job < n && k == 0) {
+  Fp f, r;
+  p12::miller(f, lines0, neg_g2_lines, l2, yp0, yp1, skip0, skip1, L);
+  p12::final_exp(r, f, L);
+  const bool one = p12::is_one(r, L);
+  if (mine && lane % 12 == 0) {
     if (pks == 0) status[j.item] = 1;
     valid[j.item] = (live && one) ? 1 : 0;
   }
 }
 void launch_pairing(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
                     const uint8_t* pinf, const uint8_t* pk_status, const uint32_t* pk_lines,
-                    const uint32_t* neg_g2_lines, uint8_t* status, const uint8_t* gate,
-                    uint8_t* valid) {
+                    const uint32_t* neg_g2_lines, uint32_t* l2buf, uint8_t* status,
+                    const uint8_t* gate, uint8_t* valid) {
   if (!n) return;
-  hipLaunchKernelGGL(k_pairing, dim3((n + 7) / 8), dim3(64), 0, s, jobs, n, points, pinf,
-                     pk_status, pk_lines, neg_g2_lines, status, gate, valid);
+  hipLaunchKernelGGL(k_pairing, dim3((n + p12::CHECKS_PER_BLOCK - 1) / p12::CHECKS_PER_BLOCK),
+                     dim3(64), 0, s, jobs, n, points, pinf, pk_status, pk_lines, neg_g2_lines, l2buf,
+                     status, gate, valid);
 }
 
 // ------------------------------------------------------------------ Sign

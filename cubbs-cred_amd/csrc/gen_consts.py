@@ -116,6 +116,7 @@ def main():
     emit("static constexpr int FP_LIMB_BITS = %d;" % LB)
     arr("P28", [P], "fp")
     arr("P2_28", [2 * P], "fp")
+    arr("P4_28", [4 * P], "fp")
     emit("static constexpr uint32_t P28_INV = 0x%08xu;" % pinv28)
     arr("P_R2", [RP * RP % P], "fp")
     arr("P_ONE", [mont(1)], "fp")

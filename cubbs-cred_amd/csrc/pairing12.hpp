@@ -1,0 +1,502 @@
+// Pairing check on 12 lanes: one lane per Fp coefficient of Fp12.
+//
+// An Fp12 element is held in the basis 1, w, .., w^5 (w^6 = xi = 1 + u); lane
+// (k, c) of a 12-lane group owns component c (u^0 or u^1) of the Fp2
+// coefficient of w^k.  A wavefront carries 5 checks (lanes 0..59); the last
+// 4 lanes form a sixth group that computes on its own scratch LDS region and
+// is discarded, so every lane runs the same instruction stream.
+//
+// Every output coefficient is a short sum of Fp products.  The products are
+// accumulated unreduced in 27 64-bit columns (28-bit limbs, at most 12
+// products of factors below 4p) and reduced once (Montgomery, R = 2^392):
+// a full Fp12 multiplication costs 12 products + 1 reduction per lane, a
+// squaring 8 + 1, a sparse line 5 + 1, a cyclotomic squaring 3 + 1.
+// Negative terms and the xi factor are folded into the factors: before a
+// product the operand is published to LDS together with derived values
+// (2y, y0 + 2p - y1, y0 + y1, 4p - y0 - y1, 2p - y1), so a lane forms
+//   plain  c=0: x0 y0 + x1 (2p - y1)      c=1: x0 y1 + x1 y0
+//   xi     c=0: x0 (y0 - y1) - x1 (y0 + y1)   c=1: x0 (y0 + y1) + x1 (y0 - y1)
+// with only LDS address selection, no branches.
+//
+// All functions contain __syncthreads(): every thread of the 64-thread block
+// calls them in the same order.  Slot k maps to the tower coordinate
+// c_{k mod 2}.c_{k div 2} of pairing.hpp.
+#pragma once
+#include "pairing.hpp"
+
+namespace p12 {
+
+constexpr int E = FPW;                                   // words per LDS entry
+enum { EY0, EY1, E2Y0, E2Y1, ED, ES, ENS, EN1, NE };     // derived entries per slot
+constexpr int SLOT = NE * E;
+constexpr int REGION = 6 * SLOT;
+constexpr int MISC = 16;
+constexpr int LDS_WORDS = 2 * REGION + E + MISC;         // F, G, zero entry, flags
+constexpr int GROUPS = 6;                                // 5 checks + the spare lanes
+constexpr int CHECKS_PER_BLOCK = 5;
+// per check: l2 = -lambda x_P for 68 lines x 2 pairings x 2 components
+constexpr int L2_WORDS = MILLER_LINES * 2 * 2 * E;
+constexpr int LINE_WORDS = 4 * E;  // a prepared line: lambda, c in Fp2
+
+// LDS words (address space 3, so accesses compile to ds_* instructions even
+// across the non-inlined operations below)
+typedef __attribute__((address_space(3))) uint32_t lds_t;
+
+struct Lane {
+  lds_t* F;  // rich region of f
+  lds_t* G;  // rich region of the second operand
+  lds_t* Z;  // a zero entry
+  lds_t* M;  // flags
+  int k, c;
+};
+
+struct Wide {
+  uint64_t c[27];
+};
+
+DEV void w_zero(Wide& w) {
+#pragma unroll
+  for (int i = 0; i < 27; i++) w.c[i] = 0;
+}
+DEV void w_mac(Wide& w, const Fp& x, const Fp& y) {
+#pragma unroll
+  for (int i = 0; i < 14; i++)
+#pragma unroll
+    for (int j = 0; j < 14; j++) w.c[i + j] += (uint64_t)x.v[i] * y.v[j];
+}
+// r = T / R mod p for T < p R, r < 2p.  Column sums stay below 2^64: at most
+// 13 x 14 partial products of < 2^56 each per column.
+DEV void w_reduce(Fp& r, Wide& w) {
+  const uint32_t* M = bls::P28;
+  uint64_t carry = 0;
+#pragma unroll
+  for (int s = 0; s < 14; s++) {
+    uint64_t x = w.c[s] + carry;
+    const uint32_t m = ((uint32_t)x * bls::P28_INV) & FP_MASK;
+    x += (uint64_t)m * M[0];
+    carry = x >> 28;
+#pragma unroll
+    for (int j = 1; j < 14; j++) w.c[s + j] += (uint64_t)m * M[j];
+  }
+#pragma unroll
+  for (int s = 14; s < 27; s++) {
+    const uint64_t x = w.c[s] + carry;
+    r.v[s - 14] = (uint32_t)x & FP_MASK;
+    carry = x >> 28;
+  }
+  r.v[13] = (uint32_t)carry;
+}
+
+DEV void ld(Fp& r, const uint32_t* p) {
+#pragma unroll
+  for (int i = 0; i < E; i++) r.v[i] = p[i];
+}
+DEV void st(uint32_t* p, const Fp& v) {
+#pragma unroll
+  for (int i = 0; i < E; i++) p[i] = v.v[i];
+}
+DEV void ld(Fp& r, const lds_t* p) {
+#pragma unroll
+  for (int i = 0; i < E; i++) r.v[i] = p[i];
+}
+DEV void st(lds_t* p, const Fp& v) {
+#pragma unroll
+  for (int i = 0; i < E; i++) p[i] = v.v[i];
+}
+DEV void mac_at(Wide& w, const lds_t* xa, const lds_t* ya) {
+  Fp x, y;
+  ld(x, xa);
+  ld(y, ya);
+  w_mac(w, x, y);
+}
+DEV void sel(Fp& r, bool t, const Fp& a, const Fp& b) {
+#pragma unroll
+  for (int i = 0; i < E; i++) r.v[i] = t ? a.v[i] : b.v[i];
+}
+// value of the adjacent lane (lane ^ 1): the other component of the slot
+DEV void partner(Fp& o, const Fp& v) {
+#pragma unroll
+  for (int i = 0; i < E; i++) o.v[i] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v.v[i], 0xB1, 0xF, 0xF, false);
+}
+// r = a + b + K - s - t on normalized limbs without reduction; the caller
+// guarantees a non-negative result below 2^392.  K is 0, 2p or 4p (kk).
+DEV void lz5(Fp& r, const Fp& a, const Fp& b, int kk, const Fp& s, const Fp& t) {
+  int32_t cy = 0;
+#pragma unroll
+  for (int i = 0; i < E; i++) {
+    const uint32_t K = kk == 2 ? bls::P2_28[i] : (kk == 4 ? bls::P4_28[i] : 0u);
+    const int32_t u = (int32_t)(a.v[i] + b.v[i] + K) - (int32_t)s.v[i] - (int32_t)t.v[i] + cy;
+    cy = u >> 28;
+    r.v[i] = (uint32_t)u & FP_MASK;
+  }
+}
+
+// Entry of the second factor for component c of x * y (or xi x y).
+DEV int ysel0(int c, bool xi) { return c ? (xi ? ES : EY1) : (xi ? ED : EY0); }
+DEV int ysel1(int c, bool xi) { return c ? (xi ? ED : EY0) : (xi ? ENS : EN1); }
+
+// Publish the lane's coefficient v and its share of the derived entries.
+// Lane c = 0 writes y0, 2 y0, y0 + 2p - y1, y0 + y1; lane c = 1 writes y1,
+// 2 y1, 2p - y1, 4p - y0 - y1.  Values below 2p in, derived values below 4p.
+DEV void put_rich(lds_t* R, const Fp& v, const Lane& L) {
+  Fp o, y0, y1, z, t, a, b, s2, t2;
+  partner(o, v);
+  sel(y0, L.c, o, v);
+  sel(y1, L.c, v, o);
+  fp_zero(z);
+  lds_t* S = R + L.k * SLOT;
+  st(S + (L.c ? EY1 : EY0) * E, v);
+  lz5(t, v, v, 0, z, z);
+  st(S + (L.c ? E2Y1 : E2Y0) * E, t);
+  sel(a, L.c, z, y0);
+  lz5(t, a, z, 2, y1, z);  // c=0: y0 + 2p - y1   c=1: 2p - y1
+  st(S + (L.c ? EN1 : ED) * E, t);
+  sel(b, L.c, z, y1);
+  sel(s2, L.c, y0, z);
+  sel(t2, L.c, y1, z);
+  lz5(t, a, b, L.c ? 4 : 0, s2, t2);  // c=0: y0 + y1   c=1: 4p - y0 - y1
+  st(S + (L.c ? ENS : ES) * E, t);
+}
+
+// f <- f * g
+DEVN void mul(Fp& f, const Fp& g, Lane L) {
+  __syncthreads();
+  put_rich(L.F, f, L);
+  put_rich(L.G, g, L);
+  __syncthreads();
+  Wide w;
+  w_zero(w);
+#pragma unroll 1
+  for (int i = 0; i < 6; i++) {
+    int j = L.k - i;
+    const bool xi = j < 0;
+    j += xi ? 6 : 0;
+    const lds_t* X = L.F + i * SLOT;
+    const lds_t* Y = L.G + j * SLOT;
+    mac_at(w, X + EY0 * E, Y + ysel0(L.c, xi) * E);
+    mac_at(w, X + EY1 * E, Y + ysel1(L.c, xi) * E);
+  }
+  w_reduce(f, w);
+}
+
+// f <- f^2: terms {i, j, times2, times_xi} of (f^2)_k, i < 0 = empty
+__constant__ static const int8_t SQ_TERMS[6][4][4] = {
+    {{0, 0, 0, 0}, {1, 5, 1, 1}, {2, 4, 1, 1}, {3, 3, 0, 1}},
+    {{0, 1, 1, 0}, {2, 5, 1, 1}, {3, 4, 1, 1}, {-1, 0, 0, 0}},
+    {{0, 2, 1, 0}, {1, 1, 0, 0}, {3, 5, 1, 1}, {4, 4, 0, 1}},
+    {{0, 3, 1, 0}, {1, 2, 1, 0}, {4, 5, 1, 1}, {-1, 0, 0, 0}},
+    {{0, 4, 1, 0}, {1, 3, 1, 0}, {2, 2, 0, 0}, {5, 5, 0, 1}},
+    {{0, 5, 1, 0}, {1, 4, 1, 0}, {2, 3, 1, 0}, {-1, 0, 0, 0}}};
+
+DEVN void sqr(Fp& f, Lane L) {
+  __syncthreads();
+  put_rich(L.F, f, L);
+  __syncthreads();
+  Wide w;
+  w_zero(w);
+#pragma unroll 1
+  for (int t = 0; t < 4; t++) {
+    const int i = SQ_TERMS[L.k][t][0];
+    const int j = SQ_TERMS[L.k][t][1];
+    const bool two = SQ_TERMS[L.k][t][2] != 0;
+    const bool xi = SQ_TERMS[L.k][t][3] != 0;
+    const bool none = i < 0;
+    const lds_t* x0 = none ? L.Z : L.F + i * SLOT + (two ? E2Y0 : EY0) * E;
+    const lds_t* x1 = none ? L.Z : L.F + i * SLOT + (two ? E2Y1 : EY1) * E;
+    const lds_t* y0 = none ? L.Z : L.F + j * SLOT + ysel0(L.c, xi) * E;
+    const lds_t* y1 = none ? L.Z : L.F + j * SLOT + ysel1(L.c, xi) * E;
+    mac_at(w, x0, y0);
+    mac_at(w, x1, y1);
+  }
+  w_reduce(f, w);
+}
+
+// f <- f * (l0 + l2 w^2 + yp w^3): l0 = (c0, c1), l2 = (m0, m1) below 2p + 1,
+// yp in Fp (the evaluated Miller-loop line, see pairing.hpp).
+DEVN void mul_line(Fp& f, const Fp& c0, const Fp& c1, const Fp& m0, const Fp& m1, const Fp& yp,
+                   Lane L) {
+  __syncthreads();
+  put_rich(L.F, f, L);
+  __syncthreads();
+  Fp z, a, b, t1, t2;
+  fp_zero(z);
+  Wide w;
+  w_zero(w);
+  // term w^0: f_k * l0 (plain)
+  {
+    Fp x0, x1, y0, y1, n1;
+    ld(x0, L.F + L.k * SLOT + EY0 * E);
+    ld(x1, L.F + L.k * SLOT + EY1 * E);
+    lz5(n1, z, z, 2, c1, z);  // 2p - c1
+    sel(y0, L.c, c1, c0);
+    sel(y1, L.c, c0, n1);
+    w_mac(w, x0, y0);
+    w_mac(w, x1, y1);
+  }
+  // term w^2: f_{k-2} * l2, times xi when k < 2.  Second factors:
+  //   plain c=0: (m0, 2p - m1)                 c=1: (m1, m0)
+  //   xi    c=0: (m0 + 2p - m1, 4p - m0 - m1)  c=1: (m0 + m1, m0 + 2p - m1)
+  {
+    const bool xi = L.k < 2;
+    const bool cx = L.c && xi;
+    const int i = xi ? L.k + 4 : L.k - 2;
+    Fp s1, s2, y0, y1, x0, x1, u0, u1;
+    // t1 = (xi ? m0 : 0) + (cx ? m1 : 0) + (cx ? 0 : 2p) - (cx ? 0 : m1)
+    sel(a, xi, m0, z);
+    sel(b, cx, m1, z);
+    sel(s1, cx, z, m1);
+    lz5(t1, a, b, cx ? 0 : 2, s1, z);
+    // t2 = (cx ? m0 : 0) + (xi ? (c ? 2p : 4p) : 0) - (xi ? (c ? m1 : m0) : 0) - (xi && !c ? m1 : 0)
+    sel(a, cx, m0, z);
+    sel(u0, L.c, m1, m0);
+    sel(s1, xi, u0, z);
+    sel(s2, xi && !L.c, m1, z);
+    lz5(t2, a, z, xi ? (L.c ? 2 : 4) : 0, s1, s2);
+    sel(u0, L.c, m1, m0);
+    sel(u1, L.c, m0, t1);
+    sel(y0, xi, t1, u0);
+    sel(y1, xi, t2, u1);
+    ld(x0, L.F + i * SLOT + EY0 * E);
+    ld(x1, L.F + i * SLOT + EY1 * E);
+    w_mac(w, x0, y0);
+    w_mac(w, x1, y1);
+  }
+  // term w^3: f_{k-3} * yp, times xi when k < 3
+  {
+    const bool xi = L.k < 3;
+    const int i = xi ? L.k + 3 : L.k - 3;
+    Fp x;
+    ld(x, L.F + i * SLOT + ysel0(L.c, xi) * E);
+    w_mac(w, x, yp);
+  }
+  w_reduce(f, w);
+}
+
+// Granger-Scott squaring in the cyclotomic subgroup.  Fp4 pairs (a, b) =
+// (w^P, w^(P+3)); with s0 = a^2 + xi b^2 and s1 = 2ab the new coefficients are
+//   w0 = 3 s0(P=0) - 2 w0   w3 = 3 s1(P=0) + 2 w3
+//   w2 = 3 s0(P=1) - 2 w2   w5 = 3 s1(P=1) + 2 w5
+//   w4 = 3 s0(P=2) - 2 w4   w1 = 3 xi s1(P=2) + 2 w1
+// Each component of s0, s1, xi s1 is a sum of three products of derived
+// entries ({which, entry, which, entry}, which: 0 = a, 1 = b, 2 = zero).
+__constant__ static const int8_t GS_TERMS[3][2][3][4] = {
+    // s0.c0 = (a0+a1)(a0-a1) + (b0+b1)(b0-b1) - 2 b0 b1
+    // s0.c1 = 2 a0 a1 + (b0+b1)(b0-b1) + 2 b0 b1
+    {{{0, ES, 0, ED}, {1, ES, 1, ED}, {1, E2Y0, 1, EN1}},
+     {{0, E2Y0, 0, EY1}, {1, ES, 1, ED}, {1, E2Y0, 1, EY1}}},
+    // s1 = 2ab
+    {{{0, E2Y0, 1, EY0}, {0, E2Y1, 1, EN1}, {2, 0, 2, 0}},
+     {{0, E2Y0, 1, EY1}, {0, E2Y1, 1, EY0}, {2, 0, 2, 0}}},
+    // xi s1
+    {{{0, E2Y0, 1, ED}, {0, E2Y1, 1, ENS}, {2, 0, 2, 0}},
+     {{0, E2Y0, 1, ES}, {0, E2Y1, 1, ED}, {2, 0, 2, 0}}}};
+__constant__ static const int8_t GS_SLOT[6][2] = {  // {P, type}
+    {0, 0}, {2, 2}, {1, 0}, {0, 1}, {2, 0}, {1, 1}};
+
+DEVN void cyc_sqr(Fp& f, Lane L) {
+  __syncthreads();
+  put_rich(L.F, f, L);
+  __syncthreads();
+  const int P = GS_SLOT[L.k][0], ty = GS_SLOT[L.k][1];
+  Wide w;
+  w_zero(w);
+#pragma unroll 1
+  for (int t = 0; t < 3; t++) {
+    const int8_t* T = GS_TERMS[ty][L.c][t];
+    const lds_t* x = T[0] == 2 ? L.Z : L.F + (P + 3 * T[0]) * SLOT + T[1] * E;
+    const lds_t* y = T[2] == 2 ? L.Z : L.F + (P + 3 * T[2]) * SLOT + T[3] * E;
+    mac_at(w, x, y);
+  }
+  Fp s, t3, d, u, v;
+  w_reduce(s, w);
+  fp_dbl(t3, s);
+  fp_add(t3, t3, s);
+  fp_dbl(d, f);
+  fp_sub(u, t3, d);
+  fp_add(v, t3, d);
+  sel(f, (L.k & 1) == 0, u, v);
+}
+
+// f <- conj(f) (x -> x^(p^6)): odd slots negated
+DEV void conj(Fp& f, const Lane& L) {
+  Fp n;
+  fp_neg(n, f);
+  sel(f, (L.k & 1) != 0, n, f);
+}
+// x -> x^p: slot k conjugated, times gamma1[k] = xi^(k (p-1)/6)
+DEVN void frob(Fp& f, Lane L) {
+  Fp o, x0, x1, g0, g1, z, n0, y0, y1;
+  partner(o, f);
+  sel(x0, L.c, o, f);
+  sel(x1, L.c, f, o);
+  fp_load_const(g0, bls::FROB1 + 2 * FPW * L.k);
+  fp_load_const(g1, bls::FROB1 + 2 * FPW * L.k + FPW);
+  fp_zero(z);
+  lz5(n0, z, z, 2, g0, z);
+  // (x0 - x1 u)(g0 + g1 u) = (x0 g0 + x1 g1) + (x0 g1 - x1 g0) u
+  sel(y0, L.c, g1, g0);
+  sel(y1, L.c, n0, g1);
+  Wide w;
+  w_zero(w);
+  w_mac(w, x0, y0);
+  w_mac(w, x1, y1);
+  w_reduce(f, w);
+}
+// x -> x^(p^2): slot k times gamma2[k] in Fp
+DEVN void frob2(Fp& f, Lane L) {
+  Fp g;
+  fp_load_const(g, bls::FROB2 + FPW * L.k);
+  fp_mul(f, f, g);
+}
+
+// Inverse: lane (0, 0) gathers the element, inverts it in the tower, scatters.
+DEVN void inv(Fp& f, Lane L) {
+  lds_t* A = L.F;
+  __syncthreads();
+  st(A + L.k * SLOT + (L.c ? EY1 : EY0) * E, f);
+  __syncthreads();
+  if (L.k == 0 && L.c == 0) {
+    Fp12 x, y;
+    Fp2* xs[6] = {&x.c0.c0, &x.c1.c0, &x.c0.c1, &x.c1.c1, &x.c0.c2, &x.c1.c2};
+    Fp2* ys[6] = {&y.c0.c0, &y.c1.c0, &y.c0.c1, &y.c1.c1, &y.c0.c2, &y.c1.c2};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      ld(xs[k]->c0, A + k * SLOT + EY0 * E);
+      ld(xs[k]->c1, A + k * SLOT + EY1 * E);
+    }
+    fp12_inv(y, x);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      st(A + k * SLOT + EY0 * E, ys[k]->c0);
+      st(A + k * SLOT + EY1 * E, ys[k]->c1);
+    }
+  }
+  __syncthreads();
+  ld(f, A + L.k * SLOT + (L.c ? EY1 : EY0) * E);
+}
+
+// a^x for a cyclotomic element, x = -|x|
+DEVN void exp_by_x(Fp& r, const Fp& a, Lane L) {
+  Fp acc = a;
+#pragma unroll 1
+  for (int i = 62; i >= 0; i--) {
+    cyc_sqr(acc, L);
+    if ((BLS_X_ABS >> i) & 1) mul(acc, a, L);
+  }
+  conj(acc, L);
+  r = acc;
+}
+
+// Same exponent chain as final_exponentiation() in pairing.hpp.
+DEVN void final_exp(Fp& r, const Fp& f, Lane L) {
+  Fp f1, t, a, b, c;
+  t = f;
+  inv(t, L);
+  f1 = f;
+  conj(f1, L);
+  mul(f1, t, L);  // f^(p^6 - 1)
+  t = f1;
+  frob2(t, L);
+  mul(f1, t, L);  // ^(p^2 + 1)
+  exp_by_x(a, f1, L);
+  t = f1;
+  conj(t, L);
+  mul(a, t, L);  // a = f1^(x-1)
+  exp_by_x(t, a, L);
+  b = a;
+  conj(b, L);
+  mul(t, b, L);
+  a = t;  // a = f1^((x-1)^2)
+  exp_by_x(b, a, L);
+  t = a;
+  frob(t, L);
+  mul(b, t, L);  // b = a^(x+p)
+  exp_by_x(c, b, L);
+  exp_by_x(c, c, L);
+  t = b;
+  frob2(t, L);
+  mul(c, t, L);
+  t = b;
+  conj(t, L);
+  mul(c, t, L);  // c = b^(x^2 + p^2 - 1)
+  t = f1;
+  cyc_sqr(t, L);
+  mul(t, f1, L);  // f1^3
+  mul(c, t, L);
+  r = c;
+}
+
+// Product of the two Miller loops e(P0, Q0) e(P1, Q1) with prepared lines:
+// lines_q holds the 68 (lambda, c) of Q_q, l2 the per-check values
+// -lambda x_Pq ([line][q][component]), yp_q the y of P_q.  A skipped pair
+// contributes the constant line 1.  Conjugated at the end (x < 0).
+DEVN void miller(Fp& f, const uint32_t* lines0, const uint32_t* lines1, const uint32_t* l2,
+                 const Fp& yp0, const Fp& yp1, bool skip0, bool skip1, Lane L) {
+  Fp one, z;
+  fp_one(one);
+  fp_zero(z);
+  sel(f, L.k == 0 && L.c == 0, one, z);
+  int idx = 0;
+#pragma unroll 1
+  for (int i = 62; i >= 0; i--) {
+    if (i != 62) sqr(f, L);
+    const int steps = ((BLS_X_ABS >> i) & 1) ? 2 : 1;
+#pragma unroll 1
+    for (int s = 0; s < steps; s++) {
+#pragma unroll 1
+      for (int q = 0; q < 2; q++) {
+        const bool skip = q ? skip1 : skip0;
+        const uint32_t* ln = (q ? lines1 : lines0) + (size_t)idx * LINE_WORDS;
+        const uint32_t* m = l2 + ((size_t)idx * 2 + q) * 2 * E;
+        Fp c0, c1, m0, m1, yp, t;
+        ld(t, ln + 2 * E);  // c.c0 (after lambda.c0, lambda.c1)
+        sel(c0, skip, one, t);
+        ld(t, ln + 3 * E);
+        sel(c1, skip, z, t);
+        ld(t, m);
+        sel(m0, skip, z, t);
+        ld(t, m + E);
+        sel(m1, skip, z, t);
+        sel(yp, skip, z, q ? yp1 : yp0);
+        mul_line(f, c0, c1, m0, m1, yp, L);
+      }
+      idx++;
+    }
+  }
+  conj(f, L);
+}
+
+// 1 when the element is the identity (the same answer on every lane of the group)
+DEVN bool is_one(const Fp& f, Lane L) {
+  __syncthreads();
+  Fp one;
+  fp_one(one);
+  const bool ok = (L.k == 0 && L.c == 0) ? fp_eq(f, one) : fp_is_zero(f);
+  L.M[2 * L.k + L.c] = ok ? 1u : 0u;
+  __syncthreads();
+  bool all = true;
+  for (int i = 0; i < 12; i++) all = all && L.M[i] != 0;
+  return all;
+}
+
+// Lane setup for a 64-thread block: group g = lane / 12 (g = 5 is the spare
+// group of 4 lanes), slot k and component c inside the group.
+DEV Lane lane_setup(uint32_t* lds_all, int lane) {
+  Lane L;
+  const int g = lane / 12, gl = lane % 12;
+  lds_t* base = (lds_t*)lds_all + g * LDS_WORDS;
+  L.F = base;
+  L.G = base + REGION;
+  L.Z = base + 2 * REGION;
+  L.M = base + 2 * REGION + E;
+  L.k = gl >> 1;
+  L.c = gl & 1;
+  // the zero entry
+  if (gl == 0) {
+#pragma unroll
+    for (int i = 0; i < E; i++) L.Z[i] = 0;
+  }
+  return L;
+}
+
+}  // namespace p12
