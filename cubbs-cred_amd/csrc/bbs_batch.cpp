@@ -69,10 +69,16 @@ void Arena::reset() {
 Arena::~Arena() {
   for (hipStream_t s : streams)
     if (s) (void)hipStreamSynchronize(s);
+  for (hipStream_t s : aux)
+    if (s) (void)hipStreamSynchronize(s);
   for (auto& bl : dev) (void)hipFree(bl.p);
   for (auto& bl : host) (void)hipHostFree(bl.p);
-  for (hipStream_t s : streams)
-    if (s) (void)hipStreamDestroy(s);
+  for (int i = 0; i < 4; i++) {
+    if (streams[i]) (void)hipStreamDestroy(streams[i]);
+    if (aux[i]) (void)hipStreamDestroy(aux[i]);
+    if (fork[i]) (void)hipEventDestroy(fork[i]);
+    if (join[i]) (void)hipEventDestroy(join[i]);
+  }
 }
 
 // ------------------------------------------------------------------ planning helpers
@@ -244,10 +250,17 @@ static void materialize(Batch& b, Plan& p) {
   b.n_scalars = p.n_scalars;
   b.n_points = p.n_points;
   b.n_partials = p.n_partials;
-  if (ar)
+  if (ar) {
     b.stream = ar->streams[ar->cur];
-  else
+    b.aux = ar->aux[ar->cur];
+    b.ev_fork = ar->fork[ar->cur];
+    b.ev_join = ar->join[ar->cur];
+  } else {
     check(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+    check(hipStreamCreateWithFlags(&b.aux, hipStreamNonBlocking));
+    check(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
+    check(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
+  }
   check(hipMemcpyAsync(b.d_in, b.h_in, b.in_bytes, hipMemcpyHostToDevice, b.stream));
 }
 
@@ -325,7 +338,7 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
     p.dec_g1.push_back(DecodeG1Job{sig, pa, k});
     p.dec_fr.push_back(DecodeFrJob{sig + 48, se, k, 1u, 0u});  // store r - e
     uint32_t first = p.n_partials;
-    p.var.push_back(VarJob{pa, se, p.partials(1)});         // (-e) A
+    p.var.push_back(VarJob{pa, se, p.partials(VAR_PARTIALS)});  // (-e) A
     p.comb(term, L + 2, CH);                                  // B
     p.groups1.push_back(ReduceGroup{first, p.n_partials - first});
     p.out1.push_back(pb);                                     // B - e A
@@ -450,13 +463,13 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
     }
     // T1 = c Bbar + e^ Abar + r1^ D
     uint32_t f1 = p.n_partials;
-    p.var.push_back(VarJob{pbb, s_c, p.partials(1)});
-    p.var.push_back(VarJob{pab, s_ehat, p.partials(1)});
-    p.var.push_back(VarJob{pd, s_r1, p.partials(1)});
-    p.groups1.push_back(ReduceGroup{f1, 3});
+    p.var.push_back(VarJob{pbb, s_c, p.partials(VAR_PARTIALS)});
+    p.var.push_back(VarJob{pab, s_ehat, p.partials(VAR_PARTIALS)});
+    p.var.push_back(VarJob{pd, s_r1, p.partials(VAR_PARTIALS)});
+    p.groups1.push_back(ReduceGroup{f1, 3 * VAR_PARTIALS});
     p.out1.push_back(pt1);
     uint32_t f2 = p.n_partials;
-    p.var.push_back(VarJob{pd, s_r3, p.partials(1)});
+    p.var.push_back(VarJob{pd, s_r3, p.partials(VAR_PARTIALS)});
     p.comb(term, L + 2, CH);
     p.groups1.push_back(ReduceGroup{f2, p.n_partials - f2});
     p.out1.push_back(pt2);
@@ -648,28 +661,29 @@ std::unique_ptr<Batch> plan_proof_gen(const CuBbsBbsGpuProofGenItem* items, size
         u++;
       }
     }
-    uint32_t q = p.partials(7);
-    p.var.push_back(VarJob{pb, kk + 0, q + 0});   // D = r2 B
-    p.var.push_back(VarJob{pa, kk + 1, q + 1});   // Abar = r1 r2 A
-    p.var.push_back(VarJob{pb, kk + 2, q + 2});   // Bbar = r1 r2 B
-    p.var.push_back(VarJob{pa, kk + 3, q + 3});   //       - r1 r2 e A
-    p.var.push_back(VarJob{pa, kk + 4, q + 4});   // T1 = r1 r2 e~ A
-    p.var.push_back(VarJob{pb, kk + 5, q + 5});   //      + r2 r1~ B
-    p.var.push_back(VarJob{pb, kk + 6, q + 6});   // T2 = r2 r3~ B + sum
+    constexpr uint32_t V = VAR_PARTIALS;
+    uint32_t q = p.partials(7 * V);
+    p.var.push_back(VarJob{pb, kk + 0, q + 0 * V});   // D = r2 B
+    p.var.push_back(VarJob{pa, kk + 1, q + 1 * V});   // Abar = r1 r2 A
+    p.var.push_back(VarJob{pb, kk + 2, q + 2 * V});   // Bbar = r1 r2 B
+    p.var.push_back(VarJob{pa, kk + 3, q + 3 * V});   //       - r1 r2 e A
+    p.var.push_back(VarJob{pa, kk + 4, q + 4 * V});   // T1 = r1 r2 e~ A
+    p.var.push_back(VarJob{pb, kk + 5, q + 5 * V});   //      + r2 r1~ B
+    p.var.push_back(VarJob{pb, kk + 6, q + 6 * V});   // T2 = r2 r3~ B + sum
     uint32_t ft2 = p.n_partials;
     p.comb(term, U, CH);
     term += U;
-    p.groups2.push_back(ReduceGroup{q + 0, 1});
+    p.groups2.push_back(ReduceGroup{q + 0 * V, V});
     p.out2.push_back(pd);
-    p.groups2.push_back(ReduceGroup{q + 1, 1});
+    p.groups2.push_back(ReduceGroup{q + 1 * V, V});
     p.out2.push_back(pab);
-    p.groups2.push_back(ReduceGroup{q + 2, 2});
+    p.groups2.push_back(ReduceGroup{q + 2 * V, 2 * V});
     p.out2.push_back(pbb);
-    p.groups2.push_back(ReduceGroup{q + 4, 2});
+    p.groups2.push_back(ReduceGroup{q + 4 * V, 2 * V});
     p.out2.push_back(pt1);
-    // T2 partials: q + 6 then the comb chunks [ft2, n_partials) -- contiguous
-    // only if ft2 == q + 7, which holds because partials(7) was taken just before.
-    p.groups2.push_back(ReduceGroup{q + 6, 1 + (p.n_partials - ft2)});
+    // T2 partials: the last variable job's then the comb chunks [ft2, n_partials),
+    // contiguous because ft2 == q + 7 V (partials(7 V) was taken just before).
+    p.groups2.push_back(ReduceGroup{q + 6 * V, V + (p.n_partials - ft2)});
     p.out2.push_back(pt2);
     PgJob job{};
     job.ph_off = ph;
@@ -709,8 +723,9 @@ std::unique_ptr<Batch> plan_proof_gen(const CuBbsBbsGpuProofGenItem* items, size
 // ------------------------------------------------------------------ launching
 struct Timed {
   Batch& b;
+  hipStream_t st;
   size_t idx = 0;
-  Timed(Batch& bb, const char* name) : b(bb) {
+  Timed(Batch& bb, const char* name, hipStream_t on = nullptr) : b(bb), st(on ? on : bb.stream) {
     KernelTimer& t = b.timer;
     if (!t.on) return;
     auto it = std::find(t.names.begin(), t.names.end(), name);
@@ -727,11 +742,11 @@ struct Timed {
     t.starts.push_back(e0);
     t.stops.push_back(e1);
     t.event_name.push_back(idx);
-    check(hipEventRecord(e0, b.stream));
+    check(hipEventRecord(e0, st));
   }
   ~Timed() {
     if (!b.timer.on) return;
-    (void)hipEventRecord(b.timer.stops.back(), b.stream);
+    (void)hipEventRecord(b.timer.stops.back(), st);
     b.timer.count[idx]++;
   }
 };
@@ -743,13 +758,30 @@ void launch_batch(Batch& b) {
     Timed t(b, "fill_status");
     launch_fill_status(s, b.nv, b.status, b.gate, 1);
   }
-  if (b.n_dec_g1) {
-    Timed t(b, "decode_g1");
-    launch_decode_g1(s, b.dec_g1, b.n_dec_g1, b.blob, b.points, b.pinf, b.status);
-  }
+  // Verify / ProofVerify: point decode and the variable-base products run on
+  // the auxiliary stream while the message hashing, domain and comb MSM run
+  // here; both need the decoded scalars and join before the reduction.
+  const bool fork = (b.kind == CUBBS_HIP_BATCH_VERIFY || b.kind == CUBBS_HIP_BATCH_PROOF_VERIFY) &&
+                    b.aux != nullptr;
   if (b.n_dec_fr) {
     Timed t(b, "decode_fr");
     launch_decode_fr(s, b.dec_fr, b.n_dec_fr, b.blob, b.scalars, b.status);
+  }
+  if (fork) {
+    check(hipEventRecord(b.ev_fork, s));
+    check(hipStreamWaitEvent(b.aux, b.ev_fork, 0));
+    if (b.n_dec_g1) {
+      Timed t(b, "decode_g1", b.aux);
+      launch_decode_g1(b.aux, b.dec_g1, b.n_dec_g1, b.blob, b.points, b.pinf, b.status);
+    }
+    if (b.n_var) {
+      Timed t(b, "var_mul_glv", b.aux);
+      launch_var_mul(b.aux, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials);
+    }
+    check(hipEventRecord(b.ev_join, b.aux));
+  } else if (b.n_dec_g1) {
+    Timed t(b, "decode_g1");
+    launch_decode_g1(s, b.dec_g1, b.n_dec_g1, b.blob, b.points, b.pinf, b.status);
   }
   if (b.n_h2s) {
     Timed t(b, "h2s_messages");
@@ -771,7 +803,7 @@ void launch_batch(Batch& b) {
     Timed t(b, "sign_e");
     launch_sign_e(s, b.sign, b.n_sign, b.blob, b.pk_status, b.scalars, b.status);
   }
-  if (b.kind != KIND_PROOF_GEN && b.n_var) {
+  if (!fork && b.kind != KIND_PROOF_GEN && b.n_var) {
     Timed t(b, "var_mul_glv");
     launch_var_mul(s, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials);
   }
@@ -779,6 +811,7 @@ void launch_batch(Batch& b) {
     Timed t(b, "comb_msm");
     launch_comb_msm(s, b.chunks, b.n_chunks, b.term_gen, b.scalars, b.tables, b.partials);
   }
+  if (fork) check(hipStreamWaitEvent(s, b.ev_join, 0));
   if (b.n_groups1) {
     Timed t(b, "reduce");
     launch_reduce(s, b.groups1, b.out1, b.n_groups1, b.partials, b.points, b.pinf);
@@ -873,7 +906,11 @@ Batch::~Batch() {
     (void)hipEventDestroy(timer.starts[i]);
     (void)hipEventDestroy(timer.stops[i]);
   }
-  if (from_arena) return;  // the arena owns memory and stream
+  if (aux) (void)hipStreamSynchronize(aux);
+  if (from_arena) return;  // the arena owns memory and streams
+  if (aux) (void)hipStreamDestroy(aux);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
   if (h_in) (void)hipHostFree(h_in);
   if (d_in) (void)hipFree(d_in);
   if (d_work) (void)hipFree(d_work);

@@ -34,6 +34,8 @@ struct Arena {
     size_t cap, used;
   };
   hipStream_t streams[4] = {};  // one per operation kind, so sub-batches overlap
+  hipStream_t aux[4] = {};      // second stream of each kind (concurrent stages)
+  hipEvent_t fork[4] = {}, join[4] = {};
   int cur = 0;                   // stream used by the next planned batch
   std::vector<Block> dev, host;
   size_t dev_bytes = 0, host_bytes = 0;  // held in blocks
@@ -50,7 +52,11 @@ void set_thread_arena(Arena* a);
 struct Batch {
   int kind = 0;
   hipStream_t stream = nullptr;
-  bool from_arena = false;           // memory and stream belong to an Arena
+  bool from_arena = false;           // memory and streams belong to an Arena
+  // Verify / ProofVerify run the point decode + variable-base multiplication
+  // on `aux`, concurrently with hashing and the comb MSM on `stream`.
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   size_t n_total = 0;                // items passed in
   std::vector<int32_t> pre_code;     // per item: -1 = decided on the device
   std::vector<uint8_t> pre_value;    // value for items decided on the host

@@ -77,6 +77,9 @@ struct DecodeFrJob {
 };
 
 // partials[partial] = scalars[scalar] * points[point] (identity-aware).
+// [k] P with the GLV split k = k1 + k2 lambda on two lanes: partials
+// [partial] = [k1] P and [partial + 1] = [k2] phi(P).
+constexpr uint32_t VAR_PARTIALS = 2;
 struct VarJob {
   uint32_t point;
   uint32_t scalar;

@@ -108,13 +108,16 @@ void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t*
 }
 
 // ------------------------------------------------------------------ variable-base multiplication
+// Two lanes per job: lane 0 forms [k1] P, lane 1 [k2] phi(P) (k = k1 + k2 lambda);
+// the reduction stage adds the two partials.
 __global__ __launch_bounds__(64) void k_var_mul(const VarJob* __restrict__ jobs, uint32_t n,
                                                const uint32_t* __restrict__ points,
                                                const uint8_t* __restrict__ pinf,
                                                const uint32_t* __restrict__ scalars,
                                                uint32_t* __restrict__ partials) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * n) return;
+  const uint32_t i = t >> 1, half = t & 1;
   VarJob j = jobs[i];
   G1J r;
   pt_set_inf(r);
@@ -123,14 +126,14 @@ __global__ __launch_bounds__(64) void k_var_mul(const VarJob* __restrict__ jobs,
     load_g1a(p, points + (size_t)j.point * G1A_WORDS);
     Fr k;
     load_fr(k, scalars + (size_t)j.scalar * FR_WORDS);
-    g1_mul_glv(r, p, k.v);
+    g1_mul_glv_half(r, p, k.v, half);
   }
-  store_g1j(partials + (size_t)j.partial * G1J_WORDS, r);
+  store_g1j(partials + (size_t)(j.partial + half) * G1J_WORDS, r);
 }
 void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_t* points,
                     const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials) {
   if (!n) return;
-  hipLaunchKernelGGL(k_var_mul, GRID(n, 64), 0, s, jobs, n, points, pinf, scalars, partials);
+  hipLaunchKernelGGL(k_var_mul, GRID(2 * n, 64), 0, s, jobs, n, points, pinf, scalars, partials);
 }
 
 // ------------------------------------------------------------------ reduction

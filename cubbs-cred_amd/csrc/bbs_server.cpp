@@ -273,7 +273,12 @@ void Server::worker(unsigned slot) {
   (void)slot;
   Arena ar;
   try {
-    for (hipStream_t& st : ar.streams) check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    for (int i = 0; i < 4; i++) {
+      check(hipStreamCreateWithFlags(&ar.streams[i], hipStreamNonBlocking));
+      check(hipStreamCreateWithFlags(&ar.aux[i], hipStreamNonBlocking));
+      check(hipEventCreateWithFlags(&ar.fork[i], hipEventDisableTiming));
+      check(hipEventCreateWithFlags(&ar.join[i], hipEventDisableTiming));
+    }
   } catch (...) {
     fault = CUBBS_BBS_GPU_ERR_INTERNAL;
   }

@@ -233,6 +233,21 @@ DEVN void pt_mul_u64(PtJ<F>& r, const PtJ<F>& p, uint64_t k) {
   r = acc;
 }
 
+// r = [k] q for an affine base, k a non-zero small constant: starts at the top
+// set bit, mixed additions.
+template <class F>
+DEVN void pt_mul_u64_affine(PtJ<F>& r, const PtA<F>& q, uint64_t k) {
+  PtJ<F> acc;
+  pt_from_affine(acc, q);
+  const int top = 63 - __builtin_clzll(k);
+#pragma unroll 1
+  for (int i = top - 1; i >= 0; i--) {
+    pt_dbl(acc, acc);
+    if ((k >> i) & 1) pt_madd(acc, acc, q);
+  }
+  r = acc;
+}
+
 // r = [k] q for a canonical scalar k given as N little-endian 32-bit limbs
 // and an affine base (binary left-to-right with mixed additions).
 template <class F, int N>
@@ -265,13 +280,12 @@ DEV void g1_endo(G1J& r, const G1J& p) {
 // Subgroup test for points of E(Fp) (Bowe, "Faster subgroup checks for
 // BLS12-381", 2019): P in G1 iff phi'(P) = [-x^2] P where phi' = phi^2 acts
 // as lambda^2 = -x^2.  Here: [x^2] P == (beta^2 x, -y).
-DEVN bool g1_in_subgroup(const G1J& p) {
-  if (pt_is_inf(p)) return true;
+// The point is given in affine form (not the identity).
+DEVN bool g1_in_subgroup(const G1A& pa) {
   G1J t;
-  pt_mul_u64(t, p, 0xd201000000010000ull);
+  pt_mul_u64_affine(t, pa, 0xd201000000010000ull);
   pt_mul_u64(t, t, 0xd201000000010000ull);
-  G1A q, pa;
-  pt_to_affine(pa, p);
+  G1A q;
   Fp beta;
   fp_load_const(beta, bls::GLV_BETA);
   fp_mul(q.x, pa.x, beta);
@@ -310,7 +324,10 @@ DEVN bool g1_decompress(G1J& r, const uint8_t* in) {
   r.x = x;
   r.y = y;
   fp_one(r.z);
-  return g1_in_subgroup(r);
+  G1A a;
+  a.x = x;
+  a.y = y;
+  return g1_in_subgroup(a);
 }
 
 DEVN void g1_compress_affine(uint8_t* out, const G1A& a) {
@@ -472,6 +489,24 @@ DEVN void glv_split(uint32_t k1[4], uint32_t k2[4], const uint32_t k[8]) {
     k1[i] = rem[i];
     k2[i] = q[i];
   }
+}
+
+// One half of the GLV product: half 0 gives [k1] P, half 1 gives [k2] phi(P),
+// with k = k1 + k2 lambda (P affine, not the identity; 128-bit binary ladder
+// with mixed additions).
+DEVN void g1_mul_glv_half(G1J& r, const G1A& p, const uint32_t k[8], uint32_t half) {
+  uint32_t k1[4], k2[4];
+  glv_split(k1, k2, k);
+  G1A q;
+  Fp beta;
+  fp_load_const(beta, bls::GLV_BETA);
+  fp_mul(q.x, p.x, beta);
+  if (!half) q.x = p.x;
+  q.y = p.y;
+  uint32_t e[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) e[i] = half ? k2[i] : k1[i];
+  pt_mul_limbs_affine<Fp, 4>(r, q, e);
 }
 
 // [k] P on G1 (P affine, not the identity) with the GLV split
