@@ -123,6 +123,14 @@ void ensure_generators_locked(Context& c, uint32_t slots) {
   }
   check(hipStreamSynchronize(c.setup));
   check(hipFree(scratch));
+  // host mirror of the encodings: recognises generator points in the
+  // primitive MSM entries (their comb tables are resident)
+  c.gen_comp_host.resize((size_t)slots * 48);
+  check(hipMemcpy(c.gen_comp_host.data() + (size_t)c.gen_slots * 48, c.gen_comp + (size_t)c.gen_slots * 48,
+                  (size_t)(slots - c.gen_slots) * 48, hipMemcpyDeviceToHost));
+  for (uint32_t g = c.gen_slots; g < slots; g++)
+    c.gen_index.emplace(std::string(reinterpret_cast<const char*>(c.gen_comp_host.data()) + 48 * g, 48),
+                        g);
   c.gen_slots = slots;
 }
 
@@ -427,9 +435,13 @@ int32_t cubbs_hip_batch_profile(void* batch, int iters, char* names_out, size_t 
 // ---------------------------------------------------------------- hashing
 // Shared implementation of the packed hash entries: kind 0 = SHA-256 digest,
 // 1 = expand_message_xmd(.., 48), 2 = hash_to_scalar.
+// Scalars of this thread's last hash_to_scalar / messages_to_scalar call: the
+// "resident" message scalars of sign_e_from_resident_scalars.
+static thread_local std::vector<uint8_t> tl_last_scalars;
+
 static int32_t packed_hash(int kind, const uint8_t* packed, const size_t* offsets,
                            const size_t* lengths, size_t n, const uint8_t* dst, size_t dst_len,
-                           uint8_t* out) {
+                           uint8_t* out, bool record = true) {
   if (n == 0) return CUBBS_BBS_GPU_SUCCESS;
   if (!offsets || !lengths || !out) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   if (kind != 0 && (dst_len > 255 || (dst_len && !dst))) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
@@ -467,6 +479,7 @@ static int32_t packed_hash(int kind, const uint8_t* packed, const size_t* offset
     check(hipStreamSynchronize(s));
     check(hipFree(d));
     check(hipStreamDestroy(s));
+    if (kind == 2 && record) tl_last_scalars.assign(out, out + 32 * n);
     return CUBBS_BBS_GPU_SUCCESS;
   } catch (const GpuError&) {
     return CUBBS_BBS_GPU_ERR_INTERNAL;
@@ -532,6 +545,85 @@ int32_t cubbs_bbs_gpu_messages_to_scalar_batch_v2(uint32_t abi_version,
   for (size_t i = 0; i < item_count; i++) {
     std::memcpy(items[i].scalar_out.data, out.data() + 32 * i, 32);
     items[i].scalar_out.len = 32;
+    results[i].code = CUBBS_BBS_GPU_SUCCESS;
+    results[i].value = 0;
+    results[i].bytes_written = 32;
+  }
+  return CUBBS_BBS_GPU_SUCCESS;
+}
+
+// reference header :111-119, :254-257; semantics native/cuda/src/cubbs_bbs_cuda.cu:476-716
+// (core_sign's e input, bbsplus/signature.rs:577: sk || m_1 .. m_L || domain).
+int32_t cubbs_bbs_gpu_sign_e_from_resident_scalars_batch_v2(
+    uint32_t abi_version, CuBbsBbsGpuSignEFromResidentScalarsItem* items, size_t item_count,
+    CuBbsBbsGpuByteSlice message_dst, CuBbsBbsGpuByteSlice e_dst, CuBbsBbsGpuResult* results) {
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION || message_dst.len > 255 || e_dst.len > 255 ||
+      !slice_ok(message_dst) || !slice_ok(e_dst) || (item_count && (!items || !results)))
+    return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  if (item_count == 0) return CUBBS_BBS_GPU_SUCCESS;
+  const bool raw = items[0].messages != nullptr;
+  size_t total = 0, raw_bytes = 0;
+  for (size_t i = 0; i < item_count; i++) {
+    const auto& it = items[i];
+    const size_t count = it.message_count;
+    if (count == 0 || it.message_scalar_offset + count < it.message_scalar_offset ||
+        !it.secret_key.data || it.secret_key.len != 32 || !it.domain.data || it.domain.len != 32 ||
+        !it.scalar_out.data || it.scalar_out.capacity < 32 || (it.messages != nullptr) != raw)
+      return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+    if (raw) {
+      if (it.message_scalar_offset != total || !it.message_scalars_out.data ||
+          it.message_scalars_out.capacity < count * 32)
+        return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      for (size_t m = 0; m < count; m++) {
+        if (!slice_ok(it.messages[m])) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+        raw_bytes += it.messages[m].len;
+      }
+      total += count;
+    } else if (it.message_scalar_offset + count > tl_last_scalars.size() / 32) {
+      return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+    }
+  }
+  if (raw) {
+    std::vector<uint8_t> packed(std::max<size_t>(raw_bytes, 1));
+    std::vector<size_t> offs(total), lens(total);
+    size_t k = 0, cur = 0;
+    for (size_t i = 0; i < item_count; i++)
+      for (size_t m = 0; m < items[i].message_count; m++, k++) {
+        offs[k] = cur;
+        lens[k] = items[i].messages[m].len;
+        if (lens[k]) std::memcpy(packed.data() + cur, items[i].messages[m].data, lens[k]);
+        cur += lens[k];
+      }
+    std::vector<uint8_t> sc(32 * total);
+    const int32_t st = packed_hash(2, packed.data(), offs.data(), lens.data(), total,
+                                   message_dst.data, message_dst.len, sc.data());
+    if (st != CUBBS_BBS_GPU_SUCCESS) return st;
+  }
+  const std::vector<uint8_t>& scal = tl_last_scalars;
+  std::vector<uint8_t> packed;
+  std::vector<size_t> offs(item_count), lens(item_count);
+  for (size_t i = 0; i < item_count; i++) {
+    const auto& it = items[i];
+    offs[i] = packed.size();
+    lens[i] = (it.message_count + 2) * 32;
+    packed.insert(packed.end(), it.secret_key.data, it.secret_key.data + 32);
+    const uint8_t* m = scal.data() + 32 * it.message_scalar_offset;
+    packed.insert(packed.end(), m, m + 32 * it.message_count);
+    packed.insert(packed.end(), it.domain.data, it.domain.data + 32);
+  }
+  std::vector<uint8_t> e(32 * item_count);
+  const int32_t st = packed_hash(2, packed.data(), offs.data(), lens.data(), item_count, e_dst.data,
+                                 e_dst.len, e.data(), false);
+  if (st != CUBBS_BBS_GPU_SUCCESS) return st;
+  for (size_t i = 0; i < item_count; i++) {
+    auto& it = items[i];
+    if (raw) {
+      std::memcpy(it.message_scalars_out.data, scal.data() + 32 * it.message_scalar_offset,
+                  32 * it.message_count);
+      it.message_scalars_out.len = 32 * it.message_count;
+    }
+    std::memcpy(it.scalar_out.data, e.data() + 32 * i, 32);
+    it.scalar_out.len = 32;
     results[i].code = CUBBS_BBS_GPU_SUCCESS;
     results[i].value = 0;
     results[i].bytes_written = 32;

@@ -17,6 +17,7 @@
 #include "../../include/cubbs_bbs_gpu_abi.h"
 #include "../../include/cubbs_hip_ext.h"
 #include "bbs_batch.h"
+#include "bbs_plan.h"
 #include "host_ctx.h"
 
 namespace cubbs {
@@ -81,60 +82,8 @@ Arena::~Arena() {
   }
 }
 
-// ------------------------------------------------------------------ planning helpers
-template <class T>
-static size_t put(std::vector<uint8_t>& blob, const T* p, size_t n) {
-  size_t off = blob.size();
-  blob.resize(off + n * sizeof(T));
-  if (n) std::memcpy(blob.data() + off, p, n * sizeof(T));
-  return off;
-}
-
-struct Plan {
-  // host-side job lists
-  std::vector<H2sJob> h2s;
-  std::vector<DomainJob> dom;
-  std::vector<DecodeG1Job> dec_g1;
-  std::vector<DecodeFrJob> dec_fr;
-  std::vector<TermFill> fills;
-  std::vector<VarJob> var;
-  std::vector<CombChunk> chunks;
-  std::vector<uint32_t> term_gen;
-  std::vector<ReduceGroup> groups1, groups2;
-  std::vector<uint32_t> out1, out2;
-  std::vector<PairJob> pairs;
-  std::vector<SignJob> sign;
-  std::vector<PvChallengeJob> pvch;
-  std::vector<PgJob> pg;
-  std::vector<uint32_t> disc_index, disc_scalar, und_scalar;
-  std::vector<uint8_t> blob;       // signatures, proofs, headers, messages, keys, seeds
-  std::vector<uint8_t> map_dst;    // DST of message scalars
-  uint32_t n_scalars = 0, n_points = 0, n_partials = 0;
-  size_t out_bytes = 0;            // signature / proof output area
-  bool seeded = false;
-  size_t seeds_off = 0;
-
-  uint32_t scalars(uint32_t k) { uint32_t s = n_scalars; n_scalars += k; return s; }
-  uint32_t points(uint32_t k) { uint32_t s = n_points; n_points += k; return s; }
-  uint32_t partials(uint32_t k) { uint32_t s = n_partials; n_partials += k; return s; }
-  size_t bytes(const void* p, size_t n) {
-    size_t off = blob.size();
-    blob.resize(off + n);
-    if (n) std::memcpy(blob.data() + off, p, n);
-    return off;
-  }
-  // Fixed-base terms [term, term + count) in chunks, partials appended to the
-  // caller's running partial range.
-  void comb(uint32_t term, uint32_t count, uint32_t chunk) {
-    for (uint32_t t = 0; t < count; t += chunk) {
-      uint32_t c = std::min(chunk, count - t);
-      chunks.push_back(CombChunk{term + t, c, partials(1)});
-    }
-  }
-};
-
 // Copy the plan into one pinned buffer + device mirror; allocate the work area.
-static void materialize(Batch& b, Plan& p) {
+void materialize(Batch& b, Plan& p) {
   std::vector<uint8_t> H;
   H.reserve(p.blob.size() + 4096);
   auto place = [&](const void* src, size_t bytes) {
@@ -156,6 +105,11 @@ static void materialize(Batch& b, Plan& p) {
   size_t o_g2 = place(p.groups2.data(), p.groups2.size() * sizeof(ReduceGroup));
   size_t o_o2 = place(p.out2.data(), p.out2.size() * 4);
   size_t o_pair = place(p.pairs.data(), p.pairs.size() * sizeof(PairJob));
+  std::vector<std::pair<size_t, size_t>> o_lv;
+  for (size_t l = 0; l < p.pst_groups.size(); l++)
+    o_lv.emplace_back(place(p.pst_groups[l].data(), p.pst_groups[l].size() * sizeof(ReduceGroup)),
+                      place(p.pst_out[l].data(), p.pst_out[l].size() * 4));
+  size_t o_comp = place(p.comp.data(), p.comp.size() * sizeof(CompressJob));
   size_t o_sign = place(p.sign.data(), p.sign.size() * sizeof(SignJob));
   size_t o_pvch = place(p.pvch.data(), p.pvch.size() * sizeof(PvChallengeJob));
   size_t o_pg = place(p.pg.data(), p.pg.size() * sizeof(PgJob));
@@ -200,6 +154,12 @@ static void materialize(Batch& b, Plan& p) {
   b.n_groups2 = (uint32_t)p.groups2.size();
   b.pairs = reinterpret_cast<PairJob*>(D + o_pair);
   b.n_pairs = (uint32_t)p.pairs.size();
+  for (size_t l = 0; l < o_lv.size(); l++)
+    b.levels.push_back(Batch::Level{reinterpret_cast<ReduceGroup*>(D + o_lv[l].first),
+                                    reinterpret_cast<uint32_t*>(D + o_lv[l].second),
+                                    (uint32_t)p.pst_groups[l].size()});
+  b.comp = reinterpret_cast<CompressJob*>(D + o_comp);
+  b.n_comp = (uint32_t)p.comp.size();
   b.sign = reinterpret_cast<SignJob*>(D + o_sign);
   b.n_sign = (uint32_t)p.sign.size();
   b.pvch = reinterpret_cast<PvChallengeJob*>(D + o_pvch);
@@ -265,7 +225,7 @@ static void materialize(Batch& b, Plan& p) {
 }
 
 // Lock the context, make generators/public keys resident, snapshot pointers.
-static void attach(Batch& b, uint32_t max_gen_slots, const std::vector<const uint8_t*>& pks,
+void attach(Batch& b, uint32_t max_gen_slots, const std::vector<const uint8_t*>& pks,
                    std::vector<uint32_t>& pk_slot) {
   Context& c = ctx();
   std::lock_guard<std::mutex> g(c.mu);
@@ -342,7 +302,7 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
     p.comb(term, L + 2, CH);                                  // B
     p.groups1.push_back(ReduceGroup{first, p.n_partials - first});
     p.out1.push_back(pb);                                     // B - e A
-    p.pairs.push_back(PairJob{k, pa, pb, pk_slot[k]});
+    p.pairs.push_back(PairJob{k, pa, pb, pk_slot[k], PAIR_NEG_G2, 0});
     term += L + 2;
   }
   materialize(*b, p);
@@ -475,7 +435,7 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
     p.out1.push_back(pt2);
     p.pvch.push_back(PvChallengeJob{ph, (uint32_t)it.presentation_header.len, k, R, disc_begin,
                                     pab, pbb, pd, pt1, pt2, s_dom, s_c});
-    p.pairs.push_back(PairJob{k, pab, pbb, pk_slot[k]});
+    p.pairs.push_back(PairJob{k, pab, pbb, pk_slot[k], PAIR_NEG_G2, 0});
     term += L + 2;
   }
   materialize(*b, p);
@@ -812,6 +772,10 @@ void launch_batch(Batch& b) {
     launch_comb_msm(s, b.chunks, b.n_chunks, b.term_gen, b.scalars, b.tables, b.partials);
   }
   if (fork) check(hipStreamWaitEvent(s, b.ev_join, 0));
+  for (const auto& lv : b.levels) {
+    Timed t(b, "reduce_tree");
+    launch_reduce_partials(s, lv.groups, lv.out, lv.n, b.partials);
+  }
   if (b.n_groups1) {
     Timed t(b, "reduce");
     launch_reduce(s, b.groups1, b.out1, b.n_groups1, b.partials, b.points, b.pinf);
@@ -838,6 +802,10 @@ void launch_batch(Batch& b) {
   if (b.kind == KIND_SIGN) {
     Timed t(b, "sign_finalize");
     launch_sign_finalize(s, b.sign, b.n_sign, b.points, b.pinf, b.scalars, b.status, b.out);
+  }
+  if (b.n_comp) {
+    Timed t(b, "compress");
+    launch_compress(s, b.comp, b.n_comp, b.points, b.pinf, b.out);
   }
   if (b.n_pvch) {
     Timed t(b, "pv_challenge");

@@ -15,6 +15,9 @@ namespace cubbs {
 // Batch kinds beyond the two public staged kinds of cubbs_hip_ext.h.
 constexpr int KIND_SIGN = 2;
 constexpr int KIND_PROOF_GEN = 3;
+// A primitive program (the MSM / add / pairing entries): status slots instead
+// of items, results assembled by the caller from h_status / h_valid / output.
+constexpr int KIND_PRIM = 4;
 
 struct KernelTimer {
   bool on = false;
@@ -82,6 +85,14 @@ struct Batch {
   ReduceGroup* groups2 = nullptr;
   uint32_t* out2 = nullptr;
   PairJob* pairs = nullptr;
+  struct Level {
+    ReduceGroup* groups;
+    uint32_t* out;
+    uint32_t n;
+  };
+  std::vector<Level> levels;          // partial-sum tree, in launch order
+  CompressJob* comp = nullptr;
+  uint32_t n_comp = 0;
   SignJob* sign = nullptr;
   PvChallengeJob* pvch = nullptr;
   PgJob* pg = nullptr;

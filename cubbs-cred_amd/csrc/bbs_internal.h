@@ -58,6 +58,14 @@ struct ReduceGroup {
   uint32_t count;
 };
 
+// Compressed 48-byte encoding of points[point] (identity aware) written at
+// out + off.
+struct CompressJob {
+  uint32_t point;
+  uint32_t pad;
+  uint64_t off;
+};
+
 // Decode a compressed G1 point from the input blob into points[out]; a
 // failure marks status[item] = 1 (invalid input).
 struct DecodeG1Job {
@@ -67,7 +75,8 @@ struct DecodeG1Job {
 };
 
 // Decode a 32-byte big-endian scalar (canonical check) into scalars[out];
-// flags bit0: store r - s instead of s.
+// flags bit0: store r - s instead of s; bit1: reduce mod r instead of
+// rejecting a non-canonical value.
 struct DecodeFrJob {
   uint64_t off;
   uint32_t out;
@@ -96,11 +105,17 @@ struct TermFill {
 };
 
 // e(points[a], PK) * e(points[b], -G2) == 1, and gate[item] must be set.
+// e(points[a], Q_a) * e(+-points[b], Q_b) == 1 for item `item`: Q_a is the
+// cached G2 point pk_slot (prepared lines), Q_b the cached G2 point pk_b or
+// -G2 when pk_b == PAIR_NEG_G2; flags bit0 negates points[b].
+constexpr uint32_t PAIR_NEG_G2 = 0xffffffffu;
 struct PairJob {
   uint32_t item;
   uint32_t a;
   uint32_t b;
   uint32_t pk_slot;
+  uint32_t pk_b;
+  uint32_t flags;
 };
 
 // Sign: e = H2S(sk || m_1..m_L || domain); A = B / (sk + e) computed as the
@@ -178,6 +193,11 @@ void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t*
 // l2buf: PAIR_L2_WORDS words of device scratch per job, filled by
 // launch_pair_lines and read by launch_pairing
 constexpr size_t PAIR_L2_WORDS = (size_t)68 * 2 * 2 * 14;
+// partials[out[i]] = sum of the partials of groups[i] (Jacobian, no inversion)
+void launch_reduce_partials(hipStream_t s, const ReduceGroup* groups, const uint32_t* out, uint32_t n,
+                            uint32_t* partials);
+void launch_compress(hipStream_t s, const CompressJob* jobs, uint32_t n, const uint32_t* points,
+                     const uint8_t* pinf, uint8_t* out);
 void launch_pair_lines(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
                        const uint32_t* pk_lines, const uint32_t* neg_g2_lines, uint32_t* l2buf);
 void launch_pairing(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,

@@ -68,7 +68,11 @@ __global__ __launch_bounds__(256) void k_decode_fr(const DecodeFrJob* __restrict
   DecodeFrJob j = jobs[i];
   Fr s;
   fr_from_be_bytes_raw(s, blob + j.off);
-  if (!fr_canonical(s)) {
+  if (j.flags & 2) {
+    // a 256-bit value is below 3r
+    fr_sub_if_geq(s);
+    fr_sub_if_geq(s);
+  } else if (!fr_canonical(s)) {
     status[j.item] = 1;
     fr_zero(s);
   }
@@ -170,6 +174,53 @@ void launch_reduce(hipStream_t s, const ReduceGroup* groups, const uint32_t* out
   hipLaunchKernelGGL(k_reduce, GRID(n, 64), 0, s, groups, out_point, n, partials, points, pinf);
 }
 
+// One level of a partial-sum tree: each lane adds a short run of Jacobian
+// partials into a new partial slot (large MSMs reduce in parallel levels).
+__global__ __launch_bounds__(64) void k_reduce_partials(const ReduceGroup* __restrict__ groups,
+                                                       const uint32_t* __restrict__ out, uint32_t n,
+                                                       uint32_t* __restrict__ partials) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ReduceGroup g = groups[i];
+  G1J acc;
+  pt_set_inf(acc);
+  for (uint32_t k = 0; k < g.count; k++) {
+    G1J p;
+    load_g1j(p, partials + (size_t)(g.begin + k) * G1J_WORDS);
+    pt_add(acc, acc, p);
+  }
+  store_g1j(partials + (size_t)out[i] * G1J_WORDS, acc);
+}
+void launch_reduce_partials(hipStream_t s, const ReduceGroup* groups, const uint32_t* out,
+                            uint32_t n, uint32_t* partials) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_reduce_partials, GRID(n, 64), 0, s, groups, out, n, partials);
+}
+
+__global__ __launch_bounds__(64) void k_compress(const CompressJob* __restrict__ jobs, uint32_t n,
+                                                const uint32_t* __restrict__ points,
+                                                const uint8_t* __restrict__ pinf,
+                                                uint8_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const CompressJob j = jobs[i];
+  uint8_t c[48];
+  if (pinf[j.point]) {
+    c[0] = 0xc0;
+    for (int k = 1; k < 48; k++) c[k] = 0;
+  } else {
+    G1A a;
+    load_g1a(a, points + (size_t)j.point * G1A_WORDS);
+    g1_compress_affine(c, a);
+  }
+  for (int k = 0; k < 48; k++) out[j.off + k] = c[k];
+}
+void launch_compress(hipStream_t s, const CompressJob* jobs, uint32_t n, const uint32_t* points,
+                     const uint8_t* pinf, uint8_t* out) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_compress, GRID(n, 64), 0, s, jobs, n, points, pinf, out);
+}
+
 // ------------------------------------------------------------------ pairing check
 // e(a, PK) * e(b, -G2) == 1 (identity arguments contribute 1).
 //
@@ -188,8 +239,10 @@ __global__ __launch_bounds__(256) void k_pair_lines(const PairJob* __restrict__ 
   const uint32_t r = t % (MILLER_LINES * 4);
   const uint32_t idx = r >> 2, q = (r >> 1) & 1, comp = r & 1;
   const PairJob j = jobs[job];
-  const uint32_t* line = (q ? neg_g2_lines : pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS) +
-                         (size_t)idx * MILLER_LINE_WORDS;
+  const uint32_t* qlines = q ? (j.pk_b == PAIR_NEG_G2 ? neg_g2_lines
+                                                       : pk_lines + (size_t)j.pk_b * PK_LINES_WORDS)
+                             : pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS;
+  const uint32_t* line = qlines + (size_t)idx * MILLER_LINE_WORDS;
   Fp lam, x, v;
   fp_load_const(lam, line + comp * FPW);
   fp_load_const(x, points + (size_t)(q ? j.b : j.a) * G1A_WORDS);
@@ -224,16 +277,17 @@ __global__ __launch_bounds__(64) void k_pairing(const PairJob* __restrict__ jobs
   const bool mine = grp < p12::CHECKS_PER_BLOCK && job < n;
   bool live = mine;
   PairJob j{};
-  uint8_t pks = 1;
+  uint8_t pks = 1, pkb = 1;
   if (mine) {
     j = jobs[job];
     pks = pk_status[j.pk_slot];
-    live = status[j.item] == 0 && pks != 0 && gate[j.item];
+    pkb = j.pk_b == PAIR_NEG_G2 ? 1 : pk_status[j.pk_b];
+    live = status[j.item] == 0 && pks != 0 && pkb != 0 && gate[j.item];
   }
   // blocks with no live check leave together (uniform exit, no barrier skipped)
   if (!__syncthreads_or(live)) {
     if (mine && lane % 12 == 0) {
-      if (pks == 0) status[j.item] = 1;
+      if (pks == 0 || pkb == 0) status[j.item] = 1;
       valid[j.item] = 0;
     }
     return;
@@ -244,20 +298,23 @@ __global__ __launch_bounds__(64) void k_pairing(const PairJob* __restrict__ jobs
   bool skip0 = true, skip1 = true;
   const uint32_t* lines0 = neg_g2_lines;
   const uint32_t* l2 = l2buf;
+  const uint32_t* lines1 = neg_g2_lines;
   if (live) {
     fp_load_const(yp0, points + (size_t)j.a * G1A_WORDS + FPW);
     fp_load_const(yp1, points + (size_t)j.b * G1A_WORDS + FPW);
+    if (j.flags & 1) fp_neg(yp1, yp1);
     skip0 = pinf[j.a] || pks == 2;
-    skip1 = pinf[j.b];
+    skip1 = pinf[j.b] || pkb == 2;
     lines0 = pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS;
+    if (j.pk_b != PAIR_NEG_G2) lines1 = pk_lines + (size_t)j.pk_b * PK_LINES_WORDS;
     l2 = l2buf + (size_t)job * p12::L2_WORDS;
   }
   Fp f, r;
-  p12::miller(f, lines0, neg_g2_lines, l2, yp0, yp1, skip0, skip1, L);
+  p12::miller(f, lines0, lines1, l2, yp0, yp1, skip0, skip1, L);
   p12::final_exp(r, f, L);
   const bool one = p12::is_one(r, L);
   if (mine && lane % 12 == 0) {
-    if (pks == 0) status[j.item] = 1;
+    if (pks == 0 || pkb == 0) status[j.item] = 1;
     valid[j.item] = (live && one) ? 1 : 0;
   }
 }

@@ -1,0 +1,77 @@
+// Host-side plan of a device batch: job lists, the input blob and slot
+// counters, turned into device arrays by materialize() (bbs_batch.cpp).
+#pragma once
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "bbs_batch.h"
+
+namespace cubbs {
+
+// ------------------------------------------------------------------ planning helpers
+template <class T>
+inline size_t put(std::vector<uint8_t>& blob, const T* p, size_t n) {
+  size_t off = blob.size();
+  blob.resize(off + n * sizeof(T));
+  if (n) std::memcpy(blob.data() + off, p, n * sizeof(T));
+  return off;
+}
+
+struct Plan {
+  // host-side job lists
+  std::vector<H2sJob> h2s;
+  std::vector<DomainJob> dom;
+  std::vector<DecodeG1Job> dec_g1;
+  std::vector<DecodeFrJob> dec_fr;
+  std::vector<TermFill> fills;
+  std::vector<VarJob> var;
+  std::vector<CombChunk> chunks;
+  std::vector<uint32_t> term_gen;
+  std::vector<ReduceGroup> groups1, groups2;
+  std::vector<uint32_t> out1, out2;
+  std::vector<PairJob> pairs;
+  // partial-sum tree levels (primitive MSMs): level l sums groups of level-l
+  // inputs into new partial slots; the last level feeds groups1
+  std::vector<std::vector<ReduceGroup>> pst_groups;
+  std::vector<std::vector<uint32_t>> pst_out;
+  std::vector<CompressJob> comp;  // affine point -> 48-byte encoding in the output area
+  std::vector<SignJob> sign;
+  std::vector<PvChallengeJob> pvch;
+  std::vector<PgJob> pg;
+  std::vector<uint32_t> disc_index, disc_scalar, und_scalar;
+  std::vector<uint8_t> blob;       // signatures, proofs, headers, messages, keys, seeds
+  std::vector<uint8_t> map_dst;    // DST of message scalars
+  uint32_t n_scalars = 0, n_points = 0, n_partials = 0;
+  size_t out_bytes = 0;            // signature / proof output area
+  bool seeded = false;
+  size_t seeds_off = 0;
+
+  uint32_t scalars(uint32_t k) { uint32_t s = n_scalars; n_scalars += k; return s; }
+  uint32_t points(uint32_t k) { uint32_t s = n_points; n_points += k; return s; }
+  uint32_t partials(uint32_t k) { uint32_t s = n_partials; n_partials += k; return s; }
+  size_t bytes(const void* p, size_t n) {
+    size_t off = blob.size();
+    blob.resize(off + n);
+    if (n) std::memcpy(blob.data() + off, p, n);
+    return off;
+  }
+  // Fixed-base terms [term, term + count) in chunks, partials appended to the
+  // caller's running partial range.
+  void comb(uint32_t term, uint32_t count, uint32_t chunk) {
+    for (uint32_t t = 0; t < count; t += chunk) {
+      uint32_t c = std::min(chunk, count - t);
+      chunks.push_back(CombChunk{term + t, c, partials(1)});
+    }
+  }
+};
+
+// Copy the plan into one pinned buffer + device mirror and allocate the work
+// area (from the thread's Arena when one is set).
+void materialize(Batch& b, Plan& p);
+// Lock the context, make generators [0, max_gen_slots) and the given public
+// keys resident, snapshot the resident pointers into the batch.
+void attach(Batch& b, uint32_t max_gen_slots, const std::vector<const uint8_t*>& pks,
+            std::vector<uint32_t>& pk_slot);
+
+}  // namespace cubbs

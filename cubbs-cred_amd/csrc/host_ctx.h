@@ -32,6 +32,8 @@ struct Context {
   uint32_t* gen_tables = nullptr;
   uint8_t* gen_comp = nullptr;
   uint32_t* gen_vstate = nullptr;
+  std::vector<uint8_t> gen_comp_host;                   // 48 bytes per slot
+  std::unordered_map<std::string, uint32_t> gen_index;  // encoding -> slot
   // prepared public keys (status: 0 invalid, 1 valid, 2 identity)
   std::unordered_map<std::string, uint32_t> pk_slots;
   uint32_t pk_count = 0, pk_cap = 0;

@@ -33,6 +33,19 @@ ABI_SYMBOLS = [
     "cubbs_bbs_gpu_sha256_batch_for_test_v2",
     "cubbs_bbs_gpu_expand_xmd_sha256_48_batch_for_test_v2",
     "cubbs_bbs_gpu_hash_to_scalar_sha256_batch_for_test_v2",
+    "cubbs_bbs_gpu_sign_e_from_resident_scalars_batch_v2",
+    "cubbs_bbs_gpu_prepare_b_batch_v2",
+    "cubbs_bbs_gpu_g1_msm_batch_v2",
+    "cubbs_bbs_gpu_g1_msm_common_points_batch_v2",
+    "cubbs_bbs_gpu_g1_msm_fused_common_points_batch_v2",
+    "cubbs_bbs_gpu_g1_msm_common_points_plus_variable_batch_v2",
+    "cubbs_bbs_gpu_proof_verify_msm_pair_batch_v2",
+    "cubbs_bbs_gpu_g1_dual_msm_same_points_batch_v2",
+    "cubbs_bbs_gpu_g1_dual_msm_same_points_lanes_batch_v2",
+    "cubbs_bbs_gpu_g1_add_batch_v2",
+    "cubbs_bbs_gpu_pairing_eq_batch_v2",
+    "cubbs_bbs_gpu_pairing_eq_aggregate_v2",
+    "cubbs_bbs_gpu_proof_gen_t2_prepare_batch_v2",
 ]
 EXT_SYMBOLS = [
     "cubbs_hip_batch_prepare",
@@ -107,6 +120,74 @@ class ProofVerifyItem(ctypes.Structure):
 
 class MessagesToScalarItem(ctypes.Structure):
     _fields_ = [("message", ByteSlice), ("dst", ByteSlice), ("scalar_out", ByteBuffer)]
+
+
+class SignEItem(ctypes.Structure):
+    _fields_ = [("secret_key", ByteSlice), ("domain", ByteSlice), ("messages", ctypes.c_void_p),
+                ("message_scalar_offset", ctypes.c_size_t), ("message_count", ctypes.c_size_t),
+                ("message_scalars_out", ByteBuffer), ("scalar_out", ByteBuffer)]
+
+
+class PrepareBItem(ctypes.Structure):
+    _fields_ = [("public_key", ByteSlice), ("header", ByteSlice), ("message_scalars", ctypes.c_void_p),
+                ("message_count", ctypes.c_size_t), ("b_out", ByteBuffer)]
+
+
+class G1MsmItem(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_void_p), ("scalars", ctypes.c_void_p), ("term_count", ctypes.c_size_t),
+                ("point_out", ByteBuffer)]
+
+
+class CommonPointsItem(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_void_p), ("term_count", ctypes.c_size_t),
+                ("scalar_rows", ctypes.c_void_p), ("scalar_rows_be", ctypes.c_void_p),
+                ("scalar_row_stride", ctypes.c_size_t), ("output_count", ctypes.c_size_t),
+                ("point_outs", ctypes.c_void_p)]
+
+
+class PlusVariableItem(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_void_p), ("term_count", ctypes.c_size_t),
+                ("scalar_rows", ctypes.c_void_p), ("variable_points", ctypes.c_void_p),
+                ("variable_scalars", ctypes.c_void_p), ("output_count", ctypes.c_size_t),
+                ("point_outs", ctypes.c_void_p)]
+
+
+class PvMsmPairItem(ctypes.Structure):
+    _fields_ = [("common_points", ctypes.c_void_p), ("common_term_count", ctypes.c_size_t),
+                ("common_scalar_rows", ctypes.c_void_p), ("proof_points", ctypes.c_void_p),
+                ("proof_scalars", ctypes.c_void_p), ("output_count", ctypes.c_size_t),
+                ("t1_outs", ctypes.c_void_p), ("t2_outs", ctypes.c_void_p)]
+
+
+class DualMsmItem(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_void_p), ("lhs_scalars", ctypes.c_void_p),
+                ("rhs_scalars", ctypes.c_void_p), ("term_count", ctypes.c_size_t),
+                ("lhs_out", ByteBuffer), ("rhs_out", ByteBuffer)]
+
+
+class DualMsmLanesItem(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_void_p), ("lhs_scalars", ctypes.c_void_p),
+                ("rhs_scalars", ctypes.c_void_p), ("term_count", ctypes.c_size_t),
+                ("lane_count", ctypes.c_size_t), ("lhs_outs", ByteBuffer), ("rhs_outs", ByteBuffer)]
+
+
+class G1AddItem(ctypes.Structure):
+    _fields_ = [("lhs", ByteSlice), ("rhs", ByteSlice), ("point_out", ByteBuffer)]
+
+
+class PairingEqItem(ctypes.Structure):
+    _fields_ = [("lhs_g1", ByteSlice), ("lhs_g2", ByteSlice), ("rhs_g1", ByteSlice), ("rhs_g2", ByteSlice)]
+
+
+class PairingEqScaledItem(ctypes.Structure):
+    _fields_ = [("lhs_g1", ByteSlice), ("lhs_g2", ByteSlice), ("rhs_g1", ByteSlice), ("rhs_g2", ByteSlice),
+                ("coefficient", ByteSlice)]
+
+
+class T2PrepareItem(ctypes.Structure):
+    _fields_ = [("compressed_b", ByteSlice), ("signature_a", ByteSlice), ("scalars_be", ByteSlice * 7),
+                ("seed", ByteSlice), ("d_out", ByteBuffer), ("abar_out", ByteBuffer),
+                ("bbar_out", ByteBuffer), ("t1_out", ByteBuffer)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -610,3 +691,260 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- primitive entries
+def _res(n):
+    return (Result * max(1, n))()
+
+
+def _outs(keep, n, cap=48):
+    """n output buffers of `cap` bytes: (array of ByteBuffer, list of ctypes buffers)."""
+    arr = (ByteBuffer * max(1, n))()
+    bufs = []
+    for i in range(n):
+        buf, bb = keep.out(cap)
+        arr[i] = bb
+        bufs.append(buf)
+    keep.refs.append(arr)
+    return arr, bufs
+
+
+def _r(res, i):
+    return (res[i].code, res[i].value, res[i].bytes_written)
+
+
+def g1_msm(items):
+    """items: [(points48, scalars32)] -> (status, [(code, route, point or None)])."""
+    keep = _Keep()
+    arr = (G1MsmItem * max(1, len(items)))()
+    bufs = []
+    for i, (pts, scs) in enumerate(items):
+        arr[i].points = keep.slices(pts)
+        arr[i].scalars = keep.slices(scs)
+        arr[i].term_count = len(pts)
+        buf, bb = keep.out(48)
+        arr[i].point_out = bb
+        bufs.append(buf)
+    res = _res(len(items))
+    rc = lib().cubbs_bbs_gpu_g1_msm_batch_v2(ABI_VERSION, arr, ctypes.c_size_t(len(items)), res)
+    return rc, [(res[i].code, res[i].value, bufs[i].raw[:48] if res[i].code == 0 else None)
+                for i in range(len(items))]
+
+
+def g1_msm_common_points(groups, fused=False, packed=False):
+    """groups: [(points48, [row scalars])] -> (status, [(code, route, point or None)] over all rows)."""
+    keep = _Keep()
+    arr = (CommonPointsItem * max(1, len(groups)))()
+    allbufs = []
+    for g, (pts, rows) in enumerate(groups):
+        arr[g].points = keep.slices(pts)
+        arr[g].term_count = len(pts)
+        if packed:
+            flat = b"".join(b"".join(r) for r in rows)
+            buf = ctypes.create_string_buffer(flat, max(1, len(flat)))
+            keep.refs.append(buf)
+            arr[g].scalar_rows_be = ctypes.cast(buf, ctypes.c_void_p)
+            arr[g].scalar_row_stride = 32 * len(pts)
+        else:
+            arr[g].scalar_rows = keep.slices([s for r in rows for s in r])
+        arr[g].output_count = len(rows)
+        outs, bufs = _outs(keep, len(rows))
+        arr[g].point_outs = ctypes.cast(outs, ctypes.c_void_p)
+        allbufs += bufs
+    res = _res(len(allbufs))
+    fn = (lib().cubbs_bbs_gpu_g1_msm_fused_common_points_batch_v2 if fused
+          else lib().cubbs_bbs_gpu_g1_msm_common_points_batch_v2)
+    rc = fn(ABI_VERSION, arr, ctypes.c_size_t(len(groups)), res)
+    return rc, [(res[i].code, res[i].value, allbufs[i].raw[:48] if res[i].code == 0 else None)
+                for i in range(len(allbufs))]
+
+
+def g1_msm_plus_variable(groups):
+    """groups: [(points48, [(row scalars, var_point, var_scalar)])] -> (status, rows)."""
+    keep = _Keep()
+    arr = (PlusVariableItem * max(1, len(groups)))()
+    allbufs = []
+    for g, (pts, rows) in enumerate(groups):
+        arr[g].points = keep.slices(pts)
+        arr[g].term_count = len(pts)
+        arr[g].scalar_rows = keep.slices([s for r, _, _ in rows for s in r])
+        arr[g].variable_points = keep.slices([vp for _, vp, _ in rows])
+        arr[g].variable_scalars = keep.slices([vs for _, _, vs in rows])
+        arr[g].output_count = len(rows)
+        outs, bufs = _outs(keep, len(rows))
+        arr[g].point_outs = ctypes.cast(outs, ctypes.c_void_p)
+        allbufs += bufs
+    res = _res(len(allbufs))
+    rc = lib().cubbs_bbs_gpu_g1_msm_common_points_plus_variable_batch_v2(
+        ABI_VERSION, arr, ctypes.c_size_t(len(groups)), res)
+    return rc, [(res[i].code, res[i].value, allbufs[i].raw[:48] if res[i].code == 0 else None)
+                for i in range(len(allbufs))]
+
+
+def proof_verify_msm_pair(groups):
+    """groups: [(common_points48, [(common_scalars, [Bbar, Abar, D], [c, e^, r1^, r3^])])]
+    -> (status, [(code, route, bytes, t1, t2)])."""
+    keep = _Keep()
+    arr = (PvMsmPairItem * max(1, len(groups)))()
+    t1b, t2b = [], []
+    for g, (pts, rows) in enumerate(groups):
+        arr[g].common_points = keep.slices(pts)
+        arr[g].common_term_count = len(pts)
+        arr[g].common_scalar_rows = keep.slices([s for r, _, _ in rows for s in r])
+        arr[g].proof_points = keep.slices([p for _, pp, _ in rows for p in pp])
+        arr[g].proof_scalars = keep.slices([x for _, _, ps in rows for x in ps])
+        arr[g].output_count = len(rows)
+        o1, b1 = _outs(keep, len(rows))
+        o2, b2 = _outs(keep, len(rows))
+        arr[g].t1_outs = ctypes.cast(o1, ctypes.c_void_p)
+        arr[g].t2_outs = ctypes.cast(o2, ctypes.c_void_p)
+        t1b += b1
+        t2b += b2
+    res = _res(len(t1b))
+    rc = lib().cubbs_bbs_gpu_proof_verify_msm_pair_batch_v2(ABI_VERSION, arr, ctypes.c_size_t(len(groups)), res)
+    return rc, [(res[i].code, res[i].value, res[i].bytes_written,
+                 t1b[i].raw[:48] if res[i].code == 0 else None, t2b[i].raw[:48] if res[i].code == 0 else None)
+                for i in range(len(t1b))]
+
+
+def g1_dual_msm(items, lanes=None):
+    """items: [(points, lhs_scalars, rhs_scalars)] -> (status, [(code, value, bytes, lhs, rhs)]);
+    with lanes, each output is a list of lane sums."""
+    keep = _Keep()
+    n = len(items)
+    arr = (DualMsmLanesItem * max(1, n))() if lanes else (DualMsmItem * max(1, n))()
+    bufs = []
+    for i, (pts, ls, rs) in enumerate(items):
+        arr[i].points = keep.slices(pts)
+        arr[i].lhs_scalars = keep.slices(ls)
+        arr[i].rhs_scalars = keep.slices(rs)
+        arr[i].term_count = len(pts)
+        k = lanes[i] if lanes else 1
+        lb, lbb = keep.out(48 * k)
+        rb, rbb = keep.out(48 * k)
+        if lanes:
+            arr[i].lane_count = k
+            arr[i].lhs_outs, arr[i].rhs_outs = lbb, rbb
+        else:
+            arr[i].lhs_out, arr[i].rhs_out = lbb, rbb
+        bufs.append((lb, rb, k))
+    res = _res(n)
+    fn = (lib().cubbs_bbs_gpu_g1_dual_msm_same_points_lanes_batch_v2 if lanes
+          else lib().cubbs_bbs_gpu_g1_dual_msm_same_points_batch_v2)
+    rc = fn(ABI_VERSION, arr, ctypes.c_size_t(n), res)
+    out = []
+    for i, (lb, rb, k) in enumerate(bufs):
+        if res[i].code:
+            out.append((res[i].code, res[i].value, res[i].bytes_written, None, None))
+            continue
+        L = [lb.raw[48 * j:48 * j + 48] for j in range(k)]
+        R = [rb.raw[48 * j:48 * j + 48] for j in range(k)]
+        out.append((res[i].code, res[i].value, res[i].bytes_written,
+                    L if lanes else L[0], R if lanes else R[0]))
+    return rc, out
+
+
+def g1_add(pairs):
+    """pairs: [(a48, b48)] -> (status, [(code, point or None)])."""
+    keep = _Keep()
+    arr = (G1AddItem * max(1, len(pairs)))()
+    bufs = []
+    for i, (a, b) in enumerate(pairs):
+        arr[i].lhs = keep.slice(a)
+        arr[i].rhs = keep.slice(b)
+        buf, bb = keep.out(48)
+        arr[i].point_out = bb
+        bufs.append(buf)
+    res = _res(len(pairs))
+    rc = lib().cubbs_bbs_gpu_g1_add_batch_v2(ABI_VERSION, arr, ctypes.c_size_t(len(pairs)), res)
+    return rc, [(res[i].code, bufs[i].raw[:48] if res[i].code == 0 else None) for i in range(len(pairs))]
+
+
+def pairing_eq(items):
+    """items: [(lhs_g1, lhs_g2, rhs_g1, rhs_g2)] -> (status, [(code, equal)])."""
+    keep = _Keep()
+    arr = (PairingEqItem * max(1, len(items)))()
+    for i, (a1, a2, b1, b2) in enumerate(items):
+        arr[i] = PairingEqItem(keep.slice(a1), keep.slice(a2), keep.slice(b1), keep.slice(b2))
+    res = _res(len(items))
+    rc = lib().cubbs_bbs_gpu_pairing_eq_batch_v2(ABI_VERSION, arr, ctypes.c_size_t(len(items)), res)
+    return rc, [(res[i].code, bool(res[i].value)) for i in range(len(items))]
+
+
+def pairing_eq_aggregate(items):
+    """items: [(lhs_g1, lhs_g2, rhs_g1, rhs_g2, coefficient32)] -> (status, code, equal)."""
+    keep = _Keep()
+    arr = (PairingEqScaledItem * max(1, len(items)))()
+    for i, (a1, a2, b1, b2, c) in enumerate(items):
+        arr[i] = PairingEqScaledItem(keep.slice(a1), keep.slice(a2), keep.slice(b1), keep.slice(b2),
+                                     keep.slice(c))
+    res = Result()
+    rc = lib().cubbs_bbs_gpu_pairing_eq_aggregate_v2(ABI_VERSION, arr, ctypes.c_size_t(len(items)),
+                                                     ctypes.byref(res))
+    return rc, res.code, bool(res.value)
+
+
+def prepare_b(items):
+    """items: [(pk96, header64, [message scalars])] -> (status, [(code, B or None)])."""
+    keep = _Keep()
+    arr = (PrepareBItem * max(1, len(items)))()
+    bufs = []
+    for i, (pk, hdr, scs) in enumerate(items):
+        arr[i].public_key = keep.slice(pk)
+        arr[i].header = keep.slice(hdr)
+        arr[i].message_scalars = keep.slices(scs)
+        arr[i].message_count = len(scs)
+        buf, bb = keep.out(48)
+        arr[i].b_out = bb
+        bufs.append(buf)
+    res = _res(len(items))
+    rc = lib().cubbs_bbs_gpu_prepare_b_batch_v2(ABI_VERSION, arr, ctypes.c_size_t(len(items)), res)
+    return rc, [(res[i].code, bufs[i].raw[:48] if res[i].code == 0 else None) for i in range(len(items))]
+
+
+def sign_e_from_resident(items, message_dst, e_dst):
+    """items: [(sk32, domain32, messages or None, scalar_offset, count)] ->
+    (status, [(code, e, message_scalars or None)])."""
+    keep = _Keep()
+    arr = (SignEItem * max(1, len(items)))()
+    bufs = []
+    for i, (sk, dom, msgs, off, count) in enumerate(items):
+        arr[i].secret_key = keep.slice(sk)
+        arr[i].domain = keep.slice(dom)
+        arr[i].messages = keep.slices(msgs) if msgs is not None else None
+        arr[i].message_scalar_offset = off
+        arr[i].message_count = count
+        mb, mbb = keep.out(32 * count)
+        eb, ebb = keep.out(32)
+        arr[i].message_scalars_out = mbb
+        arr[i].scalar_out = ebb
+        bufs.append((eb, mb))
+    res = _res(len(items))
+    rc = lib().cubbs_bbs_gpu_sign_e_from_resident_scalars_batch_v2(
+        ABI_VERSION, arr, ctypes.c_size_t(len(items)), keep.slice(message_dst), keep.slice(e_dst), res)
+    return rc, [(res[i].code, bufs[i][0].raw[:32] if res[i].code == 0 else None,
+                 bufs[i][1].raw[:32 * items[i][4]] if res[i].code == 0 and items[i][2] is not None else None)
+                for i in range(len(items))]
+
+
+def proof_gen_t2_prepare(items):
+    """items: [(B48, A48, [7 scalars32])] -> (status, [(code, route, (d, abar, bbar, t1) or None)])."""
+    keep = _Keep()
+    arr = (T2PrepareItem * max(1, len(items)))()
+    bufs = []
+    for i, (b, a, scs) in enumerate(items):
+        arr[i].compressed_b = keep.slice(b)
+        arr[i].signature_a = keep.slice(a)
+        for k in range(7):
+            arr[i].scalars_be[k] = keep.slice(scs[k])
+        outs = []
+        for name in ("d_out", "abar_out", "bbar_out", "t1_out"):
+            buf, bb = keep.out(48)
+            setattr(arr[i], name, bb)
+            outs.append(buf)
+        bufs.append(outs)
+    res = _res(len(items))
+    rc = lib().cubbs_bbs_gpu_proof_gen_t2_prepare_batch_v2(ABI_VERSION, arr, ctypes.c_size_t(len(items)), res)
+    return rc, [(res[i].code, res[i].value,
+                 tuple(x.raw[:48] for x in bufs[i]) if res[i].code == 0 else None) for i in range(len(items))]

@@ -310,17 +310,78 @@ int32_t cubbs_bbs_gpu_hash_to_scalar_sha256_batch_for_test_v2(
     uint32_t abi_version, const uint8_t *packed, const size_t *offsets, const size_t *lengths,
     size_t item_count, const uint8_t *dst, size_t dst_len, uint8_t *scalars_out);
 
-/* Primitive entry points of the reference header (:254-308) that this
- * library does not export yet -- the Rust loader treats each as an optional
- * capability and routes around a missing one (gpu_runtime.rs:329-428):
- *   cubbs_bbs_gpu_sign_e_from_resident_scalars_batch_v2, _prepare_b_batch_v2,
- *   _g1_msm_batch_v2, _g1_msm_common_points_batch_v2,
- *   _g1_msm_fused_common_points_batch_v2,
- *   _g1_msm_common_points_plus_variable_batch_v2,
- *   _proof_verify_msm_pair_batch_v2, _g1_dual_msm_same_points_batch_v2,
- *   _g1_dual_msm_same_points_lanes_batch_v2, _g1_add_batch_v2,
- *   _pairing_eq_batch_v2, _pairing_eq_aggregate_v2,
- *   _proof_gen_t2_prepare_batch_v2. */
+/* Primitive entry points (reference :254-308).  Uniform error convention:
+ * structural errors (lengths, NULL pointers, output capacity) fail the call
+ * with CUBBS_BBS_GPU_ERR_INVALID_INPUT; an undecodable or non-subgroup point
+ * and a non-canonical scalar give results[row].code = 1; results[row].value
+ * is the MSM route id (1 item, 2 common points, 3 large, 6 dual) or the
+ * boolean of a pairing equality. */
+
+/* e = hash_to_scalar(sk || m_1 .. m_L || domain, e_dst) per item, the message
+ * scalars hashed from `messages` with message_dst (raw form, all items) or
+ * taken from this thread's last hash_to_scalar / messages_to_scalar call
+ * starting at message_scalar_offset (resident form, `messages` NULL). */
+/* reference :254-257 */
+int32_t cubbs_bbs_gpu_sign_e_from_resident_scalars_batch_v2(
+    uint32_t abi_version, CuBbsBbsGpuSignEFromResidentScalarsItem *items, size_t item_count,
+    CuBbsBbsGpuByteSlice message_dst, CuBbsBbsGpuByteSlice e_dst, CuBbsBbsGpuResult *results);
+
+/* reference :259-262 */
+int32_t cubbs_bbs_gpu_prepare_b_batch_v2(uint32_t abi_version, CuBbsBbsGpuPrepareBItem *items,
+                                         size_t item_count, CuBbsBbsGpuResult *results);
+
+/* reference :264-267 */
+int32_t cubbs_bbs_gpu_g1_msm_batch_v2(uint32_t abi_version, CuBbsBbsGpuG1MsmItem *items,
+                                      size_t item_count, CuBbsBbsGpuResult *results);
+
+/* reference :269-275 */
+int32_t cubbs_bbs_gpu_g1_msm_common_points_batch_v2(
+    uint32_t abi_version, CuBbsBbsGpuG1MsmCommonPointsBatchItem *items, size_t group_count,
+    CuBbsBbsGpuResult *results);
+int32_t cubbs_bbs_gpu_g1_msm_fused_common_points_batch_v2(
+    uint32_t abi_version, CuBbsBbsGpuG1MsmCommonPointsBatchItem *items, size_t group_count,
+    CuBbsBbsGpuResult *results);
+
+/* reference :277-280 */
+int32_t cubbs_bbs_gpu_g1_msm_common_points_plus_variable_batch_v2(
+    uint32_t abi_version, CuBbsBbsGpuG1MsmCommonPointsPlusVariableItem *items, size_t group_count,
+    CuBbsBbsGpuResult *results);
+
+/* reference :282-284 */
+int32_t cubbs_bbs_gpu_proof_verify_msm_pair_batch_v2(uint32_t abi_version,
+                                                     CuBbsBbsGpuProofVerifyMsmPairItem *items,
+                                                     size_t group_count,
+                                                     CuBbsBbsGpuResult *results);
+
+/* reference :286-292 */
+int32_t cubbs_bbs_gpu_g1_dual_msm_same_points_batch_v2(uint32_t abi_version,
+                                                       CuBbsBbsGpuG1DualMsmSamePointsItem *items,
+                                                       size_t item_count,
+                                                       CuBbsBbsGpuResult *results);
+int32_t cubbs_bbs_gpu_g1_dual_msm_same_points_lanes_batch_v2(
+    uint32_t abi_version, CuBbsBbsGpuG1DualMsmSamePointsLanesItem *items, size_t item_count,
+    CuBbsBbsGpuResult *results);
+
+/* reference :294-297 */
+int32_t cubbs_bbs_gpu_g1_add_batch_v2(uint32_t abi_version, CuBbsBbsGpuG1AddItem *items,
+                                      size_t item_count, CuBbsBbsGpuResult *results);
+
+/* reference :299-301 */
+int32_t cubbs_bbs_gpu_pairing_eq_batch_v2(uint32_t abi_version,
+                                          const CuBbsBbsGpuPairingEqItem *items,
+                                          size_t item_count, CuBbsBbsGpuResult *results);
+
+/* reference :302-304: writes ONE result for the whole aggregate. */
+int32_t cubbs_bbs_gpu_pairing_eq_aggregate_v2(uint32_t abi_version,
+                                              const CuBbsBbsGpuPairingEqScaledItem *items,
+                                              size_t item_count, CuBbsBbsGpuResult *result);
+
+/* reference :306-308 */
+int32_t cubbs_bbs_gpu_proof_gen_t2_prepare_batch_v2(uint32_t abi_version,
+                                                    CuBbsBbsGpuProofGenT2PrepareItem *items,
+                                                    size_t item_count,
+                                                    CuBbsBbsGpuResult *results);
+
 
 #ifdef __cplusplus
 }
