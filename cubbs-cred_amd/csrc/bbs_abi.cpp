@@ -61,6 +61,17 @@ void init_locked(Context& c) {
     c.failed = true;
     throw GpuError{hipErrorNoDevice};
   }
+  // Work streams: one per hardware queue of the process (HIP maps streams
+  // onto GPU_MAX_HW_QUEUES queues, 4 unless configured), created together so
+  // that they land on distinct queues; batches take them round-robin, so
+  // concurrently running batches do not share a queue needlessly.
+  {
+    int q = 4;
+    if (const char* e = std::getenv("GPU_MAX_HW_QUEUES")) q = std::atoi(e);
+    q = std::max(2, std::min(q, 16));
+    c.pool.resize(q);
+    for (auto& st : c.pool) check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  }
   check(hipStreamCreateWithFlags(&c.setup, hipStreamNonBlocking));
   check(hipMalloc(&c.gen_vstate, 64));
   check(hipMalloc(&c.neg_g2_lines, PK_LINES_WORDS * 4));
@@ -68,6 +79,12 @@ void init_locked(Context& c) {
   check(hipGetLastError());
   check(hipStreamSynchronize(c.setup));
   c.ready = true;
+}
+
+hipStream_t next_stream_locked(Context& c) {
+  hipStream_t s = c.pool[c.pool_next % c.pool.size()];
+  c.pool_next++;
+  return s;
 }
 
 // Ensure generator slots [0, slots) exist (tables included).  ctx().mu held.
@@ -451,10 +468,12 @@ static int32_t packed_hash(int kind, const uint8_t* packed, const size_t* offset
     total = std::max(total, offsets[i] + lengths[i]);
   }
   try {
+    hipStream_t s;
     {
       Context& c = ctx();
       std::lock_guard<std::mutex> g(c.mu);
       init_locked(c);
+      s = next_stream_locked(c);
     }
     const size_t out_w = kind == 0 ? 32 : (kind == 1 ? 48 : 32);
     Layout L;
@@ -467,8 +486,6 @@ static int32_t packed_hash(int kind, const uint8_t* packed, const size_t* offset
     for (size_t i = 0; i < n; i++) jobs[i] = H2sJob{offsets[i], (uint32_t)lengths[i], (uint32_t)i};
     if (dst_len) std::memcpy(h.data() + o_dst, dst, dst_len);
     if (total) std::memcpy(h.data() + o_blob, packed, total);
-    hipStream_t s;
-    check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     uint8_t* d = nullptr;
     check(hipMalloc(&d, L.size));
     check(hipMemcpyAsync(d, h.data(), o_out, hipMemcpyHostToDevice, s));
@@ -478,7 +495,6 @@ static int32_t packed_hash(int kind, const uint8_t* packed, const size_t* offset
     check(hipMemcpyAsync(out, d + o_out, out_w * n, hipMemcpyDeviceToHost, s));
     check(hipStreamSynchronize(s));
     check(hipFree(d));
-    check(hipStreamDestroy(s));
     if (kind == 2 && record) tl_last_scalars.assign(out, out + 32 * n);
     return CUBBS_BBS_GPU_SUCCESS;
   } catch (const GpuError&) {

@@ -74,9 +74,7 @@ Arena::~Arena() {
     if (s) (void)hipStreamSynchronize(s);
   for (auto& bl : dev) (void)hipFree(bl.p);
   for (auto& bl : host) (void)hipHostFree(bl.p);
-  for (int i = 0; i < 4; i++) {
-    if (streams[i]) (void)hipStreamDestroy(streams[i]);
-    if (aux[i]) (void)hipStreamDestroy(aux[i]);
+  for (int i = 0; i < 4; i++) {  // streams belong to the context's pool
     if (fork[i]) (void)hipEventDestroy(fork[i]);
     if (join[i]) (void)hipEventDestroy(join[i]);
   }
@@ -216,8 +214,12 @@ void materialize(Batch& b, Plan& p) {
     b.ev_fork = ar->fork[ar->cur];
     b.ev_join = ar->join[ar->cur];
   } else {
-    check(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
-    check(hipStreamCreateWithFlags(&b.aux, hipStreamNonBlocking));
+    {
+      Context& c = ctx();
+      std::lock_guard<std::mutex> g(c.mu);
+      b.stream = next_stream_locked(c);
+      b.aux = next_stream_locked(c);
+    }
     check(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
     check(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
   }
@@ -875,14 +877,12 @@ Batch::~Batch() {
     (void)hipEventDestroy(timer.stops[i]);
   }
   if (aux) (void)hipStreamSynchronize(aux);
-  if (from_arena) return;  // the arena owns memory and streams
-  if (aux) (void)hipStreamDestroy(aux);
+  if (from_arena) return;  // the arena owns memory and events
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
   if (h_in) (void)hipHostFree(h_in);
   if (d_in) (void)hipFree(d_in);
-  if (d_work) (void)hipFree(d_work);
-  if (stream) (void)hipStreamDestroy(stream);
+  if (d_work) (void)hipFree(d_work);  // streams belong to the context's pool
 }
 
 }  // namespace cubbs

@@ -382,7 +382,7 @@ void launch_test_g1_mul(hipStream_t s, const uint8_t* pts, const uint8_t* scalar
 __global__ __launch_bounds__(64) void k_test_pairing_gt(const uint8_t* g1, const uint8_t* g2,
                                                        uint32_t n, uint8_t* out, uint8_t* ok,
                                                        uint32_t* scratch) {
-  __shared__ uint32_t lds_all[p12::GROUPS * p12::LDS_WORDS];
+  __shared__ uint32_t lds_all[p12::BLOCK_LDS_WORDS];
   const int lane = threadIdx.x;
   const int grp = lane / 12, gl = lane % 12;
   const p12::Lane L = p12::lane_setup(lds_all, lane);

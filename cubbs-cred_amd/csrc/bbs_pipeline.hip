@@ -259,7 +259,7 @@ void launch_pair_lines(hipStream_t s, const PairJob* jobs, uint32_t n, const uin
                      l2buf);
 }
 
-__global__ __launch_bounds__(64) void k_pairing(const PairJob* __restrict__ jobs, uint32_t n,
+__global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ jobs, uint32_t n,
                                                const uint32_t* __restrict__ points,
                                                const uint8_t* __restrict__ pinf,
                                                const uint8_t* __restrict__ pk_status,
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(64) void k_pairing(const PairJob* __restrict__ jobs
                                                uint8_t* __restrict__ status,
                                                const uint8_t* __restrict__ gate,
                                                uint8_t* __restrict__ valid) {
-  __shared__ uint32_t lds_all[p12::GROUPS * p12::LDS_WORDS];
+  __shared__ uint32_t lds_all[p12::BLOCK_LDS_WORDS];
   const int lane = threadIdx.x;
   const int grp = lane / 12;
   const p12::Lane L = p12::lane_setup(lds_all, lane);

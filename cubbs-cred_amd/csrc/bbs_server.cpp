@@ -273,9 +273,16 @@ void Server::worker(unsigned slot) {
   (void)slot;
   Arena ar;
   try {
+    {
+      Context& c = ctx();
+      std::lock_guard<std::mutex> g(c.mu);
+      init_locked(c);
+      for (int i = 0; i < 4; i++) {
+        ar.streams[i] = next_stream_locked(c);
+        ar.aux[i] = next_stream_locked(c);
+      }
+    }
     for (int i = 0; i < 4; i++) {
-      check(hipStreamCreateWithFlags(&ar.streams[i], hipStreamNonBlocking));
-      check(hipStreamCreateWithFlags(&ar.aux[i], hipStreamNonBlocking));
       check(hipEventCreateWithFlags(&ar.fork[i], hipEventDisableTiming));
       check(hipEventCreateWithFlags(&ar.join[i], hipEventDisableTiming));
     }

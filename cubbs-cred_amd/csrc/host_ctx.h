@@ -26,6 +26,8 @@ struct Context {
   bool ready = false;
   bool failed = false;
   hipStream_t setup = nullptr;
+  std::vector<hipStream_t> pool;  // work streams, one per hardware queue
+  uint32_t pool_next = 0;
   // generators: slot 0 = P1, slot 1 = Q1, slot 1 + j = H_j
   uint32_t gen_slots = 0, gen_cap = 0;
   uint32_t* gen_affine = nullptr;
@@ -52,5 +54,7 @@ void init_locked(Context& c);
 void ensure_generators_locked(Context& c, uint32_t slots);
 void lookup_pks_locked(Context& c, const std::vector<const uint8_t*>& pks,
                        std::vector<uint32_t>& slots_out);
+// Next work stream of the pool (round-robin); the pool owns it.
+hipStream_t next_stream_locked(Context& c);
 
 }  // namespace cubbs

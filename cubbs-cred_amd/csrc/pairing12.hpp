@@ -3,8 +3,8 @@
 // An Fp12 element is held in the basis 1, w, .., w^5 (w^6 = xi = 1 + u); lane
 // (k, c) of a 12-lane group owns component c (u^0 or u^1) of the Fp2
 // coefficient of w^k.  A wavefront carries 5 checks (lanes 0..59); the last
-// 4 lanes form a sixth group that computes on its own scratch LDS region and
-// is discarded, so every lane runs the same instruction stream.
+// 4 lanes run the same instruction stream on the fifth check's LDS without
+// storing, and their results are discarded.
 //
 // Every output coefficient is a short sum of Fp products.  The products are
 // accumulated unreduced in 27 64-bit columns (28-bit limbs, at most 12
@@ -29,11 +29,13 @@ namespace p12 {
 constexpr int E = FPW;                                   // words per LDS entry
 enum { EY0, EY1, E2Y0, E2Y1, ED, ES, ENS, EN1, NE };     // derived entries per slot
 constexpr int SLOT = NE * E;
-constexpr int REGION = 6 * SLOT;
-constexpr int MISC = 16;
-constexpr int LDS_WORDS = 2 * REGION + E + MISC;         // F, G, zero entry, flags
-constexpr int GROUPS = 6;                                // 5 checks + the spare lanes
+constexpr int REGION = 6 * SLOT;                         // rich entries of one operand
+constexpr int RAW = 6 * 2 * E;                           // plain coefficients of one operand
+constexpr int LDS_WORDS = REGION + RAW;                  // per check
 constexpr int CHECKS_PER_BLOCK = 5;
+// LDS of a 64-thread block: 5 checks and one shared zero entry (the 4 spare
+// lanes alias the last check's region and never store)
+constexpr int BLOCK_LDS_WORDS = CHECKS_PER_BLOCK * LDS_WORDS + E;
 // per check: l2 = -lambda x_P for 68 lines x 2 pairings x 2 components
 constexpr int L2_WORDS = MILLER_LINES * 2 * 2 * E;
 constexpr int LINE_WORDS = 4 * E;  // a prepared line: lambda, c in Fp2
@@ -43,11 +45,11 @@ constexpr int LINE_WORDS = 4 * E;  // a prepared line: lambda, c in Fp2
 typedef __attribute__((address_space(3))) uint32_t lds_t;
 
 struct Lane {
-  lds_t* F;  // rich region of f
-  lds_t* G;  // rich region of the second operand
+  lds_t* F;  // rich entries of f (of g in a multiplication)
+  lds_t* X;  // plain coefficients of f in a multiplication; flags of is_one
   lds_t* Z;  // a zero entry
-  lds_t* M;  // flags
   int k, c;
+  bool wr;   // false for the spare lanes: they compute but never store
 };
 
 struct Wide {
@@ -145,24 +147,28 @@ DEV void put_rich(lds_t* R, const Fp& v, const Lane& L) {
   sel(y1, L.c, v, o);
   fp_zero(z);
   lds_t* S = R + L.k * SLOT;
-  st(S + (L.c ? EY1 : EY0) * E, v);
-  lz5(t, v, v, 0, z, z);
-  st(S + (L.c ? E2Y1 : E2Y0) * E, t);
+  Fp t1, t2v, t3;
+  lz5(t1, v, v, 0, z, z);
   sel(a, L.c, z, y0);
-  lz5(t, a, z, 2, y1, z);  // c=0: y0 + 2p - y1   c=1: 2p - y1
-  st(S + (L.c ? EN1 : ED) * E, t);
+  lz5(t2v, a, z, 2, y1, z);  // c=0: y0 + 2p - y1   c=1: 2p - y1
   sel(b, L.c, z, y1);
   sel(s2, L.c, y0, z);
   sel(t2, L.c, y1, z);
-  lz5(t, a, b, L.c ? 4 : 0, s2, t2);  // c=0: y0 + y1   c=1: 4p - y0 - y1
-  st(S + (L.c ? ENS : ES) * E, t);
+  lz5(t3, a, b, L.c ? 4 : 0, s2, t2);  // c=0: y0 + y1   c=1: 4p - y0 - y1
+  if (L.wr) {
+    st(S + (L.c ? EY1 : EY0) * E, v);
+    st(S + (L.c ? E2Y1 : E2Y0) * E, t1);
+    st(S + (L.c ? EN1 : ED) * E, t2v);
+    st(S + (L.c ? ENS : ES) * E, t3);
+  }
 }
 
-// f <- f * g
-DEVN void mul(Fp& f, const Fp& g, Lane L) {
+// f <- f * g (inlined into the loops that use it; mul() below is the
+// out-of-line form for the straight-line parts of the final exponentiation)
+DEV void mul_in(Fp& f, const Fp& g, const Lane& L) {
   __syncthreads();
-  put_rich(L.F, f, L);
-  put_rich(L.G, g, L);
+  put_rich(L.F, g, L);
+  if (L.wr) st(L.X + (2 * L.k + L.c) * E, f);
   __syncthreads();
   Wide w;
   w_zero(w);
@@ -171,10 +177,10 @@ DEVN void mul(Fp& f, const Fp& g, Lane L) {
     int j = L.k - i;
     const bool xi = j < 0;
     j += xi ? 6 : 0;
-    const lds_t* X = L.F + i * SLOT;
-    const lds_t* Y = L.G + j * SLOT;
-    mac_at(w, X + EY0 * E, Y + ysel0(L.c, xi) * E);
-    mac_at(w, X + EY1 * E, Y + ysel1(L.c, xi) * E);
+    const lds_t* X = L.X + 2 * i * E;
+    const lds_t* Y = L.F + j * SLOT;
+    mac_at(w, X, Y + ysel0(L.c, xi) * E);
+    mac_at(w, X + E, Y + ysel1(L.c, xi) * E);
   }
   w_reduce(f, w);
 }
@@ -188,7 +194,7 @@ __constant__ static const int8_t SQ_TERMS[6][4][4] = {
     {{0, 4, 1, 0}, {1, 3, 1, 0}, {2, 2, 0, 0}, {5, 5, 0, 1}},
     {{0, 5, 1, 0}, {1, 4, 1, 0}, {2, 3, 1, 0}, {-1, 0, 0, 0}}};
 
-DEVN void sqr(Fp& f, Lane L) {
+DEV void sqr(Fp& f, const Lane& L) {
   __syncthreads();
   put_rich(L.F, f, L);
   __syncthreads();
@@ -213,8 +219,8 @@ DEVN void sqr(Fp& f, Lane L) {
 
 // f <- f * (l0 + l2 w^2 + yp w^3): l0 = (c0, c1), l2 = (m0, m1) below 2p + 1,
 // yp in Fp (the evaluated Miller-loop line, see pairing.hpp).
-DEVN void mul_line(Fp& f, const Fp& c0, const Fp& c1, const Fp& m0, const Fp& m1, const Fp& yp,
-                   Lane L) {
+DEV void mul_line(Fp& f, const Fp& c0, const Fp& c1, const Fp& m0, const Fp& m1, const Fp& yp,
+                  const Lane& L) {
   __syncthreads();
   put_rich(L.F, f, L);
   __syncthreads();
@@ -293,7 +299,7 @@ __constant__ static const int8_t GS_TERMS[3][2][3][4] = {
 __constant__ static const int8_t GS_SLOT[6][2] = {  // {P, type}
     {0, 0}, {2, 2}, {1, 0}, {0, 1}, {2, 0}, {1, 1}};
 
-DEVN void cyc_sqr(Fp& f, Lane L) {
+DEV void cyc_sqr(Fp& f, const Lane& L) {
   __syncthreads();
   put_rich(L.F, f, L);
   __syncthreads();
@@ -316,6 +322,8 @@ DEVN void cyc_sqr(Fp& f, Lane L) {
   fp_add(v, t3, d);
   sel(f, (L.k & 1) == 0, u, v);
 }
+
+DEVN void mul(Fp& f, const Fp& g, Lane L) { mul_in(f, g, L); }
 
 // f <- conj(f) (x -> x^(p^6)): odd slots negated
 DEV void conj(Fp& f, const Lane& L) {
@@ -353,9 +361,9 @@ DEVN void frob2(Fp& f, Lane L) {
 DEVN void inv(Fp& f, Lane L) {
   lds_t* A = L.F;
   __syncthreads();
-  st(A + L.k * SLOT + (L.c ? EY1 : EY0) * E, f);
+  if (L.wr) st(A + L.k * SLOT + (L.c ? EY1 : EY0) * E, f);
   __syncthreads();
-  if (L.k == 0 && L.c == 0) {
+  if (L.k == 0 && L.c == 0 && L.wr) {
     Fp12 x, y;
     Fp2* xs[6] = {&x.c0.c0, &x.c1.c0, &x.c0.c1, &x.c1.c1, &x.c0.c2, &x.c1.c2};
     Fp2* ys[6] = {&y.c0.c0, &y.c1.c0, &y.c0.c1, &y.c1.c1, &y.c0.c2, &y.c1.c2};
@@ -381,7 +389,7 @@ DEVN void exp_by_x(Fp& r, const Fp& a, Lane L) {
 #pragma unroll 1
   for (int i = 62; i >= 0; i--) {
     cyc_sqr(acc, L);
-    if ((BLS_X_ABS >> i) & 1) mul(acc, a, L);
+    if ((BLS_X_ABS >> i) & 1) mul_in(acc, a, L);
   }
   conj(acc, L);
   r = acc;
@@ -472,27 +480,27 @@ DEVN bool is_one(const Fp& f, Lane L) {
   Fp one;
   fp_one(one);
   const bool ok = (L.k == 0 && L.c == 0) ? fp_eq(f, one) : fp_is_zero(f);
-  L.M[2 * L.k + L.c] = ok ? 1u : 0u;
+  if (L.wr) L.X[2 * L.k + L.c] = ok ? 1u : 0u;
   __syncthreads();
   bool all = true;
-  for (int i = 0; i < 12; i++) all = all && L.M[i] != 0;
+  for (int i = 0; i < 12; i++) all = all && L.X[i] != 0;
   return all;
 }
 
-// Lane setup for a 64-thread block: group g = lane / 12 (g = 5 is the spare
-// group of 4 lanes), slot k and component c inside the group.
+// Lane setup for a 64-thread block (BLOCK_LDS_WORDS of LDS): group
+// g = lane / 12, slot k and component c inside the group.  Lanes 60..63
+// (g = 5) run on the last check's region without storing.
 DEV Lane lane_setup(uint32_t* lds_all, int lane) {
   Lane L;
   const int g = lane / 12, gl = lane % 12;
-  lds_t* base = (lds_t*)lds_all + g * LDS_WORDS;
+  L.wr = g < CHECKS_PER_BLOCK;
+  lds_t* base = (lds_t*)lds_all + (L.wr ? g : CHECKS_PER_BLOCK - 1) * LDS_WORDS;
   L.F = base;
-  L.G = base + REGION;
-  L.Z = base + 2 * REGION;
-  L.M = base + 2 * REGION + E;
+  L.X = base + REGION;
+  L.Z = (lds_t*)lds_all + CHECKS_PER_BLOCK * LDS_WORDS;
   L.k = gl >> 1;
   L.c = gl & 1;
-  // the zero entry
-  if (gl == 0) {
+  if (lane == 0) {
 #pragma unroll
     for (int i = 0; i < E; i++) L.Z[i] = 0;
   }
