@@ -36,6 +36,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cubbs-cred_amd"))
+# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default); the
+# serving loop keeps up to 3 slots x 8 streams busy, so give the process 16
+# (read by the HIP runtime at initialisation, before any device call below).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 N_VERIFY = 8192
 N_ISSUERS = 256
