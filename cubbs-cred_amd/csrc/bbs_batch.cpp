@@ -302,8 +302,7 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
     uint32_t first = p.n_partials;
     p.var.push_back(VarJob{pa, se, p.partials(VAR_PARTIALS)});  // (-e) A
     p.comb(term, L + 2, CH);                                  // B
-    p.groups1.push_back(ReduceGroup{first, p.n_partials - first});
-    p.out1.push_back(pb);                                     // B - e A
+    p.reduce_to(first, p.n_partials - first, pb);             // B - e A
     p.pairs.push_back(PairJob{k, pa, pb, pk_slot[k], PAIR_NEG_G2, 0});
     term += L + 2;
   }
@@ -428,13 +427,11 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
     p.var.push_back(VarJob{pbb, s_c, p.partials(VAR_PARTIALS)});
     p.var.push_back(VarJob{pab, s_ehat, p.partials(VAR_PARTIALS)});
     p.var.push_back(VarJob{pd, s_r1, p.partials(VAR_PARTIALS)});
-    p.groups1.push_back(ReduceGroup{f1, 3 * VAR_PARTIALS});
-    p.out1.push_back(pt1);
+    p.reduce_to(f1, 3 * VAR_PARTIALS, pt1);
     uint32_t f2 = p.n_partials;
     p.var.push_back(VarJob{pd, s_r3, p.partials(VAR_PARTIALS)});
     p.comb(term, L + 2, CH);
-    p.groups1.push_back(ReduceGroup{f2, p.n_partials - f2});
-    p.out1.push_back(pt2);
+    p.reduce_to(f2, p.n_partials - f2, pt2);
     p.pvch.push_back(PvChallengeJob{ph, (uint32_t)it.presentation_header.len, k, R, disc_begin,
                                     pab, pbb, pd, pt1, pt2, s_dom, s_c});
     p.pairs.push_back(PairJob{k, pab, pbb, pk_slot[k], PAIR_NEG_G2, 0});
@@ -501,8 +498,7 @@ std::unique_ptr<Batch> plan_sign(const CuBbsBbsGpuSignItem* items, size_t n) {
     uint32_t se = p.scalars(1), pa = p.points(1);
     uint32_t first = p.n_partials;
     p.comb(term, L + 2, CH);
-    p.groups1.push_back(ReduceGroup{first, p.n_partials - first});
-    p.out1.push_back(pa);
+    p.reduce_to(first, p.n_partials - first, pa);
     p.sign.push_back(SignJob{sk, k, term, L, se, pa, pk_slot[k]});
     term += L + 2;
   }
@@ -604,8 +600,7 @@ std::unique_ptr<Batch> plan_proof_gen(const CuBbsBbsGpuProofGenItem* items, size
     for (uint32_t m = 0; m < L; m++) p.fills.push_back(TermFill{term + 2 + m, s_m + m, 0, 0});
     uint32_t fb = p.n_partials;
     p.comb(term, L + 2, CH);
-    p.groups1.push_back(ReduceGroup{fb, p.n_partials - fb});
-    p.out1.push_back(pb);
+    p.reduce_to(fb, p.n_partials - fb, pb);
     term += L + 2;
     // sum_U H_j m~_j (part of T2)
     uint32_t disc_begin = (uint32_t)p.disc_index.size();
@@ -745,13 +740,10 @@ void launch_batch(Batch& b) {
     Timed t(b, "decode_g1");
     launch_decode_g1(s, b.dec_g1, b.n_dec_g1, b.blob, b.points, b.pinf, b.status);
   }
-  if (b.n_h2s) {
-    Timed t(b, "h2s_messages");
-    launch_h2s(s, b.h2s, b.n_h2s, b.blob, b.map_dst, b.map_dst_len, b.scalars);
-  }
-  if (b.n_dom) {
-    Timed t(b, "domain");
-    launch_domain(s, b.dom, b.n_dom, b.blob, b.pk_bytes, b.gen_comp, b.scalars);
+  if (b.n_h2s || b.n_dom) {
+    Timed t(b, "hash_domain");
+    launch_hash_stage(s, b.dom, b.n_dom, b.h2s, b.n_h2s, b.blob, b.map_dst, b.map_dst_len,
+                      b.pk_bytes, b.gen_comp, b.scalars);
   }
   if (b.kind == KIND_PROOF_GEN && b.seeded) {
     Timed t(b, "pg_chacha");

@@ -166,6 +166,10 @@ void launch_h2s(hipStream_t s, const H2sJob* jobs, uint32_t n, const uint8_t* bl
                 const uint8_t* dst, uint32_t dst_len, uint32_t* out_scalars);
 void launch_hash_jobs(hipStream_t s, int kind, const H2sJob* jobs, uint32_t n, const uint8_t* blob,
                       const uint8_t* dst, uint32_t dst_len, uint8_t* out);
+// message hashes (H2sJob, DST dst) and domains (DomainJob) in one launch
+void launch_hash_stage(hipStream_t s, const DomainJob* dom, uint32_t n_dom, const H2sJob* h2s,
+                       uint32_t n_h2s, const uint8_t* blob, const uint8_t* dst, uint32_t dst_len,
+                       const uint8_t* pk_bytes, const uint8_t* gen_comp, uint32_t* out);
 void launch_domain(hipStream_t s, const DomainJob* jobs, uint32_t n, const uint8_t* hdr_blob,
                    const uint8_t* pk_bytes, const uint8_t* gen_comp, uint32_t* out_scalars);
 void launch_gen_seeds(hipStream_t s, uint32_t first, uint32_t count, uint32_t* v_state,

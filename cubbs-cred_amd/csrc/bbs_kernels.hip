@@ -75,14 +75,8 @@ void launch_hash_jobs(hipStream_t s, int kind, const H2sJob* jobs, uint32_t n, c
 
 // domain = hash_to_scalar(PK || I2OSP(L, 8) || Q1 || H_1..H_L || api_id ||
 //                         I2OSP(len(header), 8) || header, api_id || "H2S_")
-__global__ __launch_bounds__(256) void k_domain(const DomainJob* __restrict__ jobs, uint32_t n,
-                                               const uint8_t* __restrict__ hdr_blob,
-                                               const uint8_t* __restrict__ pk_bytes,
-                                               const uint8_t* __restrict__ gen_comp,
-                                               uint32_t* __restrict__ out) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  DomainJob j = jobs[i];
+DEVN void domain_job(const DomainJob& j, const uint8_t* hdr_blob, const uint8_t* pk_bytes,
+                     const uint8_t* gen_comp, uint32_t* out) {
   Sha256 s;
   sha_init_after_zpad(s);
   sha_bytes_global(s, pk_bytes + (size_t)j.pk_slot * 96, 96);
@@ -97,6 +91,42 @@ __global__ __launch_bounds__(256) void k_domain(const DomainJob* __restrict__ jo
   Fr d;
   scalar_from_okm48(d, okm);
   store_fr(out + (size_t)j.out_index * FR_WORDS, d);
+}
+__global__ __launch_bounds__(256) void k_domain(const DomainJob* __restrict__ jobs, uint32_t n,
+                                               const uint8_t* __restrict__ hdr_blob,
+                                               const uint8_t* __restrict__ pk_bytes,
+                                               const uint8_t* __restrict__ gen_comp,
+                                               uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  domain_job(jobs[i], hdr_blob, pk_bytes, gen_comp, out);
+}
+// Message hashes and domains are independent: one launch runs both, the long
+// domain jobs on lanes [0, n_dom), the message jobs on [n_dom, n_dom + n_h2s).
+__global__ __launch_bounds__(256) void k_hash_stage(const DomainJob* __restrict__ dom, uint32_t n_dom,
+                                                   const H2sJob* __restrict__ h2s, uint32_t n_h2s,
+                                                   const uint8_t* __restrict__ blob,
+                                                   const uint8_t* __restrict__ dst, uint32_t dst_len,
+                                                   const uint8_t* __restrict__ pk_bytes,
+                                                   const uint8_t* __restrict__ gen_comp,
+                                                   uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_dom) {
+    domain_job(dom[i], blob, pk_bytes, gen_comp, out);
+  } else if (i < n_dom + n_h2s) {
+    const H2sJob j = h2s[i - n_dom];
+    Fr sc;
+    hash_to_scalar_global(sc, blob + j.off, j.len, dst, dst_len);
+    store_fr(out + (size_t)j.out_index * FR_WORDS, sc);
+  }
+}
+void launch_hash_stage(hipStream_t s, const DomainJob* dom, uint32_t n_dom, const H2sJob* h2s,
+                       uint32_t n_h2s, const uint8_t* blob, const uint8_t* dst, uint32_t dst_len,
+                       const uint8_t* pk_bytes, const uint8_t* gen_comp, uint32_t* out) {
+  const uint32_t n = n_dom + n_h2s;
+  if (!n) return;
+  hipLaunchKernelGGL(k_hash_stage, GRID(n, 256), 0, s, dom, n_dom, h2s, n_h2s, blob, dst, dst_len,
+                     pk_bytes, gen_comp, out);
 }
 void launch_domain(hipStream_t s, const DomainJob* jobs, uint32_t n, const uint8_t* hdr_blob,
                    const uint8_t* pk_bytes, const uint8_t* gen_comp, uint32_t* out) {

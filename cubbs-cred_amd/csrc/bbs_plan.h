@@ -56,6 +56,31 @@ struct Plan {
     if (n) std::memcpy(blob.data() + off, p, n);
     return off;
   }
+  // points[out] = sum of partials [begin, begin + count).  A long range is
+  // first summed in parallel tree levels (runs of TREE_FANIN partials per lane
+  // into new partial slots) down to at most FINAL_MAX partials.
+  static constexpr uint32_t TREE_FANIN = 8, FINAL_MAX = 16;
+  void reduce_to(uint32_t begin, uint32_t count, uint32_t out) {
+    size_t level = 0;
+    while (count > FINAL_MAX) {
+      if (pst_groups.size() <= level) {
+        pst_groups.emplace_back();
+        pst_out.emplace_back();
+      }
+      const uint32_t groups = (count + TREE_FANIN - 1) / TREE_FANIN;
+      const uint32_t first = partials(groups);
+      for (uint32_t g = 0; g < groups; g++) {
+        const uint32_t b = begin + g * TREE_FANIN;
+        pst_groups[level].push_back(ReduceGroup{b, std::min(TREE_FANIN, begin + count - b)});
+        pst_out[level].push_back(first + g);
+      }
+      begin = first;
+      count = groups;
+      level++;
+    }
+    groups1.push_back(ReduceGroup{begin, count});
+    out1.push_back(out);
+  }
   // Fixed-base terms [term, term + count) in chunks, partials appended to the
   // caller's running partial range.
   void comb(uint32_t term, uint32_t count, uint32_t chunk) {

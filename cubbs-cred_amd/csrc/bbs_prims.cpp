@@ -34,8 +34,6 @@
 namespace cubbs {
 namespace {
 
-constexpr uint32_t TREE_FANIN = 8;   // partials per lane in a tree level
-constexpr uint32_t FINAL_MAX = 16;   // partials summed by the final (affine) group
 constexpr size_t LARGE_TERMS = 1024; // route id 3 from this many terms on
 constexpr uint8_t ROUTE_ITEM = CUBBS_BBS_MSM_ROUTE_CUBBS_ITEM;
 constexpr uint8_t ROUTE_COMMON = CUBBS_BBS_MSM_ROUTE_CUBBS_COMMON_POINTS;
@@ -81,27 +79,8 @@ struct Prog {
   }
   // Sum of the partials [first, n_partials) into a new affine point slot.
   uint32_t finish(uint32_t first) {
-    uint32_t begin = first, count = p.n_partials - first;
-    size_t level = 0;
-    while (count > FINAL_MAX) {
-      if (p.pst_groups.size() <= level) {
-        p.pst_groups.emplace_back();
-        p.pst_out.emplace_back();
-      }
-      const uint32_t groups = (count + TREE_FANIN - 1) / TREE_FANIN;
-      const uint32_t out = p.partials(groups);
-      for (uint32_t g = 0; g < groups; g++) {
-        const uint32_t b = begin + g * TREE_FANIN;
-        p.pst_groups[level].push_back(ReduceGroup{b, std::min(TREE_FANIN, begin + count - b)});
-        p.pst_out[level].push_back(out + g);
-      }
-      begin = out;
-      count = groups;
-      level++;
-    }
     const uint32_t pt = p.points(1);
-    p.groups1.push_back(ReduceGroup{begin, count});
-    p.out1.push_back(pt);
+    p.reduce_to(first, p.n_partials - first, pt);
     return pt;
   }
   size_t compress(uint32_t pt) {
