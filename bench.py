@@ -230,6 +230,24 @@ def per_request_gpu(g, vreqs, n=16):
     return n / (time.time() - t)
 
 
+PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                       "bench_kernels_verify8192_proofverify2048.md")
+
+
+def profiled_traffic():
+    """HBM bytes per Verify k_pairing launch, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration written
+    there); None when the summary is absent."""
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,7 +319,7 @@ def main():
     roofline = {"bound": "valu", "kernel": "k_pairing (Verify batch)",
                 "achieved": round(achieved_mad_s / 1e12, 3), "peak": round(peak_mad_s / 1e12, 3),
                 "unit": "Tmad/s (v_mad_u64_u32)", "frac": round(achieved_mad_s / peak_mad_s, 4),
-                "traffic": None, "launch_ms": round(pairing_ms, 3),
+                "traffic": profiled_traffic(), "launch_ms": round(pairing_ms, 3),
                 "work_per_launch": "%d pairing checks x %d FpM x %d mads" % (checks, FPM_PER_PAIRING_CHECK, MADS_PER_FPM)}
     out = {"metric": "Verify+ProofVerify ops/s on 1 MI355X; p50/p99 latency @ fixed load",
            "value": round(value, 1), "unit": "ops/s", "n_gpus": 1, "steps": args.steps,
