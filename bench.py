@@ -75,6 +75,7 @@ def build_workload(g, seed=b"cubbs-hip/bench"):
     t = time.time()
     rc, sigs = g.sign_batch(sreqs)
     assert rc == 0 and all(c == 0 for c, _ in sigs)
+    calls = {"sign_8192_uneven_s": time.time() - t}
     log("signed %d in %.2fs" % (N_VERIFY, time.time() - t))
     vreqs, expect_v = [], []
     for (sk, pk, hdr, msgs), (_, sig) in zip(sreqs, sigs):
@@ -101,10 +102,78 @@ def build_workload(g, seed=b"cubbs-hip/bench"):
     t = time.time()
     rc, proofs = g.proof_gen_batch(greqs, seeds=seeds)
     assert rc == 0 and all(c == 0 for c, _ in proofs)
+    calls["proof_gen_2048_s"] = time.time() - t
     log("proofs generated %d in %.2fs" % (N_PROOFS, time.time() - t))
     pvreqs = [(pk, proof, hdr, ph, [msgs[i] for i in idx], idx, len(msgs))
               for (pk, sig, hdr, ph, msgs, idx), (_, proof) in zip(greqs, proofs)]
-    return vreqs, expect_v, pvreqs, kps
+    # whole-op ABI calls from host buffers (packing, upload and download included)
+    abi_ops = {"sign_ops_s": round(N_VERIFY / calls["sign_8192_uneven_s"], 1),
+               "proof_gen_ops_s": round(N_PROOFS / calls["proof_gen_2048_s"], 1),
+               "note": "one whole-op ABI call each (8192 uneven Sign; 2048 seeded ProofGen at L=32), "
+                       "host packing and PCIe included, first call of the process"}
+    return vreqs, expect_v, pvreqs, kps, abi_ops
+
+
+MSM_ROWS = 4096
+MSM_TERMS = 32
+PIP_POINTS = 1 << 16
+R_ORDER = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
+G1_GENERATOR = bytes.fromhex(
+    "97f1d3a73197d7942695638c4fa9ac0fc3688c4f9774b905a14e3a3f171bac586c55e83ff97a1aeffb3af00adb22c6bb")
+
+
+def msm_run(g, steps=10, warmup=2):
+    """Batched G1 MSM kernels alone: 4096 fixed-base MSMs of 32 terms over the
+    ciphersuite generators [P1, Q1, H_1..H_30] (comb route, one common-points
+    group) and one variable-base MSM of 2^16 points k_i G1 (Pippenger route),
+    scalars uniform mod r.  Both staged batches stay resident; a step
+    launches both on their own streams and waits for both."""
+    rng = random.Random(hashlib.sha256(b"cubbs-hip/msm").digest())
+    rc, gens = g.generators(MSM_TERMS)
+    assert rc == 0
+    rows = [[rng.randrange(R_ORDER).to_bytes(32, "big") for _ in range(MSM_TERMS)]
+            for _ in range(MSM_ROWS)]
+    rc, pts = g.g1_mul([G1_GENERATOR] * PIP_POINTS,
+                       [rng.randrange(1, R_ORDER).to_bytes(32, "big") for _ in range(PIP_POINTS)])
+    assert rc == 0 and all(p is not None for p in pts)
+    scs = [rng.randrange(R_ORDER).to_bytes(32, "big") for _ in range(PIP_POINTS)]
+    cb = g.StagedBatch(g.BATCH_G1_MSM_COMMON_POINTS, [(gens, rows)])
+    pb = g.StagedBatch(g.BATCH_G1_MSM, [(pts, scs)])
+    del rows, pts, scs
+    for _ in range(warmup):
+        cb.launch()
+        pb.launch()
+        cb.sync()
+        pb.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        cb.launch()
+        pb.launch()
+        cb.sync()
+        pb.sync()
+    el = time.perf_counter() - t
+    rc_ = cb.results() + pb.results()
+    assert all(c == 0 for c, _ in rc_), "MSM rows failed"
+    routes = sorted(set(v for _, v in rc_))
+    first = cb.outputs() + pb.outputs()
+    cb.launch()
+    pb.launch()
+    cb.sync()
+    pb.sync()
+    cb.results()
+    pb.results()
+    assert cb.outputs() + pb.outputs() == first, "MSM outputs differ between launches"
+    ct, ck = cb.profile(3)
+    pt, pk = pb.profile(3)
+    cb.free()
+    pb.free()
+    return {"workload": "4096 x 32-term fixed-base MSMs over [P1, Q1, H_1..H_30] (comb) + "
+                        "one 2^16-point variable-base MSM of k_i G1 (Pippenger), scalars uniform mod r",
+            "msm_per_s": round((MSM_ROWS + 1) * steps / el, 1), "ms_per_step": round(el * 1e3 / steps, 3),
+            "steps": steps, "routes": routes,
+            "comb_4096x32_ms": round(ct, 3), "pippenger_2_16_ms": round(pt, 3),
+            "kernel_ms": {"comb": {k: round(v, 3) for k, v in ck.items()},
+                          "pippenger": {k: round(v, 3) for k, v in pk.items()}}}
 
 
 def cpu_baseline(vreqs, pvreqs, budget_s=20.0):
@@ -258,6 +327,7 @@ def main():
     ap.add_argument("--serving-rate", type=float, default=20000.0)
     ap.add_argument("--serving-seconds", type=float, default=15.0)
     ap.add_argument("--profile-only", action="store_true", help="a few launches, no baselines")
+    ap.add_argument("--no-msm", action="store_true", help="skip the batched-MSM measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -280,7 +350,7 @@ def main():
     if rc != 0 or not hw:
         raise SystemExit("no usable HIP device (backend_info rc=%d)" % rc)
     t0 = time.time()
-    vreqs, expect_v, pvreqs, kps = build_workload(g)
+    vreqs, expect_v, pvreqs, kps, abi_ops = build_workload(g)
     log("workload built in %.1fs" % (time.time() - t0))
     vb = g.StagedBatch(g.BATCH_VERIFY, vreqs)
     pb = g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs)
@@ -332,8 +402,13 @@ def main():
                       "verify_items": N_VERIFY, "proof_verify_items": N_PROOFS,
                       "msm_terms": vb.shape()["msm_terms"] + pb.shape()["msm_terms"]},
            "roofline": roofline,
+           "abi_calls": abi_ops,
            "kernel_ms": {"verify": {k: round(v, 3) for k, v in vk.items()},
                          "proof_verify": {k: round(v, 3) for k, v in pk.items()}}}
+    if not args.no_msm:
+        t = time.time()
+        out["msm"] = msm_run(g, steps=3 if args.profile_only else 10, warmup=1 if args.profile_only else 2)
+        log("msm run %.1fs: %s" % (time.time() - t, {k: out["msm"][k] for k in ("msm_per_s", "ms_per_step", "comb_4096x32_ms", "pippenger_2_16_ms")}))
     if not args.profile_only:
         out["per_request_gpu_ops_s"] = round(per_request_gpu(g, vreqs), 2)
         if not args.no_serving:

@@ -346,6 +346,9 @@ void* cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void* items,
       b = plan_verify(static_cast<const CuBbsBbsGpuVerifyItem*>(items), item_count);
     } else if (kind == CUBBS_HIP_BATCH_PROOF_VERIFY) {
       b = plan_proof_verify(static_cast<const CuBbsBbsGpuProofVerifyItem*>(items), item_count);
+    } else if (kind == CUBBS_HIP_BATCH_G1_MSM || kind == CUBBS_HIP_BATCH_G1_MSM_COMMON_POINTS) {
+      b = stage_primitive(kind, items, item_count, st);
+      if (!b) return nullptr;
     } else {
       st = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
       return nullptr;
@@ -380,7 +383,11 @@ int32_t cubbs_hip_batch_sync(void* batch) {
 int32_t cubbs_hip_batch_results(void* batch, CuBbsBbsGpuResult* results) {
   if (!batch || !results) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   try {
-    collect_batch(*static_cast<Batch*>(batch), results, nullptr, nullptr);
+    Batch* b = static_cast<Batch*>(batch);
+    if (b->kind == KIND_PRIM)
+      collect_primitive(*b, results);
+    else
+      collect_batch(*b, results, nullptr, nullptr);
     return CUBBS_BBS_GPU_SUCCESS;
   } catch (const GpuError&) {
     return CUBBS_BBS_GPU_ERR_INTERNAL;
@@ -388,6 +395,16 @@ int32_t cubbs_hip_batch_results(void* batch, CuBbsBbsGpuResult* results) {
 }
 
 void cubbs_hip_batch_free(void* batch) { delete static_cast<Batch*>(batch); }
+
+int32_t cubbs_hip_batch_outputs(void* batch, uint8_t* out, size_t out_cap, size_t* out_len) {
+  if (!batch || (out_cap && !out)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  Batch* b = static_cast<Batch*>(batch);
+  if (b->kind != KIND_PRIM) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  if (out_cap < b->h_out.size()) return CUBBS_BBS_GPU_ERR_INSUFFICIENT_OUTPUT;
+  if (!b->h_out.empty()) std::memcpy(out, b->h_out.data(), b->h_out.size());
+  if (out_len) *out_len = b->h_out.size();
+  return CUBBS_BBS_GPU_SUCCESS;
+}
 
 int32_t cubbs_hip_batch_shape(void* batch, uint64_t* items, uint64_t* msm_terms,
                               uint64_t* comb_madds, uint64_t* pairings) {

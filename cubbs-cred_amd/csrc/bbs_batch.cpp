@@ -108,6 +108,8 @@ void materialize(Batch& b, Plan& p) {
     o_lv.emplace_back(place(p.pst_groups[l].data(), p.pst_groups[l].size() * sizeof(ReduceGroup)),
                       place(p.pst_out[l].data(), p.pst_out[l].size() * 4));
   size_t o_comp = place(p.comp.data(), p.comp.size() * sizeof(CompressJob));
+  size_t o_pipt = place(p.pip_terms.data(), p.pip_terms.size() * sizeof(PipTerm));
+  size_t o_pipo = place(p.pip_out.data(), p.pip_out.size() * 4);
   size_t o_sign = place(p.sign.data(), p.sign.size() * sizeof(SignJob));
   size_t o_pvch = place(p.pvch.data(), p.pvch.size() * sizeof(PvChallengeJob));
   size_t o_pg = place(p.pg.data(), p.pg.size() * sizeof(PgJob));
@@ -156,6 +158,9 @@ void materialize(Batch& b, Plan& p) {
     b.levels.push_back(Batch::Level{reinterpret_cast<ReduceGroup*>(D + o_lv[l].first),
                                     reinterpret_cast<uint32_t*>(D + o_lv[l].second),
                                     (uint32_t)p.pst_groups[l].size()});
+  b.pip_terms = reinterpret_cast<PipTerm*>(D + o_pipt);
+  b.pip_out = reinterpret_cast<uint32_t*>(D + o_pipo);
+  if (!p.pip_terms.empty()) b.pip = pip_shape((uint32_t)p.pip_terms.size(), (uint32_t)p.pip_out.size());
   b.comp = reinterpret_cast<CompressJob*>(D + o_comp);
   b.n_comp = (uint32_t)p.comp.size();
   b.sign = reinterpret_cast<SignJob*>(D + o_sign);
@@ -189,6 +194,17 @@ void materialize(Batch& b, Plan& p) {
   size_t o_va = take(std::max<uint32_t>(b.nv, 1));
   size_t o_out = take(std::max<size_t>(p.out_bytes, 1));
   size_t o_l2 = take(4ull * PAIR_L2_WORDS * std::max<size_t>(p.pairs.size(), 1));
+  const PipShape& ps = b.pip;
+  const size_t pip_on = ps.terms ? 1 : 0;
+  size_t o_pcnt = take(4ull * (ps.nbt + 1) * pip_on);
+  size_t o_poff = take(4ull * (ps.nbt + 1) * pip_on);
+  size_t o_pcur = take(4ull * ps.nbt * pip_on);
+  size_t o_ppof = take(4ull * (ps.nbt + 1) * pip_on);
+  size_t o_pmap = take(4ull * ps.max_pieces * pip_on);
+  size_t o_psrt = take(4ull * ps.max_entries * pip_on);
+  size_t o_phdr = take(16 * pip_on);
+  size_t o_ppcs = take(4ull * G1J_WORDS * ps.max_pieces * pip_on);
+  size_t o_pbkt = take(4ull * G1J_WORDS * ps.nbt * pip_on);
   b.work_bytes = w;
   if (ar)
     b.d_work = ar->alloc_dev(w);
@@ -204,6 +220,11 @@ void materialize(Batch& b, Plan& p) {
   b.valid = W + o_va;
   b.out = W + o_out;
   b.l2buf = reinterpret_cast<uint32_t*>(W + o_l2);
+  if (pip_on) {
+    auto u32 = [&](size_t o) { return reinterpret_cast<uint32_t*>(W + o); };
+    b.pipb = PipBuffers{u32(o_pcnt), u32(o_poff), u32(o_pcur), u32(o_ppof), u32(o_pmap),
+                        u32(o_psrt), u32(o_phdr), u32(o_ppcs), u32(o_pbkt)};
+  }
   b.out_bytes = p.out_bytes;
   b.n_scalars = p.n_scalars;
   b.n_points = p.n_points;
@@ -766,6 +787,31 @@ void launch_batch(Batch& b) {
     launch_comb_msm(s, b.chunks, b.n_chunks, b.term_gen, b.scalars, b.tables, b.partials);
   }
   if (fork) check(hipStreamWaitEvent(s, b.ev_join, 0));
+  if (b.pip.terms) {
+    {
+      Timed t(b, "pip_sort");
+      launch_fill_u32(s, b.pipb.cnt, b.pip.nbt + 1, 0);
+      launch_pip_count(s, b.pip, b.pip_terms, b.pinf, b.scalars, b.pipb);
+      launch_pip_scan(s, b.pip, b.pipb);
+      launch_pip_scatter(s, b.pip, b.pip_terms, b.pinf, b.scalars, b.pipb);
+    }
+    {
+      Timed t(b, "pip_accum");
+      launch_pip_accum(s, b.pip, b.pip_terms, b.points, b.pipb);
+    }
+    {
+      Timed t(b, "pip_bucket");
+      launch_pip_bucket(s, b.pip, b.pipb);
+    }
+    {
+      Timed t(b, "pip_halve");
+      for (uint32_t st = 1; st <= b.pip.K; st++) launch_pip_halve(s, b.pip, st, b.pipb);
+    }
+    {
+      Timed t(b, "pip_final");
+      launch_pip_final(s, b.pip, b.pip_out, b.pipb, b.partials);
+    }
+  }
   for (const auto& lv : b.levels) {
     Timed t(b, "reduce_tree");
     launch_reduce_partials(s, lv.groups, lv.out, lv.n, b.partials);

@@ -19,6 +19,16 @@ constexpr int KIND_PROOF_GEN = 3;
 // of items, results assembled by the caller from h_status / h_valid / output.
 constexpr int KIND_PRIM = 4;
 
+// One output point of a primitive program: its status slots (gst =
+// PRIM_NO_SLOT when the row has no group slot), the offset of its 48-byte
+// encoding in the output area and its result value (route id).
+constexpr uint32_t PRIM_NO_SLOT = 0xffffffffu;
+struct PrimRow {
+  uint32_t st, gst;
+  size_t off;
+  uint8_t value;
+};
+
 struct KernelTimer {
   bool on = false;
   std::vector<std::string> names;
@@ -91,6 +101,11 @@ struct Batch {
     uint32_t n;
   };
   std::vector<Level> levels;          // partial-sum tree, in launch order
+  // Pippenger MSMs (large variable-base MSMs of the primitive entries)
+  PipShape pip{};
+  PipTerm* pip_terms = nullptr;
+  uint32_t* pip_out = nullptr;
+  PipBuffers pipb{};
   CompressJob* comp = nullptr;
   uint32_t n_comp = 0;
   SignJob* sign = nullptr;
@@ -128,6 +143,9 @@ struct Batch {
   const uint32_t* neg_g2_lines = nullptr;
   // host results
   std::vector<uint8_t> h_status, h_valid;
+  // staged primitive programs: output rows and their collected encodings
+  std::vector<PrimRow> rows;
+  std::vector<uint8_t> h_out;
   KernelTimer timer;
   ~Batch();
 };
@@ -142,6 +160,12 @@ std::unique_ptr<Batch> plan_sign(const CuBbsBbsGpuSignItem* items, size_t n);
 std::unique_ptr<Batch> plan_proof_gen(const CuBbsBbsGpuProofGenItem* items, size_t n, int mode,
                                       const uint8_t* seeds32, const uint8_t* scalars_be,
                                       const size_t* scalar_offsets);
+// Staged primitive programs (kinds CUBBS_HIP_BATCH_G1_MSM and
+// CUBBS_HIP_BATCH_G1_MSM_COMMON_POINTS): validated and uploaded, not launched;
+// nullptr with status set on a call-level error.
+std::unique_ptr<Batch> stage_primitive(int kind, const void* items, size_t n, int32_t& status);
+// One result per output row; encodings kept in b.h_out (48 bytes per row).
+void collect_primitive(Batch& b, CuBbsBbsGpuResult* results);
 void launch_batch(Batch& b);
 void collect_batch(Batch& b, CuBbsBbsGpuResult* results, uint8_t* const* out_ptrs,
                    size_t* out_lens);

@@ -227,6 +227,53 @@ void launch_pg_finalize(hipStream_t s, const PgJob* jobs, uint32_t n, const uint
                         uint32_t* scalars, uint8_t* status, uint8_t* proof_out);
 void launch_g2_mul_gen(hipStream_t s, const uint8_t* sk, uint32_t n, uint8_t* pk_out, uint8_t* ok);
 
+// ------------------------------------------------------------------ Pippenger MSM
+// One term of a large variable-base MSM: points[point] * scalars[scalar],
+// summed into the output partial of MSM `job`.
+struct PipTerm {
+  uint32_t point;
+  uint32_t scalar;
+  uint32_t job;
+};
+// Shape shared by the Pippenger MSMs of a batch.  Each scalar is split into
+// two 128-bit GLV halves recoded into W = ceil(129 / c) signed c-bit digits;
+// a digit of magnitude m in window w goes to bucket m - 1 of window w, so a
+// window has B = 2^(c-1) = 2^K buckets.  Buckets are cut into pieces of at
+// most S entries.
+struct PipShape {
+  uint32_t c, W, B, K, S;
+  uint32_t jobs;         // MSMs
+  uint32_t terms;        // PipTerm count
+  uint32_t nbt;          // jobs * W * B buckets
+  uint32_t max_entries;  // 2 * terms * W (bucket entries)
+  uint32_t max_pieces;   // max_entries / S + nbt + 1 (bound on sum ceil(count / S))
+};
+constexpr uint32_t PIP_MAX_TERMS = 1u << 26;
+PipShape pip_shape(uint32_t terms, uint32_t jobs);
+struct PipBuffers {
+  uint32_t* cnt;      // [nbt + 1] entries per bucket
+  uint32_t* off;      // [nbt + 1] first entry of each bucket
+  uint32_t* cursor;   // [nbt] scatter cursors
+  uint32_t* poff;     // [nbt + 1] first piece of each bucket
+  uint32_t* pmap;     // [max_pieces] bucket of each piece
+  uint32_t* sorted;   // [max_entries] term << 2 | half << 1 | negative
+  uint32_t* hdr;      // [2] pieces, entries
+  uint32_t* pieces;   // [max_pieces] Jacobian piece sums
+  uint32_t* buckets;  // [nbt] Jacobian bucket sums, halved in place
+};
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
+void launch_pip_count(hipStream_t s, const PipShape& sh, const PipTerm* terms, const uint8_t* pinf,
+                      const uint32_t* scalars, const PipBuffers& bf);
+void launch_pip_scan(hipStream_t s, const PipShape& sh, const PipBuffers& bf);
+void launch_pip_scatter(hipStream_t s, const PipShape& sh, const PipTerm* terms,
+                        const uint8_t* pinf, const uint32_t* scalars, const PipBuffers& bf);
+void launch_pip_accum(hipStream_t s, const PipShape& sh, const PipTerm* terms,
+                      const uint32_t* points, const PipBuffers& bf);
+void launch_pip_bucket(hipStream_t s, const PipShape& sh, const PipBuffers& bf);
+void launch_pip_halve(hipStream_t s, const PipShape& sh, uint32_t step, const PipBuffers& bf);
+void launch_pip_final(hipStream_t s, const PipShape& sh, const uint32_t* job_out,
+                      const PipBuffers& bf, uint32_t* partials);
+
 // test-only primitives
 void launch_test_g1_mul(hipStream_t s, const uint8_t* pts, const uint8_t* scalars, uint32_t n,
                         uint8_t* out, uint8_t* ok);

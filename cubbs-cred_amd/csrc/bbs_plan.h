@@ -40,6 +40,8 @@ struct Plan {
   std::vector<PvChallengeJob> pvch;
   std::vector<PgJob> pg;
   std::vector<uint32_t> disc_index, disc_scalar, und_scalar;
+  std::vector<PipTerm> pip_terms;  // terms of the Pippenger MSMs
+  std::vector<uint32_t> pip_out;   // per Pippenger MSM: its output partial slot
   std::vector<uint8_t> blob;       // signatures, proofs, headers, messages, keys, seeds
   std::vector<uint8_t> map_dst;    // DST of message scalars
   uint32_t n_scalars = 0, n_points = 0, n_partials = 0;
@@ -50,6 +52,12 @@ struct Plan {
   uint32_t scalars(uint32_t k) { uint32_t s = n_scalars; n_scalars += k; return s; }
   uint32_t points(uint32_t k) { uint32_t s = n_points; n_points += k; return s; }
   uint32_t partials(uint32_t k) { uint32_t s = n_partials; n_partials += k; return s; }
+  // A new Pippenger MSM (terms added to pip_terms with its index); its sum
+  // lands in one new partial slot.
+  uint32_t pip_job() {
+    pip_out.push_back(partials(1));
+    return (uint32_t)pip_out.size() - 1;
+  }
   size_t bytes(const void* p, size_t n) {
     size_t off = blob.size();
     blob.resize(off + n);

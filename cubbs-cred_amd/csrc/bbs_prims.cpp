@@ -19,6 +19,7 @@
 // for MSM shapes (1 item, 2 common points, 3 large, 6 dual) and the boolean
 // of a pairing equality.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -35,6 +36,16 @@ namespace cubbs {
 namespace {
 
 constexpr size_t LARGE_TERMS = 1024; // route id 3 from this many terms on
+
+// Item MSMs of at least this many terms run as Pippenger bucket MSMs
+// (CUBBS_HIP_PIPPENGER_MIN, default LARGE_TERMS), smaller ones as two-lane
+// GLV products summed in the partial-sum tree.
+size_t pippenger_min() {
+  const char* v = std::getenv("CUBBS_HIP_PIPPENGER_MIN");
+  if (!v || !*v) return LARGE_TERMS;
+  long x = std::strtol(v, nullptr, 10);
+  return x > 0 ? (size_t)x : LARGE_TERMS;
+}
 constexpr uint8_t ROUTE_ITEM = CUBBS_BBS_MSM_ROUTE_CUBBS_ITEM;
 constexpr uint8_t ROUTE_COMMON = CUBBS_BBS_MSM_ROUTE_CUBBS_COMMON_POINTS;
 constexpr uint8_t ROUTE_LARGE = CUBBS_BBS_MSM_ROUTE_CUBBS_LARGE;
@@ -91,11 +102,8 @@ struct Prog {
   }
 };
 
-// Runs a program: plan -> device -> status / valid / output bytes on the host.
-struct Ran {
-  std::vector<uint8_t> status, valid, out;
-};
-Ran run(Prog& g) {
+// Plans a program onto the device (inputs uploaded, nothing launched).
+std::unique_ptr<Batch> stage(Prog& g) {
   auto b = std::make_unique<Batch>();
   b->kind = KIND_PRIM;
   b->nv = std::max<uint32_t>(g.slots, 1);
@@ -107,17 +115,31 @@ Ran run(Prog& g) {
   }
   if (g.p.term_gen.size() < g.p.n_scalars) g.p.term_gen.resize(g.p.n_scalars);
   materialize(*b, g.p);
-  launch_batch(*b);
+  return b;
+}
+
+// Status / valid / output bytes of a launched program, on the host.
+struct Ran {
+  std::vector<uint8_t> status, valid, out;
+};
+Ran fetch(Batch& b) {
   Ran r;
-  r.status.resize(b->nv);
-  r.valid.resize(b->nv);
-  r.out.resize(std::max<size_t>(b->out_bytes, 1));
-  check(hipMemcpyAsync(r.status.data(), b->status, b->nv, hipMemcpyDeviceToHost, b->stream));
-  check(hipMemcpyAsync(r.valid.data(), b->valid, b->nv, hipMemcpyDeviceToHost, b->stream));
-  if (b->out_bytes)
-    check(hipMemcpyAsync(r.out.data(), b->out, b->out_bytes, hipMemcpyDeviceToHost, b->stream));
-  check(hipStreamSynchronize(b->stream));
+  r.status.resize(b.nv);
+  r.valid.resize(b.nv);
+  r.out.resize(std::max<size_t>(b.out_bytes, 1));
+  check(hipMemcpyAsync(r.status.data(), b.status, b.nv, hipMemcpyDeviceToHost, b.stream));
+  check(hipMemcpyAsync(r.valid.data(), b.valid, b.nv, hipMemcpyDeviceToHost, b.stream));
+  if (b.out_bytes)
+    check(hipMemcpyAsync(r.out.data(), b.out, b.out_bytes, hipMemcpyDeviceToHost, b.stream));
+  check(hipStreamSynchronize(b.stream));
   return r;
+}
+
+// Runs a program: plan -> device -> status / valid / output bytes on the host.
+Ran run(Prog& g) {
+  auto b = stage(g);
+  launch_batch(*b);
+  return fetch(*b);
 }
 
 void set(CuBbsBbsGpuResult& r, int32_t code, uint8_t value, size_t bytes) {
@@ -161,6 +183,99 @@ void common_row(Prog& g, const std::vector<uint32_t>& pts, const std::vector<uin
   }
 }
 
+// Result of one output point: code 1 when a status slot of the row is set,
+// else success with the route id and the 48-byte point copied to `buf`.
+void row_result(const PrimRow& row, const uint8_t* status, const uint8_t* out,
+                CuBbsBbsGpuResult& res, CuBbsBbsGpuByteBuffer* buf) {
+  if (status[row.st] || (row.gst != PRIM_NO_SLOT && status[row.gst])) {
+    set(res, CUBBS_BBS_GPU_ERR_INVALID_INPUT, 0, 0);
+    return;
+  }
+  if (buf) {
+    std::memcpy(buf->data, out + row.off, 48);
+    buf->len = 48;
+  }
+  set(res, CUBBS_BBS_GPU_SUCCESS, row.value, 48);
+}
+
+// Structure of item MSMs: 48-byte points and 32-byte scalars (call level).
+bool msm_items_ok(const CuBbsBbsGpuG1MsmItem* items, size_t n) {
+  for (size_t i = 0; i < n; i++) {
+    const auto& it = items[i];
+    if (it.term_count && (!it.points || !it.scalars)) return false;
+    for (size_t t = 0; t < it.term_count; t++)
+      if (!fixed(it.points[t], 48) || !fixed(it.scalars[t], 32)) return false;
+  }
+  return true;
+}
+
+// One row per item: Pippenger from pippenger_min() terms on, else GLV
+// products summed in the partial-sum tree.
+void compile_msm(Prog& g, const CuBbsBbsGpuG1MsmItem* items, size_t n, std::vector<PrimRow>& rows) {
+  const size_t pip_min = pippenger_min();
+  for (size_t i = 0; i < n; i++) {
+    const auto& it = items[i];
+    const uint32_t st = g.slot();
+    const uint32_t first = g.p.n_partials;
+    if (it.term_count >= pip_min && g.p.pip_terms.size() + it.term_count <= PIP_MAX_TERMS) {
+      const uint32_t job = g.p.pip_job();
+      for (size_t t = 0; t < it.term_count; t++)
+        g.p.pip_terms.push_back(PipTerm{g.point(it.points[t].data, st),
+                                        g.scalar(it.scalars[t].data, st), job});
+    } else {
+      for (size_t t = 0; t < it.term_count; t++)
+        g.product(g.point(it.points[t].data, st), g.scalar(it.scalars[t].data, st));
+    }
+    rows.push_back(PrimRow{st, PRIM_NO_SLOT, g.compress(g.finish(first)),
+                           it.term_count >= LARGE_TERMS ? ROUTE_LARGE : ROUTE_ITEM});
+  }
+}
+
+// Structure of common-point groups (call level; output buffers not checked).
+bool common_items_ok(const CuBbsBbsGpuG1MsmCommonPointsBatchItem* items, size_t n) {
+  for (size_t gi = 0; gi < n; gi++) {
+    const auto& it = items[gi];
+    const bool packed = it.scalar_rows_be != nullptr;
+    if (it.term_count > (SIZE_MAX / 32u) - 1u ||
+        (it.term_count && it.output_count > SIZE_MAX / it.term_count) ||
+        (it.term_count && !it.points) ||
+        (it.output_count && !packed && !it.scalar_rows) ||
+        (packed && it.scalar_row_stride < it.term_count * 32u))
+      return false;
+    for (size_t t = 0; t < it.term_count; t++)
+      if (!fixed(it.points[t], 48)) return false;
+    if (!packed)
+      for (size_t r = 0; r < it.output_count; r++)
+        for (size_t t = 0; t < it.term_count; t++)
+          if (!fixed(it.scalar_rows[r * it.term_count + t], 32)) return false;
+  }
+  return true;
+}
+
+// One row per output of each group, in group order.
+void compile_common(Prog& g, const CuBbsBbsGpuG1MsmCommonPointsBatchItem* items, size_t n,
+                    std::vector<PrimRow>& rows) {
+  for (size_t gi = 0; gi < n; gi++) {
+    const auto& it = items[gi];
+    const uint32_t gst = g.slot();
+    std::vector<uint32_t> gen, pts;
+    if (!as_generators(it.points, it.term_count, gen)) {
+      gen.clear();
+      for (size_t t = 0; t < it.term_count; t++) pts.push_back(g.point(it.points[t].data, gst));
+    }
+    for (size_t r = 0; r < it.output_count; r++) {
+      const uint32_t st = g.slot();
+      const uint32_t first = g.p.n_partials;
+      auto sc = [&](size_t t) {
+        return it.scalar_rows_be ? it.scalar_rows_be + r * it.scalar_row_stride + 32 * t
+                                 : it.scalar_rows[r * it.term_count + t].data;
+      };
+      common_row(g, pts, gen, it.term_count, st, sc);
+      rows.push_back(PrimRow{st, gst, g.compress(g.finish(first)), ROUTE_COMMON});
+    }
+  }
+}
+
 int32_t guard(const std::function<int32_t()>& f) {
   try {
     return f();
@@ -172,6 +287,44 @@ int32_t guard(const std::function<int32_t()>& f) {
 }
 
 }  // namespace
+
+std::unique_ptr<Batch> stage_primitive(int kind, const void* items, size_t n, int32_t& status) {
+  Prog g;
+  std::vector<PrimRow> rows;
+  if (kind == CUBBS_HIP_BATCH_G1_MSM) {
+    auto* it = static_cast<const CuBbsBbsGpuG1MsmItem*>(items);
+    if (!msm_items_ok(it, n)) {
+      status = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      return nullptr;
+    }
+    compile_msm(g, it, n, rows);
+  } else if (kind == CUBBS_HIP_BATCH_G1_MSM_COMMON_POINTS) {
+    auto* it = static_cast<const CuBbsBbsGpuG1MsmCommonPointsBatchItem*>(items);
+    if (!common_items_ok(it, n)) {
+      status = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      return nullptr;
+    }
+    compile_common(g, it, n, rows);
+  } else {
+    status = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+    return nullptr;
+  }
+  auto b = stage(g);
+  b->rows = std::move(rows);
+  b->n_total = b->rows.size();
+  status = CUBBS_BBS_GPU_SUCCESS;
+  return b;
+}
+
+void collect_primitive(Batch& b, CuBbsBbsGpuResult* results) {
+  Ran r = fetch(b);
+  b.h_out.assign(48 * b.rows.size(), 0);
+  for (size_t k = 0; k < b.rows.size(); k++) {
+    CuBbsBbsGpuByteBuffer buf{b.h_out.data() + 48 * k, 48, 0};
+    row_result(b.rows[k], r.status.data(), r.out.data(), results[k], &buf);
+  }
+}
+
 }  // namespace cubbs
 
 using namespace cubbs;
@@ -183,37 +336,17 @@ int32_t cubbs_bbs_gpu_g1_msm_batch_v2(uint32_t abi_version, CuBbsBbsGpuG1MsmItem
                                       size_t item_count, CuBbsBbsGpuResult* results) {
   if (abi_version != CUBBS_BBS_GPU_ABI_VERSION || (item_count && (!items || !results)))
     return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
-  for (size_t i = 0; i < item_count; i++) {
-    const auto& it = items[i];
-    if ((it.term_count && (!it.points || !it.scalars)) || !outcap(it.point_out, 48))
-      return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
-    for (size_t t = 0; t < it.term_count; t++)
-      if (!fixed(it.points[t], 48) || !fixed(it.scalars[t], 32)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
-  }
+  if (!msm_items_ok(items, item_count)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  for (size_t i = 0; i < item_count; i++)
+    if (!outcap(items[i].point_out, 48)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   if (item_count == 0) return CUBBS_BBS_GPU_SUCCESS;
   return guard([&] {
     Prog g;
-    std::vector<uint32_t> st(item_count);
-    std::vector<size_t> off(item_count);
-    for (size_t i = 0; i < item_count; i++) {
-      const auto& it = items[i];
-      st[i] = g.slot();
-      const uint32_t first = g.p.n_partials;
-      for (size_t t = 0; t < it.term_count; t++)
-        g.product(g.point(it.points[t].data, st[i]), g.scalar(it.scalars[t].data, st[i]));
-      off[i] = g.compress(g.finish(first));
-    }
+    std::vector<PrimRow> rows;
+    compile_msm(g, items, item_count, rows);
     Ran r = run(g);
-    for (size_t i = 0; i < item_count; i++) {
-      if (r.status[st[i]]) {
-        set(results[i], CUBBS_BBS_GPU_ERR_INVALID_INPUT, 0, 0);
-        continue;
-      }
-      std::memcpy(items[i].point_out.data, r.out.data() + off[i], 48);
-      items[i].point_out.len = 48;
-      set(results[i], CUBBS_BBS_GPU_SUCCESS,
-          items[i].term_count >= LARGE_TERMS ? ROUTE_LARGE : ROUTE_ITEM, 48);
-    }
+    for (size_t i = 0; i < item_count; i++)
+      row_result(rows[i], r.status.data(), r.out.data(), results[i], &items[i].point_out);
     return CUBBS_BBS_GPU_SUCCESS;
   });
 }
@@ -227,65 +360,22 @@ static int32_t common_points_entry(uint32_t abi_version,
                                    size_t group_count, CuBbsBbsGpuResult* results) {
   if (abi_version != CUBBS_BBS_GPU_ABI_VERSION || (group_count && (!items || !results)))
     return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  if (!common_items_ok(items, group_count)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   for (size_t gi = 0; gi < group_count; gi++) {
-    const auto& it = items[gi];
-    const bool packed = it.scalar_rows_be != nullptr;
-    if (it.term_count > (SIZE_MAX / 32u) - 1u ||
-        (it.term_count && it.output_count > SIZE_MAX / it.term_count) ||
-        (it.term_count && !it.points) ||
-        (it.output_count && ((!packed && !it.scalar_rows) || !it.point_outs)) ||
-        (packed && it.scalar_row_stride < it.term_count * 32u))
-      return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
-    for (size_t t = 0; t < it.term_count; t++)
-      if (!fixed(it.points[t], 48)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
-    for (size_t r = 0; r < it.output_count; r++) {
-      if (!outcap(it.point_outs[r], 48)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
-      if (!packed)
-        for (size_t t = 0; t < it.term_count; t++)
-          if (!fixed(it.scalar_rows[r * it.term_count + t], 32)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
-    }
+    if (items[gi].output_count && !items[gi].point_outs) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+    for (size_t r = 0; r < items[gi].output_count; r++)
+      if (!outcap(items[gi].point_outs[r], 48)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   }
   if (group_count == 0) return CUBBS_BBS_GPU_SUCCESS;
   return guard([&] {
     Prog g;
-    struct Row {
-      uint32_t st, gst;
-      size_t off;
-    };
-    std::vector<Row> rows;
-    for (size_t gi = 0; gi < group_count; gi++) {
-      const auto& it = items[gi];
-      const uint32_t gst = g.slot();
-      std::vector<uint32_t> gen, pts;
-      if (!as_generators(it.points, it.term_count, gen)) {
-        gen.clear();
-        for (size_t t = 0; t < it.term_count; t++) pts.push_back(g.point(it.points[t].data, gst));
-      }
-      for (size_t r = 0; r < it.output_count; r++) {
-        const uint32_t st = g.slot();
-        const uint32_t first = g.p.n_partials;
-        auto sc = [&](size_t t) {
-          return it.scalar_rows_be ? it.scalar_rows_be + r * it.scalar_row_stride + 32 * t
-                                   : it.scalar_rows[r * it.term_count + t].data;
-        };
-        common_row(g, pts, gen, it.term_count, st, sc);
-        rows.push_back(Row{st, gst, g.compress(g.finish(first))});
-      }
-    }
+    std::vector<PrimRow> rows;
+    compile_common(g, items, group_count, rows);
     Ran ran = run(g);
     size_t k = 0;
-    for (size_t gi = 0; gi < group_count; gi++) {
-      for (size_t r = 0; r < items[gi].output_count; r++, k++) {
-        const Row& row = rows[k];
-        if (ran.status[row.st] || ran.status[row.gst]) {
-          set(results[k], CUBBS_BBS_GPU_ERR_INVALID_INPUT, 0, 0);
-          continue;
-        }
-        std::memcpy(items[gi].point_outs[r].data, ran.out.data() + row.off, 48);
-        items[gi].point_outs[r].len = 48;
-        set(results[k], CUBBS_BBS_GPU_SUCCESS, ROUTE_COMMON, 48);
-      }
-    }
+    for (size_t gi = 0; gi < group_count; gi++)
+      for (size_t r = 0; r < items[gi].output_count; r++, k++)
+        row_result(rows[k], ran.status.data(), ran.out.data(), results[k], &items[gi].point_outs[r]);
     return CUBBS_BBS_GPU_SUCCESS;
   });
 }

@@ -141,7 +141,7 @@ DEV int ysel1(int c, bool xi) { return c ? (xi ? ED : EY0) : (xi ? ENS : EN1); }
 // Lane c = 0 writes y0, 2 y0, y0 + 2p - y1, y0 + y1; lane c = 1 writes y1,
 // 2 y1, 2p - y1, 4p - y0 - y1.  Values below 2p in, derived values below 4p.
 DEV void put_rich(lds_t* R, const Fp& v, const Lane& L) {
-  Fp o, y0, y1, z, t, a, b, s2, t2;
+  Fp o, y0, y1, z, a, b, s2, t2;
   partner(o, v);
   sel(y0, L.c, o, v);
   sel(y1, L.c, v, o);

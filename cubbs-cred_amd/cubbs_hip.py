@@ -16,6 +16,7 @@ LIB_PATH = os.environ.get("CUBBS_BBS_GPU_LIBRARY") or os.path.join(HERE, "libcub
 ABI_VERSION = 2
 SUCCESS, ERR_INVALID_INPUT, ERR_INSUFFICIENT_OUTPUT, ERR_INTERNAL = 0, 1, 2, 3
 BATCH_VERIFY, BATCH_PROOF_VERIFY = 0, 1
+BATCH_G1_MSM, BATCH_G1_MSM_COMMON_POINTS = 16, 17
 
 API_ID = b"BBS_BLS12381G1_XMD:SHA-256_SSWU_RO_H2G_HM2S_"
 MAP_DST = API_ID + b"MAP_MSG_TO_SCALAR_AS_HASH_"
@@ -53,6 +54,7 @@ EXT_SYMBOLS = [
     "cubbs_hip_batch_sync",
     "cubbs_hip_batch_results",
     "cubbs_hip_batch_free",
+    "cubbs_hip_batch_outputs",
     "cubbs_hip_batch_profile",
     "cubbs_hip_batch_shape",
     "cubbs_hip_test_g1_mul",
@@ -220,6 +222,8 @@ def lib():
         _lib.cubbs_hip_batch_results.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _lib.cubbs_hip_batch_free.argtypes = [ctypes.c_void_p]
         _lib.cubbs_hip_batch_free.restype = None
+        _lib.cubbs_hip_batch_outputs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                                 ctypes.POINTER(ctypes.c_size_t)]
         _lib.cubbs_hip_batch_profile.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p,
                                                  ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int,
                                                  ctypes.c_void_p]
@@ -505,10 +509,16 @@ class StagedBatch:
             items = build_verify_items(reqs, self.keep)
         elif kind == BATCH_PROOF_VERIFY:
             items = build_proof_verify_items(reqs, self.keep)
+        elif kind == BATCH_G1_MSM:
+            items, _ = _msm_items(reqs, self.keep)
+        elif kind == BATCH_G1_MSM_COMMON_POINTS:
+            items, _ = _common_items(reqs, self.keep, packed=True, outputs=False)
         else:
             raise ValueError("unsupported batch kind")
         st = ctypes.c_int32(0)
-        self.n = len(reqs)
+        self.kind = kind
+        # results: one per item, or one per output row of the common-point groups
+        self.n = sum(len(rows) for _, rows in reqs) if kind == BATCH_G1_MSM_COMMON_POINTS else len(reqs)
         self.h = lib().cubbs_hip_batch_prepare(ABI_VERSION, kind, ctypes.cast(items, ctypes.c_void_p),
                                                ctypes.c_size_t(len(reqs)), ctypes.byref(st))
         self.keep = None  # items are only read during prepare
@@ -530,7 +540,18 @@ class StagedBatch:
         rc = lib().cubbs_hip_batch_results(self.h, res)
         if rc:
             raise RuntimeError("results failed %d" % rc)
+        if self.kind in (BATCH_G1_MSM, BATCH_G1_MSM_COMMON_POINTS):
+            return [(res[i].code, res[i].value) for i in range(self.n)]
         return [(res[i].code, bool(res[i].value)) for i in range(self.n)]
+
+    def outputs(self):
+        """MSM kinds, after results(): the 48-byte output points in result order."""
+        buf = ctypes.create_string_buffer(48 * max(1, self.n))
+        ln = ctypes.c_size_t(0)
+        rc = lib().cubbs_hip_batch_outputs(self.h, buf, ctypes.c_size_t(48 * self.n), ctypes.byref(ln))
+        if rc:
+            raise RuntimeError("outputs failed %d" % rc)
+        return [buf.raw[48 * i:48 * i + 48] for i in range(ln.value // 48)]
 
     def profile(self, iters):
         names = ctypes.create_string_buffer(4096)
@@ -714,9 +735,8 @@ def _r(res, i):
     return (res[i].code, res[i].value, res[i].bytes_written)
 
 
-def g1_msm(items):
-    """items: [(points48, scalars32)] -> (status, [(code, route, point or None)])."""
-    keep = _Keep()
+def _msm_items(items, keep):
+    """G1MsmItem array for [(points48, scalars32)] and the output buffers."""
     arr = (G1MsmItem * max(1, len(items)))()
     bufs = []
     for i, (pts, scs) in enumerate(items):
@@ -726,15 +746,12 @@ def g1_msm(items):
         buf, bb = keep.out(48)
         arr[i].point_out = bb
         bufs.append(buf)
-    res = _res(len(items))
-    rc = lib().cubbs_bbs_gpu_g1_msm_batch_v2(ABI_VERSION, arr, ctypes.c_size_t(len(items)), res)
-    return rc, [(res[i].code, res[i].value, bufs[i].raw[:48] if res[i].code == 0 else None)
-                for i in range(len(items))]
+    keep.refs.append(arr)
+    return arr, bufs
 
 
-def g1_msm_common_points(groups, fused=False, packed=False):
-    """groups: [(points48, [row scalars])] -> (status, [(code, route, point or None)] over all rows)."""
-    keep = _Keep()
+def _common_items(groups, keep, packed, outputs=True):
+    """CommonPointsItem array for [(points48, [row scalars])] and the row output buffers."""
     arr = (CommonPointsItem * max(1, len(groups)))()
     allbufs = []
     for g, (pts, rows) in enumerate(groups):
@@ -749,9 +766,28 @@ def g1_msm_common_points(groups, fused=False, packed=False):
         else:
             arr[g].scalar_rows = keep.slices([s for r in rows for s in r])
         arr[g].output_count = len(rows)
-        outs, bufs = _outs(keep, len(rows))
-        arr[g].point_outs = ctypes.cast(outs, ctypes.c_void_p)
-        allbufs += bufs
+        if outputs:
+            outs, bufs = _outs(keep, len(rows))
+            arr[g].point_outs = ctypes.cast(outs, ctypes.c_void_p)
+            allbufs += bufs
+    keep.refs.append(arr)
+    return arr, allbufs
+
+
+def g1_msm(items):
+    """items: [(points48, scalars32)] -> (status, [(code, route, point or None)])."""
+    keep = _Keep()
+    arr, bufs = _msm_items(items, keep)
+    res = _res(len(items))
+    rc = lib().cubbs_bbs_gpu_g1_msm_batch_v2(ABI_VERSION, arr, ctypes.c_size_t(len(items)), res)
+    return rc, [(res[i].code, res[i].value, bufs[i].raw[:48] if res[i].code == 0 else None)
+                for i in range(len(items))]
+
+
+def g1_msm_common_points(groups, fused=False, packed=False):
+    """groups: [(points48, [row scalars])] -> (status, [(code, route, point or None)] over all rows)."""
+    keep = _Keep()
+    arr, allbufs = _common_items(groups, keep, packed)
     res = _res(len(allbufs))
     fn = (lib().cubbs_bbs_gpu_g1_msm_fused_common_points_batch_v2 if fused
           else lib().cubbs_bbs_gpu_g1_msm_common_points_batch_v2)

@@ -38,15 +38,26 @@ int32_t cubbs_hip_proof_gen_with_scalars_batch(uint32_t abi_version,
 
 /* ---- staged batches: prepare (host packing + upload), launch (enqueue the
  * device pipeline on the batch's own stream), sync, read results, free.
- * kind: 0 = Verify, 1 = ProofVerify.  Items are read only during prepare. */
+ * kind: 0 = Verify, 1 = ProofVerify, 16 = item MSMs (CuBbsBbsGpuG1MsmItem, as
+ * cubbs_bbs_gpu_g1_msm_batch_v2), 17 = common-point MSM groups
+ * (CuBbsBbsGpuG1MsmCommonPointsBatchItem, as
+ * cubbs_bbs_gpu_g1_msm_common_points_batch_v2; point_outs are not used).
+ * Items are read only during prepare.  For the MSM kinds, results hold one
+ * entry per output point (per item, or per row of each group in order) and
+ * cubbs_hip_batch_outputs returns the 48-byte encodings in the same order. */
 #define CUBBS_HIP_BATCH_VERIFY 0
 #define CUBBS_HIP_BATCH_PROOF_VERIFY 1
+#define CUBBS_HIP_BATCH_G1_MSM 16
+#define CUBBS_HIP_BATCH_G1_MSM_COMMON_POINTS 17
 void *cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void *items, size_t item_count,
                               int32_t *status_out);
 int32_t cubbs_hip_batch_launch(void *batch);
 int32_t cubbs_hip_batch_sync(void *batch);
 int32_t cubbs_hip_batch_results(void *batch, CuBbsBbsGpuResult *results);
 void cubbs_hip_batch_free(void *batch);
+/* MSM kinds: after cubbs_hip_batch_results, copies 48 bytes per output point
+ * (zeros for rows with an error) into out; *out_len = bytes written. */
+int32_t cubbs_hip_batch_outputs(void *batch, uint8_t *out, size_t out_cap, size_t *out_len);
 /* Kernel timing with HIP events recorded on the batch's stream around every
  * launch of `iters` back-to-back pipeline runs.  names_out receives a
  * newline-separated list of kernel names (capacity names_cap bytes); ms_out
