@@ -320,7 +320,7 @@ __global__ __launch_bounds__(128) void k_comb_msm(const CombChunk* __restrict__ 
         }
         load_g1a(e, ew);
         if (d < 0) fp_neg(e.y, e.y);
-        pt_madd(acc, acc, e);
+        g1_madd_in(acc, e);
       }
     }
   }

@@ -223,7 +223,7 @@ __global__ __launch_bounds__(64) void k_pip_accum(PipShape sh, const PipTerm* __
     load_g1a(q, points + (size_t)terms[v >> 2].point * G1A_WORDS);
     if (v & 2) fp_mul(q.x, q.x, beta);
     if (v & 1) fp_neg(q.y, q.y);
-    pt_madd(acc, acc, q);
+    g1_madd_in(acc, q);
   }
   store_g1j(bf.pieces + (size_t)p * G1J_WORDS, acc);
 }
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(64) void k_pip_bucket(PipShape sh, PipBuffers bf) {
   for (uint32_t p = bf.poff[i]; p < end; p++) {
     G1J x;
     load_g1j(x, bf.pieces + (size_t)p * G1J_WORDS);
-    pt_add(acc, acc, x);
+    g1_add_in(acc, x);
   }
   store_g1j(bf.buckets + (size_t)i * G1J_WORDS, acc);
 }
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(64) void k_pip_halve(PipShape sh, uint32_t step, ui
   G1J a, b;
   load_g1j(a, base + (size_t)i * G1J_WORDS);
   load_g1j(b, base + (size_t)(i + M) * G1J_WORDS);
-  pt_add(a, a, b);
+  g1_add_in(a, b);
   store_g1j(base + (size_t)i * G1J_WORDS, a);
 }
 void launch_pip_halve(hipStream_t s, const PipShape& sh, uint32_t step, const PipBuffers& bf) {
@@ -292,12 +292,12 @@ __global__ __launch_bounds__(64) void k_pip_final(PipShape sh, const uint32_t* _
     G1J acc, x;
     pt_set_inf(acc);
     for (int j = (int)sh.K - 1; j >= 0; j--) {
-      pt_dbl(acc, acc);
+      g1_dbl_in(acc);
       load_g1j(x, base + (size_t)(1u << j) * G1J_WORDS);
-      pt_add(acc, acc, x);
+      g1_add_in(acc, x);
     }
     load_g1j(x, base);
-    pt_add(acc, acc, x);
+    g1_add_in(acc, x);
     store_g1j(win + w * G1J_WORDS, acc);
   }
   __syncthreads();
@@ -305,9 +305,9 @@ __global__ __launch_bounds__(64) void k_pip_final(PipShape sh, const uint32_t* _
     G1J r, x;
     load_g1j(r, win + (sh.W - 1) * G1J_WORDS);
     for (int v = (int)sh.W - 2; v >= 0; v--) {
-      for (uint32_t d = 0; d < sh.c; d++) pt_dbl(r, r);
+      for (uint32_t d = 0; d < sh.c; d++) g1_dbl_in(r);
       load_g1j(x, win + v * G1J_WORDS);
-      pt_add(r, r, x);
+      g1_add_in(r, x);
     }
     store_g1j(partials + (size_t)job_out[job] * G1J_WORDS, r);
   }

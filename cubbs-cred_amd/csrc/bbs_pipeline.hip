@@ -154,7 +154,7 @@ __global__ __launch_bounds__(64) void k_reduce(const ReduceGroup* __restrict__ g
   for (uint32_t k = 0; k < g.count; k++) {
     G1J p;
     load_g1j(p, partials + (size_t)(g.begin + k) * G1J_WORDS);
-    pt_add(acc, acc, p);
+    g1_add_in(acc, p);
   }
   const uint32_t o = out_point[i];
   G1A a;
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(64) void k_reduce_partials(const ReduceGroup* __res
   for (uint32_t k = 0; k < g.count; k++) {
     G1J p;
     load_g1j(p, partials + (size_t)(g.begin + k) * G1J_WORDS);
-    pt_add(acc, acc, p);
+    g1_add_in(acc, p);
   }
   store_g1j(partials + (size_t)out[i] * G1J_WORDS, acc);
 }

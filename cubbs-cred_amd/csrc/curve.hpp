@@ -262,6 +262,129 @@ DEVN void pt_mul_limbs_affine(PtJ<F>& r, const PtA<F>& q, const uint32_t* k) {
   r = acc;
 }
 
+// ------------------------------------------------------------------ inlined G1 group law
+// The same formulas as pt_dbl / pt_madd / pt_add (dbl-2009-l, madd-2007-bl,
+// add-2007-bl) for G1, expanded into the calling loop instead of called (no
+// call frame in scratch memory), with the operations ordered so that at most
+// six field temporaries are live.  The rare exceptional cases (doubling
+// inside an addition) call the out-of-line forms.
+
+// p <- 2 p
+DEV void g1_dbl_in(G1J& p) {
+  Fp A, B, C, D;
+  fp_sqr(A, p.x);          // A = X^2
+  fp_sqr(B, p.y);          // B = Y^2
+  fp_mul(p.z, p.y, p.z);   // Z3 = 2 Y Z (the identity keeps Z = 0)
+  fp_dbl(p.z, p.z);
+  fp_sqr(C, B);            // C = B^2
+  fp_add(D, p.x, B);       // D = 2 ((X + B)^2 - A - C)
+  fp_sqr(D, D);
+  fp_sub(D, D, A);
+  fp_sub(D, D, C);
+  fp_dbl(D, D);
+  fp_dbl(B, A);            // E = 3 A (in A)
+  fp_add(A, B, A);
+  fp_sqr(B, A);            // X3 = E^2 - 2 D
+  fp_sub(B, B, D);
+  fp_sub(p.x, B, D);
+  fp_sub(D, D, p.x);       // Y3 = E (D - X3) - 8 C
+  fp_mul(p.y, A, D);
+  fp_dbl(C, C);
+  fp_dbl(C, C);
+  fp_dbl(C, C);
+  fp_sub(p.y, p.y, C);
+}
+
+// p <- p + q, q affine and not the identity
+DEV void g1_madd_in(G1J& p, const G1A& q) {
+  if (fp_is_zero(p.z)) {
+    pt_from_affine(p, q);
+    return;
+  }
+  Fp t0, t1, t2, t3, t4;
+  fp_sqr(t0, p.z);          // Z1Z1
+  fp_mul(t1, q.x, t0);      // U2
+  fp_mul(t2, q.y, p.z);
+  fp_mul(t2, t2, t0);       // S2
+  fp_sub(t1, t1, p.x);      // H = U2 - X1
+  fp_sub(t2, t2, p.y);      // S2 - Y1
+  if (fp_is_zero(t1)) {
+    if (fp_is_zero(t2)) {
+      G1J qj;
+      pt_from_affine(qj, q);
+      pt_dbl(p, qj);
+    } else {
+      pt_set_inf(p);
+    }
+    return;
+  }
+  fp_dbl(t2, t2);           // r = 2 (S2 - Y1)
+  fp_sqr(t3, t1);           // HH
+  fp_add(t4, p.z, t1);      // Z3 = (Z1 + H)^2 - Z1Z1 - HH
+  fp_sqr(t4, t4);
+  fp_sub(t4, t4, t0);
+  fp_sub(p.z, t4, t3);
+  fp_dbl(t3, t3);           // I = 4 HH
+  fp_dbl(t3, t3);
+  fp_mul(t0, t1, t3);       // J = H I
+  fp_mul(t1, p.x, t3);      // V = X1 I
+  fp_mul(t4, p.y, t0);      // Y1 J
+  fp_sqr(t3, t2);           // X3 = r^2 - J - 2 V
+  fp_sub(t3, t3, t0);
+  fp_sub(t3, t3, t1);
+  fp_sub(p.x, t3, t1);
+  fp_sub(t1, t1, p.x);      // Y3 = r (V - X3) - 2 Y1 J
+  fp_mul(p.y, t2, t1);
+  fp_dbl(t4, t4);
+  fp_sub(p.y, p.y, t4);
+}
+
+// p <- p + q (Jacobian; q must not alias p)
+DEV void g1_add_in(G1J& p, const G1J& q) {
+  if (fp_is_zero(q.z)) return;
+  if (fp_is_zero(p.z)) {
+    p = q;
+    return;
+  }
+  Fp t0, t1, t2, t3, t4, t5;
+  fp_sqr(t0, p.z);          // Z1Z1
+  fp_sqr(t1, q.z);          // Z2Z2
+  fp_mul(t2, p.x, t1);      // U1
+  fp_mul(t3, q.x, t0);      // U2
+  fp_mul(t4, p.y, q.z);
+  fp_mul(t4, t4, t1);       // S1
+  fp_mul(t5, q.y, p.z);
+  fp_mul(t5, t5, t0);       // S2
+  fp_sub(t3, t3, t2);       // H = U2 - U1
+  fp_sub(t5, t5, t4);       // S2 - S1
+  if (fp_is_zero(t3)) {
+    if (fp_is_zero(t5))
+      pt_dbl(p, p);
+    else
+      pt_set_inf(p);
+    return;
+  }
+  fp_add(p.z, p.z, q.z);    // Z3 = ((Z1 + Z2)^2 - Z1Z1 - Z2Z2) H
+  fp_sqr(p.z, p.z);
+  fp_sub(p.z, p.z, t0);
+  fp_sub(p.z, p.z, t1);
+  fp_mul(p.z, p.z, t3);
+  fp_dbl(t5, t5);           // r = 2 (S2 - S1)
+  fp_dbl(t0, t3);           // I = (2 H)^2
+  fp_sqr(t0, t0);
+  fp_mul(t1, t3, t0);       // J = H I
+  fp_mul(t3, t2, t0);       // V = U1 I
+  fp_sqr(t0, t5);           // X3 = r^2 - J - 2 V
+  fp_sub(t0, t0, t1);
+  fp_sub(t0, t0, t3);
+  fp_sub(p.x, t0, t3);
+  fp_sub(t3, t3, p.x);      // Y3 = r (V - X3) - 2 S1 J
+  fp_mul(t3, t5, t3);
+  fp_mul(t4, t4, t1);
+  fp_dbl(t4, t4);
+  fp_sub(p.y, t3, t4);
+}
+
 // ------------------------------------------------------------------ G1 specifics
 DEV void g1_load_affine_const(G1A& r, const uint32_t* c) {
   fp_load_const(r.x, c);
@@ -494,7 +617,7 @@ DEVN void glv_split(uint32_t k1[4], uint32_t k2[4], const uint32_t k[8]) {
 // One half of the GLV product: half 0 gives [k1] P, half 1 gives [k2] phi(P),
 // with k = k1 + k2 lambda (P affine, not the identity; 128-bit binary ladder
 // with mixed additions).
-DEVN void g1_mul_glv_half(G1J& r, const G1A& p, const uint32_t k[8], uint32_t half) {
+DEV void g1_mul_glv_half(G1J& r, const G1A& p, const uint32_t k[8], uint32_t half) {
   uint32_t k1[4], k2[4];
   glv_split(k1, k2, k);
   G1A q;
@@ -506,7 +629,21 @@ DEVN void g1_mul_glv_half(G1J& r, const G1A& p, const uint32_t k[8], uint32_t ha
   uint32_t e[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) e[i] = half ? k2[i] : k1[i];
-  pt_mul_limbs_affine<Fp, 4>(r, q, e);
+  // left-to-right from the top set bit, group law expanded inline
+  int top = 127;
+  while (top >= 0 && !((e[top >> 5] >> (top & 31)) & 1)) top--;
+  if (top < 0) {
+    pt_set_inf(r);
+    return;
+  }
+  G1J acc;
+  pt_from_affine(acc, q);
+#pragma unroll 1
+  for (int i = top - 1; i >= 0; i--) {
+    g1_dbl_in(acc);
+    if ((e[i >> 5] >> (i & 31)) & 1) g1_madd_in(acc, q);
+  }
+  r = acc;
 }
 
 // [k] P on G1 (P affine, not the identity) with the GLV split
