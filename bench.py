@@ -122,7 +122,7 @@ G1_GENERATOR = bytes.fromhex(
     "97f1d3a73197d7942695638c4fa9ac0fc3688c4f9774b905a14e3a3f171bac586c55e83ff97a1aeffb3af00adb22c6bb")
 
 
-def msm_run(g, steps=10, warmup=2):
+def msm_run(g, steps=10, warmup=2, graph=True):
     """Batched G1 MSM kernels alone: 4096 fixed-base MSMs of 32 terms over the
     ciphersuite generators [P1, Q1, H_1..H_30] (comb route, one common-points
     group) and one variable-base MSM of 2^16 points k_i G1 (Pippenger route),
@@ -140,6 +140,9 @@ def msm_run(g, steps=10, warmup=2):
     cb = g.StagedBatch(g.BATCH_G1_MSM_COMMON_POINTS, [(gens, rows)])
     pb = g.StagedBatch(g.BATCH_G1_MSM, [(pts, scs)])
     del rows, pts, scs
+    if graph:
+        cb.use_graph(True)
+        pb.use_graph(True)
     for _ in range(warmup):
         cb.launch()
         pb.launch()
@@ -328,6 +331,7 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=15.0)
     ap.add_argument("--profile-only", action="store_true", help="a few launches, no baselines")
     ap.add_argument("--no-msm", action="store_true", help="skip the batched-MSM measurement")
+    ap.add_argument("--no-graph", action="store_true", help="launch kernel by kernel instead of HIP graphs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -354,6 +358,10 @@ def main():
     log("workload built in %.1fs" % (time.time() - t0))
     vb = g.StagedBatch(g.BATCH_VERIFY, vreqs)
     pb = g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs)
+    if not args.no_graph:
+        # each batch's whole launch sequence replayed from one captured HIP graph
+        vb.use_graph(True)
+        pb.use_graph(True)
     log("staged")
     for _ in range(args.warmup):
         vb.launch()
@@ -400,6 +408,7 @@ def main():
                                   "32-256 B messages, 64 B header, ~1% tampered) + 2048 ProofVerify "
                                   "(L=32 x 64 B, 25% disclosed, 64 B ph)",
                       "verify_items": N_VERIFY, "proof_verify_items": N_PROOFS,
+                      "launch": "kernel by kernel" if args.no_graph else "HIP graph replay per staged batch",
                       "msm_terms": vb.shape()["msm_terms"] + pb.shape()["msm_terms"]},
            "roofline": roofline,
            "abi_calls": abi_ops,
@@ -407,7 +416,8 @@ def main():
                          "proof_verify": {k: round(v, 3) for k, v in pk.items()}}}
     if not args.no_msm:
         t = time.time()
-        out["msm"] = msm_run(g, steps=3 if args.profile_only else 10, warmup=1 if args.profile_only else 2)
+        out["msm"] = msm_run(g, steps=3 if args.profile_only else 10, warmup=1 if args.profile_only else 2,
+                             graph=not args.no_graph)
         log("msm run %.1fs: %s" % (time.time() - t, {k: out["msm"][k] for k in ("msm_per_s", "ms_per_step", "comb_4096x32_ms", "pippenger_2_16_ms")}))
     if not args.profile_only:
         out["per_request_gpu_ops_s"] = round(per_request_gpu(g, vreqs), 2)

@@ -365,7 +365,11 @@ void* cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void* items,
 int32_t cubbs_hip_batch_launch(void* batch) {
   if (!batch) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   try {
-    launch_batch(*static_cast<Batch*>(batch));
+    Batch* b = static_cast<Batch*>(batch);
+    if (b->graph)
+      check(hipGraphLaunch(b->graph, b->stream));
+    else
+      launch_batch(*b);
     return CUBBS_BBS_GPU_SUCCESS;
   } catch (const GpuError&) {
     return CUBBS_BBS_GPU_ERR_INTERNAL;
@@ -395,6 +399,24 @@ int32_t cubbs_hip_batch_results(void* batch, CuBbsBbsGpuResult* results) {
 }
 
 void cubbs_hip_batch_free(void* batch) { delete static_cast<Batch*>(batch); }
+
+int32_t cubbs_hip_batch_use_graph(void* batch, int32_t on) {
+  if (!batch) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  Batch* b = static_cast<Batch*>(batch);
+  try {
+    if (on) {
+      check(hipStreamSynchronize(b->stream));
+      capture_batch(*b);
+    } else if (b->graph) {
+      check(hipStreamSynchronize(b->stream));
+      check(hipGraphExecDestroy(b->graph));
+      b->graph = nullptr;
+    }
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (const GpuError&) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+}
 
 int32_t cubbs_hip_batch_outputs(void* batch, uint8_t* out, size_t out_cap, size_t* out_len) {
   if (!batch || (out_cap && !out)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;

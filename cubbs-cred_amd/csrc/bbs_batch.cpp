@@ -908,8 +908,45 @@ void collect_batch(Batch& b, CuBbsBbsGpuResult* results, uint8_t* const* out_ptr
   }
 }
 
+void capture_batch(Batch& b) {
+  if (b.graph) return;
+  hipStream_t cs = nullptr, ca = nullptr;
+  check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipError_t e = hipStreamCreateWithFlags(&ca, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    check(e);
+  }
+  const hipStream_t s0 = b.stream, a0 = b.aux;
+  const bool timing = b.timer.on;
+  b.stream = cs;
+  b.aux = a0 ? ca : nullptr;
+  b.timer.on = false;
+  hipGraph_t g = nullptr;
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    try {
+      launch_batch(b);
+    } catch (const GpuError&) {
+      e = hipErrorUnknown;
+    }
+    hipError_t e2 = hipStreamEndCapture(cs, &g);
+    if (e == hipSuccess) e = e2;
+  }
+  b.stream = s0;
+  b.aux = a0;
+  b.timer.on = timing;
+  if (e == hipSuccess) e = hipGraphInstantiate(&b.graph, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(cs);
+  (void)hipStreamDestroy(ca);
+  if (e != hipSuccess) b.graph = nullptr;
+  check(e);
+}
+
 Batch::~Batch() {
   if (stream) (void)hipStreamSynchronize(stream);
+  if (graph) (void)hipGraphExecDestroy(graph);
   for (size_t i = 0; i < timer.starts.size(); i++) {
     (void)hipEventDestroy(timer.starts[i]);
     (void)hipEventDestroy(timer.stops[i]);

@@ -147,6 +147,8 @@ struct Batch {
   std::vector<PrimRow> rows;
   std::vector<uint8_t> h_out;
   KernelTimer timer;
+  // the whole launch_batch() sequence captured once and replayed
+  hipGraphExec_t graph = nullptr;
   ~Batch();
 };
 
@@ -167,6 +169,9 @@ std::unique_ptr<Batch> stage_primitive(int kind, const void* items, size_t n, in
 // One result per output row; encodings kept in b.h_out (48 bytes per row).
 void collect_primitive(Batch& b, CuBbsBbsGpuResult* results);
 void launch_batch(Batch& b);
+// Capture launch_batch() into b.graph (on private streams, so concurrent users
+// of the pool streams are never recorded); b.stream then replays it.
+void capture_batch(Batch& b);
 void collect_batch(Batch& b, CuBbsBbsGpuResult* results, uint8_t* const* out_ptrs,
                    size_t* out_lens);
 

@@ -55,6 +55,7 @@ EXT_SYMBOLS = [
     "cubbs_hip_batch_results",
     "cubbs_hip_batch_free",
     "cubbs_hip_batch_outputs",
+    "cubbs_hip_batch_use_graph",
     "cubbs_hip_batch_profile",
     "cubbs_hip_batch_shape",
     "cubbs_hip_test_g1_mul",
@@ -222,6 +223,7 @@ def lib():
         _lib.cubbs_hip_batch_results.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _lib.cubbs_hip_batch_free.argtypes = [ctypes.c_void_p]
         _lib.cubbs_hip_batch_free.restype = None
+        _lib.cubbs_hip_batch_use_graph.argtypes = [ctypes.c_void_p, ctypes.c_int32]
         _lib.cubbs_hip_batch_outputs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                                  ctypes.POINTER(ctypes.c_size_t)]
         _lib.cubbs_hip_batch_profile.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p,
@@ -543,6 +545,12 @@ class StagedBatch:
         if self.kind in (BATCH_G1_MSM, BATCH_G1_MSM_COMMON_POINTS):
             return [(res[i].code, res[i].value) for i in range(self.n)]
         return [(res[i].code, bool(res[i].value)) for i in range(self.n)]
+
+    def use_graph(self, on=True):
+        """Capture the launch sequence into a HIP graph (on) or drop it (off)."""
+        rc = lib().cubbs_hip_batch_use_graph(self.h, 1 if on else 0)
+        if rc:
+            raise RuntimeError("graph capture failed %d" % rc)
 
     def outputs(self):
         """MSM kinds, after results(): the 48-byte output points in result order."""

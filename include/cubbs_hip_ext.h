@@ -55,6 +55,10 @@ int32_t cubbs_hip_batch_launch(void *batch);
 int32_t cubbs_hip_batch_sync(void *batch);
 int32_t cubbs_hip_batch_results(void *batch, CuBbsBbsGpuResult *results);
 void cubbs_hip_batch_free(void *batch);
+/* on != 0: capture the batch's whole launch sequence (both streams, every
+ * kernel; no memset nodes) into a HIP graph once; later launches replay it.
+ * on == 0: drop the graph and launch kernel by kernel again. */
+int32_t cubbs_hip_batch_use_graph(void *batch, int32_t on);
 /* MSM kinds: after cubbs_hip_batch_results, copies 48 bytes per output point
  * (zeros for rows with an error) into out; *out_len = bytes written. */
 int32_t cubbs_hip_batch_outputs(void *batch, uint8_t *out, size_t out_cap, size_t *out_len);
