@@ -332,6 +332,7 @@ def main():
     ap.add_argument("--profile-only", action="store_true", help="a few launches, no baselines")
     ap.add_argument("--no-msm", action="store_true", help="skip the batched-MSM measurement")
     ap.add_argument("--no-graph", action="store_true", help="launch kernel by kernel instead of HIP graphs")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the measured launch-shape tuning")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -353,6 +354,14 @@ def main():
     rc, kind, hw = g.backend_info()
     if rc != 0 or not hw:
         raise SystemExit("no usable HIP device (backend_info rc=%d)" % rc)
+    tuning = None
+    if not args.no_autotune:
+        t0 = time.time()
+        rc, rep = g.autotune(16)
+        if rc != 0:
+            raise SystemExit("autotune failed rc=%d" % rc)
+        tuning = rep.strip().split("\n")
+        log("autotuned in %.1fs: %s" % (time.time() - t0, tuning))
     t0 = time.time()
     vreqs, expect_v, pvreqs, kps, abi_ops = build_workload(g)
     log("workload built in %.1fs" % (time.time() - t0))
@@ -411,6 +420,7 @@ def main():
                       "launch": "kernel by kernel" if args.no_graph else "HIP graph replay per staged batch",
                       "msm_terms": vb.shape()["msm_terms"] + pb.shape()["msm_terms"]},
            "roofline": roofline,
+           "tuning": tuning,
            "abi_calls": abi_ops,
            "kernel_ms": {"verify": {k: round(v, 3) for k, v in vk.items()},
                          "proof_verify": {k: round(v, 3) for k, v in pk.items()}}}

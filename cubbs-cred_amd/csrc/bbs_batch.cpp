@@ -33,7 +33,18 @@ static bool slice_ok(const CuBbsBbsGpuByteSlice& s) { return s.len == 0 || s.dat
 
 static const char* kApiId = "BBS_BLS12381G1_XMD:SHA-256_SSWU_RO_H2G_HM2S_";
 
-uint32_t comb_chunk_terms() { return std::max<uint32_t>(1, env_u32("CUBBS_HIP_COMB_CHUNK", 4)); }
+// Terms per comb lane: CUBBS_HIP_COMB_CHUNK, else (primitive MSM programs
+// only, `tuned`) the tuner's choice for the batch's size class, else 4.  The
+// whole-operation pipelines keep 4: their comb runs beside decode and the GLV
+// products and feeds the reduction, which the comb-only tuning does not see.
+uint32_t comb_chunk_terms(uint64_t batch_terms, bool tuned) {
+  const uint32_t env = env_u32("CUBBS_HIP_COMB_CHUNK", 0);
+  if (env) return env;
+  if (!tuned) return 4;
+  if (tune_force().comb_chunk) return tune_force().comb_chunk;
+  const uint32_t t = tuned_comb_chunk(batch_terms);
+  return t ? t : 4;
+}
 
 // ------------------------------------------------------------------ arenas
 static thread_local Arena* tl_arena = nullptr;
@@ -275,6 +286,7 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
   b->dev_index.assign(n, 0);
   std::vector<size_t> dev;
   uint32_t maxL = 0;
+  uint64_t est_terms = 0;  // fixed-base terms of the batch (tuning class)
   for (size_t i = 0; i < n; i++) {
     const auto& it = items[i];
     bool ok = it.public_key.len == 96 && it.public_key.data && it.signature.len == 80 &&
@@ -288,6 +300,7 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
     b->dev_index[i] = (uint32_t)dev.size();
     dev.push_back(i);
     maxL = std::max<uint32_t>(maxL, (uint32_t)it.message_count);
+    est_terms += it.message_count + 2;
   }
   b->nv = (uint32_t)dev.size();
   if (!b->nv) return b;
@@ -298,7 +311,7 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
   Plan p;
   std::string d = map_dst();
   p.map_dst.assign(d.begin(), d.end());
-  const uint32_t CH = comb_chunk_terms();
+  const uint32_t CH = comb_chunk_terms(est_terms);
   // terms first: [0, T)
   uint32_t T = 0;
   for (uint32_t k = 0; k < b->nv; k++) T += (uint32_t)items[dev[k]].message_count + 2;
@@ -350,6 +363,7 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
   b->dev_index.assign(n, 0);
   std::vector<size_t> dev;
   uint32_t maxL = 0;
+  uint64_t est_terms = 0;  // fixed-base terms of the batch (tuning class)
   for (size_t i = 0; i < n; i++) {
     const auto& it = items[i];
     // protocol.rs:774-779: bad disclosed indexes or count mismatch -> Ok(false)
@@ -380,6 +394,7 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
     b->dev_index[i] = (uint32_t)dev.size();
     dev.push_back(i);
     maxL = std::max<uint32_t>(maxL, (uint32_t)L);
+    est_terms += L + 2;
   }
   b->nv = (uint32_t)dev.size();
   if (!b->nv) return b;
@@ -390,7 +405,7 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
   Plan p;
   std::string d = map_dst();
   p.map_dst.assign(d.begin(), d.end());
-  const uint32_t CH = comb_chunk_terms();
+  const uint32_t CH = comb_chunk_terms(est_terms);
   uint32_t T = 0;
   for (uint32_t k = 0; k < b->nv; k++) {
     const auto& it = items[dev[k]];
@@ -472,6 +487,7 @@ std::unique_ptr<Batch> plan_sign(const CuBbsBbsGpuSignItem* items, size_t n) {
   b->dev_index.assign(n, 0);
   std::vector<size_t> dev;
   uint32_t maxL = 0;
+  uint64_t est_terms = 0;  // fixed-base terms of the batch (tuning class)
   for (size_t i = 0; i < n; i++) {
     const auto& it = items[i];
     bool ok = it.secret_key.len == 32 && it.secret_key.data && it.public_key.len == 96 &&
@@ -489,6 +505,7 @@ std::unique_ptr<Batch> plan_sign(const CuBbsBbsGpuSignItem* items, size_t n) {
     b->dev_index[i] = (uint32_t)dev.size();
     dev.push_back(i);
     maxL = std::max<uint32_t>(maxL, (uint32_t)it.message_count);
+    est_terms += it.message_count + 2;
   }
   b->nv = (uint32_t)dev.size();
   if (!b->nv) return b;
@@ -499,7 +516,7 @@ std::unique_ptr<Batch> plan_sign(const CuBbsBbsGpuSignItem* items, size_t n) {
   Plan p;
   std::string d = map_dst();
   p.map_dst.assign(d.begin(), d.end());
-  const uint32_t CH = comb_chunk_terms();
+  const uint32_t CH = comb_chunk_terms(est_terms);
   uint32_t T = 0;
   for (uint32_t k = 0; k < b->nv; k++) T += (uint32_t)items[dev[k]].message_count + 2;
   p.scalars(T);
@@ -542,6 +559,7 @@ std::unique_ptr<Batch> plan_proof_gen(const CuBbsBbsGpuProofGenItem* items, size
   b->dev_index.assign(n, 0);
   std::vector<size_t> dev;
   uint32_t maxL = 0;
+  uint64_t est_terms = 0;  // fixed-base terms of the batch (tuning class)
   for (size_t i = 0; i < n; i++) {
     const auto& it = items[i];
     bool ok = it.public_key.len == 96 && it.public_key.data && it.signature.len == 80 &&
@@ -561,6 +579,7 @@ std::unique_ptr<Batch> plan_proof_gen(const CuBbsBbsGpuProofGenItem* items, size
     b->dev_index[i] = (uint32_t)dev.size();
     dev.push_back(i);
     maxL = std::max<uint32_t>(maxL, (uint32_t)it.message_count);
+    est_terms += it.message_count + 2;
   }
   b->nv = (uint32_t)dev.size();
   if (!b->nv) return b;
@@ -571,7 +590,7 @@ std::unique_ptr<Batch> plan_proof_gen(const CuBbsBbsGpuProofGenItem* items, size
   Plan p;
   std::string d = map_dst();
   p.map_dst.assign(d.begin(), d.end());
-  const uint32_t CH = comb_chunk_terms();
+  const uint32_t CH = comb_chunk_terms(est_terms);
   uint32_t T = 0;
   for (uint32_t k = 0; k < b->nv; k++) {
     const auto& it = items[dev[k]];

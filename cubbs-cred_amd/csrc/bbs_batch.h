@@ -152,7 +152,7 @@ struct Batch {
   ~Batch();
 };
 
-uint32_t comb_chunk_terms();
+uint32_t comb_chunk_terms(uint64_t batch_terms = 0, bool tuned = false);
 void fill_os_random(uint8_t* out, size_t n);
 
 std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n);

@@ -227,6 +227,18 @@ void launch_pg_finalize(hipStream_t s, const PgJob* jobs, uint32_t n, const uint
                         uint32_t* scalars, uint8_t* status, uint8_t* proof_out);
 void launch_g2_mul_gen(hipStream_t s, const uint8_t* sk, uint32_t n, uint8_t* pk_out, uint8_t* ok);
 
+// ------------------------------------------------------------------ tuning
+// Launch shapes picked by the measured autotuner (bbs_tune.cpp), per size
+// class: Pippenger window width c and entries per piece S by terms per MSM,
+// comb terms per lane by fixed-base terms per batch.  0 / false = no entry.
+bool tuned_pip(uint32_t terms_per_msm, uint32_t& c, uint32_t& S);
+uint32_t tuned_comb_chunk(uint64_t batch_terms);
+// Shapes forced on the calling thread while the tuner times a candidate.
+struct TuneForce {
+  uint32_t pip_c = 0, pip_s = 0, comb_chunk = 0;
+};
+TuneForce& tune_force();
+
 // ------------------------------------------------------------------ Pippenger MSM
 // One term of a large variable-base MSM: points[point] * scalars[scalar],
 // summed into the output partial of MSM `job`.
@@ -250,6 +262,8 @@ struct PipShape {
 };
 constexpr uint32_t PIP_MAX_TERMS = 1u << 26;
 PipShape pip_shape(uint32_t terms, uint32_t jobs);
+// window width minimising the FpM cost model for n terms per MSM
+uint32_t pip_model_window(double terms_per_msm);
 struct PipBuffers {
   uint32_t* cnt;      // [nbt + 1] entries per bucket
   uint32_t* off;      // [nbt + 1] first entry of each bucket

@@ -70,12 +70,9 @@ DEV void pip_digits(const uint32_t k[8], uint32_t c, uint32_t W, F f) {
 
 }  // namespace
 
-PipShape pip_shape(uint32_t terms, uint32_t jobs) {
-  PipShape sh{};
-  jobs = std::max<uint32_t>(jobs, 1);
-  const double n = (double)terms / jobs;
-  // FpM model: 2n entries per window at ~12 FpM (mixed add + piece share),
-  // ~2 full additions per bucket for the halving and the bucket sums.
+// FpM model: 2n entries per window at ~12 FpM (mixed add + piece share),
+// ~2 full additions per bucket for the halving and the bucket sums.
+uint32_t pip_model_window(double n) {
   uint32_t best = 8;
   double best_cost = 0;
   for (uint32_t c = 4; c <= 16; c++) {
@@ -86,11 +83,28 @@ PipShape pip_shape(uint32_t terms, uint32_t jobs) {
       best_cost = cost;
     }
   }
-  sh.c = std::min<uint32_t>(16, std::max<uint32_t>(4, env_or("CUBBS_HIP_PIP_WINDOW", best)));
+  return best;
+}
+
+PipShape pip_shape(uint32_t terms, uint32_t jobs) {
+  PipShape sh{};
+  jobs = std::max<uint32_t>(jobs, 1);
+  const double n = (double)terms / jobs;
+  const uint32_t best = pip_model_window(n);
+  // precedence: environment, the tuner's forced candidate, the tuned table,
+  // the model above
+  uint32_t c = best, S = 16, tc, ts;
+  if (tuned_pip((uint32_t)n, tc, ts)) {
+    c = tc;
+    S = ts;
+  }
+  if (tune_force().pip_c) c = tune_force().pip_c;
+  if (tune_force().pip_s) S = tune_force().pip_s;
+  sh.c = std::min<uint32_t>(16, std::max<uint32_t>(4, env_or("CUBBS_HIP_PIP_WINDOW", c)));
   sh.W = (129 + sh.c - 1) / sh.c;
   sh.K = sh.c - 1;
   sh.B = 1u << sh.K;
-  sh.S = std::min<uint32_t>(256, env_or("CUBBS_HIP_PIP_PIECE", 16));
+  sh.S = std::min<uint32_t>(256, std::max<uint32_t>(1, env_or("CUBBS_HIP_PIP_PIECE", S)));
   sh.jobs = jobs;
   sh.terms = terms;
   sh.nbt = jobs * sh.W * sh.B;

@@ -60,6 +60,7 @@ struct Prog {
   uint32_t slots = 0;                       // status slots
   std::vector<const uint8_t*> g2;           // G2 points to make resident (pairings)
   uint32_t max_gen = 2;                     // generators needed by comb terms
+  uint64_t comb_terms = 0;                  // fixed-base terms planned (tuning class)
 
   uint32_t slot() { return slots++; }
   uint32_t point(const uint8_t* c48, uint32_t st) {
@@ -86,7 +87,7 @@ struct Prog {
       p.term_gen[sc + t] = gen[t];
       max_gen = std::max(max_gen, gen[t] + 1);
     }
-    p.comb(sc, (uint32_t)gen.size(), comb_chunk_terms());
+    p.comb(sc, (uint32_t)gen.size(), comb_chunk_terms(comb_terms, true));
   }
   // Sum of the partials [first, n_partials) into a new affine point slot.
   uint32_t finish(uint32_t first) {
@@ -255,6 +256,7 @@ bool common_items_ok(const CuBbsBbsGpuG1MsmCommonPointsBatchItem* items, size_t 
 // One row per output of each group, in group order.
 void compile_common(Prog& g, const CuBbsBbsGpuG1MsmCommonPointsBatchItem* items, size_t n,
                     std::vector<PrimRow>& rows) {
+  for (size_t gi = 0; gi < n; gi++) g.comb_terms += (uint64_t)items[gi].output_count * items[gi].term_count;
   for (size_t gi = 0; gi < n; gi++) {
     const auto& it = items[gi];
     const uint32_t gst = g.slot();

@@ -56,6 +56,8 @@ EXT_SYMBOLS = [
     "cubbs_hip_batch_free",
     "cubbs_hip_batch_outputs",
     "cubbs_hip_batch_use_graph",
+    "cubbs_hip_autotune",
+    "cubbs_hip_tune_load",
     "cubbs_hip_batch_profile",
     "cubbs_hip_batch_shape",
     "cubbs_hip_test_g1_mul",
@@ -487,6 +489,17 @@ def generators(count):
     out = ctypes.create_string_buffer(48 * count)
     rc = lib().cubbs_hip_generators(ctypes.c_size_t(count), out)
     return rc, [out.raw[48 * i:48 * i + 48] for i in range(count)]
+
+
+def autotune(max_log2_terms=16):
+    """Measured launch-shape tuning; returns (status, report text)."""
+    buf = ctypes.create_string_buffer(8192)
+    rc = lib().cubbs_hip_autotune(ctypes.c_uint32(max_log2_terms), buf, ctypes.c_size_t(8192))
+    return rc, buf.value.decode()
+
+
+def tune_load(text):
+    return lib().cubbs_hip_tune_load(text.encode())
 
 
 def fp_mul_bench(lanes, iters):

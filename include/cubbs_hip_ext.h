@@ -123,6 +123,18 @@ int32_t cubbs_hip_server_stats(void *server, uint64_t *batches, uint64_t *reques
 /* Drains queued requests, joins the workers and frees the server. */
 void cubbs_hip_server_destroy(void *server);
 
+/* ---- launch-shape autotuner.  Times candidate shapes on synthetic inputs on
+ * the current device and keeps the fastest per size class for this process:
+ * Pippenger window width and entries per piece for MSMs of 2^10 ..
+ * 2^max_log2_terms terms (10 <= max_log2_terms <= 20), comb terms per lane
+ * for batches of 2^12 .. 2^18 fixed-base terms.  Writes one line per class
+ * ("pippenger log2_terms=K c=C S=S ms=T" / "comb log2_terms=K chunk=N ms=T")
+ * into report (NUL-terminated, truncated to report_cap). */
+int32_t cubbs_hip_autotune(uint32_t max_log2_terms, char *report, size_t report_cap);
+/* Installs a table in the report format (e.g. saved from an earlier run);
+ * an empty string clears the table.  A malformed line fails the call. */
+int32_t cubbs_hip_tune_load(const char *table);
+
 /* ---- primitive probes (parity tests) */
 /* out = k * P for n compressed points and 32-byte big-endian scalars; ok[i] = 0
  * when the point or scalar does not decode. */

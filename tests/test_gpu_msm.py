@@ -141,3 +141,28 @@ def test_staged_msm_batches_relaunch(gpu, oracle, multiples):
     finally:
         sb.free()
         cb.free()
+
+
+def test_autotune_table_and_parity(gpu, oracle, multiples):
+    """The measured tuner reports one line per size class, and MSMs planned
+    with the tuned shapes still equal the oracle."""
+    _, _, pts = multiples
+    try:
+        rc, rep = gpu.autotune(11)
+        assert rc == 0
+        lines = rep.strip().split("\n")
+        assert [ln.split()[0] for ln in lines] == ["pippenger"] * 2 + ["comb"] * 4
+        assert gpu.tune_load(rep) == 0
+        rng = random.Random(31)
+        items = [(pts[:1024], [be(rng.randrange(R_ORDER)) for _ in range(1024)]),
+                 (pts[:2048], [be(rng.randrange(R_ORDER)) for _ in range(2048)])]
+        rc, res = gpu.g1_msm(items)
+        assert rc == 0
+        for (p, s), (code, route, out) in zip(items, res):
+            assert (code, route) == (0, 3) and out == oracle.g1_msm(p, s)
+        rc, gens = gpu.generators(32)
+        rows = [[be(rng.randrange(R_ORDER)) for _ in range(32)] for _ in range(3)]
+        rc, cres = gpu.g1_msm_common_points([(gens, rows)], packed=True)
+        assert rc == 0 and cres[0][2] == oracle.g1_msm(gens, rows[0])
+    finally:
+        assert gpu.tune_load("") == 0

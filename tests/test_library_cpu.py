@@ -145,3 +145,12 @@ def test_isogeny_derivation_reproduces_fixture_generators():
                          capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
     assert "reproduces Q1 and H_1..H_10" in out.stderr
+
+
+def test_tune_table_load_parses_and_rejects():
+    lib = cubbs_hip.lib()
+    ok = b"pippenger log2_terms=16 c=13 S=16 ms=5.1\ncomb log2_terms=18 chunk=4 ms=2.0\n"
+    assert lib.cubbs_hip_tune_load(ok) == 0
+    assert lib.cubbs_hip_tune_load(b"pippenger log2_terms=16 c=3 S=16\n") == 1   # c out of range
+    assert lib.cubbs_hip_tune_load(b"window 5\n") == 1
+    assert lib.cubbs_hip_tune_load(b"") == 0   # clears the table
