@@ -322,7 +322,181 @@ DEVN void fp_pow_const(Fp& r, const Fp& a, const uint32_t* e) {
   }
   r = acc;
 }
-DEVI void fp_inv(Fp& r, const Fp& a) { fp_pow_const(r, a, bls::P_MINUS_2); }
+// ------------------------------------------------------------------ inversion
+// Bernstein-Yang safegcd with half-delta divsteps.  f = p, g = a (canonical);
+// each of 30 batches runs 30 divsteps on the low 32 bits of f and g, which
+// fix the batch's transition matrix [u v; q r] (|entries| <= 2^30), then
+// applies it to the full f, g (an exact division by 2^30) and to the
+// coefficients d, e with f = d a, g = e a (mod p), divided by 2^30 by adding
+// the multiple of p that clears the low 30 bits and kept in [0, p).  900
+// divsteps (>= 879, the bound for 381-bit inputs) leave g = 0 and f = +-1, so
+// a^-1 = +-d.  Numbers are 13 limbs of 30 bits, the top limb signed.  The
+// divsteps are branchless and the iteration count fixed, so the lanes of a
+// wave stay together.  For Montgomery inputs a = x R the result is
+// multiplied by R^3 (giving x^-1 R).  a = 0 gives 0.
+namespace inv30 {
+constexpr int L = 13;
+constexpr int32_t M30 = (1 << 30) - 1;
+
+DEV void divsteps30(int32_t& delta2, uint32_t f, uint32_t g, int32_t& u, int32_t& v, int32_t& q,
+                    int32_t& r) {
+  u = 1;
+  v = 0;
+  q = 0;
+  r = 1;
+#pragma unroll
+  for (int i = 0; i < 30; i++) {
+    const int32_t c2 = -(int32_t)(g & 1);                // g odd
+    const int32_t c1 = (delta2 > 0 ? -1 : 0) & c2;       // swap f and g
+    const uint32_t x = (f ^ (uint32_t)c1) - (uint32_t)c1;  // c1 ? -f : f
+    const int32_t y = (u ^ c1) - c1, z = (v ^ c1) - c1;
+    g += x & (uint32_t)c2;
+    q += y & c2;
+    r += z & c2;
+    delta2 = c1 ? 2 - delta2 : 2 + delta2;
+    f += g & (uint32_t)c1;
+    u += q & c1;
+    v += r & c1;
+    g >>= 1;
+    u = (int32_t)((uint32_t)u << 1);
+    v = (int32_t)((uint32_t)v << 1);
+  }
+}
+
+// (x, y) <- ((u x + v y) / 2^30, (q x + r y) / 2^30), exact
+DEV void update_fg(int32_t* x, int32_t* y, int32_t u, int32_t v, int32_t q, int32_t r) {
+  int64_t cx = (int64_t)u * x[0] + (int64_t)v * y[0];
+  int64_t cy = (int64_t)q * x[0] + (int64_t)r * y[0];
+  cx >>= 30;
+  cy >>= 30;
+#pragma unroll
+  for (int i = 1; i < L; i++) {
+    cx += (int64_t)u * x[i] + (int64_t)v * y[i];
+    cy += (int64_t)q * x[i] + (int64_t)r * y[i];
+    x[i - 1] = (int32_t)cx & M30;
+    y[i - 1] = (int32_t)cy & M30;
+    cx >>= 30;
+    cy >>= 30;
+  }
+  x[L - 1] = (int32_t)cx;
+  y[L - 1] = (int32_t)cy;
+}
+
+// a += p & mask (carries propagated; top limb keeps the sign)
+DEV void add_p_masked(int32_t* a, int32_t mask) {
+  int32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < L - 1; i++) {
+    c += a[i] + (bls::P_S30[i] & mask);
+    a[i] = c & M30;
+    c >>= 30;
+  }
+  a[L - 1] += c + (bls::P_S30[L - 1] & mask);
+}
+
+// a in (-p, 2p) -> [0, p)
+DEV void normalize(int32_t* a) {
+  add_p_masked(a, a[L - 1] >> 31);
+  int32_t b[L];
+  int32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < L - 1; i++) {
+    c += a[i] - bls::P_S30[i];
+    b[i] = c & M30;
+    c >>= 30;
+  }
+  b[L - 1] = a[L - 1] - bls::P_S30[L - 1] + c;
+  const int32_t keep = b[L - 1] >> 31;  // a - p < 0: keep a
+#pragma unroll
+  for (int i = 0; i < L; i++) a[i] = keep ? a[i] : b[i];
+}
+
+// (d, e) <- ((u d + v e) / 2^30, (q d + r e) / 2^30) mod p, d, e in [0, p)
+DEV void update_de(int32_t* d, int32_t* e, int32_t u, int32_t v, int32_t q, int32_t r) {
+  int64_t cd = (int64_t)u * d[0] + (int64_t)v * e[0];
+  int64_t ce = (int64_t)q * d[0] + (int64_t)r * e[0];
+  const int32_t md = (int32_t)(((uint32_t)cd * bls::P_INV30) & (uint32_t)M30);
+  const int32_t me = (int32_t)(((uint32_t)ce * bls::P_INV30) & (uint32_t)M30);
+  cd += (int64_t)md * bls::P_S30[0];
+  ce += (int64_t)me * bls::P_S30[0];
+  cd >>= 30;
+  ce >>= 30;
+#pragma unroll
+  for (int i = 1; i < L; i++) {
+    cd += (int64_t)u * d[i] + (int64_t)v * e[i] + (int64_t)md * bls::P_S30[i];
+    ce += (int64_t)q * d[i] + (int64_t)r * e[i] + (int64_t)me * bls::P_S30[i];
+    d[i - 1] = (int32_t)cd & M30;
+    e[i - 1] = (int32_t)ce & M30;
+    cd >>= 30;
+    ce >>= 30;
+  }
+  d[L - 1] = (int32_t)cd;
+  e[L - 1] = (int32_t)ce;
+  normalize(d);
+  normalize(e);
+}
+
+// canonical 28-bit limbs -> 30-bit limbs (bit 30j starts in limb 30j/28 at
+// offset 2j mod 28 <= 24, so two source limbs always cover it)
+DEV void from_fp(int32_t* o, const Fp& a) {
+#pragma unroll
+  for (int j = 0; j < L; j++) {
+    const int bit = 30 * j, i0 = bit / 28, s = bit % 28;
+    uint32_t w = a.v[i0] >> s;
+    if (i0 + 1 < 14) w |= a.v[i0 + 1] << (28 - s);
+    o[j] = (int32_t)(w & (uint32_t)M30);
+  }
+}
+// non-negative 30-bit limbs -> 28-bit limbs
+DEV void to_fp(Fp& r, const int32_t* d) {
+#pragma unroll
+  for (int i = 0; i < 14; i++) {
+    const int bit = 28 * i, j0 = bit / 30, s = bit % 30;
+    uint32_t w = j0 < L ? (uint32_t)d[j0] >> s : 0u;
+    if (j0 + 1 < L) w |= (uint32_t)d[j0 + 1] << (30 - s);
+    r.v[i] = w & FP_MASK;
+  }
+}
+}  // namespace inv30
+
+DEVN void fp_inv(Fp& r, const Fp& a) {
+  using namespace inv30;
+  Fp c;
+  fp_canon(c, a);
+  int32_t f[L], g[L], d[L], e[L];
+#pragma unroll
+  for (int i = 0; i < L; i++) {
+    f[i] = bls::P_S30[i];
+    d[i] = 0;
+    e[i] = 0;
+  }
+  e[0] = 1;
+  from_fp(g, c);
+  int32_t delta2 = 1;
+#pragma unroll 1
+  for (int b = 0; b < 30; b++) {
+    int32_t u, v, q, rr;
+    divsteps30(delta2, (uint32_t)f[0] | ((uint32_t)f[1] << 30),
+               (uint32_t)g[0] | ((uint32_t)g[1] << 30), u, v, q, rr);
+    update_fg(f, g, u, v, q, rr);
+    update_de(d, e, u, v, q, rr);
+  }
+  // f = -1: the inverse is p - d (p when d = 0, still below 2p)
+  const int32_t neg = f[L - 1] >> 31;
+  int32_t cy = 0;
+#pragma unroll
+  for (int i = 0; i < L - 1; i++) {
+    cy += (bls::P_S30[i] - d[i]);
+    const int32_t nd = cy & M30;
+    cy >>= 30;
+    d[i] = neg ? nd : d[i];
+  }
+  d[L - 1] = neg ? bls::P_S30[L - 1] - d[L - 1] + cy : d[L - 1];
+  Fp x, r3;
+  to_fp(x, d);
+  fp_load_const(r3, bls::P_R3);
+  fp_mul(r, x, r3);
+}
 // returns false if a is not a square
 DEVI bool fp_sqrt(Fp& r, const Fp& a) {
   Fp s, t;

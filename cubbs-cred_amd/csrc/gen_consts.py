@@ -122,6 +122,12 @@ def main():
     arr("P_ONE", [mont(1)], "fp")
     # 2^384 R^2 mod p: mont_mul(hi, P_WIDE_HI) = Montgomery form of hi * 2^384
     arr("P_WIDE_HI", [(1 << 384) * RP * RP % P], "fp")
+    # safegcd inversion (fp_inv): p in 13 signed 30-bit limbs, -p^-1 mod 2^30,
+    # and R^3 mod p (fp_mul(x^-1, R^3) = Montgomery form of the inverse)
+    emit("static constexpr int32_t P_S30[13] = {%s};" % ", ".join(
+        "0x%08x" % ((P >> (30 * i)) & ((1 << 30) - 1)) for i in range(13)))
+    emit("static constexpr uint32_t P_INV30 = 0x%08xu;" % ((-pow(P, -1, 1 << 30)) % (1 << 30)))
+    arr("P_R3", [RP ** 3 % P], "fp")
     arr("P_MINUS_2", [P - 2], 12)
     arr("P_SQRT_EXP", [(P + 1) // 4], 12)
     arr("P_LEGENDRE_EXP", [(P - 1) // 2], 12)
