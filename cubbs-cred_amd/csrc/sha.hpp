@@ -107,13 +107,44 @@ DEVI void sha_word(Sha256& s, uint32_t wv) {
   }
 }
 
+// Big-endian word of the 4 bytes at q[0] >> sh (sh = 8 * (address & 3)),
+// taking the high bytes from the next aligned dword when unaligned.
+DEV uint32_t sha_be_word(uint32_t lo, uint32_t hi, uint32_t sh) {
+  return __builtin_bswap32((uint32_t)((((uint64_t)hi << 32) | lo) >> sh));
+}
+
+// Bytes from global memory.  After aligning the stream to a word, bytes are
+// read as aligned dwords (funnel-shifted when the source is unaligned; the
+// next dword is read only when it holds bytes of the range), and whole 64-byte
+// blocks go straight into the block registers from 16-17 loads issued together.
 DEVI void sha_bytes_global(Sha256& s, const uint8_t* p, uint32_t len) {
   uint32_t i = 0;
-  // byte-wise until the stream is word aligned, then words built from 4 loads
   while (i < len && (s.n & 3)) sha_byte(s, p[i++]);
+  const uintptr_t a = (uintptr_t)(p + i);
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(a & ~(uintptr_t)3);
+  const uint32_t sh = (uint32_t)(a & 3) * 8;
+  while (i + 4 <= len && (s.n & 63)) {
+    const uint32_t lo = q[0], hi = sh ? q[1] : 0u;
+    sha_word(s, sha_be_word(lo, hi, sh));
+    q++;
+    i += 4;
+  }
+  while (i + 64 <= len) {
+    uint32_t d[17];
+#pragma unroll
+    for (int k = 0; k < 16; k++) d[k] = q[k];
+    d[16] = sh ? q[16] : 0u;
+#pragma unroll
+    for (int k = 0; k < 16; k++) s.w[k] = sha_be_word(d[k], d[k + 1], sh);
+    s.n += 64;
+    sha256_compress(s.h, s.w);
+    q += 16;
+    i += 64;
+  }
   while (i + 4 <= len) {
-    uint32_t wv = ((uint32_t)p[i] << 24) | ((uint32_t)p[i + 1] << 16) | ((uint32_t)p[i + 2] << 8) | p[i + 3];
-    sha_word(s, wv);
+    const uint32_t lo = q[0], hi = sh ? q[1] : 0u;
+    sha_word(s, sha_be_word(lo, hi, sh));
+    q++;
     i += 4;
   }
   while (i < len) sha_byte(s, p[i++]);
