@@ -767,13 +767,16 @@ void launch_batch(Batch& b) {
   if (fork) {
     check(hipEventRecord(b.ev_fork, s));
     check(hipStreamWaitEvent(b.aux, b.ev_fork, 0));
+    // the subgroup check of the decoded points runs as extra lanes of the
+    // product kernel, beside the products rather than before them
     if (b.n_dec_g1) {
       Timed t(b, "decode_g1", b.aux);
-      launch_decode_g1(b.aux, b.dec_g1, b.n_dec_g1, b.blob, b.points, b.pinf, b.status);
+      launch_decode_g1(b.aux, b.dec_g1, b.n_dec_g1, b.blob, b.points, b.pinf, b.status, false);
     }
-    if (b.n_var) {
+    if (b.n_var || b.n_dec_g1) {
       Timed t(b, "var_mul_glv", b.aux);
-      launch_var_mul(b.aux, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials);
+      launch_var_mul(b.aux, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials, b.dec_g1,
+                     b.n_dec_g1, b.status);
     }
     check(hipEventRecord(b.ev_join, b.aux));
   } else if (b.n_dec_g1) {

@@ -187,12 +187,15 @@ void launch_reduce(hipStream_t s, const ReduceGroup* groups, const uint32_t* out
 void launch_prepare_pk(hipStream_t s, const uint32_t* slots, uint32_t n, const uint8_t* pk_bytes,
                        uint8_t* pk_status, uint32_t* pk_lines);
 void launch_prepare_neg_g2(hipStream_t s, uint32_t* lines);
+// check_subgroup = false: launch_var_mul's extra lanes check the subgroup
 void launch_decode_g1(hipStream_t s, const DecodeG1Job* jobs, uint32_t n, const uint8_t* blob,
-                      uint32_t* points, uint8_t* pinf, uint8_t* status);
+                      uint32_t* points, uint8_t* pinf, uint8_t* status, bool check_subgroup = true);
 void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const uint8_t* blob,
                       uint32_t* scalars, uint8_t* status);
+// n_sub > 0: also check the subgroup membership of the points of sub[]
 void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_t* points,
-                    const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials);
+                    const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials,
+                    const DecodeG1Job* sub = nullptr, uint32_t n_sub = 0, uint8_t* status = nullptr);
 void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t* scalars);
 // l2buf: PAIR_L2_WORDS words of device scratch per job, filled by
 // launch_pair_lines and read by launch_pairing

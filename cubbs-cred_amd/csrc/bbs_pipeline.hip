@@ -30,16 +30,19 @@ void launch_fill_status(hipStream_t s, uint32_t n, uint8_t* status, uint8_t* gat
 }
 
 // ------------------------------------------------------------------ decoding
+// check_subgroup = 0 leaves the prime-order subgroup check to the lanes that
+// k_var_mul appends for these jobs.
 __global__ __launch_bounds__(64) void k_decode_g1(const DecodeG1Job* __restrict__ jobs, uint32_t n,
                                                  const uint8_t* __restrict__ blob,
                                                  uint32_t* __restrict__ points,
                                                  uint8_t* __restrict__ pinf,
-                                                 uint8_t* __restrict__ status) {
+                                                 uint8_t* __restrict__ status,
+                                                 uint32_t check_subgroup) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   DecodeG1Job j = jobs[i];
   G1J p;
-  bool ok = g1_decompress(p, blob + j.off);
+  bool ok = g1_decompress(p, blob + j.off, check_subgroup != 0);
   G1A a;
   if (ok && !pt_is_inf(p)) {
     a.x = p.x;
@@ -54,9 +57,10 @@ __global__ __launch_bounds__(64) void k_decode_g1(const DecodeG1Job* __restrict_
   if (!ok) status[j.item] = 1;
 }
 void launch_decode_g1(hipStream_t s, const DecodeG1Job* jobs, uint32_t n, const uint8_t* blob,
-                      uint32_t* points, uint8_t* pinf, uint8_t* status) {
+                      uint32_t* points, uint8_t* pinf, uint8_t* status, bool check_subgroup) {
   if (!n) return;
-  hipLaunchKernelGGL(k_decode_g1, GRID(n, 64), 0, s, jobs, n, blob, points, pinf, status);
+  hipLaunchKernelGGL(k_decode_g1, GRID(n, 64), 0, s, jobs, n, blob, points, pinf, status,
+                     check_subgroup ? 1u : 0u);
 }
 
 __global__ __launch_bounds__(256) void k_decode_fr(const DecodeFrJob* __restrict__ jobs, uint32_t n,
@@ -114,13 +118,28 @@ void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t*
 // ------------------------------------------------------------------ variable-base multiplication
 // Two lanes per job: lane 0 forms [k1] P, lane 1 [k2] phi(P) (k = k1 + k2 lambda);
 // the reduction stage adds the two partials.
+// Lanes [0, 2n): the two GLV halves of each product.  Lanes [2n, 2n + n_sub):
+// the prime-order subgroup check of the points decoded by jobs sub[] without
+// it, run beside the products; a failure marks the item's status (the
+// products of such a point are computed but never used).
 __global__ __launch_bounds__(64) void k_var_mul(const VarJob* __restrict__ jobs, uint32_t n,
                                                const uint32_t* __restrict__ points,
                                                const uint8_t* __restrict__ pinf,
                                                const uint32_t* __restrict__ scalars,
-                                               uint32_t* __restrict__ partials) {
+                                               uint32_t* __restrict__ partials,
+                                               const DecodeG1Job* __restrict__ sub, uint32_t n_sub,
+                                               uint8_t* __restrict__ status) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2 * n) return;
+  if (t >= 2 * n) {
+    const uint32_t k = t - 2 * n;
+    if (k >= n_sub) return;
+    const DecodeG1Job j = sub[k];
+    if (pinf[j.out]) return;  // the identity, or not decoded (status already set)
+    G1A a;
+    load_g1a(a, points + (size_t)j.out * G1A_WORDS);
+    if (!g1_in_subgroup(a)) status[j.item] = 1;
+    return;
+  }
   const uint32_t i = t >> 1, half = t & 1;
   VarJob j = jobs[i];
   G1J r;
@@ -135,9 +154,11 @@ __global__ __launch_bounds__(64) void k_var_mul(const VarJob* __restrict__ jobs,
   store_g1j(partials + (size_t)(j.partial + half) * G1J_WORDS, r);
 }
 void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_t* points,
-                    const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_var_mul, GRID(2 * n, 64), 0, s, jobs, n, points, pinf, scalars, partials);
+                    const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials,
+                    const DecodeG1Job* sub, uint32_t n_sub, uint8_t* status) {
+  if (!n && !n_sub) return;
+  hipLaunchKernelGGL(k_var_mul, GRID(2 * n + n_sub, 64), 0, s, jobs, n, points, pinf, scalars,
+                     partials, sub, n_sub, status);
 }
 
 // ------------------------------------------------------------------ reduction

@@ -420,7 +420,7 @@ DEVN bool g1_in_subgroup(const G1A& pa) {
 // Decode a 48-byte compressed G1 point (bls12_381 G1Affine::from_compressed
 // semantics): compression flag set; infinity encoding must be 0xc0 00..00;
 // x < p; on curve; in the prime-order subgroup.  Returns false on any failure.
-DEVN bool g1_decompress(G1J& r, const uint8_t* in) {
+DEVN bool g1_decompress(G1J& r, const uint8_t* in, bool check_subgroup = true) {
   uint8_t b[48];
 #pragma unroll
   for (int i = 0; i < 48; i++) b[i] = in[i];
@@ -447,6 +447,7 @@ DEVN bool g1_decompress(G1J& r, const uint8_t* in) {
   r.x = x;
   r.y = y;
   fp_one(r.z);
+  if (!check_subgroup) return true;
   G1A a;
   a.x = x;
   a.y = y;

@@ -497,10 +497,40 @@ DEVN void fp_inv(Fp& r, const Fp& a) {
   fp_load_const(r3, bls::P_R3);
   fp_mul(r, x, r3);
 }
+// r = a^e for e given as a sliding-window schedule with 4-bit odd digits
+// (first digit, then `squarings, digit` steps, then `tail` squarings); the
+// odd powers a, a^3, .., a^15 are picked by selects (no indexed registers).
+DEVN void fp_pow_window4(Fp& r, const Fp& a, const uint8_t* sq, const uint8_t* dig, int steps,
+                         int tail) {
+  Fp T[8], a2;
+  fp_sqr(a2, a);
+  T[0] = a;
+#pragma unroll
+  for (int j = 1; j < 8; j++) fp_mul(T[j], T[j - 1], a2);
+  auto pick = [&](uint32_t d) {
+    Fp t = T[0];
+#pragma unroll
+    for (int j = 1; j < 8; j++)
+      if ((d >> 1) == (uint32_t)j) t = T[j];
+    return t;
+  };
+  Fp acc = pick(dig[0]);
+#pragma unroll 1
+  for (int i = 1; i < steps; i++) {
+#pragma unroll 1
+    for (int k = 0; k < sq[i]; k++) fp_sqr(acc, acc);
+    const Fp t = pick(dig[i]);
+    fp_mul(acc, acc, t);
+  }
+#pragma unroll 1
+  for (int k = 0; k < tail; k++) fp_sqr(acc, acc);
+  r = acc;
+}
+
 // returns false if a is not a square
 DEVI bool fp_sqrt(Fp& r, const Fp& a) {
   Fp s, t;
-  fp_pow_const(s, a, bls::P_SQRT_EXP);
+  fp_pow_window4(s, a, bls::SQRT_W4_SQ, bls::SQRT_W4_DIG, bls::SQRT_W4_STEPS, bls::SQRT_W4_TAIL);
   fp_sqr(t, s);
   r = s;
   return fp_eq(t, a);

@@ -93,6 +93,28 @@ def decompress_g1(hexs):
     return x, y
 
 
+def sliding_window(e, w):
+    """Left-to-right sliding-window schedule of e: first digit, then (squarings,
+    odd digit) steps, then trailing squarings."""
+    bits = bin(e)[2:]
+    steps, i, pending = [], 0, 0
+    while i < len(bits):
+        if bits[i] == "0":
+            pending += 1
+            i += 1
+            continue
+        j = min(i + w, len(bits))
+        while bits[j - 1] == "0":
+            j -= 1
+        steps.append((pending + (j - i) if steps else 0, int(bits[i:j], 2)))
+        pending, i = 0, j
+    v = 0
+    for sq, d in steps:
+        v = (v << sq) + d
+    assert v << pending == e
+    return steps, pending
+
+
 def main():
     out = []
     emit = out.append
@@ -130,6 +152,12 @@ def main():
     arr("P_R3", [RP ** 3 % P], "fp")
     arr("P_MINUS_2", [P - 2], 12)
     arr("P_SQRT_EXP", [(P + 1) // 4], 12)
+    # (p+1)/4 as a 4-bit sliding-window schedule for fp_sqrt
+    steps, tail = sliding_window((P + 1) // 4, 4)
+    emit("static constexpr int SQRT_W4_STEPS = %d;" % len(steps))
+    emit("static constexpr int SQRT_W4_TAIL = %d;" % tail)
+    emit("static constexpr uint8_t SQRT_W4_SQ[%d] = {%s};" % (len(steps), ", ".join(str(a) for a, _ in steps)))
+    emit("static constexpr uint8_t SQRT_W4_DIG[%d] = {%s};" % (len(steps), ", ".join(str(b) for _, b in steps)))
     arr("P_LEGENDRE_EXP", [(P - 1) // 2], 12)
     arr("P_MINUS3_DIV4", [(P - 3) // 4], 12)
     arr("P_HALF", [(P - 1) // 2], "fp")  # canonical (not Montgomery), lexicographic sign

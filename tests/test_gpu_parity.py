@@ -379,3 +379,42 @@ def test_staged_batch_relaunch_is_stable(gpu, oracle):
         assert r == first
     assert [v for _, v in first] == [i % 4 != 3 for i in range(16)]
     b.free()
+
+
+def _on_curve_outside_g1(oracle):
+    """Compressed encoding of a point on E(Fp) outside the prime-order subgroup."""
+    p = 0x1A0111EA397FE69A4B1BA7B6434BACD764774B84F38512BF6730D2A0F6B0F6241EABFFFEB153FFFFB9FEFFFFFFFFAAAB
+    for x in range(1, 400):
+        rhs = (x ** 3 + 4) % p
+        y = pow(rhs, (p + 1) // 4, p)
+        if y * y % p == rhs:
+            enc = bytearray(x.to_bytes(48, "big"))
+            enc[0] |= 0x80
+            assert not oracle.g1_valid(bytes(enc))
+            return bytes(enc)
+    raise AssertionError("no point found")
+
+
+def test_points_outside_subgroup_rejected(gpu, oracle):
+    """Verify's A and ProofVerify's Abar / Bbar / D on the curve but outside G1
+    (the subgroup check runs beside the GLV products) give code 1; the other
+    items of the batch are unaffected."""
+    rng = random.Random(41)
+    (sk, pk), = _issuers(gpu, rng, 1)
+    bad = _on_curve_outside_g1(oracle)
+    msgs = [rng.randbytes(40) for _ in range(6)]
+    sig = oracle.sign(sk, pk, b"hdr", msgs)
+    rc, res = gpu.verify_batch([(pk, sig, b"hdr", msgs), (pk, bad + sig[48:], b"hdr", msgs)])
+    assert rc == 0 and res == [(0, True), (1, False)]
+    idx = [1, 4]
+    rc, pr = gpu.proof_gen_batch([(pk, sig, b"hdr", b"ph", msgs, idx)], seeds=[bytes(32)])
+    assert rc == 0 and pr[0][0] == 0
+    proof = pr[0][1]
+    reqs = [(pk, proof, b"hdr", b"ph", [msgs[i] for i in idx], idx, len(msgs))]
+    for k in range(3):  # Abar, Bbar, D
+        broken = proof[:48 * k] + bad + proof[48 * (k + 1):]
+        reqs.append((pk, broken, b"hdr", b"ph", [msgs[i] for i in idx], idx, len(msgs)))
+    rc, res = gpu.proof_verify_batch(reqs)
+    assert rc == 0
+    assert res[0] == (0, True)
+    assert [c for c, _ in res[1:]] == [1, 1, 1]
