@@ -18,6 +18,7 @@
 #include "../../include/cubbs_hip_ext.h"
 #include "bbs_batch.h"
 #include "bbs_plan.h"
+#include "bls12_381_consts.hpp"
 #include "host_ctx.h"
 
 namespace cubbs {
@@ -114,6 +115,7 @@ void materialize(Batch& b, Plan& p) {
   size_t o_g2 = place(p.groups2.data(), p.groups2.size() * sizeof(ReduceGroup));
   size_t o_o2 = place(p.out2.data(), p.out2.size() * 4);
   size_t o_pair = place(p.pairs.data(), p.pairs.size() * sizeof(PairJob));
+  size_t o_agg = place(p.agg_var.data(), p.agg_var.size() * sizeof(VarJob));
   std::vector<std::pair<size_t, size_t>> o_lv;
   for (size_t l = 0; l < p.pst_groups.size(); l++)
     o_lv.emplace_back(place(p.pst_groups[l].data(), p.pst_groups[l].size() * sizeof(ReduceGroup)),
@@ -165,6 +167,10 @@ void materialize(Batch& b, Plan& p) {
   b.n_groups2 = (uint32_t)p.groups2.size();
   b.pairs = reinterpret_cast<PairJob*>(D + o_pair);
   b.n_pairs = (uint32_t)p.pairs.size();
+  b.agg_var = reinterpret_cast<VarJob*>(D + o_agg);
+  b.n_agg_var = (uint32_t)p.agg_var.size();
+  b.n_agg_slots = p.n_agg_slots;
+  b.n_gpairs = p.n_group_pairs;
   for (size_t l = 0; l < o_lv.size(); l++)
     b.levels.push_back(Batch::Level{reinterpret_cast<ReduceGroup*>(D + o_lv[l].first),
                                     reinterpret_cast<uint32_t*>(D + o_lv[l].second),
@@ -200,9 +206,10 @@ void materialize(Batch& b, Plan& p) {
   size_t o_pt = take(4ull * G1A_WORDS * std::max<uint32_t>(p.n_points, 1));
   size_t o_pi = take(std::max<uint32_t>(p.n_points, 1));
   size_t o_pa = take(4ull * G1J_WORDS * std::max<uint32_t>(p.n_partials, 1));
-  size_t o_st = take(std::max<uint32_t>(b.nv, 1));
-  size_t o_ga = take(std::max<uint32_t>(b.nv, 1));
-  size_t o_va = take(std::max<uint32_t>(b.nv, 1));
+  const uint32_t slots = b.nv + p.n_agg_slots;  // items, then aggregated groups
+  size_t o_st = take(std::max<uint32_t>(slots, 1));
+  size_t o_ga = take(std::max<uint32_t>(slots, 1));
+  size_t o_va = take(std::max<uint32_t>(slots, 1));
   size_t o_out = take(std::max<size_t>(p.out_bytes, 1));
   size_t o_l2 = take(4ull * PAIR_L2_WORDS * std::max<size_t>(p.pairs.size(), 1));
   const PipShape& ps = b.pip;
@@ -256,6 +263,101 @@ void materialize(Batch& b, Plan& p) {
     check(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
   }
   check(hipMemcpyAsync(b.d_in, b.h_in, b.in_bytes, hipMemcpyHostToDevice, b.stream));
+}
+
+// ------------------------------------------------------------------ aggregated pairing checks
+// Random linear combination of same-key checks (the batch test of the
+// reference's verify_homogeneous_pre_b_aggregate, crates/cubbs-bbs/src/
+// execution.rs:548-838, applied per group of one key): for checks
+// e(a_i, PK) e(b_i, -G2) = 1, i in a group, the combined check
+// e(sum c_i a_i, PK) e(sum c_i b_i, -G2) = 1 runs first, with coefficients
+// c_i = s_i + t_i lambda (s_i, t_i uniform 64-bit from the OS CSPRNG, drawn
+// after the inputs are fixed: 2^128 distinct values, and the GLV split of c_i
+// is (s_i, t_i), so each product is a 64-bit ladder).  If it holds, every
+// item's equation holds except with probability 2^-128 and the item is
+// accepted without its own pairing; if not, each item is checked on its own.
+// The per-item results are those of the individual checks either way.
+// Off unless CUBBS_HIP_AGG_GROUP (2..8) is set: a pairing launch costs about
+// one check's latency however few checks it holds, so the extra combined-check
+// launch (and the re-check launch when a group fails) outweighs the pairings
+// saved on batches that already fill the GPU.
+uint32_t agg_group_max() { return env_u32("CUBBS_HIP_AGG_GROUP", 0); }
+
+static void agg_coefficient(uint8_t out[32], uint64_t s, uint64_t t) {
+  // s + t * lambda, lambda = GLV_LAMBDA (127.4 bits): below 2^192 < r
+  const uint64_t l0 = (uint64_t)bls::GLV_LAMBDA[0] | (uint64_t)bls::GLV_LAMBDA[1] << 32;
+  const uint64_t l1 = (uint64_t)bls::GLV_LAMBDA[2] | (uint64_t)bls::GLV_LAMBDA[3] << 32;
+  const unsigned __int128 p0 = (unsigned __int128)t * l0 + s;
+  const unsigned __int128 p1 = (unsigned __int128)t * l1 + (uint64_t)(p0 >> 64);
+  const uint64_t w[4] = {(uint64_t)p0, (uint64_t)p1, (uint64_t)(p1 >> 64), 0};
+  for (int k = 0; k < 4; k++)
+    for (int j = 0; j < 8; j++) out[31 - 8 * k - j] = (uint8_t)(w[k] >> (8 * j));
+}
+
+void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max) {
+  if (group_max < 2 || group_max > 8) return;  // 2 x 8 partials per sum: one final reduction
+  std::vector<PairJob> in = std::move(p.pairs);
+  p.pairs.clear();
+  std::vector<uint32_t> order(in.size());
+  for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+  auto plain = [&](const PairJob& j) { return j.pk_b == PAIR_NEG_G2 && j.flags == 0; };
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    const bool px = plain(in[x]), py = plain(in[y]);
+    if (px != py) return px;
+    return px && in[x].pk_slot < in[y].pk_slot;
+  });
+  std::vector<uint8_t> rnd(16 * in.size());
+  fill_os_random(rnd.data(), rnd.size());
+  std::vector<PairJob> groups, grouped, single;
+  size_t i = 0;
+  while (i < order.size()) {
+    const PairJob& j0 = in[order[i]];
+    size_t e = i + 1;
+    if (plain(j0))
+      while (e < order.size() && e - i < group_max && plain(in[order[e]]) &&
+             in[order[e]].pk_slot == j0.pk_slot)
+        e++;
+    if (!plain(j0) || e - i < 2) {
+      for (size_t k = i; k < e; k++) single.push_back(in[order[k]]);
+      i = e;
+      continue;
+    }
+    const uint32_t gslot = nv + p.n_agg_slots++;
+    const uint32_t cnt = (uint32_t)(e - i);
+    std::vector<uint32_t> sc(cnt);
+    for (uint32_t k = 0; k < cnt; k++) {
+      uint64_t s, t;
+      std::memcpy(&s, &rnd[16 * order[i + k]], 8);
+      std::memcpy(&t, &rnd[16 * order[i + k] + 8], 8);
+      uint8_t c[32];
+      agg_coefficient(c, s, t);
+      sc[k] = p.scalars(1);
+      p.dec_fr.push_back(DecodeFrJob{p.bytes(c, 32), sc[k], gslot, 0u, 0u});
+    }
+    const uint32_t fa = p.n_partials;
+    for (uint32_t k = 0; k < cnt; k++)
+      p.agg_var.push_back(VarJob{in[order[i + k]].a, sc[k], p.partials(VAR_PARTIALS)});
+    const uint32_t fb = p.n_partials;
+    for (uint32_t k = 0; k < cnt; k++)
+      p.agg_var.push_back(VarJob{in[order[i + k]].b, sc[k], p.partials(VAR_PARTIALS)});
+    const uint32_t la = p.points(1), lb = p.points(1);
+    p.groups2.push_back(ReduceGroup{fa, VAR_PARTIALS * cnt});
+    p.out2.push_back(la);
+    p.groups2.push_back(ReduceGroup{fb, VAR_PARTIALS * cnt});
+    p.out2.push_back(lb);
+    groups.push_back(PairJob{gslot, la, lb, j0.pk_slot, PAIR_NEG_G2, 0, 0});
+    for (size_t k = i; k < e; k++) {
+      PairJob g = in[order[k]];
+      g.flags |= PAIR_GROUPED;
+      g.group = gslot;
+      grouped.push_back(g);
+    }
+    i = e;
+  }
+  p.n_group_pairs = (uint32_t)groups.size();
+  p.pairs = std::move(groups);
+  p.pairs.insert(p.pairs.end(), grouped.begin(), grouped.end());
+  p.pairs.insert(p.pairs.end(), single.begin(), single.end());
 }
 
 // Lock the context, make generators/public keys resident, snapshot pointers.
@@ -340,6 +442,7 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
     p.pairs.push_back(PairJob{k, pa, pb, pk_slot[k], PAIR_NEG_G2, 0});
     term += L + 2;
   }
+  aggregate_pairs(p, b->nv, agg_group_max());
   materialize(*b, p);
   return b;
 }
@@ -473,6 +576,7 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
     p.pairs.push_back(PairJob{k, pab, pbb, pk_slot[k], PAIR_NEG_G2, 0});
     term += L + 2;
   }
+  aggregate_pairs(p, b->nv, agg_group_max());
   materialize(*b, p);
   return b;
 }
@@ -753,7 +857,7 @@ void launch_batch(Batch& b) {
   hipStream_t s = b.stream;
   {
     Timed t(b, "fill_status");
-    launch_fill_status(s, b.nv, b.status, b.gate, 1);
+    launch_fill_status(s, b.nv + b.n_agg_slots, b.status, b.gate, 1);
   }
   // Verify / ProofVerify: point decode and the variable-base products run on
   // the auxiliary stream while the message hashing, domain and comb MSM run
@@ -874,14 +978,30 @@ void launch_batch(Batch& b) {
     launch_pv_challenge(s, b.pvch, b.n_pvch, b.blob, b.disc_index, b.disc_scalar, b.points,
                         b.pinf, b.scalars, b.gate);
   }
+  if (b.n_agg_var) {
+    // sums c_i a_i, c_i b_i of each aggregated group
+    {
+      Timed t(b, "agg_mul");
+      launch_var_mul(s, b.agg_var, b.n_agg_var, b.points, b.pinf, b.scalars, b.partials);
+    }
+    Timed t(b, "agg_reduce");
+    launch_reduce(s, b.groups2, b.out2, b.n_groups2, b.partials, b.points, b.pinf);
+  }
   if (b.n_pairs) {
     {
       Timed t(b, "pair_lines");
       launch_pair_lines(s, b.pairs, b.n_pairs, b.points, b.pk_lines, b.neg_g2_lines, b.l2buf);
     }
+    // the group checks first: their results gate the per-item checks
+    if (b.n_gpairs) {
+      Timed t(b, "pairing_groups");
+      launch_pairing(s, b.pairs, b.n_gpairs, b.points, b.pinf, b.pk_status, b.pk_lines,
+                     b.neg_g2_lines, b.l2buf, b.status, b.gate, b.valid);
+    }
     Timed t(b, "pairing");
-    launch_pairing(s, b.pairs, b.n_pairs, b.points, b.pinf, b.pk_status, b.pk_lines,
-                   b.neg_g2_lines, b.l2buf, b.status, b.gate, b.valid);
+    launch_pairing(s, b.pairs + b.n_gpairs, b.n_pairs - b.n_gpairs, b.points, b.pinf, b.pk_status,
+                   b.pk_lines, b.neg_g2_lines, b.l2buf + (size_t)b.n_gpairs * PAIR_L2_WORDS,
+                   b.status, b.gate, b.valid);
   }
   check(hipGetLastError());
 }

@@ -95,6 +95,8 @@ struct Batch {
   ReduceGroup* groups2 = nullptr;
   uint32_t* out2 = nullptr;
   PairJob* pairs = nullptr;
+  VarJob* agg_var = nullptr;          // products of the aggregated pairing checks
+  uint32_t n_agg_var = 0, n_agg_slots = 0, n_gpairs = 0;
   struct Level {
     ReduceGroup* groups;
     uint32_t* out;

@@ -107,8 +107,11 @@ struct TermFill {
 // e(points[a], PK) * e(points[b], -G2) == 1, and gate[item] must be set.
 // e(points[a], Q_a) * e(+-points[b], Q_b) == 1 for item `item`: Q_a is the
 // cached G2 point pk_slot (prepared lines), Q_b the cached G2 point pk_b or
-// -G2 when pk_b == PAIR_NEG_G2; flags bit0 negates points[b].
+// -G2 when pk_b == PAIR_NEG_G2; flags bit0 negates points[b].  flags bit1:
+// the check belongs to an aggregated group whose combined check wrote
+// valid[group]; when that is set the item is accepted without its own pairing.
 constexpr uint32_t PAIR_NEG_G2 = 0xffffffffu;
+constexpr uint32_t PAIR_NEG_B = 1, PAIR_GROUPED = 2;
 struct PairJob {
   uint32_t item;
   uint32_t a;
@@ -116,6 +119,7 @@ struct PairJob {
   uint32_t pk_slot;
   uint32_t pk_b;
   uint32_t flags;
+  uint32_t group;
 };
 
 // Sign: e = H2S(sk || m_1..m_L || domain); A = B / (sk + e) computed as the

@@ -296,7 +296,7 @@ __global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ j
   const p12::Lane L = p12::lane_setup(lds_all, lane);
   const uint32_t job = blockIdx.x * p12::CHECKS_PER_BLOCK + grp;
   const bool mine = grp < p12::CHECKS_PER_BLOCK && job < n;
-  bool live = mine;
+  bool live = mine, accepted = false;
   PairJob j{};
   uint8_t pks = 1, pkb = 1;
   if (mine) {
@@ -304,12 +304,18 @@ __global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ j
     pks = pk_status[j.pk_slot];
     pkb = j.pk_b == PAIR_NEG_G2 ? 1 : pk_status[j.pk_b];
     live = status[j.item] == 0 && pks != 0 && pkb != 0 && gate[j.item];
+    // the group's combined check (an earlier launch) passed: the item's own
+    // equation holds except with probability 2^-128
+    if (live && (j.flags & PAIR_GROUPED) && valid[j.group]) {
+      accepted = true;
+      live = false;
+    }
   }
   // blocks with no live check leave together (uniform exit, no barrier skipped)
   if (!__syncthreads_or(live)) {
     if (mine && lane % 12 == 0) {
       if (pks == 0 || pkb == 0) status[j.item] = 1;
-      valid[j.item] = 0;
+      valid[j.item] = accepted ? 1 : 0;
     }
     return;
   }
@@ -323,7 +329,7 @@ __global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ j
   if (live) {
     fp_load_const(yp0, points + (size_t)j.a * G1A_WORDS + FPW);
     fp_load_const(yp1, points + (size_t)j.b * G1A_WORDS + FPW);
-    if (j.flags & 1) fp_neg(yp1, yp1);
+    if (j.flags & PAIR_NEG_B) fp_neg(yp1, yp1);
     skip0 = pinf[j.a] || pks == 2;
     skip1 = pinf[j.b] || pkb == 2;
     lines0 = pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS;
@@ -336,7 +342,7 @@ __global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ j
   const bool one = p12::is_one(r, L);
   if (mine && lane % 12 == 0) {
     if (pks == 0 || pkb == 0) status[j.item] = 1;
-    valid[j.item] = (live && one) ? 1 : 0;
+    valid[j.item] = (accepted || (live && one)) ? 1 : 0;
   }
 }
 void launch_pairing(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,

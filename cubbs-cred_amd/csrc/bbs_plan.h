@@ -40,6 +40,11 @@ struct Plan {
   std::vector<PvChallengeJob> pvch;
   std::vector<PgJob> pg;
   std::vector<uint32_t> disc_index, disc_scalar, und_scalar;
+  // aggregated pairing checks: products c_i a_i, c_i b_i of the grouped
+  // checks (summed per group by groups2 / out2), extra status slots for the
+  // groups, and how many of pairs[] (at its front) are group checks
+  std::vector<VarJob> agg_var;
+  uint32_t n_agg_slots = 0, n_group_pairs = 0;
   std::vector<PipTerm> pip_terms;  // terms of the Pippenger MSMs
   std::vector<uint32_t> pip_out;   // per Pippenger MSM: its output partial slot
   std::vector<uint8_t> blob;       // signatures, proofs, headers, messages, keys, seeds
@@ -102,6 +107,11 @@ struct Plan {
 // Copy the plan into one pinned buffer + device mirror and allocate the work
 // area (from the thread's Arena when one is set).
 void materialize(Batch& b, Plan& p);
+// Group the plain pairing checks e(a_i, PK) e(b_i, -G2) = 1 of p.pairs by key
+// into groups of at most group_max (status slots from nv on) and add their
+// random-linear-combination checks; see bbs_batch.cpp.
+void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max);
+uint32_t agg_group_max();
 // Lock the context, make generators [0, max_gen_slots) and the given public
 // keys resident, snapshot the resident pointers into the batch.
 void attach(Batch& b, uint32_t max_gen_slots, const std::vector<const uint8_t*>& pks,
