@@ -36,10 +36,12 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cubbs-cred_amd"))
-# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default); the
-# serving loop keeps up to 3 slots x 8 streams busy, so give the process 16
-# (read by the HIP runtime at initialisation, before any device call below).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default), read
+# once at initialisation.  The library keeps one work stream per queue.  The
+# staged Verify/ProofVerify batches need 2 each (4 queues suffice, and measure
+# best); the serving loop's slots take 8 each, so the serving measurement runs
+# in a child process with SERVING_HW_QUEUES (2 slots).
+SERVING_HW_QUEUES = "16"
 
 N_VERIFY = 8192
 N_ISSUERS = 256
@@ -214,7 +216,7 @@ def cpu_baseline(vreqs, pvreqs, budget_s=20.0):
                       "%d threads)" % (nv, len(sample) - nv, threads)}
 
 
-def serving_run(g, rate=20000.0, warm_s=3.0, steady_s=15.0, seed=b"cubbs-hip/serving"):
+def serving_run(g, rate=20000.0, warm_s=3.0, steady_s=15.0, seed=b"cubbs-hip/serving", slots=0):
     """Open-loop serving load through the library's continuous-batching server:
     Poisson arrivals at `rate` req/s, mix 40% Verify / 30% ProofVerify / 20% Sign
     / 10% ProofGen, 64 issuer keys, message count log-uniform in [1, 128] with
@@ -268,7 +270,7 @@ def serving_run(g, rate=20000.0, warm_s=3.0, steady_s=15.0, seed=b"cubbs-hip/ser
         arrivals.append(int(t * 1e9))
         k = rng.choices(kinds, weights)[0]
         order.append(starts[k] + rng.randrange(n_pool[k]))
-    srv = g.Server()
+    srv = g.Server(slots=slots)
     try:
         # warm the server (generator tables up to L = 128, issuer keys resident)
         srv.run_open_loop(pool, list(range(0, len(pool), 7)), [0] * len(range(0, len(pool), 7)))
@@ -290,7 +292,23 @@ def serving_run(g, rate=20000.0, warm_s=3.0, steady_s=15.0, seed=b"cubbs-hip/ser
             "p50_ms": round(pct(0.50), 2), "p99_ms": round(pct(0.99), 2), "max_ms": round(steady[-1] / 1e6, 2),
             "achieved_rps": round(n / wall, 1), "verify_proofverify_rps": round(vps / steady_s, 1),
             "batches": st["batches"], "avg_batch": round(st["requests"] / max(1, st["batches"]), 1),
+            "host_plan_ms_per_batch": st["host_plan_ms_per_batch"], "batch_ms_per_batch": st["batch_ms_per_batch"],
+            "host_cpus": len(os.sched_getaffinity(0)),
             "max_concurrent_batches": st["max_concurrent_batches"]}
+
+
+def serving_child(args):
+    """The serving measurement in a child process (its own HIP runtime with
+    SERVING_HW_QUEUES hardware queues); returns its latency object."""
+    import subprocess
+    env = dict(os.environ, GPU_MAX_HW_QUEUES=SERVING_HW_QUEUES)
+    cmd = [sys.executable, os.path.abspath(__file__), "--serving-only",
+           "--serving-rate", str(args.serving_rate), "--serving-seconds", str(args.serving_seconds),
+           "--serving-slots", str(args.serving_slots)]
+    p = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, check=True)
+    lat = json.loads(p.stdout.decode().strip().splitlines()[-1])
+    lat["hw_queues"] = int(SERVING_HW_QUEUES)
+    return lat
 
 
 def per_request_gpu(g, vreqs, n=16):
@@ -329,6 +347,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency run")
     ap.add_argument("--serving-rate", type=float, default=20000.0)
     ap.add_argument("--serving-seconds", type=float, default=15.0)
+    ap.add_argument("--serving-slots", type=int, default=0, help="server slots (0 = library default)")
+    ap.add_argument("--serving-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--profile-only", action="store_true", help="a few launches, no baselines")
     ap.add_argument("--no-msm", action="store_true", help="skip the batched-MSM measurement")
     ap.add_argument("--no-graph", action="store_true", help="launch kernel by kernel instead of HIP graphs")
@@ -345,6 +365,11 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": n}))
+        return
+    if args.serving_only:
+        import cubbs_hip as g
+        lat = serving_run(g, rate=args.serving_rate, steady_s=args.serving_seconds, slots=args.serving_slots)
+        print(json.dumps(lat), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -437,7 +462,7 @@ def main():
         out["per_request_gpu_ops_s"] = round(per_request_gpu(g, vreqs), 2)
         if not args.no_serving:
             t = time.time()
-            out["latency"] = serving_run(g, rate=args.serving_rate, steady_s=args.serving_seconds)
+            out["latency"] = serving_child(args)
             log("serving run %.1fs: %s" % (time.time() - t, out["latency"]))
         if not args.no_cpu_baseline:
             out["cpu_baseline"] = cpu_baseline(vreqs, pvreqs)

@@ -68,7 +68,7 @@ void init_locked(Context& c) {
   {
     int q = 4;
     if (const char* e = std::getenv("GPU_MAX_HW_QUEUES")) q = std::atoi(e);
-    q = std::max(2, std::min(q, 16));
+    q = std::max(2, std::min(q, 32));
     c.pool.resize(q);
     for (auto& st : c.pool) check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   }

@@ -262,7 +262,13 @@ void materialize(Batch& b, Plan& p) {
     check(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
     check(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
   }
-  check(hipMemcpyAsync(b.d_in, b.h_in, b.in_bytes, hipMemcpyHostToDevice, b.stream));
+  b.res_stride = align256(std::max<uint32_t>(slots, 1));
+  const size_t res_bytes = 2 * b.res_stride + align256(std::max<size_t>(p.out_bytes, 1));
+  if (ar)
+    b.h_res = ar->alloc_host(res_bytes);
+  else
+    check(hipHostMalloc(&b.h_res, res_bytes, hipHostMallocDefault));
+  launch_copy(b.stream, b.d_in, b.h_in, b.in_bytes);
 }
 
 // ------------------------------------------------------------------ aggregated pairing checks
@@ -1006,6 +1012,14 @@ void launch_batch(Batch& b) {
   check(hipGetLastError());
 }
 
+void fetch_results(Batch& b) {
+  launch_copy(b.stream, b.h_res, b.status, b.nv);
+  launch_copy(b.stream, b.h_res + b.res_stride, b.valid, b.nv);
+  launch_copy(b.stream, b.h_res + 2 * b.res_stride, b.out, b.out_bytes);
+  check(hipGetLastError());
+  check(hipStreamSynchronize(b.stream));
+}
+
 // Copy results back and fill the caller's result array.  For Sign and
 // ProofGen, outputs are written into the items' buffers (out_ptrs).
 void collect_batch(Batch& b, CuBbsBbsGpuResult* results, uint8_t* const* out_ptrs,
@@ -1014,13 +1028,10 @@ void collect_batch(Batch& b, CuBbsBbsGpuResult* results, uint8_t* const* out_ptr
   b.h_valid.resize(b.nv);
   std::vector<uint8_t> hout;
   if (b.nv) {
-    check(hipMemcpyAsync(b.h_status.data(), b.status, b.nv, hipMemcpyDeviceToHost, b.stream));
-    check(hipMemcpyAsync(b.h_valid.data(), b.valid, b.nv, hipMemcpyDeviceToHost, b.stream));
-    if (b.out_bytes) {
-      hout.resize(b.out_bytes);
-      check(hipMemcpyAsync(hout.data(), b.out, b.out_bytes, hipMemcpyDeviceToHost, b.stream));
-    }
-    check(hipStreamSynchronize(b.stream));
+    fetch_results(b);
+    std::memcpy(b.h_status.data(), b.h_res, b.nv);
+    std::memcpy(b.h_valid.data(), b.h_res + b.res_stride, b.nv);
+    if (b.out_bytes) hout.assign(b.h_res + 2 * b.res_stride, b.h_res + 2 * b.res_stride + b.out_bytes);
   }
   for (size_t i = 0; i < b.n_total; i++) {
     results[i].bytes_written = 0;
@@ -1098,6 +1109,7 @@ Batch::~Batch() {
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
   if (h_in) (void)hipHostFree(h_in);
+  if (h_res) (void)hipHostFree(h_res);
   if (d_in) (void)hipFree(d_in);
   if (d_work) (void)hipFree(d_work);  // streams belong to the context's pool
 }

@@ -80,6 +80,10 @@ struct Batch {
   size_t in_bytes = 0, work_bytes = 0, out_bytes = 0;
   uint8_t* h_in = nullptr;
   uint8_t* d_in = nullptr;
+  // pinned staging of the results: status, valid (each align256(slots)) and
+  // the output area, written by a copy kernel and read after the stream sync
+  uint8_t* h_res = nullptr;
+  size_t res_stride = 0;
   uint8_t* d_work = nullptr;
   // job arrays (device)
   H2sJob* h2s = nullptr;
@@ -171,6 +175,8 @@ std::unique_ptr<Batch> stage_primitive(int kind, const void* items, size_t n, in
 // One result per output row; encodings kept in b.h_out (48 bytes per row).
 void collect_primitive(Batch& b, CuBbsBbsGpuResult* results);
 void launch_batch(Batch& b);
+// Copy status, valid and the output area into b.h_res and wait for the stream.
+void fetch_results(Batch& b);
 // Capture launch_batch() into b.graph (on private streams, so concurrent users
 // of the pool streams are never recorded); b.stream then replays it.
 void capture_batch(Batch& b);

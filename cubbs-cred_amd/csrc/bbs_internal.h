@@ -166,6 +166,8 @@ struct PgJob {
 // ------------------------------------------------------------------ launchers
 // (all asynchronous on the given stream)
 void launch_fill_status(hipStream_t s, uint32_t n, uint8_t* status, uint8_t* gate, uint8_t gate_value);
+// dst <- src, bytes; either end may be pinned host memory
+void launch_copy(hipStream_t s, void* dst, const void* src, size_t bytes);
 void launch_h2s(hipStream_t s, const H2sJob* jobs, uint32_t n, const uint8_t* blob,
                 const uint8_t* dst, uint32_t dst_len, uint32_t* out_scalars);
 void launch_hash_jobs(hipStream_t s, int kind, const H2sJob* jobs, uint32_t n, const uint8_t* blob,

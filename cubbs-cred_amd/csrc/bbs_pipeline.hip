@@ -29,6 +29,28 @@ void launch_fill_status(hipStream_t s, uint32_t n, uint8_t* status, uint8_t* gat
   hipLaunchKernelGGL(k_fill_status, GRID(n, 256), 0, s, n, status, gate, gv);
 }
 
+// ------------------------------------------------------------------ transfers
+// Copies between pinned host memory and device memory by a kernel on the
+// batch's stream (the device reads and writes host memory directly), so the
+// host thread only enqueues a launch: 16-byte accesses when both ends allow.
+__global__ __launch_bounds__(256) void k_copy(uint8_t* __restrict__ dst,
+                                             const uint8_t* __restrict__ src, size_t n) {
+  const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
+  if (i >= n) return;
+  if (i + 16 <= n && ((((uintptr_t)dst | (uintptr_t)src) & 15) == 0)) {
+    *reinterpret_cast<uint4*>(dst + i) = *reinterpret_cast<const uint4*>(src + i);
+  } else {
+    const size_t e = i + 16 < n ? i + 16 : n;
+    for (size_t k = i; k < e; k++) dst[k] = src[k];
+  }
+}
+void launch_copy(hipStream_t s, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  const size_t lanes = (bytes + 15) / 16;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s,
+                     static_cast<uint8_t*>(dst), static_cast<const uint8_t*>(src), bytes);
+}
+
 // ------------------------------------------------------------------ decoding
 // check_subgroup = 0 leaves the prime-order subgroup check to the lanes that
 // k_var_mul appends for these jobs.

@@ -124,15 +124,11 @@ struct Ran {
   std::vector<uint8_t> status, valid, out;
 };
 Ran fetch(Batch& b) {
+  fetch_results(b);
   Ran r;
-  r.status.resize(b.nv);
-  r.valid.resize(b.nv);
-  r.out.resize(std::max<size_t>(b.out_bytes, 1));
-  check(hipMemcpyAsync(r.status.data(), b.status, b.nv, hipMemcpyDeviceToHost, b.stream));
-  check(hipMemcpyAsync(r.valid.data(), b.valid, b.nv, hipMemcpyDeviceToHost, b.stream));
-  if (b.out_bytes)
-    check(hipMemcpyAsync(r.out.data(), b.out, b.out_bytes, hipMemcpyDeviceToHost, b.stream));
-  check(hipStreamSynchronize(b.stream));
+  r.status.assign(b.h_res, b.h_res + b.nv);
+  r.valid.assign(b.h_res + b.res_stride, b.h_res + b.res_stride + b.nv);
+  r.out.assign(b.h_res + 2 * b.res_stride, b.h_res + 2 * b.res_stride + std::max<size_t>(b.out_bytes, 1));
   return r;
 }
 

@@ -202,6 +202,10 @@ struct Server {
   std::vector<std::thread> workers;
   // statistics
   std::atomic<uint64_t> batches{0}, requests{0}, on_device{0}, max_on_device{0}, peak_bytes{0};
+  // summed over batches: host time planning + packing the sub-batches, and
+  // the whole batch from the first plan to the last result
+  std::atomic<uint64_t> plan_ns{0}, batch_ns{0};
+
   std::atomic<int32_t> fault{0};
 
   void worker(unsigned slot);
@@ -223,23 +227,32 @@ void Server::run_batch(std::vector<Req*>& take, Arena& ar) {
   }
   // each kind is planned onto its own stream of the slot and launched as soon
   // as it is planned, so the host packs the next kind while the device runs
+  const uint64_t t0 = now_ns();
+  uint64_t t_plan = 0;
+  auto planned = [&](uint64_t ts) { t_plan += now_ns() - ts; };
   set_thread_arena(&ar);
   if (!by[CUBBS_HIP_REQ_VERIFY].empty()) {
     for (Req* r : by[CUBBS_HIP_REQ_VERIFY]) vi.push_back(r->v);
     ar.cur = 0;
+    const uint64_t ts = now_ns();
     bs[0] = plan_verify(vi.data(), vi.size());
+    planned(ts);
     launch_batch(*bs[0]);
   }
   if (!by[CUBBS_HIP_REQ_PROOF_VERIFY].empty()) {
     for (Req* r : by[CUBBS_HIP_REQ_PROOF_VERIFY]) pvi.push_back(r->pv);
     ar.cur = 1;
+    const uint64_t ts = now_ns();
     bs[1] = plan_proof_verify(pvi.data(), pvi.size());
+    planned(ts);
     launch_batch(*bs[1]);
   }
   if (!by[CUBBS_HIP_REQ_SIGN].empty()) {
     for (Req* r : by[CUBBS_HIP_REQ_SIGN]) si.push_back(r->s);
     ar.cur = 2;
+    const uint64_t ts = now_ns();
     bs[2] = plan_sign(si.data(), si.size());
+    planned(ts);
     launch_batch(*bs[2]);
   }
   if (!by[CUBBS_HIP_REQ_PROOF_GEN].empty()) {
@@ -248,7 +261,9 @@ void Server::run_batch(std::vector<Req*>& take, Arena& ar) {
       seeds.insert(seeds.end(), r->seed, r->seed + 32);
     }
     ar.cur = 3;
+    const uint64_t ts = now_ns();
     bs[3] = plan_proof_gen(pgi.data(), pgi.size(), 1, seeds.data(), nullptr, nullptr);
+    planned(ts);
     launch_batch(*bs[3]);
   }
   set_thread_arena(nullptr);
@@ -267,6 +282,8 @@ void Server::run_batch(std::vector<Req*>& take, Arena& ar) {
   --on_device;
   bs.clear();
   ar.reset();
+  plan_ns += t_plan;
+  batch_ns += now_ns() - t0;
 }
 
 void Server::worker(unsigned slot) {
@@ -385,7 +402,16 @@ void* cubbs_hip_server_create(const CubbsHipServerConfig* cfg, int32_t* status_o
       cubbs::init_locked(c);
     }
     auto* s = new Server();
-    CubbsHipServerConfig d{3, 4096, 4ull << 30};
+    // default slots: as many as the stream pool serves without two slots
+    // sharing a stream (a slot uses 8), since a launch on a stream that is
+    // busy with another slot's work can stall the host thread
+    uint32_t pool = 0;
+    {
+      cubbs::Context& c = cubbs::ctx();
+      std::lock_guard<std::mutex> g(c.mu);
+      pool = (uint32_t)c.pool.size();
+    }
+    CubbsHipServerConfig d{std::max<uint32_t>(1, pool / 8), 4096, 4ull << 30};
     if (cfg) {
       if (cfg->slots) d.slots = cfg->slots;
       if (cfg->max_batch) d.max_batch = cfg->max_batch;
@@ -458,6 +484,15 @@ int32_t cubbs_hip_server_stats(void* srv, uint64_t* batches, uint64_t* requests,
   if (requests) *requests = s->requests;
   if (max_concurrent_batches) *max_concurrent_batches = s->max_on_device;
   if (peak_slot_bytes) *peak_slot_bytes = s->peak_bytes;
+  return s->fault;
+}
+
+int32_t cubbs_hip_server_timing(void* srv, double* plan_ms_per_batch, double* batch_ms_per_batch) {
+  if (!srv) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  auto* s = static_cast<Server*>(srv);
+  const double n = std::max<double>(1.0, (double)s->batches.load());
+  if (plan_ms_per_batch) *plan_ms_per_batch = (double)s->plan_ns.load() / n * 1e-6;
+  if (batch_ms_per_batch) *batch_ms_per_batch = (double)s->batch_ns.load() / n * 1e-6;
   return s->fault;
 }
 

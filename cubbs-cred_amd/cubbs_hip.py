@@ -56,6 +56,7 @@ EXT_SYMBOLS = [
     "cubbs_hip_batch_free",
     "cubbs_hip_batch_outputs",
     "cubbs_hip_batch_use_graph",
+    "cubbs_hip_server_timing",
     "cubbs_hip_autotune",
     "cubbs_hip_tune_load",
     "cubbs_hip_batch_profile",
@@ -721,6 +722,11 @@ class Server:
         d = dict(zip(["batches", "requests", "max_concurrent_batches", "peak_slot_bytes"],
                      [x.value for x in v]))
         d["fault"] = rc
+        pm, bm = ctypes.c_double(), ctypes.c_double()
+        lib().cubbs_hip_server_timing.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib().cubbs_hip_server_timing(self.h, ctypes.byref(pm), ctypes.byref(bm))
+        d["host_plan_ms_per_batch"] = round(pm.value, 3)
+        d["batch_ms_per_batch"] = round(bm.value, 3)
         return d
 
     def close(self):

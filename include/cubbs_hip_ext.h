@@ -79,7 +79,8 @@ int32_t cubbs_hip_batch_shape(void *batch, uint64_t *items, uint64_t *msm_terms,
  * memory; a free slot takes every pending request up to max_batch and its
  * share of the HBM budget (hbm_budget_bytes / slots), runs one sub-batch per
  * operation kind and completes them, while other slots' batches run on the
- * device.  Zero fields take the defaults (3 slots, 4096, 4 GiB). */
+ * device.  Zero fields take the defaults (slots: the library's work streams
+ * / 8, i.e. 2 with GPU_MAX_HW_QUEUES=16; 4096; 4 GiB). */
 typedef struct CubbsHipServerConfig {
   uint32_t slots;
   uint32_t max_batch;
@@ -120,6 +121,9 @@ int32_t cubbs_hip_server_run_open_loop(void *server, const CubbsHipRequest *requ
 /* Counters since creation; returns the server's fault code (0 = healthy). */
 int32_t cubbs_hip_server_stats(void *server, uint64_t *batches, uint64_t *requests,
                                uint64_t *max_concurrent_batches, uint64_t *peak_slot_bytes);
+/* Averages per batch since creation: host time planning and packing the
+ * sub-batches, and the whole batch from its first plan to its last result. */
+int32_t cubbs_hip_server_timing(void *server, double *plan_ms_per_batch, double *batch_ms_per_batch);
 /* Drains queued requests, joins the workers and frees the server. */
 void cubbs_hip_server_destroy(void *server);
 
