@@ -357,30 +357,94 @@ DEVN void frob2(Fp& f, Lane L) {
   fp_mul(f, f, g);
 }
 
-// Inverse: lane (0, 0) gathers the element, inverts it in the tower, scatters.
+// Inverse through the norm to Fp6: with g = conj(f) (odd slots negated),
+// N = f g = a^2 - b^2 v lies in Fp6 (the even slots 0, 2, 4 hold n0, n1, n2
+// of n0 + n1 v + n2 v^2, the odd slots are 0), and f^-1 = g N^-1.  With
+// v^3 = xi, N^-1 = (t0 + t1 v + t2 v^2) / d for t0 = n0^2 - xi n1 n2,
+// t1 = xi n2^2 - n0 n1, t2 = n1^2 - n0 n2 (lanes (k, 0), k < 3) and
+// d = n0 t0 + xi (n2 t1 + n1 t2), d^-1 = conj(d) / (d0^2 + d1^2) (lane (0, 0),
+// one Fp inversion).  Two 12-lane products and a few Fp2 operations, with
+// values exchanged through the check's LDS area.
 DEVN void inv(Fp& f, Lane L) {
-  lds_t* A = L.F;
+  Fp g = f;
+  conj(g, L);
+  Fp n = f;
+  mul_in(n, g, L);  // N
+  lds_t* X = L.X;   // N's coefficients: slot s at X + 2 s E (c0), + E (c1)
+  lds_t* T = L.F;   // t0, t1, t2, d^-1: Fp2 j at T + 2 j E
+  auto ld2 = [&](Fp2& r, const lds_t* q) {
+    ld(r.c0, q);
+    ld(r.c1, q + E);
+  };
+  auto st2 = [&](lds_t* q, const Fp2& v) {
+    st(q, v.c0);
+    st(q + E, v.c1);
+  };
   __syncthreads();
-  if (L.wr) st(A + L.k * SLOT + (L.c ? EY1 : EY0) * E, f);
+  if (L.wr) st(X + (2 * L.k + L.c) * E, n);
   __syncthreads();
-  if (L.k == 0 && L.c == 0 && L.wr) {
-    Fp12 x, y;
-    Fp2* xs[6] = {&x.c0.c0, &x.c1.c0, &x.c0.c1, &x.c1.c1, &x.c0.c2, &x.c1.c2};
-    Fp2* ys[6] = {&y.c0.c0, &y.c1.c0, &y.c0.c1, &y.c1.c1, &y.c0.c2, &y.c1.c2};
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-      ld(xs[k]->c0, A + k * SLOT + EY0 * E);
-      ld(xs[k]->c1, A + k * SLOT + EY1 * E);
-    }
-    fp12_inv(y, x);
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-      st(A + k * SLOT + EY0 * E, ys[k]->c0);
-      st(A + k * SLOT + EY1 * E, ys[k]->c1);
-    }
+  if (L.c == 0 && L.k < 3 && L.wr) {
+    // t_k = sq - pr: sq = n_k^2 (times xi for k = 1), pr = the cross product
+    // n1 n2 (times xi for k = 0), n0 n1 (k = 1), n0 n2 (k = 2)
+    const int sq_slot = L.k == 0 ? 0 : (L.k == 1 ? 4 : 2);
+    const int pa = L.k == 0 ? 2 : 0, pb = L.k == 1 ? 2 : 4;
+    Fp2 a, b, x, y;
+    ld2(a, X + 2 * sq_slot * E);
+    fp2_sqr(x, a);
+    if (L.k == 1) fp2_mul_xi(x, x);
+    ld2(a, X + 2 * pa * E);
+    ld2(b, X + 2 * pb * E);
+    fp2_mul(y, a, b);
+    if (L.k == 0) fp2_mul_xi(y, y);
+    fp2_sub(x, x, y);
+    st2(T + 2 * L.k * E, x);
   }
   __syncthreads();
-  ld(f, A + L.k * SLOT + (L.c ? EY1 : EY0) * E);
+  if (L.k == 0 && L.c == 0 && L.wr) {
+    Fp2 a, b, d, s;
+    ld2(a, X + 2 * 2 * E);  // n1
+    ld2(b, T + 2 * 2 * E);  // t2
+    fp2_mul(s, a, b);
+    ld2(a, X + 2 * 4 * E);  // n2
+    ld2(b, T + 2 * 1 * E);  // t1
+    fp2_mul(d, a, b);
+    fp2_add(s, s, d);
+    fp2_mul_xi(s, s);
+    ld2(a, X);              // n0
+    ld2(b, T);              // t0
+    fp2_mul(d, a, b);
+    fp2_add(d, d, s);
+    Fp nrm, t;
+    fp_sqr(nrm, d.c0);
+    fp_sqr(t, d.c1);
+    fp_add(nrm, nrm, t);
+    fp_inv(nrm, nrm);
+    fp_mul(d.c0, d.c0, nrm);
+    fp_mul(d.c1, d.c1, nrm);
+    fp_neg(d.c1, d.c1);
+    st2(T + 2 * 3 * E, d);
+  }
+  __syncthreads();
+  // this lane's coefficient of N^-1: component c of t_{k/2} d^-1 for even k
+  Fp h;
+  {
+    Fp t0, t1, d0, d1, p0, p1;
+    const int j = (L.k >> 1) % 3;
+    ld(t0, T + 2 * j * E);
+    ld(t1, T + 2 * j * E + E);
+    ld(d0, T + 2 * 3 * E);
+    ld(d1, T + 2 * 3 * E + E);
+    // c = 0: t0 d0 - t1 d1, c = 1: t0 d1 + t1 d0
+    fp_mul(p0, t0, L.c ? d1 : d0);
+    fp_mul(p1, t1, L.c ? d0 : d1);
+    Fp q;
+    fp_sub(q, p0, p1);
+    fp_add(h, p0, p1);
+    h = L.c ? h : q;
+    if (L.k & 1) fp_zero(h);
+  }
+  mul_in(g, h, L);
+  f = g;
 }
 
 // a^x for a cyclotomic element, x = -|x|
