@@ -87,6 +87,38 @@ hipStream_t next_stream_locked(Context& c) {
   return s;
 }
 
+// Wide comb tables for slots [0, min(slots, wide_cap)), after their 8-bit
+// tables exist.  The capacity (CUBBS_HIP_COMB16_GENERATORS, default 130 =
+// P1, Q1, H_1..H_128; 0 = none) is allocated at the first call and never
+// reallocated, so batches that snapshot the pointer stay valid; when the
+// allocation fails the library keeps to the 8-bit tables.  ctx().mu held.
+static void ensure_wide_tables_locked(Context& c, uint32_t slots) {
+  if (!c.wide_init) {
+    c.wide_init = true;
+    const char* v = std::getenv("CUBBS_HIP_COMB16_GENERATORS");
+    c.wide_cap = v && *v ? (uint32_t)std::strtoul(v, nullptr, 10) : COMB16_DEFAULT_SLOTS;
+    if (c.wide_cap && hipMalloc(&c.gen_tables16, (size_t)c.wide_cap * COMB16_TABLE_WORDS * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      c.gen_tables16 = nullptr;
+      c.wide_cap = 0;
+    }
+  }
+  const uint32_t upto = std::min(slots, c.wide_cap);
+  if (upto <= c.wide_slots) return;
+  const uint32_t slice = 32;  // generators per build launch (1.9 GB of scratch)
+  uint32_t* scratch = nullptr;
+  const uint32_t k0 = std::min(slice, upto - c.wide_slots);
+  check(hipMalloc(&scratch, (size_t)k0 * COMB16_WINDOWS * COMB16_ENTRIES * COMB_ENTRY_WORDS * 4));
+  for (uint32_t s = c.wide_slots; s < upto; s += slice) {
+    const uint32_t k = std::min(slice, upto - s);
+    launch_comb16_build(c.setup, s, k, c.gen_tables, c.gen_tables16, scratch);
+    check(hipGetLastError());
+  }
+  check(hipStreamSynchronize(c.setup));
+  check(hipFree(scratch));
+  c.wide_slots = upto;
+}
+
 // Ensure generator slots [0, slots) exist (tables included).  ctx().mu held.
 void ensure_generators_locked(Context& c, uint32_t slots) {
   if (slots <= c.gen_slots) return;
@@ -140,6 +172,7 @@ void ensure_generators_locked(Context& c, uint32_t slots) {
   }
   check(hipStreamSynchronize(c.setup));
   check(hipFree(scratch));
+  ensure_wide_tables_locked(c, slots);
   // host mirror of the encodings: recognises generator points in the
   // primitive MSM entries (their comb tables are resident)
   c.gen_comp_host.resize((size_t)slots * 48);
@@ -434,7 +467,7 @@ int32_t cubbs_hip_batch_shape(void* batch, uint64_t* items, uint64_t* msm_terms,
   Batch* b = static_cast<Batch*>(batch);
   if (items) *items = b->nv;
   if (msm_terms) *msm_terms = b->comb_terms;
-  if (comb_madds) *comb_madds = 32ull * b->comb_terms;
+  if (comb_madds) *comb_madds = b->comb_madds;
   if (pairings) *pairings = 2ull * b->n_pairs;
   return CUBBS_BBS_GPU_SUCCESS;
 }

@@ -194,7 +194,11 @@ void materialize(Batch& b, Plan& p) {
   b.blob = D + o_blob;
   b.seeded = p.seeded;
   b.seeds = D + o_blob + p.seeds_off;
-  for (const auto& c : p.chunks) b.comb_terms += c.term_count;
+  for (const auto& c : p.chunks) {
+    b.comb_terms += c.term_count;
+    for (uint32_t t = c.term_begin; t < c.term_begin + c.term_count; t++)
+      b.comb_madds += p.term_gen[t] < b.n16 ? COMB16_WINDOWS : COMB_WINDOWS;
+  }
   // work area
   size_t w = 0;
   auto take = [&](size_t bytes) {
@@ -375,6 +379,8 @@ void attach(Batch& b, uint32_t max_gen_slots, const std::vector<const uint8_t*>&
   ensure_generators_locked(c, std::max<uint32_t>(max_gen_slots, 2));
   lookup_pks_locked(c, pks, pk_slot);
   b.tables = c.gen_tables;
+  b.tables16 = c.gen_tables16;
+  b.n16 = c.wide_slots;
   b.gen_comp = c.gen_comp;
   b.pk_bytes = c.pk_bytes;
   b.pk_status = c.pk_status;
@@ -916,7 +922,8 @@ void launch_batch(Batch& b) {
   }
   if (b.n_chunks) {
     Timed t(b, "comb_msm");
-    launch_comb_msm(s, b.chunks, b.n_chunks, b.term_gen, b.scalars, b.tables, b.partials);
+    launch_comb_msm(s, b.chunks, b.n_chunks, b.term_gen, b.scalars, b.tables, b.tables16, b.n16,
+                    b.partials);
   }
   if (fork) check(hipStreamWaitEvent(s, b.ev_join, 0));
   if (b.pip.terms) {

@@ -129,6 +129,7 @@ struct Batch {
            n_terms = 0, n_groups1 = 0, n_groups2 = 0, n_pairs = 0, n_sign = 0, n_pvch = 0,
            n_pg = 0;
   uint64_t comb_terms = 0;
+  uint64_t comb_madds = 0;            // mixed additions of the comb launch (at most)
   // work arrays (device)
   uint32_t* scalars = nullptr;
   uint32_t* points = nullptr;
@@ -142,6 +143,8 @@ struct Batch {
   uint32_t n_scalars = 0, n_points = 0, n_partials = 0;
   // snapshot of resident state
   const uint32_t* tables = nullptr;
+  const uint32_t* tables16 = nullptr;
+  uint32_t n16 = 0;                   // generator slots with wide comb tables
   const uint8_t* gen_comp = nullptr;
   const uint8_t* pk_bytes = nullptr;
   const uint8_t* pk_status = nullptr;

@@ -21,6 +21,18 @@ constexpr int COMB_ENTRIES = 128;
 constexpr size_t COMB_ENTRY_WORDS = 28;  // affine x, y: 2 x 14 limbs of 28 bits (Montgomery), 112 bytes
 constexpr size_t COMB_TABLE_WORDS = (size_t)COMB_WINDOWS * COMB_ENTRIES * COMB_ENTRY_WORDS;
 
+// Wide fixed-base comb for the first generator slots: 16 windows of 16 bits,
+// signed digits in [-2^15, 2^15], affine entries d * 2^(16w) * G for
+// d = 1..2^15, each padded to 32 words so that an entry is one aligned
+// 128-byte line (64 MiB per generator).  Half the mixed additions of the
+// 8-bit comb per term; the table is read at one random line per window.
+constexpr int COMB16_WINDOWS = 16;
+constexpr int COMB16_ENTRIES = 32768;
+constexpr size_t COMB16_ENTRY_WORDS = 32;
+constexpr size_t COMB16_TABLE_WORDS = (size_t)COMB16_WINDOWS * COMB16_ENTRIES * COMB16_ENTRY_WORDS;
+constexpr uint32_t COMB16_RUN = 256;             // consecutive entries per lane of the build
+constexpr uint32_t COMB16_DEFAULT_SLOTS = 130;   // P1, Q1, H_1..H_128
+
 constexpr int MILLER_LINE_WORDS = 56;   // (lambda, c) in Fp2: 4 x 14 limbs, 224 bytes
 constexpr int MILLER_LINE_COUNT = 68;
 constexpr size_t PK_LINES_WORDS = (size_t)MILLER_LINE_WORDS * MILLER_LINE_COUNT;
@@ -185,8 +197,14 @@ void launch_gen_points(hipStream_t s, uint32_t first, uint32_t count, const uint
 // scratch: count * COMB_WINDOWS * COMB_ENTRIES * COMB_ENTRY_WORDS words
 void launch_comb_build(hipStream_t s, uint32_t first, uint32_t count, const uint32_t* gen_affine,
                        uint32_t* tables, uint32_t* scratch);
+// Wide tables of generators [first, first + count) from their 8-bit tables.
+// scratch: count * COMB16_WINDOWS * COMB16_ENTRIES * COMB_ENTRY_WORDS words
+void launch_comb16_build(hipStream_t s, uint32_t first, uint32_t count, const uint32_t* tables,
+                         uint32_t* tables16, uint32_t* scratch);
+// Terms whose generator slot is below n16 use the wide tables.
 void launch_comb_msm(hipStream_t s, const CombChunk* chunks, uint32_t n, const uint32_t* term_gen,
-                     const uint32_t* scalars, const uint32_t* tables, uint32_t* partials);
+                     const uint32_t* scalars, const uint32_t* tables, const uint32_t* tables16,
+                     uint32_t n16, uint32_t* partials);
 // groups[i] sums into points[out_point[i]]
 void launch_reduce(hipStream_t s, const ReduceGroup* groups, const uint32_t* out_point, uint32_t n,
                    const uint32_t* partials, uint32_t* points, uint8_t* pinf);

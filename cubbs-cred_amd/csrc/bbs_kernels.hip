@@ -25,7 +25,7 @@ __global__ __launch_bounds__(256) void k_h2s(const H2sJob* __restrict__ jobs, ui
 void launch_h2s(hipStream_t s, const H2sJob* jobs, uint32_t n, const uint8_t* blob,
                 const uint8_t* dst, uint32_t dst_len, uint32_t* out) {
   if (!n) return;
-  hipLaunchKernelGGL(k_h2s, GRID(n, 256), 0, s, jobs, n, blob, dst, dst_len, out);
+  hipLaunchKernelGGL(k_h2s, GRID(n, 64), 0, s, jobs, n, blob, dst, dst_len, out);
 }
 
 // Packed hash jobs with a caller DST in device memory: kind 0 = SHA-256
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_hash_jobs(int kind, const H2sJob* __res
 void launch_hash_jobs(hipStream_t s, int kind, const H2sJob* jobs, uint32_t n, const uint8_t* blob,
                       const uint8_t* dst, uint32_t dst_len, uint8_t* out) {
   if (!n) return;
-  hipLaunchKernelGGL(k_hash_jobs, GRID(n, 256), 0, s, kind, jobs, n, blob, dst, dst_len, out);
+  hipLaunchKernelGGL(k_hash_jobs, GRID(n, 64), 0, s, kind, jobs, n, blob, dst, dst_len, out);
 }
 
 // domain = hash_to_scalar(PK || I2OSP(L, 8) || Q1 || H_1..H_L || api_id ||
@@ -125,13 +125,13 @@ void launch_hash_stage(hipStream_t s, const DomainJob* dom, uint32_t n_dom, cons
                        const uint8_t* pk_bytes, const uint8_t* gen_comp, uint32_t* out) {
   const uint32_t n = n_dom + n_h2s;
   if (!n) return;
-  hipLaunchKernelGGL(k_hash_stage, GRID(n, 256), 0, s, dom, n_dom, h2s, n_h2s, blob, dst, dst_len,
+  hipLaunchKernelGGL(k_hash_stage, GRID(n, 64), 0, s, dom, n_dom, h2s, n_h2s, blob, dst, dst_len,
                      pk_bytes, gen_comp, out);
 }
 void launch_domain(hipStream_t s, const DomainJob* jobs, uint32_t n, const uint8_t* hdr_blob,
                    const uint8_t* pk_bytes, const uint8_t* gen_comp, uint32_t* out) {
   if (!n) return;
-  hipLaunchKernelGGL(k_domain, GRID(n, 256), 0, s, jobs, n, hdr_blob, pk_bytes, gen_comp, out);
+  hipLaunchKernelGGL(k_domain, GRID(n, 64), 0, s, jobs, n, hdr_blob, pk_bytes, gen_comp, out);
 }
 
 // ------------------------------------------------------------------ generators
@@ -273,64 +273,192 @@ void launch_comb_build(hipStream_t s, uint32_t first, uint32_t count, const uint
   hipLaunchKernelGGL(k_comb_build, GRID(n, 64), 0, s, first, count, gen_affine, tables, scratch);
 }
 
+// Wide comb tables: one lane per (generator, window, run of COMB16_RUN
+// entries).  base = 2^(16w) G is entry d = 1 of window 2w of the generator's
+// 8-bit table; the run starts at d0 * base (short double-and-add), continues
+// by mixed additions in Jacobian form and is normalised to affine with one
+// inversion (Montgomery's batch trick).  `scratch` holds z_d and the running
+// product, 28 words per entry and lane.
+__global__ __launch_bounds__(64) void k_comb16_build(uint32_t first, uint32_t count,
+                                                    const uint32_t* __restrict__ tables,
+                                                    uint32_t* __restrict__ tables16,
+                                                    uint32_t* __restrict__ scratch) {
+  constexpr uint32_t RUNS = COMB16_ENTRIES / COMB16_RUN;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count * COMB16_WINDOWS * RUNS) return;
+  const uint32_t g = first + t / (COMB16_WINDOWS * RUNS);
+  const uint32_t w = (t / RUNS) % COMB16_WINDOWS;
+  const uint32_t d0 = (t % RUNS) * COMB16_RUN + 1;
+  G1A base;
+  load_g1a(base, tables + ((size_t)g * COMB_WINDOWS + 2 * w) * COMB_ENTRIES * COMB_ENTRY_WORDS);
+  uint32_t* out = tables16 + (size_t)g * COMB16_TABLE_WORDS +
+                  ((size_t)w * COMB16_ENTRIES + (d0 - 1)) * COMB16_ENTRY_WORDS;
+  uint32_t* sc = scratch + (size_t)t * COMB16_RUN * COMB_ENTRY_WORDS;
+  G1J acc;
+  pt_mul_u64_affine(acc, base, d0);
+  Fp prod;
+  fp_one(prod);
+  for (uint32_t i = 0; i < COMB16_RUN; i++) {
+    fp_mul(prod, prod, acc.z);
+    uint32_t* e = out + (size_t)i * COMB16_ENTRY_WORDS;
+    uint32_t* z = sc + (size_t)i * COMB_ENTRY_WORDS;
+    for (int k = 0; k < FPW; k++) {
+      e[k] = acc.x.v[k];
+      e[FPW + k] = acc.y.v[k];
+      z[k] = acc.z.v[k];
+      z[FPW + k] = prod.v[k];
+    }
+    for (int k = 2 * FPW; k < (int)COMB16_ENTRY_WORDS; k++) e[k] = 0;
+    if (i + 1 < COMB16_RUN) pt_madd(acc, acc, base);
+  }
+  Fp inv;
+  fp_inv(inv, prod);  // 1 / (z_0 ... z_{RUN-1})
+  for (int i = (int)COMB16_RUN - 1; i >= 0; i--) {
+    uint32_t* e = out + (size_t)i * COMB16_ENTRY_WORDS;
+    const uint32_t* z = sc + (size_t)i * COMB_ENTRY_WORDS;
+    Fp zd, pre, zi;
+    for (int k = 0; k < FPW; k++) zd.v[k] = z[k];
+    if (i > 0) {
+      for (int k = 0; k < FPW; k++) pre.v[k] = sc[(size_t)(i - 1) * COMB_ENTRY_WORDS + FPW + k];
+      fp_mul(zi, inv, pre);
+    } else {
+      zi = inv;
+    }
+    fp_mul(inv, inv, zd);
+    Fp X, Y, zi2, zi3;
+    for (int k = 0; k < FPW; k++) {
+      X.v[k] = e[k];
+      Y.v[k] = e[FPW + k];
+    }
+    fp_sqr(zi2, zi);
+    fp_mul(zi3, zi2, zi);
+    G1A a;
+    fp_mul(a.x, X, zi2);
+    fp_mul(a.y, Y, zi3);
+    store_g1a(e, a);
+  }
+}
+void launch_comb16_build(hipStream_t s, uint32_t first, uint32_t count, const uint32_t* tables,
+                         uint32_t* tables16, uint32_t* scratch) {
+  if (!count) return;
+  const uint32_t n = count * COMB16_WINDOWS * (COMB16_ENTRIES / COMB16_RUN);
+  hipLaunchKernelGGL(k_comb16_build, GRID(n, 64), 0, s, first, count, tables, tables16, scratch);
+}
+
 // ------------------------------------------------------------------ fixed-base comb MSM
 // Each lane sums up to term_count terms of one chunk into one Jacobian partial.
-// Scalar k (canonical, < r < 2^255) is recoded into 32 signed 8-bit digits.
+// Scalar k (canonical, < r < 2^255) is recoded into signed digits: 16 of 16
+// bits over the wide table when the generator slot is below n16, else 32 of
+// 8 bits.  Windows never straddle a 32-bit word of k.  The loop is software
+// pipelined: the table entry of the next (term, window) is loaded while the
+// mixed addition of the current one runs, which hides the latency of the
+// random reads from the wide tables.
+struct CombCursor {
+  const uint32_t* __restrict__ term_gen;
+  const uint32_t* __restrict__ term_scalar;
+  const uint32_t* __restrict__ tables;
+  const uint32_t* __restrict__ tables16;
+  uint32_t n16;
+  uint32_t t, tend, w, carry;
+  // shape of the current term
+  const uint32_t* tab;
+  uint32_t bits, windows, wstride, estride;
+  uint32_t k[8];
+
+  DEV void term() {
+    const uint32_t g = term_gen[t];
+    const bool wide = g < n16;
+    bits = wide ? 16 : 8;
+    windows = wide ? COMB16_WINDOWS : COMB_WINDOWS;
+    wstride = wide ? COMB16_ENTRIES * (uint32_t)COMB16_ENTRY_WORDS : COMB_ENTRIES * (uint32_t)COMB_ENTRY_WORDS;
+    estride = wide ? (uint32_t)COMB16_ENTRY_WORDS : (uint32_t)COMB_ENTRY_WORDS;
+    tab = wide ? tables16 + (size_t)g * COMB16_TABLE_WORDS : tables + (size_t)g * COMB_TABLE_WORDS;
+#pragma unroll
+    for (int q = 0; q < 8; q++) k[q] = term_scalar[(size_t)t * FR_WORDS + q];
+    w = 0;
+    carry = 0;
+  }
+  // Next digit: false when the chunk is exhausted; entry = nullptr for a zero
+  // digit, else the table entry of |digit| and neg its sign.
+  DEV bool next(const uint4*& entry, bool& neg) {
+    if (t >= tend) return false;
+    const uint32_t o = w * bits;
+    const uint32_t j = o >> 5;
+    uint32_t word = k[0];
+#pragma unroll
+    for (int q = 1; q < 8; q++) word = j == (uint32_t)q ? k[q] : word;
+    const uint32_t b = ((word >> (o & 31)) & ((1u << bits) - 1)) + carry;
+    neg = b > (1u << (bits - 1));
+    carry = neg ? 1 : 0;
+    const uint32_t ad = neg ? (1u << bits) - b : b;
+    entry = ad ? reinterpret_cast<const uint4*>(tab + w * wstride + (ad - 1) * estride) : nullptr;
+    if (++w == windows && ++t < tend) term();
+    return true;
+  }
+};
+
 __global__ __launch_bounds__(128) void k_comb_msm(const CombChunk* __restrict__ chunks, uint32_t n,
                                                  const uint32_t* __restrict__ term_gen,
                                                  const uint32_t* __restrict__ term_scalar,
                                                  const uint32_t* __restrict__ tables,
+                                                 const uint32_t* __restrict__ tables16, uint32_t n16,
                                                  uint32_t* __restrict__ partials) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  CombChunk c = chunks[i];
+  const CombChunk c = chunks[i];
   G1J acc;
   pt_set_inf(acc);
-  for (uint32_t t = c.term_begin; t < c.term_begin + c.term_count; t++) {
-    const uint32_t g = term_gen[t];
-    uint32_t k[8];
+  CombCursor cur;
+  cur.term_gen = term_gen;
+  cur.term_scalar = term_scalar;
+  cur.tables = tables;
+  cur.tables16 = tables16;
+  cur.n16 = n16;
+  cur.t = c.term_begin;
+  cur.tend = c.term_begin + c.term_count;
+  if (cur.t < cur.tend) cur.term();
+  constexpr int Q = (int)COMB_ENTRY_WORDS / 4;
+  uint4 buf[Q];
+  const uint4* e4;
+  bool neg;
+  bool have = cur.next(e4, neg);
+  if (have && e4) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) k[q] = term_scalar[(size_t)t * FR_WORDS + q];
-    const uint32_t* tab = tables + (size_t)g * COMB_TABLE_WORDS;
-    uint32_t carry = 0;
-#pragma unroll 1
-    for (int w = 0; w < COMB_WINDOWS; w++) {
-      uint32_t b = ((k[w >> 2] >> ((w & 3) * 8)) & 0xff) + carry;
-      int d;
-      if (b > 128) {
-        d = (int)b - 256;
-        carry = 1;
-      } else {
-        d = (int)b;
-        carry = 0;
-      }
-      if (d != 0) {
-        uint32_t ad = d < 0 ? (uint32_t)(-d) : (uint32_t)d;
-        const uint4* e4 = reinterpret_cast<const uint4*>(
-            tab + ((size_t)w * COMB_ENTRIES + (ad - 1)) * COMB_ENTRY_WORDS);
-        G1A e;
-        uint32_t ew[COMB_ENTRY_WORDS];
+    for (int q = 0; q < Q; q++) buf[q] = e4[q];
+  }
+  while (have) {
+    const bool add = e4 != nullptr;
+    const bool sub = neg;
+    G1A e;
+    {
+      uint32_t ew[COMB_ENTRY_WORDS];
 #pragma unroll
-        for (int q = 0; q < (int)COMB_ENTRY_WORDS / 4; q++) {
-          uint4 v = e4[q];
-          ew[4 * q] = v.x;
-          ew[4 * q + 1] = v.y;
-          ew[4 * q + 2] = v.z;
-          ew[4 * q + 3] = v.w;
-        }
-        load_g1a(e, ew);
-        if (d < 0) fp_neg(e.y, e.y);
-        g1_madd_in(acc, e);
+      for (int q = 0; q < Q; q++) {
+        ew[4 * q] = buf[q].x;
+        ew[4 * q + 1] = buf[q].y;
+        ew[4 * q + 2] = buf[q].z;
+        ew[4 * q + 3] = buf[q].w;
       }
+      load_g1a(e, ew);
+    }
+    have = cur.next(e4, neg);
+    if (have && e4) {
+#pragma unroll
+      for (int q = 0; q < Q; q++) buf[q] = e4[q];
+    }
+    if (add) {
+      if (sub) fp_neg(e.y, e.y);
+      g1_madd_in(acc, e);
     }
   }
   store_g1j(partials + (size_t)c.partial * G1J_WORDS, acc);
 }
 void launch_comb_msm(hipStream_t s, const CombChunk* chunks, uint32_t n, const uint32_t* term_gen,
-                     const uint32_t* term_scalar, const uint32_t* tables, uint32_t* partials) {
+                     const uint32_t* term_scalar, const uint32_t* tables, const uint32_t* tables16,
+                     uint32_t n16, uint32_t* partials) {
   if (!n) return;
-  hipLaunchKernelGGL(k_comb_msm, GRID(n, 128), 0, s, chunks, n, term_gen, term_scalar, tables,
-                     partials);
+  hipLaunchKernelGGL(k_comb_msm, GRID(n, 64), 0, s, chunks, n, term_gen, term_scalar, tables,
+                     tables16, n16, partials);
 }
 
 // ------------------------------------------------------------------ public keys

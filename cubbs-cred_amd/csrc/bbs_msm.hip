@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t n, uint3
 }
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v) {
   if (!n) return;
-  hipLaunchKernelGGL(k_fill_u32, GRID(n, 256), 0, s, p, n, v);
+  hipLaunchKernelGGL(k_fill_u32, GRID(n, 64), 0, s, p, n, v);
 }
 
 // ------------------------------------------------------------------ histogram
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_pip_count(PipShape sh, const PipTerm* _
 void launch_pip_count(hipStream_t s, const PipShape& sh, const PipTerm* terms, const uint8_t* pinf,
                       const uint32_t* scalars, const PipBuffers& bf) {
   if (!sh.terms) return;
-  hipLaunchKernelGGL(k_pip_count, GRID(sh.terms, 256), 0, s, sh, terms, pinf, scalars, bf.cnt);
+  hipLaunchKernelGGL(k_pip_count, GRID(sh.terms, 64), 0, s, sh, terms, pinf, scalars, bf.cnt);
 }
 
 // ------------------------------------------------------------------ scan
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void k_pip_scatter(PipShape sh, const PipTerm*
 void launch_pip_scatter(hipStream_t s, const PipShape& sh, const PipTerm* terms,
                         const uint8_t* pinf, const uint32_t* scalars, const PipBuffers& bf) {
   if (!sh.terms) return;
-  hipLaunchKernelGGL(k_pip_scatter, GRID(sh.terms, 256), 0, s, sh, terms, pinf, scalars, bf.cursor,
+  hipLaunchKernelGGL(k_pip_scatter, GRID(sh.terms, 64), 0, s, sh, terms, pinf, scalars, bf.cursor,
                      bf.sorted);
 }
 

@@ -16,6 +16,10 @@
 
 namespace cubbs {
 
+// Every kernel of the pipelines launches single-wave (64-lane) workgroups: a
+// pairing launch of another batch can hold most VGPRs of every SIMD, leaving
+// one or two wave slots per CU, and a multi-wave workgroup (which needs all
+// its waves on one CU) would wait until those pairing waves retire.
 #define GRID(n, b) dim3(((n) + (b) - 1) / (b)), dim3(b)
 
 __global__ void k_fill_status(uint32_t n, uint8_t* status, uint8_t* gate, uint8_t gv) {
@@ -26,7 +30,7 @@ __global__ void k_fill_status(uint32_t n, uint8_t* status, uint8_t* gate, uint8_
 }
 void launch_fill_status(hipStream_t s, uint32_t n, uint8_t* status, uint8_t* gate, uint8_t gv) {
   if (!n) return;
-  hipLaunchKernelGGL(k_fill_status, GRID(n, 256), 0, s, n, status, gate, gv);
+  hipLaunchKernelGGL(k_fill_status, GRID(n, 64), 0, s, n, status, gate, gv);
 }
 
 // ------------------------------------------------------------------ transfers
@@ -47,7 +51,7 @@ __global__ __launch_bounds__(256) void k_copy(uint8_t* __restrict__ dst,
 void launch_copy(hipStream_t s, void* dst, const void* src, size_t bytes) {
   if (!bytes) return;
   const size_t lanes = (bytes + 15) / 16;
-  hipLaunchKernelGGL(k_copy, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)((lanes + 63) / 64)), dim3(64), 0, s,
                      static_cast<uint8_t*>(dst), static_cast<const uint8_t*>(src), bytes);
 }
 
@@ -108,7 +112,7 @@ __global__ __launch_bounds__(256) void k_decode_fr(const DecodeFrJob* __restrict
 void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const uint8_t* blob,
                       uint32_t* scalars, uint8_t* status) {
   if (!n) return;
-  hipLaunchKernelGGL(k_decode_fr, GRID(n, 256), 0, s, jobs, n, blob, scalars, status);
+  hipLaunchKernelGGL(k_decode_fr, GRID(n, 64), 0, s, jobs, n, blob, scalars, status);
 }
 
 // ------------------------------------------------------------------ term scalars
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(256) void k_term_fill(const TermFill* __restrict__ 
 }
 void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t* scalars) {
   if (!n) return;
-  hipLaunchKernelGGL(k_term_fill, GRID(n, 256), 0, s, jobs, n, scalars);
+  hipLaunchKernelGGL(k_term_fill, GRID(n, 64), 0, s, jobs, n, scalars);
 }
 
 // ------------------------------------------------------------------ variable-base multiplication
@@ -298,7 +302,7 @@ void launch_pair_lines(hipStream_t s, const PairJob* jobs, uint32_t n, const uin
                        const uint32_t* pk_lines, const uint32_t* neg_g2_lines, uint32_t* l2buf) {
   if (!n) return;
   const uint32_t nl = n * (uint32_t)(MILLER_LINES * 4);
-  hipLaunchKernelGGL(k_pair_lines, GRID(nl, 256), 0, s, jobs, n, points, pk_lines, neg_g2_lines,
+  hipLaunchKernelGGL(k_pair_lines, GRID(nl, 64), 0, s, jobs, n, points, pk_lines, neg_g2_lines,
                      l2buf);
 }
 

@@ -32,6 +32,11 @@ struct Context {
   uint32_t gen_slots = 0, gen_cap = 0;
   uint32_t* gen_affine = nullptr;
   uint32_t* gen_tables = nullptr;
+  // wide (16-bit window) comb tables of slots [0, wide_slots); the buffer
+  // holds wide_cap slots and is allocated once, never moved
+  uint32_t* gen_tables16 = nullptr;
+  uint32_t wide_cap = 0, wide_slots = 0;
+  bool wide_init = false;
   uint8_t* gen_comp = nullptr;
   uint32_t* gen_vstate = nullptr;
   std::vector<uint8_t> gen_comp_host;                   // 48 bytes per slot

@@ -70,7 +70,8 @@ int32_t cubbs_hip_batch_outputs(void *batch, uint8_t *out, size_t out_cap, size_
 int32_t cubbs_hip_batch_profile(void *batch, int iters, char *names_out, size_t names_cap,
                                 double *ms_out, int max_kernels, double *total_ms_out);
 /* Work units of the batch for the roofline: number of items, total terms of
- * the fixed-base MSM, pairings. */
+ * the fixed-base MSM, mixed additions they take (16 per term over a generator
+ * with wide 16-bit comb tables, 32 per term otherwise), pairings. */
 int32_t cubbs_hip_batch_shape(void *batch, uint64_t *items, uint64_t *msm_terms,
                               uint64_t *comb_madds, uint64_t *pairings);
 
