@@ -560,6 +560,14 @@ class StagedBatch:
             return [(res[i].code, res[i].value) for i in range(self.n)]
         return [(res[i].code, bool(res[i].value)) for i in range(self.n)]
 
+    def results_raw(self):
+        """The result array as bytes (cheap to compare between launches)."""
+        res = (Result * max(1, self.n))()
+        rc = lib().cubbs_hip_batch_results(self.h, res)
+        if rc:
+            raise RuntimeError("results failed %d" % rc)
+        return bytes(res)
+
     def use_graph(self, on=True):
         """Capture the launch sequence into a HIP graph (on) or drop it (off)."""
         rc = lib().cubbs_hip_batch_use_graph(self.h, 1 if on else 0)
