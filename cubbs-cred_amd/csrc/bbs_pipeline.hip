@@ -211,7 +211,8 @@ __global__ __launch_bounds__(64) void k_reduce(const ReduceGroup* __restrict__ g
     fp_zero(a.y);
   } else {
     pinf[o] = 0;
-    pt_to_affine(a, acc);
+    G1J t = acc;  // the out-of-line conversion takes a copy's address, not acc's
+    pt_to_affine(a, t);
   }
   store_g1a(points + (size_t)o * G1A_WORDS, a);
 }

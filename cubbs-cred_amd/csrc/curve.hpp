@@ -310,9 +310,14 @@ DEV void g1_madd_in(G1J& p, const G1A& q) {
   fp_sub(t2, t2, p.y);      // S2 - Y1
   if (fp_is_zero(t1)) {
     if (fp_is_zero(t2)) {
-      G1J qj;
+      // the out-of-line doubling works on locals only: passing p itself would
+      // take its address and keep the caller's accumulator in scratch memory
+      // (an inline doubling here instead doubles the code of every caller and
+      // slowed the single-lane loops)
+      G1J qj, r;
       pt_from_affine(qj, q);
-      pt_dbl(p, qj);
+      pt_dbl(r, qj);
+      p = r;
     } else {
       pt_set_inf(p);
     }
@@ -358,10 +363,13 @@ DEV void g1_add_in(G1J& p, const G1J& q) {
   fp_sub(t3, t3, t2);       // H = U2 - U1
   fp_sub(t5, t5, t4);       // S2 - S1
   if (fp_is_zero(t3)) {
-    if (fp_is_zero(t5))
-      pt_dbl(p, p);
-    else
+    if (fp_is_zero(t5)) {
+      G1J a = p, r;  // locals only, as in g1_madd_in
+      pt_dbl(r, a);
+      p = r;
+    } else {
       pt_set_inf(p);
+    }
     return;
   }
   fp_add(p.z, p.z, q.z);    // Z3 = ((Z1 + Z2)^2 - Z1Z1 - Z2Z2) H
