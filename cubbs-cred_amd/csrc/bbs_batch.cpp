@@ -216,6 +216,8 @@ void materialize(Batch& b, Plan& p) {
   size_t o_va = take(std::max<uint32_t>(slots, 1));
   size_t o_out = take(std::max<size_t>(p.out_bytes, 1));
   size_t o_l2 = take(4ull * PAIR_L2_WORDS * std::max<size_t>(p.pairs.size(), 1));
+  // product workspaces: the plan's jobs, then the aggregation jobs
+  size_t o_vt = take(4ull * VAR_WS_WORDS * std::max<size_t>(p.var.size() + p.agg_var.size(), 1));
   const PipShape& ps = b.pip;
   const size_t pip_on = ps.terms ? 1 : 0;
   size_t o_pcnt = take(4ull * (ps.nbt + 1) * pip_on);
@@ -242,6 +244,8 @@ void materialize(Batch& b, Plan& p) {
   b.valid = W + o_va;
   b.out = W + o_out;
   b.l2buf = reinterpret_cast<uint32_t*>(W + o_l2);
+  b.vtab = reinterpret_cast<uint32_t*>(W + o_vt);
+  b.vtab_agg = b.vtab + (size_t)VAR_WS_WORDS * p.var.size();
   if (pip_on) {
     auto u32 = [&](size_t o) { return reinterpret_cast<uint32_t*>(W + o); };
     b.pipb = PipBuffers{u32(o_pcnt), u32(o_poff), u32(o_pcur), u32(o_ppof), u32(o_pmap),
@@ -891,7 +895,7 @@ void launch_batch(Batch& b) {
     }
     if (b.n_var || b.n_dec_g1) {
       Timed t(b, "var_mul_glv", b.aux);
-      launch_var_mul(b.aux, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials, b.dec_g1,
+      launch_var_mul(b.aux, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials, b.vtab, b.dec_g1,
                      b.n_dec_g1, b.status);
     }
     check(hipEventRecord(b.ev_join, b.aux));
@@ -918,7 +922,7 @@ void launch_batch(Batch& b) {
   }
   if (!fork && b.kind != KIND_PROOF_GEN && b.n_var) {
     Timed t(b, "var_mul_glv");
-    launch_var_mul(s, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials);
+    launch_var_mul(s, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials, b.vtab);
   }
   if (b.n_chunks) {
     Timed t(b, "comb_msm");
@@ -966,7 +970,7 @@ void launch_batch(Batch& b) {
     }
     {
       Timed t(b, "var_mul_glv");
-      launch_var_mul(s, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials);
+      launch_var_mul(s, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials, b.vtab);
     }
     {
       Timed t(b, "reduce2");
@@ -995,7 +999,8 @@ void launch_batch(Batch& b) {
     // sums c_i a_i, c_i b_i of each aggregated group
     {
       Timed t(b, "agg_mul");
-      launch_var_mul(s, b.agg_var, b.n_agg_var, b.points, b.pinf, b.scalars, b.partials);
+      launch_var_mul(s, b.agg_var, b.n_agg_var, b.points, b.pinf, b.scalars, b.partials,
+                     b.vtab_agg);
     }
     Timed t(b, "agg_reduce");
     launch_reduce(s, b.groups2, b.out2, b.n_groups2, b.partials, b.points, b.pinf);

@@ -97,10 +97,13 @@ struct DecodeFrJob {
   uint32_t pad;
 };
 
-// partials[partial] = scalars[scalar] * points[point] (identity-aware).
-// [k] P with the GLV split k = k1 + k2 lambda on two lanes: partials
-// [partial] = [k1] P and [partial + 1] = [k2] phi(P).
-constexpr uint32_t VAR_PARTIALS = 2;
+// partials[partial] = scalars[scalar] * points[point] (identity-aware), on
+// one lane: the GLV split k = k1 + k2 lambda evaluated as k1 P + k2 phi(P)
+// with shared doublings and signed 4-bit windows (Straus), over two tables of
+// affine multiples 1..8 built in a per-job workspace of VAR_WS_WORDS words.
+constexpr uint32_t VAR_PARTIALS = 1;
+constexpr uint32_t VAR_TABLE_WORDS = 8 * 28;                      // 8 affine points
+constexpr uint32_t VAR_WS_WORDS = 2 * VAR_TABLE_WORDS + 7 * 42;   // + 7 Jacobian temps
 struct VarJob {
   uint32_t point;
   uint32_t scalar;
@@ -216,9 +219,10 @@ void launch_decode_g1(hipStream_t s, const DecodeG1Job* jobs, uint32_t n, const 
                       uint32_t* points, uint8_t* pinf, uint8_t* status, bool check_subgroup = true);
 void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const uint8_t* blob,
                       uint32_t* scalars, uint8_t* status);
-// n_sub > 0: also check the subgroup membership of the points of sub[]
+// ws: n * VAR_WS_WORDS words of device workspace.  n_sub > 0: also check
+// the subgroup membership of the points of sub[]
 void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_t* points,
-                    const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials,
+                    const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials, uint32_t* ws,
                     const DecodeG1Job* sub = nullptr, uint32_t n_sub = 0, uint8_t* status = nullptr);
 void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t* scalars);
 // l2buf: PAIR_L2_WORDS words of device scratch per job, filled by

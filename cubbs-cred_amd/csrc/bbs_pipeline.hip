@@ -141,23 +141,160 @@ void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t*
   hipLaunchKernelGGL(k_term_fill, GRID(n, 64), 0, s, jobs, n, scalars);
 }
 
-// ------------------------------------------------------------------ variable-base multiplication
-// Two lanes per job: lane 0 forms [k1] P, lane 1 [k2] phi(P) (k = k1 + k2 lambda);
-// the reduction stage adds the two partials.
-// Lanes [0, 2n): the two GLV halves of each product.  Lanes [2n, 2n + n_sub):
-// the prime-order subgroup check of the points decoded by jobs sub[] without
-// it, run beside the products; a failure marks the item's status (the
-// products of such a point are computed but never used).
+// ------------------------------------------------------------------ variable-base products
+// Signed 4-bit windows of a 128-bit GLV half e: E' = e + sum_{i<=32} 8 * 16^i
+// (five words), digit i = nibble i of E' minus 8, in [-8, 7]; digit 32 is
+// the carry (0 or 1).  sum_i d_i 16^i = e.
+DEV void var_digits(uint32_t ed[5], const uint32_t e[4]) {
+  uint64_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    c += (uint64_t)(i < 4 ? e[i] : 0u) + (i < 4 ? 0x88888888u : 0x8u);
+    ed[i] = (uint32_t)c;
+    c >>= 32;
+  }
+}
+DEV int var_digit(const uint32_t ed[5], int i) {
+  const int w = i >> 3;
+  uint32_t v = ed[0];
+#pragma unroll
+  for (int q = 1; q < 5; q++) v = w == q ? ed[q] : v;
+  return (int)((v >> ((i & 7) * 4)) & 15u) - 8;
+}
+
+// tab[j] = (j + 1) q, j = 0..7, affine: 2q..8q in Jacobian form (jw, 7 points)
+// then normalised with one inversion (Montgomery's trick; the running
+// products are parked in tab[1..7] until the backward pass overwrites them).
+DEV void var_table(const G1A& q, uint32_t* tab, uint32_t* jw) {
+  store_g1a(tab, q);
+  G1J a, b;
+  pt_from_affine(a, q);
+  g1_dbl_in(a);                          // 2q
+  store_g1j(jw + 0 * G1J_WORDS, a);
+  b = a;
+  g1_dbl_in(b);                          // 4q
+  store_g1j(jw + 2 * G1J_WORDS, b);
+  g1_madd_in(a, q);                      // 3q
+  store_g1j(jw + 1 * G1J_WORDS, a);
+  g1_dbl_in(a);                          // 6q
+  store_g1j(jw + 4 * G1J_WORDS, a);
+  g1_madd_in(a, q);                      // 7q
+  store_g1j(jw + 5 * G1J_WORDS, a);
+  a = b;
+  g1_madd_in(a, q);                      // 5q
+  store_g1j(jw + 3 * G1J_WORDS, a);
+  g1_dbl_in(b);                          // 8q
+  store_g1j(jw + 6 * G1J_WORDS, b);
+  Fp run, z;
+#pragma unroll 1
+  for (int j = 0; j < 7; j++) {
+    ld_fp(z, jw + j * G1J_WORDS + 2 * FPW);
+    if (j)
+      fp_mul(run, run, z);
+    else
+      run = z;
+    st_fp(tab + (j + 1) * G1A_WORDS, run);
+  }
+  Fp inv;
+  fp_inv(inv, run);
+#pragma unroll 1
+  for (int j = 6; j >= 0; j--) {
+    Fp zi, pre, zi2, zi3, X, Y;
+    ld_fp(z, jw + j * G1J_WORDS + 2 * FPW);
+    if (j) {
+      ld_fp(pre, tab + j * G1A_WORDS);
+      fp_mul(zi, inv, pre);
+    } else {
+      zi = inv;
+    }
+    fp_mul(inv, inv, z);
+    fp_sqr(zi2, zi);
+    fp_mul(zi3, zi2, zi);
+    ld_fp(X, jw + j * G1J_WORDS);
+    ld_fp(Y, jw + j * G1J_WORDS + FPW);
+    G1A o;
+    fp_mul(o.x, X, zi2);
+    fp_mul(o.y, Y, zi3);
+    store_g1a(tab + (j + 1) * G1A_WORDS, o);
+  }
+}
+
+DEV void var_entry(G1A& e, const uint32_t* tab, int d) {
+  const int a = d < 0 ? -d : d;
+  load_g1a(e, tab + (a - 1) * G1A_WORDS);
+}
+
+// [k] P for an affine P (not the identity) on one lane: k = k1 + k2 lambda,
+// k1 P + k2 phi(P) evaluated from the top window down with 4 shared
+// doublings and at most two mixed additions per window (uniform control
+// flow across the lanes of a wave, unlike a bit-by-bit ladder).  The entries
+// of the next window are loaded before the doublings.
+DEV void var_mul_one(G1J& r, const G1A& p, const uint32_t k[8], uint32_t* ws) {
+  uint32_t k1[4], k2[4];
+  glv_split(k1, k2, k);
+  uint32_t* t1 = ws;
+  uint32_t* t2 = ws + VAR_TABLE_WORDS;
+  var_table(p, t1, ws + 2 * VAR_TABLE_WORDS);
+  {
+    Fp beta;
+    fp_load_const(beta, bls::GLV_BETA);
+#pragma unroll 1
+    for (int j = 0; j < 8; j++) {  // phi(j P) = (beta x, y)
+      G1A e;
+      load_g1a(e, t1 + j * G1A_WORDS);
+      fp_mul(e.x, e.x, beta);
+      store_g1a(t2 + j * G1A_WORDS, e);
+    }
+  }
+  uint32_t d1[5], d2[5];
+  var_digits(d1, k1);
+  var_digits(d2, k2);
+  int top = 32;
+  while (top >= 0 && var_digit(d1, top) == 0 && var_digit(d2, top) == 0) top--;
+  G1J acc;
+  pt_set_inf(acc);
+  // one copy each of the doubling and of the addition in the loop body keeps
+  // it within the instruction cache
+#pragma unroll 1
+  for (int i = top; i >= 0; i--) {
+    const int a = var_digit(d1, i), b = var_digit(d2, i);
+    G1A ea, eb;
+    if (a) var_entry(ea, t1, a);
+    if (b) var_entry(eb, t2, b);
+    if (i != top) {
+#pragma unroll 1
+      for (int s = 0; s < 4; s++) g1_dbl_in(acc);
+    }
+#pragma unroll 1
+    for (int s = 0; s < 2; s++) {
+      const int d = s ? b : a;
+      if (d) {
+        G1A e;
+        e.x = s ? eb.x : ea.x;
+        e.y = s ? eb.y : ea.y;
+        if (d < 0) fp_neg(e.y, e.y);
+        g1_madd_in(acc, e);
+      }
+    }
+  }
+  r = acc;
+}
+
+// Lanes [0, n): one product per job (workspace ws + t * VAR_WS_WORDS).
+// Lanes [n, n + n_sub): the prime-order subgroup check of the points decoded
+// by jobs sub[] without it, run beside the products; a failure marks the
+// item's status (the products of such a point are computed but never used).
 __global__ __launch_bounds__(64) void k_var_mul(const VarJob* __restrict__ jobs, uint32_t n,
                                                const uint32_t* __restrict__ points,
                                                const uint8_t* __restrict__ pinf,
                                                const uint32_t* __restrict__ scalars,
                                                uint32_t* __restrict__ partials,
+                                               uint32_t* __restrict__ ws,
                                                const DecodeG1Job* __restrict__ sub, uint32_t n_sub,
                                                uint8_t* __restrict__ status) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2 * n) {
-    const uint32_t k = t - 2 * n;
+  if (t >= n) {
+    const uint32_t k = t - n;
     if (k >= n_sub) return;
     const DecodeG1Job j = sub[k];
     if (pinf[j.out]) return;  // the identity, or not decoded (status already set)
@@ -166,8 +303,7 @@ __global__ __launch_bounds__(64) void k_var_mul(const VarJob* __restrict__ jobs,
     if (!g1_in_subgroup(a)) status[j.item] = 1;
     return;
   }
-  const uint32_t i = t >> 1, half = t & 1;
-  VarJob j = jobs[i];
+  const VarJob j = jobs[t];
   G1J r;
   pt_set_inf(r);
   if (!pinf[j.point]) {
@@ -175,16 +311,16 @@ __global__ __launch_bounds__(64) void k_var_mul(const VarJob* __restrict__ jobs,
     load_g1a(p, points + (size_t)j.point * G1A_WORDS);
     Fr k;
     load_fr(k, scalars + (size_t)j.scalar * FR_WORDS);
-    g1_mul_glv_half(r, p, k.v, half);
+    var_mul_one(r, p, k.v, ws + (size_t)t * VAR_WS_WORDS);
   }
-  store_g1j(partials + (size_t)(j.partial + half) * G1J_WORDS, r);
+  store_g1j(partials + (size_t)j.partial * G1J_WORDS, r);
 }
 void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_t* points,
-                    const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials,
+                    const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials, uint32_t* ws,
                     const DecodeG1Job* sub, uint32_t n_sub, uint8_t* status) {
   if (!n && !n_sub) return;
-  hipLaunchKernelGGL(k_var_mul, GRID(2 * n + n_sub, 64), 0, s, jobs, n, points, pinf, scalars,
-                     partials, sub, n_sub, status);
+  hipLaunchKernelGGL(k_var_mul, GRID(n + n_sub, 64), 0, s, jobs, n, points, pinf, scalars,
+                     partials, ws, sub, n_sub, status);
 }
 
 // ------------------------------------------------------------------ reduction

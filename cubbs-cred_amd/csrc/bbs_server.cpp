@@ -95,8 +95,11 @@ bool msgs_valid(const CuBbsBbsGpuByteSlice* m, size_t n) {
 
 // Device bytes a request adds to a batch (staged inputs + work area), an
 // upper estimate: payload twice (blob + job records), per message term
-// about 0.4 KiB (scalar, term records, comb partial share), per item 4 KiB.
-size_t estimate(size_t payload, size_t terms) { return 4096 + 2 * payload + 400 * (terms + 4); }
+// about 0.4 KiB (scalar, term records, comb partial share), per item 4 KiB,
+// per variable-base product its table workspace.
+size_t estimate(size_t payload, size_t terms, size_t products) {
+  return 4096 + 2 * payload + 400 * (terms + 4) + 4 * VAR_WS_WORDS * products;
+}
 
 // Builds an owned request; returns 0 or an error code.
 int32_t copy_request(const CubbsHipRequest& q, Req& r) {
@@ -117,7 +120,7 @@ int32_t copy_request(const CubbsHipRequest& q, Req& r) {
       pk.finish(r, sl);
       r.msgs.assign(sl.begin() + 3, sl.end());
       r.v = CuBbsBbsGpuVerifyItem{sl[0], sl[1], sl[2], r.msgs.data(), r.msgs.size()};
-      r.est = estimate(r.bytes.size(), it.message_count + 2);
+      r.est = estimate(r.bytes.size(), it.message_count + 2, 3);
       return 0;
     }
     case CUBBS_HIP_REQ_PROOF_VERIFY: {
@@ -138,7 +141,7 @@ int32_t copy_request(const CubbsHipRequest& q, Req& r) {
       r.pv = CuBbsBbsGpuProofVerifyItem{sl[0], sl[1], sl[2], sl[3], r.msgs.data(), r.msgs.size(),
                                         CuBbsBbsGpuIndexSlice{r.idx.data(), r.idx.size()},
                                         it.total_message_count};
-      r.est = estimate(r.bytes.size(), it.total_message_count + 8);
+      r.est = estimate(r.bytes.size(), it.total_message_count + 8, 6);
       return 0;
     }
     case CUBBS_HIP_REQ_SIGN: {
@@ -155,7 +158,7 @@ int32_t copy_request(const CubbsHipRequest& q, Req& r) {
       r.out.assign(80, 0);
       r.s = CuBbsBbsGpuSignItem{sl[0], sl[1], sl[2], r.msgs.data(), r.msgs.size(),
                                 CuBbsBbsGpuByteBuffer{r.out.data(), r.out.size(), 0}};
-      r.est = estimate(r.bytes.size(), it.message_count + 2);
+      r.est = estimate(r.bytes.size(), it.message_count + 2, 0);
       return 0;
     }
     case CUBBS_HIP_REQ_PROOF_GEN: {
@@ -182,7 +185,7 @@ int32_t copy_request(const CubbsHipRequest& q, Req& r) {
         std::memcpy(r.seed, q.seed32, 32);
       else
         fill_os_random(r.seed, 32);
-      r.est = estimate(r.bytes.size(), 2 * it.message_count + 12);
+      r.est = estimate(r.bytes.size(), 2 * it.message_count + 12, 7);
       return 0;
     }
     default:

@@ -623,38 +623,6 @@ DEVN void glv_split(uint32_t k1[4], uint32_t k2[4], const uint32_t k[8]) {
   }
 }
 
-// One half of the GLV product: half 0 gives [k1] P, half 1 gives [k2] phi(P),
-// with k = k1 + k2 lambda (P affine, not the identity; 128-bit binary ladder
-// with mixed additions).
-DEV void g1_mul_glv_half(G1J& r, const G1A& p, const uint32_t k[8], uint32_t half) {
-  uint32_t k1[4], k2[4];
-  glv_split(k1, k2, k);
-  G1A q;
-  Fp beta;
-  fp_load_const(beta, bls::GLV_BETA);
-  fp_mul(q.x, p.x, beta);
-  if (!half) q.x = p.x;
-  q.y = p.y;
-  uint32_t e[4];
-#pragma unroll
-  for (int i = 0; i < 4; i++) e[i] = half ? k2[i] : k1[i];
-  // left-to-right from the top set bit, group law expanded inline
-  int top = 127;
-  while (top >= 0 && !((e[top >> 5] >> (top & 31)) & 1)) top--;
-  if (top < 0) {
-    pt_set_inf(r);
-    return;
-  }
-  G1J acc;
-  pt_from_affine(acc, q);
-#pragma unroll 1
-  for (int i = top - 1; i >= 0; i--) {
-    g1_dbl_in(acc);
-    if ((e[i >> 5] >> (i & 31)) & 1) g1_madd_in(acc, q);
-  }
-  r = acc;
-}
-
 // [k] P on G1 (P affine, not the identity) with the GLV split
 // k = k1 + k2 lambda: 128 shared doublings, mixed additions of P and phi(P).
 DEVN void g1_mul_glv(G1J& r, const G1A& p, const uint32_t k[8]) {

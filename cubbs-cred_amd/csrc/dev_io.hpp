@@ -12,6 +12,14 @@ DEV void store_fr(uint32_t* p, const Fr& r) {
 #pragma unroll
   for (int k = 0; k < 8; k++) p[k] = r.v[k];
 }
+DEV void ld_fp(Fp& r, const uint32_t* p) {
+#pragma unroll
+  for (int k = 0; k < FPW; k++) r.v[k] = p[k];
+}
+DEV void st_fp(uint32_t* p, const Fp& v) {
+#pragma unroll
+  for (int k = 0; k < FPW; k++) p[k] = v.v[k];
+}
 DEV void load_g1a(G1A& r, const uint32_t* p) {
 #pragma unroll
   for (int k = 0; k < FPW; k++) {

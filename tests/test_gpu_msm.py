@@ -166,3 +166,27 @@ def test_autotune_table_and_parity(gpu, oracle, multiples):
         assert rc == 0 and cres[0][2] == oracle.g1_msm(gens, rows[0])
     finally:
         assert gpu.tune_load("") == 0
+
+
+def test_variable_base_products_window_edges(gpu, oracle, multiples):
+    """One-term MSMs over non-generator points run as single-lane GLV products
+    with signed 4-bit windows (k = k1 + k2 lambda, k1 P + k2 phi(P)).  Scalars
+    put both halves on the recoding edges (nibbles 7, 8, 9, F; k1 or k2 zero;
+    0, small values, r - 1); a batch of 120 items mixes lanes whose top window
+    differs."""
+    _, _, pts = multiples
+    x = 0xD201000000010000
+    lam = x * x - 1
+    pats = [int(c * 32, 16) for c in "789F"] + [int("80" * 16, 16), int("0F" * 16, 16)]
+    scalars = [0, 1, 2, 7, 8, 9, 15, 16, 17, 255, (1 << 64) + 3, R_ORDER - 1, R_ORDER - 2,
+               lam, lam - 1, lam + 1, 2 * lam, R_ORDER - lam]
+    scalars += [(p + p * lam) % R_ORDER for p in pats] + pats + [(p * lam) % R_ORDER for p in pats]
+    rng = random.Random(77)
+    scalars += [rng.randrange(R_ORDER) for _ in range(120 - len(scalars))]
+    items = [([pts[5 + i]], [be(s)]) for i, s in enumerate(scalars)]
+    rc, res = gpu.g1_msm(items)
+    assert rc == 0
+    for (p, s), (code, _, out) in zip(items, res):
+        assert code == 0
+        k = int.from_bytes(s[0], "big")
+        assert out == (INF if k == 0 else oracle.g1_mul(p[0], s[0])), k
