@@ -93,7 +93,28 @@ Arena::~Arena() {
 }
 
 // Copy the plan into one pinned buffer + device mirror; allocate the work area.
+// Counting sort of job records by a small integer key (stable).
+template <class T, class F>
+static void sort_jobs(std::vector<T>& v, F key) {
+  if (v.size() < 2) return;
+  uint32_t kmax = 0;
+  for (const T& x : v) kmax = std::max(kmax, key(x));
+  std::vector<uint32_t> at((size_t)kmax + 2, 0);
+  for (const T& x : v) at[key(x) + 1]++;
+  for (size_t i = 1; i < at.size(); i++) at[i] += at[i - 1];
+  std::vector<T> out(v.size());
+  for (const T& x : v) out[at[key(x)]++] = x;
+  v.swap(out);
+}
+
 void materialize(Batch& b, Plan& p) {
+  // The lanes of a wave wait for the longest of their jobs: order message
+  // hashes by SHA-256 block count and domains by input length (each job
+  // names its output slot, so the order does not change any result).
+  sort_jobs(p.h2s, [](const H2sJob& j) { return std::min<uint32_t>(j.len >> 6, 1u << 16); });
+  sort_jobs(p.dom, [](const DomainJob& j) {
+    return std::min<uint32_t>((48u * j.L + j.hdr_len) >> 6, 1u << 16);
+  });
   std::vector<uint8_t> H;
   H.reserve(p.blob.size() + 4096);
   auto place = [&](const void* src, size_t bytes) {

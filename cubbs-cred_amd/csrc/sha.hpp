@@ -150,10 +150,8 @@ DEVI void sha_bytes_global(Sha256& s, const uint8_t* p, uint32_t len) {
   while (i < len) sha_byte(s, p[i++]);
 }
 
-DEVI void sha_bytes_const(Sha256& s, const uint8_t* p, uint32_t len) {
-#pragma unroll 1
-  for (uint32_t i = 0; i < len; i++) sha_byte(s, p[i]);
-}
+// Bytes of a constant string (a DST, the API id): the same word-level path.
+DEVI void sha_bytes_const(Sha256& s, const uint8_t* p, uint32_t len) { sha_bytes_global(s, p, len); }
 
 DEV void sha_u64_be(Sha256& s, uint64_t v) {
   sha_word(s, (uint32_t)(v >> 32));
@@ -161,11 +159,26 @@ DEV void sha_u64_be(Sha256& s, uint64_t v) {
 }
 
 // Finish: padding and length; digest words left in s.h (big-endian words).
+// After the 0x80 byte and the zero bytes that complete its word, the rest of
+// the block is zeroed as words (the compression leaves its schedule in w, so
+// unwritten words are not zero); the 64-bit length takes words 14 and 15,
+// in one more block when fewer than two words are left.
 DEVN void sha_final(Sha256& s) {
   const uint64_t bits = (uint64_t)s.n * 8;
   sha_byte(s, 0x80);
-  while ((s.n & 63) != 56) sha_byte(s, 0);
-  sha_u64_be(s, bits);
+  while (s.n & 3) sha_byte(s, 0);
+  const uint32_t wi = (s.n & 63) >> 2;  // next free word of the block
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+    if ((uint32_t)k >= wi) s.w[k] = 0;
+  if (wi > 14) {
+    sha256_compress(s.h, s.w);
+#pragma unroll
+    for (int k = 0; k < 14; k++) s.w[k] = 0;
+  }
+  s.w[14] = (uint32_t)(bits >> 32);
+  s.w[15] = (uint32_t)bits;
+  sha256_compress(s.h, s.w);
 }
 
 // ------------------------------------------------------------------ domain-separation tags
