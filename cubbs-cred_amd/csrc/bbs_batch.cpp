@@ -107,6 +107,18 @@ static void sort_jobs(std::vector<T>& v, F key) {
   v.swap(out);
 }
 
+// The same for reduce groups and their parallel output slots.
+static void sort_groups(std::vector<ReduceGroup>& g, std::vector<uint32_t>& out) {
+  if (g.size() < 2 || out.size() != g.size()) return;
+  std::vector<std::pair<ReduceGroup, uint32_t>> z(g.size());
+  for (size_t i = 0; i < g.size(); i++) z[i] = {g[i], out[i]};
+  sort_jobs(z, [](const std::pair<ReduceGroup, uint32_t>& e) { return std::min<uint32_t>(e.first.count, 1u << 16); });
+  for (size_t i = 0; i < g.size(); i++) {
+    g[i] = z[i].first;
+    out[i] = z[i].second;
+  }
+}
+
 void materialize(Batch& b, Plan& p) {
   // The lanes of a wave wait for the longest of their jobs: order message
   // hashes by SHA-256 block count and domains by input length (each job
@@ -115,6 +127,9 @@ void materialize(Batch& b, Plan& p) {
   sort_jobs(p.dom, [](const DomainJob& j) {
     return std::min<uint32_t>((48u * j.L + j.hdr_len) >> 6, 1u << 16);
   });
+  // the same for comb chunks (a row's last chunk is usually short)
+  sort_jobs(p.chunks, [](const CombChunk& c) { return std::min<uint32_t>(c.term_count, 1u << 16); });
+  sort_groups(p.groups1, p.out1);
   std::vector<uint8_t> H;
   H.reserve(p.blob.size() + 4096);
   auto place = [&](const void* src, size_t bytes) {
