@@ -6,7 +6,7 @@
 // Every entry compiles its items into one device program (a KIND_PRIM batch
 // of bbs_batch.cpp): compressed points are decoded with the subgroup check,
 // 32-byte big-endian scalars decoded (canonical check), each product
-// k * P runs as a two-lane GLV multiplication, or as comb terms over the
+// k * P runs as a one-lane windowed GLV multiplication, or as comb terms over the
 // resident tables when the points are the ciphersuite generators, partial
 // sums are added in a parallel tree (large MSMs) and the results compressed.
 // Pairing equalities use the prepared-line cache and the 12-lane pairing.
@@ -38,8 +38,8 @@ namespace {
 constexpr size_t LARGE_TERMS = 1024; // route id 3 from this many terms on
 
 // Item MSMs of at least this many terms run as Pippenger bucket MSMs
-// (CUBBS_HIP_PIPPENGER_MIN, default LARGE_TERMS), smaller ones as two-lane
-// GLV products summed in the partial-sum tree.
+// (CUBBS_HIP_PIPPENGER_MIN, default LARGE_TERMS), smaller ones as one-lane
+// windowed GLV products summed in the partial-sum tree.
 size_t pippenger_min() {
   const char* v = std::getenv("CUBBS_HIP_PIPPENGER_MIN");
   if (!v || !*v) return LARGE_TERMS;
