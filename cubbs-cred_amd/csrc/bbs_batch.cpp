@@ -916,6 +916,13 @@ void launch_batch(Batch& b) {
   // here; both need the decoded scalars and join before the reduction.
   const bool fork = (b.kind == CUBBS_HIP_BATCH_VERIFY || b.kind == CUBBS_HIP_BATCH_PROOF_VERIFY) &&
                     b.aux != nullptr;
+  // Primitive programs decode without the subgroup check too.  The check
+  // runs as extra lanes of the product launch, or, with a Pippenger MSM, as
+  // its own launch on the auxiliary stream beside the MSM's last stage (the
+  // cross-window Horner: one wave per MSM, which leaves the GPU idle), joined
+  // before the reduction.
+  const bool prim_split = b.kind == KIND_PRIM && b.n_dec_g1 && b.aux != nullptr;
+  const bool side_check = prim_split && b.pip.terms;
   if (b.n_dec_fr) {
     Timed t(b, "decode_fr");
     launch_decode_fr(s, b.dec_fr, b.n_dec_fr, b.blob, b.scalars, b.status);
@@ -936,8 +943,10 @@ void launch_batch(Batch& b) {
     }
     check(hipEventRecord(b.ev_join, b.aux));
   } else if (b.n_dec_g1) {
-    Timed t(b, "decode_g1");
-    launch_decode_g1(s, b.dec_g1, b.n_dec_g1, b.blob, b.points, b.pinf, b.status);
+    {
+      Timed t(b, "decode_g1");
+      launch_decode_g1(s, b.dec_g1, b.n_dec_g1, b.blob, b.points, b.pinf, b.status, !prim_split);
+    }
   }
   if (b.n_h2s || b.n_dom) {
     Timed t(b, "hash_domain");
@@ -956,9 +965,11 @@ void launch_batch(Batch& b) {
     Timed t(b, "sign_e");
     launch_sign_e(s, b.sign, b.n_sign, b.blob, b.pk_status, b.scalars, b.status);
   }
-  if (!fork && b.kind != KIND_PROOF_GEN && b.n_var) {
+  if (!fork && b.kind != KIND_PROOF_GEN && (b.n_var || (prim_split && !side_check))) {
+    const bool with_check = prim_split && !side_check;
     Timed t(b, "var_mul_glv");
-    launch_var_mul(s, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials, b.vtab);
+    launch_var_mul(s, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials, b.vtab,
+                   with_check ? b.dec_g1 : nullptr, with_check ? b.n_dec_g1 : 0, b.status);
   }
   if (b.n_chunks) {
     Timed t(b, "comb_msm");
@@ -986,9 +997,20 @@ void launch_batch(Batch& b) {
       Timed t(b, "pip_halve");
       for (uint32_t st = 1; st <= b.pip.K; st++) launch_pip_halve(s, b.pip, st, b.pipb);
     }
+    if (side_check) {
+      check(hipEventRecord(b.ev_fork, s));
+      check(hipStreamWaitEvent(b.aux, b.ev_fork, 0));
+      Timed t(b, "subgroup", b.aux);
+      launch_var_mul(b.aux, nullptr, 0, b.points, b.pinf, b.scalars, b.partials, b.vtab, b.dec_g1,
+                     b.n_dec_g1, b.status);
+    }
     {
       Timed t(b, "pip_final");
       launch_pip_final(s, b.pip, b.pip_out, b.pipb, b.partials);
+    }
+    if (side_check) {
+      check(hipEventRecord(b.ev_join, b.aux));
+      check(hipStreamWaitEvent(s, b.ev_join, 0));
     }
   }
   for (const auto& lv : b.levels) {
