@@ -86,3 +86,34 @@ def test_graph_replays_match_eager(gpu, oracle):
         vb.free()
         pb.free()
         mb.free()
+
+
+def test_batch_sets_in_flight_match_alone(gpu, oracle):
+    """Several batches in flight, as in the benchmark: four staged Verify +
+    ProofVerify sets replayed from graphs and all launched back to back
+    before any is waited for, over several rounds; every set's results equal
+    those of one set run alone and the oracle's verdicts."""
+    g = gpu
+    vreqs, pvreqs = _verify_and_proof_batches(g)
+    expect_v = [(0, bool(oracle.verify(*r))) for r in vreqs]
+    expect_pv = [(0, bool(oracle.proof_verify(*r))) for r in pvreqs]
+    sets = [(g.StagedBatch(g.BATCH_VERIFY, vreqs), g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs))
+            for _ in range(4)]
+    try:
+        alone = _run(sets[0][0]), _run(sets[0][1])
+        assert alone == (expect_v, expect_pv)
+        for v, p in sets:
+            v.use_graph(True)
+            p.use_graph(True)
+        for rnd in range(3):
+            for v, p in sets:
+                v.launch()
+                p.launch()
+            for v, p in sets:
+                v.sync()
+                p.sync()
+                assert (v.results(), p.results()) == alone, rnd
+    finally:
+        for v, p in sets:
+            v.free()
+            p.free()
