@@ -1066,18 +1066,18 @@ void launch_batch(Batch& b) {
   if (b.n_pairs) {
     {
       Timed t(b, "pair_lines");
-      launch_pair_lines(s, b.pairs, b.n_pairs, b.points, b.pk_lines, b.neg_g2_lines, b.l2buf);
+      launch_pair_lines(s, b.pairs, b.n_pairs, b.points, b.pinf, b.pk_status, b.pk_lines,
+                        b.neg_g2_lines, b.l2buf);
     }
     // the group checks first: their results gate the per-item checks
     if (b.n_gpairs) {
       Timed t(b, "pairing_groups");
-      launch_pairing(s, b.pairs, b.n_gpairs, b.points, b.pinf, b.pk_status, b.pk_lines,
-                     b.neg_g2_lines, b.l2buf, b.status, b.gate, b.valid);
+      launch_pairing(s, b.pairs, b.n_gpairs, b.points, b.pinf, b.pk_status, b.l2buf, b.status,
+                     b.gate, b.valid);
     }
     Timed t(b, "pairing");
     launch_pairing(s, b.pairs + b.n_gpairs, b.n_pairs - b.n_gpairs, b.points, b.pinf, b.pk_status,
-                   b.pk_lines, b.neg_g2_lines, b.l2buf + (size_t)b.n_gpairs * PAIR_L2_WORDS,
-                   b.status, b.gate, b.valid);
+                   b.l2buf + (size_t)b.n_gpairs * PAIR_L2_WORDS, b.status, b.gate, b.valid);
   }
   check(hipGetLastError());
 }

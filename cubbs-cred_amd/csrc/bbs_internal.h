@@ -226,18 +226,19 @@ void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_
                     const DecodeG1Job* sub = nullptr, uint32_t n_sub = 0, uint8_t* status = nullptr);
 void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t* scalars);
 // l2buf: PAIR_L2_WORDS words of device scratch per job, filled by
-// launch_pair_lines and read by launch_pairing
-constexpr size_t PAIR_L2_WORDS = (size_t)68 * 2 * 2 * 14;
+// launch_pair_lines and read by launch_pairing: per line and pairing the 9
+// second factors of the sparse line product (68 x 2 x 9 x 14 words)
+constexpr size_t PAIR_L2_WORDS = (size_t)68 * 2 * 9 * 14;
 // partials[out[i]] = sum of the partials of groups[i] (Jacobian, no inversion)
 void launch_reduce_partials(hipStream_t s, const ReduceGroup* groups, const uint32_t* out, uint32_t n,
                             uint32_t* partials);
 void launch_compress(hipStream_t s, const CompressJob* jobs, uint32_t n, const uint32_t* points,
                      const uint8_t* pinf, uint8_t* out);
 void launch_pair_lines(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
-                       const uint32_t* pk_lines, const uint32_t* neg_g2_lines, uint32_t* l2buf);
+                       const uint8_t* pinf, const uint8_t* pk_status, const uint32_t* pk_lines,
+                       const uint32_t* neg_g2_lines, uint32_t* l2buf);
 void launch_pairing(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
-                    const uint8_t* pinf, const uint8_t* pk_status, const uint32_t* pk_lines,
-                    const uint32_t* neg_g2_lines, uint32_t* l2buf, uint8_t* status,
+                    const uint8_t* pinf, const uint8_t* pk_status, uint32_t* l2buf, uint8_t* status,
                     const uint8_t* gate, uint8_t* valid);
 void launch_sign_e(hipStream_t s, const SignJob* jobs, uint32_t n, const uint8_t* blob,
                    const uint8_t* pk_status, uint32_t* scalars, uint8_t* status);

@@ -536,7 +536,8 @@ void launch_test_g1_mul(hipStream_t s, const uint8_t* pts, const uint8_t* scalar
 // e(P, Q) as a GT value through the same 12-lane Miller loop / final
 // exponentiation as the pairing checks (pairing12.hpp, 5 pairs per block).
 // The group's lanes share the line work: lane 0 prepares Q's lines, then
-// every lane evaluates -lambda x_P for its share of the 68 lines.
+// every lane writes the line blocks (p12::line_values) of its share of the
+// 68 lines x 2 pairings; the second pairing is unused (the line 1).
 __global__ __launch_bounds__(64) void k_test_pairing_gt(const uint8_t* g1, const uint8_t* g2,
                                                        uint32_t n, uint8_t* out, uint8_t* ok,
                                                        uint32_t* scratch) {
@@ -570,21 +571,34 @@ __global__ __launch_bounds__(64) void k_test_pairing_gt(const uint8_t* g1, const
   }
   __syncthreads();
   __threadfence_block();
-  if (mine && good) {
+  if (mine) {
     for (int e = gl; e < MILLER_LINES * 2; e += 12) {
-      const int idx = e >> 1, comp = e & 1;
-      Fp lam, v;
-      fp_load_const(lam, lines + (size_t)idx * MILLER_LINE_WORDS + comp * FPW);
-      fp_mul(v, lam, pa.x);
-      fp_neg(v, v);
-      p12::st(l2 + ((size_t)idx * 2) * 2 * FPW + comp * FPW, v);
+      const int idx = e >> 1, q = e & 1;
+      const bool use = good && q == 0;
+      Fp l0, l1, c0, c1, m0, m1;
+      fp_zero(m0);
+      fp_zero(m1);
+      fp_zero(c0);
+      fp_zero(c1);
+      if (use) {
+        const uint32_t* line = lines + (size_t)idx * MILLER_LINE_WORDS;
+        fp_load_const(l0, line);
+        fp_load_const(l1, line + FPW);
+        fp_load_const(c0, line + 2 * FPW);
+        fp_load_const(c1, line + 3 * FPW);
+        fp_mul(m0, l0, pa.x);
+        fp_neg(m0, m0);
+        fp_mul(m1, l1, pa.x);
+        fp_neg(m1, m1);
+      }
+      p12::line_values(l2 + ((size_t)idx * 2 + q) * p12::LV_N * FPW, m0, m1, c0, c1, !use);
     }
   }
   __syncthreads();
   __threadfence_block();
   Fp f, r, zero;
   fp_zero(zero);
-  p12::miller(f, lines, lines, l2, pa.y, zero, !good, true, L);
+  p12::miller(f, l2, pa.y, zero, L);
   p12::final_exp(r, f, L);
   if (mine) {
     if (gl == 0) ok[i] = good;

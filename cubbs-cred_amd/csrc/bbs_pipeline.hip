@@ -408,47 +408,60 @@ void launch_compress(hipStream_t s, const CompressJob* jobs, uint32_t n, const u
 // ------------------------------------------------------------------ pairing check
 // e(a, PK) * e(b, -G2) == 1 (identity arguments contribute 1).
 //
-// k_pair_lines evaluates the item-dependent part of every Miller-loop line,
-// l2 = -lambda x_P, one lane per (check, line, pairing, component); the
-// pairing kernel then runs each check on 12 lanes (pairing12.hpp), 5 checks
-// per 64-thread block.
+// k_pair_lines prepares the item-dependent part of every Miller-loop line;
+// the pairing kernel then runs each check on 12 lanes (pairing12.hpp), 5
+// checks per 64-thread block.
+//
+// Line blocks of the pairing checks: one lane per (check, line, pairing)
+// forms m = -lambda x_P and writes the line's 9 second factors
+// (p12::line_values); a pairing with an identity point or an identity key
+// (skipped by the check) gets the constant line 1.
 __global__ __launch_bounds__(256) void k_pair_lines(const PairJob* __restrict__ jobs, uint32_t n,
                                                    const uint32_t* __restrict__ points,
+                                                   const uint8_t* __restrict__ pinf,
+                                                   const uint8_t* __restrict__ pk_status,
                                                    const uint32_t* __restrict__ pk_lines,
                                                    const uint32_t* __restrict__ neg_g2_lines,
                                                    uint32_t* __restrict__ l2buf) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * (uint32_t)(MILLER_LINES * 4)) return;
-  const uint32_t job = t / (MILLER_LINES * 4);
-  const uint32_t r = t % (MILLER_LINES * 4);
-  const uint32_t idx = r >> 2, q = (r >> 1) & 1, comp = r & 1;
+  if (t >= n * (uint32_t)(MILLER_LINES * 2)) return;
+  const uint32_t job = t / (MILLER_LINES * 2);
+  const uint32_t r = t % (MILLER_LINES * 2);
+  const uint32_t idx = r >> 1, q = r & 1;
   const PairJob j = jobs[job];
   const uint32_t* qlines = q ? (j.pk_b == PAIR_NEG_G2 ? neg_g2_lines
                                                        : pk_lines + (size_t)j.pk_b * PK_LINES_WORDS)
                              : pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS;
   const uint32_t* line = qlines + (size_t)idx * MILLER_LINE_WORDS;
-  Fp lam, x, v;
-  fp_load_const(lam, line + comp * FPW);
+  Fp l0, l1, c0, c1, x, m0, m1;
+  fp_load_const(l0, line);
+  fp_load_const(l1, line + FPW);
+  fp_load_const(c0, line + 2 * FPW);
+  fp_load_const(c1, line + 3 * FPW);
   fp_load_const(x, points + (size_t)(q ? j.b : j.a) * G1A_WORDS);
-  fp_mul(v, lam, x);
-  fp_neg(v, v);
-  p12::st(l2buf + (size_t)job * p12::L2_WORDS + ((size_t)idx * 2 + q) * 2 * FPW + comp * FPW, v);
+  fp_mul(m0, l0, x);
+  fp_neg(m0, m0);
+  fp_mul(m1, l1, x);
+  fp_neg(m1, m1);
+  const uint8_t pks = pk_status[j.pk_slot];
+  const uint8_t pkb = j.pk_b == PAIR_NEG_G2 ? 1 : pk_status[j.pk_b];
+  const bool skip = q ? (pinf[j.b] || pkb == 2) : (pinf[j.a] || pks == 2);
+  p12::line_values(l2buf + (size_t)job * p12::L2_WORDS + ((size_t)idx * 2 + q) * p12::LV_N * FPW,
+                   m0, m1, c0, c1, skip);
 }
-
 void launch_pair_lines(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
-                       const uint32_t* pk_lines, const uint32_t* neg_g2_lines, uint32_t* l2buf) {
+                       const uint8_t* pinf, const uint8_t* pk_status, const uint32_t* pk_lines,
+                       const uint32_t* neg_g2_lines, uint32_t* l2buf) {
   if (!n) return;
-  const uint32_t nl = n * (uint32_t)(MILLER_LINES * 4);
-  hipLaunchKernelGGL(k_pair_lines, GRID(nl, 64), 0, s, jobs, n, points, pk_lines, neg_g2_lines,
-                     l2buf);
+  const uint32_t nl = n * (uint32_t)(MILLER_LINES * 2);
+  hipLaunchKernelGGL(k_pair_lines, GRID(nl, 64), 0, s, jobs, n, points, pinf, pk_status, pk_lines,
+                     neg_g2_lines, l2buf);
 }
 
 __global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ jobs, uint32_t n,
                                                const uint32_t* __restrict__ points,
                                                const uint8_t* __restrict__ pinf,
                                                const uint8_t* __restrict__ pk_status,
-                                               const uint32_t* __restrict__ pk_lines,
-                                               const uint32_t* __restrict__ neg_g2_lines,
                                                const uint32_t* __restrict__ l2buf,
                                                uint8_t* __restrict__ status,
                                                const uint8_t* __restrict__ gate,
@@ -482,25 +495,22 @@ __global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ j
     }
     return;
   }
+  // a skipped pairing (identity point or key) has the line 1 in its blocks
+  // (k_pair_lines) and y = 0
   Fp yp0, yp1;
   fp_zero(yp0);
   fp_zero(yp1);
-  bool skip0 = true, skip1 = true;
-  const uint32_t* lines0 = neg_g2_lines;
   const uint32_t* l2 = l2buf;
-  const uint32_t* lines1 = neg_g2_lines;
   if (live) {
-    fp_load_const(yp0, points + (size_t)j.a * G1A_WORDS + FPW);
-    fp_load_const(yp1, points + (size_t)j.b * G1A_WORDS + FPW);
-    if (j.flags & PAIR_NEG_B) fp_neg(yp1, yp1);
-    skip0 = pinf[j.a] || pks == 2;
-    skip1 = pinf[j.b] || pkb == 2;
-    lines0 = pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS;
-    if (j.pk_b != PAIR_NEG_G2) lines1 = pk_lines + (size_t)j.pk_b * PK_LINES_WORDS;
+    if (!(pinf[j.a] || pks == 2)) fp_load_const(yp0, points + (size_t)j.a * G1A_WORDS + FPW);
+    if (!(pinf[j.b] || pkb == 2)) {
+      fp_load_const(yp1, points + (size_t)j.b * G1A_WORDS + FPW);
+      if (j.flags & PAIR_NEG_B) fp_neg(yp1, yp1);
+    }
     l2 = l2buf + (size_t)job * p12::L2_WORDS;
   }
   Fp f, r;
-  p12::miller(f, lines0, lines1, l2, yp0, yp1, skip0, skip1, L);
+  p12::miller(f, l2, yp0, yp1, L);
   p12::final_exp(r, f, L);
   const bool one = p12::is_one(r, L);
   if (mine && lane % 12 == 0) {
@@ -509,13 +519,11 @@ __global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ j
   }
 }
 void launch_pairing(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
-                    const uint8_t* pinf, const uint8_t* pk_status, const uint32_t* pk_lines,
-                    const uint32_t* neg_g2_lines, uint32_t* l2buf, uint8_t* status,
+                    const uint8_t* pinf, const uint8_t* pk_status, uint32_t* l2buf, uint8_t* status,
                     const uint8_t* gate, uint8_t* valid) {
   if (!n) return;
   hipLaunchKernelGGL(k_pairing, dim3((n + p12::CHECKS_PER_BLOCK - 1) / p12::CHECKS_PER_BLOCK),
-                     dim3(64), 0, s, jobs, n, points, pinf, pk_status, pk_lines, neg_g2_lines, l2buf,
-                     status, gate, valid);
+                     dim3(64), 0, s, jobs, n, points, pinf, pk_status, l2buf, status, gate, valid);
 }
 
 // ------------------------------------------------------------------ Sign

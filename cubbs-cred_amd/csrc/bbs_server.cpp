@@ -96,9 +96,11 @@ bool msgs_valid(const CuBbsBbsGpuByteSlice* m, size_t n) {
 // Device bytes a request adds to a batch (staged inputs + work area), an
 // upper estimate: payload twice (blob + job records), per message term
 // about 0.4 KiB (scalar, term records, comb partial share), per item 4 KiB,
-// per variable-base product its table workspace.
-size_t estimate(size_t payload, size_t terms, size_t products) {
-  return 4096 + 2 * payload + 400 * (terms + 4) + 4 * VAR_WS_WORDS * products;
+// per variable-base product its table workspace, per pairing check its line
+// blocks.
+size_t estimate(size_t payload, size_t terms, size_t products, size_t checks = 0) {
+  return 4096 + 2 * payload + 400 * (terms + 4) + 4 * VAR_WS_WORDS * products +
+         4 * PAIR_L2_WORDS * checks;
 }
 
 // Builds an owned request; returns 0 or an error code.
@@ -120,7 +122,7 @@ int32_t copy_request(const CubbsHipRequest& q, Req& r) {
       pk.finish(r, sl);
       r.msgs.assign(sl.begin() + 3, sl.end());
       r.v = CuBbsBbsGpuVerifyItem{sl[0], sl[1], sl[2], r.msgs.data(), r.msgs.size()};
-      r.est = estimate(r.bytes.size(), it.message_count + 2, 3);
+      r.est = estimate(r.bytes.size(), it.message_count + 2, 3, 2);
       return 0;
     }
     case CUBBS_HIP_REQ_PROOF_VERIFY: {
@@ -141,7 +143,7 @@ int32_t copy_request(const CubbsHipRequest& q, Req& r) {
       r.pv = CuBbsBbsGpuProofVerifyItem{sl[0], sl[1], sl[2], sl[3], r.msgs.data(), r.msgs.size(),
                                         CuBbsBbsGpuIndexSlice{r.idx.data(), r.idx.size()},
                                         it.total_message_count};
-      r.est = estimate(r.bytes.size(), it.total_message_count + 8, 6);
+      r.est = estimate(r.bytes.size(), it.total_message_count + 8, 6, 2);
       return 0;
     }
     case CUBBS_HIP_REQ_SIGN: {

@@ -36,9 +36,14 @@ constexpr int CHECKS_PER_BLOCK = 5;
 // LDS of a 64-thread block: 5 checks and one shared zero entry (the 4 spare
 // lanes alias the last check's region and never store)
 constexpr int BLOCK_LDS_WORDS = CHECKS_PER_BLOCK * LDS_WORDS + E;
-// per check: l2 = -lambda x_P for 68 lines x 2 pairings x 2 components
-constexpr int L2_WORDS = MILLER_LINES * 2 * 2 * E;
-constexpr int LINE_WORDS = 4 * E;  // a prepared line: lambda, c in Fp2
+// Per check, line and pairing: the second factors of the sparse line
+// product, so that the Miller loop forms no derived values.  With
+// m = -lambda x_P (Fp2) and c the line's constant (Fp2):
+//   M0 = m0, M1 = m1, NM1 = 2p - m1, D = m0 + 2p - m1, NS = 4p - m0 - m1,
+//   S = m0 + m1, C0 = c0, C1 = c1, NC1 = 2p - c1.
+// A skipped pairing (identity point or key) stores the line 1.
+enum { LV_M0, LV_M1, LV_NM1, LV_D, LV_NS, LV_S, LV_C0, LV_C1, LV_NC1, LV_N };
+constexpr int L2_WORDS = MILLER_LINES * 2 * LV_N * E;
 
 // LDS words (address space 3, so accesses compile to ds_* instructions even
 // across the non-inlined operations below)
@@ -217,62 +222,73 @@ DEV void sqr(Fp& f, const Lane& L) {
   w_reduce(f, w);
 }
 
-// f <- f * (l0 + l2 w^2 + yp w^3): l0 = (c0, c1), l2 = (m0, m1) below 2p + 1,
-// yp in Fp (the evaluated Miller-loop line, see pairing.hpp).
-DEV void mul_line(Fp& f, const Fp& c0, const Fp& c1, const Fp& m0, const Fp& m1, const Fp& yp,
-                  const Lane& L) {
+// Fill one block of LV_N values (see L2_WORDS) for line constant (c0, c1)
+// and m = (m0, m1), all below 2p + 1.
+DEV void line_values(uint32_t* v, const Fp& m0, const Fp& m1, const Fp& c0, const Fp& c1,
+                     bool skip) {
+  Fp z, one, a0, a1, b0, b1, t;
+  fp_zero(z);
+  fp_one(one);
+  sel(a0, skip, z, m0);
+  sel(a1, skip, z, m1);
+  sel(b0, skip, one, c0);
+  sel(b1, skip, z, c1);
+  st(v + LV_M0 * E, a0);
+  st(v + LV_M1 * E, a1);
+  lz5(t, z, z, 2, a1, z);
+  st(v + LV_NM1 * E, t);
+  lz5(t, a0, z, 2, a1, z);
+  st(v + LV_D * E, t);
+  lz5(t, z, z, 4, a0, a1);
+  st(v + LV_NS * E, t);
+  lz5(t, a0, a1, 0, z, z);
+  st(v + LV_S * E, t);
+  st(v + LV_C0 * E, b0);
+  st(v + LV_C1 * E, b1);
+  lz5(t, z, z, 2, b1, z);
+  st(v + LV_NC1 * E, t);
+}
+
+// f <- f * (l0 + l2 w^2 + yp w^3) for one prepared line, yp in Fp (the
+// evaluated Miller-loop line, see pairing.hpp), the other second factors
+// read from the line's block v:
+//   w^0: f_k * l0             c=0: (c0, 2p - c1)            c=1: (c1, c0)
+//   w^2: f_(k-2) * l2         c=0: (m0, 2p - m1)            c=1: (m1, m0)
+//        times xi when k < 2  c=0: (m0 + 2p - m1, 4p - m0 - m1)  c=1: (m0 + m1, m0 + 2p - m1)
+//   w^3: f_(k-3) * yp, times xi when k < 3
+// The factors are loaded before the barrier so the loads overlap put_rich.
+DEV void mul_line(Fp& f, const uint32_t* v, const Fp& yp, const Lane& L) {
+  const bool xi = L.k < 2;
+  Fp a0, a1, b0, b1;
+  ld(a0, v + (L.c ? LV_C1 : LV_C0) * E);
+  ld(a1, v + (L.c ? LV_C0 : LV_NC1) * E);
+  ld(b0, v + (xi ? (L.c ? LV_S : LV_D) : (L.c ? LV_M1 : LV_M0)) * E);
+  ld(b1, v + (xi ? (L.c ? LV_D : LV_NS) : (L.c ? LV_M0 : LV_NM1)) * E);
   __syncthreads();
   put_rich(L.F, f, L);
   __syncthreads();
-  Fp z, a, b, t1, t2;
-  fp_zero(z);
   Wide w;
   w_zero(w);
-  // term w^0: f_k * l0 (plain)
   {
-    Fp x0, x1, y0, y1, n1;
+    Fp x0, x1;
     ld(x0, L.F + L.k * SLOT + EY0 * E);
     ld(x1, L.F + L.k * SLOT + EY1 * E);
-    lz5(n1, z, z, 2, c1, z);  // 2p - c1
-    sel(y0, L.c, c1, c0);
-    sel(y1, L.c, c0, n1);
-    w_mac(w, x0, y0);
-    w_mac(w, x1, y1);
+    w_mac(w, x0, a0);
+    w_mac(w, x1, a1);
   }
-  // term w^2: f_{k-2} * l2, times xi when k < 2.  Second factors:
-  //   plain c=0: (m0, 2p - m1)                 c=1: (m1, m0)
-  //   xi    c=0: (m0 + 2p - m1, 4p - m0 - m1)  c=1: (m0 + m1, m0 + 2p - m1)
   {
-    const bool xi = L.k < 2;
-    const bool cx = L.c && xi;
     const int i = xi ? L.k + 4 : L.k - 2;
-    Fp s1, s2, y0, y1, x0, x1, u0, u1;
-    // t1 = (xi ? m0 : 0) + (cx ? m1 : 0) + (cx ? 0 : 2p) - (cx ? 0 : m1)
-    sel(a, xi, m0, z);
-    sel(b, cx, m1, z);
-    sel(s1, cx, z, m1);
-    lz5(t1, a, b, cx ? 0 : 2, s1, z);
-    // t2 = (cx ? m0 : 0) + (xi ? (c ? 2p : 4p) : 0) - (xi ? (c ? m1 : m0) : 0) - (xi && !c ? m1 : 0)
-    sel(a, cx, m0, z);
-    sel(u0, L.c, m1, m0);
-    sel(s1, xi, u0, z);
-    sel(s2, xi && !L.c, m1, z);
-    lz5(t2, a, z, xi ? (L.c ? 2 : 4) : 0, s1, s2);
-    sel(u0, L.c, m1, m0);
-    sel(u1, L.c, m0, t1);
-    sel(y0, xi, t1, u0);
-    sel(y1, xi, t2, u1);
+    Fp x0, x1;
     ld(x0, L.F + i * SLOT + EY0 * E);
     ld(x1, L.F + i * SLOT + EY1 * E);
-    w_mac(w, x0, y0);
-    w_mac(w, x1, y1);
+    w_mac(w, x0, b0);
+    w_mac(w, x1, b1);
   }
-  // term w^3: f_{k-3} * yp, times xi when k < 3
   {
-    const bool xi = L.k < 3;
-    const int i = xi ? L.k + 3 : L.k - 3;
+    const bool xi3 = L.k < 3;
+    const int i = xi3 ? L.k + 3 : L.k - 3;
     Fp x;
-    ld(x, L.F + i * SLOT + ysel0(L.c, xi) * E);
+    ld(x, L.F + i * SLOT + ysel0(L.c, xi3) * E);
     w_mac(w, x, yp);
   }
   w_reduce(f, w);
@@ -524,11 +540,10 @@ DEVN void final_exp(Fp& r, const Fp& f, Lane L) {
 }
 
 // Product of the two Miller loops e(P0, Q0) e(P1, Q1) with prepared lines:
-// lines_q holds the 68 (lambda, c) of Q_q, l2 the per-check values
-// -lambda x_Pq ([line][q][component]), yp_q the y of P_q.  A skipped pair
-// contributes the constant line 1.  Conjugated at the end (x < 0).
-DEVN void miller(Fp& f, const uint32_t* lines0, const uint32_t* lines1, const uint32_t* l2,
-                 const Fp& yp0, const Fp& yp1, bool skip0, bool skip1, Lane L) {
+// l2 holds the per-check line blocks ([line][q], line_values()), yp_q the y
+// of P_q, zero for a skipped pair (its blocks hold the line 1).  Conjugated
+// at the end (x < 0).
+DEVN void miller(Fp& f, const uint32_t* l2, const Fp& yp0, const Fp& yp1, Lane L) {
   Fp one, z;
   fp_one(one);
   fp_zero(z);
@@ -541,22 +556,8 @@ DEVN void miller(Fp& f, const uint32_t* lines0, const uint32_t* lines1, const ui
 #pragma unroll 1
     for (int s = 0; s < steps; s++) {
 #pragma unroll 1
-      for (int q = 0; q < 2; q++) {
-        const bool skip = q ? skip1 : skip0;
-        const uint32_t* ln = (q ? lines1 : lines0) + (size_t)idx * LINE_WORDS;
-        const uint32_t* m = l2 + ((size_t)idx * 2 + q) * 2 * E;
-        Fp c0, c1, m0, m1, yp, t;
-        ld(t, ln + 2 * E);  // c.c0 (after lambda.c0, lambda.c1)
-        sel(c0, skip, one, t);
-        ld(t, ln + 3 * E);
-        sel(c1, skip, z, t);
-        ld(t, m);
-        sel(m0, skip, z, t);
-        ld(t, m + E);
-        sel(m1, skip, z, t);
-        sel(yp, skip, z, q ? yp1 : yp0);
-        mul_line(f, c0, c1, m0, m1, yp, L);
-      }
+      for (int q = 0; q < 2; q++)
+        mul_line(f, l2 + ((size_t)idx * 2 + q) * LV_N * E, q ? yp1 : yp0, L);
       idx++;
     }
   }
