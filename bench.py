@@ -330,6 +330,52 @@ def per_request_gpu(g, vreqs, n=16):
     return n / (time.time() - t)
 
 
+def codec_run(n=2048, reps=5):
+    """Host throughput of the bbs-2023 proofValue codec (the wire format a
+    ProofVerify request arrives in): serialize and parse n derived proofValues
+    of the ProofVerify workload's shape (1040-byte proof: L = 32, 24 hidden;
+    four label-map entries, 4 + 4 disclosed indexes, 64-byte presentation
+    header), timed as direct C-ABI calls with a preallocated arena, one host
+    thread."""
+    import ctypes
+    import proofvalue as pv
+    lib = pv.lib()
+    rng = random.Random(5)
+    vals = [pv.DerivedProofValue(rng.randbytes(1040), {"c14n%d" % i: "b%d" % (3 - i) for i in range(4)},
+                                 [0, 1, 2, 3], [9, 14, 20, 27], rng.randbytes(64)) for _ in range(n)]
+    strs = [pv.serialize_derived(v).encode() for v in vals]
+    # the C structs of the inputs, built once
+    keep, cvals = [], []
+    for v in vals:
+        k = pv._Keep()
+        labels = (pv.CLabel * 4)(*[pv.CLabel(k.slice(a.encode()), k.slice(b.encode()))
+                                   for a, b in v.label_map.items()])
+        man = (ctypes.c_uint64 * 4)(*v.mandatory_indexes)
+        sel = (ctypes.c_uint64 * 4)(*v.selective_indexes)
+        cvals.append(pv.CDerived(k.slice(v.bbs_proof), labels, 4, man, 4, sel, 4, k.slice(v.presentation_header)))
+        keep.append((k, labels, man, sel))
+    need = ctypes.c_size_t(0)
+    out_buf = ctypes.create_string_buffer(4096)
+    arena = ctypes.create_string_buffer(8192)
+    cout = pv.CDerived()
+    ser, par = lib.cubbs_proofvalue_serialize_derived, lib.cubbs_proofvalue_parse_derived
+    best_s = best_p = float("inf")
+    for _ in range(reps):
+        t = time.perf_counter()
+        for c in cvals:
+            if ser(ctypes.byref(c), out_buf, 4096, ctypes.byref(need)) != 0:
+                raise SystemExit("proofValue serialize failed")
+        best_s = min(best_s, time.perf_counter() - t)
+        t = time.perf_counter()
+        for sv in strs:
+            if par(sv, len(sv), arena, 8192, ctypes.byref(need), ctypes.byref(cout)) != 0:
+                raise SystemExit("proofValue parse failed")
+        best_p = min(best_p, time.perf_counter() - t)
+    return {"items": n, "proof_value_chars": len(strs[0]), "threads": 1,
+            "serialize_derived_per_s": round(n / best_s), "parse_derived_per_s": round(n / best_p),
+            "timing": "best of %d passes, direct C-ABI calls (ctypes call overhead included)" % reps}
+
+
 PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                        "bench_kernels_verify8192_proofverify2048.md")
 
@@ -515,6 +561,8 @@ def main():
         log("msm run %.1fs: %s" % (time.time() - t, {k: out["msm"][k] for k in ("msm_per_s", "ms_per_step", "comb_4096x32_ms", "pippenger_2_16_ms")}))
     if not args.profile_only:
         out["per_request_gpu_ops_s"] = round(per_request_gpu(g, vreqs), 2)
+        out["proofvalue_codec"] = codec_run()
+        log("proofValue codec: %s" % out["proofvalue_codec"])
         if not args.no_serving:
             t = time.time()
             out["latency"] = serving_child(args)

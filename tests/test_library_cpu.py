@@ -19,7 +19,7 @@ import cubbs_hip  # noqa: E402
 
 def header_functions():
     names = []
-    for h in ("cubbs_bbs_gpu_abi.h", "cubbs_hip_ext.h"):
+    for h in ("cubbs_bbs_gpu_abi.h", "cubbs_hip_ext.h", "cubbs_proofvalue.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         names += re.findall(r"\b(cubbs_\w+)\s*\(", text)
