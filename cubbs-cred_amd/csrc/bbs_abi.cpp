@@ -476,6 +476,14 @@ int32_t cubbs_hip_batch_shape(void* batch, uint64_t* items, uint64_t* msm_terms,
   return CUBBS_BBS_GPU_SUCCESS;
 }
 
+int32_t cubbs_hip_batch_group_checks(void* batch, uint64_t* groups, uint64_t* passed) {
+  if (!batch) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  Batch* b = static_cast<Batch*>(batch);
+  if (groups) *groups = b->n_agg_slots;
+  if (passed) *passed = b->agg_passed;
+  return CUBBS_BBS_GPU_SUCCESS;
+}
+
 int32_t cubbs_hip_batch_profile(void* batch, int iters, char* names_out, size_t names_cap,
                                 double* ms_out, int max_kernels, double* total_ms_out) {
   if (!batch || iters <= 0) return -CUBBS_BBS_GPU_ERR_INVALID_INPUT;

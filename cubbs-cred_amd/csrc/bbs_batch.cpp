@@ -1083,11 +1083,15 @@ void launch_batch(Batch& b) {
 }
 
 void fetch_results(Batch& b) {
-  launch_copy(b.stream, b.h_res, b.status, b.nv);
-  launch_copy(b.stream, b.h_res + b.res_stride, b.valid, b.nv);
+  // items, then the aggregated group checks (status and verdict slots)
+  const uint32_t slots = b.nv + b.n_agg_slots;
+  launch_copy(b.stream, b.h_res, b.status, slots);
+  launch_copy(b.stream, b.h_res + b.res_stride, b.valid, slots);
   launch_copy(b.stream, b.h_res + 2 * b.res_stride, b.out, b.out_bytes);
   check(hipGetLastError());
   check(hipStreamSynchronize(b.stream));
+  b.agg_passed = 0;
+  for (uint32_t g = 0; g < b.n_agg_slots; g++) b.agg_passed += b.h_res[b.res_stride + b.nv + g] == 1;
 }
 
 // Copy results back and fill the caller's result array.  For Sign and

@@ -101,6 +101,7 @@ struct Batch {
   PairJob* pairs = nullptr;
   VarJob* agg_var = nullptr;          // products of the aggregated pairing checks
   uint32_t n_agg_var = 0, n_agg_slots = 0, n_gpairs = 0;
+  uint32_t agg_passed = 0;  // group checks that held at the last fetch of results
   struct Level {
     ReduceGroup* groups;
     uint32_t* out;

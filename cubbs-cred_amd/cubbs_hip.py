@@ -61,6 +61,7 @@ EXT_SYMBOLS = [
     "cubbs_hip_tune_load",
     "cubbs_hip_batch_profile",
     "cubbs_hip_batch_shape",
+    "cubbs_hip_batch_group_checks",
     "cubbs_hip_test_g1_mul",
     "cubbs_hip_test_pairing_gt",
     "cubbs_hip_test_g2_roundtrip",
@@ -233,6 +234,7 @@ def lib():
                                                  ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int,
                                                  ctypes.c_void_p]
         _lib.cubbs_hip_batch_shape.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+        _lib.cubbs_hip_batch_group_checks.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 2
         _lib.cubbs_hip_server_create.restype = ctypes.c_void_p
         _lib.cubbs_hip_server_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
         _lib.cubbs_hip_server_submit.restype = ctypes.c_int64
@@ -597,6 +599,13 @@ class StagedBatch:
         v = [ctypes.c_uint64() for _ in range(4)]
         lib().cubbs_hip_batch_shape(self.h, *[ctypes.byref(x) for x in v])
         return dict(zip(["items", "msm_terms", "comb_madds", "pairings"], [x.value for x in v]))
+
+    def group_checks(self):
+        """(aggregated group checks in the batch, how many held at the last
+        results fetch)."""
+        v = [ctypes.c_uint64() for _ in range(2)]
+        lib().cubbs_hip_batch_group_checks(self.h, *[ctypes.byref(x) for x in v])
+        return v[0].value, v[1].value
 
     def free(self):
         if self.h:

@@ -75,6 +75,12 @@ int32_t cubbs_hip_batch_profile(void *batch, int iters, char *names_out, size_t 
 int32_t cubbs_hip_batch_shape(void *batch, uint64_t *items, uint64_t *msm_terms,
                               uint64_t *comb_madds, uint64_t *pairings);
 
+/* Aggregated checks of a staged Verify / ProofVerify batch
+ * (CUBBS_HIP_AGG_GROUP at prepare time): the number of group checks it
+ * holds, and how many of them held at the last cubbs_hip_batch_results.
+ * Items of a group that held are accepted without their own pairing check. */
+int32_t cubbs_hip_batch_group_checks(void *batch, uint64_t *groups, uint64_t *passed);
+
 /* ---- serving loop: continuous batching of mixed requests.
  * `slots` worker slots, each with its own stream and reusable device/pinned
  * memory; a free slot takes every pending request up to max_batch and its

@@ -112,10 +112,52 @@ def test_aggregated_staged_batch_graph_replays(gpu, oracle):
         b.launch()
         b.sync()
         assert b.results() == expect
+        # 20 items per key in groups of 8, 8, 4; items 5, 18 and 31 are
+        # tampered and sit in three different groups: the other three hold
+        assert b.group_checks() == (6, 3)
         b.use_graph(True)
         for _ in range(3):
             b.launch()
             b.sync()
             assert b.results() == expect
+    finally:
+        b.free()
+
+
+def test_aggregated_groups_hold_for_valid_proofs(gpu, oracle):
+    """Every group check of an all-valid ProofVerify batch holds, so no item
+    needs its own pairing; one tampered proof fails its group only."""
+    g = gpu
+    rng = random.Random(31)
+    kps = _issuers(g, rng, 2)
+    sreqs = [(kps[i % 2][0], kps[i % 2][1], rng.randbytes(8), [rng.randbytes(24) for _ in range(5)])
+             for i in range(32)]
+    rc, sigs = g.sign_batch(sreqs)
+    assert rc == 0
+    greqs = [(pk, sig, h, rng.randbytes(10), m, [0, 3]) for (sk, pk, h, m), (_, sig) in zip(sreqs, sigs)]
+    rc, proofs = g.proof_gen_batch(greqs, seeds=[rng.randbytes(32) for _ in greqs])
+    assert rc == 0
+    reqs = [(pk, proof, h, ph, [m[0], m[3]], [0, 3], 5)
+            for (pk, sig, h, ph, m, idx), (_, proof) in zip(greqs, proofs)]
+    b = _with_group(8, lambda: g.StagedBatch(g.BATCH_PROOF_VERIFY, reqs))
+    try:
+        b.launch()
+        b.sync()
+        assert b.results() == [(0, True)] * 32
+        assert b.group_checks() == (4, 4)
+    finally:
+        b.free()
+    bad = list(reqs)
+    pk, proof, h, ph, disc, idx, L = bad[6]
+    # Abar of another proof under the same key: a valid point for which the
+    # pairing equation fails (and so does the challenge)
+    bad[6] = (pk, reqs[8][1][:48] + proof[48:], h, ph, disc, idx, L)
+    b = _with_group(8, lambda: g.StagedBatch(g.BATCH_PROOF_VERIFY, bad))
+    try:
+        b.launch()
+        b.sync()
+        res = b.results()
+        assert [v for _, v in res] == [i != 6 for i in range(32)]
+        assert b.group_checks() == (4, 3)
     finally:
         b.free()
