@@ -204,3 +204,22 @@ def test_point_codec_rejections(oracle):
             enc[0] |= 0x80
             assert not oracle.g1_valid(bytes(enc))
             break
+
+
+def test_max_messages_fixture_inputs(oracle):
+    """tests/golden/max_messages/l65536.json: the key comes from its IKM and
+    the header follows the reference's 65,536-message test
+    (crates/cubbs-bbs/src/tests.rs:919-921); the signature's e is canonical."""
+    import json
+    import os
+    from conftest import ROOT
+    v = json.load(open(os.path.join(ROOT, "tests", "golden", "max_messages", "l65536.json")))
+    sk, pk = oracle.keygen(bytes.fromhex(v["ikm"]))
+    assert (sk.hex(), pk.hex()) == (v["sk"], v["pk"])
+    hdr = bytes.fromhex(v["header"])
+    assert len(hdr) == 64 and hdr[:8] == v["L"].to_bytes(8, "big") and hdr[8:33] == b"omniid-route4-correctness"
+    assert hdr[33:] == bytes(31)
+    sig = bytes.fromhex(v["signature"])
+    assert len(sig) == 80 and oracle.g1_valid(sig[:48])
+    assert int.from_bytes(sig[48:], "big") < R_ORDER
+    assert v["proof_len"] == 272 + 32 * (v["L"] // 2)
