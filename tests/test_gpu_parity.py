@@ -418,3 +418,28 @@ def test_points_outside_subgroup_rejected(gpu, oracle):
     assert rc == 0
     assert res[0] == (0, True)
     assert [c for c, _ in res[1:]] == [1, 1, 1]
+
+
+def test_zero_messages_all_operations(gpu, oracle):
+    """L = 0: a signature and a proof over no messages (R = U = 0), with and
+    without header / presentation header, against the oracle."""
+    rng = random.Random(404)
+    (sk, pk), = _issuers(gpu, rng, 1)
+    cases = [(b"", b""), (b"hdr", b""), (b"", b"ph"), (rnd_bytes(rng, 64), rnd_bytes(rng, 64))]
+    rc, sigs = gpu.sign_batch([(sk, pk, h, []) for h, _ in cases])
+    assert rc == 0
+    for (h, _), (code, sig) in zip(cases, sigs):
+        assert (code, sig) == (0, oracle.sign(sk, pk, h, []))
+    rc, res = gpu.verify_batch([(pk, sig, h, []) for (h, _), (_, sig) in zip(cases, sigs)] +
+                               [(pk, sigs[0][1], b"x", [])])
+    assert (rc, res) == (0, [(0, True)] * 4 + [(0, False)])
+    seeds = [rnd_bytes(rng, 32) for _ in cases]
+    greqs = [(pk, sig, h, ph, [], []) for (h, ph), (_, sig) in zip(cases, sigs)]
+    rc, proofs = gpu.proof_gen_batch(greqs, seeds=seeds)
+    assert rc == 0
+    for (h, ph), (_, sig), seed, (code, proof) in zip(cases, sigs, seeds, proofs):
+        assert code == 0 and len(proof) == 272
+        assert proof == oracle.proof_gen(pk, sig, h, ph, [], [], oracle.chacha_random_scalars(seed, 5))
+    pv = [(pk, proof, h, ph, [], [], 0) for (h, ph), (_, proof) in zip(cases, proofs)]
+    rc, res = gpu.proof_verify_batch(pv + [(pk, proofs[0][1], b"", b"other", [], [], 0)])
+    assert (rc, res) == (0, [(0, True)] * 4 + [(0, False)])
