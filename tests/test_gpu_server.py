@@ -114,3 +114,35 @@ def test_server_submit_wait_and_item_errors(gpu):
             assert g.lib().cubbs_hip_server_submit(srv.h, ctypes.byref(req)) == -g.ERR_INVALID_INPUT
     finally:
         srv.close()
+
+
+def test_server_hbm_budget_bounds_batches(gpu):
+    """Admission follows the HBM budget: with a share per slot below one
+    request's estimate a batch holds one such request (an oversized request
+    still runs, alone); with room they batch.  Results equal the requests
+    run alone either way."""
+    g = gpu
+    pool = _pool(g, b"server-budget", n_verify=6, n_pv=4, n_sign=3, n_pg=3)
+    alone = [_alone(g, *p) for p in pool]
+    order = list(range(len(pool)))
+    arrivals = [0] * len(order)  # all pending at once
+    for budget, one_each in ((2 * 64 * 1024, True), (2 * (1 << 30), False)):
+        srv = g.Server(slots=2, max_batch=64, hbm_budget=budget)
+        try:
+            rc, res, lat, outs = srv.run_open_loop(pool, order, arrivals, outputs=True)
+            assert rc == 0
+            for i, k in enumerate(order):
+                _, code, value, out = alone[k]
+                assert (res[i], outs[i]) == ((code, value), out), (budget, i)
+            st = srv.stats()
+            assert st["fault"] == 0 and st["requests"] == len(order)
+            if one_each:
+                # each Verify, ProofVerify and ProofGen request (pairing line
+                # blocks or product tables) exceeds the share and runs alone;
+                # the 3 Sign requests are estimated small and may share one
+                n_sign = sum(1 for k in order if pool[k][0] == g.REQ_SIGN)
+                assert len(order) - n_sign + 1 <= st["batches"] <= len(order)
+            else:
+                assert st["batches"] < len(order)
+        finally:
+            srv.close()
