@@ -487,7 +487,7 @@ uint8_t* aligned(void* arena) {
 
 }  // namespace
 
-extern "C" int32_t cubbs_proofvalue_serialize_base(const cubbs_pv_base* value, char* out,
+static int32_t serialize_base_impl(const cubbs_pv_base* value, char* out,
                                                    size_t out_cap, size_t* out_len) {
   if (!value || !out_len) return CUBBS_PV_ERR_ARGUMENT;
   *out_len = 0;
@@ -512,7 +512,7 @@ extern "C" int32_t cubbs_proofvalue_serialize_base(const cubbs_pv_base* value, c
   return emit(w.b, out, out_cap, out_len);
 }
 
-extern "C" int32_t cubbs_proofvalue_serialize_derived(const cubbs_pv_derived* value, char* out,
+static int32_t serialize_derived_impl(const cubbs_pv_derived* value, char* out,
                                                       size_t out_cap, size_t* out_len) {
   if (!value || !out_len) return CUBBS_PV_ERR_ARGUMENT;
   *out_len = 0;
@@ -603,7 +603,7 @@ bool label_less(const Label& a, const Label& b) {
 
 }  // namespace
 
-extern "C" int32_t cubbs_proofvalue_parse_base(const char* proof_value, size_t len, void* arena,
+static int32_t parse_base_impl(const char* proof_value, size_t len, void* arena,
                                                size_t arena_cap, size_t* arena_needed, cubbs_pv_base* out) {
   if (!arena_needed || !out || (!proof_value && len)) return CUBBS_PV_ERR_ARGUMENT;
   *arena_needed = 0;
@@ -639,7 +639,7 @@ extern "C" int32_t cubbs_proofvalue_parse_base(const char* proof_value, size_t l
   return CUBBS_PV_OK;
 }
 
-extern "C" int32_t cubbs_proofvalue_parse_derived(const char* proof_value, size_t len, void* arena,
+static int32_t parse_derived_impl(const char* proof_value, size_t len, void* arena,
                                                   size_t arena_cap, size_t* arena_needed,
                                                   cubbs_pv_derived* out) {
   if (!arena_needed || !out || (!proof_value && len)) return CUBBS_PV_ERR_ARGUMENT;
@@ -707,4 +707,33 @@ extern "C" int32_t cubbs_proofvalue_parse_derived(const char* proof_value, size_
   out->n_selective_indexes = t.selective.size();
   out->presentation_header = a.copy(ph);
   return CUBBS_PV_OK;
+}
+
+// The C entries: no C++ exception crosses the boundary (the only one the
+// codec can raise is an allocation failure).
+template <class F>
+static int32_t guarded(F&& f) {
+  try {
+    return f();
+  } catch (...) {
+    return CUBBS_PV_ERR_INTERNAL;
+  }
+}
+
+extern "C" int32_t cubbs_proofvalue_serialize_base(const cubbs_pv_base* value, char* out,
+                                                   size_t out_cap, size_t* out_len) {
+  return guarded([&] { return serialize_base_impl(value, out, out_cap, out_len); });
+}
+extern "C" int32_t cubbs_proofvalue_serialize_derived(const cubbs_pv_derived* value, char* out,
+                                                      size_t out_cap, size_t* out_len) {
+  return guarded([&] { return serialize_derived_impl(value, out, out_cap, out_len); });
+}
+extern "C" int32_t cubbs_proofvalue_parse_base(const char* proof_value, size_t len, void* arena,
+                                               size_t arena_cap, size_t* arena_needed, cubbs_pv_base* out) {
+  return guarded([&] { return parse_base_impl(proof_value, len, arena, arena_cap, arena_needed, out); });
+}
+extern "C" int32_t cubbs_proofvalue_parse_derived(const char* proof_value, size_t len, void* arena,
+                                                  size_t arena_cap, size_t* arena_needed,
+                                                  cubbs_pv_derived* out) {
+  return guarded([&] { return parse_derived_impl(proof_value, len, arena, arena_cap, arena_needed, out); });
 }

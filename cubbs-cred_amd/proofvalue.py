@@ -14,11 +14,11 @@ from typing import Dict, List
 import cubbs_hip
 
 OK, ERR_MULTIBASE, ERR_CBOR_PREFIX, ERR_INVALID_BASE, ERR_INVALID_DERIVED, ERR_BASE64, ERR_CBOR, \
-    ERR_INSUFFICIENT_OUTPUT, ERR_ARGUMENT = range(9)
+    ERR_INSUFFICIENT_OUTPUT, ERR_ARGUMENT, ERR_INTERNAL = range(10)
 KINDS = {ERR_MULTIBASE: "InvalidMultibasePrefix", ERR_CBOR_PREFIX: "InvalidCborPrefix",
          ERR_INVALID_BASE: "InvalidBaseProof", ERR_INVALID_DERIVED: "InvalidDerivedProof",
          ERR_BASE64: "Base64", ERR_CBOR: "Cbor", ERR_INSUFFICIENT_OUTPUT: "InsufficientOutput",
-         ERR_ARGUMENT: "Argument"}
+         ERR_ARGUMENT: "Argument", ERR_INTERNAL: "Internal"}
 SYMBOLS = ["cubbs_proofvalue_serialize_base", "cubbs_proofvalue_parse_base",
            "cubbs_proofvalue_serialize_derived", "cubbs_proofvalue_parse_derived"]
 

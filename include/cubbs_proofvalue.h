@@ -17,8 +17,9 @@
  * Label maps travel compressed: "c14n<k>" -> "b<v>" becomes k -> v
  * (lib.rs:165-189).
  *
- * Host code only; no call touches a GPU.  Every entry is thread-safe and
- * keeps no state between calls. */
+ * Host code only; no call touches a GPU.  Every entry is thread-safe; the
+ * only memory kept between calls is per-thread scratch capacity, whose
+ * contents never carry over from one call to the next. */
 #ifndef CUBBS_PROOFVALUE_H
 #define CUBBS_PROOFVALUE_H
 
@@ -43,6 +44,7 @@ extern "C" {
 #define CUBBS_PV_ERR_ARGUMENT 8            /* NULL pointer with a non-zero
                                               length, or a text field that is
                                               not UTF-8 (a Rust String is) */
+#define CUBBS_PV_ERR_INTERNAL 9            /* host memory exhausted */
 
 /* A byte or UTF-8 text run.  ptr may be NULL when len is 0. */
 typedef struct {
