@@ -270,7 +270,7 @@ int32_t cubbs_bbs_gpu_backend_info_v2(uint32_t abi_version, CuBbsBbsGpuBackendIn
     init_locked(c);
     info_out->hardware_acceleration = 1;
     return CUBBS_BBS_GPU_SUCCESS;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }
@@ -285,7 +285,7 @@ int32_t cubbs_bbs_gpu_verify_batch_v2(uint32_t abi_version, const CuBbsBbsGpuVer
     launch_batch(*b);
     collect_batch(*b, results, nullptr, nullptr);
     return CUBBS_BBS_GPU_SUCCESS;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }
@@ -301,7 +301,7 @@ int32_t cubbs_bbs_gpu_proof_verify_batch_v2(uint32_t abi_version,
     launch_batch(*b);
     collect_batch(*b, results, nullptr, nullptr);
     return CUBBS_BBS_GPU_SUCCESS;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }
@@ -321,7 +321,7 @@ int32_t cubbs_bbs_gpu_sign_batch_v2(uint32_t abi_version, CuBbsBbsGpuSignItem* i
     for (size_t i = 0; i < item_count; i++)
       if (results[i].code == CUBBS_BBS_GPU_SUCCESS) items[i].signature_out.len = lens[i];
     return CUBBS_BBS_GPU_SUCCESS;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }
@@ -343,7 +343,7 @@ static int32_t proof_gen_common(uint32_t abi_version, CuBbsBbsGpuProofGenItem* i
     for (size_t i = 0; i < item_count; i++)
       if (results[i].code == CUBBS_BBS_GPU_SUCCESS) items[i].proof_out.len = lens[i];
     return CUBBS_BBS_GPU_SUCCESS;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }
@@ -393,7 +393,7 @@ void* cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void* items,
     if (b->stream) check(hipStreamSynchronize(b->stream));
     st = CUBBS_BBS_GPU_SUCCESS;
     return b.release();
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     st = CUBBS_BBS_GPU_ERR_INTERNAL;
     return nullptr;
   }
@@ -408,7 +408,7 @@ int32_t cubbs_hip_batch_launch(void* batch) {
     else
       launch_batch(*b);
     return CUBBS_BBS_GPU_SUCCESS;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }
@@ -430,7 +430,7 @@ int32_t cubbs_hip_batch_results(void* batch, CuBbsBbsGpuResult* results) {
     else
       collect_batch(*b, results, nullptr, nullptr);
     return CUBBS_BBS_GPU_SUCCESS;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }
@@ -450,7 +450,7 @@ int32_t cubbs_hip_batch_use_graph(void* batch, int32_t on) {
       b->graph = nullptr;
     }
     return CUBBS_BBS_GPU_SUCCESS;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }
@@ -528,7 +528,7 @@ int32_t cubbs_hip_batch_profile(void* batch, int iters, char* names_out, size_t 
     }
     t.on = false;
     return nk;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return -CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }
@@ -581,7 +581,7 @@ static int32_t packed_hash(int kind, const uint8_t* packed, const size_t* offset
     check(hipFree(d));
     if (kind == 2 && record) tl_last_scalars.assign(out, out + 32 * n);
     return CUBBS_BBS_GPU_SUCCESS;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }
@@ -760,7 +760,7 @@ static int32_t run_probe(size_t in_bytes, const void* in, size_t out_bytes, void
     check(hipFree(d));
     check(hipStreamDestroy(s));
     return CUBBS_BBS_GPU_SUCCESS;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }
@@ -811,7 +811,7 @@ int32_t cubbs_hip_generators(size_t count, uint8_t* out48) {
     ensure_generators_locked(c, (uint32_t)count);
     check(hipMemcpy(out48, c.gen_comp, 48 * count, hipMemcpyDeviceToHost));
     return CUBBS_BBS_GPU_SUCCESS;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }
@@ -850,7 +850,7 @@ static int32_t time_probe(uint32_t lanes, uint32_t iters, double* ms,
     check(hipFree(d));
     check(hipStreamDestroy(s));
     return CUBBS_BBS_GPU_SUCCESS;
-  } catch (const GpuError&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }

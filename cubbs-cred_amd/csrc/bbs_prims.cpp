@@ -277,9 +277,7 @@ void compile_common(Prog& g, const CuBbsBbsGpuG1MsmCommonPointsBatchItem* items,
 int32_t guard(const std::function<int32_t()>& f) {
   try {
     return f();
-  } catch (const GpuError&) {
-    return CUBBS_BBS_GPU_ERR_INTERNAL;
-  } catch (const std::bad_alloc&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
 }

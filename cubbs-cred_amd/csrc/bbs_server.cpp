@@ -436,7 +436,11 @@ void* cubbs_hip_server_create(const CubbsHipServerConfig* cfg, int32_t* status_o
 
 int64_t cubbs_hip_server_submit(void* srv, const CubbsHipRequest* req) {
   if (!srv || !req) return -(int64_t)CUBBS_BBS_GPU_ERR_INVALID_INPUT;
-  return cubbs::submit_at(static_cast<Server*>(srv), req, cubbs::now_ns());
+  try {
+    return cubbs::submit_at(static_cast<Server*>(srv), req, cubbs::now_ns());
+  } catch (...) {  // host memory exhausted while copying the request
+    return -(int64_t)CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
 }
 
 int32_t cubbs_hip_server_wait(void* srv, int64_t ticket, CuBbsBbsGpuResult* result, uint8_t* out,
@@ -451,11 +455,18 @@ int32_t cubbs_hip_server_run_open_loop(void* srv, const CubbsHipRequest* reqs,
                                        uint8_t* const* outs, const size_t* out_caps) {
   if (!srv || (n && (!reqs || !arrival_ns || !results || !latency_ns)))
     return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  // arrivals must be in order: checked before anything is submitted
+  for (size_t i = 1; i < n; i++)
+    if (arrival_ns[i] < arrival_ns[i - 1]) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   auto* s = static_cast<Server*>(srv);
-  std::vector<int64_t> tickets(n);
+  std::vector<int64_t> tickets;
+  try {
+    tickets.resize(n);
+  } catch (...) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
   const uint64_t t0 = cubbs::now_ns();
   for (size_t i = 0; i < n; i++) {
-    if (i && arrival_ns[i] < arrival_ns[i - 1]) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
     const uint64_t due = t0 + arrival_ns[i];
     uint64_t t = cubbs::now_ns();
     if (t < due) {
@@ -464,7 +475,11 @@ int32_t cubbs_hip_server_run_open_loop(void* srv, const CubbsHipRequest* reqs,
       }
     }
     // latency counts from the scheduled arrival, so a late submission is charged
-    tickets[i] = cubbs::submit_at(s, &reqs[i], due);
+    try {
+      tickets[i] = cubbs::submit_at(s, &reqs[i], due);
+    } catch (...) {  // host memory exhausted: this request fails alone
+      tickets[i] = -(int64_t)CUBBS_BBS_GPU_ERR_INTERNAL;
+    }
   }
   int32_t rc = CUBBS_BBS_GPU_SUCCESS;
   for (size_t i = 0; i < n; i++) {

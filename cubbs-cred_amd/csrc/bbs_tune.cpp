@@ -217,9 +217,7 @@ int32_t cubbs_hip_autotune(uint32_t max_log2_terms, char* report, size_t report_
       std::snprintf(line, sizeof line, "comb log2_terms=%u chunk=%u ms=%.3f\n", k, bch, best);
       rep += line;
     }
-  } catch (const GpuError&) {
-    return CUBBS_BBS_GPU_ERR_INTERNAL;
-  } catch (const std::bad_alloc&) {
+  } catch (...) {  // a GPU error, or host memory exhausted
     return CUBBS_BBS_GPU_ERR_INTERNAL;
   }
   if (report && report_cap) {

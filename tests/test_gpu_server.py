@@ -146,3 +146,20 @@ def test_server_hbm_budget_bounds_batches(gpu):
                 assert st["batches"] < len(order)
         finally:
             srv.close()
+
+
+def test_server_open_loop_rejects_unordered_arrivals(gpu):
+    """Arrival times must not decrease; the call is rejected before any
+    request is submitted, and the server keeps serving afterwards."""
+    g = gpu
+    pool = _pool(g, b"server-order", n_verify=2, n_pv=1, n_sign=1, n_pg=1)
+    srv = g.Server(slots=2)
+    try:
+        rc, _, _, _ = srv.run_open_loop(pool, [0, 1, 2], [0, 2_000_000, 1_000_000])
+        assert rc == g.ERR_INVALID_INPUT
+        assert srv.stats()["requests"] == 0
+        alone = [_alone(g, *p) for p in pool]
+        rc, res, _, _ = srv.run_open_loop(pool, list(range(len(pool))), [0] * len(pool))
+        assert rc == 0 and res == [(c, v) for _, c, v, _ in alone]
+    finally:
+        srv.close()
