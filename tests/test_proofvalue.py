@@ -431,3 +431,23 @@ def test_c_abi_buffers_and_arguments():
         PV.ERR_INSUFFICIENT_OUTPUT
     payload = 3 + 1 + (2 + 80) + (2 + 64) + (2 + 96) + (2 + 32) + 1
     assert need.value == 1 + len(O.b64url_encode(bytes(payload)))
+
+
+def test_codec_under_sanitizers(tmp_path):
+    """The codec's C entries over 60k mutated inputs, built with g++ under
+    AddressSanitizer and UBSan (tests/native/proofvalue_fuzz.cpp): no
+    report, and both decode errors and successes occur."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    exe = str(tmp_path / "pv_fuzz")
+    subprocess.check_call(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined",
+                           "-fno-sanitize-recover=all", "-I", os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "native", "proofvalue_fuzz.cpp"),
+                           os.path.join(ROOT, "cubbs-cred_amd", "csrc", "proofvalue.cpp"), "-o", exe])
+    out = subprocess.run([exe, "60000"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-3000:]
+    counts = [int(x) for x in out.stdout.split()]
+    assert len(counts) == 10 and sum(counts) == 60000
+    assert counts[0] > 0 and counts[6] > 0   # successes and CBOR errors both occur
