@@ -407,6 +407,8 @@ def main():
     ap.add_argument("--serving-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--profile-only", action="store_true", help="a few launches, no baselines")
     ap.add_argument("--no-msm", action="store_true", help="skip the batched-MSM measurement")
+    ap.add_argument("--msm-only", action="store_true",
+                    help="only the batched-MSM measurement (after the measured tuning): for profiling")
     ap.add_argument("--no-graph", action="store_true", help="launch kernel by kernel instead of HIP graphs")
     ap.add_argument("--no-autotune", action="store_true", help="skip the measured launch-shape tuning")
     ap.add_argument("--inflight", type=int, default=INFLIGHT,
@@ -456,6 +458,16 @@ def main():
     rc, kind, hw = g.backend_info()
     if rc != 0 or not hw:
         raise SystemExit("no usable HIP device (backend_info rc=%d)" % rc)
+    if args.msm_only:
+        out = {"workload": "batched MSMs only"}
+        if not args.no_autotune:
+            rc, rep = g.autotune(16)
+            if rc != 0:
+                raise SystemExit("autotune failed rc=%d" % rc)
+            out["tuning"] = rep.strip().split("\n")
+        out["msm"] = msm_run(g, steps=args.steps, warmup=args.warmup, graph=not args.no_graph)
+        print(json.dumps(out), flush=True)
+        return
     t0 = time.time()
     vreqs, expect_v, pvreqs, kps, abi_ops = build_workload(g)
     log("workload built in %.1fs" % (time.time() - t0))
