@@ -369,6 +369,20 @@ int32_t cubbs_hip_proof_gen_with_scalars_batch(uint32_t abi_version,
                           results);
 }
 
+void* cubbs_hip_batch_prepare_flags(uint32_t abi_version, int kind, const void* items,
+                                    size_t item_count, uint32_t flags, int32_t* status_out) {
+  int32_t dummy;
+  int32_t& st = status_out ? *status_out : dummy;
+  if (flags & ~(uint32_t)CUBBS_HIP_BATCH_SINGLE_STREAM) {
+    st = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+    return nullptr;
+  }
+  set_thread_single_stream((flags & CUBBS_HIP_BATCH_SINGLE_STREAM) != 0);
+  void* b = cubbs_hip_batch_prepare(abi_version, kind, items, item_count, status_out);
+  set_thread_single_stream(false);
+  return b;
+}
+
 void* cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void* items, size_t item_count,
                               int32_t* status_out) {
   int32_t dummy;

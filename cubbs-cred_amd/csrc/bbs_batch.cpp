@@ -50,6 +50,8 @@ uint32_t comb_chunk_terms(uint64_t batch_terms, bool tuned) {
 // ------------------------------------------------------------------ arenas
 static thread_local Arena* tl_arena = nullptr;
 void set_thread_arena(Arena* a) { tl_arena = a; }
+static thread_local bool tl_single_stream = false;
+void set_thread_single_stream(bool on) { tl_single_stream = on; }
 
 static uint8_t* arena_take(std::vector<Arena::Block>& blocks, size_t& held, size_t bytes,
                            size_t min_block, bool host) {
@@ -301,7 +303,7 @@ void materialize(Batch& b, Plan& p) {
       Context& c = ctx();
       std::lock_guard<std::mutex> g(c.mu);
       b.stream = next_stream_locked(c);
-      b.aux = next_stream_locked(c);
+      b.aux = tl_single_stream ? nullptr : next_stream_locked(c);
     }
     check(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
     check(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));

@@ -61,6 +61,10 @@ struct Arena {
 // While set on a thread, batches planned on that thread take their memory and
 // stream from the arena instead of allocating their own.
 void set_thread_arena(Arena* a);
+// Batches this thread prepares next run on one stream (no auxiliary stream
+// for the decode / product branch): for callers that keep many batches in
+// flight, where the fork's second queue costs more than it hides.
+void set_thread_single_stream(bool on);
 
 struct Batch {
   int kind = 0;

@@ -17,6 +17,7 @@ ABI_VERSION = 2
 SUCCESS, ERR_INVALID_INPUT, ERR_INSUFFICIENT_OUTPUT, ERR_INTERNAL = 0, 1, 2, 3
 BATCH_VERIFY, BATCH_PROOF_VERIFY = 0, 1
 BATCH_G1_MSM, BATCH_G1_MSM_COMMON_POINTS = 16, 17
+BATCH_SINGLE_STREAM = 1
 
 API_ID = b"BBS_BLS12381G1_XMD:SHA-256_SSWU_RO_H2G_HM2S_"
 MAP_DST = API_ID + b"MAP_MSG_TO_SCALAR_AS_HASH_"
@@ -50,6 +51,7 @@ ABI_SYMBOLS = [
 ]
 EXT_SYMBOLS = [
     "cubbs_hip_batch_prepare",
+    "cubbs_hip_batch_prepare_flags",
     "cubbs_hip_batch_launch",
     "cubbs_hip_batch_sync",
     "cubbs_hip_batch_results",
@@ -222,6 +224,10 @@ def lib():
         _lib.cubbs_hip_batch_prepare.restype = ctypes.c_void_p
         _lib.cubbs_hip_batch_prepare.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
                                                  ctypes.c_size_t, ctypes.POINTER(ctypes.c_int32)]
+        _lib.cubbs_hip_batch_prepare_flags.restype = ctypes.c_void_p
+        _lib.cubbs_hip_batch_prepare_flags.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
+                                                       ctypes.c_size_t, ctypes.c_uint32,
+                                                       ctypes.POINTER(ctypes.c_int32)]
         for name in ("cubbs_hip_batch_launch", "cubbs_hip_batch_sync"):
             getattr(_lib, name).argtypes = [ctypes.c_void_p]
         _lib.cubbs_hip_batch_results.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -521,7 +527,7 @@ def mad_peak_bench(lanes, iters):
 class StagedBatch:
     """A batch whose inputs stay resident on the device (prepare once, launch many)."""
 
-    def __init__(self, kind, reqs):
+    def __init__(self, kind, reqs, single_stream=False):
         self.keep = _Keep()
         if kind == BATCH_VERIFY:
             items = build_verify_items(reqs, self.keep)
@@ -537,8 +543,10 @@ class StagedBatch:
         self.kind = kind
         # results: one per item, or one per output row of the common-point groups
         self.n = sum(len(rows) for _, rows in reqs) if kind == BATCH_G1_MSM_COMMON_POINTS else len(reqs)
-        self.h = lib().cubbs_hip_batch_prepare(ABI_VERSION, kind, ctypes.cast(items, ctypes.c_void_p),
-                                               ctypes.c_size_t(len(reqs)), ctypes.byref(st))
+        self.h = lib().cubbs_hip_batch_prepare_flags(ABI_VERSION, kind, ctypes.cast(items, ctypes.c_void_p),
+                                                     ctypes.c_size_t(len(reqs)),
+                                                     BATCH_SINGLE_STREAM if single_stream else 0,
+                                                     ctypes.byref(st))
         self.keep = None  # items are only read during prepare
         if not self.h:
             raise RuntimeError("batch prepare failed with status %d" % st.value)

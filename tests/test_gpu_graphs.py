@@ -88,17 +88,20 @@ def test_graph_replays_match_eager(gpu, oracle):
         mb.free()
 
 
-def test_batch_sets_in_flight_match_alone(gpu, oracle):
-    """Several batches in flight, as in the benchmark: four staged Verify +
-    ProofVerify sets replayed from graphs and all launched back to back
-    before any is waited for, over several rounds; every set's results equal
-    those of one set run alone and the oracle's verdicts."""
+@pytest.mark.parametrize("single_stream,nsets", [(False, 4), (True, 8)])
+def test_batch_sets_in_flight_match_alone(gpu, oracle, single_stream, nsets):
+    """Several batches in flight, as in the benchmark: staged Verify +
+    ProofVerify sets (forked, or on one stream each) replayed from graphs and
+    all launched back to back before any is waited for, over several rounds;
+    every set's results equal those of one set run alone and the oracle's
+    verdicts."""
     g = gpu
     vreqs, pvreqs = _verify_and_proof_batches(g)
     expect_v = [(0, bool(oracle.verify(*r))) for r in vreqs]
     expect_pv = [(0, bool(oracle.proof_verify(*r))) for r in pvreqs]
-    sets = [(g.StagedBatch(g.BATCH_VERIFY, vreqs), g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs))
-            for _ in range(4)]
+    sets = [(g.StagedBatch(g.BATCH_VERIFY, vreqs, single_stream=single_stream),
+             g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs, single_stream=single_stream))
+            for _ in range(nsets)]
     try:
         alone = _run(sets[0][0]), _run(sets[0][1])
         assert alone == (expect_v, expect_pv)
