@@ -373,13 +373,15 @@ void* cubbs_hip_batch_prepare_flags(uint32_t abi_version, int kind, const void* 
                                     size_t item_count, uint32_t flags, int32_t* status_out) {
   int32_t dummy;
   int32_t& st = status_out ? *status_out : dummy;
-  if (flags & ~(uint32_t)CUBBS_HIP_BATCH_SINGLE_STREAM) {
+  if (flags & ~(uint32_t)(CUBBS_HIP_BATCH_SINGLE_STREAM | CUBBS_HIP_BATCH_AGGREGATE)) {
     st = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
     return nullptr;
   }
   set_thread_single_stream((flags & CUBBS_HIP_BATCH_SINGLE_STREAM) != 0);
+  set_thread_agg_group((flags & CUBBS_HIP_BATCH_AGGREGATE) ? 8u : 0u);
   void* b = cubbs_hip_batch_prepare(abi_version, kind, items, item_count, status_out);
   set_thread_single_stream(false);
+  set_thread_agg_group(0);
   return b;
 }
 

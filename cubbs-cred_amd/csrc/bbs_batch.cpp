@@ -333,7 +333,10 @@ void materialize(Batch& b, Plan& p) {
 // one check's latency however few checks it holds, so the extra combined-check
 // launch (and the re-check launch when a group fails) outweighs the pairings
 // saved on batches that already fill the GPU.
-uint32_t agg_group_max() { return env_u32("CUBBS_HIP_AGG_GROUP", 0); }
+static thread_local uint32_t tl_agg_group = 0;
+void set_thread_agg_group(uint32_t n) { tl_agg_group = n; }
+// CUBBS_HIP_AGG_GROUP overrides; else the preparing call's choice (0 = off)
+uint32_t agg_group_max() { return env_u32("CUBBS_HIP_AGG_GROUP", tl_agg_group); }
 
 static void agg_coefficient(uint8_t out[32], uint64_t s, uint64_t t) {
   // s + t * lambda, lambda = GLV_LAMBDA (127.4 bits): below 2^192 < r

@@ -65,6 +65,9 @@ void set_thread_arena(Arena* a);
 // for the decode / product branch): for callers that keep many batches in
 // flight, where the fork's second queue costs more than it hides.
 void set_thread_single_stream(bool on);
+// Group size of the aggregated pairing checks of the Verify / ProofVerify
+// batches this thread prepares next (0 = per-item checks only).
+void set_thread_agg_group(uint32_t n);
 
 struct Batch {
   int kind = 0;

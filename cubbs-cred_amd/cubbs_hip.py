@@ -17,7 +17,7 @@ ABI_VERSION = 2
 SUCCESS, ERR_INVALID_INPUT, ERR_INSUFFICIENT_OUTPUT, ERR_INTERNAL = 0, 1, 2, 3
 BATCH_VERIFY, BATCH_PROOF_VERIFY = 0, 1
 BATCH_G1_MSM, BATCH_G1_MSM_COMMON_POINTS = 16, 17
-BATCH_SINGLE_STREAM = 1
+BATCH_SINGLE_STREAM, BATCH_AGGREGATE = 1, 2
 
 API_ID = b"BBS_BLS12381G1_XMD:SHA-256_SSWU_RO_H2G_HM2S_"
 MAP_DST = API_ID + b"MAP_MSG_TO_SCALAR_AS_HASH_"
@@ -527,7 +527,7 @@ def mad_peak_bench(lanes, iters):
 class StagedBatch:
     """A batch whose inputs stay resident on the device (prepare once, launch many)."""
 
-    def __init__(self, kind, reqs, single_stream=False):
+    def __init__(self, kind, reqs, single_stream=False, aggregate=False):
         self.keep = _Keep()
         if kind == BATCH_VERIFY:
             items = build_verify_items(reqs, self.keep)
@@ -545,7 +545,8 @@ class StagedBatch:
         self.n = sum(len(rows) for _, rows in reqs) if kind == BATCH_G1_MSM_COMMON_POINTS else len(reqs)
         self.h = lib().cubbs_hip_batch_prepare_flags(ABI_VERSION, kind, ctypes.cast(items, ctypes.c_void_p),
                                                      ctypes.c_size_t(len(reqs)),
-                                                     BATCH_SINGLE_STREAM if single_stream else 0,
+                                                     (BATCH_SINGLE_STREAM if single_stream else 0) |
+                                                     (BATCH_AGGREGATE if aggregate else 0),
                                                      ctypes.byref(st))
         self.keep = None  # items are only read during prepare
         if not self.h:

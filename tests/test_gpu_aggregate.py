@@ -161,3 +161,36 @@ def test_aggregated_groups_hold_for_valid_proofs(gpu, oracle):
         assert b.group_checks() == (4, 3)
     finally:
         b.free()
+
+
+def test_aggregate_prepare_flag(gpu, oracle):
+    """CUBBS_HIP_BATCH_AGGREGATE at prepare (no environment variable) forms
+    groups of 8 per key; results equal the per-item batch's; an unknown flag
+    bit is rejected."""
+    import ctypes
+    g = gpu
+    assert os.environ.get("CUBBS_HIP_AGG_GROUP") is None
+    rng = random.Random(91)
+    kps = _issuers(g, rng, 2)
+    sreqs = [(kps[i % 2][0], kps[i % 2][1], b"h", [rng.randbytes(16)]) for i in range(33)]
+    rc, sigs = g.sign_batch(sreqs)
+    assert rc == 0
+    reqs = [(pk, sig, h, ([b"bad"] if i == 20 else m)) for i, ((sk, pk, h, m), (_, sig)) in
+            enumerate(zip(sreqs, sigs))]
+    plain = g.StagedBatch(g.BATCH_VERIFY, reqs)
+    agg = g.StagedBatch(g.BATCH_VERIFY, reqs, single_stream=True, aggregate=True)
+    try:
+        for b in (plain, agg):
+            b.launch()
+            b.sync()
+        assert agg.results() == plain.results() == [(0, i != 20) for i in range(33)]
+        # 17 + 16 items per key in groups of at most 8: 8 + 8 (+ one item
+        # checked alone) and 8 + 8; item 20 (key 0) fails its group only
+        assert plain.group_checks() == (0, 0)
+        assert agg.group_checks() == (4, 3)
+    finally:
+        plain.free()
+        agg.free()
+    st = ctypes.c_int32(0)
+    h = g.lib().cubbs_hip_batch_prepare_flags(g.ABI_VERSION, g.BATCH_VERIFY, None, 0, 4, ctypes.byref(st))
+    assert not h and st.value == g.ERR_INVALID_INPUT

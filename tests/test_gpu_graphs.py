@@ -88,8 +88,8 @@ def test_graph_replays_match_eager(gpu, oracle):
         mb.free()
 
 
-@pytest.mark.parametrize("single_stream,nsets", [(False, 4), (True, 8)])
-def test_batch_sets_in_flight_match_alone(gpu, oracle, single_stream, nsets):
+@pytest.mark.parametrize("single_stream,aggregate,nsets", [(False, False, 4), (True, False, 8), (True, True, 8)])
+def test_batch_sets_in_flight_match_alone(gpu, oracle, single_stream, aggregate, nsets):
     """Several batches in flight, as in the benchmark: staged Verify +
     ProofVerify sets (forked, or on one stream each) replayed from graphs and
     all launched back to back before any is waited for, over several rounds;
@@ -99,8 +99,8 @@ def test_batch_sets_in_flight_match_alone(gpu, oracle, single_stream, nsets):
     vreqs, pvreqs = _verify_and_proof_batches(g)
     expect_v = [(0, bool(oracle.verify(*r))) for r in vreqs]
     expect_pv = [(0, bool(oracle.proof_verify(*r))) for r in pvreqs]
-    sets = [(g.StagedBatch(g.BATCH_VERIFY, vreqs, single_stream=single_stream),
-             g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs, single_stream=single_stream))
+    sets = [(g.StagedBatch(g.BATCH_VERIFY, vreqs, single_stream=single_stream, aggregate=aggregate),
+             g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs, single_stream=single_stream, aggregate=aggregate))
             for _ in range(nsets)]
     try:
         alone = _run(sets[0][0]), _run(sets[0][1])
