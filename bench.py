@@ -11,12 +11,22 @@ inputs resident in HBM):
     seeded ChaCha20.
 All inputs (keys, signatures, proofs) are produced by the library itself
 (keygen, Sign and ProofGen run on the device) from fixed seeds.  The timed
-region keeps four staged sets of both batches in flight (each set's Verify
-and ProofVerify pipelines on streams of their own): each step waits for the
-set launched four steps before, compares its per-item results with the
-expected ones, and launches it again; after the last step every set is waited
-for.  While one set's pairing launches finish, the next sets' decode, hashing
-and MSM stages already run.
+region keeps eight staged sets of both batches in flight, each batch on one
+stream of its own (single-stream batches) with its pairing checks aggregated
+per key in groups of 8 (random linear combinations, per-item checks for the
+items of failing groups), each replayed from its captured HIP graph: each
+step waits for the set launched eight steps before, compares its per-item
+results with the expected ones, and launches it again; after the last step
+every set is waited for.  While one set's pairing launches finish, the next
+sets' decode, hashing and MSM stages already run.
+
+The roofline object prices the timed step's work with the cost model in
+cost_model.json (field multiplications per algorithmic unit, units counted by
+the library for each batch: comb additions, GLV products, aggregation
+products, decoded points, reduction additions, pairing checks actually
+computed) and divides it by the measured step time and the measured
+v_mad_u64_u32 peak of the device.  A secondary object gives the same for the
+pairing kernel alone on a batch of 8192 per-item checks.
 
 After the timed steps, a serving run pushes an open-loop Poisson stream
 (20k req/s, 40% Verify / 30% ProofVerify / 20% Sign / 10% ProofGen, 64 issuers,
@@ -59,12 +69,20 @@ N_VERIFY = 8192
 N_ISSUERS = 256
 N_PROOFS = 2048
 PROOF_L = 32
-# algorithmic work model (Fp multiplications) of one 2-pairing check with
-# prepared G2 lines: two Miller loops (~5,200 FpM each) and one final
-# exponentiation (~11,000 FpM); one FpM is valued at 300 v_mad_u64_u32 (the
-# canonical 12x32-bit CIOS count; the library's 14x28-bit product uses 392).
-FPM_PER_PAIRING_CHECK = 2 * 5200 + 11000
-MADS_PER_FPM = 300
+# algorithmic work model: Fp multiplications per unit of work, and
+# v_mad_u64_u32 per Fp multiplication
+with open(os.path.join(ROOT, "cost_model.json")) as _f:
+    COST = json.load(_f)
+FPM_PER_PAIRING_CHECK = COST["fpm"]["pair_check"]
+MADS_PER_FPM = COST["mads_per_fpm"]
+
+
+def work_fpm(w):
+    """Field multiplications of one batch launch from its work units."""
+    f = COST["fpm"]
+    return (w["comb_madds"] * f["comb_madd"] + w["glv_products"] * f["glv_product"] +
+            w["agg_products"] * f["agg_product"] + w["decoded_points"] * f["decoded_point"] +
+            w["point_adds"] * f["point_add"] + w["pair_checks_run"] * f["pair_check"])
 
 
 def log(msg):
@@ -228,7 +246,7 @@ def cpu_baseline(vreqs, pvreqs, budget_s=20.0):
                       "%d threads)" % (nv, len(sample) - nv, threads)}
 
 
-def serving_run(g, rate=20000.0, warm_s=3.0, steady_s=15.0, seed=b"cubbs-hip/serving", slots=0):
+def serving_run(g, rate=20000.0, warm_s=5.0, steady_s=30.0, seed=b"cubbs-hip/serving", slots=0):
     """Open-loop serving load through the library's continuous-batching server:
     Poisson arrivals at `rate` req/s, mix 40% Verify / 30% ProofVerify / 20% Sign
     / 10% ProofGen, 64 issuer keys, message count log-uniform in [1, 128] with
@@ -378,22 +396,29 @@ def codec_run(n=2048, reps=5):
             "timing": "best of %d passes, direct C-ABI calls (ctypes call overhead included)" % reps}
 
 
-PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                       "bench_kernels_verify8192_proofverify2048.md")
+PROFILE_STEP = os.path.join(ROOT, "profiles", "step_verify8192_proofverify2048_8sets_aggregated.md")
+PROFILE_PAIRING = os.path.join(ROOT, "profiles", "pairing_verify8192_per_item.md")
 
 
-def profiled_traffic():
-    """HBM bytes per Verify k_pairing launch, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration written
-    there); None when the summary is absent."""
+def profiled(path, key):
+    """A figure from a committed rocprofv3 summary (the line `key: value`),
+    None when the summary or the line is absent."""
     try:
-        with open(PROFILE) as f:
+        with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return int(line.split(":", 1)[1])
+                if line.startswith(key + ":"):
+                    return int(float(line.split(":", 1)[1]))
     except OSError:
         pass
     return None
+
+
+def mark():
+    """A one-block probe kernel launched (and waited for) just outside the
+    timed region, so that a kernel trace of this run can find the region's
+    dispatches between the two marker dispatches."""
+    import cubbs_hip as g
+    g.fp_mul_bench(64, 1)
 
 
 def main():
@@ -404,7 +429,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency run")
     ap.add_argument("--serving-rate", type=float, default=20000.0)
-    ap.add_argument("--serving-seconds", type=float, default=15.0)
+    ap.add_argument("--serving-seconds", type=float, default=30.0)
     ap.add_argument("--serving-slots", type=int, default=0, help="server slots (0 = library default)")
     ap.add_argument("--serving-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--profile-only", action="store_true", help="a few launches, no baselines")
@@ -512,6 +537,7 @@ def main():
         if (v.results_raw(), p.results_raw()) != expect[i]:
             raise SystemExit("results of a timed step differ from the expected ones")
 
+    mark()
     t = time.perf_counter()
     for step in range(args.steps):
         i = step % nset
@@ -522,8 +548,13 @@ def main():
     for step in range(max(0, args.steps - nset), args.steps):
         finish(step % nset)
     el = time.perf_counter() - t
+    mark()
     ms_per_step = el * 1e3 / args.steps
     ops = N_VERIFY + N_PROOFS
+    # the algorithmic work of one step: both batches of a set (every set holds
+    # the same items; pair_checks_run counts the checks of their last launch)
+    step_work = [vb.work(), pb.work()]
+    step_fpm = sum(work_fpm(w) for w in step_work)
     value = ops * args.steps / el
     for v, p in sets[1:]:
         v.free()
@@ -534,9 +565,9 @@ def main():
     pt, pk = pb.profile(2)
     log("verify kernels (ms): %s" % {k: round(v, 3) for k, v in vk.items()})
     log("proof-verify kernels (ms): %s" % {k: round(v, 3) for k, v in pk.items()})
-    # dominant kernel: the pairing launch of a Verify batch checked item by
-    # item (the timed sets aggregate their checks, so their per-item launch
-    # holds only the items of failing groups): 8192 full checks
+    # the pairing kernel alone, on a Verify batch checked item by item (the
+    # timed sets aggregate their checks, so their per-item launches hold only
+    # the items of failing groups): 8192 full checks
     rb = g.StagedBatch(g.BATCH_VERIFY, vreqs)
     _, rk = rb.profile(2)
     rb.free()
@@ -544,12 +575,26 @@ def main():
     checks = N_VERIFY
     rc, mad_ms = g.mad_peak_bench(1 << 20, 256)
     peak_mad_s = (1 << 20) * 256 * 64 / (mad_ms * 1e-3)
-    achieved_mad_s = checks * FPM_PER_PAIRING_CHECK * MADS_PER_FPM / (pairing_ms * 1e-3)
-    roofline = {"bound": "valu", "kernel": "k_pairing (Verify batch of 8192 per-item checks)",
-                "achieved": round(achieved_mad_s / 1e12, 3), "peak": round(peak_mad_s / 1e12, 3),
-                "unit": "Tmad/s (v_mad_u64_u32)", "frac": round(achieved_mad_s / peak_mad_s, 4),
-                "traffic": profiled_traffic(), "launch_ms": round(pairing_ms, 3),
-                "work_per_launch": "%d pairing checks x %d FpM x %d mads" % (checks, FPM_PER_PAIRING_CHECK, MADS_PER_FPM)}
+    step_mad_s = step_fpm * MADS_PER_FPM / (ms_per_step * 1e-3)
+    pair_mad_s = checks * FPM_PER_PAIRING_CHECK * MADS_PER_FPM / (pairing_ms * 1e-3)
+    units = {k: sum(w[k] for w in step_work) for k in step_work[0] if k != "items"}
+    roofline = {"bound": "valu",
+                "kernel": "whole step: every kernel of the timed Verify+ProofVerify sets (comb, GLV and "
+                          "aggregation products, decode, reductions, pairing checks)",
+                "achieved": round(step_mad_s / 1e12, 3), "peak": round(peak_mad_s / 1e12, 3),
+                "unit": "Tmad/s (v_mad_u64_u32)", "frac": round(step_mad_s / peak_mad_s, 4),
+                "traffic": profiled(PROFILE_STEP, "traffic_bytes_per_step"),
+                "traffic_unit": "HBM bytes per step (rocprofv3 FETCH_SIZE/WRITE_SIZE of this configuration)",
+                "work_per_step": "%.4g FpM x %d mads (cost_model.json) over %s" % (step_fpm, MADS_PER_FPM, units),
+                "measured_int64_lane_ops_per_step": profiled(PROFILE_STEP, "int64_lane_ops_per_step"),
+                "peak_source": "k_mad_peak: 2^20 lanes x 256 independent v_mad_u64_u32, HIP events",
+                "pairing_kernel": {
+                    "kernel": "k_pairing on a Verify batch of 8192 per-item checks (secondary)",
+                    "achieved": round(pair_mad_s / 1e12, 3), "frac": round(pair_mad_s / peak_mad_s, 4),
+                    "launch_ms": round(pairing_ms, 3),
+                    "traffic": profiled(PROFILE_PAIRING, "traffic_bytes_per_launch"),
+                    "work_per_launch": "%d checks x %d FpM x %d mads" % (checks, FPM_PER_PAIRING_CHECK,
+                                                                       MADS_PER_FPM)}}
     out = {"metric": "Verify+ProofVerify ops/s on 1 MI355X; p50/p99 latency @ fixed load",
            "value": round(value, 1), "unit": "ops/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,

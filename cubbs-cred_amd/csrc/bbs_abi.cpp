@@ -9,6 +9,9 @@
 #include <hip/hip_runtime.h>
 #include <sys/random.h>
 
+#include <cerrno>
+#include <stdexcept>
+
 #include <algorithm>
 #include <array>
 #include <cstdlib>
@@ -41,9 +44,35 @@ void fill_os_random(uint8_t* out, size_t n) {
   size_t got = 0;
   while (got < n) {
     ssize_t r = getrandom(out + got, n - got, 0);
-    if (r <= 0) throw GpuError{hipErrorUnknown};
+    if (r < 0 && errno == EINTR) continue;  // interrupted by a signal: try again
+    if (r <= 0) throw std::runtime_error("getrandom failed");
     got += (size_t)r;
   }
+}
+
+// ------------------------------------------------------------------ settings
+static uint64_t env_u64(const char* name, uint64_t dflt) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  char* end = nullptr;
+  const unsigned long long x = std::strtoull(v, &end, 10);
+  return end == v ? dflt : (uint64_t)x;
+}
+
+const Knobs& knobs() {
+  static const Knobs k = [] {
+    Knobs d;
+    d.comb_chunk = (uint32_t)std::min<uint64_t>(env_u64("CUBBS_HIP_COMB_CHUNK", 0), 64);
+    d.pippenger_min = (uint32_t)std::min<uint64_t>(env_u64("CUBBS_HIP_PIPPENGER_MIN", 1024), 1u << 30);
+    d.pip_window = (uint32_t)std::min<uint64_t>(env_u64("CUBBS_HIP_PIP_WINDOW", 0), 16);
+    d.pip_piece = (uint32_t)std::min<uint64_t>(env_u64("CUBBS_HIP_PIP_PIECE", 0), 256);
+    d.comb16_slots = (uint32_t)std::min<uint64_t>(env_u64("CUBBS_HIP_COMB16_GENERATORS", COMB16_DEFAULT_SLOTS), 4096);
+    d.hw_queues = (uint32_t)std::min<uint64_t>(env_u64("GPU_MAX_HW_QUEUES", 4), 64);
+    d.pk_cache_entries = (uint32_t)std::max<uint64_t>(16, std::min<uint64_t>(env_u64("CUBBS_HIP_PK_CACHE_ENTRIES", 4096), 1u << 20));
+    d.gen_table_budget = env_u64("CUBBS_HIP_GEN_TABLE_BUDGET_BYTES", 2ull << 30);
+    return d;
+  }();
+  return k;
 }
 
 // ------------------------------------------------------------------ device context
@@ -66,9 +95,7 @@ void init_locked(Context& c) {
   // that they land on distinct queues; batches take them round-robin, so
   // concurrently running batches do not share a queue needlessly.
   {
-    int q = 4;
-    if (const char* e = std::getenv("GPU_MAX_HW_QUEUES")) q = std::atoi(e);
-    q = std::max(2, std::min(q, 32));
+    const int q = std::max(2, std::min((int)knobs().hw_queues, 32));
     c.pool.resize(q);
     for (auto& st : c.pool) check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   }
@@ -95,8 +122,7 @@ hipStream_t next_stream_locked(Context& c) {
 static void ensure_wide_tables_locked(Context& c, uint32_t slots) {
   if (!c.wide_init) {
     c.wide_init = true;
-    const char* v = std::getenv("CUBBS_HIP_COMB16_GENERATORS");
-    c.wide_cap = v && *v ? (uint32_t)std::strtoul(v, nullptr, 10) : COMB16_DEFAULT_SLOTS;
+    c.wide_cap = knobs().comb16_slots;
     if (c.wide_cap && hipMalloc(&c.gen_tables16, (size_t)c.wide_cap * COMB16_TABLE_WORDS * 4) != hipSuccess) {
       (void)hipGetLastError();
       c.gen_tables16 = nullptr;
@@ -373,12 +399,15 @@ void* cubbs_hip_batch_prepare_flags(uint32_t abi_version, int kind, const void* 
                                     size_t item_count, uint32_t flags, int32_t* status_out) {
   int32_t dummy;
   int32_t& st = status_out ? *status_out : dummy;
-  if (flags & ~(uint32_t)(CUBBS_HIP_BATCH_SINGLE_STREAM | CUBBS_HIP_BATCH_AGGREGATE)) {
+  const uint32_t group = (flags >> 8) & 15u;
+  if ((flags & ~(uint32_t)(CUBBS_HIP_BATCH_SINGLE_STREAM | CUBBS_HIP_BATCH_AGGREGATE |
+                           CUBBS_HIP_BATCH_AGG_GROUP(15))) ||
+      (group && (group < 2 || group > 8 || !(flags & CUBBS_HIP_BATCH_AGGREGATE)))) {
     st = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
     return nullptr;
   }
   set_thread_single_stream((flags & CUBBS_HIP_BATCH_SINGLE_STREAM) != 0);
-  set_thread_agg_group((flags & CUBBS_HIP_BATCH_AGGREGATE) ? 8u : 0u);
+  set_thread_agg_group((flags & CUBBS_HIP_BATCH_AGGREGATE) ? (group ? group : 8u) : 0u);
   void* b = cubbs_hip_batch_prepare(abi_version, kind, items, item_count, status_out);
   set_thread_single_stream(false);
   set_thread_agg_group(0);
@@ -489,6 +518,12 @@ int32_t cubbs_hip_batch_shape(void* batch, uint64_t* items, uint64_t* msm_terms,
   if (msm_terms) *msm_terms = b->comb_terms;
   if (comb_madds) *comb_madds = b->comb_madds;
   if (pairings) *pairings = 2ull * b->n_pairs;
+  return CUBBS_BBS_GPU_SUCCESS;
+}
+
+int32_t cubbs_hip_batch_work(void* batch, CubbsHipBatchWork* out) {
+  if (!batch || !out) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  *out = static_cast<Batch*>(batch)->work;
   return CUBBS_BBS_GPU_SUCCESS;
 }
 

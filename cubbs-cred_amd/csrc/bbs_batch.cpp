@@ -23,26 +23,19 @@
 
 namespace cubbs {
 
-static uint32_t env_u32(const char* name, uint32_t def) {
-  const char* v = std::getenv(name);
-  if (!v || !*v) return def;
-  char* end = nullptr;
-  unsigned long x = std::strtoul(v, &end, 10);
-  return end == v ? def : (uint32_t)x;
-}
 static bool slice_ok(const CuBbsBbsGpuByteSlice& s) { return s.len == 0 || s.data != nullptr; }
 
 static const char* kApiId = "BBS_BLS12381G1_XMD:SHA-256_SSWU_RO_H2G_HM2S_";
 
-// Terms per comb lane: CUBBS_HIP_COMB_CHUNK, else (primitive MSM programs
-// only, `tuned`) the tuner's choice for the batch's size class, else 4.  The
-// whole-operation pipelines keep 4: their comb runs beside decode and the GLV
-// products and feeds the reduction, which the comb-only tuning does not see.
+// Terms per comb lane: the calling thread's forced shape, else
+// CUBBS_HIP_COMB_CHUNK, else (primitive MSM programs only, `tuned`) the
+// tuner's choice for the batch's size class, else 4.  The whole-operation
+// pipelines keep 4: their comb runs beside decode and the GLV products and
+// feeds the reduction, which the comb-only tuning does not see.
 uint32_t comb_chunk_terms(uint64_t batch_terms, bool tuned) {
-  const uint32_t env = env_u32("CUBBS_HIP_COMB_CHUNK", 0);
-  if (env) return env;
-  if (!tuned) return 4;
   if (tune_force().comb_chunk) return tune_force().comb_chunk;
+  if (knobs().comb_chunk) return knobs().comb_chunk;
+  if (!tuned) return 4;
   const uint32_t t = tuned_comb_chunk(batch_terms);
   return t ? t : 4;
 }
@@ -119,6 +112,41 @@ static void sort_groups(std::vector<ReduceGroup>& g, std::vector<uint32_t>& out)
     g[i] = z[i].first;
     out[i] = z[i].second;
   }
+}
+
+// SHA-256 compressions of expand_message_xmd(msg, DST, 48) for a message of
+// `len` bytes: b_0 over Z_pad (64) || msg || I2OSP(48, 2) || 0x00 || DST',
+// then b_1 and b_2 over 32 + 1 + |DST'| bytes (RFC 9380 section 5.3.1).
+static uint64_t xmd48_blocks(uint64_t len, uint64_t dst_len) {
+  auto blocks = [](uint64_t n) { return (n + 9 + 63) / 64; };
+  return blocks(64 + len + 3 + dst_len + 1) + 2 * blocks(32 + 1 + dst_len + 1);
+}
+
+// Work units of the plan (cubbs_hip_batch_work).
+static void count_work(Batch& b, const Plan& p) {
+  CubbsHipBatchWork& w = b.work;
+  w.items = b.nv;
+  w.comb_terms = b.comb_terms;
+  w.comb_madds = b.comb_madds;
+  w.glv_products = p.var.size();
+  w.agg_products = p.agg_var.size();
+  w.decoded_points = p.dec_g1.size();
+  uint64_t adds = 0;
+  for (const auto& g : p.groups1) adds += g.count ? g.count - 1 : 0;
+  for (const auto& g : p.groups2) adds += g.count ? g.count - 1 : 0;
+  for (const auto& lv : p.pst_groups)
+    for (const auto& g : lv) adds += g.count ? g.count - 1 : 0;
+  w.point_adds = adds;
+  w.group_checks = p.n_group_pairs;
+  w.pair_checks = p.pairs.size() - p.n_group_pairs;
+  w.pair_checks_run = p.pairs.size();
+  uint64_t sha = 0;
+  for (const auto& j : p.h2s) sha += xmd48_blocks(j.len, p.map_dst.size());
+  // domain input: PK || I2OSP(L, 8) || Q1 || H_1..H_L || api_id || I2OSP(|hdr|, 8) || hdr
+  for (const auto& j : p.dom) sha += xmd48_blocks(96 + 8 + 48ull * (j.L + 1) + 44 + 8 + j.hdr_len, 48);
+  // challenge: I2OSP(R, 8) || (I2OSP(i, 8) || m_i)_R || Abar, Bbar, D, T1, T2 || domain || I2OSP(|ph|, 8) || ph
+  for (const auto& j : p.pvch) sha += xmd48_blocks(8 + 40ull * j.R + 240 + 32 + 8 + j.ph_len, 48);
+  w.sha_blocks = sha;
 }
 
 void materialize(Batch& b, Plan& p) {
@@ -237,6 +265,13 @@ void materialize(Batch& b, Plan& p) {
     for (uint32_t t = c.term_begin; t < c.term_begin + c.term_count; t++)
       b.comb_madds += p.term_gen[t] < b.n16 ? COMB16_WINDOWS : COMB_WINDOWS;
   }
+  count_work(b, p);
+  if (p.n_agg_slots) {
+    b.pair_group.resize(p.pairs.size() - p.n_group_pairs);
+    for (size_t i = p.n_group_pairs; i < p.pairs.size(); i++)
+      b.pair_group[i - p.n_group_pairs] =
+          (p.pairs[i].flags & PAIR_GROUPED) ? p.pairs[i].group : PRIM_NO_SLOT;
+  }
   // work area
   size_t w = 0;
   auto take = [&](size_t bytes) {
@@ -329,14 +364,13 @@ void materialize(Batch& b, Plan& p) {
 // item's equation holds except with probability 2^-128 and the item is
 // accepted without its own pairing; if not, each item is checked on its own.
 // The per-item results are those of the individual checks either way.
-// Off unless CUBBS_HIP_AGG_GROUP (2..8) is set: a pairing launch costs about
-// one check's latency however few checks it holds, so the extra combined-check
-// launch (and the re-check launch when a group fails) outweighs the pairings
-// saved on batches that already fill the GPU.
+// Off unless the preparing call asks for it (the prepare flags): a pairing
+// launch costs about one check's latency however few checks it holds, so the
+// extra combined-check launch (and the re-check launch when a group fails)
+// outweighs the pairings saved unless several batches are in flight.
 static thread_local uint32_t tl_agg_group = 0;
 void set_thread_agg_group(uint32_t n) { tl_agg_group = n; }
-// CUBBS_HIP_AGG_GROUP overrides; else the preparing call's choice (0 = off)
-uint32_t agg_group_max() { return env_u32("CUBBS_HIP_AGG_GROUP", tl_agg_group); }
+uint32_t agg_group_max() { return tl_agg_group; }
 
 static void agg_coefficient(uint8_t out[32], uint64_t s, uint64_t t) {
   // s + t * lambda, lambda = GLV_LAMBDA (127.4 bits): below 2^192 < r
@@ -1097,6 +1131,16 @@ void fetch_results(Batch& b) {
   check(hipStreamSynchronize(b.stream));
   b.agg_passed = 0;
   for (uint32_t g = 0; g < b.n_agg_slots; g++) b.agg_passed += b.h_res[b.res_stride + b.nv + g] == 1;
+  if (b.n_agg_slots) {
+    // per-item checks computed: those whose group check did not hold
+    uint64_t run = b.n_gpairs;
+    const uint32_t n_item = b.n_pairs - b.n_gpairs;
+    for (uint32_t i = 0; i < n_item; i++) {
+      const uint32_t g = b.pair_group[i];
+      run += g == PRIM_NO_SLOT || b.h_res[b.res_stride + g] != 1;
+    }
+    b.work.pair_checks_run = run;
+  }
 }
 
 // Copy results back and fill the caller's result array.  For Sign and

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/cubbs_bbs_gpu_abi.h"
+#include "../../include/cubbs_hip_ext.h"
 #include "bbs_internal.h"
 
 namespace cubbs {
@@ -109,6 +110,7 @@ struct Batch {
   VarJob* agg_var = nullptr;          // products of the aggregated pairing checks
   uint32_t n_agg_var = 0, n_agg_slots = 0, n_gpairs = 0;
   uint32_t agg_passed = 0;  // group checks that held at the last fetch of results
+  std::vector<uint32_t> pair_group;   // per-item checks: status slot of the group (or PRIM_NO_SLOT)
   struct Level {
     ReduceGroup* groups;
     uint32_t* out;
@@ -138,6 +140,7 @@ struct Batch {
            n_pg = 0;
   uint64_t comb_terms = 0;
   uint64_t comb_madds = 0;            // mixed additions of the comb launch (at most)
+  CubbsHipBatchWork work{};           // algorithmic work units (cubbs_hip_batch_work)
   // work arrays (device)
   uint32_t* scalars = nullptr;
   uint32_t* points = nullptr;

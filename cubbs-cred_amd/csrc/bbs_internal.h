@@ -265,11 +265,27 @@ void launch_g2_mul_gen(hipStream_t s, const uint8_t* sk, uint32_t n, uint8_t* pk
 // comb terms per lane by fixed-base terms per batch.  0 / false = no entry.
 bool tuned_pip(uint32_t terms_per_msm, uint32_t& c, uint32_t& S);
 uint32_t tuned_comb_chunk(uint64_t batch_terms);
-// Shapes forced on the calling thread while the tuner times a candidate.
+// Shapes forced on the calling thread while the tuner times a candidate (or
+// by cubbs_hip_test_force_shapes); 0 = not forced.
 struct TuneForce {
   uint32_t pip_c = 0, pip_s = 0, comb_chunk = 0;
 };
 TuneForce& tune_force();
+
+// Process-wide settings from the environment, read once at the first call
+// (never per call: the entry points run on many threads at once, and a
+// getenv racing a setenv is undefined).  0 = the library's own choice.
+struct Knobs {
+  uint32_t comb_chunk = 0;          // CUBBS_HIP_COMB_CHUNK: comb terms per lane
+  uint32_t pippenger_min = 1024;    // CUBBS_HIP_PIPPENGER_MIN: item MSMs from this size
+  uint32_t pip_window = 0;          // CUBBS_HIP_PIP_WINDOW (4..16)
+  uint32_t pip_piece = 0;           // CUBBS_HIP_PIP_PIECE (1..256)
+  uint32_t comb16_slots = 130;      // CUBBS_HIP_COMB16_GENERATORS
+  uint32_t hw_queues = 4;           // GPU_MAX_HW_QUEUES (HIP's own setting)
+  uint32_t pk_cache_entries = 4096; // CUBBS_HIP_PK_CACHE_ENTRIES: resident prepared keys
+  uint64_t gen_table_budget = 2ull << 30;  // CUBBS_HIP_GEN_TABLE_BUDGET_BYTES: 8-bit comb tables
+};
+const Knobs& knobs();
 
 // ------------------------------------------------------------------ Pippenger MSM
 // One term of a large variable-base MSM: points[point] * scalars[scalar],

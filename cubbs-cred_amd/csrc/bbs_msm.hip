@@ -31,13 +31,6 @@ namespace cubbs {
 
 namespace {
 
-uint32_t env_or(const char* name, uint32_t dflt) {
-  const char* v = std::getenv(name);
-  if (!v || !*v) return dflt;
-  long x = std::strtol(v, nullptr, 10);
-  return x > 0 ? (uint32_t)x : dflt;
-}
-
 // Signed c-bit digits of the two 128-bit GLV halves of k: calls
 // f(half, window, bucket = |digit| - 1, negative) for each non-zero digit.
 // Digits lie in (-2^(c-1), 2^(c-1)]; W c >= 129 leaves no final carry.
@@ -91,20 +84,22 @@ PipShape pip_shape(uint32_t terms, uint32_t jobs) {
   jobs = std::max<uint32_t>(jobs, 1);
   const double n = (double)terms / jobs;
   const uint32_t best = pip_model_window(n);
-  // precedence: environment, the tuner's forced candidate, the tuned table,
-  // the model above
+  // precedence: the calling thread's forced shape, the process setting, the
+  // tuned table, the model above
   uint32_t c = best, S = 16, tc, ts;
   if (tuned_pip((uint32_t)n, tc, ts)) {
     c = tc;
     S = ts;
   }
+  if (knobs().pip_window) c = knobs().pip_window;
+  if (knobs().pip_piece) S = knobs().pip_piece;
   if (tune_force().pip_c) c = tune_force().pip_c;
   if (tune_force().pip_s) S = tune_force().pip_s;
-  sh.c = std::min<uint32_t>(16, std::max<uint32_t>(4, env_or("CUBBS_HIP_PIP_WINDOW", c)));
+  sh.c = std::min<uint32_t>(16, std::max<uint32_t>(4, c));
   sh.W = (129 + sh.c - 1) / sh.c;
   sh.K = sh.c - 1;
   sh.B = 1u << sh.K;
-  sh.S = std::min<uint32_t>(256, std::max<uint32_t>(1, env_or("CUBBS_HIP_PIP_PIECE", S)));
+  sh.S = std::min<uint32_t>(256, std::max<uint32_t>(1, S));
   sh.jobs = jobs;
   sh.terms = terms;
   sh.nbt = jobs * sh.W * sh.B;

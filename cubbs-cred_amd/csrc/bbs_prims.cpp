@@ -40,12 +40,7 @@ constexpr size_t LARGE_TERMS = 1024; // route id 3 from this many terms on
 // Item MSMs of at least this many terms run as Pippenger bucket MSMs
 // (CUBBS_HIP_PIPPENGER_MIN, default LARGE_TERMS), smaller ones as one-lane
 // windowed GLV products summed in the partial-sum tree.
-size_t pippenger_min() {
-  const char* v = std::getenv("CUBBS_HIP_PIPPENGER_MIN");
-  if (!v || !*v) return LARGE_TERMS;
-  long x = std::strtol(v, nullptr, 10);
-  return x > 0 ? (size_t)x : LARGE_TERMS;
-}
+size_t pippenger_min() { return knobs().pippenger_min ? knobs().pippenger_min : LARGE_TERMS; }
 constexpr uint8_t ROUTE_ITEM = CUBBS_BBS_MSM_ROUTE_CUBBS_ITEM;
 constexpr uint8_t ROUTE_COMMON = CUBBS_BBS_MSM_ROUTE_CUBBS_COMMON_POINTS;
 constexpr uint8_t ROUTE_LARGE = CUBBS_BBS_MSM_ROUTE_CUBBS_LARGE;

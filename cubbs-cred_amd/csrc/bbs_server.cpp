@@ -212,6 +212,7 @@ struct Server {
   std::atomic<uint64_t> plan_ns{0}, batch_ns{0};
 
   std::atomic<int32_t> fault{0};
+  std::atomic<uint64_t> failed_batches{0};  // batches failed alone (memory, host errors)
 
   void worker(unsigned slot);
   void run_batch(std::vector<Req*>& take, Arena& ar);
@@ -226,6 +227,13 @@ void Server::run_batch(std::vector<Req*>& take, Arena& ar) {
   std::vector<CuBbsBbsGpuSignItem> si;
   std::vector<CuBbsBbsGpuProofGenItem> pgi;
   std::vector<uint8_t> seeds;
+  struct OnDevice {  // counts this batch as on the device until it leaves, by any path
+    std::atomic<uint64_t>& n;
+    ~OnDevice() {
+      --n;
+      set_thread_arena(nullptr);
+    }
+  } guard{on_device};
   const uint64_t od = ++on_device;
   uint64_t m = max_on_device.load();
   while (od > m && !max_on_device.compare_exchange_weak(m, od)) {
@@ -284,7 +292,6 @@ void Server::run_batch(std::vector<Req*>& take, Arena& ar) {
     collect_batch(*bs[k], res.data(), outs.data(), nullptr);
     for (size_t i = 0; i < by[k].size(); i++) by[k][i]->res = res[i];
   }
-  --on_device;
   bs.clear();
   ar.reset();
   plan_ns += t_plan;
@@ -326,20 +333,37 @@ void Server::worker(unsigned slot) {
         pending.pop_front();
       }
     }
+    // A batch that fails for want of memory (device or host) or on a host
+    // error fails alone: its requests get code 3 and the slot goes on.  Any
+    // other device error leaves the device in an unknown state: the server
+    // then fails every later request too (the fault is reported by stats).
+    bool failed = false;
     if (fault == 0) {
       try {
         run_batch(take, ar);
-      } catch (const GpuError&) {
-        fault = CUBBS_BBS_GPU_ERR_INTERNAL;
+      } catch (const GpuError& e) {
+        failed = true;
+        if (e.err != hipErrorOutOfMemory) fault = CUBBS_BBS_GPU_ERR_INTERNAL;
       } catch (...) {
-        fault = CUBBS_BBS_GPU_ERR_INTERNAL;
+        failed = true;
+      }
+      if (failed) {
+        // the batches of the slot may still run: wait for every stream of the
+        // slot before its memory is used again
+        for (int i = 0; i < 4; i++) {
+          if (ar.streams[i]) (void)hipStreamSynchronize(ar.streams[i]);
+          if (ar.aux[i]) (void)hipStreamSynchronize(ar.aux[i]);
+        }
+        (void)hipGetLastError();
+        ar.reset();
+        failed_batches++;
       }
     }
     const uint64_t t = now_ns();
     {
       std::lock_guard<std::mutex> lk(mu);
       for (Req* r : take) {
-        if (fault) {
+        if (fault || failed) {
           r->res = CuBbsBbsGpuResult{CUBBS_BBS_GPU_ERR_INTERNAL, 0, 0};
         }
         r->t_done = t;
@@ -356,6 +380,9 @@ int64_t submit_at(Server* s, const CubbsHipRequest* q, uint64_t t_arrive) {
   auto r = std::make_unique<Req>();
   int32_t rc = copy_request(*q, *r);
   if (rc) return -(int64_t)rc;
+  // a request that alone exceeds a slot's share of the HBM budget is refused
+  // at admission (code 1) rather than failing its batch on the device
+  if (r->est > s->share) return -(int64_t)CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   r->t_arrive = t_arrive;
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return -(int64_t)CUBBS_BBS_GPU_ERR_INTERNAL;

@@ -228,6 +228,13 @@ int32_t cubbs_hip_autotune(uint32_t max_log2_terms, char* report, size_t report_
   return CUBBS_BBS_GPU_SUCCESS;
 }
 
+int32_t cubbs_hip_test_force_shapes(uint32_t pip_c, uint32_t pip_s, uint32_t comb_chunk) {
+  if ((pip_c && (pip_c < 4 || pip_c > 16)) || pip_s > 256 || comb_chunk > 64)
+    return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  tune_force() = TuneForce{pip_c, pip_s, comb_chunk};
+  return CUBBS_BBS_GPU_SUCCESS;
+}
+
 int32_t cubbs_hip_tune_load(const char* text) {
   if (!text) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   Table& t = table();

@@ -241,6 +241,7 @@ def lib():
                                                  ctypes.c_void_p]
         _lib.cubbs_hip_batch_shape.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
         _lib.cubbs_hip_batch_group_checks.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 2
+        _lib.cubbs_hip_batch_work.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _lib.cubbs_hip_server_create.restype = ctypes.c_void_p
         _lib.cubbs_hip_server_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
         _lib.cubbs_hip_server_submit.restype = ctypes.c_int64
@@ -511,6 +512,12 @@ def tune_load(text):
     return lib().cubbs_hip_tune_load(text.encode())
 
 
+def force_shapes(pip_c=0, pip_s=0, comb_chunk=0):
+    """Launch shapes forced on the calling thread for its next calls (0 = not forced)."""
+    return lib().cubbs_hip_test_force_shapes(ctypes.c_uint32(pip_c), ctypes.c_uint32(pip_s),
+                                             ctypes.c_uint32(comb_chunk))
+
+
 def fp_mul_bench(lanes, iters):
     ms = ctypes.c_double()
     rc = lib().cubbs_hip_fp_mul_bench(ctypes.c_uint32(lanes), ctypes.c_uint32(iters), ctypes.byref(ms))
@@ -528,6 +535,7 @@ class StagedBatch:
     """A batch whose inputs stay resident on the device (prepare once, launch many)."""
 
     def __init__(self, kind, reqs, single_stream=False, aggregate=False):
+        """aggregate: False, True (groups of 8) or a group size 2..8."""
         self.keep = _Keep()
         if kind == BATCH_VERIFY:
             items = build_verify_items(reqs, self.keep)
@@ -546,7 +554,9 @@ class StagedBatch:
         self.h = lib().cubbs_hip_batch_prepare_flags(ABI_VERSION, kind, ctypes.cast(items, ctypes.c_void_p),
                                                      ctypes.c_size_t(len(reqs)),
                                                      (BATCH_SINGLE_STREAM if single_stream else 0) |
-                                                     (BATCH_AGGREGATE if aggregate else 0),
+                                                     (BATCH_AGGREGATE if aggregate else 0) |
+                                                     (0 if aggregate is True or not aggregate
+                                                      else (int(aggregate) & 15) << 8),
                                                      ctypes.byref(st))
         self.keep = None  # items are only read during prepare
         if not self.h:
@@ -608,6 +618,18 @@ class StagedBatch:
         v = [ctypes.c_uint64() for _ in range(4)]
         lib().cubbs_hip_batch_shape(self.h, *[ctypes.byref(x) for x in v])
         return dict(zip(["items", "msm_terms", "comb_madds", "pairings"], [x.value for x in v]))
+
+    WORK_FIELDS = ("items", "comb_terms", "comb_madds", "glv_products", "agg_products", "decoded_points",
+                   "point_adds", "pair_checks", "group_checks", "pair_checks_run", "sha_blocks")
+
+    def work(self):
+        """Algorithmic work units of the batch (cubbs_hip_batch_work); pair_checks_run
+        reflects the last results()."""
+        v = (ctypes.c_uint64 * len(self.WORK_FIELDS))()
+        rc = lib().cubbs_hip_batch_work(self.h, v)
+        if rc:
+            raise RuntimeError("work failed %d" % rc)
+        return dict(zip(self.WORK_FIELDS, list(v)))
 
     def group_checks(self):
         """(aggregated group checks in the batch, how many held at the last

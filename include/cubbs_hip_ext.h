@@ -57,7 +57,7 @@ void *cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void *items,
  * single batch then takes longer, but it uses one hardware queue instead of
  * two.  Callers that keep many batches in flight get more throughput this
  * way: eight Verify+ProofVerify sets at GPU_MAX_HW_QUEUES=16 ran 8% faster
- * than four forked sets.  Bits other than the two flags below are rejected
+ * than four forked sets.  Bits other than the flags below are rejected
  * (code 1). */
 #define CUBBS_HIP_BATCH_SINGLE_STREAM 1u
 /* CUBBS_HIP_BATCH_AGGREGATE (Verify / ProofVerify): the pairing checks of
@@ -71,9 +71,10 @@ void *cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void *items,
  * is the batch test of the reference's default same-key verify path.  It
  * lengthens one batch's chain by a pairing launch, so it pays when several
  * batches are in flight: eight single-stream sets ran 4.5 ms per step with
- * it against 6.5 ms without.  CUBBS_HIP_AGG_GROUP=n in the environment
- * overrides the group size (0 = off). */
+ * it against 6.5 ms without.  The group size is 8 unless
+ * CUBBS_HIP_BATCH_AGG_GROUP(n) (2 <= n <= 8) is or-ed into the flags. */
 #define CUBBS_HIP_BATCH_AGGREGATE 2u
+#define CUBBS_HIP_BATCH_AGG_GROUP(n) (((uint32_t)(n) & 15u) << 8)
 void *cubbs_hip_batch_prepare_flags(uint32_t abi_version, int kind, const void *items,
                                     size_t item_count, uint32_t flags, int32_t *status_out);
 int32_t cubbs_hip_batch_launch(void *batch);
@@ -100,8 +101,30 @@ int32_t cubbs_hip_batch_profile(void *batch, int iters, char *names_out, size_t 
 int32_t cubbs_hip_batch_shape(void *batch, uint64_t *items, uint64_t *msm_terms,
                               uint64_t *comb_madds, uint64_t *pairings);
 
+/* Algorithmic work units of a staged batch (not instruction counts), for a
+ * roofline over whole pipeline steps: a cost model prices each unit in field
+ * multiplications.  pair_checks_run is filled in by cubbs_hip_batch_results:
+ * the group checks plus the per-item checks that were not accepted through
+ * their group at that launch (items already rejected earlier are counted as
+ * well, so it is an upper bound); before the first results it equals
+ * group_checks + pair_checks. */
+typedef struct CubbsHipBatchWork {
+  uint64_t items;           /* items on the device */
+  uint64_t comb_terms;      /* fixed-base MSM terms */
+  uint64_t comb_madds;      /* their mixed additions: 16 per term over a wide (16-bit) table, 32 over an 8-bit table */
+  uint64_t glv_products;    /* variable-base products with full scalars (two 128-bit GLV halves) */
+  uint64_t agg_products;    /* variable-base products with two 64-bit halves (aggregation coefficients) */
+  uint64_t decoded_points;  /* G1 decompressions (square root) with their subgroup checks */
+  uint64_t point_adds;      /* Jacobian additions of the partial-sum reductions */
+  uint64_t pair_checks;     /* per-item pairing checks planned (2 Miller loops + 1 final exponentiation each) */
+  uint64_t group_checks;    /* random-linear-combination group checks planned */
+  uint64_t pair_checks_run; /* checks computed at the last launch, see above */
+  uint64_t sha_blocks;      /* SHA-256 compressions of message hashes, domains and challenges */
+} CubbsHipBatchWork;
+int32_t cubbs_hip_batch_work(void *batch, CubbsHipBatchWork *out);
+
 /* Aggregated checks of a staged Verify / ProofVerify batch
- * (CUBBS_HIP_AGG_GROUP at prepare time): the number of group checks it
+ * (CUBBS_HIP_BATCH_AGGREGATE at prepare): the number of group checks it
  * holds, and how many of them held at the last cubbs_hip_batch_results.
  * Items of a group that held are accepted without their own pairing check. */
 int32_t cubbs_hip_batch_group_checks(void *batch, uint64_t *groups, uint64_t *passed);
@@ -170,6 +193,13 @@ int32_t cubbs_hip_autotune(uint32_t max_log2_terms, char *report, size_t report_
 /* Installs a table in the report format (e.g. saved from an earlier run);
  * an empty string clears the table.  A malformed line fails the call. */
 int32_t cubbs_hip_tune_load(const char *table);
+
+/* Launch shapes forced on the calling thread for the calls it makes next
+ * (parity tests of every Pippenger window width and comb chunk): pip_c
+ * Pippenger window width (4..16), pip_s entries per piece (1..256), comb_chunk
+ * comb terms per lane (1..64); 0 = not forced.  Other threads are not
+ * affected. */
+int32_t cubbs_hip_test_force_shapes(uint32_t pip_c, uint32_t pip_s, uint32_t comb_chunk);
 
 /* ---- primitive probes (parity tests) */
 /* out = k * P for n compressed points and 32-byte big-endian scalars; ok[i] = 0

@@ -1,11 +1,12 @@
 """Aggregated pairing checks (random linear combination per group of checks
 under one key, then per-item checks only for failing groups): every per-item
-result must be the same with aggregation off (CUBBS_HIP_AGG_GROUP=0), with
-groups of 2 and with groups of 8, and equal the oracle's verdict, for batches
-that mix valid items, tampered items spread over groups, a fully tampered
-group, undecodable encodings and identity points.
+result must be the same with aggregation off, with groups of 2 and with
+groups of 8 (the group size is a prepare flag of the staged batch), and equal
+the oracle's verdict, for batches that mix valid items, tampered items spread
+over groups, a fully tampered group, undecodable encodings, identity points,
+points on the curve outside G1 and non-canonical scalars next to valid items
+of the same key.
 """
-import os
 import random
 
 import pytest
@@ -15,16 +16,29 @@ from conftest import R_ORDER
 pytestmark = pytest.mark.gpu
 
 
-def _with_group(n, fn):
-    old = os.environ.get("CUBBS_HIP_AGG_GROUP")
-    os.environ["CUBBS_HIP_AGG_GROUP"] = str(n)
+def _run(g, kind, reqs, group):
+    """(status, per-item results) of one staged batch; group 0 = per-item checks."""
+    b = g.StagedBatch(kind, reqs, aggregate=group if group else False)
     try:
-        return fn()
+        b.launch()
+        b.sync()
+        return 0, b.results()
     finally:
-        if old is None:
-            del os.environ["CUBBS_HIP_AGG_GROUP"]
-        else:
-            os.environ["CUBBS_HIP_AGG_GROUP"] = old
+        b.free()
+
+
+def _on_curve_outside_g1(oracle):
+    """Compressed encoding of a point on E(Fp) outside the prime-order subgroup."""
+    p = 0x1A0111EA397FE69A4B1BA7B6434BACD764774B84F38512BF6730D2A0F6B0F6241EABFFFEB153FFFFB9FEFFFFFFFFAAAB
+    for x in range(1, 400):
+        rhs = (x ** 3 + 4) % p
+        y = pow(rhs, (p + 1) // 4, p)
+        if y * y % p == rhs:
+            enc = bytearray(x.to_bytes(48, "big"))
+            enc[0] |= 0x80
+            assert not oracle.g1_valid(bytes(enc))
+            return bytes(enc)
+    raise AssertionError("no point found")
 
 
 def _issuers(g, rng, n):
@@ -53,7 +67,7 @@ def test_verify_aggregation_matches_individual_checks(gpu, oracle):
             sig = b"\xc0" + bytes(47) + sig[48:]               # identity A
         reqs.append((pk, sig, h, m))
     expect = [oracle.verify(*r) for r in reqs]
-    runs = {n: _with_group(n, lambda: g.verify_batch(reqs)) for n in (0, 2, 8)}
+    runs = {n: _run(g, g.BATCH_VERIFY, reqs, n) for n in (0, 2, 8)}
     base = runs[0]
     assert base[0] == 0
     assert [v for _, v in base[1]] == [bool(e) for e in expect]
@@ -87,7 +101,7 @@ def test_proof_verify_aggregation_matches_individual_checks(gpu, oracle):
         if i == 13:
             proof = b"\xc0" + bytes(47) + proof[48:]          # identity Abar
         reqs.append((pk, proof, h, ph, disc, idx, len(m)))
-    runs = {n: _with_group(n, lambda: g.proof_verify_batch(reqs)) for n in (0, 2, 8)}
+    runs = {n: _run(g, g.BATCH_PROOF_VERIFY, reqs, n) for n in (0, 2, 8)}
     base = runs[0]
     assert base[0] == 0
     valid = [v for _, v in base[1]]
@@ -107,7 +121,7 @@ def test_aggregated_staged_batch_graph_replays(gpu, oracle):
     reqs = [(pk, sig, h, ([b"bad"] if i % 13 == 5 else m)) for i, ((sk, pk, h, m), (_, sig)) in
             enumerate(zip(sreqs, sigs))]
     expect = [(0, i % 13 != 5) for i in range(40)]
-    b = _with_group(8, lambda: g.StagedBatch(g.BATCH_VERIFY, reqs))
+    b = g.StagedBatch(g.BATCH_VERIFY, reqs, aggregate=8)
     try:
         b.launch()
         b.sync()
@@ -139,7 +153,7 @@ def test_aggregated_groups_hold_for_valid_proofs(gpu, oracle):
     assert rc == 0
     reqs = [(pk, proof, h, ph, [m[0], m[3]], [0, 3], 5)
             for (pk, sig, h, ph, m, idx), (_, proof) in zip(greqs, proofs)]
-    b = _with_group(8, lambda: g.StagedBatch(g.BATCH_PROOF_VERIFY, reqs))
+    b = g.StagedBatch(g.BATCH_PROOF_VERIFY, reqs, aggregate=8)
     try:
         b.launch()
         b.sync()
@@ -152,7 +166,7 @@ def test_aggregated_groups_hold_for_valid_proofs(gpu, oracle):
     # Abar of another proof under the same key: a valid point for which the
     # pairing equation fails (and so does the challenge)
     bad[6] = (pk, reqs[8][1][:48] + proof[48:], h, ph, disc, idx, L)
-    b = _with_group(8, lambda: g.StagedBatch(g.BATCH_PROOF_VERIFY, bad))
+    b = g.StagedBatch(g.BATCH_PROOF_VERIFY, bad, aggregate=8)
     try:
         b.launch()
         b.sync()
@@ -164,12 +178,11 @@ def test_aggregated_groups_hold_for_valid_proofs(gpu, oracle):
 
 
 def test_aggregate_prepare_flag(gpu, oracle):
-    """CUBBS_HIP_BATCH_AGGREGATE at prepare (no environment variable) forms
-    groups of 8 per key; results equal the per-item batch's; an unknown flag
-    bit is rejected."""
+    """CUBBS_HIP_BATCH_AGGREGATE at prepare forms groups of 8 per key; results
+    equal the per-item batch's; an unknown flag bit, a group size outside
+    2..8 and a group size without the aggregate flag are rejected."""
     import ctypes
     g = gpu
-    assert os.environ.get("CUBBS_HIP_AGG_GROUP") is None
     rng = random.Random(91)
     kps = _issuers(g, rng, 2)
     sreqs = [(kps[i % 2][0], kps[i % 2][1], b"h", [rng.randbytes(16)]) for i in range(33)]
@@ -191,6 +204,57 @@ def test_aggregate_prepare_flag(gpu, oracle):
     finally:
         plain.free()
         agg.free()
-    st = ctypes.c_int32(0)
-    h = g.lib().cubbs_hip_batch_prepare_flags(g.ABI_VERSION, g.BATCH_VERIFY, None, 0, 4, ctypes.byref(st))
-    assert not h and st.value == g.ERR_INVALID_INPUT
+    for flags in (4, g.BATCH_AGGREGATE | (1 << 8), g.BATCH_AGGREGATE | (9 << 8), 4 << 8):
+        st = ctypes.c_int32(0)
+        h = g.lib().cubbs_hip_batch_prepare_flags(g.ABI_VERSION, g.BATCH_VERIFY, None, 0, flags,
+                                                  ctypes.byref(st))
+        assert not h and st.value == g.ERR_INVALID_INPUT, flags
+
+
+def test_aggregated_groups_with_points_outside_g1_and_noncanonical_scalars(gpu, oracle):
+    """Inside one key's groups: a Verify A and ProofVerify Abar / Bbar on the
+    curve but outside G1 (the group sums apply the GLV endomorphism to them
+    while the subgroup check runs beside), a non-canonical e and a
+    non-canonical proof response scalar, each next to valid items of the same
+    key.  The aggregated results equal the per-item run and the oracle's."""
+    g = gpu
+    rng = random.Random(612)
+    kps = _issuers(g, rng, 2)
+    bad_pt = _on_curve_outside_g1(oracle)
+    sreqs = [(kps[i % 2][0], kps[i % 2][1], rng.randbytes(8), [rng.randbytes(24) for _ in range(4)])
+             for i in range(32)]
+    rc, sigs = g.sign_batch(sreqs)
+    assert rc == 0
+    vreqs = []
+    for i, ((sk, pk, h, m), (_, sig)) in enumerate(zip(sreqs, sigs)):
+        if i in (2, 9):
+            sig = bad_pt + sig[48:]                                   # A outside G1
+        if i == 12:
+            e = int.from_bytes(sig[48:], "big") + R_ORDER            # e + r, non-canonical
+            sig = sig[:48] + e.to_bytes(32, "big")
+        vreqs.append((pk, sig, h, m))
+    base = _run(g, g.BATCH_VERIFY, vreqs, 0)
+    assert [c for c, _ in base[1]] == [1 if i in (2, 9, 12) else 0 for i in range(32)]
+    # the oracle's None is the reference's Err: code 1
+    assert base[1] == [(0, e) if e is not None else (1, False) for e in (oracle.verify(*r) for r in vreqs)]
+    for n in (2, 8):
+        assert _run(g, g.BATCH_VERIFY, vreqs, n) == base, n
+    greqs = [(pk, sig, h, rng.randbytes(10), m, [1, 2]) for (sk, pk, h, m), (_, sig) in zip(sreqs, sigs)]
+    rc, proofs = g.proof_gen_batch(greqs, seeds=[rng.randbytes(32) for _ in greqs])
+    assert rc == 0
+    preqs = []
+    for i, ((pk, sig, h, ph, m, idx), (_, proof)) in enumerate(zip(greqs, proofs)):
+        if i == 3:
+            proof = bad_pt + proof[48:]                               # Abar outside G1
+        if i == 10:
+            proof = proof[:48] + bad_pt + proof[96:]                  # Bbar outside G1
+        if i == 17:
+            s = int.from_bytes(proof[176:208], "big") + R_ORDER      # r1^ + r, non-canonical
+            proof = proof[:176] + s.to_bytes(32, "big") + proof[208:]
+        preqs.append((pk, proof, h, ph, [m[k] for k in idx], idx, len(m)))
+    base = _run(g, g.BATCH_PROOF_VERIFY, preqs, 0)
+    assert [c for c, _ in base[1]] == [1 if i in (3, 10, 17) else 0 for i in range(32)]
+    assert base[1] == [(0, e) if e is not None else (1, False)
+                       for e in (oracle.proof_verify(*r) for r in preqs)]
+    for n in (2, 8):
+        assert _run(g, g.BATCH_PROOF_VERIFY, preqs, n) == base, n

@@ -3,10 +3,10 @@ cubbs_bbs_gpu_g1_msm_batch_v2, route id 3), checked against the CPU oracle
 at sizes it finishes in seconds and, at 2^16 terms, by linearity: with
 P_i = a_i G, sum k_i P_i = (sum a_i k_i mod r) G, one oracle multiplication.
 
-Window widths are forced through CUBBS_HIP_PIP_WINDOW to exercise the signed
-digit recoding and the bucket halving at every shape class.
+Window widths are forced on the test's thread (cubbs_hip_test_force_shapes)
+to exercise the signed digit recoding and the bucket halving at every shape
+class.
 """
-import os
 import random
 
 import pytest
@@ -35,16 +35,13 @@ def multiples(gpu, oracle):
     return g1, a, pts
 
 
-def _with_window(c, fn):
-    old = os.environ.get("CUBBS_HIP_PIP_WINDOW")
-    os.environ["CUBBS_HIP_PIP_WINDOW"] = str(c)
+def _with_window(g, c, fn):
+    """fn() with the Pippenger window width forced on this thread."""
+    assert g.force_shapes(pip_c=c) == 0
     try:
         return fn()
     finally:
-        if old is None:
-            del os.environ["CUBBS_HIP_PIP_WINDOW"]
-        else:
-            os.environ["CUBBS_HIP_PIP_WINDOW"] = old
+        g.force_shapes()
 
 
 @pytest.mark.parametrize("window", [0, 4, 7, 11, 13, 16])
@@ -61,7 +58,7 @@ def test_pippenger_matches_oracle(gpu, oracle, multiples, window):
     p[5] = p[6] = p[7]
     s[5] = s[6] = s[7]
     run = lambda: gpu.g1_msm([(p, s)])
-    rc, res = _with_window(window, run) if window else run()
+    rc, res = _with_window(gpu, window, run) if window else run()
     assert rc == 0
     assert res[0] == (0, 3, oracle.g1_msm(p, s))
 
