@@ -94,6 +94,10 @@ def log_uniform_L(rng):
 
 
 def build_workload(g, seed=b"cubbs-hip/bench"):
+    """The benchmark's batches, made by the library from fixed seeds: Verify
+    requests and their expected verdicts, ProofVerify requests, the issuer
+    keys, whole-op call rates, and the seeded ProofGen requests, seeds and
+    proofs the ProofVerify batch was made from."""
     rng = random.Random(hashlib.sha256(seed).digest())
     rc, kps = g.keygen([rng.randbytes(32) for _ in range(N_ISSUERS)])
     assert rc == 0
@@ -143,7 +147,7 @@ def build_workload(g, seed=b"cubbs-hip/bench"):
                "proof_gen_ops_s": round(N_PROOFS / calls["proof_gen_2048_s"], 1),
                "note": "one whole-op ABI call each (8192 uneven Sign; 2048 seeded ProofGen at L=32), "
                        "host packing and PCIe included, first call of the process"}
-    return vreqs, expect_v, pvreqs, kps, abi_ops
+    return vreqs, expect_v, pvreqs, kps, abi_ops, (greqs, seeds, [p for _, p in proofs])
 
 
 MSM_ROWS = 4096
@@ -501,7 +505,7 @@ def main():
         print(json.dumps(out), flush=True)
         return
     t0 = time.time()
-    vreqs, expect_v, pvreqs, kps, abi_ops = build_workload(g)
+    vreqs, expect_v, pvreqs, kps, abi_ops, _ = build_workload(g)
     log("workload built in %.1fs" % (time.time() - t0))
     nset = max(1, args.inflight)
     single, agg = not args.fork, not args.no_aggregate

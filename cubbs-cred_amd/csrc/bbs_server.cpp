@@ -380,9 +380,10 @@ int64_t submit_at(Server* s, const CubbsHipRequest* q, uint64_t t_arrive) {
   auto r = std::make_unique<Req>();
   int32_t rc = copy_request(*q, *r);
   if (rc) return -(int64_t)rc;
-  // a request that alone exceeds a slot's share of the HBM budget is refused
-  // at admission (code 1) rather than failing its batch on the device
-  if (r->est > s->share) return -(int64_t)CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  // a request above a slot's share runs in a batch of its own; one above the
+  // whole HBM budget is refused at admission (code 1) instead of failing on
+  // the device
+  if (r->est > s->cfg.hbm_budget_bytes) return -(int64_t)CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   r->t_arrive = t_arrive;
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return -(int64_t)CUBBS_BBS_GPU_ERR_INTERNAL;

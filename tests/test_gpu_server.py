@@ -118,16 +118,18 @@ def test_server_submit_wait_and_item_errors(gpu):
 
 def test_server_hbm_budget_bounds_batches(gpu):
     """Admission follows the HBM budget: with a share per slot below one
-    request's estimate a batch holds one such request (an oversized request
-    still runs, alone); with room they batch.  Results equal the requests
-    run alone either way."""
+    request's estimate a batch holds one such request (a request above the
+    share still runs, alone); with room they batch.  Results equal the
+    requests run alone either way."""
     g = gpu
     pool = _pool(g, b"server-budget", n_verify=6, n_pv=4, n_sign=3, n_pg=3)
     alone = [_alone(g, *p) for p in pool]
     order = list(range(len(pool)))
     arrivals = [0] * len(order)  # all pending at once
-    for budget, one_each in ((2 * 64 * 1024, True), (2 * (1 << 30), False)):
-        srv = g.Server(slots=2, max_batch=64, hbm_budget=budget)
+    # 4 slots x 60 KiB: every Verify / ProofVerify / ProofGen request is above
+    # a slot's share (pairing line blocks, product tables) but below the budget
+    for slots, budget, one_each in ((4, 4 * 60 * 1024, True), (2, 2 * (1 << 30), False)):
+        srv = g.Server(slots=slots, max_batch=64, hbm_budget=budget)
         try:
             rc, res, lat, outs = srv.run_open_loop(pool, order, arrivals, outputs=True)
             assert rc == 0
@@ -137,8 +139,6 @@ def test_server_hbm_budget_bounds_batches(gpu):
             st = srv.stats()
             assert st["fault"] == 0 and st["requests"] == len(order)
             if one_each:
-                # each Verify, ProofVerify and ProofGen request (pairing line
-                # blocks or product tables) exceeds the share and runs alone;
                 # the 3 Sign requests are estimated small and may share one
                 n_sign = sum(1 for k in order if pool[k][0] == g.REQ_SIGN)
                 assert len(order) - n_sign + 1 <= st["batches"] <= len(order)
@@ -146,6 +146,33 @@ def test_server_hbm_budget_bounds_batches(gpu):
                 assert st["batches"] < len(order)
         finally:
             srv.close()
+
+
+def test_server_refuses_requests_above_the_budget(gpu):
+    """A request whose estimate exceeds the whole HBM budget is refused at
+    admission with code 1 (it never reaches the device); the requests around
+    it, and later ones, are served normally and the server stays healthy."""
+    g = gpu
+    pool = _pool(g, b"server-refuse", n_verify=3, n_pv=0, n_sign=3, n_pg=0)
+    alone = [_alone(g, *p) for p in pool]
+    # 2 x 64 KiB: a Verify request (two pairings' line blocks, ~137 KiB) is
+    # above the budget, a Sign request below it
+    srv = g.Server(slots=2, max_batch=64, hbm_budget=2 * 64 * 1024)
+    try:
+        order = list(range(len(pool)))
+        rc, res, lat, outs = srv.run_open_loop(pool, order, [0] * len(order), outputs=True)
+        for i, k in enumerate(order):
+            kind = pool[k][0]
+            if kind == g.REQ_VERIFY:
+                assert res[i] == (g.ERR_INVALID_INPUT, 0)
+            else:
+                _, code, value, out = alone[k]
+                assert (res[i], outs[i]) == ((code, value), out)
+        st = srv.stats()
+        assert st["fault"] == 0
+        assert st["requests"] == sum(1 for k in order if pool[k][0] != g.REQ_VERIFY)
+    finally:
+        srv.close()
 
 
 def test_server_open_loop_rejects_unordered_arrivals(gpu):
