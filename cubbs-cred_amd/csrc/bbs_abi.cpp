@@ -69,7 +69,7 @@ const Knobs& knobs() {
     d.comb16_slots = (uint32_t)std::min<uint64_t>(env_u64("CUBBS_HIP_COMB16_GENERATORS", COMB16_DEFAULT_SLOTS), 4096);
     d.hw_queues = (uint32_t)std::min<uint64_t>(env_u64("GPU_MAX_HW_QUEUES", 4), 64);
     d.pk_cache_entries = (uint32_t)std::max<uint64_t>(16, std::min<uint64_t>(env_u64("CUBBS_HIP_PK_CACHE_ENTRIES", 4096), 1u << 20));
-    d.gen_table_budget = env_u64("CUBBS_HIP_GEN_TABLE_BUDGET_BYTES", 2ull << 30);
+    d.gen_table_budget = env_u64("CUBBS_HIP_GEN_TABLE_BUDGET_BYTES", 32ull << 30);
     return d;
   }();
   return k;
@@ -102,6 +102,7 @@ void init_locked(Context& c) {
   check(hipStreamCreateWithFlags(&c.setup, hipStreamNonBlocking));
   check(hipMalloc(&c.gen_vstate, 64));
   check(hipMalloc(&c.neg_g2_lines, PK_LINES_WORDS * 4));
+  c.resident_bytes += 64 + PK_LINES_WORDS * 4;
   launch_prepare_neg_g2(c.setup, c.neg_g2_lines);
   check(hipGetLastError());
   check(hipStreamSynchronize(c.setup));
@@ -128,8 +129,10 @@ static void ensure_wide_tables_locked(Context& c, uint32_t slots) {
       c.gen_tables16 = nullptr;
       c.wide_cap = 0;
     }
+    c.resident_bytes += (size_t)c.wide_cap * COMB16_TABLE_WORDS * 4;
   }
-  const uint32_t upto = std::min(slots, c.wide_cap);
+  // built from the slots' 8-bit tables
+  const uint32_t upto = std::min({slots, c.wide_cap, c.table_slots});
   if (upto <= c.wide_slots) return;
   const uint32_t slice = 32;  // generators per build launch (1.9 GB of scratch)
   uint32_t* scratch = nullptr;
@@ -149,134 +152,226 @@ static void ensure_wide_tables_locked(Context& c, uint32_t slots) {
   c.wide_slots = upto;
 }
 
-// Ensure generator slots [0, slots) exist (tables included).  ctx().mu held.
+// ------------------------------------------------------------------ resident state
+void retire_locked(Context& c, void* p, size_t bytes) {
+  if (!p) return;
+  c.retired.push_back(Context::Retired{p, c.epoch, bytes});
+  c.epoch++;  // batches attached from now on cannot see p
+}
+
+static void free_retired_locked(Context& c) {
+  const uint64_t oldest = c.live.empty() ? UINT64_MAX : c.live.begin()->first;
+  size_t keep = 0;
+  for (auto& r : c.retired) {
+    if (r.epoch < oldest) {
+      (void)hipFree(r.p);
+      c.resident_bytes -= r.bytes;
+    } else {
+      c.retired[keep++] = r;
+    }
+  }
+  c.retired.resize(keep);
+}
+
+void hold_locked(Context& c, ResidentHold& h) {
+  if (h.held) return;
+  h.held = true;
+  h.epoch = c.epoch;
+  c.live[h.epoch]++;
+}
+
+void release_locked(Context& c, ResidentHold& h) {
+  if (!h.held) return;
+  for (uint32_t s : h.pk_slots)
+    if (s < c.pk_pins.size() && c.pk_pins[s]) c.pk_pins[s]--;
+  h.pk_slots.clear();
+  auto it = c.live.find(h.epoch);
+  if (it != c.live.end() && --it->second == 0) c.live.erase(it);
+  h.held = false;
+  free_retired_locked(c);
+}
+
+template <class T>
+static T* grow_copy(Context& c, T* old, size_t old_bytes, size_t new_bytes) {
+  T* p = nullptr;
+  check(hipMalloc(&p, new_bytes));
+  c.resident_bytes += new_bytes;
+  if (old && old_bytes) check(hipMemcpyAsync(p, old, old_bytes, hipMemcpyDeviceToDevice, c.setup));
+  retire_locked(c, old, old_bytes);
+  return p;
+}
+
+// Generator points and encodings of slots [0, slots); 8-bit comb tables of
+// slots [0, min(slots, table_max)).  ctx().mu held.
 void ensure_generators_locked(Context& c, uint32_t slots) {
-  if (slots <= c.gen_slots) return;
-  if (slots > c.gen_cap) {
-    uint32_t cap = std::max<uint32_t>({slots, c.gen_cap * 2, 64u});
-    uint32_t* aff = nullptr;
-    uint32_t* tab = nullptr;
-    uint8_t* comp = nullptr;
-    check(hipMalloc(&aff, (size_t)cap * G1A_WORDS * 4));
-    check(hipMalloc(&tab, (size_t)cap * COMB_TABLE_WORDS * 4));
-    check(hipMalloc(&comp, (size_t)cap * 48));
-    if (c.gen_slots) {
-      check(hipMemcpyAsync(aff, c.gen_affine, (size_t)c.gen_slots * G1A_WORDS * 4,
-                           hipMemcpyDeviceToDevice, c.setup));
-      check(hipMemcpyAsync(tab, c.gen_tables, (size_t)c.gen_slots * COMB_TABLE_WORDS * 4,
-                           hipMemcpyDeviceToDevice, c.setup));
-      check(hipMemcpyAsync(comp, c.gen_comp, (size_t)c.gen_slots * 48, hipMemcpyDeviceToDevice,
-                           c.setup));
-    }
-    if (c.gen_affine) {
-      c.retired.push_back(c.gen_affine);
-      c.retired.push_back(c.gen_tables);
-      c.retired.push_back(c.gen_comp);
-    }
-    c.gen_affine = aff;
-    c.gen_tables = tab;
-    c.gen_comp = comp;
-    c.gen_cap = cap;
+  if (!c.table_max) {
+    const uint64_t per = (uint64_t)COMB_TABLE_WORDS * 4;
+    c.table_max = (uint32_t)std::max<uint64_t>(2, std::min<uint64_t>(knobs().gen_table_budget / per, 1u << 24));
   }
-  uint32_t first_chain = c.gen_slots == 0 ? 1 : c.gen_slots;
-  uint32_t count = slots - first_chain;
-  if (count) {
-    uint32_t* seeds = nullptr;
-    check(hipMalloc(&seeds, (size_t)count * 12 * 4));
-    launch_gen_seeds(c.setup, first_chain, count, c.gen_vstate, seeds);
-    launch_gen_points(c.setup, first_chain, count, seeds, c.gen_affine, c.gen_comp);
-    check(hipGetLastError());
-    check(hipStreamSynchronize(c.setup));
-    check(hipFree(seeds));
+  if (slots > c.gen_slots) {
+    if (slots > c.gen_cap) {
+      const uint32_t cap = std::max<uint32_t>({slots, c.gen_cap * 2, 64u});
+      c.gen_affine = grow_copy(c, c.gen_affine, (size_t)c.gen_slots * G1A_WORDS * 4, (size_t)cap * G1A_WORDS * 4);
+      c.gen_comp = grow_copy(c, c.gen_comp, (size_t)c.gen_slots * 48, (size_t)cap * 48);
+      c.gen_cap = cap;
+    }
+    const uint32_t first_chain = c.gen_slots == 0 ? 1 : c.gen_slots;
+    const uint32_t count = slots - first_chain;
+    if (count) {
+      uint32_t* seeds = nullptr;
+      check(hipMalloc(&seeds, (size_t)count * 12 * 4));
+      launch_gen_seeds(c.setup, first_chain, count, c.gen_vstate, seeds);
+      launch_gen_points(c.setup, first_chain, count, seeds, c.gen_affine, c.gen_comp);
+      check(hipGetLastError());
+      check(hipStreamSynchronize(c.setup));
+      check(hipFree(seeds));
+    }
+    // host mirror of the encodings: recognises generator points in the
+    // primitive MSM entries (their comb tables are resident)
+    c.gen_comp_host.resize((size_t)slots * 48);
+    check(hipMemcpy(c.gen_comp_host.data() + (size_t)c.gen_slots * 48, c.gen_comp + (size_t)c.gen_slots * 48,
+                    (size_t)(slots - c.gen_slots) * 48, hipMemcpyDeviceToHost));
+    for (uint32_t g = c.gen_slots; g < slots; g++)
+      c.gen_index.emplace(std::string(reinterpret_cast<const char*>(c.gen_comp_host.data()) + 48 * g, 48), g);
+    c.gen_slots = slots;
   }
-  uint32_t first_table = c.gen_slots;
-  uint32_t tcount = slots - first_table;
-  // build tables in slices to bound the scratch buffer
+  const uint32_t want = std::min(slots, c.table_max);
+  if (want > c.table_slots) {
+    const size_t per = (size_t)COMB_TABLE_WORDS * 4;
+    if (want > c.table_cap) {
+      const uint32_t cap = std::min<uint32_t>(c.table_max, std::max<uint32_t>({want, c.table_cap * 2, 64u}));
+      c.gen_tables = grow_copy(c, c.gen_tables, (size_t)c.table_slots * per, (size_t)cap * per);
+      c.table_cap = cap;
+    }
+    build_comb_tables(c, c.gen_tables, c.table_slots, want - c.table_slots);
+    c.table_slots = want;
+  }
+  ensure_wide_tables_locked(c, slots);
+}
+
+// 8-bit comb tables of generator slots [first, first + count) into `tables`
+// (indexed by slot), built on the setup stream in slices that bound the
+// scratch buffer.  ctx().mu held.
+void build_comb_tables(Context& c, uint32_t* tables, uint32_t first, uint32_t count) {
+  if (!count) return;
   const uint32_t slice = 64;
   uint32_t* scratch = nullptr;
-  check(hipMalloc(&scratch, (size_t)slice * COMB_WINDOWS * COMB_ENTRIES * COMB_ENTRY_WORDS * 4));
-  for (uint32_t s = 0; s < tcount; s += slice) {
-    uint32_t k = std::min(slice, tcount - s);
-    launch_comb_build(c.setup, first_table + s, k, c.gen_affine, c.gen_tables, scratch);
+  check(hipMalloc(&scratch, (size_t)std::min(slice, count) * COMB_WINDOWS * COMB_ENTRIES * COMB_ENTRY_WORDS * 4));
+  for (uint32_t s = 0; s < count; s += slice) {
+    const uint32_t k = std::min(slice, count - s);
+    launch_comb_build(c.setup, first + s, k, c.gen_affine, tables, scratch);
     check(hipGetLastError());
   }
   check(hipStreamSynchronize(c.setup));
   check(hipFree(scratch));
-  ensure_wide_tables_locked(c, slots);
-  // host mirror of the encodings: recognises generator points in the
-  // primitive MSM entries (their comb tables are resident)
-  c.gen_comp_host.resize((size_t)slots * 48);
-  check(hipMemcpy(c.gen_comp_host.data() + (size_t)c.gen_slots * 48, c.gen_comp + (size_t)c.gen_slots * 48,
-                  (size_t)(slots - c.gen_slots) * 48, hipMemcpyDeviceToHost));
-  for (uint32_t g = c.gen_slots; g < slots; g++)
-    c.gen_index.emplace(std::string(reinterpret_cast<const char*>(c.gen_comp_host.data()) + 48 * g, 48),
-                        g);
-  c.gen_slots = slots;
 }
 
-// Map public keys to cache slots, preparing unseen ones (decompression,
-// subgroup check, Miller-loop lines) on the device.  ctx().mu held.
-void lookup_pks_locked(Context& c, const std::vector<const uint8_t*>& pks,
-                       std::vector<uint32_t>& slots_out) {
-  std::vector<uint32_t> fresh;
-  std::vector<const uint8_t*> fresh_bytes;
-  slots_out.resize(pks.size());
-  for (size_t i = 0; i < pks.size(); i++) {
-    std::string key(reinterpret_cast<const char*>(pks[i]), 96);
-    auto it = c.pk_slots.find(key);
-    if (it != c.pk_slots.end()) {
-      slots_out[i] = it->second;
-      continue;
-    }
-    uint32_t slot = c.pk_count++;
-    c.pk_slots.emplace(std::move(key), slot);
-    slots_out[i] = slot;
-    fresh.push_back(slot);
-    fresh_bytes.push_back(pks[i]);
-  }
-  if (fresh.empty()) return;
-  if (c.pk_count > c.pk_cap) {
-    uint32_t cap = std::max<uint32_t>({c.pk_count, c.pk_cap * 2, 64u});
-    uint8_t* b = nullptr;
-    uint8_t* st = nullptr;
-    uint32_t* ln = nullptr;
-    check(hipMalloc(&b, (size_t)cap * 96));
-    check(hipMalloc(&st, cap));
-    check(hipMalloc(&ln, (size_t)cap * PK_LINES_WORDS * 4));
-    uint32_t old = c.pk_count - (uint32_t)fresh.size();
-    if (old) {
-      check(hipMemcpyAsync(b, c.pk_bytes, (size_t)old * 96, hipMemcpyDeviceToDevice, c.setup));
-      check(hipMemcpyAsync(st, c.pk_status, old, hipMemcpyDeviceToDevice, c.setup));
-      check(hipMemcpyAsync(ln, c.pk_lines, (size_t)old * PK_LINES_WORDS * 4,
-                           hipMemcpyDeviceToDevice, c.setup));
-    }
-    if (c.pk_bytes) {
-      c.retired.push_back(c.pk_bytes);
-      c.retired.push_back(c.pk_status);
-      c.retired.push_back(c.pk_lines);
-    }
-    c.pk_bytes = b;
-    c.pk_status = st;
-    c.pk_lines = ln;
-    c.pk_cap = cap;
-  }
-  std::vector<uint8_t> hb(fresh.size() * 96);
-  for (size_t i = 0; i < fresh.size(); i++) std::memcpy(hb.data() + 96 * i, fresh_bytes[i], 96);
+// Prepare public keys (decompression, subgroup check, Miller-loop lines) into
+// the given slots of the arrays.  ctx().mu held.
+void prepare_pks_locked(Context& c, const std::vector<const uint8_t*>& keys,
+                        const std::vector<uint32_t>& slots, uint8_t* pk_bytes, uint8_t* pk_status,
+                        uint32_t* pk_lines) {
+  if (keys.empty()) return;
+  std::vector<uint8_t> hb(keys.size() * 96);
+  for (size_t i = 0; i < keys.size(); i++) std::memcpy(hb.data() + 96 * i, keys[i], 96);
   uint32_t* d_slots = nullptr;
   uint8_t* d_b = nullptr;
-  check(hipMalloc(&d_slots, fresh.size() * 4));
+  check(hipMalloc(&d_slots, slots.size() * 4));
   check(hipMalloc(&d_b, hb.size()));
-  check(hipMemcpyAsync(d_slots, fresh.data(), fresh.size() * 4, hipMemcpyHostToDevice, c.setup));
+  check(hipMemcpyAsync(d_slots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, c.setup));
   check(hipMemcpyAsync(d_b, hb.data(), hb.size(), hipMemcpyHostToDevice, c.setup));
-  // scatter the bytes into their slots, then prepare
-  for (size_t i = 0; i < fresh.size(); i++)
-    check(hipMemcpyAsync(c.pk_bytes + (size_t)fresh[i] * 96, d_b + 96 * i, 96,
-                         hipMemcpyDeviceToDevice, c.setup));
-  launch_prepare_pk(c.setup, d_slots, (uint32_t)fresh.size(), c.pk_bytes, c.pk_status, c.pk_lines);
+  for (size_t i = 0; i < keys.size(); i++)  // scatter the bytes into their slots
+    check(hipMemcpyAsync(pk_bytes + (size_t)slots[i] * 96, d_b + 96 * i, 96, hipMemcpyDeviceToDevice,
+                         c.setup));
+  launch_prepare_pk(c.setup, d_slots, (uint32_t)keys.size(), pk_bytes, pk_status, pk_lines);
   check(hipGetLastError());
   check(hipStreamSynchronize(c.setup));
   check(hipFree(d_slots));
   check(hipFree(d_b));
+}
+
+bool lookup_pks_locked(Context& c, const std::vector<const uint8_t*>& pks,
+                       std::vector<uint32_t>& slots_out, ResidentHold& hold) {
+  if (!c.pk_cap) {  // the table, allocated once
+    const uint32_t cap = knobs().pk_cache_entries;
+    check(hipMalloc(&c.pk_bytes, (size_t)cap * 96));
+    check(hipMalloc(&c.pk_status, cap));
+    check(hipMalloc(&c.pk_lines, (size_t)cap * PK_LINES_WORDS * 4));
+    c.resident_bytes += (size_t)cap * (97 + PK_LINES_WORDS * 4);
+    c.pk_key.assign(cap, std::string());
+    c.pk_pins.assign(cap, 0);
+    c.pk_used.assign(cap, 0);
+    c.pk_cap = cap;
+  }
+  const uint64_t now = ++c.tick;
+  slots_out.assign(pks.size(), 0);
+  // distinct keys of the call: hits, and misses that need a slot
+  std::unordered_map<std::string, uint32_t> seen;  // key -> index into slots_out's key list
+  std::vector<std::string> miss;
+  std::vector<int64_t> where(pks.size(), -1);       // hit slot, or -(1 + miss index)
+  size_t distinct = 0;
+  for (size_t i = 0; i < pks.size(); i++) {
+    std::string key(reinterpret_cast<const char*>(pks[i]), 96);
+    auto it = c.pk_slots.find(key);
+    if (it != c.pk_slots.end()) {
+      where[i] = it->second;
+      if (seen.emplace(key, 0).second) distinct++;
+      continue;
+    }
+    auto sm = seen.find(key);
+    if (sm == seen.end()) {
+      seen.emplace(key, (uint32_t)miss.size() + 1);
+      distinct++;
+      miss.push_back(key);
+      where[i] = -(int64_t)miss.size();
+    } else {
+      where[i] = -(int64_t)sm->second;
+    }
+  }
+  if (distinct > c.pk_cap) return false;
+  // slots for the misses: free ones first, then the least recently used
+  // entries that no live batch uses and this call does not hit
+  std::vector<uint32_t> fresh;
+  if (!miss.empty()) {
+    std::vector<uint32_t> cand;
+    for (uint32_t s = 0; s < c.pk_cap; s++)
+      if (c.pk_key[s].empty() || (c.pk_pins[s] == 0 && c.pk_used[s] != now)) cand.push_back(s);
+    if (cand.size() < miss.size()) return false;
+    std::partial_sort(cand.begin(), cand.begin() + miss.size(), cand.end(), [&](uint32_t a, uint32_t b) {
+      const bool fa = c.pk_key[a].empty(), fb = c.pk_key[b].empty();
+      if (fa != fb) return fa;
+      return c.pk_used[a] < c.pk_used[b];
+    });
+    fresh.assign(cand.begin(), cand.begin() + miss.size());
+  }
+  // mark the hits used now (so a later eviction in this call skips them)
+  for (size_t i = 0; i < pks.size(); i++)
+    if (where[i] >= 0) c.pk_used[(uint32_t)where[i]] = now;
+  std::vector<const uint8_t*> fresh_bytes;
+  for (size_t m = 0; m < miss.size(); m++) {
+    const uint32_t s = fresh[m];
+    if (!c.pk_key[s].empty()) c.pk_slots.erase(c.pk_key[s]);
+    else c.pk_count++;
+    c.pk_key[s] = miss[m];
+    c.pk_slots[miss[m]] = s;
+    c.pk_used[s] = now;
+    fresh_bytes.push_back(reinterpret_cast<const uint8_t*>(c.pk_key[s].data()));
+  }
+  prepare_pks_locked(c, fresh_bytes, fresh, c.pk_bytes, c.pk_status, c.pk_lines);
+  std::vector<uint32_t> pinned;
+  for (size_t i = 0; i < pks.size(); i++) {
+    const uint32_t s = where[i] >= 0 ? (uint32_t)where[i] : fresh[(size_t)(-where[i] - 1)];
+    slots_out[i] = s;
+  }
+  // pin each distinct slot once
+  std::vector<uint32_t> uniq(slots_out);
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  for (uint32_t s : uniq) {
+    c.pk_pins[s]++;
+    hold.pk_slots.push_back(s);
+  }
+  return true;
 }
 
 }  // namespace cubbs
@@ -519,6 +614,30 @@ int32_t cubbs_hip_batch_shape(void* batch, uint64_t* items, uint64_t* msm_terms,
   if (comb_madds) *comb_madds = b->comb_madds;
   if (pairings) *pairings = 2ull * b->n_pairs;
   return CUBBS_BBS_GPU_SUCCESS;
+}
+
+int32_t cubbs_hip_resident_stats(CubbsHipResidentStats* out) {
+  if (!out) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  try {
+    Context& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    init_locked(c);
+    CubbsHipResidentStats r{};
+    r.resident_bytes = c.resident_bytes;
+    r.pk_capacity = c.pk_cap;
+    r.pk_entries = c.pk_count;
+    for (uint32_t p : c.pk_pins) r.pk_pinned += p != 0;
+    r.generator_slots = c.gen_slots;
+    r.table_slots = c.table_slots;
+    r.table_max_slots = c.table_max;
+    r.wide_slots = c.wide_slots;
+    r.retired_buffers = c.retired.size();
+    for (const auto& e : c.live) r.live_batches += e.second;
+    *out = r;
+    return CUBBS_BBS_GPU_SUCCESS;
+  } catch (...) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
 }
 
 int32_t cubbs_hip_batch_work(void* batch, CubbsHipBatchWork* out) {

@@ -173,7 +173,10 @@ void materialize(Batch& b, Plan& p) {
   size_t o_dg1 = place(p.dec_g1.data(), p.dec_g1.size() * sizeof(DecodeG1Job));
   size_t o_dfr = place(p.dec_fr.data(), p.dec_fr.size() * sizeof(DecodeFrJob));
   size_t o_fill = place(p.fills.data(), p.fills.size() * sizeof(TermFill));
-  size_t o_var = place(p.var.data(), p.var.size() * sizeof(VarJob));
+  // one product launch: the plan's products, then the aggregation products
+  std::vector<VarJob> all_var(p.var);
+  all_var.insert(all_var.end(), p.agg_var.begin(), p.agg_var.end());
+  size_t o_var = place(all_var.data(), all_var.size() * sizeof(VarJob));
   size_t o_chk = place(p.chunks.data(), p.chunks.size() * sizeof(CombChunk));
   size_t o_tg = place(p.term_gen.data(), p.term_gen.size() * 4);
   size_t o_g1 = place(p.groups1.data(), p.groups1.size() * sizeof(ReduceGroup));
@@ -181,7 +184,7 @@ void materialize(Batch& b, Plan& p) {
   size_t o_g2 = place(p.groups2.data(), p.groups2.size() * sizeof(ReduceGroup));
   size_t o_o2 = place(p.out2.data(), p.out2.size() * 4);
   size_t o_pair = place(p.pairs.data(), p.pairs.size() * sizeof(PairJob));
-  size_t o_agg = place(p.agg_var.data(), p.agg_var.size() * sizeof(VarJob));
+  size_t o_pcp = place(p.pt_copy.data(), p.pt_copy.size() * sizeof(PointCopy));
   std::vector<std::pair<size_t, size_t>> o_lv;
   for (size_t l = 0; l < p.pst_groups.size(); l++)
     o_lv.emplace_back(place(p.pst_groups[l].data(), p.pst_groups[l].size() * sizeof(ReduceGroup)),
@@ -220,7 +223,7 @@ void materialize(Batch& b, Plan& p) {
   b.fills = reinterpret_cast<TermFill*>(D + o_fill);
   b.n_fills = (uint32_t)p.fills.size();
   b.var = reinterpret_cast<VarJob*>(D + o_var);
-  b.n_var = (uint32_t)p.var.size();
+  b.n_var = (uint32_t)all_var.size();
   b.chunks = reinterpret_cast<CombChunk*>(D + o_chk);
   b.n_chunks = (uint32_t)p.chunks.size();
   b.term_gen = reinterpret_cast<uint32_t*>(D + o_tg);
@@ -233,8 +236,8 @@ void materialize(Batch& b, Plan& p) {
   b.n_groups2 = (uint32_t)p.groups2.size();
   b.pairs = reinterpret_cast<PairJob*>(D + o_pair);
   b.n_pairs = (uint32_t)p.pairs.size();
-  b.agg_var = reinterpret_cast<VarJob*>(D + o_agg);
-  b.n_agg_var = (uint32_t)p.agg_var.size();
+  b.pt_copy = reinterpret_cast<PointCopy*>(D + o_pcp);
+  b.n_pt_copy = (uint32_t)p.pt_copy.size();
   b.n_agg_slots = p.n_agg_slots;
   b.n_gpairs = p.n_group_pairs;
   for (size_t l = 0; l < o_lv.size(); l++)
@@ -290,7 +293,7 @@ void materialize(Batch& b, Plan& p) {
   size_t o_out = take(std::max<size_t>(p.out_bytes, 1));
   size_t o_l2 = take(4ull * PAIR_L2_WORDS * std::max<size_t>(p.pairs.size(), 1));
   // product workspaces: the plan's jobs, then the aggregation jobs
-  size_t o_vt = take(4ull * VAR_WS_WORDS * std::max<size_t>(p.var.size() + p.agg_var.size(), 1));
+  size_t o_vt = take(4ull * VAR_WS_WORDS * std::max<size_t>(all_var.size(), 1));
   const PipShape& ps = b.pip;
   const size_t pip_on = ps.terms ? 1 : 0;
   size_t o_pcnt = take(4ull * (ps.nbt + 1) * pip_on);
@@ -318,7 +321,6 @@ void materialize(Batch& b, Plan& p) {
   b.out = W + o_out;
   b.l2buf = reinterpret_cast<uint32_t*>(W + o_l2);
   b.vtab = reinterpret_cast<uint32_t*>(W + o_vt);
-  b.vtab_agg = b.vtab + (size_t)VAR_WS_WORDS * p.var.size();
   if (pip_on) {
     auto u32 = [&](size_t o) { return reinterpret_cast<uint32_t*>(W + o); };
     b.pipb = PipBuffers{u32(o_pcnt), u32(o_poff), u32(o_pcur), u32(o_ppof), u32(o_pmap),
@@ -383,7 +385,7 @@ static void agg_coefficient(uint8_t out[32], uint64_t s, uint64_t t) {
     for (int j = 0; j < 8; j++) out[31 - 8 * k - j] = (uint8_t)(w[k] >> (8 * j));
 }
 
-void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max) {
+void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, bool scaled) {
   if (group_max < 2 || group_max > 8) return;  // 2 x 8 partials per sum: one final reduction
   std::vector<PairJob> in = std::move(p.pairs);
   p.pairs.clear();
@@ -395,8 +397,8 @@ void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max) {
     if (px != py) return px;
     return px && in[x].pk_slot < in[y].pk_slot;
   });
-  std::vector<uint8_t> rnd(16 * in.size());
-  fill_os_random(rnd.data(), rnd.size());
+  std::vector<uint8_t> rnd(scaled ? 0 : 16 * in.size());
+  if (!scaled) fill_os_random(rnd.data(), rnd.size());
   std::vector<PairJob> groups, grouped, single;
   size_t i = 0;
   while (i < order.size()) {
@@ -413,22 +415,33 @@ void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max) {
     }
     const uint32_t gslot = nv + p.n_agg_slots++;
     const uint32_t cnt = (uint32_t)(e - i);
-    std::vector<uint32_t> sc(cnt);
-    for (uint32_t k = 0; k < cnt; k++) {
-      uint64_t s, t;
-      std::memcpy(&s, &rnd[16 * order[i + k]], 8);
-      std::memcpy(&t, &rnd[16 * order[i + k] + 8], 8);
-      uint8_t c[32];
-      agg_coefficient(c, s, t);
-      sc[k] = p.scalars(1);
-      p.dec_fr.push_back(DecodeFrJob{p.bytes(c, 32), sc[k], gslot, 0u, 0u});
+    uint32_t fa, fb;
+    if (scaled) {
+      // the checks' points are c_i a_i and c_i b_i already
+      fa = p.partials(cnt);
+      fb = p.partials(cnt);
+      for (uint32_t k = 0; k < cnt; k++) {
+        p.pt_copy.push_back(PointCopy{in[order[i + k]].a, fa + k});
+        p.pt_copy.push_back(PointCopy{in[order[i + k]].b, fb + k});
+      }
+    } else {
+      std::vector<uint32_t> sc(cnt);
+      for (uint32_t k = 0; k < cnt; k++) {
+        uint64_t s, t;
+        std::memcpy(&s, &rnd[16 * order[i + k]], 8);
+        std::memcpy(&t, &rnd[16 * order[i + k] + 8], 8);
+        uint8_t c[32];
+        agg_coefficient(c, s | 1, t);
+        sc[k] = p.scalars(1);
+        p.dec_fr.push_back(DecodeFrJob{p.bytes(c, 32), sc[k], gslot, 0u, 0u});
+      }
+      fa = p.n_partials;
+      for (uint32_t k = 0; k < cnt; k++)
+        p.agg_var.push_back(VarJob{in[order[i + k]].a, sc[k], p.partials(VAR_PARTIALS)});
+      fb = p.n_partials;
+      for (uint32_t k = 0; k < cnt; k++)
+        p.agg_var.push_back(VarJob{in[order[i + k]].b, sc[k], p.partials(VAR_PARTIALS)});
     }
-    const uint32_t fa = p.n_partials;
-    for (uint32_t k = 0; k < cnt; k++)
-      p.agg_var.push_back(VarJob{in[order[i + k]].a, sc[k], p.partials(VAR_PARTIALS)});
-    const uint32_t fb = p.n_partials;
-    for (uint32_t k = 0; k < cnt; k++)
-      p.agg_var.push_back(VarJob{in[order[i + k]].b, sc[k], p.partials(VAR_PARTIALS)});
     const uint32_t la = p.points(1), lb = p.points(1);
     p.groups2.push_back(ReduceGroup{fa, VAR_PARTIALS * cnt});
     p.out2.push_back(la);
@@ -450,20 +463,60 @@ void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max) {
 }
 
 // Lock the context, make generators/public keys resident, snapshot pointers.
+// When the budget of 8-bit tables or the key table cannot serve the batch,
+// the batch gets private copies (freed with it).
 void attach(Batch& b, uint32_t max_gen_slots, const std::vector<const uint8_t*>& pks,
                    std::vector<uint32_t>& pk_slot) {
   Context& c = ctx();
   std::lock_guard<std::mutex> g(c.mu);
   init_locked(c);
-  ensure_generators_locked(c, std::max<uint32_t>(max_gen_slots, 2));
-  lookup_pks_locked(c, pks, pk_slot);
+  const uint32_t gens = std::max<uint32_t>(max_gen_slots, 2);
+  ensure_generators_locked(c, gens);
+  hold_locked(c, b.hold);
   b.tables = c.gen_tables;
-  b.tables16 = c.gen_tables16;
-  b.n16 = c.wide_slots;
-  b.gen_comp = c.gen_comp;
+  if (gens > c.table_slots) {
+    // slots [0, gens) privately: the resident tables copied, the rest built
+    const size_t per = (size_t)COMB_TABLE_WORDS * 4;
+    uint32_t* t = nullptr;
+    check(hipMalloc(&t, (size_t)gens * per));
+    b.owned.push_back(t);
+    check(hipMemcpyAsync(t, c.gen_tables, (size_t)c.table_slots * per, hipMemcpyDeviceToDevice, c.setup));
+    build_comb_tables(c, t, c.table_slots, gens - c.table_slots);
+    b.tables = t;
+  }
   b.pk_bytes = c.pk_bytes;
   b.pk_status = c.pk_status;
   b.pk_lines = c.pk_lines;
+  if (!lookup_pks_locked(c, pks, pk_slot, b.hold)) {
+    // more distinct keys than the table can hold at once: this batch's keys
+    // are prepared privately
+    std::unordered_map<std::string, uint32_t> idx;
+    std::vector<const uint8_t*> keys;
+    pk_slot.assign(pks.size(), 0);
+    for (size_t i = 0; i < pks.size(); i++) {
+      auto r = idx.emplace(std::string(reinterpret_cast<const char*>(pks[i]), 96), (uint32_t)keys.size());
+      if (r.second) keys.push_back(pks[i]);
+      pk_slot[i] = r.first->second;
+    }
+    const size_t n = keys.size();
+    uint8_t *bytes = nullptr, *status = nullptr;
+    uint32_t* lines = nullptr;
+    check(hipMalloc(&bytes, n * 96));
+    b.owned.push_back(bytes);
+    check(hipMalloc(&status, n));
+    b.owned.push_back(status);
+    check(hipMalloc(&lines, n * PK_LINES_WORDS * 4));
+    b.owned.push_back(lines);
+    std::vector<uint32_t> slots(n);
+    for (size_t i = 0; i < n; i++) slots[i] = (uint32_t)i;
+    prepare_pks_locked(c, keys, slots, bytes, status, lines);
+    b.pk_bytes = bytes;
+    b.pk_status = status;
+    b.pk_lines = lines;
+  }
+  b.tables16 = c.gen_tables16;
+  b.n16 = c.wide_slots;
+  b.gen_comp = c.gen_comp;
   b.neg_g2_lines = c.neg_g2_lines;
 }
 
@@ -510,30 +563,67 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
   for (uint32_t k = 0; k < b->nv; k++) T += (uint32_t)items[dev[k]].message_count + 2;
   p.scalars(T);
   p.term_gen.resize(T);
+  // Aggregated checks: every item's check is scaled by its coefficient c
+  // (s + t lambda, s and t 64-bit from the OS CSPRNG, s odd so c != 0), as
+  // e(c A, PK) e(c (B - e A), -G2) = 1, which holds exactly when the item's
+  // own check does (c is invertible mod r).  c (B - e A) comes from the comb
+  // over the scaled term scalars and the product (-c e) A, so the only
+  // extra work per item is the 64-bit-half product c A; the group check adds
+  // the scaled points.
+  const uint32_t group = agg_group_max();
+  const bool scaled = group >= 2;
+  std::vector<uint8_t> rnd(scaled ? 16ull * b->nv : 0);
+  if (scaled) fill_os_random(rnd.data(), rnd.size());
   uint32_t term = 0;
   for (uint32_t k = 0; k < b->nv; k++) {
     const auto& it = items[dev[k]];
     const uint32_t L = (uint32_t)it.message_count;
     size_t sig = p.bytes(it.signature.data, 80);
     size_t hdr = p.bytes(it.header.data, it.header.len);
-    p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, term + 1});
-    for (uint32_t m = 0; m < L; m++) {
-      size_t mo = p.bytes(it.messages[m].data, it.messages[m].len);
-      p.h2s.push_back(H2sJob{mo, (uint32_t)it.messages[m].len, term + 2 + m});
-    }
-    for (uint32_t t = 0; t < L + 2; t++) p.term_gen[term + t] = t;
-    p.fills.push_back(TermFill{term, 0, 0, 2});  // P1 scalar = 1
     uint32_t pa = p.points(1), pb = p.points(1), se = p.scalars(1);
     p.dec_g1.push_back(DecodeG1Job{sig, pa, k});
-    p.dec_fr.push_back(DecodeFrJob{sig + 48, se, k, 1u, 0u});  // store r - e
+    for (uint32_t t = 0; t < L + 2; t++) p.term_gen[term + t] = t;
+    uint32_t pair_a = pa;
+    if (!scaled) {
+      p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, term + 1});
+      for (uint32_t m = 0; m < L; m++) {
+        size_t mo = p.bytes(it.messages[m].data, it.messages[m].len);
+        p.h2s.push_back(H2sJob{mo, (uint32_t)it.messages[m].len, term + 2 + m});
+      }
+      p.fills.push_back(TermFill{term, 0, 0, 2});               // P1 scalar = 1
+      p.dec_fr.push_back(DecodeFrJob{sig + 48, se, k, 1u, 0u});  // store r - e
+    } else {
+      uint64_t s0, t0;
+      std::memcpy(&s0, &rnd[16ull * k], 8);
+      std::memcpy(&t0, &rnd[16ull * k + 8], 8);
+      uint8_t c[32];
+      agg_coefficient(c, s0 | 1, t0);
+      const size_t c_off = p.bytes(c, 32);
+      const uint32_t sc = p.scalars(1), raw = p.scalars(L + 1);
+      p.dec_fr.push_back(DecodeFrJob{c_off, sc, k, 0u, 0u});
+      // domain and message scalars land in raw slots, scaled into the terms
+      p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, raw});
+      for (uint32_t m = 0; m < L; m++) {
+        size_t mo = p.bytes(it.messages[m].data, it.messages[m].len);
+        p.h2s.push_back(H2sJob{mo, (uint32_t)it.messages[m].len, raw + 1 + m});
+      }
+      p.fills.push_back(TermFill{term, sc, 0, 0});  // P1 scalar = c
+      for (uint32_t t = 0; t <= L; t++) p.fills.push_back(TermFill{term + 1 + t, raw + t, sc, 1});
+      p.dec_fr.push_back(DecodeFrJob{sig + 48, se, k, 1u | 4u, 0u, c_off});  // r - c e
+      // c A
+      pair_a = p.points(1);
+      const uint32_t fx = p.partials(VAR_PARTIALS);
+      p.agg_var.push_back(VarJob{pa, sc, fx});
+      p.reduce_to(fx, VAR_PARTIALS, pair_a);
+    }
     uint32_t first = p.n_partials;
-    p.var.push_back(VarJob{pa, se, p.partials(VAR_PARTIALS)});  // (-e) A
-    p.comb(term, L + 2, CH);                                  // B
-    p.reduce_to(first, p.n_partials - first, pb);             // B - e A
-    p.pairs.push_back(PairJob{k, pa, pb, pk_slot[k], PAIR_NEG_G2, 0});
+    p.var.push_back(VarJob{pa, se, p.partials(VAR_PARTIALS)});  // (-e) A, or (-c e) A
+    p.comb(term, L + 2, CH);                                  // B, or c B
+    p.reduce_to(first, p.n_partials - first, pb);             // B - e A, or c (B - e A)
+    p.pairs.push_back(PairJob{k, pair_a, pb, pk_slot[k], PAIR_NEG_G2, 0});
     term += L + 2;
   }
-  aggregate_pairs(p, b->nv, agg_group_max());
+  aggregate_pairs(p, b->nv, group, scaled);
   materialize(*b, p);
   return b;
 }
@@ -1092,27 +1182,38 @@ void launch_batch(Batch& b) {
     launch_pv_challenge(s, b.pvch, b.n_pvch, b.blob, b.disc_index, b.disc_scalar, b.points,
                         b.pinf, b.scalars, b.gate);
   }
-  if (b.n_agg_var) {
-    // sums c_i a_i, c_i b_i of each aggregated group
-    {
-      Timed t(b, "agg_mul");
-      launch_var_mul(s, b.agg_var, b.n_agg_var, b.points, b.pinf, b.scalars, b.partials,
-                     b.vtab_agg);
+  if (b.n_agg_slots) {
+    // sums c_i a_i, c_i b_i of each aggregated group: of the products
+    // computed in the product launch, or of the pre-scaled checks' points
+    if (b.n_pt_copy) {
+      Timed t(b, "agg_copy");
+      launch_point_to_partial(s, b.pt_copy, b.n_pt_copy, b.points, b.pinf, b.partials);
     }
     Timed t(b, "agg_reduce");
     launch_reduce(s, b.groups2, b.out2, b.n_groups2, b.partials, b.points, b.pinf);
   }
   if (b.n_pairs) {
-    {
+    // the group checks first: their results gate the per-item checks, whose
+    // line blocks are then formed only for items of groups that failed
+    if (b.n_gpairs) {
+      {
+        Timed t(b, "pair_lines");
+        launch_pair_lines(s, b.pairs, b.n_gpairs, b.points, b.pinf, b.pk_status, b.pk_lines,
+                          b.neg_g2_lines, b.l2buf);
+      }
+      {
+        Timed t(b, "pairing_groups");
+        launch_pairing(s, b.pairs, b.n_gpairs, b.points, b.pinf, b.pk_status, b.l2buf, b.status,
+                       b.gate, b.valid);
+      }
+      Timed t(b, "pair_lines");
+      launch_pair_lines(s, b.pairs + b.n_gpairs, b.n_pairs - b.n_gpairs, b.points, b.pinf,
+                        b.pk_status, b.pk_lines, b.neg_g2_lines,
+                        b.l2buf + (size_t)b.n_gpairs * PAIR_L2_WORDS, b.valid);
+    } else {
       Timed t(b, "pair_lines");
       launch_pair_lines(s, b.pairs, b.n_pairs, b.points, b.pinf, b.pk_status, b.pk_lines,
                         b.neg_g2_lines, b.l2buf);
-    }
-    // the group checks first: their results gate the per-item checks
-    if (b.n_gpairs) {
-      Timed t(b, "pairing_groups");
-      launch_pairing(s, b.pairs, b.n_gpairs, b.points, b.pinf, b.pk_status, b.l2buf, b.status,
-                     b.gate, b.valid);
     }
     Timed t(b, "pairing");
     launch_pairing(s, b.pairs + b.n_gpairs, b.n_pairs - b.n_gpairs, b.points, b.pinf, b.pk_status,
@@ -1228,6 +1329,12 @@ Batch::~Batch() {
     (void)hipEventDestroy(timer.stops[i]);
   }
   if (aux) (void)hipStreamSynchronize(aux);
+  for (void* p : owned) (void)hipFree(p);
+  if (hold.held) {
+    Context& c = ctx();
+    std::lock_guard<std::mutex> g(c.mu);
+    release_locked(c, hold);
+  }
   if (from_arena) return;  // the arena owns memory and events
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);

@@ -10,6 +10,7 @@
 #include "../../include/cubbs_bbs_gpu_abi.h"
 #include "../../include/cubbs_hip_ext.h"
 #include "bbs_internal.h"
+#include "host_ctx.h"
 
 namespace cubbs {
 
@@ -107,8 +108,8 @@ struct Batch {
   ReduceGroup* groups2 = nullptr;
   uint32_t* out2 = nullptr;
   PairJob* pairs = nullptr;
-  VarJob* agg_var = nullptr;          // products of the aggregated pairing checks
-  uint32_t n_agg_var = 0, n_agg_slots = 0, n_gpairs = 0;
+  PointCopy* pt_copy = nullptr;       // affine inputs of the aggregated group sums
+  uint32_t n_pt_copy = 0, n_agg_slots = 0, n_gpairs = 0;
   uint32_t agg_passed = 0;  // group checks that held at the last fetch of results
   std::vector<uint32_t> pair_group;   // per-item checks: status slot of the group (or PRIM_NO_SLOT)
   struct Level {
@@ -152,7 +153,6 @@ struct Batch {
   uint8_t* out = nullptr;
   uint32_t* l2buf = nullptr;          // per pairing check: the line values -lambda x_P
   uint32_t* vtab = nullptr;           // VAR_WS_WORDS per variable-base product job
-  uint32_t* vtab_agg = nullptr;       // the same for the aggregation products
   uint32_t n_scalars = 0, n_points = 0, n_partials = 0;
   // snapshot of resident state
   const uint32_t* tables = nullptr;
@@ -171,6 +171,10 @@ struct Batch {
   KernelTimer timer;
   // the whole launch_batch() sequence captured once and replayed
   hipGraphExec_t graph = nullptr;
+  // resident state the batch uses (pinned keys, epoch of its snapshot), and
+  // private copies it owns when the resident tables cannot hold its needs
+  ResidentHold hold;
+  std::vector<void*> owned;
   ~Batch();
 };
 

@@ -88,13 +88,22 @@ struct DecodeG1Job {
 
 // Decode a 32-byte big-endian scalar (canonical check) into scalars[out];
 // flags bit0: store r - s instead of s; bit1: reduce mod r instead of
-// rejecting a non-canonical value.
+// rejecting a non-canonical value; bit2: multiply by the canonical 32-byte
+// big-endian scalar at blob + mul_off (before the negation of bit0).
 struct DecodeFrJob {
   uint64_t off;
   uint32_t out;
   uint32_t item;
   uint32_t flags;
   uint32_t pad;
+  uint64_t mul_off;
+};
+
+// partials[partial] = points[point] as a Jacobian point (the identity when
+// pinf[point]): feeds affine results of one reduction into another.
+struct PointCopy {
+  uint32_t point;
+  uint32_t partial;
 };
 
 // partials[partial] = scalars[scalar] * points[point] (identity-aware), on
@@ -217,6 +226,8 @@ void launch_prepare_neg_g2(hipStream_t s, uint32_t* lines);
 // check_subgroup = false: launch_var_mul's extra lanes check the subgroup
 void launch_decode_g1(hipStream_t s, const DecodeG1Job* jobs, uint32_t n, const uint8_t* blob,
                       uint32_t* points, uint8_t* pinf, uint8_t* status, bool check_subgroup = true);
+void launch_point_to_partial(hipStream_t s, const PointCopy* jobs, uint32_t n, const uint32_t* points,
+                             const uint8_t* pinf, uint32_t* partials);
 void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const uint8_t* blob,
                       uint32_t* scalars, uint8_t* status);
 // ws: n * VAR_WS_WORDS words of device workspace.  n_sub > 0: also check
@@ -236,7 +247,8 @@ void launch_compress(hipStream_t s, const CompressJob* jobs, uint32_t n, const u
                      const uint8_t* pinf, uint8_t* out);
 void launch_pair_lines(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
                        const uint8_t* pinf, const uint8_t* pk_status, const uint32_t* pk_lines,
-                       const uint32_t* neg_g2_lines, uint32_t* l2buf);
+                       const uint32_t* neg_g2_lines, uint32_t* l2buf,
+                       const uint8_t* group_valid = nullptr);
 void launch_pairing(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
                     const uint8_t* pinf, const uint8_t* pk_status, uint32_t* l2buf, uint8_t* status,
                     const uint8_t* gate, uint8_t* valid);
@@ -283,7 +295,7 @@ struct Knobs {
   uint32_t comb16_slots = 130;      // CUBBS_HIP_COMB16_GENERATORS
   uint32_t hw_queues = 4;           // GPU_MAX_HW_QUEUES (HIP's own setting)
   uint32_t pk_cache_entries = 4096; // CUBBS_HIP_PK_CACHE_ENTRIES: resident prepared keys
-  uint64_t gen_table_budget = 2ull << 30;  // CUBBS_HIP_GEN_TABLE_BUDGET_BYTES: 8-bit comb tables
+  uint64_t gen_table_budget = 32ull << 30;  // CUBBS_HIP_GEN_TABLE_BUDGET_BYTES: 8-bit comb tables
 };
 const Knobs& knobs();
 

@@ -106,8 +106,38 @@ __global__ __launch_bounds__(256) void k_decode_fr(const DecodeFrJob* __restrict
     status[j.item] = 1;
     fr_zero(s);
   }
+  if (j.flags & 4) {
+    Fr m;
+    fr_from_be_bytes_raw(m, blob + j.mul_off);
+    fr_to_mont(m, m);
+    fr_mul(s, s, m);  // canonical * Montgomery -> canonical product
+  }
   if (j.flags & 1) fr_neg(s, s);
   store_fr(scalars + (size_t)j.out * FR_WORDS, s);
+}
+
+__global__ __launch_bounds__(256) void k_point_to_partial(const PointCopy* __restrict__ jobs,
+                                                         uint32_t n,
+                                                         const uint32_t* __restrict__ points,
+                                                         const uint8_t* __restrict__ pinf,
+                                                         uint32_t* __restrict__ partials) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const PointCopy j = jobs[i];
+  G1J r;
+  if (pinf[j.point]) {
+    pt_set_inf(r);
+  } else {
+    G1A a;
+    load_g1a(a, points + (size_t)j.point * G1A_WORDS);
+    pt_from_affine(r, a);
+  }
+  store_g1j(partials + (size_t)j.partial * G1J_WORDS, r);
+}
+void launch_point_to_partial(hipStream_t s, const PointCopy* jobs, uint32_t n, const uint32_t* points,
+                             const uint8_t* pinf, uint32_t* partials) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_point_to_partial, GRID(n, 64), 0, s, jobs, n, points, pinf, partials);
 }
 void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const uint8_t* blob,
                       uint32_t* scalars, uint8_t* status) {
@@ -415,20 +445,24 @@ void launch_compress(hipStream_t s, const CompressJob* jobs, uint32_t n, const u
 // Line blocks of the pairing checks: one lane per (check, line, pairing)
 // forms m = -lambda x_P and writes the line's 9 second factors
 // (p12::line_values); a pairing with an identity point or an identity key
-// (skipped by the check) gets the constant line 1.
+// (skipped by the check) gets the constant line 1.  With `group_valid`, the
+// checks of items whose group check held (an earlier launch) are skipped:
+// the pairing kernel accepts them without reading their blocks.
 __global__ __launch_bounds__(256) void k_pair_lines(const PairJob* __restrict__ jobs, uint32_t n,
                                                    const uint32_t* __restrict__ points,
                                                    const uint8_t* __restrict__ pinf,
                                                    const uint8_t* __restrict__ pk_status,
                                                    const uint32_t* __restrict__ pk_lines,
                                                    const uint32_t* __restrict__ neg_g2_lines,
-                                                   uint32_t* __restrict__ l2buf) {
+                                                   uint32_t* __restrict__ l2buf,
+                                                   const uint8_t* __restrict__ group_valid) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * (uint32_t)(MILLER_LINES * 2)) return;
   const uint32_t job = t / (MILLER_LINES * 2);
   const uint32_t r = t % (MILLER_LINES * 2);
   const uint32_t idx = r >> 1, q = r & 1;
   const PairJob j = jobs[job];
+  if (group_valid && (j.flags & PAIR_GROUPED) && group_valid[j.group]) return;
   const uint32_t* qlines = q ? (j.pk_b == PAIR_NEG_G2 ? neg_g2_lines
                                                        : pk_lines + (size_t)j.pk_b * PK_LINES_WORDS)
                              : pk_lines + (size_t)j.pk_slot * PK_LINES_WORDS;
@@ -451,11 +485,11 @@ __global__ __launch_bounds__(256) void k_pair_lines(const PairJob* __restrict__ 
 }
 void launch_pair_lines(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
                        const uint8_t* pinf, const uint8_t* pk_status, const uint32_t* pk_lines,
-                       const uint32_t* neg_g2_lines, uint32_t* l2buf) {
+                       const uint32_t* neg_g2_lines, uint32_t* l2buf, const uint8_t* group_valid) {
   if (!n) return;
   const uint32_t nl = n * (uint32_t)(MILLER_LINES * 2);
   hipLaunchKernelGGL(k_pair_lines, GRID(nl, 64), 0, s, jobs, n, points, pinf, pk_status, pk_lines,
-                     neg_g2_lines, l2buf);
+                     neg_g2_lines, l2buf, group_valid);
 }
 
 __global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ jobs, uint32_t n,

@@ -43,7 +43,8 @@ struct Plan {
   // aggregated pairing checks: products c_i a_i, c_i b_i of the grouped
   // checks (summed per group by groups2 / out2), extra status slots for the
   // groups, and how many of pairs[] (at its front) are group checks
-  std::vector<VarJob> agg_var;
+  std::vector<VarJob> agg_var;       // run in the same launch as `var`, after it
+  std::vector<PointCopy> pt_copy;    // affine inputs of the group sums (pre-scaled checks)
   uint32_t n_agg_slots = 0, n_group_pairs = 0;
   std::vector<PipTerm> pip_terms;  // terms of the Pippenger MSMs
   std::vector<uint32_t> pip_out;   // per Pippenger MSM: its output partial slot
@@ -110,7 +111,10 @@ void materialize(Batch& b, Plan& p);
 // Group the plain pairing checks e(a_i, PK) e(b_i, -G2) = 1 of p.pairs by key
 // into groups of at most group_max (status slots from nv on) and add their
 // random-linear-combination checks; see bbs_batch.cpp.
-void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max);
+// scaled: the checks' points are already multiplied by their coefficients
+// (the planner drew them), so the group sums add the points themselves;
+// otherwise the products c_i a_i, c_i b_i are planned here.
+void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, bool scaled = false);
 uint32_t agg_group_max();
 // Lock the context, make generators [0, max_gen_slots) and the given public
 // keys resident, snapshot the resident pointers into the batch.

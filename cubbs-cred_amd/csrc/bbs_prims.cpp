@@ -109,6 +109,7 @@ std::unique_ptr<Batch> stage(Prog& g) {
     pj.pk_slot = pk_slot[pj.pk_slot];
     if (pj.pk_b != PAIR_NEG_G2) pj.pk_b = pk_slot[pj.pk_b];
   }
+  for (auto& d : g.p.dom) d.pk_slot = pk_slot[d.pk_slot];  // the same for domains
   if (g.p.term_gen.size() < g.p.n_scalars) g.p.term_gen.resize(g.p.n_scalars);
   materialize(*b, g.p);
   return b;
@@ -843,8 +844,7 @@ int32_t cubbs_bbs_gpu_prepare_b_batch_v2(uint32_t abi_version, CuBbsBbsGpuPrepar
       g.p.fills.push_back(TermFill{sc, 0, 0, 2});  // P1 scalar 1
       const uint32_t pk = (uint32_t)g.g2.size();
       g.g2.push_back(it.public_key.data);
-      // pk_slot is an index into g.g2 until run() maps it; the domain job
-      // needs the cache slot, so the mapping is repeated there
+      // pk_slot is an index into g.g2 until stage() maps it
       g.p.dom.push_back(DomainJob{g.p.bytes(it.header.data, it.header.len), (uint32_t)it.header.len,
                                   pk, L, sc + 1});
       for (uint32_t m = 0; m < L; m++) g.scalar_at(sc + 2 + m, it.message_scalars[m].data, st[i], 0);
@@ -854,16 +854,6 @@ int32_t cubbs_bbs_gpu_prepare_b_batch_v2(uint32_t abi_version, CuBbsBbsGpuPrepar
       g.comb(sc, gen);
       off[i] = g.compress(g.finish(first));
     }
-    // map the domain jobs' public-key indexes to cache slots
-    {
-      Context& c = ctx();
-      std::lock_guard<std::mutex> lk(c.mu);
-      init_locked(c);
-      std::vector<uint32_t> slots;
-      lookup_pks_locked(c, g.g2, slots);
-      for (auto& d : g.p.dom) d.pk_slot = slots[d.pk_slot];
-    }
-    g.g2.clear();
     Ran ran = run(g);
     for (size_t i = 0; i < item_count; i++) {
       if (ran.status[st[i]]) {

@@ -195,9 +195,20 @@ int32_t copy_request(const CubbsHipRequest& q, Req& r) {
   }
 }
 
+// Device bytes of the library's resident state (tables, keys, retired buffers).
+size_t resident_bytes() {
+  Context& c = ctx();
+  std::lock_guard<std::mutex> g(c.mu);
+  return c.resident_bytes;
+}
+
 struct Server {
   CubbsHipServerConfig cfg{};
-  size_t share = 0;  // device bytes per slot
+  // a slot's share of the budget once the resident state is counted
+  size_t slot_share() const {
+    const size_t res = resident_bytes();
+    return cfg.hbm_budget_bytes > res ? (size_t)((cfg.hbm_budget_bytes - res) / cfg.slots) : 0;
+  }
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Req*> pending;
@@ -325,6 +336,7 @@ void Server::worker(unsigned slot) {
       cv_work.wait(lk, [&] { return stop || !pending.empty(); });
       if (pending.empty() && stop) return;
       size_t bytes = 0;
+      const size_t share = slot_share();
       while (!pending.empty() && take.size() < cfg.max_batch) {
         Req* r = pending.front();
         if (!take.empty() && bytes + r->est > share) break;
@@ -444,7 +456,7 @@ void* cubbs_hip_server_create(const CubbsHipServerConfig* cfg, int32_t* status_o
       std::lock_guard<std::mutex> g(c.mu);
       pool = (uint32_t)c.pool.size();
     }
-    CubbsHipServerConfig d{std::max<uint32_t>(1, pool / 8), 4096, 4ull << 30};
+    CubbsHipServerConfig d{std::max<uint32_t>(1, pool / 8), 4096, cubbs::resident_bytes() + (4ull << 30)};
     if (cfg) {
       if (cfg->slots) d.slots = cfg->slots;
       if (cfg->max_batch) d.max_batch = cfg->max_batch;
@@ -452,7 +464,6 @@ void* cubbs_hip_server_create(const CubbsHipServerConfig* cfg, int32_t* status_o
     }
     d.slots = std::min<uint32_t>(d.slots, 16);
     s->cfg = d;
-    s->share = (size_t)(d.hbm_budget_bytes / d.slots);
     for (unsigned i = 0; i < d.slots; i++) s->workers.emplace_back([s, i] { s->worker(i); });
     set(CUBBS_BBS_GPU_SUCCESS);
     return s;

@@ -26,16 +26,25 @@
 
 namespace p12 {
 
-constexpr int E = FPW;                                   // words per LDS entry
+constexpr int E = FPW;                                   // words of an Fp value
 enum { EY0, EY1, E2Y0, E2Y1, ED, ES, ENS, EN1, NE };     // derived entries per slot
-constexpr int SLOT = NE * E;
+// LDS layout, padded so that the 60 working lanes' reads of one product term
+// spread over the 32 banks: entries LE words apart, slots SLOT words apart,
+// checks LDS_WORDS apart.  With the unpadded strides (14, 112, 840) the
+// lanes of a check hit only a few banks (slot stride 112 = 16 mod 32), which
+// cost about 3.5 LDS cycles per access against 1.37 with these (a model of
+// every access pattern below over 32 banks per 32 lanes).
+constexpr int LE = 15;
+constexpr int SLOT = 127;
 constexpr int REGION = 6 * SLOT;                         // rich entries of one operand
-constexpr int RAW = 6 * 2 * E;                           // plain coefficients of one operand
-constexpr int LDS_WORDS = REGION + RAW;                  // per check
+constexpr int RAW = 6 * 2 * LE;                          // plain coefficients of one operand
+constexpr int LDS_WORDS = 949;                           // per check
+static_assert(LE >= E && SLOT >= NE * LE && LDS_WORDS >= REGION + RAW, "LDS layout");
 constexpr int CHECKS_PER_BLOCK = 5;
 // LDS of a 64-thread block: 5 checks and one shared zero entry (the 4 spare
-// lanes alias the last check's region and never store)
-constexpr int BLOCK_LDS_WORDS = CHECKS_PER_BLOCK * LDS_WORDS + E;
+// lanes alias the last check's region and never store); 19,040 bytes, so the
+// eight blocks of a CU at two per SIMD fit its 160 KiB
+constexpr int BLOCK_LDS_WORDS = CHECKS_PER_BLOCK * LDS_WORDS + LE;
 // Per check, line and pairing: the second factors of the sparse line
 // product, so that the Miller loop forms no derived values.  With
 // m = -lambda x_P (Fp2) and c the line's constant (Fp2):
@@ -161,10 +170,10 @@ DEV void put_rich(lds_t* R, const Fp& v, const Lane& L) {
   sel(t2, L.c, y1, z);
   lz5(t3, a, b, L.c ? 4 : 0, s2, t2);  // c=0: y0 + y1   c=1: 4p - y0 - y1
   if (L.wr) {
-    st(S + (L.c ? EY1 : EY0) * E, v);
-    st(S + (L.c ? E2Y1 : E2Y0) * E, t1);
-    st(S + (L.c ? EN1 : ED) * E, t2v);
-    st(S + (L.c ? ENS : ES) * E, t3);
+    st(S + (L.c ? EY1 : EY0) * LE, v);
+    st(S + (L.c ? E2Y1 : E2Y0) * LE, t1);
+    st(S + (L.c ? EN1 : ED) * LE, t2v);
+    st(S + (L.c ? ENS : ES) * LE, t3);
   }
 }
 
@@ -173,7 +182,7 @@ DEV void put_rich(lds_t* R, const Fp& v, const Lane& L) {
 DEV void mul_in(Fp& f, const Fp& g, const Lane& L) {
   __syncthreads();
   put_rich(L.F, g, L);
-  if (L.wr) st(L.X + (2 * L.k + L.c) * E, f);
+  if (L.wr) st(L.X + (2 * L.k + L.c) * LE, f);
   __syncthreads();
   Wide w;
   w_zero(w);
@@ -182,10 +191,10 @@ DEV void mul_in(Fp& f, const Fp& g, const Lane& L) {
     int j = L.k - i;
     const bool xi = j < 0;
     j += xi ? 6 : 0;
-    const lds_t* X = L.X + 2 * i * E;
+    const lds_t* X = L.X + 2 * i * LE;
     const lds_t* Y = L.F + j * SLOT;
-    mac_at(w, X, Y + ysel0(L.c, xi) * E);
-    mac_at(w, X + E, Y + ysel1(L.c, xi) * E);
+    mac_at(w, X, Y + ysel0(L.c, xi) * LE);
+    mac_at(w, X + LE, Y + ysel1(L.c, xi) * LE);
   }
   w_reduce(f, w);
 }
@@ -212,10 +221,10 @@ DEV void sqr(Fp& f, const Lane& L) {
     const bool two = SQ_TERMS[L.k][t][2] != 0;
     const bool xi = SQ_TERMS[L.k][t][3] != 0;
     const bool none = i < 0;
-    const lds_t* x0 = none ? L.Z : L.F + i * SLOT + (two ? E2Y0 : EY0) * E;
-    const lds_t* x1 = none ? L.Z : L.F + i * SLOT + (two ? E2Y1 : EY1) * E;
-    const lds_t* y0 = none ? L.Z : L.F + j * SLOT + ysel0(L.c, xi) * E;
-    const lds_t* y1 = none ? L.Z : L.F + j * SLOT + ysel1(L.c, xi) * E;
+    const lds_t* x0 = none ? L.Z : L.F + i * SLOT + (two ? E2Y0 : EY0) * LE;
+    const lds_t* x1 = none ? L.Z : L.F + i * SLOT + (two ? E2Y1 : EY1) * LE;
+    const lds_t* y0 = none ? L.Z : L.F + j * SLOT + ysel0(L.c, xi) * LE;
+    const lds_t* y1 = none ? L.Z : L.F + j * SLOT + ysel1(L.c, xi) * LE;
     mac_at(w, x0, y0);
     mac_at(w, x1, y1);
   }
@@ -271,16 +280,16 @@ DEV void mul_line(Fp& f, const uint32_t* v, const Fp& yp, const Lane& L) {
   w_zero(w);
   {
     Fp x0, x1;
-    ld(x0, L.F + L.k * SLOT + EY0 * E);
-    ld(x1, L.F + L.k * SLOT + EY1 * E);
+    ld(x0, L.F + L.k * SLOT + EY0 * LE);
+    ld(x1, L.F + L.k * SLOT + EY1 * LE);
     w_mac(w, x0, a0);
     w_mac(w, x1, a1);
   }
   {
     const int i = xi ? L.k + 4 : L.k - 2;
     Fp x0, x1;
-    ld(x0, L.F + i * SLOT + EY0 * E);
-    ld(x1, L.F + i * SLOT + EY1 * E);
+    ld(x0, L.F + i * SLOT + EY0 * LE);
+    ld(x1, L.F + i * SLOT + EY1 * LE);
     w_mac(w, x0, b0);
     w_mac(w, x1, b1);
   }
@@ -288,7 +297,7 @@ DEV void mul_line(Fp& f, const uint32_t* v, const Fp& yp, const Lane& L) {
     const bool xi3 = L.k < 3;
     const int i = xi3 ? L.k + 3 : L.k - 3;
     Fp x;
-    ld(x, L.F + i * SLOT + ysel0(L.c, xi3) * E);
+    ld(x, L.F + i * SLOT + ysel0(L.c, xi3) * LE);
     w_mac(w, x, yp);
   }
   w_reduce(f, w);
@@ -325,8 +334,8 @@ DEV void cyc_sqr(Fp& f, const Lane& L) {
 #pragma unroll 1
   for (int t = 0; t < 3; t++) {
     const int8_t* T = GS_TERMS[ty][L.c][t];
-    const lds_t* x = T[0] == 2 ? L.Z : L.F + (P + 3 * T[0]) * SLOT + T[1] * E;
-    const lds_t* y = T[2] == 2 ? L.Z : L.F + (P + 3 * T[2]) * SLOT + T[3] * E;
+    const lds_t* x = T[0] == 2 ? L.Z : L.F + (P + 3 * T[0]) * SLOT + T[1] * LE;
+    const lds_t* y = T[2] == 2 ? L.Z : L.F + (P + 3 * T[2]) * SLOT + T[3] * LE;
     mac_at(w, x, y);
   }
   Fp s;
@@ -411,18 +420,18 @@ DEVN void inv(Fp& f, Lane L) {
   conj(g, L);
   Fp n = f;
   mul_in(n, g, L);  // N
-  lds_t* X = L.X;   // N's coefficients: slot s at X + 2 s E (c0), + E (c1)
-  lds_t* T = L.F;   // t0, t1, t2, d^-1: Fp2 j at T + 2 j E
+  lds_t* X = L.X;   // N's coefficients: slot s at X + 2 s LE (c0), + LE (c1)
+  lds_t* T = L.F;   // t0, t1, t2, d^-1: Fp2 j at T + 2 j LE
   auto ld2 = [&](Fp2& r, const lds_t* q) {
     ld(r.c0, q);
-    ld(r.c1, q + E);
+    ld(r.c1, q + LE);
   };
   auto st2 = [&](lds_t* q, const Fp2& v) {
     st(q, v.c0);
-    st(q + E, v.c1);
+    st(q + LE, v.c1);
   };
   __syncthreads();
-  if (L.wr) st(X + (2 * L.k + L.c) * E, n);
+  if (L.wr) st(X + (2 * L.k + L.c) * LE, n);
   __syncthreads();
   if (L.c == 0 && L.k < 3 && L.wr) {
     // t_k = sq - pr: sq = n_k^2 (times xi for k = 1), pr = the cross product
@@ -430,24 +439,24 @@ DEVN void inv(Fp& f, Lane L) {
     const int sq_slot = L.k == 0 ? 0 : (L.k == 1 ? 4 : 2);
     const int pa = L.k == 0 ? 2 : 0, pb = L.k == 1 ? 2 : 4;
     Fp2 a, b, x, y;
-    ld2(a, X + 2 * sq_slot * E);
+    ld2(a, X + 2 * sq_slot * LE);
     fp2_sqr(x, a);
     if (L.k == 1) fp2_mul_xi(x, x);
-    ld2(a, X + 2 * pa * E);
-    ld2(b, X + 2 * pb * E);
+    ld2(a, X + 2 * pa * LE);
+    ld2(b, X + 2 * pb * LE);
     fp2_mul(y, a, b);
     if (L.k == 0) fp2_mul_xi(y, y);
     fp2_sub(x, x, y);
-    st2(T + 2 * L.k * E, x);
+    st2(T + 2 * L.k * LE, x);
   }
   __syncthreads();
   if (L.k == 0 && L.c == 0 && L.wr) {
     Fp2 a, b, d, s;
-    ld2(a, X + 2 * 2 * E);  // n1
-    ld2(b, T + 2 * 2 * E);  // t2
+    ld2(a, X + 2 * 2 * LE);  // n1
+    ld2(b, T + 2 * 2 * LE);  // t2
     fp2_mul(s, a, b);
-    ld2(a, X + 2 * 4 * E);  // n2
-    ld2(b, T + 2 * 1 * E);  // t1
+    ld2(a, X + 2 * 4 * LE);  // n2
+    ld2(b, T + 2 * 1 * LE);  // t1
     fp2_mul(d, a, b);
     fp2_add(s, s, d);
     fp2_mul_xi(s, s);
@@ -463,7 +472,7 @@ DEVN void inv(Fp& f, Lane L) {
     fp_mul(d.c0, d.c0, nrm);
     fp_mul(d.c1, d.c1, nrm);
     fp_neg(d.c1, d.c1);
-    st2(T + 2 * 3 * E, d);
+    st2(T + 2 * 3 * LE, d);
   }
   __syncthreads();
   // this lane's coefficient of N^-1: component c of t_{k/2} d^-1 for even k
@@ -471,10 +480,10 @@ DEVN void inv(Fp& f, Lane L) {
   {
     Fp t0, t1, d0, d1, p0, p1;
     const int j = (L.k >> 1) % 3;
-    ld(t0, T + 2 * j * E);
-    ld(t1, T + 2 * j * E + E);
-    ld(d0, T + 2 * 3 * E);
-    ld(d1, T + 2 * 3 * E + E);
+    ld(t0, T + 2 * j * LE);
+    ld(t1, T + 2 * j * LE + LE);
+    ld(d0, T + 2 * 3 * LE);
+    ld(d1, T + 2 * 3 * LE + LE);
     // c = 0: t0 d0 - t1 d1, c = 1: t0 d1 + t1 d0
     fp_mul(p0, t0, L.c ? d1 : d0);
     fp_mul(p1, t1, L.c ? d0 : d1);

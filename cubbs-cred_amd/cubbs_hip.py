@@ -512,6 +512,19 @@ def tune_load(text):
     return lib().cubbs_hip_tune_load(text.encode())
 
 
+RESIDENT_FIELDS = ("resident_bytes", "pk_capacity", "pk_entries", "pk_pinned", "generator_slots",
+                   "table_slots", "table_max_slots", "wide_slots", "retired_buffers", "live_batches")
+
+
+def resident_stats():
+    """The library's resident device state (cubbs_hip_resident_stats)."""
+    v = (ctypes.c_uint64 * len(RESIDENT_FIELDS))()
+    rc = lib().cubbs_hip_resident_stats(v)
+    if rc:
+        raise RuntimeError("resident_stats failed %d" % rc)
+    return dict(zip(RESIDENT_FIELDS, list(v)))
+
+
 def force_shapes(pip_c=0, pip_s=0, comb_chunk=0):
     """Launch shapes forced on the calling thread for its next calls (0 = not forced)."""
     return lib().cubbs_hip_test_force_shapes(ctypes.c_uint32(pip_c), ctypes.c_uint32(pip_s),

@@ -129,13 +129,41 @@ int32_t cubbs_hip_batch_work(void *batch, CubbsHipBatchWork *out);
  * Items of a group that held are accepted without their own pairing check. */
 int32_t cubbs_hip_batch_group_checks(void *batch, uint64_t *groups, uint64_t *passed);
 
+/* ---- resident state: what the library keeps on the device across calls
+ * (generator points and comb tables, prepared public keys, -G2 lines, and
+ * buffers replaced by a growth step that live batches may still use).  The
+ * prepared keys live in a fixed table of CUBBS_HIP_PK_CACHE_ENTRIES entries
+ * (default 4096; least recently used entries that no live batch uses are
+ * replaced); the 8-bit comb tables are bounded by
+ * CUBBS_HIP_GEN_TABLE_BUDGET_BYTES (default 32 GiB).  A batch that needs more
+ * distinct keys or generator tables than these hold at once gets private
+ * copies for its lifetime.  Both settings are read once, at the first call. */
+typedef struct CubbsHipResidentStats {
+  uint64_t resident_bytes;       /* device bytes held, retired buffers included */
+  uint64_t pk_capacity;          /* entries of the key table (0 before the first key) */
+  uint64_t pk_entries;           /* entries holding a key */
+  uint64_t pk_pinned;            /* entries used by a live batch */
+  uint64_t generator_slots;      /* generator points made (P1, Q1, H_1..) */
+  uint64_t table_slots;          /* generators with a resident 8-bit comb table */
+  uint64_t table_max_slots;      /* the budget, in generators */
+  uint64_t wide_slots;           /* generators with a 16-bit comb table */
+  uint64_t retired_buffers;      /* replaced buffers not yet freed */
+  uint64_t live_batches;         /* batches holding resident state */
+} CubbsHipResidentStats;
+int32_t cubbs_hip_resident_stats(CubbsHipResidentStats *out);
+
 /* ---- serving loop: continuous batching of mixed requests.
  * `slots` worker slots, each with its own stream and reusable device/pinned
  * memory; a free slot takes every pending request up to max_batch and its
  * share of the HBM budget (hbm_budget_bytes / slots), runs one sub-batch per
  * operation kind and completes them, while other slots' batches run on the
- * device.  Zero fields take the defaults (slots: the library's work streams
- * / 8, i.e. 2 with GPU_MAX_HW_QUEUES=16; 4096; 4 GiB). */
+ * device.  The budget covers the library's resident state too (see
+ * cubbs_hip_resident_stats): a slot's share is (budget - resident bytes) /
+ * slots at the time it admits a batch, and a slot always admits at least one
+ * request.  A request whose estimate exceeds the whole budget is refused at
+ * submission (code 1).  Zero fields take the defaults (slots: the library's
+ * work streams / 8, i.e. 2 with GPU_MAX_HW_QUEUES=16; 4096; the resident
+ * bytes at creation + 4 GiB). */
 typedef struct CubbsHipServerConfig {
   uint32_t slots;
   uint32_t max_batch;

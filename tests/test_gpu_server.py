@@ -117,9 +117,9 @@ def test_server_submit_wait_and_item_errors(gpu):
 
 
 def test_server_hbm_budget_bounds_batches(gpu):
-    """Admission follows the HBM budget: with a share per slot below one
-    request's estimate a batch holds one such request (a request above the
-    share still runs, alone); with room they batch.  Results equal the
+    """Admission follows the HBM budget (resident state included): with a
+    share per slot below one request's estimate a batch holds one such request
+    (a request above the share still runs, alone); with room they batch.  Results equal the
     requests run alone either way."""
     g = gpu
     pool = _pool(g, b"server-budget", n_verify=6, n_pv=4, n_sign=3, n_pg=3)
@@ -128,7 +128,9 @@ def test_server_hbm_budget_bounds_batches(gpu):
     arrivals = [0] * len(order)  # all pending at once
     # 4 slots x 60 KiB: every Verify / ProofVerify / ProofGen request is above
     # a slot's share (pairing line blocks, product tables) but below the budget
-    for slots, budget, one_each in ((4, 4 * 60 * 1024, True), (2, 2 * (1 << 30), False)):
+    # the budget covers the library's resident state too
+    roomy = g.resident_stats()["resident_bytes"] + 2 * (1 << 30)
+    for slots, budget, one_each in ((4, 4 * 60 * 1024, True), (2, roomy, False)):
         srv = g.Server(slots=slots, max_batch=64, hbm_budget=budget)
         try:
             rc, res, lat, outs = srv.run_open_loop(pool, order, arrivals, outputs=True)
