@@ -435,6 +435,22 @@ def messages_to_scalars(msgs, dst=MAP_DST):
     return rc, [o.raw[:32] for o in outs]
 
 
+def messages_to_scalars_noalloc(msgs, dst=MAP_DST):
+    """cubbs_hip_messages_to_scalar_noalloc_batch: (status, scalars, results)."""
+    keep = _Keep()
+    items = (MessagesToScalarItem * max(1, len(msgs)))()
+    outs = []
+    for i, m in enumerate(msgs):
+        items[i].message = keep.slice(m)
+        items[i].dst = keep.slice(dst)
+        buf, bb = keep.out(32)
+        items[i].scalar_out = bb
+        outs.append(buf)
+    res = (Result * max(1, len(msgs)))()
+    rc = lib().cubbs_hip_messages_to_scalar_noalloc_batch(ABI_VERSION, items, ctypes.c_size_t(len(msgs)), res)
+    return rc, [o.raw[:32] for o in outs], [(res[i].code, res[i].bytes_written) for i in range(len(msgs))]
+
+
 def _packed(msgs):
     data = b"".join(msgs)
     offs, cur = [], 0

@@ -36,6 +36,24 @@ int32_t cubbs_hip_proof_gen_with_scalars_batch(uint32_t abi_version,
                                                const uint8_t *scalars_be, const size_t *offsets,
                                                CuBbsBbsGpuResult *results);
 
+/* cubbs_bbs_gpu_messages_to_scalar_batch_v2 without allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same validation, results and outputs as the allocating entry,
+ * on the same hash kernel; the device memory is one static array of
+ * CUBBS_HIP_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) and
+ * the host staging a static buffer of the same size.  A call whose packed
+ * job records, DST, message bytes and 32-byte outputs (each part rounded up
+ * to 256 bytes) exceed that size returns CUBBS_BBS_GPU_ERR_INVALID_INPUT
+ * without touching the device.  Calls share the array, so they are serialised
+ * by a lock (overlapping calls would race on it).  Unlike the allocating entry
+ * it does not record the scalars for
+ * cubbs_bbs_gpu_sign_e_from_resident_scalars_batch_v2. */
+#define CUBBS_HIP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int32_t cubbs_hip_messages_to_scalar_noalloc_batch(uint32_t abi_version,
+                                                   CuBbsBbsGpuMessagesToScalarItem *items,
+                                                   size_t item_count, CuBbsBbsGpuResult *results);
+
 /* ---- staged batches: prepare (host packing + upload), launch (enqueue the
  * device pipeline on the batch's own stream), sync, read results, free.
  * kind: 0 = Verify, 1 = ProofVerify, 16 = item MSMs (CuBbsBbsGpuG1MsmItem, as
