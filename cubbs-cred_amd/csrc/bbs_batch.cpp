@@ -484,10 +484,12 @@ void attach(Batch& b, uint32_t max_gen_slots, const std::vector<const uint8_t*>&
     build_comb_tables(c, t, c.table_slots, gens - c.table_slots);
     b.tables = t;
   }
+  // the key table is allocated by the first lookup: snapshot it afterwards
+  const bool resident = lookup_pks_locked(c, pks, pk_slot, b.hold);
   b.pk_bytes = c.pk_bytes;
   b.pk_status = c.pk_status;
   b.pk_lines = c.pk_lines;
-  if (!lookup_pks_locked(c, pks, pk_slot, b.hold)) {
+  if (!resident) {
     // more distinct keys than the table can hold at once: this batch's keys
     // are prepared privately
     std::unordered_map<std::string, uint32_t> idx;

@@ -226,10 +226,66 @@ struct Server {
   std::atomic<uint64_t> failed_batches{0};  // batches failed alone (memory, host errors)
 
   void worker(unsigned slot);
-  void run_batch(std::vector<Req*>& take, Arena& ar);
+  void run_batch(std::vector<Req*>& take, Arena& ar, SlotGraphs* sg);
 };
 
-void Server::run_batch(std::vector<Req*>& take, Arena& ar) {
+// The instantiated graph of each sub-batch kind of a slot, and the private
+// streams its launch sequences are captured on.
+struct SlotGraphs {
+  hipStream_t cap[4] = {}, cap_aux[4] = {};
+  hipGraphExec_t exec[4] = {};
+  ~SlotGraphs() {
+    for (int k = 0; k < 4; k++) {
+      if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+      if (cap[k]) (void)hipStreamDestroy(cap[k]);
+      if (cap_aux[k]) (void)hipStreamDestroy(cap_aux[k]);
+    }
+  }
+};
+
+// Launch a planned sub-batch: captured and replayed through the slot's graph
+// of its kind, or kernel by kernel.
+void launch_sub(Batch& b, SlotGraphs* sg, int k) {
+  if (!sg) {
+    launch_batch(b);
+    return;
+  }
+  if (!sg->cap[k]) {
+    check(hipStreamCreateWithFlags(&sg->cap[k], hipStreamNonBlocking));
+    check(hipStreamCreateWithFlags(&sg->cap_aux[k], hipStreamNonBlocking));
+  }
+  const hipStream_t s0 = b.stream, a0 = b.aux;
+  b.stream = sg->cap[k];
+  b.aux = a0 ? sg->cap_aux[k] : nullptr;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    try {
+      launch_batch(b);
+    } catch (const GpuError&) {
+      e = hipErrorUnknown;
+    }
+    const hipError_t e2 = hipStreamEndCapture(b.stream, &g);
+    if (e == hipSuccess) e = e2;
+  }
+  b.stream = s0;
+  b.aux = a0;
+  if (e == hipSuccess && sg->exec[k]) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult r{};
+    if (hipGraphExecUpdate(sg->exec[k], g, &bad, &r) != hipSuccess || r != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(sg->exec[k]);
+      sg->exec[k] = nullptr;
+    }
+  }
+  if (e == hipSuccess && !sg->exec[k]) e = hipGraphInstantiate(&sg->exec[k], g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  check(e);
+  check(hipGraphLaunch(sg->exec[k], b.stream));
+}
+
+void Server::run_batch(std::vector<Req*>& take, Arena& ar, SlotGraphs* sg) {
   std::vector<Req*> by[4];
   for (Req* r : take) by[r->kind].push_back(r);
   std::vector<std::unique_ptr<Batch>> bs(4);
@@ -243,6 +299,7 @@ void Server::run_batch(std::vector<Req*>& take, Arena& ar) {
     ~OnDevice() {
       --n;
       set_thread_arena(nullptr);
+      set_thread_agg_group(0);
     }
   } guard{on_device};
   const uint64_t od = ++on_device;
@@ -255,13 +312,14 @@ void Server::run_batch(std::vector<Req*>& take, Arena& ar) {
   uint64_t t_plan = 0;
   auto planned = [&](uint64_t ts) { t_plan += now_ns() - ts; };
   set_thread_arena(&ar);
+  set_thread_agg_group((cfg.flags & CUBBS_HIP_SERVER_AGGREGATE) ? 8u : 0u);
   if (!by[CUBBS_HIP_REQ_VERIFY].empty()) {
     for (Req* r : by[CUBBS_HIP_REQ_VERIFY]) vi.push_back(r->v);
     ar.cur = 0;
     const uint64_t ts = now_ns();
     bs[0] = plan_verify(vi.data(), vi.size());
     planned(ts);
-    launch_batch(*bs[0]);
+    launch_sub(*bs[0], sg, 0);
   }
   if (!by[CUBBS_HIP_REQ_PROOF_VERIFY].empty()) {
     for (Req* r : by[CUBBS_HIP_REQ_PROOF_VERIFY]) pvi.push_back(r->pv);
@@ -269,15 +327,16 @@ void Server::run_batch(std::vector<Req*>& take, Arena& ar) {
     const uint64_t ts = now_ns();
     bs[1] = plan_proof_verify(pvi.data(), pvi.size());
     planned(ts);
-    launch_batch(*bs[1]);
+    launch_sub(*bs[1], sg, 1);
   }
+  set_thread_agg_group(0);
   if (!by[CUBBS_HIP_REQ_SIGN].empty()) {
     for (Req* r : by[CUBBS_HIP_REQ_SIGN]) si.push_back(r->s);
     ar.cur = 2;
     const uint64_t ts = now_ns();
     bs[2] = plan_sign(si.data(), si.size());
     planned(ts);
-    launch_batch(*bs[2]);
+    launch_sub(*bs[2], sg, 2);
   }
   if (!by[CUBBS_HIP_REQ_PROOF_GEN].empty()) {
     for (Req* r : by[CUBBS_HIP_REQ_PROOF_GEN]) {
@@ -288,7 +347,7 @@ void Server::run_batch(std::vector<Req*>& take, Arena& ar) {
     const uint64_t ts = now_ns();
     bs[3] = plan_proof_gen(pgi.data(), pgi.size(), 1, seeds.data(), nullptr, nullptr);
     planned(ts);
-    launch_batch(*bs[3]);
+    launch_sub(*bs[3], sg, 3);
   }
   set_thread_arena(nullptr);
   const uint64_t held = ar.dev_bytes;
@@ -312,6 +371,8 @@ void Server::run_batch(std::vector<Req*>& take, Arena& ar) {
 void Server::worker(unsigned slot) {
   (void)slot;
   Arena ar;
+  SlotGraphs graphs;
+  SlotGraphs* sg = (cfg.flags & CUBBS_HIP_SERVER_GRAPHS) ? &graphs : nullptr;
   try {
     {
       Context& c = ctx();
@@ -352,7 +413,7 @@ void Server::worker(unsigned slot) {
     bool failed = false;
     if (fault == 0) {
       try {
-        run_batch(take, ar);
+        run_batch(take, ar, sg);
       } catch (const GpuError& e) {
         failed = true;
         if (e.err != hipErrorOutOfMemory) fault = CUBBS_BBS_GPU_ERR_INTERNAL;
@@ -365,6 +426,11 @@ void Server::worker(unsigned slot) {
         for (int i = 0; i < 4; i++) {
           if (ar.streams[i]) (void)hipStreamSynchronize(ar.streams[i]);
           if (ar.aux[i]) (void)hipStreamSynchronize(ar.aux[i]);
+        }
+        for (int k = 0; k < 4; k++) {
+          if (graphs.cap[k]) (void)hipStreamSynchronize(graphs.cap[k]);
+          if (graphs.exec[k]) (void)hipGraphExecDestroy(graphs.exec[k]);
+          graphs.exec[k] = nullptr;
         }
         (void)hipGetLastError();
         ar.reset();
@@ -446,6 +512,10 @@ void* cubbs_hip_server_create(const CubbsHipServerConfig* cfg, int32_t* status_o
       std::lock_guard<std::mutex> g(c.mu);
       cubbs::init_locked(c);
     }
+    if (cfg && (cfg->flags & ~(CUBBS_HIP_SERVER_GRAPHS | CUBBS_HIP_SERVER_AGGREGATE))) {
+      set(CUBBS_BBS_GPU_ERR_INVALID_INPUT);
+      return nullptr;
+    }
     auto* s = new Server();
     // default slots: as many as the stream pool serves without two slots
     // sharing a stream (a slot uses 8), since a launch on a stream that is
@@ -456,11 +526,12 @@ void* cubbs_hip_server_create(const CubbsHipServerConfig* cfg, int32_t* status_o
       std::lock_guard<std::mutex> g(c.mu);
       pool = (uint32_t)c.pool.size();
     }
-    CubbsHipServerConfig d{std::max<uint32_t>(1, pool / 8), 4096, cubbs::resident_bytes() + (4ull << 30)};
+    CubbsHipServerConfig d{std::max<uint32_t>(1, pool / 8), 4096, cubbs::resident_bytes() + (4ull << 30), 0, 0};
     if (cfg) {
       if (cfg->slots) d.slots = cfg->slots;
       if (cfg->max_batch) d.max_batch = cfg->max_batch;
       if (cfg->hbm_budget_bytes) d.hbm_budget_bytes = cfg->hbm_budget_bytes;
+      d.flags = cfg->flags;
     }
     d.slots = std::min<uint32_t>(d.slots, 16);
     s->cfg = d;

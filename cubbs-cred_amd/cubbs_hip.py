@@ -201,7 +201,11 @@ class T2PrepareItem(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
-                ("hbm_budget_bytes", ctypes.c_uint64)]
+                ("hbm_budget_bytes", ctypes.c_uint64), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+SERVER_GRAPHS, SERVER_AGGREGATE = 1, 2
 
 
 class Request(ctypes.Structure):
@@ -725,8 +729,9 @@ class Server:
     verify_batch / proof_verify_batch / sign_batch / proof_gen_batch and seed a
     32-byte ProofGen blinding seed or None."""
 
-    def __init__(self, slots=0, max_batch=0, hbm_budget=0):
-        cfg = ServerConfig(slots, max_batch, hbm_budget)
+    def __init__(self, slots=0, max_batch=0, hbm_budget=0, graphs=True, aggregate=False):
+        cfg = ServerConfig(slots, max_batch, hbm_budget,
+                           (SERVER_GRAPHS if graphs else 0) | (SERVER_AGGREGATE if aggregate else 0), 0)
         st = ctypes.c_int32(0)
         self.h = lib().cubbs_hip_server_create(ctypes.byref(cfg), ctypes.byref(st))
         if not self.h:

@@ -186,7 +186,17 @@ typedef struct CubbsHipServerConfig {
   uint32_t slots;
   uint32_t max_batch;
   uint64_t hbm_budget_bytes;
+  uint32_t flags;  /* CUBBS_HIP_SERVER_* below */
+  uint32_t reserved;
 } CubbsHipServerConfig;
+/* Each sub-batch's launch sequence is captured into a HIP graph, and the
+ * slot's instantiated graph of that kind is updated from it
+ * (hipGraphExecUpdate) and launched, instead of launching kernel by kernel;
+ * a new graph is instantiated when the sequence's shape changes. */
+#define CUBBS_HIP_SERVER_GRAPHS 1u
+/* Verify / ProofVerify sub-batches check their pairings in per-key groups of
+ * 8 (as CUBBS_HIP_BATCH_AGGREGATE). */
+#define CUBBS_HIP_SERVER_AGGREGATE 2u
 
 #define CUBBS_HIP_REQ_VERIFY 0
 #define CUBBS_HIP_REQ_PROOF_VERIFY 1
