@@ -195,6 +195,8 @@ int32_t copy_request(const CubbsHipRequest& q, Req& r) {
   }
 }
 
+struct SlotGraphs;
+
 // Device bytes of the library's resident state (tables, keys, retired buffers).
 size_t resident_bytes() {
   Context& c = ctx();
