@@ -250,7 +250,8 @@ def cpu_baseline(vreqs, pvreqs, budget_s=20.0):
                       "%d threads)" % (nv, len(sample) - nv, threads)}
 
 
-def serving_run(g, rate=20000.0, warm_s=5.0, steady_s=30.0, seed=b"cubbs-hip/serving", slots=0):
+def serving_run(g, rate=20000.0, warm_s=5.0, steady_s=30.0, seed=b"cubbs-hip/serving", slots=0,
+                graphs=True, aggregate=False):
     """Open-loop serving load through the library's continuous-batching server:
     Poisson arrivals at `rate` req/s, mix 40% Verify / 30% ProofVerify / 20% Sign
     / 10% ProofGen, 64 issuer keys, message count log-uniform in [1, 128] with
@@ -304,7 +305,7 @@ def serving_run(g, rate=20000.0, warm_s=5.0, steady_s=30.0, seed=b"cubbs-hip/ser
         arrivals.append(int(t * 1e9))
         k = rng.choices(kinds, weights)[0]
         order.append(starts[k] + rng.randrange(n_pool[k]))
-    srv = g.Server(slots=slots)
+    srv = g.Server(slots=slots, graphs=graphs, aggregate=aggregate)
     try:
         # warm the server (generator tables up to L = 128, issuer keys resident)
         srv.run_open_loop(pool, list(range(0, len(pool), 7)), [0] * len(range(0, len(pool), 7)))
@@ -328,6 +329,8 @@ def serving_run(g, rate=20000.0, warm_s=5.0, steady_s=30.0, seed=b"cubbs-hip/ser
             "batches": st["batches"], "avg_batch": round(st["requests"] / max(1, st["batches"]), 1),
             "host_plan_ms_per_batch": st["host_plan_ms_per_batch"], "batch_ms_per_batch": st["batch_ms_per_batch"],
             "host_cpus": len(os.sched_getaffinity(0)),
+            "sub_batches": "HIP graph replay (per-slot graph updated per batch)" if graphs else "kernel by kernel",
+            "pairing_checks": "per-key groups of 8" if aggregate else "per item",
             "max_concurrent_batches": st["max_concurrent_batches"]}
 
 
@@ -338,7 +341,9 @@ def serving_child(args):
     env = dict(os.environ, GPU_MAX_HW_QUEUES=SERVING_HW_QUEUES)
     cmd = [sys.executable, os.path.abspath(__file__), "--serving-only",
            "--serving-rate", str(args.serving_rate), "--serving-seconds", str(args.serving_seconds),
-           "--serving-slots", str(args.serving_slots)]
+           "--serving-slots", str(args.serving_slots)] + \
+        (["--serving-no-graphs"] if args.serving_no_graphs else []) + \
+        (["--serving-aggregate"] if args.serving_aggregate else [])
     p = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, check=True)
     lat = json.loads(p.stdout.decode().strip().splitlines()[-1])
     lat["hw_queues"] = int(SERVING_HW_QUEUES)
@@ -436,6 +441,10 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=30.0)
     ap.add_argument("--serving-slots", type=int, default=0, help="server slots (0 = library default)")
     ap.add_argument("--serving-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--serving-no-graphs", action="store_true",
+                    help="serving sub-batches launched kernel by kernel instead of replayed graphs")
+    ap.add_argument("--serving-aggregate", action="store_true",
+                    help="serving Verify / ProofVerify sub-batches with aggregated pairing checks")
     ap.add_argument("--profile-only", action="store_true", help="a few launches, no baselines")
     ap.add_argument("--no-msm", action="store_true", help="skip the batched-MSM measurement")
     ap.add_argument("--msm-only", action="store_true",
@@ -464,7 +473,8 @@ def main():
         return
     if args.serving_only:
         import cubbs_hip as g
-        lat = serving_run(g, rate=args.serving_rate, steady_s=args.serving_seconds, slots=args.serving_slots)
+        lat = serving_run(g, rate=args.serving_rate, steady_s=args.serving_seconds, slots=args.serving_slots,
+                          graphs=not args.serving_no_graphs, aggregate=args.serving_aggregate)
         print(json.dumps(lat), flush=True)
         return
     if args.gpus != 1:

@@ -70,6 +70,8 @@ const Knobs& knobs() {
     d.hw_queues = (uint32_t)std::min<uint64_t>(env_u64("GPU_MAX_HW_QUEUES", 4), 64);
     d.pk_cache_entries = (uint32_t)std::max<uint64_t>(16, std::min<uint64_t>(env_u64("CUBBS_HIP_PK_CACHE_ENTRIES", 4096), 1u << 20));
     d.gen_table_budget = env_u64("CUBBS_HIP_GEN_TABLE_BUDGET_BYTES", 32ull << 30);
+    d.coalesce = env_u64("CUBBS_HIP_COALESCE", 1) != 0;
+    d.coalesce_inflight = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(env_u64("CUBBS_HIP_COALESCE_INFLIGHT", 2), 16));
     return d;
   }();
   return k;
@@ -329,6 +331,9 @@ bool lookup_pks_locked(Context& c, const std::vector<const uint8_t*>& pks,
     }
   }
   if (distinct > c.pk_cap) return false;
+  // the hits are used now: a miss of this call must not take their slots
+  for (size_t i = 0; i < pks.size(); i++)
+    if (where[i] >= 0) c.pk_used[(uint32_t)where[i]] = now;
   // slots for the misses: free ones first, then the least recently used
   // entries that no live batch uses and this call does not hit
   std::vector<uint32_t> fresh;
@@ -344,9 +349,6 @@ bool lookup_pks_locked(Context& c, const std::vector<const uint8_t*>& pks,
     });
     fresh.assign(cand.begin(), cand.begin() + miss.size());
   }
-  // mark the hits used now (so a later eviction in this call skips them)
-  for (size_t i = 0; i < pks.size(); i++)
-    if (where[i] >= 0) c.pk_used[(uint32_t)where[i]] = now;
   std::vector<const uint8_t*> fresh_bytes;
   for (size_t m = 0; m < miss.size(); m++) {
     const uint32_t s = fresh[m];
@@ -358,7 +360,6 @@ bool lookup_pks_locked(Context& c, const std::vector<const uint8_t*>& pks,
     fresh_bytes.push_back(reinterpret_cast<const uint8_t*>(c.pk_key[s].data()));
   }
   prepare_pks_locked(c, fresh_bytes, fresh, c.pk_bytes, c.pk_status, c.pk_lines);
-  std::vector<uint32_t> pinned;
   for (size_t i = 0; i < pks.size(); i++) {
     const uint32_t s = where[i] >= 0 ? (uint32_t)where[i] : fresh[(size_t)(-where[i] - 1)];
     slots_out[i] = s;

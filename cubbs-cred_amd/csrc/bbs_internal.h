@@ -296,6 +296,8 @@ struct Knobs {
   uint32_t hw_queues = 4;           // GPU_MAX_HW_QUEUES (HIP's own setting)
   uint32_t pk_cache_entries = 4096; // CUBBS_HIP_PK_CACHE_ENTRIES: resident prepared keys
   uint64_t gen_table_budget = 32ull << 30;  // CUBBS_HIP_GEN_TABLE_BUDGET_BYTES: 8-bit comb tables
+  uint32_t coalesce = 1;            // CUBBS_HIP_COALESCE: combine small concurrent primitive calls
+  uint32_t coalesce_inflight = 2;   // CUBBS_HIP_COALESCE_INFLIGHT: combined calls at once
 };
 const Knobs& knobs();
 

@@ -19,6 +19,8 @@
 // for MSM shapes (1 item, 2 common points, 3 large, 6 dual) and the boolean
 // of a pairing equality.
 #include <algorithm>
+#include <condition_variable>
+#include <deque>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
@@ -317,11 +319,156 @@ void collect_primitive(Batch& b, CuBbsBbsGpuResult* results) {
   }
 }
 
+// ------------------------------------------------------------------ coalescing of small calls
+// Many host threads issuing small independent calls (the Rust engine's
+// per-item pairing checks and singleton MSMs from its worker threads) each
+// pay a whole launch chain.  Calls of at most COALESCE_MAX_CALL_ITEMS items
+// queue here; a caller that finds fewer than `inflight` combined calls
+// running takes every queued call (up to COALESCE_MAX_ITEMS items) and runs
+// them as one call, the others wait for their results.  Nothing waits for a
+// timer: while combined calls run, new calls accumulate, so an idle library
+// adds no latency.  This is the reference's pairing microbatcher
+// (native/cuda/src/g1_msm_blst.cpp:1278-1406) without its gather window,
+// applied to pairing_eq and item MSMs; results equal the direct call's (the
+// items are independent rows).  CUBBS_HIP_COALESCE=0 turns it off.
+constexpr size_t COALESCE_MAX_CALL_ITEMS = 16, COALESCE_MAX_ITEMS = 1024;
+
+template <class Item>
+struct Coalescer {
+  struct Call {
+    Item* items;
+    size_t n;
+    CuBbsBbsGpuResult* results;
+    int32_t rc = CUBBS_BBS_GPU_ERR_INTERNAL;
+    bool done = false;
+  };
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Call*> pending;
+  uint32_t running = 0;
+  uint64_t combined_calls = 0, coalesced_calls = 0;
+
+  template <class Direct>
+  int32_t submit(Item* items, size_t n, CuBbsBbsGpuResult* results, Direct direct) {
+    Call me{items, n, results};
+    std::unique_lock<std::mutex> lk(mu);
+    pending.push_back(&me);
+    for (;;) {
+      if (me.done) return me.rc;
+      if (running < knobs().coalesce_inflight && !pending.empty()) {
+        std::vector<Call*> take;
+        size_t total = 0;
+        while (!pending.empty() && (take.empty() || total + pending.front()->n <= COALESCE_MAX_ITEMS)) {
+          take.push_back(pending.front());
+          total += pending.front()->n;
+          pending.pop_front();
+        }
+        running++;
+        combined_calls++;
+        coalesced_calls += take.size();
+        lk.unlock();
+        int32_t rc = CUBBS_BBS_GPU_ERR_INTERNAL;
+        std::vector<Item> all;
+        std::vector<CuBbsBbsGpuResult> res;
+        try {
+          all.reserve(total);
+          for (Call* c : take) all.insert(all.end(), c->items, c->items + c->n);
+          res.resize(total);
+          rc = direct(all.data(), all.size(), res.data());
+        } catch (...) {  // host memory exhausted
+          rc = CUBBS_BBS_GPU_ERR_INTERNAL;
+        }
+        lk.lock();
+        size_t off = 0;
+        for (Call* c : take) {
+          if (rc == CUBBS_BBS_GPU_SUCCESS)
+            for (size_t i = 0; i < c->n; i++) {
+              c->results[i] = res[off + i];
+              c->items[i] = all[off + i];  // outputs written through the items (lengths)
+            }
+          c->rc = rc;
+          c->done = true;
+          off += c->n;
+        }
+        running--;
+        cv.notify_all();
+        continue;
+      }
+      cv.wait(lk);
+    }
+  }
+};
+
+Coalescer<CuBbsBbsGpuPairingEqItem>& pairing_coalescer() {
+  static auto* c = new Coalescer<CuBbsBbsGpuPairingEqItem>();
+  return *c;
+}
+Coalescer<CuBbsBbsGpuG1MsmItem>& msm_coalescer() {
+  static auto* c = new Coalescer<CuBbsBbsGpuG1MsmItem>();
+  return *c;
+}
+
+int32_t msm_direct(CuBbsBbsGpuG1MsmItem* items, size_t item_count, CuBbsBbsGpuResult* results) {
+  return guard([&] {
+    Prog g;
+    std::vector<PrimRow> rows;
+    compile_msm(g, items, item_count, rows);
+    Ran r = run(g);
+    for (size_t i = 0; i < item_count; i++)
+      row_result(rows[i], r.status.data(), r.out.data(), results[i], &items[i].point_out);
+    return CUBBS_BBS_GPU_SUCCESS;
+  });
+}
+
+int32_t pairing_direct(CuBbsBbsGpuPairingEqItem* items, size_t item_count, CuBbsBbsGpuResult* results) {
+  return guard([&] {
+    Prog g;
+    std::vector<uint32_t> st(item_count);
+    std::vector<bool> shape(item_count);
+    for (size_t i = 0; i < item_count; i++) {
+      const auto& it = items[i];
+      st[i] = g.slot();
+      shape[i] = fixed(it.lhs_g1, 48) && fixed(it.rhs_g1, 48) && fixed(it.lhs_g2, 96) &&
+                 fixed(it.rhs_g2, 96);
+      if (!shape[i]) continue;
+      const uint32_t a = g.point(it.lhs_g1.data, st[i]);
+      const uint32_t b = g.point(it.rhs_g1.data, st[i]);
+      const uint32_t qa = (uint32_t)g.g2.size();
+      g.g2.push_back(it.lhs_g2.data);
+      g.g2.push_back(it.rhs_g2.data);
+      g.p.pairs.push_back(PairJob{st[i], a, b, qa, qa + 1, 1u});
+    }
+    Ran ran = run(g);
+    for (size_t i = 0; i < item_count; i++) {
+      if (!shape[i] || ran.status[st[i]])
+        set(results[i], CUBBS_BBS_GPU_ERR_INVALID_INPUT, 0, 0);
+      else
+        set(results[i], CUBBS_BBS_GPU_SUCCESS, ran.valid[st[i]] ? 1 : 0, 0);
+    }
+    return CUBBS_BBS_GPU_SUCCESS;
+  });
+}
+
 }  // namespace cubbs
 
 using namespace cubbs;
 
 extern "C" {
+
+int32_t cubbs_hip_coalesce_stats(uint64_t* pairing_combined, uint64_t* pairing_calls,
+                                 uint64_t* msm_combined, uint64_t* msm_calls) {
+  auto& p = pairing_coalescer();
+  auto& m = msm_coalescer();
+  {
+    std::lock_guard<std::mutex> g(p.mu);
+    if (pairing_combined) *pairing_combined = p.combined_calls;
+    if (pairing_calls) *pairing_calls = p.coalesced_calls;
+  }
+  std::lock_guard<std::mutex> g(m.mu);
+  if (msm_combined) *msm_combined = m.combined_calls;
+  if (msm_calls) *msm_calls = m.coalesced_calls;
+  return CUBBS_BBS_GPU_SUCCESS;
+}
 
 // reference header :264-267; semantics native/cuda/src/cubbs_bbs_cuda.cu:875-918
 int32_t cubbs_bbs_gpu_g1_msm_batch_v2(uint32_t abi_version, CuBbsBbsGpuG1MsmItem* items,
@@ -332,15 +479,9 @@ int32_t cubbs_bbs_gpu_g1_msm_batch_v2(uint32_t abi_version, CuBbsBbsGpuG1MsmItem
   for (size_t i = 0; i < item_count; i++)
     if (!outcap(items[i].point_out, 48)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   if (item_count == 0) return CUBBS_BBS_GPU_SUCCESS;
-  return guard([&] {
-    Prog g;
-    std::vector<PrimRow> rows;
-    compile_msm(g, items, item_count, rows);
-    Ran r = run(g);
-    for (size_t i = 0; i < item_count; i++)
-      row_result(rows[i], r.status.data(), r.out.data(), results[i], &items[i].point_out);
-    return CUBBS_BBS_GPU_SUCCESS;
-  });
+  if (knobs().coalesce && item_count <= COALESCE_MAX_CALL_ITEMS)
+    return msm_coalescer().submit(items, item_count, results, msm_direct);
+  return msm_direct(items, item_count, results);
 }
 
 // reference header :269-275; semantics cubbs_bbs_cuda.cu:756-807 (validation),
@@ -714,32 +855,16 @@ int32_t cubbs_bbs_gpu_pairing_eq_batch_v2(uint32_t abi_version, const CuBbsBbsGp
   if (abi_version != CUBBS_BBS_GPU_ABI_VERSION || (item_count && (!items || !results)))
     return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   if (item_count == 0) return CUBBS_BBS_GPU_SUCCESS;
-  return guard([&] {
-    Prog g;
-    std::vector<uint32_t> st(item_count);
-    std::vector<bool> shape(item_count);
-    for (size_t i = 0; i < item_count; i++) {
-      const auto& it = items[i];
-      st[i] = g.slot();
-      shape[i] = fixed(it.lhs_g1, 48) && fixed(it.rhs_g1, 48) && fixed(it.lhs_g2, 96) &&
-                 fixed(it.rhs_g2, 96);
-      if (!shape[i]) continue;
-      const uint32_t a = g.point(it.lhs_g1.data, st[i]);
-      const uint32_t b = g.point(it.rhs_g1.data, st[i]);
-      const uint32_t qa = (uint32_t)g.g2.size();
-      g.g2.push_back(it.lhs_g2.data);
-      g.g2.push_back(it.rhs_g2.data);
-      g.p.pairs.push_back(PairJob{st[i], a, b, qa, qa + 1, 1u});
-    }
-    Ran ran = run(g);
-    for (size_t i = 0; i < item_count; i++) {
-      if (!shape[i] || ran.status[st[i]])
-        set(results[i], CUBBS_BBS_GPU_ERR_INVALID_INPUT, 0, 0);
-      else
-        set(results[i], CUBBS_BBS_GPU_SUCCESS, ran.valid[st[i]] ? 1 : 0, 0);
-    }
-    return CUBBS_BBS_GPU_SUCCESS;
-  });
+  // the items are only read: a private copy serves both paths
+  std::vector<CuBbsBbsGpuPairingEqItem> own;
+  try {
+    own.assign(items, items + item_count);
+  } catch (...) {
+    return CUBBS_BBS_GPU_ERR_INTERNAL;
+  }
+  if (knobs().coalesce && item_count <= COALESCE_MAX_CALL_ITEMS)
+    return pairing_coalescer().submit(own.data(), item_count, results, pairing_direct);
+  return pairing_direct(own.data(), item_count, results);
 }
 
 // reference header :204-210, :302-304; semantics g1_msm_blst.cpp:1422-1640: one

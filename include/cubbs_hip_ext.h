@@ -170,6 +170,16 @@ typedef struct CubbsHipResidentStats {
 } CubbsHipResidentStats;
 int32_t cubbs_hip_resident_stats(CubbsHipResidentStats *out);
 
+/* ---- coalescing of small calls.  cubbs_bbs_gpu_pairing_eq_batch_v2 and
+ * cubbs_bbs_gpu_g1_msm_batch_v2 calls of at most 16 items from concurrent
+ * host threads are combined into one device call while earlier combined
+ * calls run (at most CUBBS_HIP_COALESCE_INFLIGHT, default 2, at once); each
+ * caller gets exactly its own results.  CUBBS_HIP_COALESCE=0 turns it off
+ * (both read once).  Counters: combined device calls and the calls they
+ * served, per entry point. */
+int32_t cubbs_hip_coalesce_stats(uint64_t *pairing_combined, uint64_t *pairing_calls,
+                                 uint64_t *msm_combined, uint64_t *msm_calls);
+
 /* ---- serving loop: continuous batching of mixed requests.
  * `slots` worker slots, each with its own stream and reusable device/pinned
  * memory; a free slot takes every pending request up to max_batch and its
