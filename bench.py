@@ -251,7 +251,7 @@ def cpu_baseline(vreqs, pvreqs, budget_s=20.0):
 
 
 def serving_run(g, rate=20000.0, warm_s=5.0, steady_s=30.0, seed=b"cubbs-hip/serving", slots=0,
-                graphs=True, aggregate=False):
+                graphs=False, aggregate=False):
     """Open-loop serving load through the library's continuous-batching server:
     Poisson arrivals at `rate` req/s, mix 40% Verify / 30% ProofVerify / 20% Sign
     / 10% ProofGen, 64 issuer keys, message count log-uniform in [1, 128] with
@@ -342,12 +342,165 @@ def serving_child(args):
     cmd = [sys.executable, os.path.abspath(__file__), "--serving-only",
            "--serving-rate", str(args.serving_rate), "--serving-seconds", str(args.serving_seconds),
            "--serving-slots", str(args.serving_slots)] + \
-        (["--serving-no-graphs"] if args.serving_no_graphs else []) + \
+        (["--serving-graphs"] if args.serving_graphs else []) + \
         (["--serving-aggregate"] if args.serving_aggregate else [])
     p = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, check=True)
     lat = json.loads(p.stdout.decode().strip().splitlines()[-1])
     lat["hw_queues"] = int(SERVING_HW_QUEUES)
     return lat
+
+
+ABI_THREADS = 8
+G2_GENERATOR = bytes.fromhex(
+    "93e02b6052719f607dacd3a088274f65596bd0d09920b61ab5da61bbdc7f5049334cf11213945d57e5ac7d055d042b7e"
+    "024aa2b2f08f0a91260805272dc51051c6e47ad4fa403b02b4510b647ae3d1770bac0326a805bbefd48056c8c121bdb8")
+
+
+class _PrimitiveVerify:
+    """The calls the unchanged Rust engine issues for a Verify batch of mixed
+    issuers (crates/cubbs-bbs/src/execution.rs:390-477 and its callers):
+    messages_to_scalar over every message; the domain of each item (the
+    engine hashes it on the host; here the library's hash_to_scalar export);
+    B = P1 + Q1 dom + sum H_j m_j through the common-points MSM, one group per
+    message count; the two-term MSMs B * 1 + A * (-e); pairing_eq of
+    (A, PK, B - e A, G2).  Every call's items are built once, pointing at the
+    previous call's output buffers, so a timed round is the five calls."""
+
+    def __init__(self, g, reqs, gens):
+        import ctypes
+        self.g, self.c = g, ctypes
+        k = self.keep = g._Keep()
+        n = len(reqs)
+        self.n = n
+        msgs = [m for _, _, _, ms in reqs for m in ms]
+        self.n_msgs = len(msgs)
+        self.scal = ctypes.create_string_buffer(32 * max(1, len(msgs)))
+        self.m2s = (g.MessagesToScalarItem * max(1, len(msgs)))()
+        base = ctypes.addressof(self.scal)
+        for i, m in enumerate(msgs):
+            self.m2s[i].message = k.slice(m)
+            self.m2s[i].dst = k.slice(g.MAP_DST)
+            self.m2s[i].scalar_out = g.ByteBuffer(base + 32 * i, 32, 0)
+        self.m2s_res = (g.Result * max(1, len(msgs)))()
+        # domain inputs: PK || I2OSP(L, 8) || Q1 || H_1..H_L || api_id || I2OSP(|hdr|, 8) || hdr
+        dom = [pk + len(ms).to_bytes(8, "big") + b"".join(gens[1:len(ms) + 2]) + g.API_ID +
+               len(h).to_bytes(8, "big") + h for pk, _, h, ms in reqs]
+        self.dom_packed, self.dom_offs, self.dom_lens = g._packed(dom)
+        self.dom_out = ctypes.create_string_buffer(32 * n)
+        dbase = ctypes.addressof(self.dom_out)
+        one = k.slice((1).to_bytes(32, "big"))
+        # B per item: common-points groups by message count
+        self.b_out = ctypes.create_string_buffer(48 * n)
+        bbase = ctypes.addressof(self.b_out)
+        by_L = {}
+        off, m0 = [], 0
+        for i, (_, _, _, ms) in enumerate(reqs):
+            by_L.setdefault(len(ms), []).append(i)
+            off.append(m0)
+            m0 += len(ms)
+        self.groups = (g.CommonPointsItem * len(by_L))()
+        for gi, (L, idx) in enumerate(sorted(by_L.items())):
+            self.groups[gi].points = k.slices(gens[:L + 2])
+            self.groups[gi].term_count = L + 2
+            rows = (g.ByteSlice * ((L + 2) * len(idx)))()
+            for r, i in enumerate(idx):
+                rows[r * (L + 2)] = one
+                rows[r * (L + 2) + 1] = g.ByteSlice(dbase + 32 * i, 32)
+                for j in range(L):
+                    rows[r * (L + 2) + 2 + j] = g.ByteSlice(base + 32 * (off[i] + j), 32)
+            k.refs.append(rows)
+            self.groups[gi].scalar_rows = ctypes.cast(rows, ctypes.c_void_p)
+            self.groups[gi].output_count = len(idx)
+            outs = (g.ByteBuffer * len(idx))(*[g.ByteBuffer(bbase + 48 * i, 48, 0) for i in idx])
+            k.refs.append(outs)
+            self.groups[gi].point_outs = ctypes.cast(outs, ctypes.c_void_p)
+        self.n_groups = len(by_L)
+        self.grp_res = (g.Result * n)()
+        # B * 1 + A * (-e)
+        self.r_out = ctypes.create_string_buffer(48 * n)
+        rbase = ctypes.addressof(self.r_out)
+        self.msm = (g.G1MsmItem * n)()
+        for i, (pk, sig, _, _) in enumerate(reqs):
+            e = int.from_bytes(sig[48:80], "big")
+            pts = (g.ByteSlice * 2)(g.ByteSlice(bbase + 48 * i, 48), k.slice(sig[:48]))
+            scs = (g.ByteSlice * 2)(one, k.slice(((-e) % R_ORDER).to_bytes(32, "big")))
+            k.refs += [pts, scs]
+            self.msm[i].points = ctypes.cast(pts, ctypes.c_void_p)
+            self.msm[i].scalars = ctypes.cast(scs, ctypes.c_void_p)
+            self.msm[i].term_count = 2
+            self.msm[i].point_out = g.ByteBuffer(rbase + 48 * i, 48, 0)
+        self.msm_res = (g.Result * n)()
+        g2 = k.slice(G2_GENERATOR)
+        self.pair = (g.PairingEqItem * n)()
+        for i, (pk, sig, _, _) in enumerate(reqs):
+            self.pair[i] = g.PairingEqItem(k.slice(sig[:48]), k.slice(pk), g.ByteSlice(rbase + 48 * i, 48), g2)
+        self.pair_res = (g.Result * n)()
+
+    def run(self):
+        L, c, A = self.g.lib(), self.c, self.g.ABI_VERSION
+        ok = L.cubbs_bbs_gpu_messages_to_scalar_batch_v2(A, self.m2s, c.c_size_t(self.n_msgs), self.m2s_res) == 0
+        ok = ok and L.cubbs_bbs_gpu_hash_to_scalar_sha256_batch_for_test_v2(
+            A, self.dom_packed, self.dom_offs, self.dom_lens, c.c_size_t(self.n), self.g.H2S_DST,
+            c.c_size_t(len(self.g.H2S_DST)), self.dom_out) == 0
+        ok = ok and L.cubbs_bbs_gpu_g1_msm_common_points_batch_v2(A, self.groups, c.c_size_t(self.n_groups),
+                                                                  self.grp_res) == 0
+        ok = ok and L.cubbs_bbs_gpu_g1_msm_batch_v2(A, self.msm, c.c_size_t(self.n), self.msm_res) == 0
+        ok = ok and L.cubbs_bbs_gpu_pairing_eq_batch_v2(A, self.pair, c.c_size_t(self.n), self.pair_res) == 0
+        if not ok:
+            raise SystemExit("primitive sequence call failed")
+        return [(self.pair_res[i].code, bool(self.pair_res[i].value)) for i in range(self.n)]
+
+
+def abi_paths(g, vreqs, expect_v, pvreqs, threads=ABI_THREADS, rounds=3):
+    """Throughput of the reference ABI's own entry points fed from host
+    buffers (host packing, upload and download included) by `threads` host
+    threads, each on its own slice of the benchmark batches: the whole-op
+    entries cubbs_bbs_gpu_verify_batch_v2 / proof_verify_batch_v2, and the
+    primitive sequence the unchanged Rust engine issues for Verify."""
+    import ctypes
+    rc, gens = g.generators(130)
+    assert rc == 0
+    nv, npv = len(vreqs) // threads, len(pvreqs) // threads
+    vs = [vreqs[t * nv:(t + 1) * nv] for t in range(threads)]
+    ps = [pvreqs[t * npv:(t + 1) * npv] for t in range(threads)]
+    keep = g._Keep()
+    vitems = [g.build_verify_items(v, keep) for v in vs]
+    pitems = [g.build_proof_verify_items(p, keep) for p in ps]
+    vres = [(g.Result * nv)() for _ in range(threads)]
+    pres = [(g.Result * npv)() for _ in range(threads)]
+    L, A = g.lib(), g.ABI_VERSION
+
+    def whole(t):
+        a = L.cubbs_bbs_gpu_verify_batch_v2(A, vitems[t], ctypes.c_size_t(nv), vres[t])
+        b = L.cubbs_bbs_gpu_proof_verify_batch_v2(A, pitems[t], ctypes.c_size_t(npv), pres[t])
+        return a == 0 and b == 0
+
+    with cf.ThreadPoolExecutor(threads) as ex:
+        assert all(ex.map(whole, range(threads)))  # warm (tables, keys)
+        for t in range(threads):
+            got = [bool(vres[t][i].value) for i in range(nv)]
+            assert got == expect_v[t * nv:(t + 1) * nv], "whole-op Verify results differ"
+            assert all(pres[t][i].code == 0 and pres[t][i].value for i in range(npv))
+        t0 = time.perf_counter()
+        for _ in range(rounds):
+            assert all(ex.map(whole, range(threads)))
+        el_whole = time.perf_counter() - t0
+        prims = [_PrimitiveVerify(g, v, gens) for v in vs]
+        first = list(ex.map(lambda p: p.run(), prims))
+        for t in range(threads):
+            assert [v for _, v in first[t]] == expect_v[t * nv:(t + 1) * nv], "primitive path results differ"
+        t0 = time.perf_counter()
+        for _ in range(rounds):
+            list(ex.map(lambda p: p.run(), prims))
+        el_prim = time.perf_counter() - t0
+    return {"threads": threads, "rounds": rounds,
+            "whole_op_ops_s": round(threads * (nv + npv) * rounds / el_whole, 1),
+            "whole_op": "cubbs_bbs_gpu_verify_batch_v2 (%d uneven items) + proof_verify_batch_v2 (%d) per thread "
+                        "from host buffers" % (nv, npv),
+            "primitive_verify_ops_s": round(threads * nv * rounds / el_prim, 1),
+            "primitive_verify": "per thread, %d Verify as the unchanged Rust engine issues them: messages_to_scalar, "
+                                "domain hash_to_scalar, common-points MSM per message count, 2-term MSMs B - e A, "
+                                "pairing_eq" % nv}
 
 
 def per_request_gpu(g, vreqs, n=16):
@@ -441,8 +594,9 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=30.0)
     ap.add_argument("--serving-slots", type=int, default=0, help="server slots (0 = library default)")
     ap.add_argument("--serving-only", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--serving-no-graphs", action="store_true",
-                    help="serving sub-batches launched kernel by kernel instead of replayed graphs")
+    ap.add_argument("--serving-graphs", action="store_true",
+                    help="serving sub-batches captured into per-slot HIP graphs and replayed (default: kernel by "
+                         "kernel, which measured lower latency)")
     ap.add_argument("--serving-aggregate", action="store_true",
                     help="serving Verify / ProofVerify sub-batches with aggregated pairing checks")
     ap.add_argument("--profile-only", action="store_true", help="a few launches, no baselines")
@@ -474,7 +628,7 @@ def main():
     if args.serving_only:
         import cubbs_hip as g
         lat = serving_run(g, rate=args.serving_rate, steady_s=args.serving_seconds, slots=args.serving_slots,
-                          graphs=not args.serving_no_graphs, aggregate=args.serving_aggregate)
+                          graphs=args.serving_graphs, aggregate=args.serving_aggregate)
         print(json.dumps(lat), flush=True)
         return
     if args.gpus != 1:
@@ -498,6 +652,13 @@ def main():
             t = time.time()
             out["latency"] = serving_child(args)
             log("serving run %.1fs: %s" % (time.time() - t, out["latency"]))
+            if not args.serving_graphs:
+                # the same load with graph-replayed sub-batches, shorter window
+                g_args = argparse.Namespace(**dict(vars(args), serving_graphs=True,
+                                                   serving_seconds=min(10.0, args.serving_seconds)))
+                t = time.time()
+                out["latency_graphs"] = serving_child(g_args)
+                log("serving run (graphs) %.1fs: %s" % (time.time() - t, out["latency_graphs"]))
         print(json.dumps(out), flush=True)
         return
     import cubbs_hip as g
@@ -649,6 +810,9 @@ def main():
         log("msm run %.1fs: %s" % (time.time() - t, {k: out["msm"][k] for k in ("msm_per_s", "ms_per_step", "comb_4096x32_ms", "pippenger_2_16_ms")}))
     if not args.profile_only:
         out["per_request_gpu_ops_s"] = round(per_request_gpu(g, vreqs), 2)
+        t = time.time()
+        out["abi_paths"] = abi_paths(g, vreqs, expect_v, pvreqs)
+        log("reference-ABI paths %.1fs: %s" % (time.time() - t, out["abi_paths"]))
         out["proofvalue_codec"] = codec_run()
         log("proofValue codec: %s" % out["proofvalue_codec"])
         if not args.no_serving:

@@ -729,7 +729,7 @@ class Server:
     verify_batch / proof_verify_batch / sign_batch / proof_gen_batch and seed a
     32-byte ProofGen blinding seed or None."""
 
-    def __init__(self, slots=0, max_batch=0, hbm_budget=0, graphs=True, aggregate=False):
+    def __init__(self, slots=0, max_batch=0, hbm_budget=0, graphs=False, aggregate=False):
         cfg = ServerConfig(slots, max_batch, hbm_budget,
                            (SERVER_GRAPHS if graphs else 0) | (SERVER_AGGREGATE if aggregate else 0), 0)
         st = ctypes.c_int32(0)
