@@ -105,6 +105,9 @@ void init_locked(Context& c) {
   check(hipMalloc(&c.gen_vstate, 64));
   check(hipMalloc(&c.neg_g2_lines, PK_LINES_WORDS * 4));
   c.resident_bytes += 64 + PK_LINES_WORDS * 4;
+  check(hipMalloc(&c.sha_zpad, 32));
+  c.resident_bytes += 32;
+  launch_sha_zpad(c.setup, c.sha_zpad);
   launch_prepare_neg_g2(c.setup, c.neg_g2_lines);
   check(hipGetLastError());
   check(hipStreamSynchronize(c.setup));

@@ -141,7 +141,8 @@ static void count_work(Batch& b, const Plan& p) {
   w.pair_checks = p.pairs.size() - p.n_group_pairs;
   w.pair_checks_run = p.pairs.size();
   uint64_t sha = 0;
-  for (const auto& j : p.h2s) sha += xmd48_blocks(j.len, p.map_dst.size());
+  // message hashes: their b_0 blocks after Z_pad, and b_1, b_2
+  for (const auto& j : p.msg) sha += j.blocks + 2ull * p.tail_blocks;
   // domain input: PK || I2OSP(L, 8) || Q1 || H_1..H_L || api_id || I2OSP(|hdr|, 8) || hdr
   for (const auto& j : p.dom) sha += xmd48_blocks(96 + 8 + 48ull * (j.L + 1) + 44 + 8 + j.hdr_len, 48);
   // challenge: I2OSP(R, 8) || (I2OSP(i, 8) || m_i)_R || Abar, Bbar, D, T1, T2 || domain || I2OSP(|ph|, 8) || ph
@@ -153,7 +154,8 @@ void materialize(Batch& b, Plan& p) {
   // The lanes of a wave wait for the longest of their jobs: order message
   // hashes by SHA-256 block count and domains by input length (each job
   // names its output slot, so the order does not change any result).
-  sort_jobs(p.h2s, [](const H2sJob& j) { return std::min<uint32_t>(j.len >> 6, 1u << 16); });
+  sort_jobs(p.msg, [](const MsgHashJob& j) { return std::min<uint32_t>(j.blocks, 1u << 16); });
+  if (!p.msg.empty() && !p.tail_blocks) p.message_tails();
   sort_jobs(p.dom, [](const DomainJob& j) {
     return std::min<uint32_t>((48u * j.L + j.hdr_len) >> 6, 1u << 16);
   });
@@ -168,7 +170,7 @@ void materialize(Batch& b, Plan& p) {
     if (bytes) std::memcpy(H.data() + off, src, bytes);
     return off;
   };
-  size_t o_h2s = place(p.h2s.data(), p.h2s.size() * sizeof(H2sJob));
+  size_t o_msg = place(p.msg.data(), p.msg.size() * sizeof(MsgHashJob));
   size_t o_dom = place(p.dom.data(), p.dom.size() * sizeof(DomainJob));
   size_t o_dg1 = place(p.dec_g1.data(), p.dec_g1.size() * sizeof(DecodeG1Job));
   size_t o_dfr = place(p.dec_fr.data(), p.dec_fr.size() * sizeof(DecodeFrJob));
@@ -212,8 +214,8 @@ void materialize(Batch& b, Plan& p) {
   }
   std::memcpy(b.h_in, H.data(), H.size());
   uint8_t* D = b.d_in;
-  b.h2s = reinterpret_cast<H2sJob*>(D + o_h2s);
-  b.n_h2s = (uint32_t)p.h2s.size();
+  b.msg = reinterpret_cast<MsgHashJob*>(D + o_msg);
+  b.n_msg = (uint32_t)p.msg.size();
   b.dom = reinterpret_cast<DomainJob*>(D + o_dom);
   b.n_dom = (uint32_t)p.dom.size();
   b.dec_g1 = reinterpret_cast<DecodeG1Job*>(D + o_dg1);
@@ -261,6 +263,8 @@ void materialize(Batch& b, Plan& p) {
   b.map_dst = D + o_dst;
   b.map_dst_len = (uint32_t)p.map_dst.size();
   b.blob = D + o_blob;
+  b.tail = reinterpret_cast<const uint32_t*>(b.blob + p.tail_off);
+  b.tail_blocks = p.tail_blocks;
   b.seeded = p.seeded;
   b.seeds = D + o_blob + p.seeds_off;
   for (const auto& c : p.chunks) {
@@ -520,6 +524,7 @@ void attach(Batch& b, uint32_t max_gen_slots, const std::vector<const uint8_t*>&
   b.n16 = c.wide_slots;
   b.gen_comp = c.gen_comp;
   b.neg_g2_lines = c.neg_g2_lines;
+  b.sha_zpad = c.sha_zpad;
 }
 
 static std::string map_dst() { return std::string(kApiId) + "MAP_MSG_TO_SCALAR_AS_HASH_"; }
@@ -589,8 +594,7 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
     if (!scaled) {
       p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, term + 1});
       for (uint32_t m = 0; m < L; m++) {
-        size_t mo = p.bytes(it.messages[m].data, it.messages[m].len);
-        p.h2s.push_back(H2sJob{mo, (uint32_t)it.messages[m].len, term + 2 + m});
+        p.hash_message(it.messages[m].data, it.messages[m].len, term + 2 + m);
       }
       p.fills.push_back(TermFill{term, 0, 0, 2});               // P1 scalar = 1
       p.dec_fr.push_back(DecodeFrJob{sig + 48, se, k, 1u, 0u});  // store r - e
@@ -606,8 +610,7 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
       // domain and message scalars land in raw slots, scaled into the terms
       p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, raw});
       for (uint32_t m = 0; m < L; m++) {
-        size_t mo = p.bytes(it.messages[m].data, it.messages[m].len);
-        p.h2s.push_back(H2sJob{mo, (uint32_t)it.messages[m].len, raw + 1 + m});
+        p.hash_message(it.messages[m].data, it.messages[m].len, raw + 1 + m);
       }
       p.fills.push_back(TermFill{term, sc, 0, 0});  // P1 scalar = c
       for (uint32_t t = 0; t <= L; t++) p.fills.push_back(TermFill{term + 1 + t, raw + t, sc, 1});
@@ -725,8 +728,7 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
     uint32_t s_m = p.scalars(R);
     uint32_t disc_begin = (uint32_t)p.disc_index.size();
     for (uint32_t r = 0; r < R; r++) {
-      size_t mo = p.bytes(it.disclosed_messages[r].data, it.disclosed_messages[r].len);
-      p.h2s.push_back(H2sJob{mo, (uint32_t)it.disclosed_messages[r].len, s_m + r});
+      p.hash_message(it.disclosed_messages[r].data, it.disclosed_messages[r].len, s_m + r);
       p.disc_index.push_back((uint32_t)it.disclosed_indexes.data[r]);
       p.disc_scalar.push_back(s_m + r);
     }
@@ -816,8 +818,7 @@ std::unique_ptr<Batch> plan_sign(const CuBbsBbsGpuSignItem* items, size_t n) {
     size_t hdr = p.bytes(it.header.data, it.header.len);
     p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, term + 1});
     for (uint32_t m = 0; m < L; m++) {
-      size_t mo = p.bytes(it.messages[m].data, it.messages[m].len);
-      p.h2s.push_back(H2sJob{mo, (uint32_t)it.messages[m].len, term + 2 + m});
+      p.hash_message(it.messages[m].data, it.messages[m].len, term + 2 + m);
     }
     for (uint32_t t = 0; t < L + 2; t++) p.term_gen[term + t] = t;
     uint32_t se = p.scalars(1), pa = p.points(1);
@@ -913,8 +914,7 @@ std::unique_ptr<Batch> plan_proof_gen(const CuBbsBbsGpuProofGenItem* items, size
     p.dec_fr.push_back(DecodeFrJob{sig + 48, s_e, k, 0, 0});
     p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, s_dom});
     for (uint32_t m = 0; m < L; m++) {
-      size_t mo = p.bytes(it.messages[m].data, it.messages[m].len);
-      p.h2s.push_back(H2sJob{mo, (uint32_t)it.messages[m].len, s_m + m});
+      p.hash_message(it.messages[m].data, it.messages[m].len, s_m + m);
     }
     if (mode == 2) {
       size_t so = p.bytes(scalars_be + 32 * scalar_offsets[dev[k]], 32ull * (5 + U));
@@ -1079,9 +1079,9 @@ void launch_batch(Batch& b) {
       launch_decode_g1(s, b.dec_g1, b.n_dec_g1, b.blob, b.points, b.pinf, b.status, !prim_split);
     }
   }
-  if (b.n_h2s || b.n_dom) {
+  if (b.n_msg || b.n_dom) {
     Timed t(b, "hash_domain");
-    launch_hash_stage(s, b.dom, b.n_dom, b.h2s, b.n_h2s, b.blob, b.map_dst, b.map_dst_len,
+    launch_hash_stage(s, b.dom, b.n_dom, b.msg, b.n_msg, b.blob, b.sha_zpad, b.tail, b.tail_blocks,
                       b.pk_bytes, b.gen_comp, b.scalars);
   }
   if (b.kind == KIND_PROOF_GEN && b.seeded) {

@@ -95,7 +95,10 @@ struct Batch {
   size_t res_stride = 0;
   uint8_t* d_work = nullptr;
   // job arrays (device)
-  H2sJob* h2s = nullptr;
+  MsgHashJob* msg = nullptr;
+  const uint32_t* tail = nullptr;     // b_1 / b_2 input words of the message hashes
+  uint32_t tail_blocks = 0;
+  const uint32_t* sha_zpad = nullptr;
   DomainJob* dom = nullptr;
   DecodeG1Job* dec_g1 = nullptr;
   DecodeFrJob* dec_fr = nullptr;
@@ -136,7 +139,7 @@ struct Batch {
   uint8_t* blob = nullptr;
   uint8_t* seeds = nullptr;
   bool seeded = false;
-  uint32_t n_h2s = 0, n_dom = 0, n_dec_g1 = 0, n_dec_fr = 0, n_fills = 0, n_var = 0, n_chunks = 0,
+  uint32_t n_msg = 0, n_dom = 0, n_dec_g1 = 0, n_dec_fr = 0, n_fills = 0, n_var = 0, n_chunks = 0,
            n_terms = 0, n_groups1 = 0, n_groups2 = 0, n_pairs = 0, n_sign = 0, n_pvch = 0,
            n_pg = 0;
   uint64_t comb_terms = 0;

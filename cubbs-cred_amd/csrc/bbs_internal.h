@@ -48,6 +48,17 @@ struct H2sJob {
   uint32_t out_index;
 };
 
+// Message hash of a batch pipeline (hash_to_scalar with the MAP DST): blob +
+// off holds the input of expand_message_xmd's b_0 after its Z_pad block --
+// msg || I2OSP(48, 2) || 0x00 || DST' and the SHA-256 padding -- as `blocks`
+// 64-byte blocks of big-endian 32-bit words, 16-byte aligned; the scalar goes
+// to out[out_index].
+struct MsgHashJob {
+  uint64_t off;
+  uint32_t blocks;
+  uint32_t out_index;
+};
+
 // domain job (crates/cubbs-bbs-protocol/src/utils/util.rs:147-195)
 struct DomainJob {
   uint64_t hdr_off;
@@ -196,10 +207,15 @@ void launch_h2s(hipStream_t s, const H2sJob* jobs, uint32_t n, const uint8_t* bl
                 const uint8_t* dst, uint32_t dst_len, uint32_t* out_scalars);
 void launch_hash_jobs(hipStream_t s, int kind, const H2sJob* jobs, uint32_t n, const uint8_t* blob,
                       const uint8_t* dst, uint32_t dst_len, uint8_t* out);
-// message hashes (H2sJob, DST dst) and domains (DomainJob) in one launch
-void launch_hash_stage(hipStream_t s, const DomainJob* dom, uint32_t n_dom, const H2sJob* h2s,
-                       uint32_t n_h2s, const uint8_t* blob, const uint8_t* dst, uint32_t dst_len,
-                       const uint8_t* pk_bytes, const uint8_t* gen_comp, uint32_t* out);
+// message hashes (MsgHashJob; the b_1 / b_2 input words in `tail`, tail_blocks
+// each; `zpad` the SHA-256 state after a zero block) and domains (DomainJob)
+// in one launch
+void launch_hash_stage(hipStream_t s, const DomainJob* dom, uint32_t n_dom, const MsgHashJob* msg,
+                       uint32_t n_msg, const uint8_t* blob, const uint32_t* zpad, const uint32_t* tail,
+                       uint32_t tail_blocks, const uint8_t* pk_bytes, const uint8_t* gen_comp,
+                       uint32_t* out);
+// the SHA-256 state after one all-zero block (8 words)
+void launch_sha_zpad(hipStream_t s, uint32_t* out);
 void launch_domain(hipStream_t s, const DomainJob* jobs, uint32_t n, const uint8_t* hdr_blob,
                    const uint8_t* pk_bytes, const uint8_t* gen_comp, uint32_t* out_scalars);
 void launch_gen_seeds(hipStream_t s, uint32_t first, uint32_t count, uint32_t* v_state,

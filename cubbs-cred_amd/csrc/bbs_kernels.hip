@@ -102,31 +102,43 @@ __global__ __launch_bounds__(256) void k_domain(const DomainJob* __restrict__ jo
   domain_job(jobs[i], hdr_blob, pk_bytes, gen_comp, out);
 }
 // Message hashes and domains are independent: one launch runs both, the long
-// domain jobs on lanes [0, n_dom), the message jobs on [n_dom, n_dom + n_h2s).
+// domain jobs on lanes [0, n_dom), the message jobs on [n_dom, n_dom + n_msg)
+// (ordered by block count, so the lanes of a wave compress in step).
 __global__ __launch_bounds__(256) void k_hash_stage(const DomainJob* __restrict__ dom, uint32_t n_dom,
-                                                   const H2sJob* __restrict__ h2s, uint32_t n_h2s,
+                                                   const MsgHashJob* __restrict__ msg, uint32_t n_msg,
                                                    const uint8_t* __restrict__ blob,
-                                                   const uint8_t* __restrict__ dst, uint32_t dst_len,
+                                                   const uint32_t* __restrict__ zpad,
+                                                   const uint32_t* __restrict__ tail, uint32_t tail_blocks,
                                                    const uint8_t* __restrict__ pk_bytes,
                                                    const uint8_t* __restrict__ gen_comp,
                                                    uint32_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_dom) {
     domain_job(dom[i], blob, pk_bytes, gen_comp, out);
-  } else if (i < n_dom + n_h2s) {
-    const H2sJob j = h2s[i - n_dom];
+  } else if (i < n_dom + n_msg) {
+    const MsgHashJob j = msg[i - n_dom];
     Fr sc;
-    hash_to_scalar_global(sc, blob + j.off, j.len, dst, dst_len);
+    h2s_blocks(sc, reinterpret_cast<const uint32_t*>(blob + j.off), j.blocks, zpad, tail, tail_blocks);
     store_fr(out + (size_t)j.out_index * FR_WORDS, sc);
   }
 }
-void launch_hash_stage(hipStream_t s, const DomainJob* dom, uint32_t n_dom, const H2sJob* h2s,
-                       uint32_t n_h2s, const uint8_t* blob, const uint8_t* dst, uint32_t dst_len,
-                       const uint8_t* pk_bytes, const uint8_t* gen_comp, uint32_t* out) {
-  const uint32_t n = n_dom + n_h2s;
+void launch_hash_stage(hipStream_t s, const DomainJob* dom, uint32_t n_dom, const MsgHashJob* msg,
+                       uint32_t n_msg, const uint8_t* blob, const uint32_t* zpad, const uint32_t* tail,
+                       uint32_t tail_blocks, const uint8_t* pk_bytes, const uint8_t* gen_comp,
+                       uint32_t* out) {
+  const uint32_t n = n_dom + n_msg;
   if (!n) return;
-  hipLaunchKernelGGL(k_hash_stage, GRID(n, 64), 0, s, dom, n_dom, h2s, n_h2s, blob, dst, dst_len,
-                     pk_bytes, gen_comp, out);
+  hipLaunchKernelGGL(k_hash_stage, GRID(n, 64), 0, s, dom, n_dom, msg, n_msg, blob, zpad, tail,
+                     tail_blocks, pk_bytes, gen_comp, out);
+}
+__global__ void k_sha_zpad(uint32_t* out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  Sha256 st;
+  sha_init_after_zpad(st);
+  for (int k = 0; k < 8; k++) out[k] = st.h[k];
+}
+void launch_sha_zpad(hipStream_t s, uint32_t* out) {
+  hipLaunchKernelGGL(k_sha_zpad, dim3(1), dim3(1), 0, s, out);
 }
 void launch_domain(hipStream_t s, const DomainJob* jobs, uint32_t n, const uint8_t* hdr_blob,
                    const uint8_t* pk_bytes, const uint8_t* gen_comp, uint32_t* out) {

@@ -20,7 +20,7 @@ inline size_t put(std::vector<uint8_t>& blob, const T* p, size_t n) {
 
 struct Plan {
   // host-side job lists
-  std::vector<H2sJob> h2s;
+  std::vector<MsgHashJob> msg;     // message hashes (MAP DST), inputs laid out below
   std::vector<DomainJob> dom;
   std::vector<DecodeG1Job> dec_g1;
   std::vector<DecodeFrJob> dec_fr;
@@ -63,6 +63,64 @@ struct Plan {
   uint32_t pip_job() {
     pip_out.push_back(partials(1));
     return (uint32_t)pip_out.size() - 1;
+  }
+  size_t tail_off = 0;            // b_1 / b_2 input words of the message hashes
+  uint32_t tail_blocks = 0;
+  // Lay out `n` bytes at a 16-byte aligned offset as big-endian 32-bit words
+  // (the byte order of SHA-256's words, so the device loads them as is).
+  static void be_words(uint8_t* q, size_t n) {
+    for (size_t i = 0; i + 4 <= n; i += 4) {
+      uint32_t w;
+      std::memcpy(&w, q + i, 4);
+      w = __builtin_bswap32(w);
+      std::memcpy(q + i, &w, 4);
+    }
+  }
+  size_t aligned16(size_t n) {
+    const size_t off = (blob.size() + 15) & ~(size_t)15;
+    blob.resize(off + n, 0);
+    return off;
+  }
+  // hash_to_scalar(m, map_dst) of one message into scalars[out_index]: the
+  // input of expand_message_xmd's b_0 after Z_pad, m || I2OSP(48, 2) || 0x00 ||
+  // DST || I2OSP(|DST|, 1), with SHA-256's padding (RFC 9380 5.3.1, FIPS
+  // 180-4 5.1.1), in whole blocks; the hashing itself runs on the device.
+  void hash_message(const uint8_t* m, size_t len, uint32_t out_index) {
+    const size_t dl = map_dst.size();
+    const size_t bytes_in = len + 3 + dl + 1;
+    const size_t nb = (bytes_in + 9 + 63) / 64;
+    const size_t off = aligned16(64 * nb);
+    uint8_t* q = blob.data() + off;
+    if (len) std::memcpy(q, m, len);
+    q[len] = 0;
+    q[len + 1] = 48;
+    q[len + 2] = 0;
+    if (dl) std::memcpy(q + len + 3, map_dst.data(), dl);
+    q[len + 3 + dl] = (uint8_t)dl;
+    q[bytes_in] = 0x80;
+    const uint64_t bits = (uint64_t)(64 + bytes_in) * 8;  // the Z_pad block counts
+    for (int k = 0; k < 8; k++) q[64 * nb - 1 - k] = (uint8_t)(bits >> (8 * k));
+    be_words(q, 64 * nb);
+    msg.push_back(MsgHashJob{off, (uint32_t)nb, out_index});
+  }
+  // The inputs of b_1 and b_2 for the message hashes: 32 bytes (filled on
+  // the device) || I2OSP(i, 1) || DST || I2OSP(|DST|, 1), padded.
+  void message_tails() {
+    const size_t dl = map_dst.size();
+    const size_t bytes_in = 32 + 1 + dl + 1;
+    const size_t nb = (bytes_in + 9 + 63) / 64;
+    tail_off = aligned16(2 * 64 * nb);
+    tail_blocks = (uint32_t)nb;
+    for (int i = 1; i <= 2; i++) {
+      uint8_t* q = blob.data() + tail_off + 64 * nb * (i - 1);
+      q[32] = (uint8_t)i;
+      if (dl) std::memcpy(q + 33, map_dst.data(), dl);
+      q[33 + dl] = (uint8_t)dl;
+      q[bytes_in] = 0x80;
+      const uint64_t bits = (uint64_t)bytes_in * 8;
+      for (int k = 0; k < 8; k++) q[64 * nb - 1 - k] = (uint8_t)(bits >> (8 * k));
+      be_words(q, 64 * nb);
+    }
   }
   size_t bytes(const void* p, size_t n) {
     size_t off = blob.size();

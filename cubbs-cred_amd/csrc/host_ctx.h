@@ -41,6 +41,7 @@ struct Context {
   uint32_t* gen_affine = nullptr;
   uint8_t* gen_comp = nullptr;
   uint32_t* gen_vstate = nullptr;
+  uint32_t* sha_zpad = nullptr;  // SHA-256 state after one zero block (8 words)
   // 8-bit comb tables of slots [0, table_slots), capacity table_cap slots,
   // never more than table_max (the budget)
   uint32_t* gen_tables = nullptr;
