@@ -575,6 +575,109 @@ def profiled(path, key):
     return None
 
 
+def run_in_flight(sets, steps, expect):
+    """Seconds for `steps` steps with the staged (Verify, ProofVerify) sets
+    in flight round-robin: a set is launched again once its previous results
+    are back and checked against `expect`."""
+    n = len(sets)
+
+    def finish(i):
+        v, p = sets[i]
+        v.sync()
+        p.sync()
+        if (v.results_raw(), p.results_raw()) != expect[i]:
+            raise SystemExit("results of a timed step differ from the expected ones")
+
+    t = time.perf_counter()
+    for step in range(steps):
+        i = step % n
+        if step >= n:
+            finish(i)
+        sets[i][0].launch()
+        sets[i][1].launch()
+    for step in range(max(0, steps - n), steps):
+        finish(step % n)
+    return time.perf_counter() - t
+
+
+# pipeline tuning table (bench.py --tune-pipeline), loaded by the headline run
+PIPELINE_TUNING = os.path.join(ROOT, "cubbs-cred_amd", "pipeline_tuning.json")
+TUNE_CLASSES = ((2048, 512), (N_VERIFY, N_PROOFS))
+TUNE_GROUPS = (4, 8)
+TUNE_CHUNKS = (2, 4, 8)
+TUNE_INFLIGHT = (4, 6, 8, 12)
+
+
+def tune_pipeline(g, vreqs, expect_v, pvreqs, graph=True):
+    """Measures every (aggregation group, comb terms per lane, sets in
+    flight) candidate for each size class (Verify, ProofVerify items per
+    set, the benchmark's requests cut to that size) as the headline runs
+    them: single-stream sets replayed from graphs, several in flight, results
+    checked.  Returns the table: all candidates, the fastest per class, and
+    the library's pipeline lines for cubbs_hip_tune_load."""
+    classes, lines = [], []
+    for nv, npv in TUNE_CLASSES:
+        vr, pr, ev = vreqs[:nv], pvreqs[:npv], expect_v[:nv]
+        cands = []
+        for grp in TUNE_GROUPS:
+            for ch in TUNE_CHUNKS:
+                sets = [(g.StagedBatch(g.BATCH_VERIFY, vr, single_stream=True, aggregate=grp, comb_chunk=ch),
+                         g.StagedBatch(g.BATCH_PROOF_VERIFY, pr, single_stream=True, aggregate=grp,
+                                       comb_chunk=ch))
+                        for _ in range(max(TUNE_INFLIGHT))]
+                if graph:
+                    for v, p in sets:
+                        v.use_graph(True)
+                        p.use_graph(True)
+                expect = []
+                for v, p in sets:
+                    v.launch()
+                    p.launch()
+                for v, p in sets:
+                    v.sync()
+                    p.sync()
+                    if [x for _, x in v.results()] != ev or not all(c == 0 and x for c, x in p.results()):
+                        raise SystemExit("tuning candidate results differ from the expected ones")
+                    expect.append((v.results_raw(), p.results_raw()))
+                for n in TUNE_INFLIGHT:
+                    steps = max(24, 3 * n)
+                    run_in_flight(sets[:n], n, expect)  # warm-up
+                    ms = min(run_in_flight(sets[:n], steps, expect) for _ in range(2)) * 1e3 / steps
+                    cands.append({"agg_group": grp, "comb_chunk": ch, "sets_in_flight": n,
+                                  "ms_per_step": round(ms, 4)})
+                    log("tune %d+%d group=%d chunk=%d inflight=%d: %.3f ms/step" % (nv, npv, grp, ch, n, ms))
+                for v, p in sets:
+                    v.free()
+                    p.free()
+        best = min(cands, key=lambda c: c["ms_per_step"])
+        classes.append({"verify_items": nv, "proof_verify_items": npv, "best": best, "candidates": cands})
+        for kind, items in (("verify", nv), ("proof_verify", npv)):
+            lines.append("pipeline kind=%s log2_items=%d chunk=%d group=%d" %
+                         (kind, round(math.log2(items)), best["comb_chunk"], best["agg_group"]))
+    return {"description": "Measured Verify/ProofVerify pipeline shapes per size class (items per set): "
+                           "aggregation group size, fixed-base comb terms per lane, staged sets in flight; "
+                           "single-stream sets replayed from HIP graphs, results checked",
+            "command": "python bench.py --tune-pipeline cubbs-cred_amd/pipeline_tuning.json",
+            "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+            "classes": classes, "table": lines}
+
+
+def load_pipeline_tuning(g):
+    """Installs the committed pipeline table in the library and returns the
+    choice for the headline's size class (None without a table)."""
+    if not os.path.exists(PIPELINE_TUNING):
+        return None
+    with open(PIPELINE_TUNING) as f:
+        table = json.load(f)
+    rc = g.tune_load("\n".join(table["table"]))
+    if rc != 0:
+        raise SystemExit("pipeline tuning table rejected (rc=%d)" % rc)
+    for c in table["classes"]:
+        if (c["verify_items"], c["proof_verify_items"]) == (N_VERIFY, N_PROOFS):
+            return dict(c["best"])
+    return None
+
+
 def mark():
     """A one-block probe kernel launched (and waited for) just outside the
     timed region, so that a kernel trace of this run can find the region's
@@ -610,8 +713,14 @@ def main():
     ap.add_argument("--fork", action="store_true",
                     help="staged batches fork their decode/product branch onto a second stream "
                          "(lower latency per batch, two hardware queues each)")
-    ap.add_argument("--inflight", type=int, default=INFLIGHT,
-                    help="staged batch sets in flight in the timed loop (1 = launch, wait, repeat)")
+    ap.add_argument("--inflight", type=int, default=0,
+                    help="staged batch sets in flight in the timed loop (1 = launch, wait, repeat; "
+                         "default: the pipeline tuning table's choice, else %d)" % INFLIGHT)
+    ap.add_argument("--tune-pipeline", metavar="PATH", default="",
+                    help="measure the Verify/ProofVerify pipeline shapes (aggregation group, comb "
+                         "terms per lane, sets in flight) per size class and write the table to PATH")
+    ap.add_argument("--no-pipeline-tuning", action="store_true",
+                    help="ignore the committed pipeline tuning table")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -648,7 +757,7 @@ def main():
             sys.stdout.write(p.stdout.decode())
             sys.exit(p.returncode)
         out = json.loads(p.stdout.decode().strip().splitlines()[-1])
-        if not args.no_serving and not args.profile_only:
+        if not args.no_serving and not args.profile_only and not args.msm_only and not args.tune_pipeline:
             t = time.time()
             out["latency"] = serving_child(args)
             log("serving run %.1fs: %s" % (time.time() - t, out["latency"]))
@@ -678,7 +787,15 @@ def main():
     t0 = time.time()
     vreqs, expect_v, pvreqs, kps, abi_ops, _ = build_workload(g)
     log("workload built in %.1fs" % (time.time() - t0))
-    nset = max(1, args.inflight)
+    if args.tune_pipeline:
+        table = tune_pipeline(g, vreqs, expect_v, pvreqs, graph=not args.no_graph)
+        with open(args.tune_pipeline, "w") as f:
+            json.dump(table, f, indent=1)
+            f.write("\n")
+        print(json.dumps({"pipeline_tuning": table["classes"]}), flush=True)
+        return
+    tuned = None if args.no_pipeline_tuning else load_pipeline_tuning(g)
+    nset = max(1, args.inflight or (tuned["sets_in_flight"] if tuned else INFLIGHT))
     single, agg = not args.fork, not args.no_aggregate
     sets = [(g.StagedBatch(g.BATCH_VERIFY, vreqs, single_stream=single, aggregate=agg),
              g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs, single_stream=single, aggregate=agg))
@@ -705,24 +822,8 @@ def main():
         assert all(c == 0 and x for c, x in rp), "proof verify results differ from expectation"
         expect.append((v.results_raw(), p.results_raw()))
 
-    def finish(i):
-        v, p = sets[i]
-        v.sync()
-        p.sync()
-        if (v.results_raw(), p.results_raw()) != expect[i]:
-            raise SystemExit("results of a timed step differ from the expected ones")
-
     mark()
-    t = time.perf_counter()
-    for step in range(args.steps):
-        i = step % nset
-        if step >= nset:
-            finish(i)
-        sets[i][0].launch()
-        sets[i][1].launch()
-    for step in range(max(0, args.steps - nset), args.steps):
-        finish(step % nset)
-    el = time.perf_counter() - t
+    el = run_in_flight(sets, args.steps, expect)
     mark()
     ms_per_step = el * 1e3 / args.steps
     ops = N_VERIFY + N_PROOFS
@@ -781,9 +882,12 @@ def main():
                       "verify_items": N_VERIFY, "proof_verify_items": N_PROOFS,
                       "launch": "kernel by kernel" if args.no_graph else "HIP graph replay per staged batch",
                       "batch_sets_in_flight": nset,
+                      "pipeline_tuning": (dict(tuned, source=os.path.relpath(PIPELINE_TUNING, ROOT))
+                                          if tuned else None),
                       "streams_per_batch": 1 if single else 2,
-                      "pairing_checks": ("per-key groups of 8 as random linear combinations, "
-                                         "per-item fallback for failing groups") if agg else "per item",
+                      "pairing_checks": ("per-key groups of %d as random linear combinations, "
+                                         "per-item fallback for failing groups"
+                                         % (tuned["agg_group"] if tuned else 8)) if agg else "per item",
                       "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                       "msm_terms": vb.shape()["msm_terms"] + pb.shape()["msm_terms"]},
            "roofline": roofline,

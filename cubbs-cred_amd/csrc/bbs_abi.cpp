@@ -498,18 +498,29 @@ void* cubbs_hip_batch_prepare_flags(uint32_t abi_version, int kind, const void* 
                                     size_t item_count, uint32_t flags, int32_t* status_out) {
   int32_t dummy;
   int32_t& st = status_out ? *status_out : dummy;
-  const uint32_t group = (flags >> 8) & 15u;
+  uint32_t group = (flags >> 8) & 15u;
+  const uint32_t chunk = (flags >> 12) & 127u;
   if ((flags & ~(uint32_t)(CUBBS_HIP_BATCH_SINGLE_STREAM | CUBBS_HIP_BATCH_AGGREGATE |
-                           CUBBS_HIP_BATCH_AGG_GROUP(15))) ||
-      (group && (group < 2 || group > 8 || !(flags & CUBBS_HIP_BATCH_AGGREGATE)))) {
+                           CUBBS_HIP_BATCH_AGG_GROUP(15) | CUBBS_HIP_BATCH_COMB_CHUNK(127))) ||
+      (group && (group < 2 || group > 8 || !(flags & CUBBS_HIP_BATCH_AGGREGATE))) || chunk > 64 ||
+      (chunk && kind != CUBBS_HIP_BATCH_VERIFY && kind != CUBBS_HIP_BATCH_PROOF_VERIFY)) {
     st = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
     return nullptr;
   }
+  if ((flags & CUBBS_HIP_BATCH_AGGREGATE) && !group) {
+    // the tuned group size of the batch's size class, else 8
+    uint32_t c = 0, gsz = 0;
+    const int kd = kind == CUBBS_HIP_BATCH_VERIFY ? 0 : kind == CUBBS_HIP_BATCH_PROOF_VERIFY ? 1 : -1;
+    group = tuned_pipeline(kd, item_count, c, gsz) && gsz ? gsz : 8u;
+  }
+  const TuneForce saved = tune_force();
+  if (chunk) tune_force().comb_chunk = chunk;
   set_thread_single_stream((flags & CUBBS_HIP_BATCH_SINGLE_STREAM) != 0);
-  set_thread_agg_group((flags & CUBBS_HIP_BATCH_AGGREGATE) ? (group ? group : 8u) : 0u);
+  set_thread_agg_group((flags & CUBBS_HIP_BATCH_AGGREGATE) ? group : 0u);
   void* b = cubbs_hip_batch_prepare(abi_version, kind, items, item_count, status_out);
   set_thread_single_stream(false);
   set_thread_agg_group(0);
+  tune_force() = saved;
   return b;
 }
 

@@ -27,17 +27,26 @@ static bool slice_ok(const CuBbsBbsGpuByteSlice& s) { return s.len == 0 || s.dat
 
 static const char* kApiId = "BBS_BLS12381G1_XMD:SHA-256_SSWU_RO_H2G_HM2S_";
 
-// Terms per comb lane: the calling thread's forced shape, else
-// CUBBS_HIP_COMB_CHUNK, else (primitive MSM programs only, `tuned`) the
-// tuner's choice for the batch's size class, else 4.  The whole-operation
-// pipelines keep 4: their comb runs beside decode and the GLV products and
-// feeds the reduction, which the comb-only tuning does not see.
+// Terms per comb lane: the calling thread's forced shape (the tuner's
+// candidate, or CUBBS_HIP_BATCH_COMB_CHUNK), else CUBBS_HIP_COMB_CHUNK, else
+// (primitive MSM programs only, `tuned`) the comb tuner's choice for the
+// batch's size class, else 4.  The whole-operation pipelines use
+// pipeline_comb_chunk: their comb runs beside decode and the GLV products
+// and feeds the reduction, which the comb-only tuning does not see.
 uint32_t comb_chunk_terms(uint64_t batch_terms, bool tuned) {
   if (tune_force().comb_chunk) return tune_force().comb_chunk;
   if (knobs().comb_chunk) return knobs().comb_chunk;
   if (!tuned) return 4;
   const uint32_t t = tuned_comb_chunk(batch_terms);
   return t ? t : 4;
+}
+// As comb_chunk_terms for a Verify (kind 0) or ProofVerify (1) batch of
+// `items`: the pipeline tuner's choice for its size class, else 4.
+static uint32_t pipeline_comb_chunk(int kind, uint64_t items) {
+  if (tune_force().comb_chunk) return tune_force().comb_chunk;
+  if (knobs().comb_chunk) return knobs().comb_chunk;
+  uint32_t chunk = 0, group = 0;
+  return tuned_pipeline(kind, items, chunk, group) && chunk ? chunk : 4;
 }
 
 // ------------------------------------------------------------------ arenas
@@ -539,7 +548,6 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
   b->dev_index.assign(n, 0);
   std::vector<size_t> dev;
   uint32_t maxL = 0;
-  uint64_t est_terms = 0;  // fixed-base terms of the batch (tuning class)
   for (size_t i = 0; i < n; i++) {
     const auto& it = items[i];
     bool ok = it.public_key.len == 96 && it.public_key.data && it.signature.len == 80 &&
@@ -553,7 +561,6 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
     b->dev_index[i] = (uint32_t)dev.size();
     dev.push_back(i);
     maxL = std::max<uint32_t>(maxL, (uint32_t)it.message_count);
-    est_terms += it.message_count + 2;
   }
   b->nv = (uint32_t)dev.size();
   if (!b->nv) return b;
@@ -564,7 +571,7 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
   Plan p;
   std::string d = map_dst();
   p.map_dst.assign(d.begin(), d.end());
-  const uint32_t CH = comb_chunk_terms(est_terms);
+  const uint32_t CH = pipeline_comb_chunk(0, n);
   // terms first: [0, T)
   uint32_t T = 0;
   for (uint32_t k = 0; k < b->nv; k++) T += (uint32_t)items[dev[k]].message_count + 2;
@@ -652,7 +659,6 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
   b->dev_index.assign(n, 0);
   std::vector<size_t> dev;
   uint32_t maxL = 0;
-  uint64_t est_terms = 0;  // fixed-base terms of the batch (tuning class)
   for (size_t i = 0; i < n; i++) {
     const auto& it = items[i];
     // protocol.rs:774-779: bad disclosed indexes or count mismatch -> Ok(false)
@@ -683,7 +689,6 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
     b->dev_index[i] = (uint32_t)dev.size();
     dev.push_back(i);
     maxL = std::max<uint32_t>(maxL, (uint32_t)L);
-    est_terms += L + 2;
   }
   b->nv = (uint32_t)dev.size();
   if (!b->nv) return b;
@@ -694,7 +699,7 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
   Plan p;
   std::string d = map_dst();
   p.map_dst.assign(d.begin(), d.end());
-  const uint32_t CH = comb_chunk_terms(est_terms);
+  const uint32_t CH = pipeline_comb_chunk(1, n);
   uint32_t T = 0;
   for (uint32_t k = 0; k < b->nv; k++) {
     const auto& it = items[dev[k]];
@@ -1050,8 +1055,9 @@ void launch_batch(Batch& b) {
   // Primitive programs decode without the subgroup check too.  The check
   // runs as extra lanes of the product launch, or, with a Pippenger MSM, as
   // its own launch on the auxiliary stream beside the MSM's last stage (the
-  // cross-window Horner: one wave per MSM, which leaves the GPU idle), joined
-  // before the reduction.
+  // window combination: one block per MSM, which leaves the rest of the GPU
+  // to it), joined before the reduction.  Beside the sorting and the piece
+  // sums, or the halving steps, it only slowed them.
   const bool prim_split = b.kind == KIND_PRIM && b.n_dec_g1 && b.aux != nullptr;
   const bool side_check = prim_split && b.pip.terms;
   if (b.n_dec_fr) {

@@ -21,6 +21,7 @@
 // and the caller converts the result to affine form.
 #include <algorithm>
 #include <cstdlib>
+#include <stdexcept>
 
 #include "bbs_internal.h"
 #include "dev_io.hpp"
@@ -261,70 +262,201 @@ void launch_pip_bucket(hipStream_t s, const PipShape& sh, const PipBuffers& bf) 
   hipLaunchKernelGGL(k_pip_bucket, GRID(sh.nbt, 64), 0, s, sh, bf);
 }
 
+// ------------------------------------------------------------------ window combination
+// The window sums and their combination are chains of dependent group
+// operations (about 129 doublings in all), so their latency, not their work,
+// sets the time.  Each chain runs on the four lanes of a quad: the
+// independent field products of one doubling or addition run side by side,
+// one per lane, and DPP quad broadcasts (quad_perm [j, j, j, j]) hand every
+// product to all four lanes.  A doubling takes 3 product latencies instead
+// of 7, an addition 5 instead of 16.  The four lanes of a quad always hold
+// the same point values, so every branch is uniform within the quad.
+
+template <int J>
+DEV void fp_quad_bcast(Fp& r, const Fp& a) {
+#pragma unroll
+  for (int k = 0; k < FPW; k++)
+    r.v[k] = (uint32_t)__builtin_amdgcn_mov_dpp((int)a.v[k], J * 0x55, 0xf, 0xf, false);
+}
+// r = (a0, a1, a2, a3)[q] by two levels of selects on the bits of q (a
+// chain of comparisons with q would be turned into an indexed table in
+// scratch memory)
+DEV void fp_sel4(Fp& r, uint32_t q, const Fp& a0, const Fp& a1, const Fp& a2, const Fp& a3) {
+  const bool b0 = (q & 1) != 0, b1 = (q & 2) != 0;
+#pragma unroll
+  for (int k = 0; k < FPW; k++) {
+    const uint32_t lo = b0 ? a1.v[k] : a0.v[k], hi = b0 ? a3.v[k] : a2.v[k];
+    r.v[k] = b1 ? hi : lo;
+  }
+}
+
+// p <- 2 p (dbl-2009-l); q = lane index within the quad
+DEV void g1_dbl_quad(G1J& p, uint32_t q) {
+  Fp a, b, r, A, B, YZ;
+  fp_sel4(a, q, p.x, p.y, p.y, p.y);  // X^2 | Y^2 | Y Z | Y Z
+  fp_sel4(b, q, p.x, p.y, p.z, p.z);
+  fp_mul(r, a, b);
+  fp_quad_bcast<0>(A, r);
+  fp_quad_bcast<1>(B, r);
+  fp_quad_bcast<2>(YZ, r);
+  Fp E, S;
+  fp_dbl(E, A);  // E = 3 A
+  fp_add(E, E, A);
+  fp_add(S, p.x, B);
+  fp_sel4(a, q, E, B, S, S);  // F = E^2 | C = B^2 | T = (X + B)^2
+  fp_sqr(r, a);
+  Fp F, C;
+  fp_quad_bcast<0>(F, r);
+  fp_quad_bcast<1>(C, r);
+  fp_quad_bcast<2>(S, r);
+  fp_sub(S, S, A);  // D = 2 (T - A - C)
+  fp_sub(S, S, C);
+  fp_dbl(S, S);
+  fp_sub(p.x, F, S);  // X3 = F - 2 D
+  fp_sub(p.x, p.x, S);
+  fp_sub(S, S, p.x);  // Y3 = E (D - X3) - 8 C
+  fp_mul(p.y, E, S);
+  fp_dbl(C, C);
+  fp_dbl(C, C);
+  fp_dbl(C, C);
+  fp_sub(p.y, p.y, C);
+  fp_dbl(p.z, YZ);  // Z3 = 2 Y Z (the identity keeps Z = 0)
+}
+
+// p <- p + o (add-2007-bl, both Jacobian, o not aliasing p)
+DEV void g1_add_quad(G1J& p, const G1J& o, uint32_t q) {
+  if (fp_is_zero(o.z)) return;
+  if (fp_is_zero(p.z)) {
+    p = o;
+    return;
+  }
+  Fp a, b, r, Z1Z1, Z2Z2, U1, S1, H, R;
+  fp_sel4(a, q, p.z, o.z, p.y, o.y);  // Z1^2 | Z2^2 | Y1 Z2 | Y2 Z1
+  fp_sel4(b, q, p.z, o.z, o.z, p.z);
+  fp_mul(r, a, b);
+  fp_quad_bcast<0>(Z1Z1, r);
+  fp_quad_bcast<1>(Z2Z2, r);
+  fp_quad_bcast<2>(S1, r);
+  fp_quad_bcast<3>(R, r);
+  fp_sel4(a, q, p.x, o.x, S1, R);  // U1 | U2 | S1 | S2
+  fp_sel4(b, q, Z2Z2, Z1Z1, Z2Z2, Z1Z1);
+  fp_mul(r, a, b);
+  fp_quad_bcast<0>(U1, r);
+  fp_quad_bcast<1>(H, r);
+  fp_quad_bcast<2>(S1, r);
+  fp_quad_bcast<3>(R, r);
+  fp_sub(H, H, U1);  // H = U2 - U1
+  fp_sub(R, R, S1);  // S2 - S1
+  if (fp_is_zero(H)) {
+    if (fp_is_zero(R))
+      g1_dbl_quad(p, q);
+    else
+      pt_set_inf(p);
+    return;
+  }
+  fp_dbl(R, R);  // r = 2 (S2 - S1)
+  fp_add(b, p.z, o.z);
+  fp_dbl(a, H);
+  fp_sel4(a, q, a, b, R, R);  // I = (2 H)^2 | (Z1 + Z2)^2 | r^2
+  fp_sqr(r, a);
+  Fp I, R2;
+  fp_quad_bcast<0>(I, r);
+  fp_quad_bcast<1>(b, r);
+  fp_quad_bcast<2>(R2, r);
+  fp_sub(b, b, Z1Z1);  // (Z1 + Z2)^2 - Z1Z1 - Z2Z2
+  fp_sub(b, b, Z2Z2);
+  fp_sel4(a, q, H, U1, b, b);  // J = H I | V = U1 I | Z3 = (..) H
+  fp_sel4(b, q, I, I, H, H);
+  fp_mul(r, a, b);
+  Fp Jv, V;
+  fp_quad_bcast<0>(Jv, r);
+  fp_quad_bcast<1>(V, r);
+  fp_quad_bcast<2>(p.z, r);
+  fp_sub(p.x, R2, Jv);  // X3 = r^2 - J - 2 V
+  fp_sub(p.x, p.x, V);
+  fp_sub(p.x, p.x, V);
+  fp_sub(V, V, p.x);
+  fp_sel4(a, q, R, S1, R, R);  // r (V - X3) | S1 J
+  fp_sel4(b, q, V, Jv, V, V);
+  fp_mul(r, a, b);
+  fp_quad_bcast<0>(V, r);
+  fp_quad_bcast<1>(Jv, r);
+  fp_dbl(Jv, Jv);  // Y3 = r (V - X3) - 2 S1 J
+  fp_sub(p.y, V, Jv);
+}
+
 // ------------------------------------------------------------------ halving step
 // Step s (1..K), M = B >> s: X_i += X_{i+M} for i in [0, M) and, for
 // u = 1..s-1, in [B >> u, (B >> u) + M); reads and writes of one step are
 // disjoint (the ranges written start at multiples of 2M, those read at an
-// odd multiple of M).
+// odd multiple of M).  A step holds one addition per element and few
+// elements, so each addition runs on a quad (g1_add_quad).
 __global__ __launch_bounds__(64) void k_pip_halve(PipShape sh, uint32_t step, uint32_t M,
                                                  uint32_t n, uint32_t* __restrict__ X) {
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t idx = t >> 2, q = t & 3;
+  if (idx >= n) return;  // whole quads leave together
   const uint32_t per = step * M;
-  const uint32_t jw = idx / per, q = idx % per;
-  const uint32_t u = q / M, r = q % M;
+  const uint32_t jw = idx / per, rq = idx % per;
+  const uint32_t u = rq / M, r = rq % M;
   const uint32_t i = (u ? (sh.B >> u) : 0u) + r;
   uint32_t* base = X + (size_t)jw * sh.B * G1J_WORDS;
   G1J a, b;
   load_g1j(a, base + (size_t)i * G1J_WORDS);
   load_g1j(b, base + (size_t)(i + M) * G1J_WORDS);
-  g1_add_in(a, b);
-  store_g1j(base + (size_t)i * G1J_WORDS, a);
+  g1_add_quad(a, b, q);
+  if (q == 0) store_g1j(base + (size_t)i * G1J_WORDS, a);
 }
 void launch_pip_halve(hipStream_t s, const PipShape& sh, uint32_t step, const PipBuffers& bf) {
   if (!sh.terms || step < 1 || step > sh.K) return;
   const uint32_t M = sh.B >> step;
   const uint32_t n = sh.jobs * sh.W * step * M;
-  hipLaunchKernelGGL(k_pip_halve, GRID(n, 64), 0, s, sh, step, M, n, bf.buckets);
+  hipLaunchKernelGGL(k_pip_halve, GRID(4 * (size_t)n, 64), 0, s, sh, step, M, n, bf.buckets);
 }
 
-// ------------------------------------------------------------------ window combination
-// One block per MSM: lane w forms window w's sum X_0 + sum_j 2^j X_{2^j};
-// lane 0 then combines the windows, c doublings apart.
-__global__ __launch_bounds__(64) void k_pip_final(PipShape sh, const uint32_t* __restrict__ job_out,
-                                                 const uint32_t* __restrict__ X,
-                                                 uint32_t* __restrict__ partials) {
-  __shared__ uint32_t win[64 * G1J_WORDS];
-  const uint32_t job = blockIdx.x, w = threadIdx.x;
+// One block per MSM, one quad per window: quad w forms window w's sum
+// X_0 + sum_j 2^j X_{2^j} (Horner over j), then quad 0 combines the windows,
+// c doublings apart.
+constexpr uint32_t PIP_FINAL_MAX_W = 33;  // c = 4
+__global__ __launch_bounds__(256) void k_pip_final(PipShape sh, const uint32_t* __restrict__ job_out,
+                                                  const uint32_t* __restrict__ X,
+                                                  uint32_t* __restrict__ partials) {
+  __shared__ uint32_t win[PIP_FINAL_MAX_W * G1J_WORDS];
+  const uint32_t job = blockIdx.x, w = threadIdx.x >> 2, q = threadIdx.x & 3;
   if (w < sh.W) {
     const uint32_t* base = X + (size_t)(job * sh.W + w) * sh.B * G1J_WORDS;
     G1J acc, x;
     pt_set_inf(acc);
+#pragma unroll 1
     for (int j = (int)sh.K - 1; j >= 0; j--) {
-      g1_dbl_in(acc);
+      g1_dbl_quad(acc, q);
       load_g1j(x, base + (size_t)(1u << j) * G1J_WORDS);
-      g1_add_in(acc, x);
+      g1_add_quad(acc, x, q);
     }
     load_g1j(x, base);
-    g1_add_in(acc, x);
-    store_g1j(win + w * G1J_WORDS, acc);
+    g1_add_quad(acc, x, q);
+    if (q == 0) store_g1j(win + w * G1J_WORDS, acc);
   }
   __syncthreads();
   if (w == 0) {
     G1J r, x;
     load_g1j(r, win + (sh.W - 1) * G1J_WORDS);
+#pragma unroll 1
     for (int v = (int)sh.W - 2; v >= 0; v--) {
-      for (uint32_t d = 0; d < sh.c; d++) g1_dbl_in(r);
+#pragma unroll 1
+      for (uint32_t d = 0; d < sh.c; d++) g1_dbl_quad(r, q);
       load_g1j(x, win + v * G1J_WORDS);
-      g1_add_in(r, x);
+      g1_add_quad(r, x, q);
     }
-    store_g1j(partials + (size_t)job_out[job] * G1J_WORDS, r);
+    if (q == 0) store_g1j(partials + (size_t)job_out[job] * G1J_WORDS, r);
   }
 }
 void launch_pip_final(hipStream_t s, const PipShape& sh, const uint32_t* job_out,
                       const PipBuffers& bf, uint32_t* partials) {
   if (!sh.terms) return;
-  hipLaunchKernelGGL(k_pip_final, dim3(sh.jobs), dim3(64), 0, s, sh, job_out, bf.buckets, partials);
+  if (sh.W > PIP_FINAL_MAX_W) throw std::runtime_error("pippenger: more windows than the final stage holds");
+  const uint32_t threads = (4 * sh.W + 63) / 64 * 64;
+  hipLaunchKernelGGL(k_pip_final, dim3(sh.jobs), dim3(threads), 0, s, sh, job_out, bf.buckets, partials);
 }
 
 }  // namespace cubbs

@@ -58,7 +58,7 @@ void launch_copy(hipStream_t s, void* dst, const void* src, size_t bytes) {
 // ------------------------------------------------------------------ decoding
 // check_subgroup = 0 leaves the prime-order subgroup check to the lanes that
 // k_var_mul appends for these jobs.
-__global__ __launch_bounds__(64) void k_decode_g1(const DecodeG1Job* __restrict__ jobs, uint32_t n,
+__global__ __launch_bounds__(64, 2) void k_decode_g1(const DecodeG1Job* __restrict__ jobs, uint32_t n,
                                                  const uint8_t* __restrict__ blob,
                                                  uint32_t* __restrict__ points,
                                                  uint8_t* __restrict__ pinf,
@@ -68,9 +68,10 @@ __global__ __launch_bounds__(64) void k_decode_g1(const DecodeG1Job* __restrict_
   if (i >= n) return;
   DecodeG1Job j = jobs[i];
   G1J p;
-  bool ok = g1_decompress(p, blob + j.off, check_subgroup != 0);
+  bool ok = g1_decompress(p, blob + j.off, false);
   G1A a;
-  if (ok && !pt_is_inf(p)) {
+  const bool finite = ok && !pt_is_inf(p);
+  if (finite) {
     a.x = p.x;
     a.y = p.y;
     pinf[j.out] = 0;
@@ -79,7 +80,10 @@ __global__ __launch_bounds__(64) void k_decode_g1(const DecodeG1Job* __restrict_
     fp_zero(a.y);
     pinf[j.out] = 1;
   }
-  store_g1a(points + (size_t)j.out * G1A_WORDS, a);
+  uint32_t* const at = points + (size_t)j.out * G1A_WORDS;
+  store_g1a(at, a);
+  // the check reads the point back from memory rather than keeping it live
+  if (finite && check_subgroup) ok = g1_in_subgroup_at(at);
   if (!ok) status[j.item] = 1;
 }
 void launch_decode_g1(hipStream_t s, const DecodeG1Job* jobs, uint32_t n, const uint8_t* blob,
@@ -314,7 +318,7 @@ DEV void var_mul_one(G1J& r, const G1A& p, const uint32_t k[8], uint32_t* ws) {
 // Lanes [n, n + n_sub): the prime-order subgroup check of the points decoded
 // by jobs sub[] without it, run beside the products; a failure marks the
 // item's status (the products of such a point are computed but never used).
-__global__ __launch_bounds__(64) void k_var_mul(const VarJob* __restrict__ jobs, uint32_t n,
+__global__ __launch_bounds__(64, 2) void k_var_mul(const VarJob* __restrict__ jobs, uint32_t n,
                                                const uint32_t* __restrict__ points,
                                                const uint8_t* __restrict__ pinf,
                                                const uint32_t* __restrict__ scalars,
@@ -328,9 +332,7 @@ __global__ __launch_bounds__(64) void k_var_mul(const VarJob* __restrict__ jobs,
     if (k >= n_sub) return;
     const DecodeG1Job j = sub[k];
     if (pinf[j.out]) return;  // the identity, or not decoded (status already set)
-    G1A a;
-    load_g1a(a, points + (size_t)j.out * G1A_WORDS);
-    if (!g1_in_subgroup(a)) status[j.item] = 1;
+    if (!g1_in_subgroup_at(points + (size_t)j.out * G1A_WORDS)) status[j.item] = 1;
     return;
   }
   const VarJob j = jobs[t];

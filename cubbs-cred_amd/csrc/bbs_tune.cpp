@@ -9,7 +9,10 @@
 //    fixed-base terms (k = 12, 14, 16, 18): 1, 2, 4, 8 or 16 terms per lane.
 // Lookups use the nearest tuned class within two powers of two.  The table is
 // reported as text (one line per class) and can be installed again at start-up
-// with cubbs_hip_tune_load.
+// with cubbs_hip_tune_load.  The table also takes lines for the whole-operation
+// Verify / ProofVerify pipelines (comb terms per lane and aggregation group
+// size by items per batch), which a caller measures by running sets of such
+// batches in flight, as it runs them, and loads the same way.
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -33,6 +36,9 @@ struct Table {
   uint8_t pip_c[CLASSES] = {};
   uint16_t pip_s[CLASSES] = {};
   uint8_t comb[CLASSES] = {};
+  // whole-operation pipelines, [kind][log2 items]: comb chunk, group size
+  uint8_t pipe_chunk[2][CLASSES] = {};
+  uint8_t pipe_group[2][CLASSES] = {};
 };
 Table& table() {
   static Table t;
@@ -118,6 +124,17 @@ uint32_t tuned_comb_chunk(uint64_t terms) {
   std::lock_guard<std::mutex> g(t.mu);
   const int k = nearest_class(log2_near(terms), [&](int kk) { return t.comb[kk] != 0; });
   return k < 0 ? 0 : t.comb[k];
+}
+
+bool tuned_pipeline(int kind, uint64_t items, uint32_t& comb_chunk, uint32_t& agg_group) {
+  if (kind < 0 || kind > 1) return false;
+  Table& t = table();
+  std::lock_guard<std::mutex> g(t.mu);
+  const int k = nearest_class(log2_near(items), [&](int kk) { return t.pipe_chunk[kind][kk] != 0; });
+  if (k < 0) return false;
+  comb_chunk = t.pipe_chunk[kind][k];
+  agg_group = t.pipe_group[kind][k];
+  return true;
 }
 
 }  // namespace cubbs
@@ -243,6 +260,8 @@ int32_t cubbs_hip_tune_load(const char* text) {
     std::memset(t.pip_c, 0, sizeof t.pip_c);
     std::memset(t.pip_s, 0, sizeof t.pip_s);
     std::memset(t.comb, 0, sizeof t.comb);
+    std::memset(t.pipe_chunk, 0, sizeof t.pipe_chunk);
+    std::memset(t.pipe_group, 0, sizeof t.pipe_group);
     return CUBBS_BBS_GPU_SUCCESS;
   }
   const char* p = text;
@@ -250,6 +269,7 @@ int32_t cubbs_hip_tune_load(const char* text) {
     const char* e = std::strchr(p, '\n');
     const std::string ln(p, e ? (size_t)(e - p) : std::strlen(p));
     unsigned k = 0, a = 0, b = 0;
+    char kind[16] = {};
     if (std::sscanf(ln.c_str(), "pippenger log2_terms=%u c=%u S=%u", &k, &a, &b) == 3) {
       if (k >= CLASSES || a < 4 || a > 16 || b < 1 || b > 256) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
       t.pip_c[k] = (uint8_t)a;
@@ -257,6 +277,12 @@ int32_t cubbs_hip_tune_load(const char* text) {
     } else if (std::sscanf(ln.c_str(), "comb log2_terms=%u chunk=%u", &k, &a) == 2) {
       if (k >= CLASSES || a < 1 || a > 64) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
       t.comb[k] = (uint8_t)a;
+    } else if (std::sscanf(ln.c_str(), "pipeline kind=%15s log2_items=%u chunk=%u group=%u", kind, &k, &a,
+                           &b) == 4) {
+      const int kd = std::strcmp(kind, "verify") == 0 ? 0 : std::strcmp(kind, "proof_verify") == 0 ? 1 : -1;
+      if (kd < 0 || k >= CLASSES || a < 1 || a > 64 || b < 2 || b > 8) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      t.pipe_chunk[kd][k] = (uint8_t)a;
+      t.pipe_group[kd][k] = (uint8_t)b;
     } else if (!ln.empty()) {
       return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
     }

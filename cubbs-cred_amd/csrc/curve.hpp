@@ -412,17 +412,69 @@ DEV void g1_endo(G1J& r, const G1J& p) {
 // BLS12-381", 2019): P in G1 iff phi'(P) = [-x^2] P where phi' = phi^2 acts
 // as lambda^2 = -x^2.  Here: [x^2] P == (beta^2 x, -y).
 // The point is given in affine form (not the identity).
-DEVN bool g1_in_subgroup(const G1A& pa) {
+//
+// |x| = 0xd201000000010000 has the bits 63, 62, 60, 57, 48 and 16 set: after
+// the top bit, runs of 1, 2, 3, 9 and 32 doublings each end in an addition
+// of the base and 16 doublings follow.  Both products [|x|] P and
+// [|x|] ([|x|] P) walk these runs with the in-line group law on locals (no
+// call frames): 126 doublings, 5 mixed and 5 full additions.
+DEV int g1_x_run(int s) { return s == 0 ? 1 : s == 1 ? 2 : s == 2 ? 3 : s == 3 ? 9 : s == 4 ? 32 : 16; }
+// `load(a)` fetches the affine point (not the identity) wherever it is
+// needed, so that callers holding it in memory keep it out of registers
+// across the doubling runs.
+template <class Load>
+DEV bool g1_subgroup_test(Load load) {
   G1J t;
-  pt_mul_u64_affine(t, pa, 0xd201000000010000ull);
-  pt_mul_u64(t, t, 0xd201000000010000ull);
-  G1A q;
-  Fp beta;
+  {
+    G1A pa;
+    load(pa);
+    pt_from_affine(t, pa);
+  }
+#pragma unroll 1
+  for (int s = 0; s < 6; s++) {
+#pragma unroll 1
+    for (int i = 0; i < g1_x_run(s); i++) g1_dbl_in(t);
+    if (s < 5) {
+      G1A pa;
+      load(pa);
+      g1_madd_in(t, pa);
+    }
+  }
+  const G1J q = t;
+#pragma unroll 1
+  for (int s = 0; s < 6; s++) {
+#pragma unroll 1
+    for (int i = 0; i < g1_x_run(s); i++) g1_dbl_in(t);
+    if (s < 5) g1_add_in(t, q);
+  }
+  // t == (beta^2 x, -y)?  (t the identity: no, as the point is not)
+  if (fp_is_zero(t.z)) return false;
+  G1A pa;
+  load(pa);
+  Fp beta, u, z2;
   fp_load_const(beta, bls::GLV_BETA);
-  fp_mul(q.x, pa.x, beta);
-  fp_mul(q.x, q.x, beta);
-  fp_neg(q.y, pa.y);
-  return pt_eq_affine(t, q);
+  fp_mul(u, pa.x, beta);
+  fp_mul(u, u, beta);
+  fp_sqr(z2, t.z);
+  fp_mul(u, u, z2);
+  if (!fp_eq(u, t.x)) return false;
+  fp_mul(z2, z2, t.z);
+  fp_neg(u, pa.y);
+  fp_mul(u, u, z2);
+  return fp_eq(u, t.y);
+}
+DEV bool g1_in_subgroup(const G1A& pa) {
+  return g1_subgroup_test([&](G1A& a) { a = pa; });
+}
+// The point stored affine at pm (G1A_WORDS words).
+DEV bool g1_in_subgroup_at(const uint32_t* pm) {
+  return g1_subgroup_test([&](G1A& a) {
+#pragma unroll
+    for (int k = 0; k < FPW; k++) {
+      a.x.v[k] = pm[k];
+      a.y.v[k] = pm[FPW + k];
+    }
+  });
 }
 
 // Decode a 48-byte compressed G1 point (bls12_381 G1Affine::from_compressed

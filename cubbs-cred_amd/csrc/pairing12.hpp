@@ -373,7 +373,16 @@ DEV void cyc_sqr(Fp& f, const Lane& L) {
   sel(f, b2 < 0, d, e);
 }
 
-DEVN void mul(Fp& f, const Fp& g, Lane L) { mul_in(f, g, L); }
+// The out-of-line routines below copy their reference operands into locals
+// on entry and write the result back once: working on the references would
+// keep the operand in the caller's stack frame, stored and reloaded through
+// flat memory around every LDS access of the loops.
+DEVN void mul(Fp& f, const Fp& g, Lane L) {
+  Fp a = f;
+  const Fp b = g;
+  mul_in(a, b, L);
+  f = a;
+}
 
 // f <- conj(f) (x -> x^(p^6)): odd slots negated
 DEV void conj(Fp& f, const Lane& L) {
@@ -383,10 +392,11 @@ DEV void conj(Fp& f, const Lane& L) {
 }
 // x -> x^p: slot k conjugated, times gamma1[k] = xi^(k (p-1)/6)
 DEVN void frob(Fp& f, Lane L) {
+  const Fp v = f;
   Fp o, x0, x1, g0, g1, z, n0, y0, y1;
-  partner(o, f);
-  sel(x0, L.c, o, f);
-  sel(x1, L.c, f, o);
+  partner(o, v);
+  sel(x0, L.c, o, v);
+  sel(x1, L.c, v, o);
   fp_load_const(g0, bls::FROB1 + 2 * FPW * L.k);
   fp_load_const(g1, bls::FROB1 + 2 * FPW * L.k + FPW);
   fp_zero(z);
@@ -398,13 +408,16 @@ DEVN void frob(Fp& f, Lane L) {
   w_zero(w);
   w_mac(w, x0, y0);
   w_mac(w, x1, y1);
-  w_reduce(f, w);
+  Fp r;
+  w_reduce(r, w);
+  f = r;
 }
 // x -> x^(p^2): slot k times gamma2[k] in Fp
 DEVN void frob2(Fp& f, Lane L) {
-  Fp g;
+  Fp g, v = f;
   fp_load_const(g, bls::FROB2 + FPW * L.k);
-  fp_mul(f, f, g);
+  fp_mul(v, v, g);
+  f = v;
 }
 
 // Inverse through the norm to Fp6: with g = conj(f) (odd slots negated),
@@ -499,11 +512,12 @@ DEVN void inv(Fp& f, Lane L) {
 
 // a^x for a cyclotomic element, x = -|x|
 DEVN void exp_by_x(Fp& r, const Fp& a, Lane L) {
-  Fp acc = a;
+  const Fp base = a;
+  Fp acc = base;
 #pragma unroll 1
   for (int i = 62; i >= 0; i--) {
     cyc_sqr(acc, L);
-    if ((BLS_X_ABS >> i) & 1) mul_in(acc, a, L);
+    if ((BLS_X_ABS >> i) & 1) mul_in(acc, base, L);
   }
   conj(acc, L);
   r = acc;
@@ -553,32 +567,35 @@ DEVN void final_exp(Fp& r, const Fp& f, Lane L) {
 // of P_q, zero for a skipped pair (its blocks hold the line 1).  Conjugated
 // at the end (x < 0).
 DEVN void miller(Fp& f, const uint32_t* l2, const Fp& yp0, const Fp& yp1, Lane L) {
-  Fp one, z;
+  const Fp y0 = yp0, y1 = yp1;
+  Fp acc, one, z;
   fp_one(one);
   fp_zero(z);
-  sel(f, L.k == 0 && L.c == 0, one, z);
+  sel(acc, L.k == 0 && L.c == 0, one, z);
   int idx = 0;
 #pragma unroll 1
   for (int i = 62; i >= 0; i--) {
-    if (i != 62) sqr(f, L);
+    if (i != 62) sqr(acc, L);
     const int steps = ((BLS_X_ABS >> i) & 1) ? 2 : 1;
 #pragma unroll 1
     for (int s = 0; s < steps; s++) {
 #pragma unroll 1
       for (int q = 0; q < 2; q++)
-        mul_line(f, l2 + ((size_t)idx * 2 + q) * LV_N * E, q ? yp1 : yp0, L);
+        mul_line(acc, l2 + ((size_t)idx * 2 + q) * LV_N * E, q ? y1 : y0, L);
       idx++;
     }
   }
-  conj(f, L);
+  conj(acc, L);
+  f = acc;
 }
 
 // 1 when the element is the identity (the same answer on every lane of the group)
 DEVN bool is_one(const Fp& f, Lane L) {
+  const Fp v = f;
   __syncthreads();
   Fp one;
   fp_one(one);
-  const bool ok = (L.k == 0 && L.c == 0) ? fp_eq(f, one) : fp_is_zero(f);
+  const bool ok = (L.k == 0 && L.c == 0) ? fp_eq(v, one) : fp_is_zero(v);
   if (L.wr) L.X[2 * L.k + L.c] = ok ? 1u : 0u;
   __syncthreads();
   bool all = true;

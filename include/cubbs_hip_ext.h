@@ -89,10 +89,16 @@ void *cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void *items,
  * is the batch test of the reference's default same-key verify path.  It
  * lengthens one batch's chain by a pairing launch, so it pays when several
  * batches are in flight: eight single-stream sets ran 4.5 ms per step with
- * it against 6.5 ms without.  The group size is 8 unless
- * CUBBS_HIP_BATCH_AGG_GROUP(n) (2 <= n <= 8) is or-ed into the flags. */
+ * it against 6.5 ms without.  The group size is the one of CUBBS_HIP_BATCH_AGG_GROUP(n)
+ * (2 <= n <= 8) when that is or-ed into the flags, else the pipeline tuning
+ * table's for the batch's size class (cubbs_hip_tune_load), else 8.
+ * CUBBS_HIP_BATCH_COMB_CHUNK(n) (1 <= n <= 64, Verify / ProofVerify only)
+ * sets the fixed-base comb terms per lane of this batch (otherwise the
+ * table's choice for its size class, else 4); the pipeline tuner stages its
+ * candidates this way. */
 #define CUBBS_HIP_BATCH_AGGREGATE 2u
 #define CUBBS_HIP_BATCH_AGG_GROUP(n) (((uint32_t)(n) & 15u) << 8)
+#define CUBBS_HIP_BATCH_COMB_CHUNK(n) (((uint32_t)(n) & 127u) << 12)
 void *cubbs_hip_batch_prepare_flags(uint32_t abi_version, int kind, const void *items,
                                     size_t item_count, uint32_t flags, int32_t *status_out);
 int32_t cubbs_hip_batch_launch(void *batch);
@@ -257,7 +263,12 @@ void cubbs_hip_server_destroy(void *server);
  * into report (NUL-terminated, truncated to report_cap). */
 int32_t cubbs_hip_autotune(uint32_t max_log2_terms, char *report, size_t report_cap);
 /* Installs a table in the report format (e.g. saved from an earlier run);
- * an empty string clears the table.  A malformed line fails the call. */
+ * an empty string clears the table.  A malformed line fails the call.  The
+ * table also takes whole-operation pipeline lines,
+ * "pipeline kind=verify|proof_verify log2_items=K chunk=N group=G", measured
+ * by the caller with sets of such batches in flight: batches of about 2^K
+ * items then plan their comb with N terms per lane and, when aggregated
+ * without an explicit group, groups of G (2 <= G <= 8). */
 int32_t cubbs_hip_tune_load(const char *table);
 
 /* Launch shapes forced on the calling thread for the calls it makes next

@@ -4,6 +4,7 @@ struct layouts (crates/cubbs-bbs/src/gpu_abi.rs mirrors them byte for byte),
 and rejects malformed calls with the reference's call-level codes before any
 device work (native/cuda/src/cubbs_bbs_cuda.cu:21-23,356-358,400-410)."""
 import ctypes
+import json
 import os
 import re
 import subprocess
@@ -154,3 +155,48 @@ def test_tune_table_load_parses_and_rejects():
     assert lib.cubbs_hip_tune_load(b"pippenger log2_terms=16 c=3 S=16\n") == 1   # c out of range
     assert lib.cubbs_hip_tune_load(b"window 5\n") == 1
     assert lib.cubbs_hip_tune_load(b"") == 0   # clears the table
+
+
+def test_pipeline_tuning_lines_parse_and_reject():
+    lib = cubbs_hip.lib()
+    ok = (b"pipeline kind=verify log2_items=13 chunk=4 group=8\n"
+          b"pipeline kind=proof_verify log2_items=11 chunk=8 group=4\n")
+    assert lib.cubbs_hip_tune_load(ok) == 0
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=sign log2_items=13 chunk=4 group=8\n") == 1
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=0 group=8\n") == 1
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=4 group=9\n") == 1
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=41 chunk=4 group=8\n") == 1
+    assert lib.cubbs_hip_tune_load(b"") == 0
+
+
+def test_committed_pipeline_table_loads():
+    # the table bench.py installs for the headline run (written by
+    # bench.py --tune-pipeline on the GPU)
+    path = os.path.join(ROOT, "cubbs-cred_amd", "pipeline_tuning.json")
+    if not os.path.exists(path):
+        pytest.skip("no pipeline tuning table committed")
+    with open(path) as f:
+        table = json.load(f)
+    lib = cubbs_hip.lib()
+    try:
+        assert lib.cubbs_hip_tune_load("\n".join(table["table"]).encode()) == 0
+    finally:
+        lib.cubbs_hip_tune_load(b"")
+    for c in table["classes"]:
+        best = c["best"]
+        assert best in c["candidates"]
+        assert best["ms_per_step"] == min(x["ms_per_step"] for x in c["candidates"])
+
+
+def test_prepare_flags_reject_bad_comb_chunk_before_device():
+    lib = cubbs_hip.lib()
+    st = ctypes.c_int32(0)
+    items = (cubbs_hip.VerifyItem * 1)()
+    for flags in ((65 << 12), (127 << 12)):
+        h = lib.cubbs_hip_batch_prepare_flags(2, cubbs_hip.BATCH_VERIFY, ctypes.cast(items, ctypes.c_void_p),
+                                              ctypes.c_size_t(1), flags, ctypes.byref(st))
+        assert not h and st.value == cubbs_hip.ERR_INVALID_INPUT
+    # comb chunk on an MSM kind
+    h = lib.cubbs_hip_batch_prepare_flags(2, cubbs_hip.BATCH_G1_MSM, ctypes.cast(items, ctypes.c_void_p),
+                                          ctypes.c_size_t(1), 4 << 12, ctypes.byref(st))
+    assert not h and st.value == cubbs_hip.ERR_INVALID_INPUT
