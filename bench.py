@@ -604,59 +604,64 @@ def run_in_flight(sets, steps, expect):
 PIPELINE_TUNING = os.path.join(ROOT, "cubbs-cred_amd", "pipeline_tuning.json")
 TUNE_CLASSES = ((2048, 512), (N_VERIFY, N_PROOFS))
 TUNE_GROUPS = (4, 8)
-TUNE_CHUNKS = (2, 4, 8)
-TUNE_INFLIGHT = (4, 6, 8, 12)
+TUNE_SUPER = (0, 4)
+TUNE_CHUNKS = (2, 4)
+TUNE_INFLIGHT = (6, 8, 10, 12)
+TUNE_STEPS = 20  # the headline's timed steps: pipeline fill and drain included
 
 
 def tune_pipeline(g, vreqs, expect_v, pvreqs, graph=True):
-    """Measures every (aggregation group, comb terms per lane, sets in
-    flight) candidate for each size class (Verify, ProofVerify items per
-    set, the benchmark's requests cut to that size) as the headline runs
-    them: single-stream sets replayed from graphs, several in flight, results
-    checked.  Returns the table: all candidates, the fastest per class, and
-    the library's pipeline lines for cubbs_hip_tune_load."""
+    """Measures every (aggregation group, groups per supergroup, comb terms
+    per lane, sets in flight) candidate for each size class (Verify,
+    ProofVerify items per set, the benchmark's requests cut to that size) as
+    the headline runs them: single-stream sets replayed from graphs, several
+    in flight, results checked.  Returns the table: all candidates, the
+    fastest per class, and the library's pipeline lines for
+    cubbs_hip_tune_load."""
     classes, lines = [], []
     for nv, npv in TUNE_CLASSES:
         vr, pr, ev = vreqs[:nv], pvreqs[:npv], expect_v[:nv]
         cands = []
-        for grp in TUNE_GROUPS:
-            for ch in TUNE_CHUNKS:
-                sets = [(g.StagedBatch(g.BATCH_VERIFY, vr, single_stream=True, aggregate=grp, comb_chunk=ch),
-                         g.StagedBatch(g.BATCH_PROOF_VERIFY, pr, single_stream=True, aggregate=grp,
-                                       comb_chunk=ch))
-                        for _ in range(max(TUNE_INFLIGHT))]
-                if graph:
-                    for v, p in sets:
-                        v.use_graph(True)
-                        p.use_graph(True)
-                expect = []
+        for grp, sup, ch in [(a, b, c) for a in TUNE_GROUPS for b in TUNE_SUPER for c in TUNE_CHUNKS]:
+            sets = [(g.StagedBatch(g.BATCH_VERIFY, vr, single_stream=True, aggregate=grp, comb_chunk=ch,
+                                   supergroup=sup),
+                     g.StagedBatch(g.BATCH_PROOF_VERIFY, pr, single_stream=True, aggregate=grp,
+                                   comb_chunk=ch, supergroup=sup))
+                    for _ in range(max(TUNE_INFLIGHT))]
+            if graph:
                 for v, p in sets:
-                    v.launch()
-                    p.launch()
-                for v, p in sets:
-                    v.sync()
-                    p.sync()
-                    if [x for _, x in v.results()] != ev or not all(c == 0 and x for c, x in p.results()):
-                        raise SystemExit("tuning candidate results differ from the expected ones")
-                    expect.append((v.results_raw(), p.results_raw()))
-                for n in TUNE_INFLIGHT:
-                    steps = max(24, 3 * n)
-                    run_in_flight(sets[:n], n, expect)  # warm-up
-                    ms = min(run_in_flight(sets[:n], steps, expect) for _ in range(2)) * 1e3 / steps
-                    cands.append({"agg_group": grp, "comb_chunk": ch, "sets_in_flight": n,
-                                  "ms_per_step": round(ms, 4)})
-                    log("tune %d+%d group=%d chunk=%d inflight=%d: %.3f ms/step" % (nv, npv, grp, ch, n, ms))
-                for v, p in sets:
-                    v.free()
-                    p.free()
+                    v.use_graph(True)
+                    p.use_graph(True)
+            expect = []
+            for v, p in sets:
+                v.launch()
+                p.launch()
+            for v, p in sets:
+                v.sync()
+                p.sync()
+                if [x for _, x in v.results()] != ev or not all(c == 0 and x for c, x in p.results()):
+                    raise SystemExit("tuning candidate results differ from the expected ones")
+                expect.append((v.results_raw(), p.results_raw()))
+            for n in TUNE_INFLIGHT:
+                run_in_flight(sets[:n], n, expect)  # warm-up
+                ms = min(run_in_flight(sets[:n], TUNE_STEPS, expect) for _ in range(3)) * 1e3 / TUNE_STEPS
+                cands.append({"agg_group": grp, "agg_super": sup, "comb_chunk": ch, "sets_in_flight": n,
+                              "ms_per_step": round(ms, 4)})
+                log("tune %d+%d group=%d super=%d chunk=%d inflight=%d: %.3f ms/step"
+                    % (nv, npv, grp, sup, ch, n, ms))
+            for v, p in sets:
+                v.free()
+                p.free()
         best = min(cands, key=lambda c: c["ms_per_step"])
         classes.append({"verify_items": nv, "proof_verify_items": npv, "best": best, "candidates": cands})
         for kind, items in (("verify", nv), ("proof_verify", npv)):
-            lines.append("pipeline kind=%s log2_items=%d chunk=%d group=%d" %
-                         (kind, round(math.log2(items)), best["comb_chunk"], best["agg_group"]))
+            lines.append("pipeline kind=%s log2_items=%d chunk=%d group=%d super=%d" %
+                         (kind, round(math.log2(items)), best["comb_chunk"], best["agg_group"],
+                          best["agg_super"]))
     return {"description": "Measured Verify/ProofVerify pipeline shapes per size class (items per set): "
-                           "aggregation group size, fixed-base comb terms per lane, staged sets in flight; "
-                           "single-stream sets replayed from HIP graphs, results checked",
+                           "aggregation group size, groups per supergroup check (0: none), fixed-base comb "
+                           "terms per lane, staged sets in flight; single-stream sets replayed from HIP "
+                           "graphs, results checked",
             "command": "python bench.py --tune-pipeline cubbs-cred_amd/pipeline_tuning.json",
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
             "classes": classes, "table": lines}
@@ -885,9 +890,11 @@ def main():
                       "pipeline_tuning": (dict(tuned, source=os.path.relpath(PIPELINE_TUNING, ROOT))
                                           if tuned else None),
                       "streams_per_batch": 1 if single else 2,
-                      "pairing_checks": ("per-key groups of %d as random linear combinations, "
+                      "pairing_checks": ("per-key groups of %d as random linear combinations%s, "
                                          "per-item fallback for failing groups"
-                                         % (tuned["agg_group"] if tuned else 8)) if agg else "per item",
+                                         % (tuned["agg_group"] if tuned else 8,
+                                            (", checked %d groups at a time first" % tuned["agg_super"])
+                                            if tuned and tuned.get("agg_super") else "")) if agg else "per item",
                       "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                       "msm_terms": vb.shape()["msm_terms"] + pb.shape()["msm_terms"]},
            "roofline": roofline,

@@ -500,26 +500,35 @@ void* cubbs_hip_batch_prepare_flags(uint32_t abi_version, int kind, const void* 
   int32_t& st = status_out ? *status_out : dummy;
   uint32_t group = (flags >> 8) & 15u;
   const uint32_t chunk = (flags >> 12) & 127u;
+  uint32_t super = (flags >> 20) & 15u;
+  const bool agg = (flags & CUBBS_HIP_BATCH_AGGREGATE) != 0;
   if ((flags & ~(uint32_t)(CUBBS_HIP_BATCH_SINGLE_STREAM | CUBBS_HIP_BATCH_AGGREGATE |
-                           CUBBS_HIP_BATCH_AGG_GROUP(15) | CUBBS_HIP_BATCH_COMB_CHUNK(127))) ||
-      (group && (group < 2 || group > 8 || !(flags & CUBBS_HIP_BATCH_AGGREGATE))) || chunk > 64 ||
-      (chunk && kind != CUBBS_HIP_BATCH_VERIFY && kind != CUBBS_HIP_BATCH_PROOF_VERIFY)) {
+                           CUBBS_HIP_BATCH_AGG_GROUP(15) | CUBBS_HIP_BATCH_COMB_CHUNK(127) |
+                           CUBBS_HIP_BATCH_AGG_SUPER(15))) ||
+      (group && (group < 2 || group > 8 || !agg)) || chunk > 64 ||
+      (chunk && kind != CUBBS_HIP_BATCH_VERIFY && kind != CUBBS_HIP_BATCH_PROOF_VERIFY) ||
+      (super && (super < 2 || super > 8 || !agg))) {
     st = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
     return nullptr;
   }
-  if ((flags & CUBBS_HIP_BATCH_AGGREGATE) && !group) {
-    // the tuned group size of the batch's size class, else 8
-    uint32_t c = 0, gsz = 0;
+  if (agg) {
+    // the tuned shape of the batch's size class where the flags leave it
+    // open: groups of 8 and no supergroups without a table entry
+    uint32_t c = 0, gsz = 0, sup = 0;
     const int kd = kind == CUBBS_HIP_BATCH_VERIFY ? 0 : kind == CUBBS_HIP_BATCH_PROOF_VERIFY ? 1 : -1;
-    group = tuned_pipeline(kd, item_count, c, gsz) && gsz ? gsz : 8u;
+    const bool tuned = tuned_pipeline(kd, item_count, c, gsz, sup);
+    if (!group) group = tuned && gsz ? gsz : 8u;
+    if (!super && !(flags & CUBBS_HIP_BATCH_AGG_GROUP(15))) super = tuned ? sup : 0u;
   }
   const TuneForce saved = tune_force();
   if (chunk) tune_force().comb_chunk = chunk;
   set_thread_single_stream((flags & CUBBS_HIP_BATCH_SINGLE_STREAM) != 0);
-  set_thread_agg_group((flags & CUBBS_HIP_BATCH_AGGREGATE) ? group : 0u);
+  set_thread_agg_group(agg ? group : 0u);
+  set_thread_agg_super(agg ? super : 0u);
   void* b = cubbs_hip_batch_prepare(abi_version, kind, items, item_count, status_out);
   set_thread_single_stream(false);
   set_thread_agg_group(0);
+  set_thread_agg_super(0);
   tune_force() = saved;
   return b;
 }

@@ -45,8 +45,8 @@ uint32_t comb_chunk_terms(uint64_t batch_terms, bool tuned) {
 static uint32_t pipeline_comb_chunk(int kind, uint64_t items) {
   if (tune_force().comb_chunk) return tune_force().comb_chunk;
   if (knobs().comb_chunk) return knobs().comb_chunk;
-  uint32_t chunk = 0, group = 0;
-  return tuned_pipeline(kind, items, chunk, group) && chunk ? chunk : 4;
+  uint32_t chunk = 0, group = 0, super = 0;
+  return tuned_pipeline(kind, items, chunk, group, super) && chunk ? chunk : 4;
 }
 
 // ------------------------------------------------------------------ arenas
@@ -250,7 +250,8 @@ void materialize(Batch& b, Plan& p) {
   b.pt_copy = reinterpret_cast<PointCopy*>(D + o_pcp);
   b.n_pt_copy = (uint32_t)p.pt_copy.size();
   b.n_agg_slots = p.n_agg_slots;
-  b.n_gpairs = p.n_group_pairs;
+  b.n_spairs = p.n_super_pairs;
+  b.n_gpairs = p.n_group_pairs - p.n_super_pairs;
   for (size_t l = 0; l < o_lv.size(); l++)
     b.levels.push_back(Batch::Level{reinterpret_cast<ReduceGroup*>(D + o_lv[l].first),
                                     reinterpret_cast<uint32_t*>(D + o_lv[l].second),
@@ -283,9 +284,9 @@ void materialize(Batch& b, Plan& p) {
   }
   count_work(b, p);
   if (p.n_agg_slots) {
-    b.pair_group.resize(p.pairs.size() - p.n_group_pairs);
-    for (size_t i = p.n_group_pairs; i < p.pairs.size(); i++)
-      b.pair_group[i - p.n_group_pairs] =
+    b.pair_group.resize(p.pairs.size() - p.n_super_pairs);
+    for (size_t i = p.n_super_pairs; i < p.pairs.size(); i++)
+      b.pair_group[i - p.n_super_pairs] =
           (p.pairs[i].flags & PAIR_GROUPED) ? p.pairs[i].group : PRIM_NO_SLOT;
   }
   // work area
@@ -383,9 +384,11 @@ void materialize(Batch& b, Plan& p) {
 // launch costs about one check's latency however few checks it holds, so the
 // extra combined-check launch (and the re-check launch when a group fails)
 // outweighs the pairings saved unless several batches are in flight.
-static thread_local uint32_t tl_agg_group = 0;
+static thread_local uint32_t tl_agg_group = 0, tl_agg_super = 0;
 void set_thread_agg_group(uint32_t n) { tl_agg_group = n; }
+void set_thread_agg_super(uint32_t n) { tl_agg_super = n; }
 uint32_t agg_group_max() { return tl_agg_group; }
+uint32_t agg_super_max() { return tl_agg_super; }
 
 static void agg_coefficient(uint8_t out[32], uint64_t s, uint64_t t) {
   // s + t * lambda, lambda = GLV_LAMBDA (127.4 bits): below 2^192 < r
@@ -398,8 +401,9 @@ static void agg_coefficient(uint8_t out[32], uint64_t s, uint64_t t) {
     for (int j = 0; j < 8; j++) out[31 - 8 * k - j] = (uint8_t)(w[k] >> (8 * j));
 }
 
-void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, bool scaled) {
+void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, bool scaled, uint32_t super_max) {
   if (group_max < 2 || group_max > 8) return;  // 2 x 8 partials per sum: one final reduction
+  if (super_max < 2 || super_max > 8) super_max = 1;
   std::vector<PairJob> in = std::move(p.pairs);
   p.pairs.clear();
   std::vector<uint32_t> order(in.size());
@@ -412,7 +416,77 @@ void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, bool scaled) {
   });
   std::vector<uint8_t> rnd(scaled ? 0 : 16 * in.size());
   if (!scaled) fill_os_random(rnd.data(), rnd.size());
-  std::vector<PairJob> groups, grouped, single;
+  std::vector<PairJob> supers, groups, grouped, single;
+  std::vector<ReduceGroup> super_red;
+  std::vector<uint32_t> super_out;
+  // the groups of one key: runs of at most group_max checks (a run of one
+  // stays a single check), taken super_max at a time
+  std::vector<std::pair<size_t, size_t>> run;  // [begin, end) in order[]
+  auto flush = [&]() {
+    for (size_t c0 = 0; c0 < run.size(); c0 += super_max) {
+      const size_t c1 = std::min(run.size(), c0 + super_max);
+      const bool sup = c1 - c0 >= 2;
+      const uint32_t total = (uint32_t)(run[c1 - 1].second - run[c0].first);
+      const size_t first = run[c0].first;
+      // the partials of the chunk's a points, then of its b points, contiguous
+      uint32_t fa, fb;
+      if (scaled) {
+        // the checks' points are c_i a_i and c_i b_i already
+        fa = p.partials(total);
+        fb = p.partials(total);
+        for (uint32_t k = 0; k < total; k++) {
+          p.pt_copy.push_back(PointCopy{in[order[first + k]].a, fa + k});
+          p.pt_copy.push_back(PointCopy{in[order[first + k]].b, fb + k});
+        }
+      } else {
+        std::vector<uint32_t> sc(total);
+        for (uint32_t k = 0; k < total; k++) {
+          uint64_t s0, t0;
+          std::memcpy(&s0, &rnd[16 * order[first + k]], 8);
+          std::memcpy(&t0, &rnd[16 * order[first + k] + 8], 8);
+          uint8_t c[32];
+          agg_coefficient(c, s0 | 1, t0);
+          sc[k] = p.scalars(1);
+          p.dec_fr.push_back(DecodeFrJob{p.bytes(c, 32), sc[k], nv + p.n_agg_slots, 0u, 0u});
+        }
+        fa = p.n_partials;
+        for (uint32_t k = 0; k < total; k++)
+          p.agg_var.push_back(VarJob{in[order[first + k]].a, sc[k], p.partials(VAR_PARTIALS)});
+        fb = p.n_partials;
+        for (uint32_t k = 0; k < total; k++)
+          p.agg_var.push_back(VarJob{in[order[first + k]].b, sc[k], p.partials(VAR_PARTIALS)});
+      }
+      const uint32_t pk = in[order[first]].pk_slot;
+      uint32_t sslot = 0;
+      if (sup) {
+        sslot = nv + p.n_agg_slots++;
+        const uint32_t la = p.points(1), lb = p.points(1);
+        super_red.push_back(ReduceGroup{fa, VAR_PARTIALS * total});
+        super_out.push_back(la);
+        super_red.push_back(ReduceGroup{fb, VAR_PARTIALS * total});
+        super_out.push_back(lb);
+        supers.push_back(PairJob{sslot, la, lb, pk, PAIR_NEG_G2, 0, 0});
+      }
+      for (size_t r = c0; r < c1; r++) {
+        const uint32_t gslot = nv + p.n_agg_slots++;
+        const uint32_t off = (uint32_t)(run[r].first - first);
+        const uint32_t cnt = (uint32_t)(run[r].second - run[r].first);
+        const uint32_t la = p.points(1), lb = p.points(1);
+        p.groups2.push_back(ReduceGroup{fa + VAR_PARTIALS * off, VAR_PARTIALS * cnt});
+        p.out2.push_back(la);
+        p.groups2.push_back(ReduceGroup{fb + VAR_PARTIALS * off, VAR_PARTIALS * cnt});
+        p.out2.push_back(lb);
+        groups.push_back(PairJob{gslot, la, lb, pk, PAIR_NEG_G2, sup ? PAIR_GROUPED : 0u, sslot});
+        for (size_t k = run[r].first; k < run[r].second; k++) {
+          PairJob g = in[order[k]];
+          g.flags |= PAIR_GROUPED;
+          g.group = gslot;
+          grouped.push_back(g);
+        }
+      }
+    }
+    run.clear();
+  };
   size_t i = 0;
   while (i < order.size()) {
     const PairJob& j0 = in[order[i]];
@@ -421,56 +495,22 @@ void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, bool scaled) {
       while (e < order.size() && e - i < group_max && plain(in[order[e]]) &&
              in[order[e]].pk_slot == j0.pk_slot)
         e++;
+    if (!run.empty() && (!plain(j0) || in[order[run[0].first]].pk_slot != j0.pk_slot)) flush();
     if (!plain(j0) || e - i < 2) {
       for (size_t k = i; k < e; k++) single.push_back(in[order[k]]);
-      i = e;
-      continue;
-    }
-    const uint32_t gslot = nv + p.n_agg_slots++;
-    const uint32_t cnt = (uint32_t)(e - i);
-    uint32_t fa, fb;
-    if (scaled) {
-      // the checks' points are c_i a_i and c_i b_i already
-      fa = p.partials(cnt);
-      fb = p.partials(cnt);
-      for (uint32_t k = 0; k < cnt; k++) {
-        p.pt_copy.push_back(PointCopy{in[order[i + k]].a, fa + k});
-        p.pt_copy.push_back(PointCopy{in[order[i + k]].b, fb + k});
-      }
     } else {
-      std::vector<uint32_t> sc(cnt);
-      for (uint32_t k = 0; k < cnt; k++) {
-        uint64_t s, t;
-        std::memcpy(&s, &rnd[16 * order[i + k]], 8);
-        std::memcpy(&t, &rnd[16 * order[i + k] + 8], 8);
-        uint8_t c[32];
-        agg_coefficient(c, s | 1, t);
-        sc[k] = p.scalars(1);
-        p.dec_fr.push_back(DecodeFrJob{p.bytes(c, 32), sc[k], gslot, 0u, 0u});
-      }
-      fa = p.n_partials;
-      for (uint32_t k = 0; k < cnt; k++)
-        p.agg_var.push_back(VarJob{in[order[i + k]].a, sc[k], p.partials(VAR_PARTIALS)});
-      fb = p.n_partials;
-      for (uint32_t k = 0; k < cnt; k++)
-        p.agg_var.push_back(VarJob{in[order[i + k]].b, sc[k], p.partials(VAR_PARTIALS)});
-    }
-    const uint32_t la = p.points(1), lb = p.points(1);
-    p.groups2.push_back(ReduceGroup{fa, VAR_PARTIALS * cnt});
-    p.out2.push_back(la);
-    p.groups2.push_back(ReduceGroup{fb, VAR_PARTIALS * cnt});
-    p.out2.push_back(lb);
-    groups.push_back(PairJob{gslot, la, lb, j0.pk_slot, PAIR_NEG_G2, 0, 0});
-    for (size_t k = i; k < e; k++) {
-      PairJob g = in[order[k]];
-      g.flags |= PAIR_GROUPED;
-      g.group = gslot;
-      grouped.push_back(g);
+      run.push_back({i, e});
     }
     i = e;
   }
-  p.n_group_pairs = (uint32_t)groups.size();
-  p.pairs = std::move(groups);
+  flush();
+  // the supergroup sums after the group sums (the longer reductions share waves)
+  p.groups2.insert(p.groups2.end(), super_red.begin(), super_red.end());
+  p.out2.insert(p.out2.end(), super_out.begin(), super_out.end());
+  p.n_super_pairs = (uint32_t)supers.size();
+  p.n_group_pairs = (uint32_t)(supers.size() + groups.size());
+  p.pairs = std::move(supers);
+  p.pairs.insert(p.pairs.end(), groups.begin(), groups.end());
   p.pairs.insert(p.pairs.end(), grouped.begin(), grouped.end());
   p.pairs.insert(p.pairs.end(), single.begin(), single.end());
 }
@@ -635,7 +675,7 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
     p.pairs.push_back(PairJob{k, pair_a, pb, pk_slot[k], PAIR_NEG_G2, 0});
     term += L + 2;
   }
-  aggregate_pairs(p, b->nv, group, scaled);
+  aggregate_pairs(p, b->nv, group, scaled, agg_super_max());
   materialize(*b, p);
   return b;
 }
@@ -766,7 +806,7 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
     p.pairs.push_back(PairJob{k, pab, pbb, pk_slot[k], PAIR_NEG_G2, 0});
     term += L + 2;
   }
-  aggregate_pairs(p, b->nv, agg_group_max());
+  aggregate_pairs(p, b->nv, agg_group_max(), false, agg_super_max());
   materialize(*b, p);
   return b;
 }
@@ -1201,31 +1241,37 @@ void launch_batch(Batch& b) {
     launch_reduce(s, b.groups2, b.out2, b.n_groups2, b.partials, b.points, b.pinf);
   }
   if (b.n_pairs) {
-    // the group checks first: their results gate the per-item checks, whose
-    // line blocks are then formed only for items of groups that failed
-    if (b.n_gpairs) {
+    // Combined checks first, each level's results gating the next: the
+    // supergroup checks, then the group checks (line blocks formed only for
+    // groups of supergroups that failed), then the per-item checks (only for
+    // items of groups that failed).
+    uint32_t done = 0;
+    const struct {
+      uint32_t n;
+      const char* name;
+    } level[2] = {{b.n_spairs, "pairing_super"}, {b.n_gpairs, "pairing_groups"}};
+    for (const auto& lv : level) {
+      if (!lv.n) continue;
       {
         Timed t(b, "pair_lines");
-        launch_pair_lines(s, b.pairs, b.n_gpairs, b.points, b.pinf, b.pk_status, b.pk_lines,
-                          b.neg_g2_lines, b.l2buf);
+        launch_pair_lines(s, b.pairs + done, lv.n, b.points, b.pinf, b.pk_status, b.pk_lines,
+                          b.neg_g2_lines, b.l2buf + (size_t)done * PAIR_L2_WORDS,
+                          done ? b.valid : nullptr);
       }
-      {
-        Timed t(b, "pairing_groups");
-        launch_pairing(s, b.pairs, b.n_gpairs, b.points, b.pinf, b.pk_status, b.l2buf, b.status,
-                       b.gate, b.valid);
-      }
+      Timed t(b, lv.name);
+      launch_pairing(s, b.pairs + done, lv.n, b.points, b.pinf, b.pk_status,
+                     b.l2buf + (size_t)done * PAIR_L2_WORDS, b.status, b.gate, b.valid);
+      done += lv.n;
+    }
+    {
       Timed t(b, "pair_lines");
-      launch_pair_lines(s, b.pairs + b.n_gpairs, b.n_pairs - b.n_gpairs, b.points, b.pinf,
-                        b.pk_status, b.pk_lines, b.neg_g2_lines,
-                        b.l2buf + (size_t)b.n_gpairs * PAIR_L2_WORDS, b.valid);
-    } else {
-      Timed t(b, "pair_lines");
-      launch_pair_lines(s, b.pairs, b.n_pairs, b.points, b.pinf, b.pk_status, b.pk_lines,
-                        b.neg_g2_lines, b.l2buf);
+      launch_pair_lines(s, b.pairs + done, b.n_pairs - done, b.points, b.pinf, b.pk_status,
+                        b.pk_lines, b.neg_g2_lines, b.l2buf + (size_t)done * PAIR_L2_WORDS,
+                        done ? b.valid : nullptr);
     }
     Timed t(b, "pairing");
-    launch_pairing(s, b.pairs + b.n_gpairs, b.n_pairs - b.n_gpairs, b.points, b.pinf, b.pk_status,
-                   b.l2buf + (size_t)b.n_gpairs * PAIR_L2_WORDS, b.status, b.gate, b.valid);
+    launch_pairing(s, b.pairs + done, b.n_pairs - done, b.points, b.pinf, b.pk_status,
+                   b.l2buf + (size_t)done * PAIR_L2_WORDS, b.status, b.gate, b.valid);
   }
   check(hipGetLastError());
 }
@@ -1241,10 +1287,11 @@ void fetch_results(Batch& b) {
   b.agg_passed = 0;
   for (uint32_t g = 0; g < b.n_agg_slots; g++) b.agg_passed += b.h_res[b.res_stride + b.nv + g] == 1;
   if (b.n_agg_slots) {
-    // per-item checks computed: those whose group check did not hold
-    uint64_t run = b.n_gpairs;
-    const uint32_t n_item = b.n_pairs - b.n_gpairs;
-    for (uint32_t i = 0; i < n_item; i++) {
+    // checks computed: the supergroup checks, and every later check whose
+    // enclosing check did not hold
+    uint64_t run = b.n_spairs;
+    const uint32_t n_rest = b.n_pairs - b.n_spairs;
+    for (uint32_t i = 0; i < n_rest; i++) {
       const uint32_t g = b.pair_group[i];
       run += g == PRIM_NO_SLOT || b.h_res[b.res_stride + g] != 1;
     }

@@ -70,6 +70,7 @@ void set_thread_single_stream(bool on);
 // Group size of the aggregated pairing checks of the Verify / ProofVerify
 // batches this thread prepares next (0 = per-item checks only).
 void set_thread_agg_group(uint32_t n);
+void set_thread_agg_super(uint32_t n);
 
 struct Batch {
   int kind = 0;
@@ -112,9 +113,11 @@ struct Batch {
   uint32_t* out2 = nullptr;
   PairJob* pairs = nullptr;
   PointCopy* pt_copy = nullptr;       // affine inputs of the aggregated group sums
-  uint32_t n_pt_copy = 0, n_agg_slots = 0, n_gpairs = 0;
+  // pairs[]: n_spairs supergroup checks, then n_gpairs group checks, then
+  // the per-item checks
+  uint32_t n_pt_copy = 0, n_agg_slots = 0, n_gpairs = 0, n_spairs = 0;
   uint32_t agg_passed = 0;  // group checks that held at the last fetch of results
-  std::vector<uint32_t> pair_group;   // per-item checks: status slot of the group (or PRIM_NO_SLOT)
+  std::vector<uint32_t> pair_group;   // checks after the supergroups': slot of the enclosing check (or PRIM_NO_SLOT)
   struct Level {
     ReduceGroup* groups;
     uint32_t* out;

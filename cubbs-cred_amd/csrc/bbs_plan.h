@@ -42,10 +42,11 @@ struct Plan {
   std::vector<uint32_t> disc_index, disc_scalar, und_scalar;
   // aggregated pairing checks: products c_i a_i, c_i b_i of the grouped
   // checks (summed per group by groups2 / out2), extra status slots for the
-  // groups, and how many of pairs[] (at its front) are group checks
+  // groups, and how many of pairs[] (at its front) are combined checks: the
+  // supergroup checks first (n_super_pairs), then the group checks
   std::vector<VarJob> agg_var;       // run in the same launch as `var`, after it
   std::vector<PointCopy> pt_copy;    // affine inputs of the group sums (pre-scaled checks)
-  uint32_t n_agg_slots = 0, n_group_pairs = 0;
+  uint32_t n_agg_slots = 0, n_group_pairs = 0, n_super_pairs = 0;
   std::vector<PipTerm> pip_terms;  // terms of the Pippenger MSMs
   std::vector<uint32_t> pip_out;   // per Pippenger MSM: its output partial slot
   std::vector<uint8_t> blob;       // signatures, proofs, headers, messages, keys, seeds
@@ -172,8 +173,13 @@ void materialize(Batch& b, Plan& p);
 // scaled: the checks' points are already multiplied by their coefficients
 // (the planner drew them), so the group sums add the points themselves;
 // otherwise the products c_i a_i, c_i b_i are planned here.
-void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, bool scaled = false);
+// super_max >= 2: consecutive groups of one key, up to super_max of them,
+// are also checked together first (a supergroup), and their groups only when
+// that check fails.
+void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, bool scaled = false,
+                     uint32_t super_max = 0);
 uint32_t agg_group_max();
+uint32_t agg_super_max();
 // Lock the context, make generators [0, max_gen_slots) and the given public
 // keys resident, snapshot the resident pointers into the batch.
 void attach(Batch& b, uint32_t max_gen_slots, const std::vector<const uint8_t*>& pks,

@@ -39,6 +39,7 @@ struct Table {
   // whole-operation pipelines, [kind][log2 items]: comb chunk, group size
   uint8_t pipe_chunk[2][CLASSES] = {};
   uint8_t pipe_group[2][CLASSES] = {};
+  uint8_t pipe_super[2][CLASSES] = {};
 };
 Table& table() {
   static Table t;
@@ -126,7 +127,8 @@ uint32_t tuned_comb_chunk(uint64_t terms) {
   return k < 0 ? 0 : t.comb[k];
 }
 
-bool tuned_pipeline(int kind, uint64_t items, uint32_t& comb_chunk, uint32_t& agg_group) {
+bool tuned_pipeline(int kind, uint64_t items, uint32_t& comb_chunk, uint32_t& agg_group,
+                    uint32_t& agg_super) {
   if (kind < 0 || kind > 1) return false;
   Table& t = table();
   std::lock_guard<std::mutex> g(t.mu);
@@ -134,6 +136,7 @@ bool tuned_pipeline(int kind, uint64_t items, uint32_t& comb_chunk, uint32_t& ag
   if (k < 0) return false;
   comb_chunk = t.pipe_chunk[kind][k];
   agg_group = t.pipe_group[kind][k];
+  agg_super = t.pipe_super[kind][k];
   return true;
 }
 
@@ -262,6 +265,7 @@ int32_t cubbs_hip_tune_load(const char* text) {
     std::memset(t.comb, 0, sizeof t.comb);
     std::memset(t.pipe_chunk, 0, sizeof t.pipe_chunk);
     std::memset(t.pipe_group, 0, sizeof t.pipe_group);
+    std::memset(t.pipe_super, 0, sizeof t.pipe_super);
     return CUBBS_BBS_GPU_SUCCESS;
   }
   const char* p = text;
@@ -279,10 +283,16 @@ int32_t cubbs_hip_tune_load(const char* text) {
       t.comb[k] = (uint8_t)a;
     } else if (std::sscanf(ln.c_str(), "pipeline kind=%15s log2_items=%u chunk=%u group=%u", kind, &k, &a,
                            &b) == 4) {
+      // optional: " super=M" (groups per supergroup, 0 or 2..8)
+      unsigned sup = 0;
+      const char* sp = std::strstr(ln.c_str(), " super=");
+      if (sp && std::sscanf(sp, " super=%u", &sup) != 1) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
       const int kd = std::strcmp(kind, "verify") == 0 ? 0 : std::strcmp(kind, "proof_verify") == 0 ? 1 : -1;
-      if (kd < 0 || k >= CLASSES || a < 1 || a > 64 || b < 2 || b > 8) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      if (kd < 0 || k >= CLASSES || a < 1 || a > 64 || b < 2 || b > 8 || sup == 1 || sup > 8)
+        return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
       t.pipe_chunk[kd][k] = (uint8_t)a;
       t.pipe_group[kd][k] = (uint8_t)b;
+      t.pipe_super[kd][k] = (uint8_t)sup;
     } else if (!ln.empty()) {
       return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
     }

@@ -373,16 +373,6 @@ DEV void cyc_sqr(Fp& f, const Lane& L) {
   sel(f, b2 < 0, d, e);
 }
 
-// The out-of-line routines below copy their reference operands into locals
-// on entry and write the result back once: working on the references would
-// keep the operand in the caller's stack frame, stored and reloaded through
-// flat memory around every LDS access of the loops.
-DEVN void mul(Fp& f, const Fp& g, Lane L) {
-  Fp a = f;
-  const Fp b = g;
-  mul_in(a, b, L);
-  f = a;
-}
 
 // f <- conj(f) (x -> x^(p^6)): odd slots negated
 DEV void conj(Fp& f, const Lane& L) {
@@ -391,7 +381,7 @@ DEV void conj(Fp& f, const Lane& L) {
   sel(f, (L.k & 1) != 0, n, f);
 }
 // x -> x^p: slot k conjugated, times gamma1[k] = xi^(k (p-1)/6)
-DEVN void frob(Fp& f, Lane L) {
+DEV void frob(Fp& f, const Lane& L) {
   const Fp v = f;
   Fp o, x0, x1, g0, g1, z, n0, y0, y1;
   partner(o, v);
@@ -413,7 +403,7 @@ DEVN void frob(Fp& f, Lane L) {
   f = r;
 }
 // x -> x^(p^2): slot k times gamma2[k] in Fp
-DEVN void frob2(Fp& f, Lane L) {
+DEV void frob2(Fp& f, const Lane& L) {
   Fp g, v = f;
   fp_load_const(g, bls::FROB2 + FPW * L.k);
   fp_mul(v, v, g);
@@ -428,7 +418,29 @@ DEVN void frob2(Fp& f, Lane L) {
 // d = n0 t0 + xi (n2 t1 + n1 t2), d^-1 = conj(d) / (d0^2 + d1^2) (lane (0, 0),
 // one Fp inversion).  Two 12-lane products and a few Fp2 operations, with
 // values exchanged through the check's LDS area.
-DEVN void inv(Fp& f, Lane L) {
+// In-line Fp2 products for the inversion below (the out-of-line forms would
+// make the caller save its live registers around each call).
+DEV void f2_mul(Fp2& r, const Fp2& a, const Fp2& b) {
+  Fp v0, v1, s0, s1;
+  fp_mul(v0, a.c0, b.c0);
+  fp_mul(v1, a.c1, b.c1);
+  fp_add(s0, a.c0, a.c1);
+  fp_add(s1, b.c0, b.c1);
+  fp_mul(s0, s0, s1);
+  fp_sub(r.c0, v0, v1);
+  fp_sub(s0, s0, v0);
+  fp_sub(r.c1, s0, v1);
+}
+DEV void f2_sqr(Fp2& r, const Fp2& a) {
+  Fp sm, df, m;
+  fp_add(sm, a.c0, a.c1);
+  fp_sub(df, a.c0, a.c1);
+  fp_mul(m, a.c0, a.c1);
+  fp_mul(r.c0, sm, df);
+  fp_dbl(r.c1, m);
+}
+
+DEV void inv(Fp& f, const Lane& L) {
   Fp g = f;
   conj(g, L);
   Fp n = f;
@@ -453,11 +465,11 @@ DEVN void inv(Fp& f, Lane L) {
     const int pa = L.k == 0 ? 2 : 0, pb = L.k == 1 ? 2 : 4;
     Fp2 a, b, x, y;
     ld2(a, X + 2 * sq_slot * LE);
-    fp2_sqr(x, a);
+    f2_sqr(x, a);
     if (L.k == 1) fp2_mul_xi(x, x);
     ld2(a, X + 2 * pa * LE);
     ld2(b, X + 2 * pb * LE);
-    fp2_mul(y, a, b);
+    f2_mul(y, a, b);
     if (L.k == 0) fp2_mul_xi(y, y);
     fp2_sub(x, x, y);
     st2(T + 2 * L.k * LE, x);
@@ -467,15 +479,15 @@ DEVN void inv(Fp& f, Lane L) {
     Fp2 a, b, d, s;
     ld2(a, X + 2 * 2 * LE);  // n1
     ld2(b, T + 2 * 2 * LE);  // t2
-    fp2_mul(s, a, b);
+    f2_mul(s, a, b);
     ld2(a, X + 2 * 4 * LE);  // n2
     ld2(b, T + 2 * 1 * LE);  // t1
-    fp2_mul(d, a, b);
+    f2_mul(d, a, b);
     fp2_add(s, s, d);
     fp2_mul_xi(s, s);
     ld2(a, X);              // n0
     ld2(b, T);              // t0
-    fp2_mul(d, a, b);
+    f2_mul(d, a, b);
     fp2_add(d, d, s);
     Fp nrm, t;
     fp_sqr(nrm, d.c0);
@@ -510,56 +522,106 @@ DEVN void inv(Fp& f, Lane L) {
   f = g;
 }
 
-// a^x for a cyclotomic element, x = -|x|
-DEVN void exp_by_x(Fp& r, const Fp& a, Lane L) {
-  const Fp base = a;
-  Fp acc = base;
+// f <- f^x for a cyclotomic element, x = -|x|
+DEV void exp_by_x(Fp& f, const Lane& L) {
+  const Fp base = f;
 #pragma unroll 1
   for (int i = 62; i >= 0; i--) {
-    cyc_sqr(acc, L);
-    if ((BLS_X_ABS >> i) & 1) mul_in(acc, base, L);
+    cyc_sqr(f, L);
+    if ((BLS_X_ABS >> i) & 1) mul_in(f, base, L);
   }
-  conj(acc, L);
-  r = acc;
+  conj(f, L);
 }
 
-// Same exponent chain as final_exponentiation() in pairing.hpp.
+// Same exponent chain as final_exponentiation() in pairing.hpp, written as a
+// schedule over five working values that one loop runs: its body holds one
+// in-line copy of each operation.  (As straight-line code the chain made 17
+// out-of-line calls, each saving and restoring the callee-saved registers
+// through scratch memory: most of the kernel's HBM writes.)  Step: R[d] =
+// R[s1] * U(R[s2]) (FE_MUL), U(R[s2]) (FE_COPY), R[s1]^-1 (FE_INV) or
+// R[s1]^x (FE_EXPX), with U one of id, conj, frob, frob2, cyclotomic square.
+// The step index is the same on every lane, so the branches are uniform.
+enum { FE_MUL, FE_COPY, FE_INV, FE_EXPX };
+enum { FU_ID, FU_CONJ, FU_FROB, FU_FROB2, FU_CYC };
+struct FeStep {
+  uint8_t op, d, s1, s2, u;
+};
+// R0 = f (later c), R1 = f1, R2 = t, R3 = a, R4 = b
+__constant__ static const FeStep FE_STEPS[] = {
+    {FE_INV, 2, 0, 0, FU_ID},      // t = f^-1
+    {FE_COPY, 1, 0, 0, FU_CONJ},   // f1 = conj(f)
+    {FE_MUL, 1, 1, 2, FU_ID},      // f1 = f^(p^6 - 1)
+    {FE_MUL, 1, 1, 1, FU_FROB2},   // f1 = f1^(p^2 + 1)
+    {FE_EXPX, 3, 1, 1, FU_ID},     // a = f1^x
+    {FE_MUL, 3, 3, 1, FU_CONJ},    // a = f1^(x - 1)
+    {FE_EXPX, 2, 3, 3, FU_ID},     // t = a^x
+    {FE_MUL, 3, 2, 3, FU_CONJ},    // a = f1^((x - 1)^2)
+    {FE_EXPX, 4, 3, 3, FU_ID},     // b = a^x
+    {FE_MUL, 4, 4, 3, FU_FROB},    // b = a^(x + p)
+    {FE_EXPX, 0, 4, 4, FU_ID},     // c = b^x
+    {FE_EXPX, 0, 0, 0, FU_ID},     // c = b^(x^2)
+    {FE_MUL, 0, 0, 4, FU_FROB2},   // c = c b^(p^2)
+    {FE_MUL, 0, 0, 4, FU_CONJ},    // c = b^(x^2 + p^2 - 1)
+    {FE_MUL, 2, 1, 1, FU_CYC},     // t = f1^3
+    {FE_MUL, 0, 0, 2, FU_ID},      // c = c f1^3
+};
+constexpr int FE_N = (int)(sizeof(FE_STEPS) / sizeof(FE_STEPS[0]));
+
+// Selections by masks: a switch over the working values (or a chain of
+// selects) is turned into an indexed array in scratch memory.
+DEV void fe_get(Fp& x, int s, const Fp& R0, const Fp& R1, const Fp& R2, const Fp& R3, const Fp& R4) {
+  const uint32_t m0 = 0u - (uint32_t)(s == 0), m1 = 0u - (uint32_t)(s == 1), m2 = 0u - (uint32_t)(s == 2);
+  const uint32_t m3 = 0u - (uint32_t)(s == 3), m4 = 0u - (uint32_t)(s == 4);
+#pragma unroll
+  for (int k = 0; k < E; k++)
+    x.v[k] = (R0.v[k] & m0) | (R1.v[k] & m1) | (R2.v[k] & m2) | (R3.v[k] & m3) | (R4.v[k] & m4);
+}
+DEV void fe_put(int d, const Fp& x, Fp& R0, Fp& R1, Fp& R2, Fp& R3, Fp& R4) {
+  const uint32_t m0 = 0u - (uint32_t)(d == 0), m1 = 0u - (uint32_t)(d == 1), m2 = 0u - (uint32_t)(d == 2);
+  const uint32_t m3 = 0u - (uint32_t)(d == 3), m4 = 0u - (uint32_t)(d == 4);
+#pragma unroll
+  for (int k = 0; k < E; k++) {
+    R0.v[k] = (x.v[k] & m0) | (R0.v[k] & ~m0);
+    R1.v[k] = (x.v[k] & m1) | (R1.v[k] & ~m1);
+    R2.v[k] = (x.v[k] & m2) | (R2.v[k] & ~m2);
+    R3.v[k] = (x.v[k] & m3) | (R3.v[k] & ~m3);
+    R4.v[k] = (x.v[k] & m4) | (R4.v[k] & ~m4);
+  }
+}
+
 DEVN void final_exp(Fp& r, const Fp& f, Lane L) {
-  Fp f1, t, a, b, c;
-  t = f;
-  inv(t, L);
-  f1 = f;
-  conj(f1, L);
-  mul(f1, t, L);  // f^(p^6 - 1)
-  t = f1;
-  frob2(t, L);
-  mul(f1, t, L);  // ^(p^2 + 1)
-  exp_by_x(a, f1, L);
-  t = f1;
-  conj(t, L);
-  mul(a, t, L);  // a = f1^(x-1)
-  exp_by_x(t, a, L);
-  b = a;
-  conj(b, L);
-  mul(t, b, L);
-  a = t;  // a = f1^((x-1)^2)
-  exp_by_x(b, a, L);
-  t = a;
-  frob(t, L);
-  mul(b, t, L);  // b = a^(x+p)
-  exp_by_x(c, b, L);
-  exp_by_x(c, c, L);
-  t = b;
-  frob2(t, L);
-  mul(c, t, L);
-  t = b;
-  conj(t, L);
-  mul(c, t, L);  // c = b^(x^2 + p^2 - 1)
-  t = f1;
-  cyc_sqr(t, L);
-  mul(t, f1, L);  // f1^3
-  mul(c, t, L);
-  r = c;
+  Fp R0 = f, R1, R2, R3, R4;
+  fp_zero(R1);
+  fp_zero(R2);
+  fp_zero(R3);
+  fp_zero(R4);
+#pragma unroll 1
+  for (int st = 0; st < FE_N; st++) {
+    const FeStep S = FE_STEPS[st];
+    Fp x, y;
+    fe_get(x, S.s1, R0, R1, R2, R3, R4);
+    if (S.op == FE_INV) {
+      inv(x, L);
+    } else if (S.op == FE_EXPX) {
+      exp_by_x(x, L);
+    } else {
+      fe_get(y, S.s2, R0, R1, R2, R3, R4);
+      if (S.u == FU_CONJ)
+        conj(y, L);
+      else if (S.u == FU_FROB)
+        frob(y, L);
+      else if (S.u == FU_FROB2)
+        frob2(y, L);
+      else if (S.u == FU_CYC)
+        cyc_sqr(y, L);
+      if (S.op == FE_MUL)
+        mul_in(x, y, L);
+      else
+        x = y;
+    }
+    fe_put(S.d, x, R0, R1, R2, R3, R4);
+  }
+  r = R0;
 }
 
 // Product of the two Miller loops e(P0, Q0) e(P1, Q1) with prepared lines:

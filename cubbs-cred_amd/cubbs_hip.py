@@ -567,9 +567,11 @@ def mad_peak_bench(lanes, iters):
 class StagedBatch:
     """A batch whose inputs stay resident on the device (prepare once, launch many)."""
 
-    def __init__(self, kind, reqs, single_stream=False, aggregate=False, comb_chunk=0):
+    def __init__(self, kind, reqs, single_stream=False, aggregate=False, comb_chunk=0, supergroup=0):
         """aggregate: False, True (the tuned group size, else 8) or a group
-        size 2..8; comb_chunk: fixed-base comb terms per lane (Verify /
+        size 2..8; supergroup: groups per supergroup check (2..8, with
+        aggregate; 0 = the tuned choice when the group size is left open,
+        else none); comb_chunk: fixed-base comb terms per lane (Verify /
         ProofVerify; 0 = the tuned choice, else 4)."""
         self.keep = _Keep()
         if kind == BATCH_VERIFY:
@@ -592,7 +594,8 @@ class StagedBatch:
                                                      (BATCH_AGGREGATE if aggregate else 0) |
                                                      (0 if aggregate is True or not aggregate
                                                       else (int(aggregate) & 15) << 8) |
-                                                     ((int(comb_chunk) & 127) << 12),
+                                                     ((int(comb_chunk) & 127) << 12) |
+                                                     ((int(supergroup) & 15) << 20),
                                                      ctypes.byref(st))
         self.keep = None  # items are only read during prepare
         if not self.h:

@@ -92,6 +92,12 @@ void *cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void *items,
  * it against 6.5 ms without.  The group size is the one of CUBBS_HIP_BATCH_AGG_GROUP(n)
  * (2 <= n <= 8) when that is or-ed into the flags, else the pipeline tuning
  * table's for the batch's size class (cubbs_hip_tune_load), else 8.
+ * CUBBS_HIP_BATCH_AGG_SUPER(m) (2 <= m <= 8, with CUBBS_HIP_BATCH_AGGREGATE)
+ * adds a level above the groups: up to m consecutive groups of one key are
+ * first checked as one combination (their c_i a_i and c_i b_i summed), and
+ * their groups only when that check fails, their items only when a group
+ * fails.  Without it a batch takes the table's choice for its size class
+ * when it leaves the group size open too, else no supergroups.
  * CUBBS_HIP_BATCH_COMB_CHUNK(n) (1 <= n <= 64, Verify / ProofVerify only)
  * sets the fixed-base comb terms per lane of this batch (otherwise the
  * table's choice for its size class, else 4); the pipeline tuner stages its
@@ -99,6 +105,7 @@ void *cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void *items,
 #define CUBBS_HIP_BATCH_AGGREGATE 2u
 #define CUBBS_HIP_BATCH_AGG_GROUP(n) (((uint32_t)(n) & 15u) << 8)
 #define CUBBS_HIP_BATCH_COMB_CHUNK(n) (((uint32_t)(n) & 127u) << 12)
+#define CUBBS_HIP_BATCH_AGG_SUPER(m) (((uint32_t)(m) & 15u) << 20)
 void *cubbs_hip_batch_prepare_flags(uint32_t abi_version, int kind, const void *items,
                                     size_t item_count, uint32_t flags, int32_t *status_out);
 int32_t cubbs_hip_batch_launch(void *batch);
@@ -265,10 +272,11 @@ int32_t cubbs_hip_autotune(uint32_t max_log2_terms, char *report, size_t report_
 /* Installs a table in the report format (e.g. saved from an earlier run);
  * an empty string clears the table.  A malformed line fails the call.  The
  * table also takes whole-operation pipeline lines,
- * "pipeline kind=verify|proof_verify log2_items=K chunk=N group=G", measured
- * by the caller with sets of such batches in flight: batches of about 2^K
- * items then plan their comb with N terms per lane and, when aggregated
- * without an explicit group, groups of G (2 <= G <= 8). */
+ * "pipeline kind=verify|proof_verify log2_items=K chunk=N group=G [super=M]",
+ * measured by the caller with sets of such batches in flight: batches of
+ * about 2^K items then plan their comb with N terms per lane and, when
+ * aggregated without an explicit group, groups of G (2 <= G <= 8) and
+ * supergroups of M groups (0: none, else 2 <= M <= 8). */
 int32_t cubbs_hip_tune_load(const char *table);
 
 /* Launch shapes forced on the calling thread for the calls it makes next

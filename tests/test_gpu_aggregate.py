@@ -17,8 +17,10 @@ pytestmark = pytest.mark.gpu
 
 
 def _run(g, kind, reqs, group):
-    """(status, per-item results) of one staged batch; group 0 = per-item checks."""
-    b = g.StagedBatch(kind, reqs, aggregate=group if group else False)
+    """(status, per-item results) of one staged batch; group 0 = per-item
+    checks, (n, m) = groups of n in supergroups of m groups."""
+    group, sup = group if isinstance(group, tuple) else (group, 0)
+    b = g.StagedBatch(kind, reqs, aggregate=group if group else False, supergroup=sup)
     try:
         b.launch()
         b.sync()
@@ -67,12 +69,12 @@ def test_verify_aggregation_matches_individual_checks(gpu, oracle):
             sig = b"\xc0" + bytes(47) + sig[48:]               # identity A
         reqs.append((pk, sig, h, m))
     expect = [oracle.verify(*r) for r in reqs]
-    runs = {n: _run(g, g.BATCH_VERIFY, reqs, n) for n in (0, 2, 8)}
+    runs = {n: _run(g, g.BATCH_VERIFY, reqs, n) for n in (0, 2, 8, (8, 4), (2, 3))}
     base = runs[0]
     assert base[0] == 0
     assert [v for _, v in base[1]] == [bool(e) for e in expect]
     assert base[1][22][0] == 1
-    for n in (2, 8):
+    for n in (2, 8, (8, 4), (2, 3)):
         assert runs[n] == base, n
 
 
@@ -101,12 +103,12 @@ def test_proof_verify_aggregation_matches_individual_checks(gpu, oracle):
         if i == 13:
             proof = b"\xc0" + bytes(47) + proof[48:]          # identity Abar
         reqs.append((pk, proof, h, ph, disc, idx, len(m)))
-    runs = {n: _run(g, g.BATCH_PROOF_VERIFY, reqs, n) for n in (0, 2, 8)}
+    runs = {n: _run(g, g.BATCH_PROOF_VERIFY, reqs, n) for n in (0, 2, 8, (2, 4), (4, 2))}
     base = runs[0]
     assert base[0] == 0
     valid = [v for _, v in base[1]]
     assert valid == [i not in (4, 5, 9, 13) for i in range(24)]
-    for n in (2, 8):
+    for n in (2, 8, (2, 4), (4, 2)):
         assert runs[n] == base, n
 
 
@@ -134,6 +136,46 @@ def test_aggregated_staged_batch_graph_replays(gpu, oracle):
             b.launch()
             b.sync()
             assert b.results() == expect
+    finally:
+        b.free()
+
+
+def test_supergroup_levels_gate_groups_and_items(gpu, oracle):
+    """Supergroups of up to 4 groups of 8 per key, replayed from a graph:
+    the per-item results equal the per-item batch's, a supergroup that holds
+    accepts its groups and items without their own checks, and a failing one
+    has its groups checked, and the items of failing groups."""
+    g = gpu
+    rng = random.Random(6)
+    kps = _issuers(g, rng, 3)
+    # keys 0 and 1: 20 items each (groups of 8, 8, 4: one supergroup of 3
+    # groups); key 2: 9 items (a group of 8 and a single check)
+    owners = [i % 2 for i in range(40)] + [2] * 9
+    sreqs = [(kps[k][0], kps[k][1], b"", [rng.randbytes(20)]) for k in owners]
+    rc, sigs = g.sign_batch(sreqs)
+    assert rc == 0
+    bad = {18}                     # key 0: the second item of its second group
+    reqs = [(pk, sig, h, ([b"bad"] if i in bad else m)) for i, ((sk, pk, h, m), (_, sig)) in
+            enumerate(zip(sreqs, sigs))]
+    expect = [(0, i not in bad) for i in range(len(reqs))]
+    assert _run(g, g.BATCH_VERIFY, reqs, 0) == (0, expect)
+    b = g.StagedBatch(g.BATCH_VERIFY, reqs, aggregate=8, supergroup=4)
+    try:
+        for graph in (False, True):
+            b.use_graph(graph)
+            for _ in range(2):
+                b.launch()
+                b.sync()
+                assert b.results() == expect
+                # 2 supergroups + 7 groups; key 1's supergroup holds (its 3
+                # groups accepted), key 0's fails and two of its groups hold,
+                # key 2's lone group holds
+                assert b.group_checks() == (9, 7)
+                w = b.work()
+                assert w["group_checks"] == 9 and w["pair_checks"] == len(reqs)
+                # run: 2 supergroup checks, key 0's 3 groups and key 2's
+                # group, the 8 items of key 0's failing group, key 2's single
+                assert w["pair_checks_run"] == 2 + 4 + 8 + 1
     finally:
         b.free()
 

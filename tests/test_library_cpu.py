@@ -166,6 +166,11 @@ def test_pipeline_tuning_lines_parse_and_reject():
     assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=0 group=8\n") == 1
     assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=4 group=9\n") == 1
     assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=41 chunk=4 group=8\n") == 1
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=4 group=8 super=4\n") == 0
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=4 group=8 super=0\n") == 0
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=4 group=8 super=1\n") == 1
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=4 group=8 super=9\n") == 1
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=4 group=8 super=x\n") == 1
     assert lib.cubbs_hip_tune_load(b"") == 0
 
 
@@ -188,7 +193,7 @@ def test_committed_pipeline_table_loads():
         assert best["ms_per_step"] == min(x["ms_per_step"] for x in c["candidates"])
 
 
-def test_prepare_flags_reject_bad_comb_chunk_before_device():
+def test_prepare_flags_reject_bad_shapes_before_device():
     lib = cubbs_hip.lib()
     st = ctypes.c_int32(0)
     items = (cubbs_hip.VerifyItem * 1)()
@@ -200,3 +205,8 @@ def test_prepare_flags_reject_bad_comb_chunk_before_device():
     h = lib.cubbs_hip_batch_prepare_flags(2, cubbs_hip.BATCH_G1_MSM, ctypes.cast(items, ctypes.c_void_p),
                                           ctypes.c_size_t(1), 4 << 12, ctypes.byref(st))
     assert not h and st.value == cubbs_hip.ERR_INVALID_INPUT
+    # supergroups: without aggregation, of one group, of more than 8 groups
+    for flags in ((4 << 20), cubbs_hip.BATCH_AGGREGATE | (1 << 20), cubbs_hip.BATCH_AGGREGATE | (9 << 20)):
+        h = lib.cubbs_hip_batch_prepare_flags(2, cubbs_hip.BATCH_VERIFY, ctypes.cast(items, ctypes.c_void_p),
+                                              ctypes.c_size_t(1), flags, ctypes.byref(st))
+        assert not h and st.value == cubbs_hip.ERR_INVALID_INPUT
