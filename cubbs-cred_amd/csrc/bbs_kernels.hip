@@ -358,7 +358,8 @@ void launch_comb16_build(hipStream_t s, uint32_t first, uint32_t count, const ui
 }
 
 // ------------------------------------------------------------------ fixed-base comb MSM
-// Each lane sums up to term_count terms of one chunk into one Jacobian partial.
+// Each lane sums up to term_count terms of one chunk in XYZZ coordinates
+// (the cheaper mixed addition) and stores one Jacobian partial.
 // Scalar k (canonical, < r < 2^255) is recoded into signed digits: 16 of 16
 // bits over the wide table when the generator slot is below n16, else 32 of
 // 8 bits.  Windows never straddle a 32-bit word of k.  The loop is software
@@ -418,8 +419,8 @@ __global__ __launch_bounds__(128) void k_comb_msm(const CombChunk* __restrict__ 
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const CombChunk c = chunks[i];
-  G1J acc;
-  pt_set_inf(acc);
+  G1Z acc;
+  g1z_set_inf(acc);
   CombCursor cur;
   cur.term_gen = term_gen;
   cur.term_scalar = term_scalar;
@@ -459,11 +460,13 @@ __global__ __launch_bounds__(128) void k_comb_msm(const CombChunk* __restrict__ 
       for (int q = 0; q < Q; q++) buf[q] = e4[q];
     }
     if (add) {
-      if (sub) fp_neg(e.y, e.y);
-      g1_madd_in(acc, e);
+      fp_cond_neg_lazy(e.y, sub);
+      g1z_madd(acc, e);
     }
   }
-  store_g1j(partials + (size_t)c.partial * G1J_WORDS, acc);
+  G1J out;
+  g1z_to_jacobian(out, acc);
+  store_g1j(partials + (size_t)c.partial * G1J_WORDS, out);
 }
 void launch_comb_msm(hipStream_t s, const CombChunk* chunks, uint32_t n, const uint32_t* term_gen,
                      const uint32_t* term_scalar, const uint32_t* tables, const uint32_t* tables16,

@@ -393,6 +393,113 @@ DEV void g1_add_in(G1J& p, const G1J& q) {
   fp_sub(p.y, t3, t4);
 }
 
+// ------------------------------------------------------------------ G1 in XYZZ coordinates
+// (X, Y, ZZ, ZZZ) stands for (X/ZZ, Y/ZZZ) with ZZ^3 = ZZZ^2; ZZ = 0 is the
+// identity.  A mixed addition (madd-2008-s) costs 8M + 2S and 7 additions,
+// against 7M + 4S and 13 for madd-2007-bl in Jacobian coordinates, so long
+// sums of affine terms (the fixed-base comb) accumulate in this form and
+// convert to Jacobian once.
+struct G1Z {
+  Fp x, y, zz, zzz;
+};
+DEV void g1z_set_inf(G1Z& p) {
+  fp_one(p.x);
+  fp_one(p.y);
+  fp_zero(p.zz);
+  fp_zero(p.zzz);
+}
+// 4p with every limb but the top one at least 2^28 - 1 (no borrows when a
+// normalized value below 2p is subtracted from it limb by limb)
+struct P4Redundant {
+  uint32_t v[14];
+  constexpr P4Redundant() : v() {
+    for (int i = 0; i < 14; i++) v[i] = bls::P4_28[i];
+    v[0] += 1u << 28;
+    for (int i = 1; i < 13; i++) v[i] += (1u << 28) - 1;
+    v[13] -= 1;
+  }
+};
+static constexpr P4Redundant P4_RED{};
+// y <- 4p - y when neg (value below 4p, limbs below 2^29: an operand for a
+// product only), else y
+DEV void fp_cond_neg_lazy(Fp& y, bool neg) {
+#pragma unroll
+  for (int k = 0; k < 14; k++) y.v[k] = neg ? P4_RED.v[k] - y.v[k] : y.v[k];
+}
+// 2 Q for an affine Q (mdbl-2008-s-1), the rare case of a mixed addition of
+// equal points; Q.y may be lazily negated
+DEVN void g1z_dbl_affine(G1Z& r, const G1A& q) {
+  Fp y, z, U, V, W, S, M, t;
+  fp_zero(z);
+  fp_add(y, q.y, z);  // below 2p, normalized
+  fp_dbl(U, y);
+  fp_sqr(V, U);
+  fp_mul(W, U, V);
+  fp_mul(S, q.x, V);
+  fp_sqr(t, q.x);
+  fp_dbl(M, t);
+  fp_add(M, M, t);
+  fp_sqr(t, M);
+  fp_sub(t, t, S);
+  fp_sub(r.x, t, S);
+  fp_sub(t, S, r.x);
+  fp_mul(t, M, t);
+  fp_mul(S, W, y);
+  fp_sub(r.y, t, S);
+  r.zz = V;
+  r.zzz = W;
+}
+// p <- p + q, q affine and not the identity (q.y may be lazily negated:
+// it only enters products and the normalizing additions below)
+DEV void g1z_madd(G1Z& p, const G1A& q) {
+  Fp z;
+  fp_zero(z);
+  if (fp_is_zero(p.zz)) {
+    p.x = q.x;
+    fp_add(p.y, q.y, z);
+    fp_one(p.zz);
+    fp_one(p.zzz);
+    return;
+  }
+  Fp P, R, PP, PPP, Q, t;
+  fp_mul(P, q.x, p.zz);   // U2
+  fp_sub(P, P, p.x);      // P = U2 - X1
+  fp_mul(R, q.y, p.zzz);  // S2
+  fp_sub(R, R, p.y);      // R = S2 - Y1
+  if (fp_is_zero(P)) {
+    if (fp_is_zero(R)) {
+      G1Z d;
+      g1z_dbl_affine(d, q);
+      p = d;
+    } else {
+      g1z_set_inf(p);
+    }
+    return;
+  }
+  fp_sqr(PP, P);
+  fp_mul(PPP, P, PP);
+  fp_mul(Q, p.x, PP);
+  fp_mul(p.zz, p.zz, PP);
+  fp_mul(p.zzz, p.zzz, PPP);
+  fp_sqr(t, R);           // X3 = R^2 - PPP - 2 Q
+  fp_sub(t, t, PPP);
+  fp_sub(t, t, Q);
+  fp_sub(p.x, t, Q);
+  fp_sub(t, Q, p.x);      // Y3 = R (Q - X3) - Y1 PPP
+  fp_mul(t, R, t);
+  fp_mul(PP, p.y, PPP);
+  fp_sub(p.y, t, PP);
+}
+// The same point in Jacobian coordinates: (X ZZ^2, Y ZZ^3, ZZZ)
+DEV void g1z_to_jacobian(G1J& r, const G1Z& p) {
+  Fp z2, z3;
+  fp_sqr(z2, p.zz);
+  fp_mul(z3, z2, p.zz);
+  fp_mul(r.x, p.x, z2);
+  fp_mul(r.y, p.y, z3);
+  r.z = p.zzz;
+}
+
 // ------------------------------------------------------------------ G1 specifics
 DEV void g1_load_affine_const(G1A& r, const uint32_t* c) {
   fp_load_const(r.x, c);
