@@ -8,7 +8,7 @@
 // (term, half, window) entries are sorted by bucket with a counting sort: an
 // atomic histogram, one scan, an atomic scatter.  Buckets are cut into
 // pieces of at most S entries, one lane per piece sums its entries with mixed
-// additions, and one lane per bucket adds its pieces.
+// additions (in XYZZ coordinates), and one lane per bucket adds its pieces.
 //
 // Per window, sum_{b=1..B} b * S_b is formed without a B-long running sum.
 // With X_i = S_{i+1} (i = 0..B-1), K = log2 B in-place steps do, at step s
@@ -225,17 +225,19 @@ __global__ __launch_bounds__(64) void k_pip_accum(PipShape sh, const PipTerm* __
   const uint32_t end = min(first + sh.S, bf.off[b] + bf.cnt[b]);
   Fp beta;
   fp_load_const(beta, bls::GLV_BETA);
-  G1J acc;
-  pt_set_inf(acc);
+  G1Z acc;  // XYZZ: the cheaper mixed addition
+  g1z_set_inf(acc);
   for (uint32_t e = first; e < end; e++) {
     const uint32_t v = bf.sorted[e];
     G1A q;
     load_g1a(q, points + (size_t)terms[v >> 2].point * G1A_WORDS);
     if (v & 2) fp_mul(q.x, q.x, beta);
-    if (v & 1) fp_neg(q.y, q.y);
-    g1_madd_in(acc, q);
+    fp_cond_neg_lazy(q.y, (v & 1) != 0);
+    g1z_madd(acc, q);
   }
-  store_g1j(bf.pieces + (size_t)p * G1J_WORDS, acc);
+  G1J out;
+  g1z_to_jacobian(out, acc);
+  store_g1j(bf.pieces + (size_t)p * G1J_WORDS, out);
 }
 void launch_pip_accum(hipStream_t s, const PipShape& sh, const PipTerm* terms,
                       const uint32_t* points, const PipBuffers& bf) {

@@ -269,30 +269,105 @@ DEVN void pt_mul_limbs_affine(PtJ<F>& r, const PtA<F>& q, const uint32_t* k) {
 // six field temporaries are live.  The rare exceptional cases (doubling
 // inside an addition) call the out-of-line forms.
 
-// p <- 2 p
+// k p in normalized 28-bit limbs (k p < 2^392)
+template <uint32_t K>
+struct PMul {
+  uint32_t v[14];
+  constexpr PMul() : v() {
+    uint64_t c = 0;
+    for (int i = 0; i < 14; i++) {
+      c += (uint64_t)K * bls::P28[i];
+      v[i] = (uint32_t)(c & 0x0fffffffu);
+      c >>= 28;
+    }
+  }
+};
+static constexpr PMul<16> P16_L{};
+static constexpr PMul<24> P24_L{};
+// a value below 2^392 in normalized limbs (r.v[k] < 2^28) that is below
+// (K + 1) 2p for K < 200: minus q 2p, q = floor(top limb / (top limb of 2p +
+// 1)) (never above the true quotient, at most one below it), then one
+// conditional subtraction of 2p: the result is below 2p.
+DEV void fp_reduce_multiple(Fp& r) {
+  constexpr uint32_t T = bls::P2_28[13] + 1;
+  const uint32_t q = r.v[13] / T;
+  Fp d, e;
+  int64_t b1 = 0;
+  int32_t b2 = 0;
+#pragma unroll
+  for (int i = 0; i < 14; i++) {
+    // q 2p_i reaches 2^36: 64-bit
+    const int64_t u = (int64_t)r.v[i] - (int64_t)q * bls::P2_28[i] + b1;
+    b1 = u >> 28;
+    d.v[i] = (uint32_t)u & FP_MASK;
+    const int32_t t = (int32_t)d.v[i] - (int32_t)bls::P2_28[i] + b2;
+    b2 = t >> 28;
+    e.v[i] = (uint32_t)t & FP_MASK;
+  }
+#pragma unroll
+  for (int i = 0; i < 14; i++) r.v[i] = b2 < 0 ? d.v[i] : e.v[i];
+}
+
+// p <- 2 p, dbl-2009-l with lazy sums: the terms that only enter products
+// stay unnormalized (X + B, 3A) or are formed in one signed limb pass, and
+// each output is one signed limb pass plus fp_reduce_multiple.  Bounds (p's
+// coordinates below 2p): X + B below 4p with limbs below 2^29; E = 3A below
+// 6p with limbs below 3 2^28 (E^2 < 36 p^2 < p R; every product column stays
+// below 2^64); D' = T + 4p - A - C in (0, 6p), D = 2 D'; X3 = F + 24p - 4 D'
+// in (0, 26p); W = 2 D' + 2p - X3 in (0, 14p); Y3 = E W + 16p - 8 C in
+// (0, 18p).
 DEV void g1_dbl_in(G1J& p) {
-  Fp A, B, C, D;
-  fp_sqr(A, p.x);          // A = X^2
-  fp_sqr(B, p.y);          // B = Y^2
-  fp_mul(p.z, p.y, p.z);   // Z3 = 2 Y Z (the identity keeps Z = 0)
-  fp_dbl(p.z, p.z);
-  fp_sqr(C, B);            // C = B^2
-  fp_add(D, p.x, B);       // D = 2 ((X + B)^2 - A - C)
-  fp_sqr(D, D);
-  fp_sub(D, D, A);
-  fp_sub(D, D, C);
-  fp_dbl(D, D);
-  fp_dbl(B, A);            // E = 3 A (in A)
-  fp_add(A, B, A);
-  fp_sqr(B, A);            // X3 = E^2 - 2 D
-  fp_sub(B, B, D);
-  fp_sub(p.x, B, D);
-  fp_sub(D, D, p.x);       // Y3 = E (D - X3) - 8 C
-  fp_mul(p.y, A, D);
-  fp_dbl(C, C);
-  fp_dbl(C, C);
-  fp_dbl(C, C);
-  fp_sub(p.y, p.y, C);
+  Fp A, B, C, D, E, t;
+  fp_sqr(A, p.x);  // A = X^2
+  fp_sqr(B, p.y);  // B = Y^2
+  fp_mul(t, p.y, p.z);
+  fp_dbl(p.z, t);  // Z3 = 2 Y Z (the identity keeps Z = 0)
+  fp_sqr(C, B);    // C = B^2
+#pragma unroll
+  for (int i = 0; i < 14; i++) t.v[i] = p.x.v[i] + B.v[i];  // X + B
+  fp_sqr(t, t);
+  {
+    int32_t cy = 0;  // D' = (X + B)^2 + 4p - A - C
+#pragma unroll
+    for (int i = 0; i < 14; i++) {
+      const int32_t u = (int32_t)(t.v[i] + bls::P4_28[i]) - (int32_t)A.v[i] - (int32_t)C.v[i] + cy;
+      cy = u >> 28;
+      D.v[i] = (uint32_t)u & FP_MASK;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 14; i++) E.v[i] = 3u * A.v[i];  // E = 3A
+  fp_sqr(t, E);                                        // F = E^2
+  {
+    int32_t cy = 0;  // X3 = F + 24p - 4 D'
+#pragma unroll
+    for (int i = 0; i < 14; i++) {
+      const int32_t u = (int32_t)(t.v[i] + P24_L.v[i]) - (int32_t)(D.v[i] << 2) + cy;
+      cy = u >> 28;
+      p.x.v[i] = (uint32_t)u & FP_MASK;
+    }
+  }
+  fp_reduce_multiple(p.x);
+  {
+    int32_t cy = 0;  // W = 2 D' + 2p - X3
+#pragma unroll
+    for (int i = 0; i < 14; i++) {
+      const int32_t u = (int32_t)((D.v[i] << 1) + bls::P2_28[i]) - (int32_t)p.x.v[i] + cy;
+      cy = u >> 28;
+      D.v[i] = (uint32_t)u & FP_MASK;
+    }
+  }
+  fp_mul(t, E, D);  // E W
+  {
+    int32_t cy = 0;  // Y3 = E W + 16p - 8C
+#pragma unroll
+    for (int i = 0; i < 14; i++) {
+      const int32_t u = (int32_t)(t.v[i] + P16_L.v[i]) - (int32_t)(C.v[i] << 3) + cy;
+      cy = u >> 28;
+      p.y.v[i] = (uint32_t)u & FP_MASK;
+    }
+  }
+  fp_reduce_multiple(p.y);
 }
 
 // p <- p + q, q affine and not the identity
