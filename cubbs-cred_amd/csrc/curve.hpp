@@ -282,6 +282,7 @@ struct PMul {
     }
   }
 };
+static constexpr PMul<6> P6_L{};
 static constexpr PMul<16> P16_L{};
 static constexpr PMul<24> P24_L{};
 // a value below 2^392 in normalized limbs (r.v[k] < 2^28) that is below
@@ -370,7 +371,11 @@ DEV void g1_dbl_in(G1J& p) {
   fp_reduce_multiple(p.y);
 }
 
-// p <- p + q, q affine and not the identity
+// p <- p + q, q affine and not the identity (madd-2007-bl with lazy sums:
+// r = 2 (S2 - Y1) and I = 4 HH stay unnormalized products operands, limbs
+// below 2^29 and 2^30; Z3 = 2 Z1 H; X3 = r^2 + 6p - J - 2V in (0, 8p) and
+// Y3 = r W + 4p - 2 Y1 J in (0, 6p), W = V + 2p - X3 in (0, 4p), each one
+// signed limb pass, the outputs reduced below 2p)
 DEV void g1_madd_in(G1J& p, const G1A& q) {
   if (fp_is_zero(p.z)) {
     pt_from_affine(p, q);
@@ -398,28 +403,53 @@ DEV void g1_madd_in(G1J& p, const G1A& q) {
     }
     return;
   }
-  fp_dbl(t2, t2);           // r = 2 (S2 - Y1)
+#pragma unroll
+  for (int k = 0; k < 14; k++) t2.v[k] <<= 1;  // r = 2 (S2 - Y1), lazy
+  fp_mul(p.z, p.z, t1);     // Z3 = 2 Z1 H
+  fp_dbl(p.z, p.z);
   fp_sqr(t3, t1);           // HH
-  fp_add(t4, p.z, t1);      // Z3 = (Z1 + H)^2 - Z1Z1 - HH
-  fp_sqr(t4, t4);
-  fp_sub(t4, t4, t0);
-  fp_sub(p.z, t4, t3);
-  fp_dbl(t3, t3);           // I = 4 HH
-  fp_dbl(t3, t3);
+#pragma unroll
+  for (int k = 0; k < 14; k++) t3.v[k] <<= 2;  // I = 4 HH, lazy
   fp_mul(t0, t1, t3);       // J = H I
   fp_mul(t1, p.x, t3);      // V = X1 I
   fp_mul(t4, p.y, t0);      // Y1 J
-  fp_sqr(t3, t2);           // X3 = r^2 - J - 2 V
-  fp_sub(t3, t3, t0);
-  fp_sub(t3, t3, t1);
-  fp_sub(p.x, t3, t1);
-  fp_sub(t1, t1, p.x);      // Y3 = r (V - X3) - 2 Y1 J
-  fp_mul(p.y, t2, t1);
-  fp_dbl(t4, t4);
-  fp_sub(p.y, p.y, t4);
+  fp_sqr(t3, t2);           // r^2
+  {
+    int32_t cy = 0;  // X3 = r^2 + 6p - J - 2V
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const int32_t u = (int32_t)(t3.v[k] + P6_L.v[k]) - (int32_t)t0.v[k] - (int32_t)(t1.v[k] << 1) + cy;
+      cy = u >> 28;
+      p.x.v[k] = (uint32_t)u & FP_MASK;
+    }
+  }
+  fp_reduce_multiple(p.x);
+  {
+    int32_t cy = 0;  // W = V + 2p - X3
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const int32_t u = (int32_t)(t1.v[k] + bls::P2_28[k]) - (int32_t)p.x.v[k] + cy;
+      cy = u >> 28;
+      t1.v[k] = (uint32_t)u & FP_MASK;
+    }
+  }
+  fp_mul(t3, t2, t1);       // r W
+  {
+    int32_t cy = 0;  // Y3 = r W + 4p - 2 Y1 J
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const int32_t u = (int32_t)(t3.v[k] + bls::P4_28[k]) - (int32_t)(t4.v[k] << 1) + cy;
+      cy = u >> 28;
+      p.y.v[k] = (uint32_t)u & FP_MASK;
+    }
+  }
+  fp_reduce_multiple(p.y);
 }
 
-// p <- p + q (Jacobian; q must not alias p)
+// p <- p + q (Jacobian; q must not alias p); add-2007-bl with lazy sums:
+// Z3 = ((Z1 + Z2)^2 + 4p - Z1Z1 - Z2Z2) H, r = 2 (S2 - S1) unnormalized,
+// X3 = r^2 + 6p - J - 2V and Y3 = r W + 4p - 2 S1 J (W = V + 2p - X3), each
+// one signed limb pass, the outputs reduced below 2p
 DEV void g1_add_in(G1J& p, const G1J& q) {
   if (fp_is_zero(q.z)) return;
   if (fp_is_zero(p.z)) {
@@ -447,25 +477,59 @@ DEV void g1_add_in(G1J& p, const G1J& q) {
     }
     return;
   }
-  fp_add(p.z, p.z, q.z);    // Z3 = ((Z1 + Z2)^2 - Z1Z1 - Z2Z2) H
-  fp_sqr(p.z, p.z);
-  fp_sub(p.z, p.z, t0);
-  fp_sub(p.z, p.z, t1);
-  fp_mul(p.z, p.z, t3);
-  fp_dbl(t5, t5);           // r = 2 (S2 - S1)
-  fp_dbl(t0, t3);           // I = (2 H)^2
-  fp_sqr(t0, t0);
+  {
+    Fp s;
+#pragma unroll
+    for (int k = 0; k < 14; k++) s.v[k] = p.z.v[k] + q.z.v[k];  // Z1 + Z2, lazy
+    fp_sqr(s, s);
+    int32_t cy = 0;  // (Z1 + Z2)^2 + 4p - Z1Z1 - Z2Z2, in (0, 6p)
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const int32_t u = (int32_t)(s.v[k] + bls::P4_28[k]) - (int32_t)t0.v[k] - (int32_t)t1.v[k] + cy;
+      cy = u >> 28;
+      s.v[k] = (uint32_t)u & FP_MASK;
+    }
+    fp_mul(p.z, s, t3);     // Z3
+  }
+#pragma unroll
+  for (int k = 0; k < 14; k++) t5.v[k] <<= 1;  // r = 2 (S2 - S1), lazy
+#pragma unroll
+  for (int k = 0; k < 14; k++) t0.v[k] = t3.v[k] << 1;  // 2H, lazy
+  fp_sqr(t0, t0);           // I = (2H)^2
   fp_mul(t1, t3, t0);       // J = H I
   fp_mul(t3, t2, t0);       // V = U1 I
-  fp_sqr(t0, t5);           // X3 = r^2 - J - 2 V
-  fp_sub(t0, t0, t1);
-  fp_sub(t0, t0, t3);
-  fp_sub(p.x, t0, t3);
-  fp_sub(t3, t3, p.x);      // Y3 = r (V - X3) - 2 S1 J
-  fp_mul(t3, t5, t3);
-  fp_mul(t4, t4, t1);
-  fp_dbl(t4, t4);
-  fp_sub(p.y, t3, t4);
+  fp_mul(t4, t4, t1);       // S1 J
+  fp_sqr(t0, t5);           // r^2
+  {
+    int32_t cy = 0;  // X3 = r^2 + 6p - J - 2V
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const int32_t u = (int32_t)(t0.v[k] + P6_L.v[k]) - (int32_t)t1.v[k] - (int32_t)(t3.v[k] << 1) + cy;
+      cy = u >> 28;
+      p.x.v[k] = (uint32_t)u & FP_MASK;
+    }
+  }
+  fp_reduce_multiple(p.x);
+  {
+    int32_t cy = 0;  // W = V + 2p - X3
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const int32_t u = (int32_t)(t3.v[k] + bls::P2_28[k]) - (int32_t)p.x.v[k] + cy;
+      cy = u >> 28;
+      t3.v[k] = (uint32_t)u & FP_MASK;
+    }
+  }
+  fp_mul(t0, t5, t3);       // r W
+  {
+    int32_t cy = 0;  // Y3 = r W + 4p - 2 S1 J
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const int32_t u = (int32_t)(t0.v[k] + bls::P4_28[k]) - (int32_t)(t4.v[k] << 1) + cy;
+      cy = u >> 28;
+      p.y.v[k] = (uint32_t)u & FP_MASK;
+    }
+  }
+  fp_reduce_multiple(p.y);
 }
 
 // ------------------------------------------------------------------ G1 in XYZZ coordinates
@@ -556,12 +620,27 @@ DEV void g1z_madd(G1Z& p, const G1A& q) {
   fp_mul(Q, p.x, PP);
   fp_mul(p.zz, p.zz, PP);
   fp_mul(p.zzz, p.zzz, PPP);
-  fp_sqr(t, R);           // X3 = R^2 - PPP - 2 Q
-  fp_sub(t, t, PPP);
-  fp_sub(t, t, Q);
-  fp_sub(p.x, t, Q);
-  fp_sub(t, Q, p.x);      // Y3 = R (Q - X3) - Y1 PPP
-  fp_mul(t, R, t);
+  fp_sqr(t, R);
+  {
+    int32_t cy = 0;  // X3 = R^2 + 6p - PPP - 2Q, in (0, 8p)
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const int32_t u = (int32_t)(t.v[k] + P6_L.v[k]) - (int32_t)PPP.v[k] - (int32_t)(Q.v[k] << 1) + cy;
+      cy = u >> 28;
+      p.x.v[k] = (uint32_t)u & FP_MASK;
+    }
+  }
+  fp_reduce_multiple(p.x);
+  {
+    int32_t cy = 0;  // W = Q + 2p - X3, in (0, 4p)
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+      const int32_t u = (int32_t)(Q.v[k] + bls::P2_28[k]) - (int32_t)p.x.v[k] + cy;
+      cy = u >> 28;
+      t.v[k] = (uint32_t)u & FP_MASK;
+    }
+  }
+  fp_mul(t, R, t);        // Y3 = R W - Y1 PPP
   fp_mul(PP, p.y, PPP);
   fp_sub(p.y, t, PP);
 }

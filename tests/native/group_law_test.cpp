@@ -3,6 +3,9 @@
 // in the limb passes):
 //  * g1_dbl_in (lazy sums, one signed limb pass per output) against
 //    dbl-2009-l written here with normalized additions only;
+//  * g1_madd_in and g1_add_in (lazy sums) against madd-2007-bl and
+//    add-2007-bl written with normalized additions, equal and opposite
+//    inputs included;
 //  * g1z_madd (XYZZ, madd-2008-s, lazily negated y) against g1_madd_in
 //    (Jacobian, madd-2007-bl), over running sums;
 //  * fp_reduce_multiple on values below 26p.
@@ -10,6 +13,10 @@
 // maximum and near zero, and 0, p, 2p - 1.  Results are compared mod p, and
 // outputs must be normalized (limbs below 2^28) and below 2p.  Formulas are
 // compared as rational functions, so the inputs need not lie on the curve.
+// The multiples of p added in each signed limb pass follow from the bounds
+// stated beside the pass (every product and normalized difference is below
+// 2p); random inputs rarely come near those bounds, so this test guards the
+// formulas and the reductions rather than the size of each offset.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -81,6 +88,57 @@ static void dbl_reference(G1J& p) {
   p.y = y3;
   p.z = z3;
 }
+// madd-2007-bl and add-2007-bl with normalized additions (the references)
+static void madd_reference(G1J& p, const G1A& q) {
+  if (fp_is_zero(p.z)) {
+    pt_from_affine(p, q);
+    return;
+  }
+  Fp Z1Z1, U2, S2, H, r, HH, I, J, V, t, x3, y3, z3;
+  fp_sqr(Z1Z1, p.z);
+  fp_mul(U2, q.x, Z1Z1);
+  fp_mul(S2, q.y, p.z);
+  fp_mul(S2, S2, Z1Z1);
+  fp_sub(H, U2, p.x);
+  fp_sub(r, S2, p.y);
+  if (fp_is_zero(H)) {
+    G1J qj;
+    pt_from_affine(qj, q);
+    if (fp_is_zero(r)) {
+      pt_dbl(p, qj);
+    } else {
+      pt_set_inf(p);
+    }
+    return;
+  }
+  fp_dbl(r, r);
+  fp_sqr(HH, H);
+  fp_add(t, p.z, H);
+  fp_sqr(t, t);
+  fp_sub(t, t, Z1Z1);
+  fp_sub(z3, t, HH);
+  fp_dbl(I, HH);
+  fp_dbl(I, I);
+  fp_mul(J, H, I);
+  fp_mul(V, p.x, I);
+  fp_sqr(x3, r);
+  fp_sub(x3, x3, J);
+  fp_sub(x3, x3, V);
+  fp_sub(x3, x3, V);
+  fp_sub(t, V, x3);
+  fp_mul(y3, r, t);
+  fp_mul(t, p.y, J);
+  fp_dbl(t, t);
+  fp_sub(y3, y3, t);
+  p.x = x3;
+  p.y = y3;
+  p.z = z3;
+}
+static void add_reference(G1J& p, const G1J& q) {
+  G1J r;
+  pt_add(r, p, q);  // the out-of-line form: add-2007-bl, normalized
+  p = r;
+}
 static bool same_point(const G1J& a, const G1J& b) {
   // projective equality X1 Z2^2 = X2 Z1^2, Y1 Z2^3 = Y2 Z1^3 (or both identity)
   const bool ia = fp_is_zero(a.z), ib = fp_is_zero(b.z);
@@ -148,6 +206,47 @@ int main(int argc, char** argv) {
     g1z_to_jacobian(c, z);
     if (!same_point(c, j)) bad_madd++;
   }
+  // Jacobian mixed and full additions against the references, with equal
+  // and opposite inputs now and then (the doubling and identity cases)
+  long bad_add = 0;
+  for (long it = 0; it < n / 2; it++) {
+    G1J p, q;
+    const int mode = (int)(it % 3);
+    value_below_2p(g, p.x, mode);
+    value_below_2p(g, p.y, (int)(g() % 3));
+    value_below_2p(g, p.z, (int)(g() % 3));
+    value_below_2p(g, q.x, (int)(g() % 3));
+    value_below_2p(g, q.y, (int)(g() % 3));
+    value_below_2p(g, q.z, (int)(g() % 3));
+    if (it < 27) {
+      special(p.x, (int)(it % 3));
+      special(q.y, (int)(it / 3 % 3));
+      special(p.z, (int)(it / 9 % 3));
+    }
+    G1A qa;
+    qa.x = q.x;
+    qa.y = q.y;
+    if (it % 50 == 7) {  // p equal to the affine q, or to its negative
+      p.x = qa.x;
+      p.y = qa.y;
+      fp_one(p.z);
+      if (it % 100 == 7) fp_neg(p.y, p.y);
+    }
+    G1J a = p, b = p;
+    g1_madd_in(a, qa);
+    madd_reference(b, qa);
+    if (!eq_mod_p(a.x, b.x) || !eq_mod_p(a.y, b.y) || !eq_mod_p(a.z, b.z) || !normalized_below_2p(a.x) ||
+        !normalized_below_2p(a.y) || !normalized_below_2p(a.z))
+      bad_add++;
+    if (it % 50 == 11) q = p;  // equal Jacobian inputs
+    G1J c = p, d = p;
+    g1_add_in(c, q);
+    add_reference(d, q);
+    if (!same_point(c, d) || !normalized_below_2p(c.x) || !normalized_below_2p(c.y) ||
+        !normalized_below_2p(c.z))
+      bad_add++;
+  }
+  bad_madd += bad_add;
   // fp_reduce_multiple on normalized values below 26p
   for (long it = 0; it < n; it++) {
     Fp a[13], s;
