@@ -154,7 +154,8 @@ DEV int ysel1(int c, bool xi) { return c ? (xi ? ED : EY0) : (xi ? ENS : EN1); }
 // Publish the lane's coefficient v and its share of the derived entries.
 // Lane c = 0 writes y0, 2 y0, y0 + 2p - y1, y0 + y1; lane c = 1 writes y1,
 // 2 y1, 2p - y1, 4p - y0 - y1.  Values below 2p in, derived values below 4p.
-DEV void put_rich(lds_t* R, const Fp& v, const Lane& L) {
+// doubles = false leaves out 2 y0 and 2 y1, which only squarings read.
+DEV void put_rich(lds_t* R, const Fp& v, const Lane& L, bool doubles = true) {
   Fp o, y0, y1, z, a, b, s2, t2;
   partner(o, v);
   sel(y0, L.c, o, v);
@@ -162,7 +163,7 @@ DEV void put_rich(lds_t* R, const Fp& v, const Lane& L) {
   fp_zero(z);
   lds_t* S = R + L.k * SLOT;
   Fp t1, t2v, t3;
-  lz5(t1, v, v, 0, z, z);
+  if (doubles) lz5(t1, v, v, 0, z, z);
   sel(a, L.c, z, y0);
   lz5(t2v, a, z, 2, y1, z);  // c=0: y0 + 2p - y1   c=1: 2p - y1
   sel(b, L.c, z, y1);
@@ -171,17 +172,35 @@ DEV void put_rich(lds_t* R, const Fp& v, const Lane& L) {
   lz5(t3, a, b, L.c ? 4 : 0, s2, t2);  // c=0: y0 + y1   c=1: 4p - y0 - y1
   if (L.wr) {
     st(S + (L.c ? EY1 : EY0) * LE, v);
-    st(S + (L.c ? E2Y1 : E2Y0) * LE, t1);
+    if (doubles) st(S + (L.c ? E2Y1 : E2Y0) * LE, t1);
     st(S + (L.c ? EN1 : ED) * LE, t2v);
     st(S + (L.c ? ENS : ES) * LE, t3);
   }
 }
 
-// f <- f * g (inlined into the loops that use it; mul() below is the
-// out-of-line form for the straight-line parts of the final exponentiation)
+// Publish what a sparse line product reads (mul_line): y0, y1 and, one per
+// lane, y0 + 2p - y1 (ED, lane c = 0) and y0 + y1 (ES, lane c = 1).  The
+// other derived entries keep stale values that the line product never reads.
+DEV void put_line(lds_t* R, const Fp& v, const Lane& L) {
+  Fp o, y0, y1, z, b, sb, t;
+  partner(o, v);
+  sel(y0, L.c, o, v);
+  sel(y1, L.c, v, o);
+  fp_zero(z);
+  sel(b, L.c, y1, z);
+  sel(sb, L.c, z, y1);
+  lz5(t, y0, b, L.c ? 0 : 2, sb, z);  // c=0: y0 + 2p - y1   c=1: y0 + y1
+  lds_t* S = R + L.k * SLOT;
+  if (L.wr) {
+    st(S + (L.c ? EY1 : EY0) * LE, v);
+    st(S + (L.c ? ES : ED) * LE, t);
+  }
+}
+
+// f <- f * g (expanded in the loops that use it)
 DEV void mul_in(Fp& f, const Fp& g, const Lane& L) {
   __syncthreads();
-  put_rich(L.F, g, L);
+  put_rich(L.F, g, L, false);
   if (L.wr) st(L.X + (2 * L.k + L.c) * LE, f);
   __syncthreads();
   Wide w;
@@ -274,7 +293,7 @@ DEV void mul_line(Fp& f, const uint32_t* v, const Fp& yp, const Lane& L) {
   ld(b0, v + (xi ? (L.c ? LV_S : LV_D) : (L.c ? LV_M1 : LV_M0)) * E);
   ld(b1, v + (xi ? (L.c ? LV_D : LV_NS) : (L.c ? LV_M0 : LV_NM1)) * E);
   __syncthreads();
-  put_rich(L.F, f, L);
+  put_line(L.F, f, L);
   __syncthreads();
   Wide w;
   w_zero(w);
