@@ -1352,12 +1352,20 @@ int orc_fr_inv(const uint8_t a[32], uint8_t out[32]) {
 }
 
 /* ===================================================================== 7. BBS */
-/* Generator cache: Q1, H_1, ... (crates/cubbs-bbs-protocol/src/bbsplus/generators.rs:119-148). */
+/* Generator cache: Q1, H_1, ... (crates/cubbs-bbs-protocol/src/bbsplus/generators.rs:119-148).
+ * It only grows, under GEN_LOCK.  A growth copies the entries into new
+ * arrays and then publishes them; the previous arrays are kept, never freed,
+ * so a thread that is reading entries it made sure of (ensure_generators)
+ * through the pointers it loaded before the growth keeps reading valid
+ * memory.  Readers load the pointers with gen_pts() / gen_comp(). */
 static pthread_mutex_t GEN_LOCK = PTHREAD_MUTEX_INITIALIZER;
 static g1j *GEN_PTS = NULL;
 static uint8_t *GEN_COMP = NULL;
 static size_t GEN_COUNT = 0, GEN_CAP = 0;
 static uint8_t GEN_V[48];
+
+static const g1j *gen_pts(void) { return __atomic_load_n(&GEN_PTS, __ATOMIC_ACQUIRE); }
+static const uint8_t *gen_comp(void) { return __atomic_load_n(&GEN_COMP, __ATOMIC_ACQUIRE); }
 
 static int ensure_generators(size_t count) {
   pthread_mutex_lock(&GEN_LOCK);
@@ -1365,8 +1373,14 @@ static int ensure_generators(size_t count) {
   if (count > GEN_CAP) {
     size_t cap = GEN_CAP ? GEN_CAP : 64;
     while (cap < count) cap *= 2;
-    GEN_PTS = (g1j *)realloc(GEN_PTS, cap * sizeof(g1j));
-    GEN_COMP = (uint8_t *)realloc(GEN_COMP, cap * 48);
+    g1j *pts = (g1j *)malloc(cap * sizeof(g1j));
+    uint8_t *comp = (uint8_t *)malloc(cap * 48);
+    if (GEN_COUNT) {
+      memcpy(pts, GEN_PTS, GEN_COUNT * sizeof(g1j));
+      memcpy(comp, GEN_COMP, GEN_COUNT * 48);
+    }
+    __atomic_store_n(&GEN_PTS, pts, __ATOMIC_RELEASE);
+    __atomic_store_n(&GEN_COMP, comp, __ATOMIC_RELEASE);
     GEN_CAP = cap;
   }
   uint8_t seed_dst[128], gen_dst[128], seed[128];
@@ -1393,7 +1407,7 @@ static int ensure_generators(size_t count) {
 int orc_generators(size_t count, uint8_t *out) {
   orc_init();
   ensure_generators(count);
-  memcpy(out, GEN_COMP, 48 * count);
+  memcpy(out, gen_comp(), 48 * count);
   return ORC_OK;
 }
 int orc_p1(uint8_t out[48]) { orc_init(); memcpy(out, P1_BYTES, 48); return ORC_OK; }
@@ -1409,7 +1423,7 @@ static void calc_domain(fr *out, const uint8_t pk[96], size_t L, const uint8_t *
   memcpy(buf + o, pk, 96); o += 96;
   for (int k = 0; k < 8; k++) buf[o + k] = (uint8_t)((uint64_t)L >> (56 - 8 * k));
   o += 8;
-  memcpy(buf + o, GEN_COMP, 48 * (L + 1)); o += 48 * (L + 1);
+  memcpy(buf + o, gen_comp(), 48 * (L + 1)); o += 48 * (L + 1);
   memcpy(buf + o, API_ID, api_len); o += api_len;
   for (int k = 0; k < 8; k++) buf[o + k] = (uint8_t)((uint64_t)hlen >> (56 - 8 * k));
   o += 8;
@@ -1447,10 +1461,10 @@ static void msg_scalars(fr *out, const uint8_t *msgs, const size_t *lens, size_t
 /* B = P1 + Q1 * domain + sum H_i * m_i  (signature.rs:620-640) */
 static void compute_b(g1j *B, const fr *domain, const fr *m, size_t L) {
   g1j t;
-  g1_mul_fr(&t, &GEN_PTS[0], domain);
+  g1_mul_fr(&t, &gen_pts()[0], domain);
   g1j_add(B, &P1_PT, &t);
   for (size_t i = 0; i < L; i++) {
-    g1_mul_fr(&t, &GEN_PTS[i + 1], &m[i]);
+    g1_mul_fr(&t, &gen_pts()[i + 1], &m[i]);
     g1j_add(B, B, &t);
   }
 }
@@ -1637,7 +1651,7 @@ long orc_proof_gen(const uint8_t pk[96], const uint8_t *sig, size_t sig_len, con
     und[u++] = i;
   }
   for (size_t j = 0; j < U; j++) {
-    g1_mul_fr(&t, &GEN_PTS[und[j] + 1], &rs[5 + j]);
+    g1_mul_fr(&t, &gen_pts()[und[j] + 1], &rs[5 + j]);
     g1j_add(&T2, &T2, &t);
   }
   uint8_t comp[5][48];
@@ -1721,10 +1735,10 @@ int orc_proof_verify(const uint8_t *pk, size_t pk_len, const uint8_t *proof, siz
   g1_mul_fr(&t, &Abar, &ecap); g1j_add(&T1, &T1, &t);
   g1_mul_fr(&t, &D, &r1cap); g1j_add(&T1, &T1, &t);
   /* Bv = P1 + Q1*dom + sum_R H_i m_i ; T2 = Bv*c + D*r3^ + sum_U H_j m^_j */
-  g1_mul_fr(&t, &GEN_PTS[0], &dom);
+  g1_mul_fr(&t, &gen_pts()[0], &dom);
   g1j_add(&Bv, &P1_PT, &t);
   for (size_t i = 0; i < R; i++) {
-    g1_mul_fr(&t, &GEN_PTS[disclosed[i] + 1], &dm[i]);
+    g1_mul_fr(&t, &gen_pts()[disclosed[i] + 1], &dm[i]);
     g1j_add(&Bv, &Bv, &t);
   }
   g1_mul_fr(&T2, &Bv, &c);
@@ -1732,7 +1746,7 @@ int orc_proof_verify(const uint8_t *pk, size_t pk_len, const uint8_t *proof, siz
   size_t u = 0, d = 0;
   for (size_t i = 0; i < L; i++) {
     if (d < R && disclosed[d] == i) { d++; continue; }
-    g1_mul_fr(&t, &GEN_PTS[i + 1], &mcap[u]);
+    g1_mul_fr(&t, &gen_pts()[i + 1], &mcap[u]);
     g1j_add(&T2, &T2, &t);
     u++;
   }
