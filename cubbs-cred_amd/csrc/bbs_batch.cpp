@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -145,6 +146,8 @@ static void count_work(Batch& b, const Plan& p) {
   for (const auto& g : p.groups2) adds += g.count ? g.count - 1 : 0;
   for (const auto& lv : p.pst_groups)
     for (const auto& g : lv) adds += g.count ? g.count - 1 : 0;
+  b.ungated_point_adds = adds;
+  for (const auto& g : b.gated_work) adds += g.adds;
   w.point_adds = adds;
   w.group_checks = p.n_group_pairs;
   w.pair_checks = p.pairs.size() - p.n_group_pairs;
@@ -170,6 +173,17 @@ void materialize(Batch& b, Plan& p) {
   });
   // the same for comb chunks (a row's last chunk is usually short)
   sort_jobs(p.chunks, [](const CombChunk& c) { return std::min<uint32_t>(c.term_count, 1u << 16); });
+  if (!p.fb_chunks.empty()) {
+    std::vector<std::pair<CombChunk, uint32_t>> cg(p.fb_chunks.size());
+    for (size_t i = 0; i < cg.size(); i++) cg[i] = {p.fb_chunks[i], p.fb_gate[i]};
+    sort_jobs(cg, [](const std::pair<CombChunk, uint32_t>& c) {
+      return std::min<uint32_t>(c.first.term_count, 1u << 16);
+    });
+    for (size_t i = 0; i < cg.size(); i++) {
+      p.fb_chunks[i] = cg[i].first;
+      p.fb_gate[i] = cg[i].second;
+    }
+  }
   sort_groups(p.groups1, p.out1);
   std::vector<uint8_t> H;
   H.reserve(p.blob.size() + 4096);
@@ -195,7 +209,10 @@ void materialize(Batch& b, Plan& p) {
   size_t o_g2 = place(p.groups2.data(), p.groups2.size() * sizeof(ReduceGroup));
   size_t o_o2 = place(p.out2.data(), p.out2.size() * 4);
   size_t o_pair = place(p.pairs.data(), p.pairs.size() * sizeof(PairJob));
-  size_t o_pcp = place(p.pt_copy.data(), p.pt_copy.size() * sizeof(PointCopy));
+  size_t o_sp = place(p.sum_pairs.data(), p.sum_pairs.size() * 4);
+  size_t o_fbc = place(p.fb_chunks.data(), p.fb_chunks.size() * sizeof(CombChunk));
+  size_t o_fbg = place(p.fb_gate.data(), p.fb_gate.size() * 4);
+  size_t o_fbr = place(p.fb_red.data(), p.fb_red.size() * sizeof(GatedReduce));
   std::vector<std::pair<size_t, size_t>> o_lv;
   for (size_t l = 0; l < p.pst_groups.size(); l++)
     o_lv.emplace_back(place(p.pst_groups[l].data(), p.pst_groups[l].size() * sizeof(ReduceGroup)),
@@ -247,8 +264,12 @@ void materialize(Batch& b, Plan& p) {
   b.n_groups2 = (uint32_t)p.groups2.size();
   b.pairs = reinterpret_cast<PairJob*>(D + o_pair);
   b.n_pairs = (uint32_t)p.pairs.size();
-  b.pt_copy = reinterpret_cast<PointCopy*>(D + o_pcp);
-  b.n_pt_copy = (uint32_t)p.pt_copy.size();
+  b.sum_pairs = reinterpret_cast<uint32_t*>(D + o_sp);
+  b.fb_chunks = reinterpret_cast<CombChunk*>(D + o_fbc);
+  b.fb_gate = reinterpret_cast<uint32_t*>(D + o_fbg);
+  b.n_fb_chunks = (uint32_t)p.fb_chunks.size();
+  b.fb_red = reinterpret_cast<GatedReduce*>(D + o_fbr);
+  b.n_fb_red = (uint32_t)p.fb_red.size();
   b.n_agg_slots = p.n_agg_slots;
   b.n_spairs = p.n_super_pairs;
   b.n_gpairs = p.n_group_pairs - p.n_super_pairs;
@@ -277,10 +298,28 @@ void materialize(Batch& b, Plan& p) {
   b.tail_blocks = p.tail_blocks;
   b.seeded = p.seeded;
   b.seeds = D + o_blob + p.seeds_off;
+  auto chunk_madds = [&](const CombChunk& c) {
+    uint32_t m = 0;
+    for (uint32_t t = c.term_begin; t < c.term_begin + c.term_count; t++)
+      m += p.term_gen[t] < b.n16 ? COMB16_WINDOWS : COMB_WINDOWS;
+    return m;
+  };
   for (const auto& c : p.chunks) {
     b.comb_terms += c.term_count;
-    for (uint32_t t = c.term_begin; t < c.term_begin + c.term_count; t++)
-      b.comb_madds += p.term_gen[t] < b.n16 ? COMB16_WINDOWS : COMB_WINDOWS;
+    b.comb_madds += chunk_madds(c);
+  }
+  b.ungated_comb_terms = b.comb_terms;
+  b.ungated_comb_madds = b.comb_madds;
+  // gated work, counted as if every group failed until results say otherwise
+  for (size_t i = 0; i < p.fb_chunks.size(); i++) {
+    const uint32_t m = chunk_madds(p.fb_chunks[i]);
+    b.gated_work.push_back(Batch::GatedWork{p.fb_gate[i], p.fb_chunks[i].term_count, m, 0});
+    b.comb_terms += p.fb_chunks[i].term_count;
+    b.comb_madds += m;
+  }
+  for (const auto& r : p.fb_red) {
+    const uint32_t in = r.count + (r.extra != REDUCE_NO_EXTRA ? 1u : 0u);
+    b.gated_work.push_back(Batch::GatedWork{r.gate, 0, 0, in ? in - 1 : 0});
   }
   count_work(b, p);
   if (p.n_agg_slots) {
@@ -401,7 +440,7 @@ static void agg_coefficient(uint8_t out[32], uint64_t s, uint64_t t) {
     for (int j = 0; j < 8; j++) out[31 - 8 * k - j] = (uint8_t)(w[k] >> (8 * j));
 }
 
-void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, bool scaled, uint32_t super_max) {
+void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, uint32_t super_max) {
   if (group_max < 2 || group_max > 8) return;  // 2 x 8 partials per sum: one final reduction
   if (super_max < 2 || super_max > 8) super_max = 1;
   std::vector<PairJob> in = std::move(p.pairs);
@@ -414,8 +453,8 @@ void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, bool scaled, uint
     if (px != py) return px;
     return px && in[x].pk_slot < in[y].pk_slot;
   });
-  std::vector<uint8_t> rnd(scaled ? 0 : 16 * in.size());
-  if (!scaled) fill_os_random(rnd.data(), rnd.size());
+  std::vector<uint8_t> rnd(16 * in.size());
+  fill_os_random(rnd.data(), rnd.size());
   std::vector<PairJob> supers, groups, grouped, single;
   std::vector<ReduceGroup> super_red;
   std::vector<uint32_t> super_out;
@@ -430,15 +469,7 @@ void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, bool scaled, uint
       const size_t first = run[c0].first;
       // the partials of the chunk's a points, then of its b points, contiguous
       uint32_t fa, fb;
-      if (scaled) {
-        // the checks' points are c_i a_i and c_i b_i already
-        fa = p.partials(total);
-        fb = p.partials(total);
-        for (uint32_t k = 0; k < total; k++) {
-          p.pt_copy.push_back(PointCopy{in[order[first + k]].a, fa + k});
-          p.pt_copy.push_back(PointCopy{in[order[first + k]].b, fb + k});
-        }
-      } else {
+      {
         std::vector<uint32_t> sc(total);
         for (uint32_t k = 0; k < total; k++) {
           uint64_t s0, t0;
@@ -579,6 +610,201 @@ void attach(Batch& b, uint32_t max_gen_slots, const std::vector<const uint8_t*>&
 static std::string map_dst() { return std::string(kApiId) + "MAP_MSG_TO_SCALAR_AS_HASH_"; }
 
 // ------------------------------------------------------------------ Verify
+// Aggregated Verify.  Every item's check is scaled by its coefficient c
+// (s + t lambda, s and t 64-bit from the OS CSPRNG, s odd so c != 0), as
+// e(c A, PK) e(c (B - e A), -G2) = 1, which holds exactly when the item's own
+// check does (c is invertible mod r).  The items of one key are grouped,
+// ordered by message count, and a group of items i checks
+//   e(sum c_i A_i, PK) e(sum c_i B_i - sum c_i e_i A_i, -G2) = 1.
+// B_i = P1 + Q1 dom_i + sum_j H_j m_ij is linear in its scalars, so the
+// group's sum c_i B_i is ONE fixed-base MSM over summed term scalars:
+//   P1 sum c_i + Q1 sum c_i dom_i + sum_j H_j (sum over i with L_i >= j of c_i m_ij),
+// max L_i + 2 terms for the group instead of sum (L_i + 2).  Per item there
+// remain the products c A (64-bit GLV halves) and (-c e) A.  An item's own
+// terms c, c dom, c m_j are filled as well, but its comb MSM, and the sums
+// forming its check points, run only when its group check fails (gated on
+// the group's status slot, after the group checks).  A key's last item, when
+// it is left alone, is checked on its own throughout.  Supergroups (runs of
+// up to super_max groups of a key) are checked before their groups.
+static void plan_verify_grouped(Batch& b, const CuBbsBbsGpuVerifyItem* items,
+                                const std::vector<size_t>& dev, const std::vector<uint32_t>& pk_slot,
+                                Plan& p, uint32_t CH, uint32_t group_max, uint32_t super_max) {
+  const uint32_t nv = b.nv;
+  if (super_max < 2 || super_max > 8) super_max = 1;
+  std::vector<uint32_t> Ls(nv);
+  for (uint32_t k = 0; k < nv; k++) Ls[k] = (uint32_t)items[dev[k]].message_count;
+  // groups: the items of one key ordered by message count, in runs of at
+  // most group_max (a run of one is a single check)
+  std::vector<uint32_t> order(nv);
+  for (uint32_t k = 0; k < nv; k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    if (pk_slot[x] != pk_slot[y]) return pk_slot[x] < pk_slot[y];
+    return Ls[x] < Ls[y];
+  });
+  struct Run {
+    uint32_t begin, end;  // into order[]
+  };
+  std::vector<std::vector<Run>> key_runs;  // per key, its groups
+  std::vector<uint32_t> singles;
+  for (uint32_t i = 0; i < nv;) {
+    uint32_t e = i;
+    while (e < nv && pk_slot[order[e]] == pk_slot[order[i]]) e++;
+    key_runs.emplace_back();
+    for (uint32_t r = i; r < e; r += group_max) {
+      const uint32_t re = std::min(e, r + group_max);
+      if (re - r >= 2)
+        key_runs.back().push_back(Run{r, re});
+      else
+        singles.push_back(order[r]);
+    }
+    i = e;
+  }
+  // term slots first: [0, T) -- the groups' summed terms, then every item's
+  // own terms
+  uint32_t T = 0;
+  for (const auto& runs : key_runs)
+    for (const Run& r : runs) T += Ls[order[r.end - 1]] + 2;
+  for (uint32_t k = 0; k < nv; k++) T += Ls[k] + 2;
+  p.scalars(T);
+  p.term_gen.resize(T);
+  uint32_t term = 0;
+  // inputs of every item: A, c, r - c e, the domain and message scalars
+  std::vector<uint8_t> rnd(16ull * nv);
+  fill_os_random(rnd.data(), rnd.size());
+  std::vector<uint32_t> pa(nv), sc(nv), se(nv), raw(nv);
+  for (uint32_t k = 0; k < nv; k++) {
+    const auto& it = items[dev[k]];
+    const uint32_t L = Ls[k];
+    const size_t sig = p.bytes(it.signature.data, 80);
+    const size_t hdr = p.bytes(it.header.data, it.header.len);
+    pa[k] = p.points(1);
+    p.dec_g1.push_back(DecodeG1Job{sig, pa[k], k});
+    uint64_t s0, t0;
+    std::memcpy(&s0, &rnd[16ull * k], 8);
+    std::memcpy(&t0, &rnd[16ull * k + 8], 8);
+    uint8_t c[32];
+    agg_coefficient(c, s0 | 1, t0);
+    const size_t c_off = p.bytes(c, 32);
+    sc[k] = p.scalars(1);
+    se[k] = p.scalars(1);
+    raw[k] = p.scalars(L + 1);
+    p.dec_fr.push_back(DecodeFrJob{c_off, sc[k], k, 0u, 0u});
+    p.dec_fr.push_back(DecodeFrJob{sig + 48, se[k], k, 1u | 4u, 0u, c_off});  // r - c e
+    p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, raw[k]});
+    for (uint32_t m = 0; m < L; m++) p.hash_message(it.messages[m].data, it.messages[m].len, raw[k] + 1 + m);
+  }
+  // an item's own terms: c, c dom, c m_1 .. c m_L
+  auto own_terms = [&](uint32_t k) {
+    const uint32_t t0 = term, L = Ls[k];
+    for (uint32_t t = 0; t < L + 2; t++) p.term_gen[t0 + t] = t;
+    p.fills.push_back(TermFill{t0, sc[k], 0, 0});
+    for (uint32_t t = 0; t <= L; t++) p.fills.push_back(TermFill{t0 + 1 + t, raw[k] + t, sc[k], 1});
+    term += L + 2;
+    return t0;
+  };
+  std::vector<PairJob> supers, groups, grouped, single;
+  std::vector<std::pair<uint32_t, uint32_t>> xy;
+  for (const auto& runs : key_runs) {
+    for (size_t c0 = 0; c0 < runs.size(); c0 += super_max) {
+      const size_t c1 = std::min(runs.size(), c0 + super_max);
+      const bool sup = c1 - c0 >= 2;
+      const uint32_t first = runs[c0].begin, total = runs[c1 - 1].end - first;
+      const uint32_t pk = pk_slot[order[first]];
+      uint32_t sslot = 0;
+      if (sup) sslot = nv + p.n_agg_slots++;
+      // partials: c_i A_i of every item of the chunk, then per group the
+      // (-c_i e_i) A_i of its items followed by its comb partials
+      const uint32_t fx = p.partials(total);
+      for (uint32_t m = 0; m < total; m++) {
+        const uint32_t k = order[first + m];
+        p.agg_var.push_back(VarJob{pa[k], sc[k], fx + m});
+      }
+      const uint32_t ybegin = p.n_partials;
+      std::vector<uint32_t> vb(c1 - c0), vend(c1 - c0), gslot(c1 - c0);
+      for (size_t r = c0; r < c1; r++) {
+        const Run& run = runs[r];
+        const uint32_t cnt = run.end - run.begin;
+        vb[r - c0] = p.partials(cnt);
+        for (uint32_t m = 0; m < cnt; m++) {
+          const uint32_t k = order[run.begin + m];
+          p.var.push_back(VarJob{pa[k], se[k], vb[r - c0] + m});
+        }
+        uint32_t lmax = 0;
+        for (uint32_t m = run.begin; m < run.end; m++) lmax = std::max(lmax, Ls[order[m]]);
+        const uint32_t gt = term;
+        term += lmax + 2;
+        for (uint32_t t = 0; t < lmax + 2; t++) {
+          p.term_gen[gt + t] = t;
+          xy.clear();
+          for (uint32_t m = run.begin; m < run.end; m++) {
+            const uint32_t k = order[m];
+            if (t == 0)
+              xy.push_back({sc[k], TERM_SUM_ONE});                 // P1: sum c_i
+            else if (t - 1 <= Ls[k])
+              xy.push_back({raw[k] + t - 1, sc[k]});               // Q1, H_j: sum c_i x_i
+          }
+          p.term_sum(gt + t, xy);
+        }
+        p.comb(gt, lmax + 2, CH);
+        vend[r - c0] = p.n_partials;
+        gslot[r - c0] = nv + p.n_agg_slots++;
+      }
+      const uint32_t yend = p.n_partials;
+      // the sums, once the chunk's partials are all allocated (a long sum's
+      // tree levels take partial slots after them)
+      for (size_t r = c0; r < c1; r++) {
+        const Run& run = runs[r];
+        const uint32_t la = p.points(1), lb = p.points(1);
+        p.reduce_to(fx + (run.begin - first), run.end - run.begin, la);
+        p.reduce_to(vb[r - c0], vend[r - c0] - vb[r - c0], lb);
+        groups.push_back(PairJob{gslot[r - c0], la, lb, pk, PAIR_NEG_G2, sup ? PAIR_GROUPED : 0u, sslot});
+      }
+      if (sup) {
+        const uint32_t la = p.points(1), lb = p.points(1);
+        p.reduce_to(fx, total, la);
+        p.reduce_to(ybegin, yend - ybegin, lb);
+        supers.push_back(PairJob{sslot, la, lb, pk, PAIR_NEG_G2, 0, 0});
+      }
+      // the items' own checks, for when their group check fails: comb
+      // chunks as in the main launch (they are on the critical path of
+      // the batch), one gated sum per check point
+      for (size_t r = c0; r < c1; r++) {
+        const Run& run = runs[r];
+        for (uint32_t m = run.begin; m < run.end; m++) {
+          const uint32_t k = order[m], L = Ls[k];
+          const uint32_t ft = own_terms(k);
+          const uint32_t g = gslot[r - c0];
+          const uint32_t cb = p.comb_gated(ft, L + 2, CH, g);
+          const uint32_t pa2 = p.points(1), pb = p.points(1);
+          p.fb_red.push_back(GatedReduce{fx + (m - first), 1, REDUCE_NO_EXTRA, g, pa2});   // c A
+          p.fb_red.push_back(GatedReduce{cb, p.n_partials - cb, vb[r - c0] + (m - run.begin), g, pb});
+          grouped.push_back(PairJob{k, pa2, pb, pk, PAIR_NEG_G2, PAIR_GROUPED, g});
+        }
+      }
+    }
+  }
+  for (uint32_t k : singles) {
+    const uint32_t L = Ls[k];
+    const uint32_t st = own_terms(k);
+    const uint32_t pair_a = p.points(1), pb = p.points(1);
+    const uint32_t fx = p.partials(1);
+    p.agg_var.push_back(VarJob{pa[k], sc[k], fx});
+    p.reduce_to(fx, 1, pair_a);                               // c A
+    const uint32_t first = p.n_partials;
+    p.var.push_back(VarJob{pa[k], se[k], p.partials(1)});     // (-c e) A
+    p.comb(st, L + 2, CH);                                    // c B
+    p.reduce_to(first, p.n_partials - first, pb);             // c (B - e A)
+    single.push_back(PairJob{k, pair_a, pb, pk_slot[k], PAIR_NEG_G2, 0, 0});
+  }
+  if (term != T) throw std::logic_error("grouped verify plan: term slots do not add up");
+  p.n_super_pairs = (uint32_t)supers.size();
+  p.n_group_pairs = (uint32_t)(supers.size() + groups.size());
+  p.pairs = std::move(supers);
+  p.pairs.insert(p.pairs.end(), groups.begin(), groups.end());
+  p.pairs.insert(p.pairs.end(), grouped.begin(), grouped.end());
+  p.pairs.insert(p.pairs.end(), single.begin(), single.end());
+}
+
 std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n) {
   auto b = std::make_unique<Batch>();
   b->kind = CUBBS_HIP_BATCH_VERIFY;
@@ -612,22 +838,17 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
   std::string d = map_dst();
   p.map_dst.assign(d.begin(), d.end());
   const uint32_t CH = pipeline_comb_chunk(0, n);
+  const uint32_t group = agg_group_max();
+  if (group >= 2 && group <= 8) {
+    plan_verify_grouped(*b, items, dev, pk_slot, p, CH, group, agg_super_max());
+    materialize(*b, p);
+    return b;
+  }
   // terms first: [0, T)
   uint32_t T = 0;
   for (uint32_t k = 0; k < b->nv; k++) T += (uint32_t)items[dev[k]].message_count + 2;
   p.scalars(T);
   p.term_gen.resize(T);
-  // Aggregated checks: every item's check is scaled by its coefficient c
-  // (s + t lambda, s and t 64-bit from the OS CSPRNG, s odd so c != 0), as
-  // e(c A, PK) e(c (B - e A), -G2) = 1, which holds exactly when the item's
-  // own check does (c is invertible mod r).  c (B - e A) comes from the comb
-  // over the scaled term scalars and the product (-c e) A, so the only
-  // extra work per item is the 64-bit-half product c A; the group check adds
-  // the scaled points.
-  const uint32_t group = agg_group_max();
-  const bool scaled = group >= 2;
-  std::vector<uint8_t> rnd(scaled ? 16ull * b->nv : 0);
-  if (scaled) fill_os_random(rnd.data(), rnd.size());
   uint32_t term = 0;
   for (uint32_t k = 0; k < b->nv; k++) {
     const auto& it = items[dev[k]];
@@ -637,45 +858,19 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
     uint32_t pa = p.points(1), pb = p.points(1), se = p.scalars(1);
     p.dec_g1.push_back(DecodeG1Job{sig, pa, k});
     for (uint32_t t = 0; t < L + 2; t++) p.term_gen[term + t] = t;
-    uint32_t pair_a = pa;
-    if (!scaled) {
-      p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, term + 1});
-      for (uint32_t m = 0; m < L; m++) {
-        p.hash_message(it.messages[m].data, it.messages[m].len, term + 2 + m);
-      }
-      p.fills.push_back(TermFill{term, 0, 0, 2});               // P1 scalar = 1
-      p.dec_fr.push_back(DecodeFrJob{sig + 48, se, k, 1u, 0u});  // store r - e
-    } else {
-      uint64_t s0, t0;
-      std::memcpy(&s0, &rnd[16ull * k], 8);
-      std::memcpy(&t0, &rnd[16ull * k + 8], 8);
-      uint8_t c[32];
-      agg_coefficient(c, s0 | 1, t0);
-      const size_t c_off = p.bytes(c, 32);
-      const uint32_t sc = p.scalars(1), raw = p.scalars(L + 1);
-      p.dec_fr.push_back(DecodeFrJob{c_off, sc, k, 0u, 0u});
-      // domain and message scalars land in raw slots, scaled into the terms
-      p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, raw});
-      for (uint32_t m = 0; m < L; m++) {
-        p.hash_message(it.messages[m].data, it.messages[m].len, raw + 1 + m);
-      }
-      p.fills.push_back(TermFill{term, sc, 0, 0});  // P1 scalar = c
-      for (uint32_t t = 0; t <= L; t++) p.fills.push_back(TermFill{term + 1 + t, raw + t, sc, 1});
-      p.dec_fr.push_back(DecodeFrJob{sig + 48, se, k, 1u | 4u, 0u, c_off});  // r - c e
-      // c A
-      pair_a = p.points(1);
-      const uint32_t fx = p.partials(VAR_PARTIALS);
-      p.agg_var.push_back(VarJob{pa, sc, fx});
-      p.reduce_to(fx, VAR_PARTIALS, pair_a);
+    p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, term + 1});
+    for (uint32_t m = 0; m < L; m++) {
+      p.hash_message(it.messages[m].data, it.messages[m].len, term + 2 + m);
     }
+    p.fills.push_back(TermFill{term, 0, 0, 2});               // P1 scalar = 1
+    p.dec_fr.push_back(DecodeFrJob{sig + 48, se, k, 1u, 0u});  // store r - e
     uint32_t first = p.n_partials;
-    p.var.push_back(VarJob{pa, se, p.partials(VAR_PARTIALS)});  // (-e) A, or (-c e) A
-    p.comb(term, L + 2, CH);                                  // B, or c B
-    p.reduce_to(first, p.n_partials - first, pb);             // B - e A, or c (B - e A)
-    p.pairs.push_back(PairJob{k, pair_a, pb, pk_slot[k], PAIR_NEG_G2, 0});
+    p.var.push_back(VarJob{pa, se, p.partials(VAR_PARTIALS)});  // (-e) A
+    p.comb(term, L + 2, CH);                                  // B
+    p.reduce_to(first, p.n_partials - first, pb);             // B - e A
+    p.pairs.push_back(PairJob{k, pa, pb, pk_slot[k], PAIR_NEG_G2, 0});
     term += L + 2;
   }
-  aggregate_pairs(p, b->nv, group, scaled, agg_super_max());
   materialize(*b, p);
   return b;
 }
@@ -806,7 +1001,7 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
     p.pairs.push_back(PairJob{k, pab, pbb, pk_slot[k], PAIR_NEG_G2, 0});
     term += L + 2;
   }
-  aggregate_pairs(p, b->nv, agg_group_max(), false, agg_super_max());
+  aggregate_pairs(p, b->nv, agg_group_max(), agg_super_max());
   materialize(*b, p);
   return b;
 }
@@ -1136,7 +1331,7 @@ void launch_batch(Batch& b) {
   }
   if (b.n_fills) {
     Timed t(b, "term_fill");
-    launch_term_fill(s, b.fills, b.n_fills, b.scalars);
+    launch_term_fill(s, b.fills, b.n_fills, b.scalars, b.sum_pairs);
   }
   if (b.kind == KIND_SIGN) {
     Timed t(b, "sign_e");
@@ -1231,12 +1426,8 @@ void launch_batch(Batch& b) {
                         b.pinf, b.scalars, b.gate);
   }
   if (b.n_agg_slots) {
-    // sums c_i a_i, c_i b_i of each aggregated group: of the products
-    // computed in the product launch, or of the pre-scaled checks' points
-    if (b.n_pt_copy) {
-      Timed t(b, "agg_copy");
-      launch_point_to_partial(s, b.pt_copy, b.n_pt_copy, b.points, b.pinf, b.partials);
-    }
+    // sums c_i a_i, c_i b_i of each aggregated group (ProofVerify; the
+    // grouped Verify plan sums its groups in the main reduction)
     Timed t(b, "agg_reduce");
     launch_reduce(s, b.groups2, b.out2, b.n_groups2, b.partials, b.points, b.pinf);
   }
@@ -1262,6 +1453,16 @@ void launch_batch(Batch& b) {
       launch_pairing(s, b.pairs + done, lv.n, b.points, b.pinf, b.pk_status,
                      b.l2buf + (size_t)done * PAIR_L2_WORDS, b.status, b.gate, b.valid);
       done += lv.n;
+    }
+    // the items of failing groups: their own comb terms and check points
+    if (b.n_fb_chunks) {
+      Timed t(b, "comb_gated");
+      launch_comb_msm(s, b.fb_chunks, b.n_fb_chunks, b.term_gen, b.scalars, b.tables, b.tables16,
+                      b.n16, b.partials, b.fb_gate, b.valid);
+    }
+    if (b.n_fb_red) {
+      Timed t(b, "reduce_gated");
+      launch_reduce_gated(s, b.fb_red, b.n_fb_red, b.partials, b.points, b.pinf, b.valid);
     }
     {
       Timed t(b, "pair_lines");
@@ -1296,6 +1497,17 @@ void fetch_results(Batch& b) {
       run += g == PRIM_NO_SLOT || b.h_res[b.res_stride + g] != 1;
     }
     b.work.pair_checks_run = run;
+    // the gated comb chunks and sums ran for the items of failing groups
+    uint64_t terms = b.ungated_comb_terms, madds = b.ungated_comb_madds, adds = b.ungated_point_adds;
+    for (const auto& g : b.gated_work) {
+      if (b.h_res[b.res_stride + g.gate] == 1) continue;
+      terms += g.terms;
+      madds += g.madds;
+      adds += g.adds;
+    }
+    b.work.comb_terms = terms;
+    b.work.comb_madds = madds;
+    b.work.point_adds = adds;
   }
 }
 

@@ -112,12 +112,25 @@ struct Batch {
   ReduceGroup* groups2 = nullptr;
   uint32_t* out2 = nullptr;
   PairJob* pairs = nullptr;
-  PointCopy* pt_copy = nullptr;       // affine inputs of the aggregated group sums
   // pairs[]: n_spairs supergroup checks, then n_gpairs group checks, then
   // the per-item checks
-  uint32_t n_pt_copy = 0, n_agg_slots = 0, n_gpairs = 0, n_spairs = 0;
+  uint32_t n_agg_slots = 0, n_gpairs = 0, n_spairs = 0;
   uint32_t agg_passed = 0;  // group checks that held at the last fetch of results
   std::vector<uint32_t> pair_group;   // checks after the supergroups': slot of the enclosing check (or PRIM_NO_SLOT)
+  // summed term fills (TermFill mode 3) and the work gated on the group
+  // checks: comb chunks and sums of the items of failing groups
+  uint32_t* sum_pairs = nullptr;
+  CombChunk* fb_chunks = nullptr;
+  uint32_t* fb_gate = nullptr;
+  GatedReduce* fb_red = nullptr;
+  uint32_t n_fb_chunks = 0, n_fb_red = 0;
+  // per gated job, for the work counts of a launch: its gate slot and the
+  // comb terms, mixed additions and point additions it runs
+  struct GatedWork {
+    uint32_t gate, terms, madds, adds;
+  };
+  std::vector<GatedWork> gated_work;
+  uint64_t ungated_comb_terms = 0, ungated_comb_madds = 0, ungated_point_adds = 0;
   struct Level {
     ReduceGroup* groups;
     uint32_t* out;

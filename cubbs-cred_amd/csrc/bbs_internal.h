@@ -110,13 +110,6 @@ struct DecodeFrJob {
   uint64_t mul_off;
 };
 
-// partials[partial] = points[point] as a Jacobian point (the identity when
-// pinf[point]): feeds affine results of one reduction into another.
-struct PointCopy {
-  uint32_t point;
-  uint32_t partial;
-};
-
 // partials[partial] = scalars[scalar] * points[point] (identity-aware), on
 // one lane: the GLV split k = k1 + k2 lambda evaluated as k1 P + k2 phi(P)
 // with shared doublings and signed 4-bit windows (Straus), over two tables of
@@ -131,12 +124,28 @@ struct VarJob {
 };
 
 // Fill one fixed-base term scalar: mode 0: scalars[dst] = scalars[a];
-// mode 1: scalars[dst] = scalars[a] * scalars[b] mod r; mode 2: scalars[dst] = 1.
+// mode 1: scalars[dst] = scalars[a] * scalars[b] mod r; mode 2: scalars[dst] = 1;
+// mode 3: scalars[dst] = sum over k in [a, a + b) of scalars[x_k] * scalars[y_k]
+// mod r, where (x_k, y_k) = (sum_pairs[2k], sum_pairs[2k + 1]) and y_k =
+// TERM_SUM_ONE stands for the factor 1.
+constexpr uint32_t TERM_SUM_ONE = 0xffffffffu;
 struct TermFill {
   uint32_t dst;
   uint32_t a;
   uint32_t b;
   uint32_t mode;
+};
+
+// points[out] = partials[extra] (unless extra == REDUCE_NO_EXTRA) + the sum
+// of partials [begin, begin + count), computed only when valid[gate] == 0:
+// the sums that a per-item pairing check needs after its group check failed.
+constexpr uint32_t REDUCE_NO_EXTRA = 0xffffffffu;
+struct GatedReduce {
+  uint32_t begin;
+  uint32_t count;
+  uint32_t extra;
+  uint32_t gate;
+  uint32_t out;
 };
 
 // e(points[a], PK) * e(points[b], -G2) == 1, and gate[item] must be set.
@@ -229,21 +238,23 @@ void launch_comb_build(hipStream_t s, uint32_t first, uint32_t count, const uint
 // scratch: count * COMB16_WINDOWS * COMB16_ENTRIES * COMB_ENTRY_WORDS words
 void launch_comb16_build(hipStream_t s, uint32_t first, uint32_t count, const uint32_t* tables,
                          uint32_t* tables16, uint32_t* scratch);
-// Terms whose generator slot is below n16 use the wide tables.
+// Terms whose generator slot is below n16 use the wide tables.  With
+// gate_slot, chunk i is skipped when valid[gate_slot[i]] != 0.
 void launch_comb_msm(hipStream_t s, const CombChunk* chunks, uint32_t n, const uint32_t* term_gen,
                      const uint32_t* scalars, const uint32_t* tables, const uint32_t* tables16,
-                     uint32_t n16, uint32_t* partials);
+                     uint32_t n16, uint32_t* partials, const uint32_t* gate_slot = nullptr,
+                     const uint8_t* valid = nullptr);
 // groups[i] sums into points[out_point[i]]
 void launch_reduce(hipStream_t s, const ReduceGroup* groups, const uint32_t* out_point, uint32_t n,
                    const uint32_t* partials, uint32_t* points, uint8_t* pinf);
+void launch_reduce_gated(hipStream_t s, const GatedReduce* jobs, uint32_t n, const uint32_t* partials,
+                         uint32_t* points, uint8_t* pinf, const uint8_t* valid);
 void launch_prepare_pk(hipStream_t s, const uint32_t* slots, uint32_t n, const uint8_t* pk_bytes,
                        uint8_t* pk_status, uint32_t* pk_lines);
 void launch_prepare_neg_g2(hipStream_t s, uint32_t* lines);
 // check_subgroup = false: launch_var_mul's extra lanes check the subgroup
 void launch_decode_g1(hipStream_t s, const DecodeG1Job* jobs, uint32_t n, const uint8_t* blob,
                       uint32_t* points, uint8_t* pinf, uint8_t* status, bool check_subgroup = true);
-void launch_point_to_partial(hipStream_t s, const PointCopy* jobs, uint32_t n, const uint32_t* points,
-                             const uint8_t* pinf, uint32_t* partials);
 void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const uint8_t* blob,
                       uint32_t* scalars, uint8_t* status);
 // ws: n * VAR_WS_WORDS words of device workspace.  n_sub > 0: also check
@@ -251,7 +262,8 @@ void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const 
 void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_t* points,
                     const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials, uint32_t* ws,
                     const DecodeG1Job* sub = nullptr, uint32_t n_sub = 0, uint8_t* status = nullptr);
-void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t* scalars);
+void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t* scalars,
+                      const uint32_t* sum_pairs = nullptr);
 // l2buf: PAIR_L2_WORDS words of device scratch per job, filled by
 // launch_pair_lines and read by launch_pairing: per line and pairing the 9
 // second factors of the sparse line product (68 x 2 x 9 x 14 words)

@@ -415,9 +415,12 @@ __global__ __launch_bounds__(128) void k_comb_msm(const CombChunk* __restrict__ 
                                                  const uint32_t* __restrict__ term_scalar,
                                                  const uint32_t* __restrict__ tables,
                                                  const uint32_t* __restrict__ tables16, uint32_t n16,
-                                                 uint32_t* __restrict__ partials) {
+                                                 uint32_t* __restrict__ partials,
+                                                 const uint32_t* __restrict__ gate_slot,
+                                                 const uint8_t* __restrict__ valid) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (gate_slot && valid[gate_slot[i]]) return;
   const CombChunk c = chunks[i];
   G1Z acc;
   g1z_set_inf(acc);
@@ -470,10 +473,11 @@ __global__ __launch_bounds__(128) void k_comb_msm(const CombChunk* __restrict__ 
 }
 void launch_comb_msm(hipStream_t s, const CombChunk* chunks, uint32_t n, const uint32_t* term_gen,
                      const uint32_t* term_scalar, const uint32_t* tables, const uint32_t* tables16,
-                     uint32_t n16, uint32_t* partials) {
+                     uint32_t n16, uint32_t* partials, const uint32_t* gate_slot,
+                     const uint8_t* valid) {
   if (!n) return;
   hipLaunchKernelGGL(k_comb_msm, GRID(n, 64), 0, s, chunks, n, term_gen, term_scalar, tables,
-                     tables16, n16, partials);
+                     tables16, n16, partials, gate_slot, valid);
 }
 
 // ------------------------------------------------------------------ public keys

@@ -120,29 +120,6 @@ __global__ __launch_bounds__(256) void k_decode_fr(const DecodeFrJob* __restrict
   store_fr(scalars + (size_t)j.out * FR_WORDS, s);
 }
 
-__global__ __launch_bounds__(256) void k_point_to_partial(const PointCopy* __restrict__ jobs,
-                                                         uint32_t n,
-                                                         const uint32_t* __restrict__ points,
-                                                         const uint8_t* __restrict__ pinf,
-                                                         uint32_t* __restrict__ partials) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const PointCopy j = jobs[i];
-  G1J r;
-  if (pinf[j.point]) {
-    pt_set_inf(r);
-  } else {
-    G1A a;
-    load_g1a(a, points + (size_t)j.point * G1A_WORDS);
-    pt_from_affine(r, a);
-  }
-  store_g1j(partials + (size_t)j.partial * G1J_WORDS, r);
-}
-void launch_point_to_partial(hipStream_t s, const PointCopy* jobs, uint32_t n, const uint32_t* points,
-                             const uint8_t* pinf, uint32_t* partials) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_point_to_partial, GRID(n, 64), 0, s, jobs, n, points, pinf, partials);
-}
 void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const uint8_t* blob,
                       uint32_t* scalars, uint8_t* status) {
   if (!n) return;
@@ -151,7 +128,8 @@ void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const 
 
 // ------------------------------------------------------------------ term scalars
 __global__ __launch_bounds__(256) void k_term_fill(const TermFill* __restrict__ jobs, uint32_t n,
-                                                  uint32_t* __restrict__ scalars) {
+                                                  uint32_t* __restrict__ scalars,
+                                                  const uint32_t* __restrict__ sum_pairs) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   TermFill j = jobs[i];
@@ -159,6 +137,20 @@ __global__ __launch_bounds__(256) void k_term_fill(const TermFill* __restrict__ 
   if (j.mode == 2) {
     fr_zero(r);
     r.v[0] = 1;
+  } else if (j.mode == 3) {
+    fr_zero(r);
+    for (uint32_t k = j.a; k < j.a + j.b; k++) {
+      const uint32_t x = sum_pairs[2 * k], y = sum_pairs[2 * k + 1];
+      Fr t;
+      load_fr(t, scalars + (size_t)x * FR_WORDS);
+      if (y != TERM_SUM_ONE) {
+        Fr m;
+        load_fr(m, scalars + (size_t)y * FR_WORDS);
+        fr_to_mont(m, m);
+        fr_mul(t, t, m);
+      }
+      fr_add(r, r, t);
+    }
   } else {
     load_fr(r, scalars + (size_t)j.a * FR_WORDS);
     if (j.mode == 1) {
@@ -170,9 +162,10 @@ __global__ __launch_bounds__(256) void k_term_fill(const TermFill* __restrict__ 
   }
   store_fr(scalars + (size_t)j.dst * FR_WORDS, r);
 }
-void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t* scalars) {
+void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t* scalars,
+                      const uint32_t* sum_pairs) {
   if (!n) return;
-  hipLaunchKernelGGL(k_term_fill, GRID(n, 64), 0, s, jobs, n, scalars);
+  hipLaunchKernelGGL(k_term_fill, GRID(n, 64), 0, s, jobs, n, scalars, sum_pairs);
 }
 
 // ------------------------------------------------------------------ variable-base products
@@ -356,6 +349,21 @@ void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_
 }
 
 // ------------------------------------------------------------------ reduction
+// points[o] = acc in affine coordinates (pinf[o] marks the identity).
+DEV void store_affine_sum(const G1J& acc, uint32_t o, uint32_t* points, uint8_t* pinf) {
+  G1A a;
+  if (pt_is_inf(acc)) {
+    pinf[o] = 1;
+    fp_zero(a.x);
+    fp_zero(a.y);
+  } else {
+    pinf[o] = 0;
+    G1J t = acc;  // the out-of-line conversion takes a copy's address, not acc's
+    pt_to_affine(a, t);
+  }
+  store_g1a(points + (size_t)o * G1A_WORDS, a);
+}
+
 __global__ __launch_bounds__(64) void k_reduce(const ReduceGroup* __restrict__ groups,
                                               const uint32_t* __restrict__ out_point, uint32_t n,
                                               const uint32_t* __restrict__ partials,
@@ -371,23 +379,40 @@ __global__ __launch_bounds__(64) void k_reduce(const ReduceGroup* __restrict__ g
     load_g1j(p, partials + (size_t)(g.begin + k) * G1J_WORDS);
     g1_add_in(acc, p);
   }
-  const uint32_t o = out_point[i];
-  G1A a;
-  if (pt_is_inf(acc)) {
-    pinf[o] = 1;
-    fp_zero(a.x);
-    fp_zero(a.y);
-  } else {
-    pinf[o] = 0;
-    G1J t = acc;  // the out-of-line conversion takes a copy's address, not acc's
-    pt_to_affine(a, t);
-  }
-  store_g1a(points + (size_t)o * G1A_WORDS, a);
+  store_affine_sum(acc, out_point[i], points, pinf);
 }
 void launch_reduce(hipStream_t s, const ReduceGroup* groups, const uint32_t* out_point, uint32_t n,
                    const uint32_t* partials, uint32_t* points, uint8_t* pinf) {
   if (!n) return;
   hipLaunchKernelGGL(k_reduce, GRID(n, 64), 0, s, groups, out_point, n, partials, points, pinf);
+}
+
+// The sums of the per-item checks of aggregated groups: only for the items of
+// groups whose combined check failed (valid[gate] == 0, written by the group
+// pairing launch).
+__global__ __launch_bounds__(64) void k_reduce_gated(const GatedReduce* __restrict__ jobs, uint32_t n,
+                                                    const uint32_t* __restrict__ partials,
+                                                    uint32_t* __restrict__ points,
+                                                    uint8_t* __restrict__ pinf,
+                                                    const uint8_t* __restrict__ valid) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const GatedReduce g = jobs[i];
+  if (valid[g.gate]) return;
+  G1J acc;
+  pt_set_inf(acc);
+  if (g.extra != REDUCE_NO_EXTRA) load_g1j(acc, partials + (size_t)g.extra * G1J_WORDS);
+  for (uint32_t k = 0; k < g.count; k++) {
+    G1J p;
+    load_g1j(p, partials + (size_t)(g.begin + k) * G1J_WORDS);
+    g1_add_in(acc, p);
+  }
+  store_affine_sum(acc, g.out, points, pinf);
+}
+void launch_reduce_gated(hipStream_t s, const GatedReduce* jobs, uint32_t n, const uint32_t* partials,
+                         uint32_t* points, uint8_t* pinf, const uint8_t* valid) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_reduce_gated, GRID(n, 64), 0, s, jobs, n, partials, points, pinf, valid);
 }
 
 // One level of a partial-sum tree: each lane adds a short run of Jacobian

@@ -45,8 +45,15 @@ struct Plan {
   // groups, and how many of pairs[] (at its front) are combined checks: the
   // supergroup checks first (n_super_pairs), then the group checks
   std::vector<VarJob> agg_var;       // run in the same launch as `var`, after it
-  std::vector<PointCopy> pt_copy;    // affine inputs of the group sums (pre-scaled checks)
   uint32_t n_agg_slots = 0, n_group_pairs = 0, n_super_pairs = 0;
+  // (x, y) scalar index pairs of the summed term fills (TermFill mode 3)
+  std::vector<uint32_t> sum_pairs;
+  // work that runs only for the items of failing groups, after the group
+  // checks: comb chunks (each gated by its group's status slot) and the
+  // sums that form the items' own check points
+  std::vector<CombChunk> fb_chunks;
+  std::vector<uint32_t> fb_gate;
+  std::vector<GatedReduce> fb_red;
   std::vector<PipTerm> pip_terms;  // terms of the Pippenger MSMs
   std::vector<uint32_t> pip_out;   // per Pippenger MSM: its output partial slot
   std::vector<uint8_t> blob;       // signatures, proofs, headers, messages, keys, seeds
@@ -162,6 +169,27 @@ struct Plan {
       chunks.push_back(CombChunk{term + t, c, partials(1)});
     }
   }
+  // The same for the gated comb launch: chunks that run only when
+  // valid[gate] == 0.  Returns the first of their consecutive partials.
+  uint32_t comb_gated(uint32_t term, uint32_t count, uint32_t chunk, uint32_t gate) {
+    const uint32_t first = n_partials;
+    for (uint32_t t = 0; t < count; t += chunk) {
+      uint32_t c = std::min(chunk, count - t);
+      fb_chunks.push_back(CombChunk{term + t, c, partials(1)});
+      fb_gate.push_back(gate);
+    }
+    return first;
+  }
+  // A summed term fill: scalars[dst] = sum of scalars[x] * scalars[y] over
+  // the (x, y) pairs given.
+  void term_sum(uint32_t dst, const std::vector<std::pair<uint32_t, uint32_t>>& xy) {
+    const uint32_t first = (uint32_t)(sum_pairs.size() / 2);
+    for (const auto& q : xy) {
+      sum_pairs.push_back(q.first);
+      sum_pairs.push_back(q.second);
+    }
+    fills.push_back(TermFill{dst, first, (uint32_t)xy.size(), 3});
+  }
 };
 
 // Copy the plan into one pinned buffer + device mirror and allocate the work
@@ -169,15 +197,12 @@ struct Plan {
 void materialize(Batch& b, Plan& p);
 // Group the plain pairing checks e(a_i, PK) e(b_i, -G2) = 1 of p.pairs by key
 // into groups of at most group_max (status slots from nv on) and add their
-// random-linear-combination checks; see bbs_batch.cpp.
-// scaled: the checks' points are already multiplied by their coefficients
-// (the planner drew them), so the group sums add the points themselves;
-// otherwise the products c_i a_i, c_i b_i are planned here.
+// random-linear-combination checks (the products c_i a_i, c_i b_i are
+// planned here); see bbs_batch.cpp.
 // super_max >= 2: consecutive groups of one key, up to super_max of them,
 // are also checked together first (a supergroup), and their groups only when
 // that check fails.
-void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, bool scaled = false,
-                     uint32_t super_max = 0);
+void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, uint32_t super_max = 0);
 uint32_t agg_group_max();
 uint32_t agg_super_max();
 // Lock the context, make generators [0, max_gen_slots) and the given public
