@@ -64,6 +64,8 @@ SERVING_HW_QUEUES = "16"
 # batch sets in flight in the timed loop: while one set's pairing launches
 # finish, the next set's decode / hashing / MSM stages already run
 INFLIGHT = 8
+# verify sets in flight (one stream each) when the tuning table has no entry
+INFLIGHT_SETS = 16
 
 N_VERIFY = 8192
 N_ISSUERS = 256
@@ -575,17 +577,35 @@ def profiled(path, key):
     return None
 
 
+def make_set(g, vreqs, pvreqs, combined, **kw):
+    """One staged set of the step's work: a verify set (the Verify and the
+    ProofVerify batch staged as one batch: every stage launches once for
+    both), or the two batches staged separately; single stream each."""
+    if combined:
+        return [g.StagedBatch(g.BATCH_VERIFY_SET, (vreqs, pvreqs), single_stream=True, **kw)]
+    return [g.StagedBatch(g.BATCH_VERIFY, vreqs, single_stream=True, **kw),
+            g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs, single_stream=True, **kw)]
+
+
+def check_set(batches, expect_v):
+    """The set's results: Verify verdicts as constructed, every proof valid."""
+    res = [r for b in batches for r in b.results()]
+    nv = len(expect_v)
+    if [x for _, x in res[:nv]] != expect_v or not all(c == 0 and x for c, x in res[nv:]):
+        raise SystemExit("results differ from the expected ones")
+    return tuple(b.results_raw() for b in batches)
+
+
 def run_in_flight(sets, steps, expect):
-    """Seconds for `steps` steps with the staged (Verify, ProofVerify) sets
-    in flight round-robin: a set is launched again once its previous results
-    are back and checked against `expect`."""
+    """Seconds for `steps` steps with the staged sets in flight round-robin:
+    a set is launched again once its previous results are back and checked
+    against `expect`."""
     n = len(sets)
 
     def finish(i):
-        v, p = sets[i]
-        v.sync()
-        p.sync()
-        if (v.results_raw(), p.results_raw()) != expect[i]:
+        for b in sets[i]:
+            b.sync()
+        if tuple(b.results_raw() for b in sets[i]) != expect[i]:
             raise SystemExit("results of a timed step differ from the expected ones")
 
     t = time.perf_counter()
@@ -593,8 +613,8 @@ def run_in_flight(sets, steps, expect):
         i = step % n
         if step >= n:
             finish(i)
-        sets[i][0].launch()
-        sets[i][1].launch()
+        for b in sets[i]:
+            b.launch()
     for step in range(max(0, steps - n), steps):
         finish(step % n)
     return time.perf_counter() - t
@@ -606,52 +626,50 @@ TUNE_CLASSES = ((2048, 512), (N_VERIFY, N_PROOFS))
 TUNE_GROUPS = (4, 8)
 TUNE_SUPER = (0, 4)
 TUNE_CHUNKS = (2, 4)
-TUNE_INFLIGHT = (6, 8, 10, 12)
+# sets in flight per staging form: verify sets take one stream each, the
+# separate form two (GPU_MAX_HW_QUEUES=16 hardware queues)
+TUNE_INFLIGHT = {True: (8, 12, 16), False: (6, 8)}
 TUNE_STEPS = 20  # the headline's timed steps: pipeline fill and drain included
 
 
 def tune_pipeline(g, vreqs, expect_v, pvreqs, graph=True):
-    """Measures every (aggregation group, groups per supergroup, comb terms
-    per lane, sets in flight) candidate for each size class (Verify,
-    ProofVerify items per set, the benchmark's requests cut to that size) as
-    the headline runs them: single-stream sets replayed from graphs, several
-    in flight, results checked.  Returns the table: all candidates, the
-    fastest per class, and the library's pipeline lines for
+    """Measures every (staging form, aggregation group, groups per
+    supergroup, comb terms per lane, sets in flight) candidate for each size
+    class (Verify, ProofVerify items per set, the benchmark's requests cut to
+    that size) as the headline runs them: single-stream sets replayed from
+    graphs, several in flight, results checked.  Returns the table: all
+    candidates, the fastest per class, and the library's pipeline lines for
     cubbs_hip_tune_load."""
     classes, lines = [], []
     for nv, npv in TUNE_CLASSES:
         vr, pr, ev = vreqs[:nv], pvreqs[:npv], expect_v[:nv]
         cands = []
-        for grp, sup, ch in [(a, b, c) for a in TUNE_GROUPS for b in TUNE_SUPER for c in TUNE_CHUNKS]:
-            sets = [(g.StagedBatch(g.BATCH_VERIFY, vr, single_stream=True, aggregate=grp, comb_chunk=ch,
-                                   supergroup=sup),
-                     g.StagedBatch(g.BATCH_PROOF_VERIFY, pr, single_stream=True, aggregate=grp,
-                                   comb_chunk=ch, supergroup=sup))
-                    for _ in range(max(TUNE_INFLIGHT))]
+        for comb, grp, sup, ch in [(f, a, b, c) for f in (True, False) for a in TUNE_GROUPS
+                                   for b in TUNE_SUPER for c in TUNE_CHUNKS]:
+            sets = [make_set(g, vr, pr, comb, aggregate=grp, comb_chunk=ch, supergroup=sup)
+                    for _ in range(max(TUNE_INFLIGHT[comb]))]
             if graph:
-                for v, p in sets:
-                    v.use_graph(True)
-                    p.use_graph(True)
+                for st in sets:
+                    for b in st:
+                        b.use_graph(True)
+            for st in sets:
+                for b in st:
+                    b.launch()
             expect = []
-            for v, p in sets:
-                v.launch()
-                p.launch()
-            for v, p in sets:
-                v.sync()
-                p.sync()
-                if [x for _, x in v.results()] != ev or not all(c == 0 and x for c, x in p.results()):
-                    raise SystemExit("tuning candidate results differ from the expected ones")
-                expect.append((v.results_raw(), p.results_raw()))
-            for n in TUNE_INFLIGHT:
+            for st in sets:
+                for b in st:
+                    b.sync()
+                expect.append(check_set(st, ev))
+            for n in TUNE_INFLIGHT[comb]:
                 run_in_flight(sets[:n], n, expect)  # warm-up
                 ms = min(run_in_flight(sets[:n], TUNE_STEPS, expect) for _ in range(3)) * 1e3 / TUNE_STEPS
-                cands.append({"agg_group": grp, "agg_super": sup, "comb_chunk": ch, "sets_in_flight": n,
-                              "ms_per_step": round(ms, 4)})
-                log("tune %d+%d group=%d super=%d chunk=%d inflight=%d: %.3f ms/step"
-                    % (nv, npv, grp, sup, ch, n, ms))
-            for v, p in sets:
-                v.free()
-                p.free()
+                cands.append({"verify_set": comb, "agg_group": grp, "agg_super": sup, "comb_chunk": ch,
+                              "sets_in_flight": n, "ms_per_step": round(ms, 4)})
+                log("tune %d+%d %s group=%d super=%d chunk=%d inflight=%d: %.3f ms/step"
+                    % (nv, npv, "set" if comb else "separate", grp, sup, ch, n, ms))
+            for st in sets:
+                for b in st:
+                    b.free()
         best = min(cands, key=lambda c: c["ms_per_step"])
         classes.append({"verify_items": nv, "proof_verify_items": npv, "best": best, "candidates": cands})
         for kind, items in (("verify", nv), ("proof_verify", npv)):
@@ -659,9 +677,10 @@ def tune_pipeline(g, vreqs, expect_v, pvreqs, graph=True):
                          (kind, round(math.log2(items)), best["comb_chunk"], best["agg_group"],
                           best["agg_super"]))
     return {"description": "Measured Verify/ProofVerify pipeline shapes per size class (items per set): "
-                           "aggregation group size, groups per supergroup check (0: none), fixed-base comb "
-                           "terms per lane, staged sets in flight; single-stream sets replayed from HIP "
-                           "graphs, results checked",
+                           "staging form (verify_set: both batches staged as one batch, else two "
+                           "batches), aggregation group size, groups per supergroup check (0: none), "
+                           "fixed-base comb terms per lane, staged sets in flight; single-stream sets "
+                           "replayed from HIP graphs, results checked",
             "command": "python bench.py --tune-pipeline cubbs-cred_amd/pipeline_tuning.json",
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
             "classes": classes, "table": lines}
@@ -715,6 +734,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="skip the measured launch-shape tuning")
     ap.add_argument("--no-aggregate", action="store_true",
                     help="per-item pairing checks only (no random-linear-combination group checks)")
+    ap.add_argument("--separate", action="store_true",
+                    help="stage each set as a Verify and a ProofVerify batch instead of one verify set")
     ap.add_argument("--fork", action="store_true",
                     help="staged batches fork their decode/product branch onto a second stream "
                          "(lower latency per batch, two hardware queues each)")
@@ -800,52 +821,52 @@ def main():
         print(json.dumps({"pipeline_tuning": table["classes"]}), flush=True)
         return
     tuned = None if args.no_pipeline_tuning else load_pipeline_tuning(g)
-    nset = max(1, args.inflight or (tuned["sets_in_flight"] if tuned else INFLIGHT))
+    combined = not args.separate and not args.fork and (tuned.get("verify_set", True) if tuned else True)
+    nset = max(1, args.inflight or (tuned["sets_in_flight"] if tuned else
+                                     (INFLIGHT_SETS if combined else INFLIGHT)))
     single, agg = not args.fork, not args.no_aggregate
-    sets = [(g.StagedBatch(g.BATCH_VERIFY, vreqs, single_stream=single, aggregate=agg),
-             g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs, single_stream=single, aggregate=agg))
-            for _ in range(nset)]
-    vb, pb = sets[0]
+    if combined:
+        sets = [make_set(g, vreqs, pvreqs, True, aggregate=agg) for _ in range(nset)]
+    else:
+        sets = [[g.StagedBatch(g.BATCH_VERIFY, vreqs, single_stream=single, aggregate=agg),
+                 g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs, single_stream=single, aggregate=agg)]
+                for _ in range(nset)]
     if not args.no_graph:
         # each batch's whole launch sequence replayed from one captured HIP graph
-        for v, p in sets:
-            v.use_graph(True)
-            p.use_graph(True)
-    log("staged %d set(s)" % nset)
+        for st in sets:
+            for b in st:
+                b.use_graph(True)
+    log("staged %d set(s)%s" % (nset, " (verify sets)" if combined else ""))
     for _ in range(max(1, args.warmup)):
-        for v, p in sets:
-            v.launch()
-            p.launch()
-        for v, p in sets:
-            v.sync()
-            p.sync()
-    expect = []
-    for v, p in sets:
-        rv = v.results()
-        rp = p.results()
-        assert [x for _, x in rv] == expect_v, "verify results differ from expectation"
-        assert all(c == 0 and x for c, x in rp), "proof verify results differ from expectation"
-        expect.append((v.results_raw(), p.results_raw()))
+        for st in sets:
+            for b in st:
+                b.launch()
+        for st in sets:
+            for b in st:
+                b.sync()
+    expect = [check_set(st, expect_v) for st in sets]
 
     mark()
     el = run_in_flight(sets, args.steps, expect)
     mark()
     ms_per_step = el * 1e3 / args.steps
     ops = N_VERIFY + N_PROOFS
-    # the algorithmic work of one step: both batches of a set (every set holds
+    # the algorithmic work of one step: the batches of a set (every set holds
     # the same items; pair_checks_run counts the checks of their last launch)
-    step_work = [vb.work(), pb.work()]
+    step_work = [b.work() for b in sets[0]]
     step_fpm = sum(work_fpm(w) for w in step_work)
+    msm_terms = sum(b.shape()["msm_terms"] for b in sets[0])
     value = ops * args.steps / el
-    for v, p in sets[1:]:
-        v.free()
-        p.free()
+    for st in sets[1:]:
+        for b in st:
+            b.free()
     log("timed %d steps (%d in flight): %.2f ms/step, %.0f ops/s" % (args.steps, nset, ms_per_step, value))
     # per-kernel timing with HIP events on each batch's stream
-    vt, vk = vb.profile(2)
-    pt, pk = pb.profile(2)
-    log("verify kernels (ms): %s" % {k: round(v, 3) for k, v in vk.items()})
-    log("proof-verify kernels (ms): %s" % {k: round(v, 3) for k, v in pk.items()})
+    kernel_ms = {}
+    for name, b in zip(("verify_set",) if combined else ("verify", "proof_verify"), sets[0]):
+        _, km = b.profile(2)
+        kernel_ms[name] = {k: round(v, 3) for k, v in km.items()}
+        log("%s kernels (ms): %s" % (name, kernel_ms[name]))
     # the pairing kernel alone, on a Verify batch checked item by item (the
     # timed sets aggregate their checks, so their per-item launches hold only
     # the items of failing groups): 8192 full checks
@@ -887,6 +908,9 @@ def main():
                       "verify_items": N_VERIFY, "proof_verify_items": N_PROOFS,
                       "launch": "kernel by kernel" if args.no_graph else "HIP graph replay per staged batch",
                       "batch_sets_in_flight": nset,
+                      "staging": ("verify set: the Verify and the ProofVerify batch staged as one batch, "
+                                  "every stage launched once for both" if combined
+                                  else "a Verify batch and a ProofVerify batch per set"),
                       "pipeline_tuning": (dict(tuned, source=os.path.relpath(PIPELINE_TUNING, ROOT))
                                           if tuned else None),
                       "streams_per_batch": 1 if single else 2,
@@ -896,13 +920,12 @@ def main():
                                             (", checked %d groups at a time first" % tuned["agg_super"])
                                             if tuned and tuned.get("agg_super") else "")) if agg else "per item",
                       "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
-                      "msm_terms": vb.shape()["msm_terms"] + pb.shape()["msm_terms"]},
+                      "msm_terms": msm_terms},
            "roofline": roofline,
            "abi_calls": abi_ops,
-           "kernel_ms": {"verify": {k: round(v, 3) for k, v in vk.items()},
-                         "proof_verify": {k: round(v, 3) for k, v in pk.items()}}}
-    vb.free()
-    pb.free()
+           "kernel_ms": kernel_ms}
+    for b in sets[0]:
+        b.free()
     # the MSM shapes are tuned (by measurement) after the Verify/ProofVerify
     # timing, which uses fixed shapes and no tuning table
     tuning = None

@@ -494,43 +494,85 @@ int32_t cubbs_hip_proof_gen_with_scalars_batch(uint32_t abi_version,
                           results);
 }
 
+// Applies prepare flags to the planning state of the calling thread for one
+// prepare call (undone by the destructor).  kd: 0 Verify, 1 ProofVerify
+// (the tuning table's size class of `items`), -1 other kinds.
+namespace {
+struct PrepareScope {
+  bool ok = false;
+  TuneForce saved;
+  PrepareScope(int kd, bool whole_op, size_t items, uint32_t flags) : saved(tune_force()) {
+    uint32_t group = (flags >> 8) & 15u;
+    const uint32_t chunk = (flags >> 12) & 127u;
+    uint32_t super = (flags >> 20) & 15u;
+    const bool agg = (flags & CUBBS_HIP_BATCH_AGGREGATE) != 0;
+    if ((flags & ~(uint32_t)(CUBBS_HIP_BATCH_SINGLE_STREAM | CUBBS_HIP_BATCH_AGGREGATE |
+                             CUBBS_HIP_BATCH_AGG_GROUP(15) | CUBBS_HIP_BATCH_COMB_CHUNK(127) |
+                             CUBBS_HIP_BATCH_AGG_SUPER(15))) ||
+        (group && (group < 2 || group > 8 || !agg)) || chunk > 64 || (chunk && !whole_op) ||
+        (super && (super < 2 || super > 8 || !agg)))
+      return;
+    if (agg) {
+      // the tuned shape of the batch's size class where the flags leave it
+      // open: groups of 8 and no supergroups without a table entry
+      uint32_t c = 0, gsz = 0, sup = 0;
+      const bool tuned = tuned_pipeline(kd, items, c, gsz, sup);
+      if (!group) group = tuned && gsz ? gsz : 8u;
+      if (!super && !(flags & CUBBS_HIP_BATCH_AGG_GROUP(15))) super = tuned ? sup : 0u;
+    }
+    if (chunk) tune_force().comb_chunk = chunk;
+    set_thread_single_stream((flags & CUBBS_HIP_BATCH_SINGLE_STREAM) != 0);
+    set_thread_agg_group(agg ? group : 0u);
+    set_thread_agg_super(agg ? super : 0u);
+    ok = true;
+  }
+  ~PrepareScope() {
+    set_thread_single_stream(false);
+    set_thread_agg_group(0);
+    set_thread_agg_super(0);
+    tune_force() = saved;
+  }
+};
+}  // namespace
+
 void* cubbs_hip_batch_prepare_flags(uint32_t abi_version, int kind, const void* items,
                                     size_t item_count, uint32_t flags, int32_t* status_out) {
   int32_t dummy;
   int32_t& st = status_out ? *status_out : dummy;
-  uint32_t group = (flags >> 8) & 15u;
-  const uint32_t chunk = (flags >> 12) & 127u;
-  uint32_t super = (flags >> 20) & 15u;
-  const bool agg = (flags & CUBBS_HIP_BATCH_AGGREGATE) != 0;
-  if ((flags & ~(uint32_t)(CUBBS_HIP_BATCH_SINGLE_STREAM | CUBBS_HIP_BATCH_AGGREGATE |
-                           CUBBS_HIP_BATCH_AGG_GROUP(15) | CUBBS_HIP_BATCH_COMB_CHUNK(127) |
-                           CUBBS_HIP_BATCH_AGG_SUPER(15))) ||
-      (group && (group < 2 || group > 8 || !agg)) || chunk > 64 ||
-      (chunk && kind != CUBBS_HIP_BATCH_VERIFY && kind != CUBBS_HIP_BATCH_PROOF_VERIFY) ||
-      (super && (super < 2 || super > 8 || !agg))) {
+  const int kd = kind == CUBBS_HIP_BATCH_VERIFY ? 0 : kind == CUBBS_HIP_BATCH_PROOF_VERIFY ? 1 : -1;
+  PrepareScope scope(kd, kd >= 0, item_count, flags);
+  if (!scope.ok) {
     st = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
     return nullptr;
   }
-  if (agg) {
-    // the tuned shape of the batch's size class where the flags leave it
-    // open: groups of 8 and no supergroups without a table entry
-    uint32_t c = 0, gsz = 0, sup = 0;
-    const int kd = kind == CUBBS_HIP_BATCH_VERIFY ? 0 : kind == CUBBS_HIP_BATCH_PROOF_VERIFY ? 1 : -1;
-    const bool tuned = tuned_pipeline(kd, item_count, c, gsz, sup);
-    if (!group) group = tuned && gsz ? gsz : 8u;
-    if (!super && !(flags & CUBBS_HIP_BATCH_AGG_GROUP(15))) super = tuned ? sup : 0u;
+  return cubbs_hip_batch_prepare(abi_version, kind, items, item_count, status_out);
+}
+
+void* cubbs_hip_batch_prepare_verify_set(uint32_t abi_version, const CuBbsBbsGpuVerifyItem* verify_items,
+                                         size_t verify_count,
+                                         const CuBbsBbsGpuProofVerifyItem* proof_items,
+                                         size_t proof_count, uint32_t flags, int32_t* status_out) {
+  int32_t dummy;
+  int32_t& st = status_out ? *status_out : dummy;
+  if (abi_version != CUBBS_BBS_GPU_ABI_VERSION || (verify_count && !verify_items) ||
+      (proof_count && !proof_items)) {
+    st = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+    return nullptr;
   }
-  const TuneForce saved = tune_force();
-  if (chunk) tune_force().comb_chunk = chunk;
-  set_thread_single_stream((flags & CUBBS_HIP_BATCH_SINGLE_STREAM) != 0);
-  set_thread_agg_group(agg ? group : 0u);
-  set_thread_agg_super(agg ? super : 0u);
-  void* b = cubbs_hip_batch_prepare(abi_version, kind, items, item_count, status_out);
-  set_thread_single_stream(false);
-  set_thread_agg_group(0);
-  set_thread_agg_super(0);
-  tune_force() = saved;
-  return b;
+  PrepareScope scope(0, true, verify_count, flags);
+  if (!scope.ok) {
+    st = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+    return nullptr;
+  }
+  try {
+    std::unique_ptr<Batch> b = plan_verify_and_proof_verify(verify_items, verify_count, proof_items, proof_count);
+    if (b->stream) check(hipStreamSynchronize(b->stream));
+    st = CUBBS_BBS_GPU_SUCCESS;
+    return b.release();
+  } catch (...) {  // a GPU error, or host memory exhausted
+    st = CUBBS_BBS_GPU_ERR_INTERNAL;
+    return nullptr;
+  }
 }
 
 void* cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void* items, size_t item_count,

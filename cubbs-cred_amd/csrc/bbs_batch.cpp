@@ -626,10 +626,14 @@ static std::string map_dst() { return std::string(kApiId) + "MAP_MSG_TO_SCALAR_A
 // the group's status slot, after the group checks).  A key's last item, when
 // it is left alone, is checked on its own throughout.  Supergroups (runs of
 // up to super_max groups of a key) are checked before their groups.
-static void plan_verify_grouped(Batch& b, const CuBbsBbsGpuVerifyItem* items,
-                                const std::vector<size_t>& dev, const std::vector<uint32_t>& pk_slot,
-                                Plan& p, uint32_t CH, uint32_t group_max, uint32_t super_max) {
-  const uint32_t nv = b.nv;
+//
+// Item k of `dev` is device item ibase + k; group status slots are numbered
+// from slot_base (past every device item of the batch).
+static void plan_verify_grouped(Plan& p, const CuBbsBbsGpuVerifyItem* items,
+                                const std::vector<size_t>& dev, const uint32_t* pk_slot,
+                                uint32_t ibase, uint32_t slot_base, uint32_t CH, uint32_t group_max,
+                                uint32_t super_max) {
+  const uint32_t nv = (uint32_t)dev.size();
   if (super_max < 2 || super_max > 8) super_max = 1;
   std::vector<uint32_t> Ls(nv);
   for (uint32_t k = 0; k < nv; k++) Ls[k] = (uint32_t)items[dev[k]].message_count;
@@ -659,15 +663,15 @@ static void plan_verify_grouped(Batch& b, const CuBbsBbsGpuVerifyItem* items,
     }
     i = e;
   }
-  // term slots first: [0, T) -- the groups' summed terms, then every item's
-  // own terms
+  // term slots first: [tbase, tbase + T) -- the groups' summed terms, then
+  // every item's own terms
   uint32_t T = 0;
   for (const auto& runs : key_runs)
     for (const Run& r : runs) T += Ls[order[r.end - 1]] + 2;
   for (uint32_t k = 0; k < nv; k++) T += Ls[k] + 2;
-  p.scalars(T);
-  p.term_gen.resize(T);
-  uint32_t term = 0;
+  const uint32_t tbase = p.scalars(T);
+  p.term_gen.resize(tbase + T);
+  uint32_t term = tbase;
   // inputs of every item: A, c, r - c e, the domain and message scalars
   std::vector<uint8_t> rnd(16ull * nv);
   fill_os_random(rnd.data(), rnd.size());
@@ -678,7 +682,7 @@ static void plan_verify_grouped(Batch& b, const CuBbsBbsGpuVerifyItem* items,
     const size_t sig = p.bytes(it.signature.data, 80);
     const size_t hdr = p.bytes(it.header.data, it.header.len);
     pa[k] = p.points(1);
-    p.dec_g1.push_back(DecodeG1Job{sig, pa[k], k});
+    p.dec_g1.push_back(DecodeG1Job{sig, pa[k], ibase + k});
     uint64_t s0, t0;
     std::memcpy(&s0, &rnd[16ull * k], 8);
     std::memcpy(&t0, &rnd[16ull * k + 8], 8);
@@ -688,8 +692,8 @@ static void plan_verify_grouped(Batch& b, const CuBbsBbsGpuVerifyItem* items,
     sc[k] = p.scalars(1);
     se[k] = p.scalars(1);
     raw[k] = p.scalars(L + 1);
-    p.dec_fr.push_back(DecodeFrJob{c_off, sc[k], k, 0u, 0u});
-    p.dec_fr.push_back(DecodeFrJob{sig + 48, se[k], k, 1u | 4u, 0u, c_off});  // r - c e
+    p.dec_fr.push_back(DecodeFrJob{c_off, sc[k], ibase + k, 0u, 0u});
+    p.dec_fr.push_back(DecodeFrJob{sig + 48, se[k], ibase + k, 1u | 4u, 0u, c_off});  // r - c e
     p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, raw[k]});
     for (uint32_t m = 0; m < L; m++) p.hash_message(it.messages[m].data, it.messages[m].len, raw[k] + 1 + m);
   }
@@ -711,7 +715,7 @@ static void plan_verify_grouped(Batch& b, const CuBbsBbsGpuVerifyItem* items,
       const uint32_t first = runs[c0].begin, total = runs[c1 - 1].end - first;
       const uint32_t pk = pk_slot[order[first]];
       uint32_t sslot = 0;
-      if (sup) sslot = nv + p.n_agg_slots++;
+      if (sup) sslot = slot_base + p.n_agg_slots++;
       // partials: c_i A_i of every item of the chunk, then per group the
       // (-c_i e_i) A_i of its items followed by its comb partials
       const uint32_t fx = p.partials(total);
@@ -747,7 +751,7 @@ static void plan_verify_grouped(Batch& b, const CuBbsBbsGpuVerifyItem* items,
         }
         p.comb(gt, lmax + 2, CH);
         vend[r - c0] = p.n_partials;
-        gslot[r - c0] = nv + p.n_agg_slots++;
+        gslot[r - c0] = slot_base + p.n_agg_slots++;
       }
       const uint32_t yend = p.n_partials;
       // the sums, once the chunk's partials are all allocated (a long sum's
@@ -778,7 +782,7 @@ static void plan_verify_grouped(Batch& b, const CuBbsBbsGpuVerifyItem* items,
           const uint32_t pa2 = p.points(1), pb = p.points(1);
           p.fb_red.push_back(GatedReduce{fx + (m - first), 1, REDUCE_NO_EXTRA, g, pa2});   // c A
           p.fb_red.push_back(GatedReduce{cb, p.n_partials - cb, vb[r - c0] + (m - run.begin), g, pb});
-          grouped.push_back(PairJob{k, pa2, pb, pk, PAIR_NEG_G2, PAIR_GROUPED, g});
+          grouped.push_back(PairJob{ibase + k, pa2, pb, pk, PAIR_NEG_G2, PAIR_GROUPED, g});
         }
       }
     }
@@ -794,9 +798,9 @@ static void plan_verify_grouped(Batch& b, const CuBbsBbsGpuVerifyItem* items,
     p.var.push_back(VarJob{pa[k], se[k], p.partials(1)});     // (-c e) A
     p.comb(st, L + 2, CH);                                    // c B
     p.reduce_to(first, p.n_partials - first, pb);             // c (B - e A)
-    single.push_back(PairJob{k, pair_a, pb, pk_slot[k], PAIR_NEG_G2, 0, 0});
+    single.push_back(PairJob{ibase + k, pair_a, pb, pk_slot[k], PAIR_NEG_G2, 0, 0});
   }
-  if (term != T) throw std::logic_error("grouped verify plan: term slots do not add up");
+  if (term != tbase + T) throw std::logic_error("grouped verify plan: term slots do not add up");
   p.n_super_pairs = (uint32_t)supers.size();
   p.n_group_pairs = (uint32_t)(supers.size() + groups.size());
   p.pairs = std::move(supers);
@@ -805,15 +809,12 @@ static void plan_verify_grouped(Batch& b, const CuBbsBbsGpuVerifyItem* items,
   p.pairs.insert(p.pairs.end(), single.begin(), single.end());
 }
 
-std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n) {
-  auto b = std::make_unique<Batch>();
-  b->kind = CUBBS_HIP_BATCH_VERIFY;
-  b->n_total = n;
-  b->pre_code.assign(n, -1);
-  b->pre_value.assign(n, 0);
-  b->dev_index.assign(n, 0);
+// Host validation of Verify items: per-item arrays of the batch at [at, at + n),
+// device items numbered from dbase; returns the device items (indices into
+// items) and the largest message count.
+static std::vector<size_t> intake_verify(Batch& b, const CuBbsBbsGpuVerifyItem* items, size_t n,
+                                         size_t at, uint32_t dbase, uint32_t& maxL) {
   std::vector<size_t> dev;
-  uint32_t maxL = 0;
   for (size_t i = 0; i < n; i++) {
     const auto& it = items[i];
     bool ok = it.public_key.len == 96 && it.public_key.data && it.signature.len == 80 &&
@@ -821,58 +822,48 @@ std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n)
               (it.message_count == 0 || it.messages != nullptr) && it.message_count < (1u << 24);
     for (size_t m = 0; ok && m < it.message_count; m++) ok = slice_ok(it.messages[m]);
     if (!ok) {
-      b->pre_code[i] = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      b.pre_code[at + i] = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
       continue;
     }
-    b->dev_index[i] = (uint32_t)dev.size();
+    b.dev_index[at + i] = dbase + (uint32_t)dev.size();
     dev.push_back(i);
     maxL = std::max<uint32_t>(maxL, (uint32_t)it.message_count);
   }
-  b->nv = (uint32_t)dev.size();
-  if (!b->nv) return b;
-  std::vector<const uint8_t*> pks(b->nv);
-  for (uint32_t k = 0; k < b->nv; k++) pks[k] = items[dev[k]].public_key.data;
-  std::vector<uint32_t> pk_slot;
-  attach(*b, maxL + 2, pks, pk_slot);
-  Plan p;
-  std::string d = map_dst();
-  p.map_dst.assign(d.begin(), d.end());
-  const uint32_t CH = pipeline_comb_chunk(0, n);
-  const uint32_t group = agg_group_max();
-  if (group >= 2 && group <= 8) {
-    plan_verify_grouped(*b, items, dev, pk_slot, p, CH, group, agg_super_max());
-    materialize(*b, p);
-    return b;
-  }
-  // terms first: [0, T)
+  return dev;
+}
+
+// Per-item Verify checks e(A, PK) e(B - e A, -G2) = 1 (no aggregation); item
+// k of `dev` is device item ibase + k.
+static void plan_verify_plain(Plan& p, const CuBbsBbsGpuVerifyItem* items, const std::vector<size_t>& dev,
+                              const uint32_t* pk_slot, uint32_t ibase, uint32_t CH) {
+  const uint32_t nv = (uint32_t)dev.size();
+  // term slots first: [tbase, tbase + T)
   uint32_t T = 0;
-  for (uint32_t k = 0; k < b->nv; k++) T += (uint32_t)items[dev[k]].message_count + 2;
-  p.scalars(T);
-  p.term_gen.resize(T);
-  uint32_t term = 0;
-  for (uint32_t k = 0; k < b->nv; k++) {
+  for (uint32_t k = 0; k < nv; k++) T += (uint32_t)items[dev[k]].message_count + 2;
+  const uint32_t tbase = p.scalars(T);
+  p.term_gen.resize(tbase + T);
+  uint32_t term = tbase;
+  for (uint32_t k = 0; k < nv; k++) {
     const auto& it = items[dev[k]];
     const uint32_t L = (uint32_t)it.message_count;
     size_t sig = p.bytes(it.signature.data, 80);
     size_t hdr = p.bytes(it.header.data, it.header.len);
     uint32_t pa = p.points(1), pb = p.points(1), se = p.scalars(1);
-    p.dec_g1.push_back(DecodeG1Job{sig, pa, k});
+    p.dec_g1.push_back(DecodeG1Job{sig, pa, ibase + k});
     for (uint32_t t = 0; t < L + 2; t++) p.term_gen[term + t] = t;
     p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, term + 1});
     for (uint32_t m = 0; m < L; m++) {
       p.hash_message(it.messages[m].data, it.messages[m].len, term + 2 + m);
     }
-    p.fills.push_back(TermFill{term, 0, 0, 2});               // P1 scalar = 1
-    p.dec_fr.push_back(DecodeFrJob{sig + 48, se, k, 1u, 0u});  // store r - e
+    p.fills.push_back(TermFill{term, 0, 0, 2});                       // P1 scalar = 1
+    p.dec_fr.push_back(DecodeFrJob{sig + 48, se, ibase + k, 1u, 0u});  // store r - e
     uint32_t first = p.n_partials;
     p.var.push_back(VarJob{pa, se, p.partials(VAR_PARTIALS)});  // (-e) A
     p.comb(term, L + 2, CH);                                  // B
     p.reduce_to(first, p.n_partials - first, pb);             // B - e A
-    p.pairs.push_back(PairJob{k, pa, pb, pk_slot[k], PAIR_NEG_G2, 0});
+    p.pairs.push_back(PairJob{ibase + k, pa, pb, pk_slot[k], PAIR_NEG_G2, 0});
     term += L + 2;
   }
-  materialize(*b, p);
-  return b;
 }
 
 // ------------------------------------------------------------------ ProofVerify
@@ -885,22 +876,17 @@ static bool indexes_strictly_increasing(const CuBbsBbsGpuIndexSlice& s, size_t b
   return true;
 }
 
-std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items, size_t n) {
-  auto b = std::make_unique<Batch>();
-  b->kind = CUBBS_HIP_BATCH_PROOF_VERIFY;
-  b->n_total = n;
-  b->pre_code.assign(n, -1);
-  b->pre_value.assign(n, 0);
-  b->dev_index.assign(n, 0);
+// Host validation of ProofVerify items, as intake_verify.
+static std::vector<size_t> intake_proof_verify(Batch& b, const CuBbsBbsGpuProofVerifyItem* items,
+                                               size_t n, size_t at, uint32_t dbase, uint32_t& maxL) {
   std::vector<size_t> dev;
-  uint32_t maxL = 0;
   for (size_t i = 0; i < n; i++) {
     const auto& it = items[i];
     // protocol.rs:774-779: bad disclosed indexes or count mismatch -> Ok(false)
     if (!indexes_strictly_increasing(it.disclosed_indexes, it.total_message_count) ||
         it.disclosed_count != it.disclosed_indexes.len ||
         (it.disclosed_count && !it.disclosed_messages)) {
-      b->pre_code[i] = CUBBS_BBS_GPU_SUCCESS;
+      b.pre_code[at + i] = CUBBS_BBS_GPU_SUCCESS;
       continue;
     }
     bool ok = it.public_key.len == 96 && it.public_key.data && slice_ok(it.header) &&
@@ -908,7 +894,7 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
               it.proof.len < (1ull << 30);
     for (size_t m = 0; ok && m < it.disclosed_count; m++) ok = slice_ok(it.disclosed_messages[m]);
     if (!ok) {
-      b->pre_code[i] = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      b.pre_code[at + i] = CUBBS_BBS_GPU_ERR_INVALID_INPUT;
       continue;
     }
     const size_t U = (it.proof.len - 240) / 32 - 1;
@@ -918,32 +904,33 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
     bool idx_ok = true;
     for (size_t r = 0; r < R; r++) idx_ok = idx_ok && it.disclosed_indexes.data[r] < L;
     if (!idx_ok) {
-      b->pre_code[i] = CUBBS_BBS_GPU_SUCCESS;
+      b.pre_code[at + i] = CUBBS_BBS_GPU_SUCCESS;
       continue;
     }
-    b->dev_index[i] = (uint32_t)dev.size();
+    b.dev_index[at + i] = dbase + (uint32_t)dev.size();
     dev.push_back(i);
     maxL = std::max<uint32_t>(maxL, (uint32_t)L);
   }
-  b->nv = (uint32_t)dev.size();
-  if (!b->nv) return b;
-  std::vector<const uint8_t*> pks(b->nv);
-  for (uint32_t k = 0; k < b->nv; k++) pks[k] = items[dev[k]].public_key.data;
-  std::vector<uint32_t> pk_slot;
-  attach(*b, maxL + 2, pks, pk_slot);
-  Plan p;
-  std::string d = map_dst();
-  p.map_dst.assign(d.begin(), d.end());
-  const uint32_t CH = pipeline_comb_chunk(1, n);
+  return dev;
+}
+
+// ProofVerify checks; item k of `dev` is device item ibase + k, group status
+// slots (group >= 2: aggregated checks) numbered from slot_base.
+static void plan_proof_verify_items(Plan& p, const CuBbsBbsGpuProofVerifyItem* items,
+                                    const std::vector<size_t>& dev, const uint32_t* pk_slot,
+                                    uint32_t ibase, uint32_t slot_base, uint32_t CH, uint32_t group,
+                                    uint32_t super) {
+  const uint32_t nv = (uint32_t)dev.size();
   uint32_t T = 0;
-  for (uint32_t k = 0; k < b->nv; k++) {
+  for (uint32_t k = 0; k < nv; k++) {
     const auto& it = items[dev[k]];
     T += (uint32_t)((it.proof.len - 240) / 32 - 1 + it.disclosed_count) + 2;
   }
-  p.scalars(T);
-  p.term_gen.resize(T);
-  uint32_t term = 0;
-  for (uint32_t k = 0; k < b->nv; k++) {
+  const uint32_t tbase = p.scalars(T);
+  p.term_gen.resize(tbase + T);
+  uint32_t term = tbase;
+  for (uint32_t k = 0; k < nv; k++) {
+    const uint32_t item = ibase + k;
     const auto& it = items[dev[k]];
     const uint32_t U = (uint32_t)((it.proof.len - 240) / 32 - 1);
     const uint32_t R = (uint32_t)it.disclosed_count;
@@ -953,16 +940,16 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
     size_t ph = p.bytes(it.presentation_header.data, it.presentation_header.len);
     uint32_t pab = p.points(1), pbb = p.points(1), pd = p.points(1), pt1 = p.points(1),
              pt2 = p.points(1);
-    p.dec_g1.push_back(DecodeG1Job{pr + 0, pab, k});
-    p.dec_g1.push_back(DecodeG1Job{pr + 48, pbb, k});
-    p.dec_g1.push_back(DecodeG1Job{pr + 96, pd, k});
+    p.dec_g1.push_back(DecodeG1Job{pr + 0, pab, item});
+    p.dec_g1.push_back(DecodeG1Job{pr + 48, pbb, item});
+    p.dec_g1.push_back(DecodeG1Job{pr + 96, pd, item});
     uint32_t sc = p.scalars(4 + U);  // e^, r1^, r3^, m^_1..m^_U, then c
     uint32_t s_ehat = sc, s_r1 = sc + 1, s_r3 = sc + 2, s_mhat = sc + 3, s_c = sc + 3 + U;
-    p.dec_fr.push_back(DecodeFrJob{pr + 144, s_ehat, k, 0, 0});
-    p.dec_fr.push_back(DecodeFrJob{pr + 176, s_r1, k, 0, 0});
-    p.dec_fr.push_back(DecodeFrJob{pr + 208, s_r3, k, 0, 0});
+    p.dec_fr.push_back(DecodeFrJob{pr + 144, s_ehat, item, 0, 0});
+    p.dec_fr.push_back(DecodeFrJob{pr + 176, s_r1, item, 0, 0});
+    p.dec_fr.push_back(DecodeFrJob{pr + 208, s_r3, item, 0, 0});
     for (uint32_t u = 0; u <= U; u++)
-      p.dec_fr.push_back(DecodeFrJob{pr + 240 + 32ull * u, s_mhat + u, k, 0, 0});
+      p.dec_fr.push_back(DecodeFrJob{pr + 240 + 32ull * u, s_mhat + u, item, 0, 0});
     uint32_t s_dom = p.scalars(1);
     p.dom.push_back(DomainJob{hdr, (uint32_t)it.header.len, pk_slot[k], L, s_dom});
     uint32_t s_m = p.scalars(R);
@@ -996,14 +983,92 @@ std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items
     p.var.push_back(VarJob{pd, s_r3, p.partials(VAR_PARTIALS)});
     p.comb(term, L + 2, CH);
     p.reduce_to(f2, p.n_partials - f2, pt2);
-    p.pvch.push_back(PvChallengeJob{ph, (uint32_t)it.presentation_header.len, k, R, disc_begin,
+    p.pvch.push_back(PvChallengeJob{ph, (uint32_t)it.presentation_header.len, item, R, disc_begin,
                                     pab, pbb, pd, pt1, pt2, s_dom, s_c});
-    p.pairs.push_back(PairJob{k, pab, pbb, pk_slot[k], PAIR_NEG_G2, 0});
+    p.pairs.push_back(PairJob{item, pab, pbb, pk_slot[k], PAIR_NEG_G2, 0});
     term += L + 2;
   }
-  aggregate_pairs(p, b->nv, agg_group_max(), agg_super_max());
+  if (term != tbase + T) throw std::logic_error("proof verify plan: term slots do not add up");
+  aggregate_pairs(p, slot_base, group, super);
+}
+
+// A Verify batch and a ProofVerify batch as one device batch: the Verify
+// items are device items [0, nv), the ProofVerify items follow, and the
+// group status slots of both follow all items.  Every stage launches once
+// for the items of both.  Pairing checks keep the level order the launches
+// need: the supergroup checks of both parts, then their group checks, then
+// their per-item checks.
+static std::unique_ptr<Batch> plan_verify_set(const CuBbsBbsGpuVerifyItem* vitems, size_t nv,
+                                              const CuBbsBbsGpuProofVerifyItem* pitems, size_t np,
+                                              int kind) {
+  auto b = std::make_unique<Batch>();
+  b->kind = kind;
+  b->n_total = nv + np;
+  b->pre_code.assign(nv + np, -1);
+  b->pre_value.assign(nv + np, 0);
+  b->dev_index.assign(nv + np, 0);
+  uint32_t maxL = 0;
+  const std::vector<size_t> dv = intake_verify(*b, vitems, nv, 0, 0, maxL);
+  const std::vector<size_t> dp = intake_proof_verify(*b, pitems, np, nv, (uint32_t)dv.size(), maxL);
+  b->nv = (uint32_t)(dv.size() + dp.size());
+  if (!b->nv) return b;
+  std::vector<const uint8_t*> pks;
+  pks.reserve(b->nv);
+  for (size_t i : dv) pks.push_back(vitems[i].public_key.data);
+  for (size_t i : dp) pks.push_back(pitems[i].public_key.data);
+  std::vector<uint32_t> pk_slot;
+  attach(*b, maxL + 2, pks, pk_slot);
+  Plan p;
+  std::string d = map_dst();
+  p.map_dst.assign(d.begin(), d.end());
+  const uint32_t group = agg_group_max(), super = agg_super_max();
+  const bool agg = group >= 2 && group <= 8;
+  if (!dv.empty()) {
+    const uint32_t CH = pipeline_comb_chunk(0, nv);
+    if (agg)
+      plan_verify_grouped(p, vitems, dv, pk_slot.data(), 0, b->nv, CH, group, super);
+    else
+      plan_verify_plain(p, vitems, dv, pk_slot.data(), 0, CH);
+  }
+  std::vector<PairJob> vpairs = std::move(p.pairs);
+  p.pairs.clear();
+  const uint32_t vs = p.n_super_pairs, vg = p.n_group_pairs;
+  p.n_super_pairs = p.n_group_pairs = 0;
+  if (!dp.empty())
+    plan_proof_verify_items(p, pitems, dp, pk_slot.data() + dv.size(), (uint32_t)dv.size(), b->nv,
+                            pipeline_comb_chunk(1, np), agg ? group : 0u, super);
+  if (!vpairs.empty()) {
+    std::vector<PairJob> pp = std::move(p.pairs);
+    const uint32_t ps = p.n_super_pairs, pg = p.n_group_pairs;
+    p.pairs.clear();
+    auto take = [&](const std::vector<PairJob>& v, size_t from, size_t to) {
+      p.pairs.insert(p.pairs.end(), v.begin() + from, v.begin() + to);
+    };
+    take(vpairs, 0, vs);
+    take(pp, 0, ps);
+    take(vpairs, vs, vg);
+    take(pp, ps, pg);
+    take(vpairs, vg, vpairs.size());
+    take(pp, pg, pp.size());
+    p.n_super_pairs = vs + ps;
+    p.n_group_pairs = vg + pg;
+  }
   materialize(*b, p);
   return b;
+}
+
+std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n) {
+  return plan_verify_set(items, n, nullptr, 0, CUBBS_HIP_BATCH_VERIFY);
+}
+
+std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items, size_t n) {
+  return plan_verify_set(nullptr, 0, items, n, CUBBS_HIP_BATCH_PROOF_VERIFY);
+}
+
+std::unique_ptr<Batch> plan_verify_and_proof_verify(const CuBbsBbsGpuVerifyItem* vitems, size_t nv,
+                                                    const CuBbsBbsGpuProofVerifyItem* pitems,
+                                                    size_t np) {
+  return plan_verify_set(vitems, nv, pitems, np, KIND_VERIFY_SET);
 }
 
 // ------------------------------------------------------------------ Sign
@@ -1285,7 +1350,8 @@ void launch_batch(Batch& b) {
   // Verify / ProofVerify: point decode and the variable-base products run on
   // the auxiliary stream while the message hashing, domain and comb MSM run
   // here; both need the decoded scalars and join before the reduction.
-  const bool fork = (b.kind == CUBBS_HIP_BATCH_VERIFY || b.kind == CUBBS_HIP_BATCH_PROOF_VERIFY) &&
+  const bool fork = (b.kind == CUBBS_HIP_BATCH_VERIFY || b.kind == CUBBS_HIP_BATCH_PROOF_VERIFY ||
+                     b.kind == KIND_VERIFY_SET) &&
                     b.aux != nullptr;
   // Primitive programs decode without the subgroup check too.  The check
   // runs as extra lanes of the product launch, or, with a Pippenger MSM, as
@@ -1538,7 +1604,8 @@ void collect_batch(Batch& b, CuBbsBbsGpuResult* results, uint8_t* const* out_ptr
       continue;
     }
     results[i].code = CUBBS_BBS_GPU_SUCCESS;
-    if (b.kind == CUBBS_HIP_BATCH_VERIFY || b.kind == CUBBS_HIP_BATCH_PROOF_VERIFY) {
+    if (b.kind == CUBBS_HIP_BATCH_VERIFY || b.kind == CUBBS_HIP_BATCH_PROOF_VERIFY ||
+        b.kind == KIND_VERIFY_SET) {
       results[i].value = b.h_valid[k];
     } else if (b.kind == KIND_SIGN) {
       if (out_ptrs) std::memcpy(out_ptrs[i], hout.data() + 80ull * k, 80);

@@ -20,6 +20,9 @@ constexpr int KIND_PROOF_GEN = 3;
 // A primitive program (the MSM / add / pairing entries): status slots instead
 // of items, results assembled by the caller from h_status / h_valid / output.
 constexpr int KIND_PRIM = 4;
+// A Verify batch and a ProofVerify batch staged as one (every stage launches
+// once for the items of both; results: the Verify items, then the others).
+constexpr int KIND_VERIFY_SET = 5;
 
 // One output point of a primitive program: its status slots (gst =
 // PRIM_NO_SLOT when the row has no group slot), the offset of its 48-byte
@@ -202,6 +205,9 @@ void fill_os_random(uint8_t* out, size_t n);
 
 std::unique_ptr<Batch> plan_verify(const CuBbsBbsGpuVerifyItem* items, size_t n);
 std::unique_ptr<Batch> plan_proof_verify(const CuBbsBbsGpuProofVerifyItem* items, size_t n);
+std::unique_ptr<Batch> plan_verify_and_proof_verify(const CuBbsBbsGpuVerifyItem* vitems, size_t nv,
+                                                    const CuBbsBbsGpuProofVerifyItem* pitems,
+                                                    size_t np);
 std::unique_ptr<Batch> plan_sign(const CuBbsBbsGpuSignItem* items, size_t n);
 // mode 0: blinding from the OS CSPRNG, 1: per-item 32-byte seeds, 2: explicit scalars
 std::unique_ptr<Batch> plan_proof_gen(const CuBbsBbsGpuProofGenItem* items, size_t n, int mode,

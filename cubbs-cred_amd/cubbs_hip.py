@@ -17,6 +17,9 @@ ABI_VERSION = 2
 SUCCESS, ERR_INVALID_INPUT, ERR_INSUFFICIENT_OUTPUT, ERR_INTERNAL = 0, 1, 2, 3
 BATCH_VERIFY, BATCH_PROOF_VERIFY = 0, 1
 BATCH_G1_MSM, BATCH_G1_MSM_COMMON_POINTS = 16, 17
+# a Verify and a ProofVerify batch staged as one (cubbs_hip_batch_prepare_verify_set);
+# a tag of this binding, not a kind value of the C ABI
+BATCH_VERIFY_SET = "verify_set"
 BATCH_SINGLE_STREAM, BATCH_AGGREGATE = 1, 2
 
 API_ID = b"BBS_BLS12381G1_XMD:SHA-256_SSWU_RO_H2G_HM2S_"
@@ -64,6 +67,7 @@ EXT_SYMBOLS = [
     "cubbs_hip_batch_profile",
     "cubbs_hip_batch_shape",
     "cubbs_hip_batch_group_checks",
+    "cubbs_hip_batch_prepare_verify_set",
     "cubbs_hip_test_g1_mul",
     "cubbs_hip_test_pairing_gt",
     "cubbs_hip_test_g2_roundtrip",
@@ -229,6 +233,10 @@ def lib():
         _lib.cubbs_hip_batch_prepare.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
                                                  ctypes.c_size_t, ctypes.POINTER(ctypes.c_int32)]
         _lib.cubbs_hip_batch_prepare_flags.restype = ctypes.c_void_p
+        _lib.cubbs_hip_batch_prepare_verify_set.restype = ctypes.c_void_p
+        _lib.cubbs_hip_batch_prepare_verify_set.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t,
+                                                            ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32,
+                                                            ctypes.POINTER(ctypes.c_int32)]
         _lib.cubbs_hip_batch_prepare_flags.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
                                                        ctypes.c_size_t, ctypes.c_uint32,
                                                        ctypes.POINTER(ctypes.c_int32)]
@@ -572,8 +580,29 @@ class StagedBatch:
         size 2..8; supergroup: groups per supergroup check (2..8, with
         aggregate; 0 = the tuned choice when the group size is left open,
         else none); comb_chunk: fixed-base comb terms per lane (Verify /
-        ProofVerify; 0 = the tuned choice, else 4)."""
+        ProofVerify; 0 = the tuned choice, else 4).  kind BATCH_VERIFY_SET:
+        reqs = (verify requests, proof-verify requests), staged as one batch
+        whose results are the Verify results followed by the others."""
         self.keep = _Keep()
+        flags = ((BATCH_SINGLE_STREAM if single_stream else 0) |
+                 (BATCH_AGGREGATE if aggregate else 0) |
+                 (0 if aggregate is True or not aggregate else (int(aggregate) & 15) << 8) |
+                 ((int(comb_chunk) & 127) << 12) |
+                 ((int(supergroup) & 15) << 20))
+        st = ctypes.c_int32(0)
+        self.kind = kind
+        if kind == BATCH_VERIFY_SET:
+            vreqs, preqs = reqs
+            vitems = build_verify_items(vreqs, self.keep)
+            pitems = build_proof_verify_items(preqs, self.keep)
+            self.n = len(vreqs) + len(preqs)
+            self.h = lib().cubbs_hip_batch_prepare_verify_set(
+                ABI_VERSION, ctypes.cast(vitems, ctypes.c_void_p), ctypes.c_size_t(len(vreqs)),
+                ctypes.cast(pitems, ctypes.c_void_p), ctypes.c_size_t(len(preqs)), flags, ctypes.byref(st))
+            self.keep = None
+            if not self.h:
+                raise RuntimeError("batch prepare failed with status %d" % st.value)
+            return
         if kind == BATCH_VERIFY:
             items = build_verify_items(reqs, self.keep)
         elif kind == BATCH_PROOF_VERIFY:
@@ -584,19 +613,10 @@ class StagedBatch:
             items, _ = _common_items(reqs, self.keep, packed=True, outputs=False)
         else:
             raise ValueError("unsupported batch kind")
-        st = ctypes.c_int32(0)
-        self.kind = kind
         # results: one per item, or one per output row of the common-point groups
         self.n = sum(len(rows) for _, rows in reqs) if kind == BATCH_G1_MSM_COMMON_POINTS else len(reqs)
         self.h = lib().cubbs_hip_batch_prepare_flags(ABI_VERSION, kind, ctypes.cast(items, ctypes.c_void_p),
-                                                     ctypes.c_size_t(len(reqs)),
-                                                     (BATCH_SINGLE_STREAM if single_stream else 0) |
-                                                     (BATCH_AGGREGATE if aggregate else 0) |
-                                                     (0 if aggregate is True or not aggregate
-                                                      else (int(aggregate) & 15) << 8) |
-                                                     ((int(comb_chunk) & 127) << 12) |
-                                                     ((int(supergroup) & 15) << 20),
-                                                     ctypes.byref(st))
+                                                     ctypes.c_size_t(len(reqs)), flags, ctypes.byref(st))
         self.keep = None  # items are only read during prepare
         if not self.h:
             raise RuntimeError("batch prepare failed with status %d" % st.value)

@@ -108,6 +108,19 @@ void *cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void *items,
 #define CUBBS_HIP_BATCH_AGG_SUPER(m) (((uint32_t)(m) & 15u) << 20)
 void *cubbs_hip_batch_prepare_flags(uint32_t abi_version, int kind, const void *items,
                                     size_t item_count, uint32_t flags, int32_t *status_out);
+/* A Verify batch and a ProofVerify batch staged as ONE batch: every stage
+ * of the device pipeline (decode, hashing, products, comb, reductions,
+ * challenge, each level of pairing checks) launches once for the items of
+ * both, so each launch holds both batches' work and the pair takes one
+ * stream instead of two.  Flags as cubbs_hip_batch_prepare_flags (the
+ * tuning table's size class is the Verify item count).  The batch's results
+ * (cubbs_hip_batch_results) are verify_count Verify results followed by
+ * proof_count ProofVerify results, each as the separate batch would give. */
+void *cubbs_hip_batch_prepare_verify_set(uint32_t abi_version,
+                                         const CuBbsBbsGpuVerifyItem *verify_items,
+                                         size_t verify_count,
+                                         const CuBbsBbsGpuProofVerifyItem *proof_items,
+                                         size_t proof_count, uint32_t flags, int32_t *status_out);
 int32_t cubbs_hip_batch_launch(void *batch);
 int32_t cubbs_hip_batch_sync(void *batch);
 int32_t cubbs_hip_batch_results(void *batch, CuBbsBbsGpuResult *results);
