@@ -11,14 +11,17 @@ inputs resident in HBM):
     seeded ChaCha20.
 All inputs (keys, signatures, proofs) are produced by the library itself
 (keygen, Sign and ProofGen run on the device) from fixed seeds.  The timed
-region keeps eight staged sets of both batches in flight, each batch on one
-stream of its own (single-stream batches) with its pairing checks aggregated
-per key in groups of 8 (random linear combinations, per-item checks for the
-items of failing groups), each replayed from its captured HIP graph: each
-step waits for the set launched eight steps before, compares its per-item
-results with the expected ones, and launches it again; after the last step
-every set is waited for.  While one set's pairing launches finish, the next
-sets' decode, hashing and MSM stages already run.
+region keeps several staged sets in flight (the pipeline tuning table's
+count, 12).  A set is a verify set: both batches staged as one device batch,
+so every stage launches once for the items of both, on one stream, with the
+pairing checks aggregated per key in groups of 8 (random linear
+combinations; the Verify groups' B sides as one comb MSM per group over
+summed term scalars; per-item checks for the items of failing groups),
+replayed from its captured HIP graph.  Each step waits for the set launched
+that many steps before, compares its per-item results with the expected
+ones, and launches it again; after the last step every set is waited for.
+While one set's pairing launches finish, the next sets' decode, hashing and
+MSM stages already run.  --separate stages each set as two batches instead.
 
 The roofline object prices the timed step's work with the cost model in
 cost_model.json (field multiplications per algorithmic unit, units counted by
@@ -52,17 +55,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cubbs-cred_amd"))
 # HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default), read
 # once at initialisation.  The library keeps one work stream per queue.  The
-# timed loop keeps INFLIGHT sets of both batches in flight, each staged batch
-# on one stream (CUBBS_HIP_BATCH_SINGLE_STREAM), so the measurement runs in a
-# child process with BENCH_HW_QUEUES queues (8 sets x 2 batches: no two
-# batches share one).  Measured: 6.5 ms per step against 7.0 ms for four
-# forked sets (2 streams per batch) at the same 16 queues; more queues, or
-# batches sharing streams, measured slower.  The serving loop's slots take 8
-# streams each (2 slots at 16).
+# timed loop keeps its sets in flight on one stream each
+# (CUBBS_HIP_BATCH_SINGLE_STREAM), so the measurement runs in a child process
+# with BENCH_HW_QUEUES queues.  24 or 32 queues measured slower than 16 (with
+# 12 to 16 sets in flight).  The serving loop's slots take 8 streams each (2
+# slots at 16).
 BENCH_HW_QUEUES = "16"
 SERVING_HW_QUEUES = "16"
-# batch sets in flight in the timed loop: while one set's pairing launches
-# finish, the next set's decode / hashing / MSM stages already run
+# two-batch sets in flight in the timed loop (--separate) when the tuning
+# table has no entry: while one set's pairing launches finish, the next
+# set's decode / hashing / MSM stages already run
 INFLIGHT = 8
 # verify sets in flight (one stream each) when the tuning table has no entry
 INFLIGHT_SETS = 16
@@ -560,7 +562,7 @@ def codec_run(n=2048, reps=5):
             "timing": "best of %d passes, direct C-ABI calls (ctypes call overhead included)" % reps}
 
 
-PROFILE_STEP = os.path.join(ROOT, "profiles", "step_verify8192_proofverify2048_8sets_aggregated.md")
+PROFILE_STEP = os.path.join(ROOT, "profiles", "step_verifyset_verify8192_proofverify2048_aggregated.md")
 PROFILE_PAIRING = os.path.join(ROOT, "profiles", "pairing_verify8192_per_item.md")
 
 
