@@ -345,6 +345,8 @@ void materialize(Batch& b, Plan& p) {
   size_t o_va = take(std::max<uint32_t>(slots, 1));
   size_t o_out = take(std::max<size_t>(p.out_bytes, 1));
   size_t o_l2 = take(4ull * PAIR_L2_WORDS * std::max<size_t>(p.pairs.size(), 1));
+  size_t o_plist = take(4ull * std::max<size_t>(p.pairs.size(), 1));
+  size_t o_pairn = take(4);
   // product workspaces: the plan's jobs, then the aggregation jobs
   size_t o_vt = take(4ull * VAR_WS_WORDS * std::max<size_t>(all_var.size(), 1));
   const PipShape& ps = b.pip;
@@ -373,6 +375,8 @@ void materialize(Batch& b, Plan& p) {
   b.valid = W + o_va;
   b.out = W + o_out;
   b.l2buf = reinterpret_cast<uint32_t*>(W + o_l2);
+  b.pair_list = reinterpret_cast<uint32_t*>(W + o_plist);
+  b.pair_count = reinterpret_cast<uint32_t*>(W + o_pairn);
   b.vtab = reinterpret_cast<uint32_t*>(W + o_vt);
   if (pip_on) {
     auto u32 = [&](size_t o) { return reinterpret_cast<uint32_t*>(W + o); };
@@ -441,7 +445,7 @@ static void agg_coefficient(uint8_t out[32], uint64_t s, uint64_t t) {
 }
 
 void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, uint32_t super_max) {
-  if (group_max < 2 || group_max > 8) return;  // 2 x 8 partials per sum: one final reduction
+  if (group_max < 2 || group_max > 8) return;
   if (super_max < 2 || super_max > 8) super_max = 1;
   std::vector<PairJob> in = std::move(p.pairs);
   p.pairs.clear();
@@ -458,8 +462,9 @@ void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, uint32_t super_ma
   std::vector<PairJob> supers, groups, grouped, single;
   std::vector<ReduceGroup> super_red;
   std::vector<uint32_t> super_out;
-  // the groups of one key: runs of at most group_max checks (a run of one
-  // stays a single check), taken super_max at a time
+  // the groups of one key: runs of at most group_max checks (a key's lone
+  // item stays a single check; its last lone item joins the previous run,
+  // which then holds group_max + 1), taken super_max at a time
   std::vector<std::pair<size_t, size_t>> run;  // [begin, end) in order[]
   auto flush = [&]() {
     for (size_t c0 = 0; c0 < run.size(); c0 += super_max) {
@@ -527,7 +532,9 @@ void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, uint32_t super_ma
              in[order[e]].pk_slot == j0.pk_slot)
         e++;
     if (!run.empty() && (!plain(j0) || in[order[run[0].first]].pk_slot != j0.pk_slot)) flush();
-    if (!plain(j0) || e - i < 2) {
+    if (plain(j0) && e - i == 1 && !run.empty() && run.back().second == i) {
+      run.back().second = e;  // a last lone item of the key joins the previous group
+    } else if (!plain(j0) || e - i < 2) {
       for (size_t k = i; k < e; k++) single.push_back(in[order[k]]);
     } else {
       run.push_back({i, e});
@@ -638,7 +645,8 @@ static void plan_verify_grouped(Plan& p, const CuBbsBbsGpuVerifyItem* items,
   std::vector<uint32_t> Ls(nv);
   for (uint32_t k = 0; k < nv; k++) Ls[k] = (uint32_t)items[dev[k]].message_count;
   // groups: the items of one key ordered by message count, in runs of at
-  // most group_max (a run of one is a single check)
+  // most group_max (a key's lone item is a single check; a last lone item
+  // joins the previous run, which then holds group_max + 1)
   std::vector<uint32_t> order(nv);
   for (uint32_t k = 0; k < nv; k++) order[k] = k;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
@@ -658,6 +666,8 @@ static void plan_verify_grouped(Plan& p, const CuBbsBbsGpuVerifyItem* items,
       const uint32_t re = std::min(e, r + group_max);
       if (re - r >= 2)
         key_runs.back().push_back(Run{r, re});
+      else if (!key_runs.back().empty())
+        key_runs.back().back().end = re;  // a last lone item joins the previous group
       else
         singles.push_back(order[r]);
     }
@@ -769,16 +779,18 @@ static void plan_verify_grouped(Plan& p, const CuBbsBbsGpuVerifyItem* items,
         p.reduce_to(ybegin, yend - ybegin, lb);
         supers.push_back(PairJob{sslot, la, lb, pk, PAIR_NEG_G2, 0, 0});
       }
-      // the items' own checks, for when their group check fails: comb
-      // chunks as in the main launch (they are on the critical path of
-      // the batch), one gated sum per check point
+      // the items' own checks, for when their group check fails: one gated
+      // sum per check point, over comb chunks of at least 4 terms (on this
+      // critical path a longer chunk costs less than the serial additions
+      // of the sum it saves)
+      const uint32_t fch = std::max(CH, 4u);
       for (size_t r = c0; r < c1; r++) {
         const Run& run = runs[r];
         for (uint32_t m = run.begin; m < run.end; m++) {
           const uint32_t k = order[m], L = Ls[k];
           const uint32_t ft = own_terms(k);
           const uint32_t g = gslot[r - c0];
-          const uint32_t cb = p.comb_gated(ft, L + 2, CH, g);
+          const uint32_t cb = p.comb_gated(ft, L + 2, fch, g);
           const uint32_t pa2 = p.points(1), pb = p.points(1);
           p.fb_red.push_back(GatedReduce{fx + (m - first), 1, REDUCE_NO_EXTRA, g, pa2});   // c A
           p.fb_red.push_back(GatedReduce{cb, p.n_partials - cb, vb[r - c0] + (m - run.begin), g, pb});
@@ -1530,15 +1542,24 @@ void launch_batch(Batch& b) {
       Timed t(b, "reduce_gated");
       launch_reduce_gated(s, b.fb_red, b.n_fb_red, b.partials, b.points, b.pinf, b.valid);
     }
+    // after group checks, only the per-item checks that still need their own
+    // pairing are launched, listed contiguously
+    const uint32_t n_rest = b.n_pairs - done;
+    uint32_t* list = done ? b.pair_list : nullptr;
+    uint32_t* count = done ? b.pair_count : nullptr;
+    if (done) {
+      Timed t(b, "pair_select");
+      launch_fill_u32(s, count, 1, 0);
+      launch_pair_select(s, b.pairs + done, n_rest, b.pk_status, b.status, b.gate, b.valid, list, count);
+    }
     {
       Timed t(b, "pair_lines");
-      launch_pair_lines(s, b.pairs + done, b.n_pairs - done, b.points, b.pinf, b.pk_status,
-                        b.pk_lines, b.neg_g2_lines, b.l2buf + (size_t)done * PAIR_L2_WORDS,
-                        done ? b.valid : nullptr);
+      launch_pair_lines(s, b.pairs + done, n_rest, b.points, b.pinf, b.pk_status, b.pk_lines,
+                        b.neg_g2_lines, b.l2buf + (size_t)done * PAIR_L2_WORDS, nullptr, list, count);
     }
     Timed t(b, "pairing");
-    launch_pairing(s, b.pairs + done, b.n_pairs - done, b.points, b.pinf, b.pk_status,
-                   b.l2buf + (size_t)done * PAIR_L2_WORDS, b.status, b.gate, b.valid);
+    launch_pairing(s, b.pairs + done, n_rest, b.points, b.pinf, b.pk_status,
+                   b.l2buf + (size_t)done * PAIR_L2_WORDS, b.status, b.gate, b.valid, list, count);
   }
   check(hipGetLastError());
 }

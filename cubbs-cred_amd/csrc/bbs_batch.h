@@ -174,6 +174,8 @@ struct Batch {
   uint8_t* valid = nullptr;
   uint8_t* out = nullptr;
   uint32_t* l2buf = nullptr;          // per pairing check: the line values -lambda x_P
+  uint32_t* pair_list = nullptr;      // per-item checks that run after the group checks
+  uint32_t* pair_count = nullptr;     // and their number
   uint32_t* vtab = nullptr;           // VAR_WS_WORDS per variable-base product job
   uint32_t n_scalars = 0, n_points = 0, n_partials = 0;
   // snapshot of resident state

@@ -273,13 +273,25 @@ void launch_reduce_partials(hipStream_t s, const ReduceGroup* groups, const uint
                             uint32_t* partials);
 void launch_compress(hipStream_t s, const CompressJob* jobs, uint32_t n, const uint32_t* points,
                      const uint8_t* pinf, uint8_t* out);
+// The per-item checks that follow group checks: every job that needs no
+// pairing (accepted through its group, or not live) gets its verdict here;
+// the others are listed in list[0 .. *count) (count zeroed beforehand), so
+// that the list-driven pair_lines / pairing launches below hold only checks
+// that run.
+void launch_pair_select(hipStream_t s, const PairJob* jobs, uint32_t n, const uint8_t* pk_status,
+                        uint8_t* status, const uint8_t* gate, uint8_t* valid, uint32_t* list,
+                        uint32_t* count);
+// With list: job list[i] for i < *count (n bounds the grid), line blocks at
+// slot i of l2buf.
 void launch_pair_lines(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
                        const uint8_t* pinf, const uint8_t* pk_status, const uint32_t* pk_lines,
                        const uint32_t* neg_g2_lines, uint32_t* l2buf,
-                       const uint8_t* group_valid = nullptr);
+                       const uint8_t* group_valid = nullptr, const uint32_t* list = nullptr,
+                       const uint32_t* count = nullptr);
 void launch_pairing(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
                     const uint8_t* pinf, const uint8_t* pk_status, uint32_t* l2buf, uint8_t* status,
-                    const uint8_t* gate, uint8_t* valid);
+                    const uint8_t* gate, uint8_t* valid, const uint32_t* list = nullptr,
+                    const uint32_t* count = nullptr);
 void launch_sign_e(hipStream_t s, const SignJob* jobs, uint32_t n, const uint8_t* blob,
                    const uint8_t* pk_status, uint32_t* scalars, uint8_t* status);
 void launch_sign_finalize(hipStream_t s, const SignJob* jobs, uint32_t n, const uint32_t* points,

@@ -482,10 +482,14 @@ __global__ __launch_bounds__(256) void k_pair_lines(const PairJob* __restrict__ 
                                                    const uint32_t* __restrict__ pk_lines,
                                                    const uint32_t* __restrict__ neg_g2_lines,
                                                    uint32_t* __restrict__ l2buf,
-                                                   const uint8_t* __restrict__ group_valid) {
+                                                   const uint8_t* __restrict__ group_valid,
+                                                   const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ count) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * (uint32_t)(MILLER_LINES * 2)) return;
-  const uint32_t job = t / (MILLER_LINES * 2);
+  const uint32_t slot = t / (MILLER_LINES * 2);
+  if (list && slot >= *count) return;
+  const uint32_t job = list ? list[slot] : slot;
   const uint32_t r = t % (MILLER_LINES * 2);
   const uint32_t idx = r >> 1, q = r & 1;
   const PairJob j = jobs[job];
@@ -507,16 +511,63 @@ __global__ __launch_bounds__(256) void k_pair_lines(const PairJob* __restrict__ 
   const uint8_t pks = pk_status[j.pk_slot];
   const uint8_t pkb = j.pk_b == PAIR_NEG_G2 ? 1 : pk_status[j.pk_b];
   const bool skip = q ? (pinf[j.b] || pkb == 2) : (pinf[j.a] || pks == 2);
-  p12::line_values(l2buf + (size_t)job * p12::L2_WORDS + ((size_t)idx * 2 + q) * p12::LV_N * FPW,
+  p12::line_values(l2buf + (size_t)slot * p12::L2_WORDS + ((size_t)idx * 2 + q) * p12::LV_N * FPW,
                    m0, m1, c0, c1, skip);
 }
 void launch_pair_lines(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
                        const uint8_t* pinf, const uint8_t* pk_status, const uint32_t* pk_lines,
-                       const uint32_t* neg_g2_lines, uint32_t* l2buf, const uint8_t* group_valid) {
+                       const uint32_t* neg_g2_lines, uint32_t* l2buf, const uint8_t* group_valid,
+                       const uint32_t* list, const uint32_t* count) {
   if (!n) return;
   const uint32_t nl = n * (uint32_t)(MILLER_LINES * 2);
   hipLaunchKernelGGL(k_pair_lines, GRID(nl, 64), 0, s, jobs, n, points, pinf, pk_status, pk_lines,
-                     neg_g2_lines, l2buf, group_valid);
+                     neg_g2_lines, l2buf, group_valid, list, count);
+}
+
+// Per-item checks after group checks: a job accepted through its group (the
+// item's own equation then holds except with probability 2^-128) or not live
+// (an item already rejected, an invalid key, a failed challenge gate) gets
+// its verdict here; the jobs that need their own pairing are listed
+// contiguously (one atomic per wave, lanes in order), so the pairing launch
+// holds only blocks of checks that run.  Left among many empty blocks, the
+// few live ones were placed unevenly by the dispatcher (two on one SIMD),
+// which lengthened the launch by about half a check.
+__global__ __launch_bounds__(64) void k_pair_select(const PairJob* __restrict__ jobs, uint32_t n,
+                                                   const uint8_t* __restrict__ pk_status,
+                                                   uint8_t* __restrict__ status,
+                                                   const uint8_t* __restrict__ gate,
+                                                   uint8_t* __restrict__ valid,
+                                                   uint32_t* __restrict__ list,
+                                                   uint32_t* __restrict__ count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool need = false;
+  if (i < n) {
+    const PairJob j = jobs[i];
+    const uint8_t pks = pk_status[j.pk_slot];
+    const uint8_t pkb = j.pk_b == PAIR_NEG_G2 ? 1 : pk_status[j.pk_b];
+    const bool live = status[j.item] == 0 && pks != 0 && pkb != 0 && gate[j.item];
+    const bool accepted = live && (j.flags & PAIR_GROUPED) && valid[j.group];
+    need = live && !accepted;
+    if (!need) {
+      if (pks == 0 || pkb == 0) status[j.item] = 1;
+      valid[j.item] = accepted ? 1 : 0;
+    }
+  }
+  const uint64_t m = __ballot(need);
+  if (!m) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t first = (uint32_t)__ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == first) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = (uint32_t)__shfl((int)base, (int)first);
+  if (need) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = i;
+}
+void launch_pair_select(hipStream_t s, const PairJob* jobs, uint32_t n, const uint8_t* pk_status,
+                        uint8_t* status, const uint8_t* gate, uint8_t* valid, uint32_t* list,
+                        uint32_t* count) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_pair_select, GRID(n, 64), 0, s, jobs, n, pk_status, status, gate, valid, list,
+                     count);
 }
 
 __global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ jobs, uint32_t n,
@@ -526,13 +577,17 @@ __global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ j
                                                const uint32_t* __restrict__ l2buf,
                                                uint8_t* __restrict__ status,
                                                const uint8_t* __restrict__ gate,
-                                               uint8_t* __restrict__ valid) {
+                                               uint8_t* __restrict__ valid,
+                                               const uint32_t* __restrict__ list,
+                                               const uint32_t* __restrict__ count) {
   __shared__ uint32_t lds_all[p12::BLOCK_LDS_WORDS];
   const int lane = threadIdx.x;
   const int grp = lane / 12;
   const p12::Lane L = p12::lane_setup(lds_all, lane);
-  const uint32_t job = blockIdx.x * p12::CHECKS_PER_BLOCK + grp;
-  const bool mine = grp < p12::CHECKS_PER_BLOCK && job < n;
+  // slot: this check's place in the launch (its line blocks); job: its record
+  const uint32_t slot = blockIdx.x * p12::CHECKS_PER_BLOCK + grp;
+  const bool mine = grp < p12::CHECKS_PER_BLOCK && slot < (list ? *count : n);
+  const uint32_t job = list && mine ? list[slot] : slot;
   bool live = mine, accepted = false;
   PairJob j{};
   uint8_t pks = 1, pkb = 1;
@@ -568,7 +623,7 @@ __global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ j
       fp_load_const(yp1, points + (size_t)j.b * G1A_WORDS + FPW);
       if (j.flags & PAIR_NEG_B) fp_neg(yp1, yp1);
     }
-    l2 = l2buf + (size_t)job * p12::L2_WORDS;
+    l2 = l2buf + (size_t)slot * p12::L2_WORDS;
   }
   Fp f, r;
   p12::miller(f, l2, yp0, yp1, L);
@@ -581,10 +636,11 @@ __global__ __launch_bounds__(64, 2) void k_pairing(const PairJob* __restrict__ j
 }
 void launch_pairing(hipStream_t s, const PairJob* jobs, uint32_t n, const uint32_t* points,
                     const uint8_t* pinf, const uint8_t* pk_status, uint32_t* l2buf, uint8_t* status,
-                    const uint8_t* gate, uint8_t* valid) {
+                    const uint8_t* gate, uint8_t* valid, const uint32_t* list, const uint32_t* count) {
   if (!n) return;
   hipLaunchKernelGGL(k_pairing, dim3((n + p12::CHECKS_PER_BLOCK - 1) / p12::CHECKS_PER_BLOCK),
-                     dim3(64), 0, s, jobs, n, points, pinf, pk_status, l2buf, status, gate, valid);
+                     dim3(64), 0, s, jobs, n, points, pinf, pk_status, l2buf, status, gate, valid, list,
+                     count);
 }
 
 // ------------------------------------------------------------------ Sign

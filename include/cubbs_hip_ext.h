@@ -79,8 +79,9 @@ void *cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void *items,
  * (code 1). */
 #define CUBBS_HIP_BATCH_SINGLE_STREAM 1u
 /* CUBBS_HIP_BATCH_AGGREGATE (Verify / ProofVerify): the pairing checks of
- * items under one key are also tested in groups of up to 8 as one random
- * linear combination, e(sum c_i a_i, PK) e(sum c_i b_i, -G2) = 1, with
+ * items under one key are also tested in groups of up to n (the group size
+ * below; n + 1 when a key's last item would otherwise be checked alone) as
+ * one random linear combination, e(sum c_i a_i, PK) e(sum c_i b_i, -G2) = 1, with
  * c_i = s_i + t_i lambda (s_i, t_i two 64-bit values from the OS CSPRNG drawn
  * at prepare, 2^128 distinct coefficients).  Items of a
  * group that holds are accepted without their own check; the items of a

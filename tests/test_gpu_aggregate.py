@@ -149,7 +149,8 @@ def test_supergroup_levels_gate_groups_and_items(gpu, oracle):
     rng = random.Random(6)
     kps = _issuers(g, rng, 3)
     # keys 0 and 1: 20 items each (groups of 8, 8, 4: one supergroup of 3
-    # groups); key 2: 9 items (a group of 8 and a single check)
+    # groups); key 2: 9 items (one group of 9: a key's last lone item joins
+    # the previous group)
     owners = [i % 2 for i in range(40)] + [2] * 9
     sreqs = [(kps[k][0], kps[k][1], b"", [rng.randbytes(20)]) for k in owners]
     rc, sigs = g.sign_batch(sreqs)
@@ -174,8 +175,8 @@ def test_supergroup_levels_gate_groups_and_items(gpu, oracle):
                 w = b.work()
                 assert w["group_checks"] == 9 and w["pair_checks"] == len(reqs)
                 # run: 2 supergroup checks, key 0's 3 groups and key 2's
-                # group, the 8 items of key 0's failing group, key 2's single
-                assert w["pair_checks_run"] == 2 + 4 + 8 + 1
+                # group, the 8 items of key 0's failing group
+                assert w["pair_checks_run"] == 2 + 4 + 8
     finally:
         b.free()
 
@@ -239,8 +240,8 @@ def test_aggregate_prepare_flag(gpu, oracle):
             b.launch()
             b.sync()
         assert agg.results() == plain.results() == [(0, i != 20) for i in range(33)]
-        # 17 + 16 items per key in groups of at most 8: 8 + 8 (+ one item
-        # checked alone) and 8 + 8; item 20 (key 0) fails its group only
+        # 17 + 16 items per key: 8 + 9 (the last lone item joins the
+        # previous group) and 8 + 8; item 20 (key 0) fails its group only
         assert plain.group_checks() == (0, 0)
         assert agg.group_checks() == (4, 3)
     finally:
