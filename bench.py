@@ -86,7 +86,8 @@ def work_fpm(w):
     f = COST["fpm"]
     return (w["comb_madds"] * f["comb_madd"] + w["glv_products"] * f["glv_product"] +
             w["agg_products"] * f["agg_product"] + w["decoded_points"] * f["decoded_point"] +
-            w["point_adds"] * f["point_add"] + w["pair_checks_run"] * f["pair_check"])
+            w["point_adds"] * f["point_add"] + w["pair_checks_run"] * f["pair_check"] +
+            w["straus_sums"] * f["straus_chain"] + w["straus_terms"] * f["straus_term"])
 
 
 def log(msg):

@@ -140,6 +140,9 @@ static void count_work(Batch& b, const Plan& p) {
   w.comb_madds = b.comb_madds;
   w.glv_products = p.var.size();
   w.agg_products = p.agg_var.size();
+  w.straus_sums = p.var_sum.size();
+  w.straus_terms = 0;
+  for (const auto& v : p.var_sum) w.straus_terms += v.count;
   w.decoded_points = p.dec_g1.size();
   uint64_t adds = 0;
   for (const auto& g : p.groups1) adds += g.count ? g.count - 1 : 0;
@@ -202,6 +205,16 @@ void materialize(Batch& b, Plan& p) {
   std::vector<VarJob> all_var(p.var);
   all_var.insert(all_var.end(), p.agg_var.begin(), p.agg_var.end());
   size_t o_var = place(all_var.data(), all_var.size() * sizeof(VarJob));
+  // multi-term products: the longest first (a wave's lanes then run the same
+  // number of terms), each with its own workspace
+  std::stable_sort(p.var_sum.begin(), p.var_sum.end(),
+                   [](const VarSum& x, const VarSum& y) { return x.count > y.count; });
+  uint64_t sum_ws_words = 0;
+  for (auto& v : p.var_sum) {
+    v.ws = (uint32_t)sum_ws_words;
+    sum_ws_words += var_sum_ws_words(v.count);
+  }
+  size_t o_vsum = place(p.var_sum.data(), p.var_sum.size() * sizeof(VarSum));
   size_t o_chk = place(p.chunks.data(), p.chunks.size() * sizeof(CombChunk));
   size_t o_tg = place(p.term_gen.data(), p.term_gen.size() * 4);
   size_t o_g1 = place(p.groups1.data(), p.groups1.size() * sizeof(ReduceGroup));
@@ -252,6 +265,8 @@ void materialize(Batch& b, Plan& p) {
   b.n_fills = (uint32_t)p.fills.size();
   b.var = reinterpret_cast<VarJob*>(D + o_var);
   b.n_var = (uint32_t)all_var.size();
+  b.var_sum = reinterpret_cast<VarSum*>(D + o_vsum);
+  b.n_var_sum = (uint32_t)p.var_sum.size();
   b.chunks = reinterpret_cast<CombChunk*>(D + o_chk);
   b.n_chunks = (uint32_t)p.chunks.size();
   b.term_gen = reinterpret_cast<uint32_t*>(D + o_tg);
@@ -349,6 +364,7 @@ void materialize(Batch& b, Plan& p) {
   size_t o_pairn = take(4);
   // product workspaces: the plan's jobs, then the aggregation jobs
   size_t o_vt = take(4ull * VAR_WS_WORDS * std::max<size_t>(all_var.size(), 1));
+  size_t o_sws = take(4ull * std::max<uint64_t>(sum_ws_words, 1));
   const PipShape& ps = b.pip;
   const size_t pip_on = ps.terms ? 1 : 0;
   size_t o_pcnt = take(4ull * (ps.nbt + 1) * pip_on);
@@ -378,6 +394,7 @@ void materialize(Batch& b, Plan& p) {
   b.pair_list = reinterpret_cast<uint32_t*>(W + o_plist);
   b.pair_count = reinterpret_cast<uint32_t*>(W + o_pairn);
   b.vtab = reinterpret_cast<uint32_t*>(W + o_vt);
+  b.sum_ws = reinterpret_cast<uint32_t*>(W + o_sws);
   if (pip_on) {
     auto u32 = [&](size_t o) { return reinterpret_cast<uint32_t*>(W + o); };
     b.pipb = PipBuffers{u32(o_pcnt), u32(o_poff), u32(o_pcur), u32(o_ppof), u32(o_pmap),
@@ -985,7 +1002,10 @@ static void plan_proof_verify_items(Plan& p, const CuBbsBbsGpuProofVerifyItem* i
         u++;
       }
     }
-    // T1 = c Bbar + e^ Abar + r1^ D
+    // T1 = c Bbar + e^ Abar + r1^ D as three single products: one
+    // multi-term job (shared doublings) does 1/3 less work but takes twice
+    // a product's latency, which a pipeline of a few batches in flight feels
+    // more than the work (measured: 3.05 -> 3.22 ms per 20-step step)
     uint32_t f1 = p.n_partials;
     p.var.push_back(VarJob{pbb, s_c, p.partials(VAR_PARTIALS)});
     p.var.push_back(VarJob{pab, s_ehat, p.partials(VAR_PARTIALS)});
@@ -1262,29 +1282,27 @@ std::unique_ptr<Batch> plan_proof_gen(const CuBbsBbsGpuProofGenItem* items, size
         u++;
       }
     }
-    constexpr uint32_t V = VAR_PARTIALS;
-    uint32_t q = p.partials(7 * V);
-    p.var.push_back(VarJob{pb, kk + 0, q + 0 * V});   // D = r2 B
-    p.var.push_back(VarJob{pa, kk + 1, q + 1 * V});   // Abar = r1 r2 A
-    p.var.push_back(VarJob{pb, kk + 2, q + 2 * V});   // Bbar = r1 r2 B
-    p.var.push_back(VarJob{pa, kk + 3, q + 3 * V});   //       - r1 r2 e A
-    p.var.push_back(VarJob{pa, kk + 4, q + 4 * V});   // T1 = r1 r2 e~ A
-    p.var.push_back(VarJob{pb, kk + 5, q + 5 * V});   //      + r2 r1~ B
-    p.var.push_back(VarJob{pb, kk + 6, q + 6 * V});   // T2 = r2 r3~ B + sum
+    // the two-term sums Bbar and T1 as multi-term products (shared doublings)
+    uint32_t q = p.partials(5);
+    p.var.push_back(VarJob{pb, kk + 0, q + 0});               // D = r2 B
+    p.var.push_back(VarJob{pa, kk + 1, q + 1});               // Abar = r1 r2 A
+    p.var_terms({{pb, kk + 2}, {pa, kk + 3}}, q + 2);         // Bbar = r1 r2 B - r1 r2 e A
+    p.var_terms({{pa, kk + 4}, {pb, kk + 5}}, q + 3);         // T1 = r1 r2 e~ A + r2 r1~ B
+    p.var.push_back(VarJob{pb, kk + 6, q + 4});               // T2 = r2 r3~ B + sum
     uint32_t ft2 = p.n_partials;
     p.comb(term, U, CH);
     term += U;
-    p.groups2.push_back(ReduceGroup{q + 0 * V, V});
+    p.groups2.push_back(ReduceGroup{q + 0, 1});
     p.out2.push_back(pd);
-    p.groups2.push_back(ReduceGroup{q + 1 * V, V});
+    p.groups2.push_back(ReduceGroup{q + 1, 1});
     p.out2.push_back(pab);
-    p.groups2.push_back(ReduceGroup{q + 2 * V, 2 * V});
+    p.groups2.push_back(ReduceGroup{q + 2, 1});
     p.out2.push_back(pbb);
-    p.groups2.push_back(ReduceGroup{q + 4 * V, 2 * V});
+    p.groups2.push_back(ReduceGroup{q + 3, 1});
     p.out2.push_back(pt1);
     // T2 partials: the last variable job's then the comb chunks [ft2, n_partials),
-    // contiguous because ft2 == q + 7 V (partials(7 V) was taken just before).
-    p.groups2.push_back(ReduceGroup{q + 6 * V, V + (p.n_partials - ft2)});
+    // contiguous because ft2 == q + 5 (partials(5) was taken just before).
+    p.groups2.push_back(ReduceGroup{q + 4, 1 + (p.n_partials - ft2)});
     p.out2.push_back(pt2);
     PgJob job{};
     job.ph_off = ph;
@@ -1386,10 +1404,10 @@ void launch_batch(Batch& b) {
       Timed t(b, "decode_g1", b.aux);
       launch_decode_g1(b.aux, b.dec_g1, b.n_dec_g1, b.blob, b.points, b.pinf, b.status, false);
     }
-    if (b.n_var || b.n_dec_g1) {
+    if (b.n_var || b.n_dec_g1 || b.n_var_sum) {
       Timed t(b, "var_mul_glv", b.aux);
       launch_var_mul(b.aux, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials, b.vtab, b.dec_g1,
-                     b.n_dec_g1, b.status);
+                     b.n_dec_g1, b.status, b.var_sum, b.n_var_sum, b.sum_ws);
     }
     check(hipEventRecord(b.ev_join, b.aux));
   } else if (b.n_dec_g1) {
@@ -1415,11 +1433,12 @@ void launch_batch(Batch& b) {
     Timed t(b, "sign_e");
     launch_sign_e(s, b.sign, b.n_sign, b.blob, b.pk_status, b.scalars, b.status);
   }
-  if (!fork && b.kind != KIND_PROOF_GEN && (b.n_var || (prim_split && !side_check))) {
+  if (!fork && b.kind != KIND_PROOF_GEN && (b.n_var || b.n_var_sum || (prim_split && !side_check))) {
     const bool with_check = prim_split && !side_check;
     Timed t(b, "var_mul_glv");
     launch_var_mul(s, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials, b.vtab,
-                   with_check ? b.dec_g1 : nullptr, with_check ? b.n_dec_g1 : 0, b.status);
+                   with_check ? b.dec_g1 : nullptr, with_check ? b.n_dec_g1 : 0, b.status, b.var_sum,
+                   b.n_var_sum, b.sum_ws);
   }
   if (b.n_chunks) {
     Timed t(b, "comb_msm");
@@ -1478,7 +1497,8 @@ void launch_batch(Batch& b) {
     }
     {
       Timed t(b, "var_mul_glv");
-      launch_var_mul(s, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials, b.vtab);
+      launch_var_mul(s, b.var, b.n_var, b.points, b.pinf, b.scalars, b.partials, b.vtab, nullptr, 0,
+                     nullptr, b.var_sum, b.n_var_sum, b.sum_ws);
     }
     {
       Timed t(b, "reduce2");

@@ -177,6 +177,9 @@ struct Batch {
   uint32_t* pair_list = nullptr;      // per-item checks that run after the group checks
   uint32_t* pair_count = nullptr;     // and their number
   uint32_t* vtab = nullptr;           // VAR_WS_WORDS per variable-base product job
+  VarSum* var_sum = nullptr;          // multi-term product jobs
+  uint32_t n_var_sum = 0;
+  uint32_t* sum_ws = nullptr;         // their workspace
   uint32_t n_scalars = 0, n_points = 0, n_partials = 0;
   // snapshot of resident state
   const uint32_t* tables = nullptr;

@@ -122,6 +122,21 @@ struct VarJob {
   uint32_t scalar;
   uint32_t partial;
 };
+// partials[partial] = the sum over t < count of scalars[scalar[t]] *
+// points[point[t]] (identity-aware) on one lane: the GLV halves of all terms
+// share one chain of doublings (Straus over 2 * count half-points).  Its
+// workspace (tables of every term and the temporaries) is
+// var_sum_ws_words(count) words from word `ws` of the sums' workspace.
+constexpr uint32_t VAR_TERMS_MAX = 4;
+struct VarSum {
+  uint32_t point[VAR_TERMS_MAX];
+  uint32_t scalar[VAR_TERMS_MAX];
+  uint32_t count;
+  uint32_t partial;
+  uint32_t ws;
+  uint32_t pad;
+};
+constexpr uint32_t var_sum_ws_words(uint32_t count) { return count * 2 * VAR_TABLE_WORDS + 7 * 42; }
 
 // Fill one fixed-base term scalar: mode 0: scalars[dst] = scalars[a];
 // mode 1: scalars[dst] = scalars[a] * scalars[b] mod r; mode 2: scalars[dst] = 1;
@@ -258,10 +273,12 @@ void launch_decode_g1(hipStream_t s, const DecodeG1Job* jobs, uint32_t n, const 
 void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const uint8_t* blob,
                       uint32_t* scalars, uint8_t* status);
 // ws: n * VAR_WS_WORDS words of device workspace.  n_sub > 0: also check
-// the subgroup membership of the points of sub[]
+// the subgroup membership of the points of sub[].  sums: multi-term jobs
+// (their workspace sum_ws), launched in the same grid before the products.
 void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_t* points,
                     const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials, uint32_t* ws,
-                    const DecodeG1Job* sub = nullptr, uint32_t n_sub = 0, uint8_t* status = nullptr);
+                    const DecodeG1Job* sub = nullptr, uint32_t n_sub = 0, uint8_t* status = nullptr,
+                    const VarSum* sums = nullptr, uint32_t n_sums = 0, uint32_t* sum_ws = nullptr);
 void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t* scalars,
                       const uint32_t* sum_pairs = nullptr);
 // l2buf: PAIR_L2_WORDS words of device scratch per job, filled by

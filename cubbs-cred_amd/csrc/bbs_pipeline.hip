@@ -307,10 +307,80 @@ DEV void var_mul_one(G1J& r, const G1A& p, const uint32_t k[8], uint32_t* ws) {
   r = acc;
 }
 
-// Lanes [0, n): one product per job (workspace ws + t * VAR_WS_WORDS).
-// Lanes [n, n + n_sub): the prime-order subgroup check of the points decoded
-// by jobs sub[] without it, run beside the products; a failure marks the
-// item's status (the products of such a point are computed but never used).
+// [sum_t k_t P_t] on one lane for the terms of a multi-term job: the
+// tables of 1..8 P_t and 1..8 phi(P_t) of every term in ws, the signed 4-bit
+// digits of both GLV halves of every term in LDS (word w of term t at
+// dg[(10 t + w) * 64 + lane]), then one chain of doublings from the highest
+// non-zero window down, with at most 2 * count mixed additions per window.
+DEV void var_mul_sum(G1J& r, const VarSum& j, const uint32_t* __restrict__ points,
+                     const uint8_t* __restrict__ pinf, const uint32_t* __restrict__ scalars,
+                     uint32_t* ws, uint32_t* dg, uint32_t lane) {
+  uint32_t* const jw = ws + j.count * 2 * VAR_TABLE_WORDS;
+  int top = -1;
+#pragma unroll 1
+  for (uint32_t t = 0; t < j.count; t++) {
+    uint32_t d1[5] = {0x88888888u, 0x88888888u, 0x88888888u, 0x88888888u, 8u};
+    uint32_t d2[5] = {0x88888888u, 0x88888888u, 0x88888888u, 0x88888888u, 8u};  // all digits 0
+    if (!pinf[j.point[t]]) {
+      G1A p;
+      load_g1a(p, points + (size_t)j.point[t] * G1A_WORDS);
+      Fr k;
+      load_fr(k, scalars + (size_t)j.scalar[t] * FR_WORDS);
+      uint32_t k1[4], k2[4];
+      glv_split(k1, k2, k.v);
+      uint32_t* const t1 = ws + t * 2 * VAR_TABLE_WORDS;
+      uint32_t* const t2 = t1 + VAR_TABLE_WORDS;
+      var_table(p, t1, jw);
+      Fp beta;
+      fp_load_const(beta, bls::GLV_BETA);
+#pragma unroll 1
+      for (int e = 0; e < 8; e++) {  // phi(e P) = (beta x, y)
+        G1A q;
+        load_g1a(q, t1 + e * G1A_WORDS);
+        fp_mul(q.x, q.x, beta);
+        store_g1a(t2 + e * G1A_WORDS, q);
+      }
+      var_digits(d1, k1);
+      var_digits(d2, k2);
+    }
+#pragma unroll
+    for (int w = 0; w < 5; w++) {
+      dg[(10 * t + w) * 64 + lane] = d1[w];
+      dg[(10 * t + 5 + w) * 64 + lane] = d2[w];
+    }
+    int tt = 32;
+    while (tt >= 0 && var_digit(d1, tt) == 0 && var_digit(d2, tt) == 0) tt--;
+    top = tt > top ? tt : top;
+  }
+  G1J acc;
+  pt_set_inf(acc);
+#pragma unroll 1
+  for (int i = top; i >= 0; i--) {
+    if (i != top) {
+#pragma unroll 1
+      for (int s = 0; s < 4; s++) g1_dbl_in(acc);
+    }
+#pragma unroll 1
+    for (uint32_t h = 0; h < 2 * j.count; h++) {  // (term, GLV half)
+      const uint32_t w = dg[(5 * h + (uint32_t)(i >> 3)) * 64 + lane];
+      const int d = (int)((w >> ((i & 7) * 4)) & 15u) - 8;
+      if (d) {
+        G1A e;
+        var_entry(e, ws + h * VAR_TABLE_WORDS, d);
+        if (d < 0) fp_neg(e.y, e.y);
+        g1_madd_in(acc, e);
+      }
+    }
+  }
+  r = acc;
+}
+
+// Lanes [0, ns): the multi-term jobs (ns = n_sums rounded up to whole waves,
+// so no wave mixes them with single products).  Lanes [ns, ns + n): one
+// product per job (workspace ws + t * VAR_WS_WORDS).  Lanes after those, up
+// to n_sub more: the prime-order subgroup check of the points decoded by jobs
+// sub[] without it, run beside the products; a failure marks the item's
+// status (the products of such a point are computed but never used).
 __global__ __launch_bounds__(64, 2) void k_var_mul(const VarJob* __restrict__ jobs, uint32_t n,
                                                const uint32_t* __restrict__ points,
                                                const uint8_t* __restrict__ pinf,
@@ -318,8 +388,21 @@ __global__ __launch_bounds__(64, 2) void k_var_mul(const VarJob* __restrict__ jo
                                                uint32_t* __restrict__ partials,
                                                uint32_t* __restrict__ ws,
                                                const DecodeG1Job* __restrict__ sub, uint32_t n_sub,
-                                               uint8_t* __restrict__ status) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+                                               uint8_t* __restrict__ status,
+                                               const VarSum* __restrict__ sums, uint32_t n_sums,
+                                               uint32_t* __restrict__ sum_ws) {
+  extern __shared__ uint32_t dg[];  // 10 * VAR_TERMS_MAX * 64 words when n_sums > 0
+  const uint32_t ns = (n_sums + 63) & ~63u;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < ns) {
+    if (g >= n_sums) return;
+    const VarSum j = sums[g];
+    G1J r;
+    var_mul_sum(r, j, points, pinf, scalars, sum_ws + j.ws, dg, threadIdx.x);
+    store_g1j(partials + (size_t)j.partial * G1J_WORDS, r);
+    return;
+  }
+  const uint32_t t = g - ns;
   if (t >= n) {
     const uint32_t k = t - n;
     if (k >= n_sub) return;
@@ -342,10 +425,13 @@ __global__ __launch_bounds__(64, 2) void k_var_mul(const VarJob* __restrict__ jo
 }
 void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_t* points,
                     const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials, uint32_t* ws,
-                    const DecodeG1Job* sub, uint32_t n_sub, uint8_t* status) {
-  if (!n && !n_sub) return;
-  hipLaunchKernelGGL(k_var_mul, GRID(n + n_sub, 64), 0, s, jobs, n, points, pinf, scalars,
-                     partials, ws, sub, n_sub, status);
+                    const DecodeG1Job* sub, uint32_t n_sub, uint8_t* status, const VarSum* sums,
+                    uint32_t n_sums, uint32_t* sum_ws) {
+  if (!n && !n_sub && !n_sums) return;
+  const uint32_t ns = (n_sums + 63) & ~63u;
+  const size_t lds = n_sums ? 4ull * 10 * VAR_TERMS_MAX * 64 : 0;
+  hipLaunchKernelGGL(k_var_mul, GRID(ns + n + n_sub, 64), lds, s, jobs, n, points, pinf, scalars,
+                     partials, ws, sub, n_sub, status, sums, n_sums, sum_ws);
 }
 
 // ------------------------------------------------------------------ reduction

@@ -45,6 +45,7 @@ struct Plan {
   // groups, and how many of pairs[] (at its front) are combined checks: the
   // supergroup checks first (n_super_pairs), then the group checks
   std::vector<VarJob> agg_var;       // run in the same launch as `var`, after it
+  std::vector<VarSum> var_sum;       // multi-term products (same launch, first)
   uint32_t n_agg_slots = 0, n_group_pairs = 0, n_super_pairs = 0;
   // (x, y) scalar index pairs of the summed term fills (TermFill mode 3)
   std::vector<uint32_t> sum_pairs;
@@ -179,6 +180,18 @@ struct Plan {
       fb_gate.push_back(gate);
     }
     return first;
+  }
+  // partials[partial] = sum of scalars[s] * points[p] over the (p, s) terms
+  // given (at most VAR_TERMS_MAX), as one multi-term product job.
+  void var_terms(const std::vector<std::pair<uint32_t, uint32_t>>& ps, uint32_t partial) {
+    VarSum v{};
+    v.count = (uint32_t)ps.size();
+    for (uint32_t t = 0; t < v.count; t++) {
+      v.point[t] = ps[t].first;
+      v.scalar[t] = ps[t].second;
+    }
+    v.partial = partial;
+    var_sum.push_back(v);
   }
   // A summed term fill: scalars[dst] = sum of scalars[x] * scalars[y] over
   // the (x, y) pairs given.

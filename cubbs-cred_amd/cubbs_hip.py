@@ -679,7 +679,8 @@ class StagedBatch:
         return dict(zip(["items", "msm_terms", "comb_madds", "pairings"], [x.value for x in v]))
 
     WORK_FIELDS = ("items", "comb_terms", "comb_madds", "glv_products", "agg_products", "decoded_points",
-                   "point_adds", "pair_checks", "group_checks", "pair_checks_run", "sha_blocks")
+                   "point_adds", "pair_checks", "group_checks", "pair_checks_run", "sha_blocks",
+                   "straus_sums", "straus_terms")
 
     def work(self):
         """Algorithmic work units of the batch (cubbs_hip_batch_work); pair_checks_run

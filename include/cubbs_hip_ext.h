@@ -165,6 +165,8 @@ typedef struct CubbsHipBatchWork {
   uint64_t group_checks;    /* random-linear-combination group checks planned */
   uint64_t pair_checks_run; /* checks computed at the last launch, see above */
   uint64_t sha_blocks;      /* SHA-256 compressions of message hashes, domains and challenges */
+  uint64_t straus_sums;     /* multi-term variable-base jobs: one shared chain of 128 doublings each */
+  uint64_t straus_terms;    /* their terms (full scalars): window and table additions of one GLV product each */
 } CubbsHipBatchWork;
 int32_t cubbs_hip_batch_work(void *batch, CubbsHipBatchWork *out);
 
