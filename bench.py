@@ -57,17 +57,20 @@ sys.path.insert(0, os.path.join(ROOT, "cubbs-cred_amd"))
 # once at initialisation.  The library keeps one work stream per queue.  The
 # timed loop keeps its sets in flight on one stream each
 # (CUBBS_HIP_BATCH_SINGLE_STREAM), so the measurement runs in a child process
-# with BENCH_HW_QUEUES queues.  24 or 32 queues measured slower than 16 (with
-# 12 to 16 sets in flight).  The serving loop's slots take 8 streams each (2
-# slots at 16).
-BENCH_HW_QUEUES = "16"
+# with BENCH_HW_QUEUES queues, one per set.  Measured with verify sets at 20
+# steps: 20 queues and 20 sets 2.70-2.73 ms per step, 16 queues and 12 sets
+# 3.07, 18 and 18 3.15; 22 queues and 22 sets ran like 20, but 24 queues
+# (20 sets) 4.27.  Sets sharing queues measured slower (24 sets on 20 queues:
+# 2.93 ms at 60 steps against 2.49).  The serving loop's slots take 8 streams
+# each (2 slots at 16).
+BENCH_HW_QUEUES = "20"
 SERVING_HW_QUEUES = "16"
 # two-batch sets in flight in the timed loop (--separate) when the tuning
 # table has no entry: while one set's pairing launches finish, the next
 # set's decode / hashing / MSM stages already run
 INFLIGHT = 8
 # verify sets in flight (one stream each) when the tuning table has no entry
-INFLIGHT_SETS = 16
+INFLIGHT_SETS = 20
 
 N_VERIFY = 8192
 N_ISSUERS = 256
@@ -630,8 +633,8 @@ TUNE_GROUPS = (4, 8)
 TUNE_SUPER = (0, 4)
 TUNE_CHUNKS = (2, 4)
 # sets in flight per staging form: verify sets take one stream each, the
-# separate form two (GPU_MAX_HW_QUEUES=16 hardware queues)
-TUNE_INFLIGHT = {True: (8, 12, 16), False: (6, 8)}
+# separate form two (BENCH_HW_QUEUES hardware queues)
+TUNE_INFLIGHT = {True: (12, 16, 20), False: (8, 10)}
 TUNE_STEPS = 20  # the headline's timed steps: pipeline fill and drain included
 
 
