@@ -90,7 +90,9 @@ def work_fpm(w):
     return (w["comb_madds"] * f["comb_madd"] + w["glv_products"] * f["glv_product"] +
             w["agg_products"] * f["agg_product"] + w["decoded_points"] * f["decoded_point"] +
             w["point_adds"] * f["point_add"] + w["pair_checks_run"] * f["pair_check"] +
-            w["straus_sums"] * f["straus_chain"] + w["straus_terms"] * f["straus_term"])
+            w["straus_sums"] * f["straus_chain"] + w["straus_terms"] * f["straus_term"] +
+            w["straus_short_sums"] * f["straus_short_chain"] +
+            w["straus_short_terms"] * f["straus_short_term"])
 
 
 def log(msg):
@@ -745,6 +747,8 @@ def main():
     ap.add_argument("--fork", action="store_true",
                     help="staged batches fork their decode/product branch onto a second stream "
                          "(lower latency per batch, two hardware queues each)")
+    ap.add_argument("--agg-straus", type=int, default=0, metavar="N",
+                    help="signatures per product job of an aggregated Verify group's sums (1..4; 0: two products per item)")
     ap.add_argument("--inflight", type=int, default=0,
                     help="staged batch sets in flight in the timed loop (1 = launch, wait, repeat; "
                          "default: the pipeline tuning table's choice, else %d)" % INFLIGHT)
@@ -831,10 +835,11 @@ def main():
     nset = max(1, args.inflight or (tuned["sets_in_flight"] if tuned else
                                      (INFLIGHT_SETS if combined else INFLIGHT)))
     single, agg = not args.fork, not args.no_aggregate
+    straus = args.agg_straus if agg else 0
     if combined:
-        sets = [make_set(g, vreqs, pvreqs, True, aggregate=agg) for _ in range(nset)]
+        sets = [make_set(g, vreqs, pvreqs, True, aggregate=agg, straus=straus) for _ in range(nset)]
     else:
-        sets = [[g.StagedBatch(g.BATCH_VERIFY, vreqs, single_stream=single, aggregate=agg),
+        sets = [[g.StagedBatch(g.BATCH_VERIFY, vreqs, single_stream=single, aggregate=agg, straus=straus),
                  g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs, single_stream=single, aggregate=agg)]
                 for _ in range(nset)]
     if not args.no_graph:
@@ -914,6 +919,7 @@ def main():
                       "verify_items": N_VERIFY, "proof_verify_items": N_PROOFS,
                       "launch": "kernel by kernel" if args.no_graph else "HIP graph replay per staged batch",
                       "batch_sets_in_flight": nset,
+                      "agg_straus": straus,
                       "staging": ("verify set: the Verify and the ProofVerify batch staged as one batch, "
                                   "every stage launched once for both" if combined
                                   else "a Verify batch and a ProofVerify batch per set"),

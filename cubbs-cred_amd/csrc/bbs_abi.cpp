@@ -505,12 +505,13 @@ struct PrepareScope {
     uint32_t group = (flags >> 8) & 15u;
     const uint32_t chunk = (flags >> 12) & 127u;
     uint32_t super = (flags >> 20) & 15u;
+    const uint32_t straus = (flags >> 24) & 15u;
     const bool agg = (flags & CUBBS_HIP_BATCH_AGGREGATE) != 0;
     if ((flags & ~(uint32_t)(CUBBS_HIP_BATCH_SINGLE_STREAM | CUBBS_HIP_BATCH_AGGREGATE |
                              CUBBS_HIP_BATCH_AGG_GROUP(15) | CUBBS_HIP_BATCH_COMB_CHUNK(127) |
-                             CUBBS_HIP_BATCH_AGG_SUPER(15))) ||
+                             CUBBS_HIP_BATCH_AGG_SUPER(15) | CUBBS_HIP_BATCH_AGG_STRAUS(15))) ||
         (group && (group < 2 || group > 8 || !agg)) || chunk > 64 || (chunk && !whole_op) ||
-        (super && (super < 2 || super > 8 || !agg)))
+        (super && (super < 2 || super > 8 || !agg)) || (straus && (straus > VAR_TERMS_MAX || !agg)))
       return;
     if (agg) {
       // the tuned shape of the batch's size class where the flags leave it
@@ -524,12 +525,14 @@ struct PrepareScope {
     set_thread_single_stream((flags & CUBBS_HIP_BATCH_SINGLE_STREAM) != 0);
     set_thread_agg_group(agg ? group : 0u);
     set_thread_agg_super(agg ? super : 0u);
+    set_thread_agg_straus(straus);
     ok = true;
   }
   ~PrepareScope() {
     set_thread_single_stream(false);
     set_thread_agg_group(0);
     set_thread_agg_super(0);
+    set_thread_agg_straus(0);
     tune_force() = saved;
   }
 };

@@ -140,9 +140,28 @@ static void count_work(Batch& b, const Plan& p) {
   w.comb_madds = b.comb_madds;
   w.glv_products = p.var.size();
   w.agg_products = p.agg_var.size();
-  w.straus_sums = p.var_sum.size();
-  w.straus_terms = 0;
-  for (const auto& v : p.var_sum) w.straus_terms += v.count;
+  // multi-term products: the ungated jobs, then the gated ones (counted as
+  // if every group failed until results say otherwise)
+  uint64_t* u = b.ungated_straus;
+  u[0] = u[1] = u[2] = u[3] = 0;
+  for (const auto& v : p.var_sum) {
+    u[0]++;
+    u[1] += v.count;
+    if (v.flags & VAR_SUM_DUAL) {
+      u[2]++;
+      u[3] += v.count;
+    }
+  }
+  w.straus_sums = u[0];
+  w.straus_terms = u[1];
+  w.straus_short_sums = u[2];
+  w.straus_short_terms = u[3];
+  for (const auto& g : b.gated_work) {
+    w.straus_sums += g.sums;
+    w.straus_terms += g.sum_terms;
+    w.straus_short_sums += g.duals;
+    w.straus_short_terms += g.dual_terms;
+  }
   w.decoded_points = p.dec_g1.size();
   uint64_t adds = 0;
   for (const auto& g : p.groups1) adds += g.count ? g.count - 1 : 0;
@@ -205,16 +224,32 @@ void materialize(Batch& b, Plan& p) {
   std::vector<VarJob> all_var(p.var);
   all_var.insert(all_var.end(), p.agg_var.begin(), p.agg_var.end());
   size_t o_var = place(all_var.data(), all_var.size() * sizeof(VarJob));
-  // multi-term products: the longest first (a wave's lanes then run the same
-  // number of terms), each with its own workspace
-  std::stable_sort(p.var_sum.begin(), p.var_sum.end(),
-                   [](const VarSum& x, const VarSum& y) { return x.count > y.count; });
+  // multi-term products: the longest first (ordered by number of terms and
+  // dual or not, so that a wave's lanes run the same chains), each with its
+  // own workspace; then the gated jobs
+  auto sum_order = [](const VarSum& x, const VarSum& y) {
+    const uint32_t kx = 2 * x.count + (x.flags & VAR_SUM_DUAL), ky = 2 * y.count + (y.flags & VAR_SUM_DUAL);
+    return kx > ky;
+  };
+  std::stable_sort(p.var_sum.begin(), p.var_sum.end(), sum_order);
+  std::stable_sort(p.fb_sum.begin(), p.fb_sum.end(), sum_order);
   uint64_t sum_ws_words = 0;
   for (auto& v : p.var_sum) {
     v.ws = (uint32_t)sum_ws_words;
     sum_ws_words += var_sum_ws_words(v.count);
   }
+  for (auto& v : p.fb_sum) {
+    v.ws = (uint32_t)sum_ws_words;
+    sum_ws_words += var_sum_ws_words(v.count);
+  }
+  if (sum_ws_words >> 32) throw std::length_error("product workspace beyond 2^32 words");
   size_t o_vsum = place(p.var_sum.data(), p.var_sum.size() * sizeof(VarSum));
+  size_t o_fbs = place(p.fb_sum.data(), p.fb_sum.size() * sizeof(VarSum));
+  std::vector<uint32_t> fb_sum_gate(p.fb_sum.size()), fb_red_gate(p.fb_red.size());
+  for (size_t i = 0; i < p.fb_sum.size(); i++) fb_sum_gate[i] = p.fb_sum[i].gate;
+  for (size_t i = 0; i < p.fb_red.size(); i++) fb_red_gate[i] = p.fb_red[i].gate;
+  size_t o_fbsg = place(fb_sum_gate.data(), fb_sum_gate.size() * 4);
+  size_t o_fbrg = place(fb_red_gate.data(), fb_red_gate.size() * 4);
   size_t o_chk = place(p.chunks.data(), p.chunks.size() * sizeof(CombChunk));
   size_t o_tg = place(p.term_gen.data(), p.term_gen.size() * 4);
   size_t o_g1 = place(p.groups1.data(), p.groups1.size() * sizeof(ReduceGroup));
@@ -285,6 +320,10 @@ void materialize(Batch& b, Plan& p) {
   b.n_fb_chunks = (uint32_t)p.fb_chunks.size();
   b.fb_red = reinterpret_cast<GatedReduce*>(D + o_fbr);
   b.n_fb_red = (uint32_t)p.fb_red.size();
+  b.fb_sum = reinterpret_cast<VarSum*>(D + o_fbs);
+  b.n_fb_sum = (uint32_t)p.fb_sum.size();
+  b.fb_sum_gate = reinterpret_cast<uint32_t*>(D + o_fbsg);
+  b.fb_red_gate = reinterpret_cast<uint32_t*>(D + o_fbrg);
   b.n_agg_slots = p.n_agg_slots;
   b.n_spairs = p.n_super_pairs;
   b.n_gpairs = p.n_group_pairs - p.n_super_pairs;
@@ -328,13 +367,17 @@ void materialize(Batch& b, Plan& p) {
   // gated work, counted as if every group failed until results say otherwise
   for (size_t i = 0; i < p.fb_chunks.size(); i++) {
     const uint32_t m = chunk_madds(p.fb_chunks[i]);
-    b.gated_work.push_back(Batch::GatedWork{p.fb_gate[i], p.fb_chunks[i].term_count, m, 0});
+    b.gated_work.push_back(Batch::GatedWork{p.fb_gate[i], p.fb_chunks[i].term_count, m, 0, 0, 0, 0, 0});
     b.comb_terms += p.fb_chunks[i].term_count;
     b.comb_madds += m;
   }
   for (const auto& r : p.fb_red) {
     const uint32_t in = r.count + (r.extra != REDUCE_NO_EXTRA ? 1u : 0u);
-    b.gated_work.push_back(Batch::GatedWork{r.gate, 0, 0, in ? in - 1 : 0});
+    b.gated_work.push_back(Batch::GatedWork{r.gate, 0, 0, in ? in - 1 : 0, 0, 0, 0, 0});
+  }
+  for (const auto& v : p.fb_sum) {
+    const bool dual = (v.flags & VAR_SUM_DUAL) != 0;
+    b.gated_work.push_back(Batch::GatedWork{v.gate, 0, 0, 0, 1, v.count, dual ? 1u : 0u, dual ? v.count : 0u});
   }
   count_work(b, p);
   if (p.n_agg_slots) {
@@ -364,6 +407,8 @@ void materialize(Batch& b, Plan& p) {
   size_t o_pairn = take(4);
   // product workspaces: the plan's jobs, then the aggregation jobs
   size_t o_vt = take(4ull * VAR_WS_WORDS * std::max<size_t>(all_var.size(), 1));
+  size_t o_glist = take(4ull * std::max<size_t>(p.fb_sum.size() + p.fb_chunks.size() + p.fb_red.size(), 1));
+  size_t o_gcnt = take(16);
   size_t o_sws = take(4ull * std::max<uint64_t>(sum_ws_words, 1));
   const PipShape& ps = b.pip;
   const size_t pip_on = ps.terms ? 1 : 0;
@@ -395,6 +440,8 @@ void materialize(Batch& b, Plan& p) {
   b.pair_count = reinterpret_cast<uint32_t*>(W + o_pairn);
   b.vtab = reinterpret_cast<uint32_t*>(W + o_vt);
   b.sum_ws = reinterpret_cast<uint32_t*>(W + o_sws);
+  b.gate_list = reinterpret_cast<uint32_t*>(W + o_glist);
+  b.gate_count = reinterpret_cast<uint32_t*>(W + o_gcnt);
   if (pip_on) {
     auto u32 = [&](size_t o) { return reinterpret_cast<uint32_t*>(W + o); };
     b.pipb = PipBuffers{u32(o_pcnt), u32(o_poff), u32(o_pcur), u32(o_ppof), u32(o_pmap),
@@ -444,11 +491,13 @@ void materialize(Batch& b, Plan& p) {
 // launch costs about one check's latency however few checks it holds, so the
 // extra combined-check launch (and the re-check launch when a group fails)
 // outweighs the pairings saved unless several batches are in flight.
-static thread_local uint32_t tl_agg_group = 0, tl_agg_super = 0;
+static thread_local uint32_t tl_agg_group = 0, tl_agg_super = 0, tl_agg_straus = 0;
 void set_thread_agg_group(uint32_t n) { tl_agg_group = n; }
 void set_thread_agg_super(uint32_t n) { tl_agg_super = n; }
+void set_thread_agg_straus(uint32_t n) { tl_agg_straus = n; }
 uint32_t agg_group_max() { return tl_agg_group; }
 uint32_t agg_super_max() { return tl_agg_super; }
+uint32_t agg_straus() { return tl_agg_straus; }
 
 static void agg_coefficient(uint8_t out[32], uint64_t s, uint64_t t) {
   // s + t * lambda, lambda = GLV_LAMBDA (127.4 bits): below 2^192 < r
@@ -643,11 +692,15 @@ static std::string map_dst() { return std::string(kApiId) + "MAP_MSG_TO_SCALAR_A
 // B_i = P1 + Q1 dom_i + sum_j H_j m_ij is linear in its scalars, so the
 // group's sum c_i B_i is ONE fixed-base MSM over summed term scalars:
 //   P1 sum c_i + Q1 sum c_i dom_i + sum_j H_j (sum over i with L_i >= j of c_i m_ij),
-// max L_i + 2 terms for the group instead of sum (L_i + 2).  Per item there
-// remain the products c A (64-bit GLV halves) and (-c e) A.  An item's own
-// terms c, c dom, c m_j are filled as well, but its comb MSM, and the sums
-// forming its check points, run only when its group check fails (gated on
-// the group's status slot, after the group checks).  A key's last item, when
+// max L_i + 2 terms for the group instead of sum (L_i + 2).  The A sides,
+// sum c_i A_i and sum (-c_i e_i) A_i, are multi-term product jobs of up to
+// VAR_TERMS_MAX items each: one table of multiples per point serves both
+// sums, and each sum takes one chain of doublings for all its points (64
+// for the coefficients' 64-bit GLV halves, 128 for the full scalars)
+// instead of one chain per item and product.  An item's own products c A,
+// (-c e) A and terms c, c dom, c m_j (filled as well) and the sums forming
+// its check points run only when its group check fails (gated on the
+// group's status slot, after the group checks).  A key's last item, when
 // it is left alone, is checked on its own throughout.  Supergroups (runs of
 // up to super_max groups of a key) are checked before their groups.
 //
@@ -656,8 +709,9 @@ static std::string map_dst() { return std::string(kApiId) + "MAP_MSG_TO_SCALAR_A
 static void plan_verify_grouped(Plan& p, const CuBbsBbsGpuVerifyItem* items,
                                 const std::vector<size_t>& dev, const uint32_t* pk_slot,
                                 uint32_t ibase, uint32_t slot_base, uint32_t CH, uint32_t group_max,
-                                uint32_t super_max) {
+                                uint32_t super_max, uint32_t straus) {
   const uint32_t nv = (uint32_t)dev.size();
+  if (straus > VAR_TERMS_MAX) straus = VAR_TERMS_MAX;
   if (super_max < 2 || super_max > 8) super_max = 1;
   std::vector<uint32_t> Ls(nv);
   for (uint32_t k = 0; k < nv; k++) Ls[k] = (uint32_t)items[dev[k]].message_count;
@@ -743,23 +797,47 @@ static void plan_verify_grouped(Plan& p, const CuBbsBbsGpuVerifyItem* items,
       const uint32_t pk = pk_slot[order[first]];
       uint32_t sslot = 0;
       if (sup) sslot = slot_base + p.n_agg_slots++;
-      // partials: c_i A_i of every item of the chunk, then per group the
-      // (-c_i e_i) A_i of its items followed by its comb partials
-      const uint32_t fx = p.partials(total);
-      for (uint32_t m = 0; m < total; m++) {
-        const uint32_t k = order[first + m];
-        p.agg_var.push_back(VarJob{pa[k], sc[k], fx + m});
+      // the A sides: per item (straus == 0) the products c_i A_i and
+      // (-c_i e_i) A_i, which the items' own checks reuse; else a group's
+      // items in product jobs of at most `straus` points (nearly equal
+      // parts), each forming the two sums of its points, with -c_i e_i and
+      // with c_i (nj: a group's partials per side)
+      std::vector<uint32_t> nj(c1 - c0);
+      uint32_t jobs = 0;
+      for (size_t r = c0; r < c1; r++) {
+        const uint32_t cnt = runs[r].end - runs[r].begin;
+        nj[r - c0] = straus ? (cnt + straus - 1) / straus : cnt;
+        jobs += nj[r - c0];
       }
+      // partials: the c_i A_i side of every group of the chunk, then per
+      // group its (-c_i e_i) A_i side followed by its comb partials
+      const uint32_t fx = p.partials(jobs);
       const uint32_t ybegin = p.n_partials;
-      std::vector<uint32_t> vb(c1 - c0), vend(c1 - c0), gslot(c1 - c0);
+      std::vector<uint32_t> xb(c1 - c0), vb(c1 - c0), vend(c1 - c0), gslot(c1 - c0);
+      uint32_t xo = fx;
       for (size_t r = c0; r < c1; r++) {
         const Run& run = runs[r];
-        const uint32_t cnt = run.end - run.begin;
-        vb[r - c0] = p.partials(cnt);
-        for (uint32_t m = 0; m < cnt; m++) {
-          const uint32_t k = order[run.begin + m];
-          p.var.push_back(VarJob{pa[k], se[k], vb[r - c0] + m});
+        const uint32_t cnt = run.end - run.begin, nr = nj[r - c0];
+        xb[r - c0] = xo;
+        vb[r - c0] = p.partials(nr);
+        for (uint32_t q = 0; q < nr; q++) {
+          if (!straus) {
+            const uint32_t k = order[run.begin + q];
+            p.agg_var.push_back(VarJob{pa[k], sc[k], xo + q});
+            p.var.push_back(VarJob{pa[k], se[k], vb[r - c0] + q});
+            continue;
+          }
+          const uint32_t a = run.begin + cnt * q / nr, e = run.begin + cnt * (q + 1) / nr;
+          uint32_t jp[VAR_TERMS_MAX], js1[VAR_TERMS_MAX], js2[VAR_TERMS_MAX];
+          for (uint32_t m = a; m < e; m++) {
+            const uint32_t k = order[m];
+            jp[m - a] = pa[k];
+            js1[m - a] = se[k];
+            js2[m - a] = sc[k];
+          }
+          p.var_dual(jp, js1, js2, e - a, vb[r - c0] + q, xo + q);
         }
+        xo += nr;
         uint32_t lmax = 0;
         for (uint32_t m = run.begin; m < run.end; m++) lmax = std::max(lmax, Ls[order[m]]);
         const uint32_t gt = term;
@@ -784,22 +862,22 @@ static void plan_verify_grouped(Plan& p, const CuBbsBbsGpuVerifyItem* items,
       // the sums, once the chunk's partials are all allocated (a long sum's
       // tree levels take partial slots after them)
       for (size_t r = c0; r < c1; r++) {
-        const Run& run = runs[r];
         const uint32_t la = p.points(1), lb = p.points(1);
-        p.reduce_to(fx + (run.begin - first), run.end - run.begin, la);
+        p.reduce_to(xb[r - c0], nj[r - c0], la);
         p.reduce_to(vb[r - c0], vend[r - c0] - vb[r - c0], lb);
         groups.push_back(PairJob{gslot[r - c0], la, lb, pk, PAIR_NEG_G2, sup ? PAIR_GROUPED : 0u, sslot});
       }
       if (sup) {
         const uint32_t la = p.points(1), lb = p.points(1);
-        p.reduce_to(fx, total, la);
+        p.reduce_to(fx, jobs, la);
         p.reduce_to(ybegin, yend - ybegin, lb);
         supers.push_back(PairJob{sslot, la, lb, pk, PAIR_NEG_G2, 0, 0});
       }
-      // the items' own checks, for when their group check fails: one gated
-      // sum per check point, over comb chunks of at least 4 terms (on this
-      // critical path a longer chunk costs less than the serial additions
-      // of the sum it saves)
+      // the items' own checks, for when their group check fails: their
+      // products (the per-item ones above, or a gated dual product of c A
+      // and (-c e) A), and one gated sum per check point over comb chunks of
+      // at least 4 terms (on this critical path a longer chunk costs less
+      // than the serial additions of the sum it saves)
       const uint32_t fch = std::max(CH, 4u);
       for (size_t r = c0; r < c1; r++) {
         const Run& run = runs[r];
@@ -807,10 +885,16 @@ static void plan_verify_grouped(Plan& p, const CuBbsBbsGpuVerifyItem* items,
           const uint32_t k = order[m], L = Ls[k];
           const uint32_t ft = own_terms(k);
           const uint32_t g = gslot[r - c0];
+          uint32_t px = xb[r - c0] + (m - run.begin), py = vb[r - c0] + (m - run.begin);
+          if (straus) {
+            px = p.partials(1);
+            py = p.partials(1);
+            p.var_dual(&pa[k], &se[k], &sc[k], 1, py, px, g);
+          }
           const uint32_t cb = p.comb_gated(ft, L + 2, fch, g);
           const uint32_t pa2 = p.points(1), pb = p.points(1);
-          p.fb_red.push_back(GatedReduce{fx + (m - first), 1, REDUCE_NO_EXTRA, g, pa2});   // c A
-          p.fb_red.push_back(GatedReduce{cb, p.n_partials - cb, vb[r - c0] + (m - run.begin), g, pb});
+          p.fb_red.push_back(GatedReduce{px, 1, REDUCE_NO_EXTRA, g, pa2});          // c A
+          p.fb_red.push_back(GatedReduce{cb, p.n_partials - cb, py, g, pb});        // c (B - e A)
           grouped.push_back(PairJob{ibase + k, pa2, pb, pk, PAIR_NEG_G2, PAIR_GROUPED, g});
         }
       }
@@ -821,10 +905,14 @@ static void plan_verify_grouped(Plan& p, const CuBbsBbsGpuVerifyItem* items,
     const uint32_t st = own_terms(k);
     const uint32_t pair_a = p.points(1), pb = p.points(1);
     const uint32_t fx = p.partials(1);
-    p.agg_var.push_back(VarJob{pa[k], sc[k], fx});
     p.reduce_to(fx, 1, pair_a);                               // c A
-    const uint32_t first = p.n_partials;
-    p.var.push_back(VarJob{pa[k], se[k], p.partials(1)});     // (-c e) A
+    const uint32_t first = p.partials(1);
+    if (straus) {
+      p.var_dual(&pa[k], &se[k], &sc[k], 1, first, fx);       // (-c e) A, c A
+    } else {
+      p.agg_var.push_back(VarJob{pa[k], sc[k], fx});          // c A
+      p.var.push_back(VarJob{pa[k], se[k], first});           // (-c e) A
+    }
     p.comb(st, L + 2, CH);                                    // c B
     p.reduce_to(first, p.n_partials - first, pb);             // c (B - e A)
     single.push_back(PairJob{ibase + k, pair_a, pb, pk_slot[k], PAIR_NEG_G2, 0, 0});
@@ -1058,7 +1146,7 @@ static std::unique_ptr<Batch> plan_verify_set(const CuBbsBbsGpuVerifyItem* vitem
   if (!dv.empty()) {
     const uint32_t CH = pipeline_comb_chunk(0, nv);
     if (agg)
-      plan_verify_grouped(p, vitems, dv, pk_slot.data(), 0, b->nv, CH, group, super);
+      plan_verify_grouped(p, vitems, dv, pk_slot.data(), 0, b->nv, CH, group, super, agg_straus());
     else
       plan_verify_plain(p, vitems, dv, pk_slot.data(), 0, CH);
   }
@@ -1552,15 +1640,39 @@ void launch_batch(Batch& b) {
                      b.l2buf + (size_t)done * PAIR_L2_WORDS, b.status, b.gate, b.valid);
       done += lv.n;
     }
-    // the items of failing groups: their own comb terms and check points
+    // the items of failing groups: their own products, comb terms and check
+    // points, each launch over the list of its live jobs (the jobs of groups
+    // whose check held would leave most waves partly idle, not empty)
+    if (b.n_fb_sum || b.n_fb_chunks || b.n_fb_red) {
+      Timed t(b, "gate_select");
+      launch_fill_u32(s, b.gate_count, 3, 0);
+      GateLists gl{};
+      gl.gate[0] = b.fb_sum_gate;
+      gl.gate[1] = b.fb_gate;
+      gl.gate[2] = b.fb_red_gate;
+      gl.n[0] = b.n_fb_sum;
+      gl.n[1] = b.n_fb_chunks;
+      gl.n[2] = b.n_fb_red;
+      gl.list[0] = b.gate_list;
+      gl.list[1] = b.gate_list + b.n_fb_sum;
+      gl.list[2] = b.gate_list + b.n_fb_sum + b.n_fb_chunks;
+      gl.count = b.gate_count;
+      launch_gate_select(s, gl, b.valid);
+    }
+    if (b.n_fb_sum) {
+      Timed t(b, "var_gated");
+      launch_var_mul(s, nullptr, 0, b.points, b.pinf, b.scalars, b.partials, b.vtab, nullptr, 0,
+                     nullptr, b.fb_sum, b.n_fb_sum, b.sum_ws, b.valid, b.gate_list, b.gate_count);
+    }
     if (b.n_fb_chunks) {
       Timed t(b, "comb_gated");
       launch_comb_msm(s, b.fb_chunks, b.n_fb_chunks, b.term_gen, b.scalars, b.tables, b.tables16,
-                      b.n16, b.partials, b.fb_gate, b.valid);
+                      b.n16, b.partials, b.fb_gate, b.valid, b.gate_list + b.n_fb_sum, b.gate_count + 1);
     }
     if (b.n_fb_red) {
       Timed t(b, "reduce_gated");
-      launch_reduce_gated(s, b.fb_red, b.n_fb_red, b.partials, b.points, b.pinf, b.valid);
+      launch_reduce_gated(s, b.fb_red, b.n_fb_red, b.partials, b.points, b.pinf, b.valid,
+                          b.gate_list + b.n_fb_sum + b.n_fb_chunks, b.gate_count + 2);
     }
     // after group checks, only the per-item checks that still need their own
     // pairing are launched, listed contiguously
@@ -1606,15 +1718,24 @@ void fetch_results(Batch& b) {
     b.work.pair_checks_run = run;
     // the gated comb chunks and sums ran for the items of failing groups
     uint64_t terms = b.ungated_comb_terms, madds = b.ungated_comb_madds, adds = b.ungated_point_adds;
+    uint64_t st[4] = {b.ungated_straus[0], b.ungated_straus[1], b.ungated_straus[2], b.ungated_straus[3]};
     for (const auto& g : b.gated_work) {
       if (b.h_res[b.res_stride + g.gate] == 1) continue;
       terms += g.terms;
       madds += g.madds;
       adds += g.adds;
+      st[0] += g.sums;
+      st[1] += g.sum_terms;
+      st[2] += g.duals;
+      st[3] += g.dual_terms;
     }
     b.work.comb_terms = terms;
     b.work.comb_madds = madds;
     b.work.point_adds = adds;
+    b.work.straus_sums = st[0];
+    b.work.straus_terms = st[1];
+    b.work.straus_short_sums = st[2];
+    b.work.straus_short_terms = st[3];
   }
 }
 

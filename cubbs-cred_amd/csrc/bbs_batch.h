@@ -74,6 +74,10 @@ void set_thread_single_stream(bool on);
 // batches this thread prepares next (0 = per-item checks only).
 void set_thread_agg_group(uint32_t n);
 void set_thread_agg_super(uint32_t n);
+// Points per product job of an aggregated Verify group's A sides (0: the
+// products c_i A_i and (-c_i e_i) A_i per item; 1..VAR_TERMS_MAX: dual
+// multi-term jobs, and the items' own products gated after the group checks).
+void set_thread_agg_straus(uint32_t n);
 
 struct Batch {
   int kind = 0;
@@ -126,14 +130,24 @@ struct Batch {
   CombChunk* fb_chunks = nullptr;
   uint32_t* fb_gate = nullptr;
   GatedReduce* fb_red = nullptr;
-  uint32_t n_fb_chunks = 0, n_fb_red = 0;
+  VarSum* fb_sum = nullptr;
+  uint32_t n_fb_chunks = 0, n_fb_red = 0, n_fb_sum = 0;
+  // gate slots of fb_sum and fb_red jobs (fb_gate: of fb_chunks), and the
+  // lists of their live jobs that the select after the group checks writes
+  // (products, chunks, sums; counts in gate_count[0..3))
+  uint32_t* fb_sum_gate = nullptr;
+  uint32_t* fb_red_gate = nullptr;
+  uint32_t* gate_list = nullptr;
+  uint32_t* gate_count = nullptr;
   // per gated job, for the work counts of a launch: its gate slot and the
   // comb terms, mixed additions and point additions it runs
   struct GatedWork {
     uint32_t gate, terms, madds, adds;
+    uint32_t sums, sum_terms, duals, dual_terms;  // multi-term products (gated jobs)
   };
   std::vector<GatedWork> gated_work;
   uint64_t ungated_comb_terms = 0, ungated_comb_madds = 0, ungated_point_adds = 0;
+  uint64_t ungated_straus[4] = {0, 0, 0, 0};  // sums, their terms, dual sums, their terms
   struct Level {
     ReduceGroup* groups;
     uint32_t* out;

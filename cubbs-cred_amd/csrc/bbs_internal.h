@@ -127,14 +127,25 @@ struct VarJob {
 // share one chain of doublings (Straus over 2 * count half-points).  Its
 // workspace (tables of every term and the temporaries) is
 // var_sum_ws_words(count) words from word `ws` of the sums' workspace.
+// VAR_SUM_DUAL: the job forms a second sum over the same points and tables,
+// partials[partial2] = sum of scalars[scalar2[t]] * points[point[t]] (its
+// chain of doublings starts at the highest non-zero window of those scalars,
+// so short scalars, such as aggregation coefficients, take a short chain).
+// VAR_SUM_GATED: the job runs only when valid[gate] == 0 (its group's check
+// did not hold); otherwise it writes nothing.
 constexpr uint32_t VAR_TERMS_MAX = 4;
+constexpr uint32_t VAR_SUM_DUAL = 1, VAR_SUM_GATED = 2;
 struct VarSum {
   uint32_t point[VAR_TERMS_MAX];
   uint32_t scalar[VAR_TERMS_MAX];
+  uint32_t scalar2[VAR_TERMS_MAX];
   uint32_t count;
   uint32_t partial;
+  uint32_t partial2;
   uint32_t ws;
-  uint32_t pad;
+  uint32_t flags;
+  uint32_t gate;
+  uint32_t pad[2];
 };
 constexpr uint32_t var_sum_ws_words(uint32_t count) { return count * 2 * VAR_TABLE_WORDS + 7 * 42; }
 
@@ -254,16 +265,33 @@ void launch_comb_build(hipStream_t s, uint32_t first, uint32_t count, const uint
 void launch_comb16_build(hipStream_t s, uint32_t first, uint32_t count, const uint32_t* tables,
                          uint32_t* tables16, uint32_t* scratch);
 // Terms whose generator slot is below n16 use the wide tables.  With
-// gate_slot, chunk i is skipped when valid[gate_slot[i]] != 0.
+// gate_slot, chunk i is skipped when valid[gate_slot[i]] != 0.  With list,
+// only the chunks list[0 .. *count) run (n: the list's capacity).
 void launch_comb_msm(hipStream_t s, const CombChunk* chunks, uint32_t n, const uint32_t* term_gen,
                      const uint32_t* scalars, const uint32_t* tables, const uint32_t* tables16,
                      uint32_t n16, uint32_t* partials, const uint32_t* gate_slot = nullptr,
-                     const uint8_t* valid = nullptr);
+                     const uint8_t* valid = nullptr, const uint32_t* list = nullptr,
+                     const uint32_t* count = nullptr);
+// The live jobs of the gated launches that follow a level of group checks:
+// job j of kind k (0 multi-term products, 1 comb chunks, 2 gated sums) is
+// live when valid[gate[k][j]] == 0.  The select writes the live jobs' indices
+// to list[k] (in no particular order) and their number to count[k]; count[]
+// (3 words) must be zero before it runs.  The gated kernels then run over the
+// lists, so their waves hold live jobs only.
+struct GateLists {
+  const uint32_t* gate[3];
+  uint32_t n[3];
+  uint32_t* list[3];
+  uint32_t* count;
+};
+void launch_gate_select(hipStream_t s, const GateLists& g, const uint8_t* valid);
 // groups[i] sums into points[out_point[i]]
 void launch_reduce(hipStream_t s, const ReduceGroup* groups, const uint32_t* out_point, uint32_t n,
                    const uint32_t* partials, uint32_t* points, uint8_t* pinf);
+// With list: only the jobs list[0 .. *count) run (selected live already).
 void launch_reduce_gated(hipStream_t s, const GatedReduce* jobs, uint32_t n, const uint32_t* partials,
-                         uint32_t* points, uint8_t* pinf, const uint8_t* valid);
+                         uint32_t* points, uint8_t* pinf, const uint8_t* valid,
+                         const uint32_t* list = nullptr, const uint32_t* count = nullptr);
 void launch_prepare_pk(hipStream_t s, const uint32_t* slots, uint32_t n, const uint8_t* pk_bytes,
                        uint8_t* pk_status, uint32_t* pk_lines);
 void launch_prepare_neg_g2(hipStream_t s, uint32_t* lines);
@@ -278,7 +306,9 @@ void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const 
 void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_t* points,
                     const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials, uint32_t* ws,
                     const DecodeG1Job* sub = nullptr, uint32_t n_sub = 0, uint8_t* status = nullptr,
-                    const VarSum* sums = nullptr, uint32_t n_sums = 0, uint32_t* sum_ws = nullptr);
+                    const VarSum* sums = nullptr, uint32_t n_sums = 0, uint32_t* sum_ws = nullptr,
+                    const uint8_t* valid = nullptr, const uint32_t* sum_list = nullptr,
+                    const uint32_t* sum_count = nullptr);
 void launch_term_fill(hipStream_t s, const TermFill* jobs, uint32_t n, uint32_t* scalars,
                       const uint32_t* sum_pairs = nullptr);
 // l2buf: PAIR_L2_WORDS words of device scratch per job, filled by

@@ -417,11 +417,14 @@ __global__ __launch_bounds__(128) void k_comb_msm(const CombChunk* __restrict__ 
                                                  const uint32_t* __restrict__ tables16, uint32_t n16,
                                                  uint32_t* __restrict__ partials,
                                                  const uint32_t* __restrict__ gate_slot,
-                                                 const uint8_t* __restrict__ valid) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (gate_slot && valid[gate_slot[i]]) return;
-  const CombChunk c = chunks[i];
+                                                 const uint8_t* __restrict__ valid,
+                                                 const uint32_t* __restrict__ list,
+                                                 const uint32_t* __restrict__ count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (list ? *count : n)) return;
+  const uint32_t ci = list ? list[i] : i;
+  if (!list && gate_slot && valid[gate_slot[ci]]) return;
+  const CombChunk c = chunks[ci];
   G1Z acc;
   g1z_set_inf(acc);
   CombCursor cur;
@@ -474,10 +477,10 @@ __global__ __launch_bounds__(128) void k_comb_msm(const CombChunk* __restrict__ 
 void launch_comb_msm(hipStream_t s, const CombChunk* chunks, uint32_t n, const uint32_t* term_gen,
                      const uint32_t* term_scalar, const uint32_t* tables, const uint32_t* tables16,
                      uint32_t n16, uint32_t* partials, const uint32_t* gate_slot,
-                     const uint8_t* valid) {
+                     const uint8_t* valid, const uint32_t* list, const uint32_t* count) {
   if (!n) return;
   hipLaunchKernelGGL(k_comb_msm, GRID(n, 64), 0, s, chunks, n, term_gen, term_scalar, tables,
-                     tables16, n16, partials, gate_slot, valid);
+                     tables16, n16, partials, gate_slot, valid, list, count);
 }
 
 // ------------------------------------------------------------------ public keys

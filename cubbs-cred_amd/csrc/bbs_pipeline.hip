@@ -307,39 +307,54 @@ DEV void var_mul_one(G1J& r, const G1A& p, const uint32_t k[8], uint32_t* ws) {
   r = acc;
 }
 
-// [sum_t k_t P_t] on one lane for the terms of a multi-term job: the
-// tables of 1..8 P_t and 1..8 phi(P_t) of every term in ws, the signed 4-bit
-// digits of both GLV halves of every term in LDS (word w of term t at
-// dg[(10 t + w) * 64 + lane]), then one chain of doublings from the highest
-// non-zero window down, with at most 2 * count mixed additions per window.
-DEV void var_mul_sum(G1J& r, const VarSum& j, const uint32_t* __restrict__ points,
-                     const uint8_t* __restrict__ pinf, const uint32_t* __restrict__ scalars,
-                     uint32_t* ws, uint32_t* dg, uint32_t lane) {
-  uint32_t* const jw = ws + j.count * 2 * VAR_TABLE_WORDS;
+// The tables of a multi-term job in its workspace ws: 1..8 P_t and 1..8
+// phi(P_t) of every term t that is not the identity (the identity's digits
+// are all zero, so its tables are never read).  Job fields are read through
+// the pointer (a copy indexed by the loop counter would sit in scratch).
+DEV void var_sum_tables(const VarSum* __restrict__ jp, const uint32_t* __restrict__ points,
+                        const uint8_t* __restrict__ pinf, uint32_t* ws) {
+  const uint32_t count = jp->count;
+  uint32_t* const jw = ws + count * 2 * VAR_TABLE_WORDS;
+#pragma unroll 1
+  for (uint32_t t = 0; t < count; t++) {
+    const uint32_t pt = jp->point[t];
+    if (pinf[pt]) continue;
+    G1A p;
+    load_g1a(p, points + (size_t)pt * G1A_WORDS);
+    uint32_t* const t1 = ws + t * 2 * VAR_TABLE_WORDS;
+    uint32_t* const t2 = t1 + VAR_TABLE_WORDS;
+    var_table(p, t1, jw);
+    Fp beta;
+    fp_load_const(beta, bls::GLV_BETA);
+#pragma unroll 1
+    for (int e = 0; e < 8; e++) {  // phi(e P) = (beta x, y)
+      G1A q;
+      load_g1a(q, t1 + e * G1A_WORDS);
+      fp_mul(q.x, q.x, beta);
+      store_g1a(t2 + e * G1A_WORDS, q);
+    }
+  }
+}
+
+// [sum_t k_t P_t] on one lane, k_t = scalars[sc[t]], over the tables that
+// var_sum_tables left in ws: the signed 4-bit digits of both GLV halves of
+// every term in LDS (word w of term t at dg[(10 t + w) * 64 + lane]), then
+// one chain of doublings from the highest non-zero window down, with at
+// most 2 * count mixed additions per window.
+DEV void var_sum_chain(G1J& r, const VarSum* __restrict__ jp, const uint32_t* __restrict__ sc,
+                       const uint8_t* __restrict__ pinf, const uint32_t* __restrict__ scalars,
+                       const uint32_t* ws, uint32_t* dg, uint32_t lane) {
+  const uint32_t count = jp->count;
   int top = -1;
 #pragma unroll 1
-  for (uint32_t t = 0; t < j.count; t++) {
+  for (uint32_t t = 0; t < count; t++) {
     uint32_t d1[5] = {0x88888888u, 0x88888888u, 0x88888888u, 0x88888888u, 8u};
     uint32_t d2[5] = {0x88888888u, 0x88888888u, 0x88888888u, 0x88888888u, 8u};  // all digits 0
-    if (!pinf[j.point[t]]) {
-      G1A p;
-      load_g1a(p, points + (size_t)j.point[t] * G1A_WORDS);
+    if (!pinf[jp->point[t]]) {
       Fr k;
-      load_fr(k, scalars + (size_t)j.scalar[t] * FR_WORDS);
+      load_fr(k, scalars + (size_t)sc[t] * FR_WORDS);
       uint32_t k1[4], k2[4];
       glv_split(k1, k2, k.v);
-      uint32_t* const t1 = ws + t * 2 * VAR_TABLE_WORDS;
-      uint32_t* const t2 = t1 + VAR_TABLE_WORDS;
-      var_table(p, t1, jw);
-      Fp beta;
-      fp_load_const(beta, bls::GLV_BETA);
-#pragma unroll 1
-      for (int e = 0; e < 8; e++) {  // phi(e P) = (beta x, y)
-        G1A q;
-        load_g1a(q, t1 + e * G1A_WORDS);
-        fp_mul(q.x, q.x, beta);
-        store_g1a(t2 + e * G1A_WORDS, q);
-      }
       var_digits(d1, k1);
       var_digits(d2, k2);
     }
@@ -361,7 +376,7 @@ DEV void var_mul_sum(G1J& r, const VarSum& j, const uint32_t* __restrict__ point
       for (int s = 0; s < 4; s++) g1_dbl_in(acc);
     }
 #pragma unroll 1
-    for (uint32_t h = 0; h < 2 * j.count; h++) {  // (term, GLV half)
+    for (uint32_t h = 0; h < 2 * count; h++) {  // (term, GLV half)
       const uint32_t w = dg[(5 * h + (uint32_t)(i >> 3)) * 64 + lane];
       const int d = (int)((w >> ((i & 7) * 4)) & 15u) - 8;
       if (d) {
@@ -376,7 +391,9 @@ DEV void var_mul_sum(G1J& r, const VarSum& j, const uint32_t* __restrict__ point
 }
 
 // Lanes [0, ns): the multi-term jobs (ns = n_sums rounded up to whole waves,
-// so no wave mixes them with single products).  Lanes [ns, ns + n): one
+// so no wave mixes them with single products); a dual job forms its second
+// sum over the same tables, a gated job returns at once when valid[gate] is
+// set.  Lanes [ns, ns + n): one
 // product per job (workspace ws + t * VAR_WS_WORDS).  Lanes after those, up
 // to n_sub more: the prime-order subgroup check of the points decoded by jobs
 // sub[] without it, run beside the products; a failure marks the item's
@@ -390,16 +407,27 @@ __global__ __launch_bounds__(64, 2) void k_var_mul(const VarJob* __restrict__ jo
                                                const DecodeG1Job* __restrict__ sub, uint32_t n_sub,
                                                uint8_t* __restrict__ status,
                                                const VarSum* __restrict__ sums, uint32_t n_sums,
-                                               uint32_t* __restrict__ sum_ws) {
+                                               uint32_t* __restrict__ sum_ws,
+                                               const uint8_t* __restrict__ valid,
+                                               const uint32_t* __restrict__ sum_list,
+                                               const uint32_t* __restrict__ sum_count) {
   extern __shared__ uint32_t dg[];  // 10 * VAR_TERMS_MAX * 64 words when n_sums > 0
   const uint32_t ns = (n_sums + 63) & ~63u;
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g < ns) {
-    if (g >= n_sums) return;
-    const VarSum j = sums[g];
+    if (g >= (sum_list ? *sum_count : n_sums)) return;
+    const VarSum* const jp = sums + (sum_list ? sum_list[g] : g);
+    const uint32_t flags = jp->flags;
+    if (!sum_list && (flags & VAR_SUM_GATED) && valid[jp->gate]) return;
+    uint32_t* const w = sum_ws + jp->ws;
+    var_sum_tables(jp, points, pinf, w);
     G1J r;
-    var_mul_sum(r, j, points, pinf, scalars, sum_ws + j.ws, dg, threadIdx.x);
-    store_g1j(partials + (size_t)j.partial * G1J_WORDS, r);
+    var_sum_chain(r, jp, jp->scalar, pinf, scalars, w, dg, threadIdx.x);
+    store_g1j(partials + (size_t)jp->partial * G1J_WORDS, r);
+    if (flags & VAR_SUM_DUAL) {
+      var_sum_chain(r, jp, jp->scalar2, pinf, scalars, w, dg, threadIdx.x);
+      store_g1j(partials + (size_t)jp->partial2 * G1J_WORDS, r);
+    }
     return;
   }
   const uint32_t t = g - ns;
@@ -426,12 +454,13 @@ __global__ __launch_bounds__(64, 2) void k_var_mul(const VarJob* __restrict__ jo
 void launch_var_mul(hipStream_t s, const VarJob* jobs, uint32_t n, const uint32_t* points,
                     const uint8_t* pinf, const uint32_t* scalars, uint32_t* partials, uint32_t* ws,
                     const DecodeG1Job* sub, uint32_t n_sub, uint8_t* status, const VarSum* sums,
-                    uint32_t n_sums, uint32_t* sum_ws) {
+                    uint32_t n_sums, uint32_t* sum_ws, const uint8_t* valid, const uint32_t* sum_list,
+                    const uint32_t* sum_count) {
   if (!n && !n_sub && !n_sums) return;
   const uint32_t ns = (n_sums + 63) & ~63u;
   const size_t lds = n_sums ? 4ull * 10 * VAR_TERMS_MAX * 64 : 0;
   hipLaunchKernelGGL(k_var_mul, GRID(ns + n + n_sub, 64), lds, s, jobs, n, points, pinf, scalars,
-                     partials, ws, sub, n_sub, status, sums, n_sums, sum_ws);
+                     partials, ws, sub, n_sub, status, sums, n_sums, sum_ws, valid, sum_list, sum_count);
 }
 
 // ------------------------------------------------------------------ reduction
@@ -480,11 +509,13 @@ __global__ __launch_bounds__(64) void k_reduce_gated(const GatedReduce* __restri
                                                     const uint32_t* __restrict__ partials,
                                                     uint32_t* __restrict__ points,
                                                     uint8_t* __restrict__ pinf,
-                                                    const uint8_t* __restrict__ valid) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const GatedReduce g = jobs[i];
-  if (valid[g.gate]) return;
+                                                    const uint8_t* __restrict__ valid,
+                                                    const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (list ? *count : n)) return;
+  const GatedReduce g = jobs[list ? list[i] : i];
+  if (!list && valid[g.gate]) return;
   G1J acc;
   pt_set_inf(acc);
   if (g.extra != REDUCE_NO_EXTRA) load_g1j(acc, partials + (size_t)g.extra * G1J_WORDS);
@@ -496,9 +527,42 @@ __global__ __launch_bounds__(64) void k_reduce_gated(const GatedReduce* __restri
   store_affine_sum(acc, g.out, points, pinf);
 }
 void launch_reduce_gated(hipStream_t s, const GatedReduce* jobs, uint32_t n, const uint32_t* partials,
-                         uint32_t* points, uint8_t* pinf, const uint8_t* valid) {
+                         uint32_t* points, uint8_t* pinf, const uint8_t* valid, const uint32_t* list,
+                         const uint32_t* count) {
   if (!n) return;
-  hipLaunchKernelGGL(k_reduce_gated, GRID(n, 64), 0, s, jobs, n, partials, points, pinf, valid);
+  hipLaunchKernelGGL(k_reduce_gated, GRID(n, 64), 0, s, jobs, n, partials, points, pinf, valid, list,
+                     count);
+}
+
+// Lanes over the concatenated jobs of the three kinds; one atomic per wave
+// and kind reserves the places of its live jobs.
+__global__ __launch_bounds__(64) void k_gate_select(GateLists g, const uint8_t* __restrict__ valid) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t k = 3, j = i;
+  if (j < g.n[0]) {
+    k = 0;
+  } else if ((j -= g.n[0]) < g.n[1]) {
+    k = 1;
+  } else if ((j -= g.n[1]) < g.n[2]) {
+    k = 2;
+  }
+  const bool need = k < 3 && valid[g.gate[k][j]] == 0;
+  const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+  for (uint32_t q = 0; q < 3; q++) {
+    const uint64_t m = __ballot(need && k == q);
+    if (!m) continue;
+    const uint32_t first = (uint32_t)__ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(g.count + q, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, (int)first);
+    if (need && k == q) g.list[q][base + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = j;
+  }
+}
+void launch_gate_select(hipStream_t s, const GateLists& g, const uint8_t* valid) {
+  const uint32_t n = g.n[0] + g.n[1] + g.n[2];
+  if (!n) return;
+  hipLaunchKernelGGL(k_gate_select, GRID(n, 64), 0, s, g, valid);
 }
 
 // One level of a partial-sum tree: each lane adds a short run of Jacobian

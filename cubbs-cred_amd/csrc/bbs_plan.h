@@ -55,6 +55,7 @@ struct Plan {
   std::vector<CombChunk> fb_chunks;
   std::vector<uint32_t> fb_gate;
   std::vector<GatedReduce> fb_red;
+  std::vector<VarSum> fb_sum;      // gated multi-term products (their own launch)
   std::vector<PipTerm> pip_terms;  // terms of the Pippenger MSMs
   std::vector<uint32_t> pip_out;   // per Pippenger MSM: its output partial slot
   std::vector<uint8_t> blob;       // signatures, proofs, headers, messages, keys, seeds
@@ -193,6 +194,30 @@ struct Plan {
     v.partial = partial;
     var_sum.push_back(v);
   }
+  // One dual job over the points p[0..count): partials[partial] = sum of
+  // scalars[s1[t]] * points[p[t]] and partials[partial2] = the same with
+  // s2.  gate != PRIM_NO_SLOT: a gated job (fb_sum), run only when its
+  // group's check did not hold.
+  void var_dual(const uint32_t* p, const uint32_t* s1, const uint32_t* s2, uint32_t count,
+                uint32_t partial, uint32_t partial2, uint32_t gate = PRIM_NO_SLOT) {
+    VarSum v{};
+    v.count = count;
+    for (uint32_t t = 0; t < count; t++) {
+      v.point[t] = p[t];
+      v.scalar[t] = s1[t];
+      v.scalar2[t] = s2[t];
+    }
+    v.partial = partial;
+    v.partial2 = partial2;
+    v.flags = VAR_SUM_DUAL;
+    if (gate != PRIM_NO_SLOT) {
+      v.flags |= VAR_SUM_GATED;
+      v.gate = gate;
+      fb_sum.push_back(v);
+    } else {
+      var_sum.push_back(v);
+    }
+  }
   // A summed term fill: scalars[dst] = sum of scalars[x] * scalars[y] over
   // the (x, y) pairs given.
   void term_sum(uint32_t dst, const std::vector<std::pair<uint32_t, uint32_t>>& xy) {
@@ -218,6 +243,7 @@ void materialize(Batch& b, Plan& p);
 void aggregate_pairs(Plan& p, uint32_t nv, uint32_t group_max, uint32_t super_max = 0);
 uint32_t agg_group_max();
 uint32_t agg_super_max();
+uint32_t agg_straus();
 // Lock the context, make generators [0, max_gen_slots) and the given public
 // keys resident, snapshot the resident pointers into the batch.
 void attach(Batch& b, uint32_t max_gen_slots, const std::vector<const uint8_t*>& pks,

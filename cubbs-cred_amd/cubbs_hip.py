@@ -575,12 +575,15 @@ def mad_peak_bench(lanes, iters):
 class StagedBatch:
     """A batch whose inputs stay resident on the device (prepare once, launch many)."""
 
-    def __init__(self, kind, reqs, single_stream=False, aggregate=False, comb_chunk=0, supergroup=0):
+    def __init__(self, kind, reqs, single_stream=False, aggregate=False, comb_chunk=0, supergroup=0,
+                 straus=0):
         """aggregate: False, True (the tuned group size, else 8) or a group
         size 2..8; supergroup: groups per supergroup check (2..8, with
         aggregate; 0 = the tuned choice when the group size is left open,
         else none); comb_chunk: fixed-base comb terms per lane (Verify /
-        ProofVerify; 0 = the tuned choice, else 4).  kind BATCH_VERIFY_SET:
+        ProofVerify; 0 = the tuned choice, else 4); straus: signatures per
+        product job of an aggregated Verify group's sums (1..4, with
+        aggregate; 0 = two products per item).  kind BATCH_VERIFY_SET:
         reqs = (verify requests, proof-verify requests), staged as one batch
         whose results are the Verify results followed by the others."""
         self.keep = _Keep()
@@ -588,7 +591,8 @@ class StagedBatch:
                  (BATCH_AGGREGATE if aggregate else 0) |
                  (0 if aggregate is True or not aggregate else (int(aggregate) & 15) << 8) |
                  ((int(comb_chunk) & 127) << 12) |
-                 ((int(supergroup) & 15) << 20))
+                 ((int(supergroup) & 15) << 20) |
+                 ((int(straus) & 15) << 24))
         st = ctypes.c_int32(0)
         self.kind = kind
         if kind == BATCH_VERIFY_SET:
@@ -680,7 +684,7 @@ class StagedBatch:
 
     WORK_FIELDS = ("items", "comb_terms", "comb_madds", "glv_products", "agg_products", "decoded_points",
                    "point_adds", "pair_checks", "group_checks", "pair_checks_run", "sha_blocks",
-                   "straus_sums", "straus_terms")
+                   "straus_sums", "straus_terms", "straus_short_sums", "straus_short_terms")
 
     def work(self):
         """Algorithmic work units of the batch (cubbs_hip_batch_work); pair_checks_run

@@ -102,11 +102,19 @@ void *cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void *items,
  * CUBBS_HIP_BATCH_COMB_CHUNK(n) (1 <= n <= 64, Verify / ProofVerify only)
  * sets the fixed-base comb terms per lane of this batch (otherwise the
  * table's choice for its size class, else 4); the pipeline tuner stages its
- * candidates this way. */
+ * candidates this way.
+ * CUBBS_HIP_BATCH_AGG_STRAUS(n) (1 <= n <= 4, with CUBBS_HIP_BATCH_AGGREGATE,
+ * Verify items): a group's sums sum c_i A_i and sum (-c_i e_i) A_i are formed
+ * by product jobs over up to n of its signatures' A points each (one table
+ * of multiples per point for both sums, one chain of doublings per sum for
+ * all n points) instead of two products per item; the items' own products
+ * then run only for groups whose check fails.  Fewer field operations per
+ * item, a longer product launch.  Without it: two products per item. */
 #define CUBBS_HIP_BATCH_AGGREGATE 2u
 #define CUBBS_HIP_BATCH_AGG_GROUP(n) (((uint32_t)(n) & 15u) << 8)
 #define CUBBS_HIP_BATCH_COMB_CHUNK(n) (((uint32_t)(n) & 127u) << 12)
 #define CUBBS_HIP_BATCH_AGG_SUPER(m) (((uint32_t)(m) & 15u) << 20)
+#define CUBBS_HIP_BATCH_AGG_STRAUS(n) (((uint32_t)(n) & 15u) << 24)
 void *cubbs_hip_batch_prepare_flags(uint32_t abi_version, int kind, const void *items,
                                     size_t item_count, uint32_t flags, int32_t *status_out);
 /* A Verify batch and a ProofVerify batch staged as ONE batch: every stage
@@ -167,6 +175,9 @@ typedef struct CubbsHipBatchWork {
   uint64_t sha_blocks;      /* SHA-256 compressions of message hashes, domains and challenges */
   uint64_t straus_sums;     /* multi-term variable-base jobs: one shared chain of 128 doublings each */
   uint64_t straus_terms;    /* their terms (full scalars): window and table additions of one GLV product each */
+  uint64_t straus_short_sums;  /* second sums of dual jobs over the same points and tables, with
+                                  aggregation coefficients (64-bit GLV halves): a chain of 64 doublings each */
+  uint64_t straus_short_terms; /* their terms: the window additions of a 64-bit-half product each */
 } CubbsHipBatchWork;
 int32_t cubbs_hip_batch_work(void *batch, CubbsHipBatchWork *out);
 

@@ -18,9 +18,10 @@ pytestmark = pytest.mark.gpu
 
 def _run(g, kind, reqs, group):
     """(status, per-item results) of one staged batch; group 0 = per-item
-    checks, (n, m) = groups of n in supergroups of m groups."""
-    group, sup = group if isinstance(group, tuple) else (group, 0)
-    b = g.StagedBatch(kind, reqs, aggregate=group if group else False, supergroup=sup)
+    checks, (n, m) = groups of n in supergroups of m groups, (n, m, s) = the
+    same with Verify groups' sums formed by product jobs of s signatures."""
+    group, sup, straus = (tuple(group) + (0, 0))[:3] if isinstance(group, tuple) else (group, 0, 0)
+    b = g.StagedBatch(kind, reqs, aggregate=group if group else False, supergroup=sup, straus=straus)
     try:
         b.launch()
         b.sync()
@@ -69,12 +70,15 @@ def test_verify_aggregation_matches_individual_checks(gpu, oracle):
             sig = b"\xc0" + bytes(47) + sig[48:]               # identity A
         reqs.append((pk, sig, h, m))
     expect = [oracle.verify(*r) for r in reqs]
-    runs = {n: _run(g, g.BATCH_VERIFY, reqs, n) for n in (0, 2, 8, (8, 4), (2, 3))}
+    # (8, 0, s): groups of 8 (9 for the large key's last ones) whose sums
+    # are product jobs of s signatures, also with supergroups
+    shapes = (2, 8, (8, 4), (2, 3), (8, 0, 1), (8, 0, 2), (8, 0, 3), (8, 0, 4), (8, 4, 4), (2, 3, 2))
+    runs = {n: _run(g, g.BATCH_VERIFY, reqs, n) for n in (0,) + shapes}
     base = runs[0]
     assert base[0] == 0
     assert [v for _, v in base[1]] == [bool(e) for e in expect]
     assert base[1][22][0] == 1
-    for n in (2, 8, (8, 4), (2, 3)):
+    for n in shapes:
         assert runs[n] == base, n
 
 
@@ -140,11 +144,14 @@ def test_aggregated_staged_batch_graph_replays(gpu, oracle):
         b.free()
 
 
-def test_supergroup_levels_gate_groups_and_items(gpu, oracle):
+@pytest.mark.parametrize("straus", [0, 3, 4])
+def test_supergroup_levels_gate_groups_and_items(gpu, oracle, straus):
     """Supergroups of up to 4 groups of 8 per key, replayed from a graph:
     the per-item results equal the per-item batch's, a supergroup that holds
     accepts its groups and items without their own checks, and a failing one
-    has its groups checked, and the items of failing groups."""
+    has its groups checked, and the items of failing groups.  straus > 0: the
+    groups' sums are product jobs of that many signatures, the items' own
+    products run only for the failing group."""
     g = gpu
     rng = random.Random(6)
     kps = _issuers(g, rng, 3)
@@ -160,7 +167,7 @@ def test_supergroup_levels_gate_groups_and_items(gpu, oracle):
             enumerate(zip(sreqs, sigs))]
     expect = [(0, i not in bad) for i in range(len(reqs))]
     assert _run(g, g.BATCH_VERIFY, reqs, 0) == (0, expect)
-    b = g.StagedBatch(g.BATCH_VERIFY, reqs, aggregate=8, supergroup=4)
+    b = g.StagedBatch(g.BATCH_VERIFY, reqs, aggregate=8, supergroup=4, straus=straus)
     try:
         for graph in (False, True):
             b.use_graph(graph)
@@ -177,6 +184,15 @@ def test_supergroup_levels_gate_groups_and_items(gpu, oracle):
                 # run: 2 supergroup checks, key 0's 3 groups and key 2's
                 # group, the 8 items of key 0's failing group
                 assert w["pair_checks_run"] == 2 + 4 + 8
+                if straus:
+                    # groups of 8, 8, 4, 8, 8, 4 and 9 items in jobs of at
+                    # most `straus` signatures, and the failing group's 8
+                    # items' own (gated) dual products
+                    jobs = sum(-(-n // straus) for n in (8, 8, 4, 8, 8, 4, 9))
+                    assert w["straus_short_sums"] == jobs + 8 and w["straus_short_terms"] == 49 + 8
+                    assert w["glv_products"] == w["agg_products"] == 0
+                else:
+                    assert w["straus_short_sums"] == 0 and w["glv_products"] == w["agg_products"] == 49
     finally:
         b.free()
 
@@ -247,7 +263,10 @@ def test_aggregate_prepare_flag(gpu, oracle):
     finally:
         plain.free()
         agg.free()
-    for flags in (4, g.BATCH_AGGREGATE | (1 << 8), g.BATCH_AGGREGATE | (9 << 8), 4 << 8):
+    # an unknown bit, group sizes 1 and 9, a group size without the flag,
+    # product jobs of 5 signatures, product jobs without the flag
+    for flags in (4, g.BATCH_AGGREGATE | (1 << 8), g.BATCH_AGGREGATE | (9 << 8), 4 << 8,
+                  g.BATCH_AGGREGATE | (5 << 24), 2 << 24):
         st = ctypes.c_int32(0)
         h = g.lib().cubbs_hip_batch_prepare_flags(g.ABI_VERSION, g.BATCH_VERIFY, None, 0, flags,
                                                   ctypes.byref(st))
@@ -280,7 +299,7 @@ def test_aggregated_groups_with_points_outside_g1_and_noncanonical_scalars(gpu, 
     assert [c for c, _ in base[1]] == [1 if i in (2, 9, 12) else 0 for i in range(32)]
     # the oracle's None is the reference's Err: code 1
     assert base[1] == [(0, e) if e is not None else (1, False) for e in (oracle.verify(*r) for r in vreqs)]
-    for n in (2, 8):
+    for n in (2, 8, (8, 0, 4), (8, 0, 3), (2, 0, 2)):
         assert _run(g, g.BATCH_VERIFY, vreqs, n) == base, n
     greqs = [(pk, sig, h, rng.randbytes(10), m, [1, 2]) for (sk, pk, h, m), (_, sig) in zip(sreqs, sigs)]
     rc, proofs = g.proof_gen_batch(greqs, seeds=[rng.randbytes(32) for _ in greqs])

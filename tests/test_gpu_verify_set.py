@@ -87,12 +87,17 @@ def test_verify_set_equals_separate_batches(gpu, mixed, agg):
             assert both[nv + i] == (0, bool(w)), i
 
 
-def test_verify_set_graph_replays_and_empty_parts(gpu, mixed):
+@pytest.mark.parametrize("straus", [0, 4])
+def test_verify_set_graph_replays_and_empty_parts(gpu, mixed, straus):
+    """Graph replays of a verify set equal the separate batches; with
+    straus = 4 the Verify groups' sums are product jobs of 4 signatures (the
+    separate Verify batch as well)."""
     g = gpu
     vreqs, preqs, _, _ = mixed
-    sep_v = _results(g, g.BATCH_VERIFY, vreqs, aggregate=8)
+    sep_v = _results(g, g.BATCH_VERIFY, vreqs, aggregate=8, straus=straus)
     sep_p = _results(g, g.BATCH_PROOF_VERIFY, preqs, aggregate=8)
-    b = g.StagedBatch(g.BATCH_VERIFY_SET, (vreqs, preqs), single_stream=True, aggregate=8)
+    assert sep_v == _results(g, g.BATCH_VERIFY, vreqs)
+    b = g.StagedBatch(g.BATCH_VERIFY_SET, (vreqs, preqs), single_stream=True, aggregate=8, straus=straus)
     try:
         b.use_graph(True)
         for _ in range(3):
@@ -103,5 +108,5 @@ def test_verify_set_graph_replays_and_empty_parts(gpu, mixed):
         assert groups > 0 and 0 < passed < groups
     finally:
         b.free()
-    assert _results(g, g.BATCH_VERIFY_SET, (vreqs, []), aggregate=8) == sep_v
+    assert _results(g, g.BATCH_VERIFY_SET, (vreqs, []), aggregate=8, straus=straus) == sep_v
     assert _results(g, g.BATCH_VERIFY_SET, ([], preqs), aggregate=8) == sep_p
