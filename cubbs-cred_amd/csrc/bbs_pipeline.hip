@@ -479,32 +479,68 @@ DEV void store_affine_sum(const G1J& acc, uint32_t o, uint32_t* points, uint8_t*
   store_g1a(points + (size_t)o * G1A_WORDS, a);
 }
 
+// The sum of partials [begin, begin + count) on the RL consecutive lanes
+// q = 0 .. RL - 1 (q = threadIdx.x % RL) of one 64-lane block's sum: lane q
+// adds every RL-th partial from begin + q, then log2(RL) halving steps
+// through LDS (xs: 64 points) leave the sum in lane q = 0.  Every lane of the
+// block calls it (it synchronises the block); lanes with live == false add
+// nothing.  A sum of count partials takes ceil(count / RL) + log2(RL)
+// additions one after the other instead of count.
+template <uint32_t RL>
+DEV void coop_sum(G1J& acc, bool live, uint32_t begin, uint32_t count,
+                  const uint32_t* __restrict__ partials, uint32_t* xs) {
+  const uint32_t q = threadIdx.x % RL;
+  pt_set_inf(acc);
+  if (live) {
+    for (uint32_t k = q; k < count; k += RL) {
+      G1J p;
+      load_g1j(p, partials + (size_t)(begin + k) * G1J_WORDS);
+      g1_add_in(acc, p);
+    }
+  }
+#pragma unroll
+  for (uint32_t h = RL / 2; h >= 1; h /= 2) {
+    store_g1j(xs + threadIdx.x * G1J_WORDS, acc);
+    __syncthreads();
+    if (live && q < h) {
+      G1J p;
+      load_g1j(p, xs + (threadIdx.x + h) * G1J_WORDS);  // lane q + h of the same sum
+      g1_add_in(acc, p);
+    }
+    __syncthreads();
+  }
+}
+
+// REDUCE_LANES lanes per group (sums of at most Plan::FINAL_MAX partials)
+constexpr uint32_t REDUCE_LANES = 4;
 __global__ __launch_bounds__(64) void k_reduce(const ReduceGroup* __restrict__ groups,
                                               const uint32_t* __restrict__ out_point, uint32_t n,
                                               const uint32_t* __restrict__ partials,
                                               uint32_t* __restrict__ points,
                                               uint8_t* __restrict__ pinf) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  ReduceGroup g = groups[i];
+  __shared__ uint32_t xs[64 * G1J_WORDS];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = t / REDUCE_LANES;
+  const bool live = i < n;
+  ReduceGroup g{0, 0};
+  if (live) g = groups[i];
   G1J acc;
-  pt_set_inf(acc);
-  for (uint32_t k = 0; k < g.count; k++) {
-    G1J p;
-    load_g1j(p, partials + (size_t)(g.begin + k) * G1J_WORDS);
-    g1_add_in(acc, p);
-  }
-  store_affine_sum(acc, out_point[i], points, pinf);
+  coop_sum<REDUCE_LANES>(acc, live, g.begin, g.count, partials, xs);
+  if (live && t % REDUCE_LANES == 0) store_affine_sum(acc, out_point[i], points, pinf);
 }
 void launch_reduce(hipStream_t s, const ReduceGroup* groups, const uint32_t* out_point, uint32_t n,
                    const uint32_t* partials, uint32_t* points, uint8_t* pinf) {
   if (!n) return;
-  hipLaunchKernelGGL(k_reduce, GRID(n, 64), 0, s, groups, out_point, n, partials, points, pinf);
+  hipLaunchKernelGGL(k_reduce, GRID(n * REDUCE_LANES, 64), 0, s, groups, out_point, n, partials, points,
+                     pinf);
 }
 
 // The sums of the per-item checks of aggregated groups: only for the items of
 // groups whose combined check failed (valid[gate] == 0, written by the group
-// pairing launch).
+// pairing launch), or the jobs list[0 .. *count) selected as such; one sum on
+// GATED_LANES lanes (up to (L + 2) / 4 comb partials for an item of L
+// messages).
+constexpr uint32_t GATED_LANES = 8;
 __global__ __launch_bounds__(64) void k_reduce_gated(const GatedReduce* __restrict__ jobs, uint32_t n,
                                                     const uint32_t* __restrict__ partials,
                                                     uint32_t* __restrict__ points,
@@ -512,16 +548,21 @@ __global__ __launch_bounds__(64) void k_reduce_gated(const GatedReduce* __restri
                                                     const uint8_t* __restrict__ valid,
                                                     const uint32_t* __restrict__ list,
                                                     const uint32_t* __restrict__ count) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (list ? *count : n)) return;
-  const GatedReduce g = jobs[list ? list[i] : i];
-  if (!list && valid[g.gate]) return;
+  __shared__ uint32_t xs[64 * G1J_WORDS];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = t / GATED_LANES;
+  bool live = i < (list ? *count : n);
+  GatedReduce g{0, 0, REDUCE_NO_EXTRA, 0, 0};
+  if (live) {
+    g = jobs[list ? list[i] : i];
+    if (!list && valid[g.gate]) live = false;  // the same for all lanes of the sum
+  }
   G1J acc;
-  pt_set_inf(acc);
-  if (g.extra != REDUCE_NO_EXTRA) load_g1j(acc, partials + (size_t)g.extra * G1J_WORDS);
-  for (uint32_t k = 0; k < g.count; k++) {
+  coop_sum<GATED_LANES>(acc, live, g.begin, g.count, partials, xs);
+  if (!live || t % GATED_LANES != 0) return;
+  if (g.extra != REDUCE_NO_EXTRA) {
     G1J p;
-    load_g1j(p, partials + (size_t)(g.begin + k) * G1J_WORDS);
+    load_g1j(p, partials + (size_t)g.extra * G1J_WORDS);
     g1_add_in(acc, p);
   }
   store_affine_sum(acc, g.out, points, pinf);
@@ -530,8 +571,8 @@ void launch_reduce_gated(hipStream_t s, const GatedReduce* jobs, uint32_t n, con
                          uint32_t* points, uint8_t* pinf, const uint8_t* valid, const uint32_t* list,
                          const uint32_t* count) {
   if (!n) return;
-  hipLaunchKernelGGL(k_reduce_gated, GRID(n, 64), 0, s, jobs, n, partials, points, pinf, valid, list,
-                     count);
+  hipLaunchKernelGGL(k_reduce_gated, GRID(n * GATED_LANES, 64), 0, s, jobs, n, partials, points, pinf,
+                     valid, list, count);
 }
 
 // Lanes over the concatenated jobs of the three kinds; one atomic per wave
