@@ -747,6 +747,8 @@ def main():
     ap.add_argument("--fork", action="store_true",
                     help="staged batches fork their decode/product branch onto a second stream "
                          "(lower latency per batch, two hardware queues each)")
+    ap.add_argument("--comb-chunk", type=int, default=0, metavar="N",
+                    help="fixed-base comb terms per lane of the timed sets (0: the tuning table's)")
     ap.add_argument("--agg-straus", type=int, default=0, metavar="N",
                     help="signatures per product job of an aggregated Verify group's sums (1..4; 0: two products per item)")
     ap.add_argument("--inflight", type=int, default=0,
@@ -837,7 +839,8 @@ def main():
     single, agg = not args.fork, not args.no_aggregate
     straus = args.agg_straus if agg else 0
     if combined:
-        sets = [make_set(g, vreqs, pvreqs, True, aggregate=agg, straus=straus) for _ in range(nset)]
+        sets = [make_set(g, vreqs, pvreqs, True, aggregate=agg, straus=straus, comb_chunk=args.comb_chunk)
+                for _ in range(nset)]
     else:
         sets = [[g.StagedBatch(g.BATCH_VERIFY, vreqs, single_stream=single, aggregate=agg, straus=straus),
                  g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs, single_stream=single, aggregate=agg)]
@@ -920,6 +923,7 @@ def main():
                       "launch": "kernel by kernel" if args.no_graph else "HIP graph replay per staged batch",
                       "batch_sets_in_flight": nset,
                       "agg_straus": straus,
+                      "comb_chunk": args.comb_chunk or (tuned or {}).get("comb_chunk", 4),
                       "staging": ("verify set: the Verify and the ProofVerify batch staged as one batch, "
                                   "every stage launched once for both" if combined
                                   else "a Verify batch and a ProofVerify batch per set"),
