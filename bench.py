@@ -12,7 +12,7 @@ inputs resident in HBM):
 All inputs (keys, signatures, proofs) are produced by the library itself
 (keygen, Sign and ProofGen run on the device) from fixed seeds.  The timed
 region keeps several staged sets in flight (the pipeline tuning table's
-count, 12).  A set is a verify set: both batches staged as one device batch,
+count, 20, one hardware queue each).  A set is a verify set: both batches staged as one device batch,
 so every stage launches once for the items of both, on one stream, with the
 pairing checks aggregated per key in groups of 8 (random linear
 combinations; the Verify groups' B sides as one comb MSM per group over

@@ -511,36 +511,37 @@ DEV void coop_sum(G1J& acc, bool live, uint32_t begin, uint32_t count,
   }
 }
 
-// REDUCE_LANES lanes per group (sums of at most Plan::FINAL_MAX partials)
-constexpr uint32_t REDUCE_LANES = 4;
+// One lane per group (at most Plan::FINAL_MAX partials): the launches are
+// wide, and the affine conversion at the end costs a wave per 64 sums.
 __global__ __launch_bounds__(64) void k_reduce(const ReduceGroup* __restrict__ groups,
                                               const uint32_t* __restrict__ out_point, uint32_t n,
                                               const uint32_t* __restrict__ partials,
                                               uint32_t* __restrict__ points,
                                               uint8_t* __restrict__ pinf) {
-  __shared__ uint32_t xs[64 * G1J_WORDS];
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t i = t / REDUCE_LANES;
-  const bool live = i < n;
-  ReduceGroup g{0, 0};
-  if (live) g = groups[i];
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ReduceGroup g = groups[i];
   G1J acc;
-  coop_sum<REDUCE_LANES>(acc, live, g.begin, g.count, partials, xs);
-  if (live && t % REDUCE_LANES == 0) store_affine_sum(acc, out_point[i], points, pinf);
+  pt_set_inf(acc);
+  for (uint32_t k = 0; k < g.count; k++) {
+    G1J p;
+    load_g1j(p, partials + (size_t)(g.begin + k) * G1J_WORDS);
+    g1_add_in(acc, p);
+  }
+  store_affine_sum(acc, out_point[i], points, pinf);
 }
 void launch_reduce(hipStream_t s, const ReduceGroup* groups, const uint32_t* out_point, uint32_t n,
                    const uint32_t* partials, uint32_t* points, uint8_t* pinf) {
   if (!n) return;
-  hipLaunchKernelGGL(k_reduce, GRID(n * REDUCE_LANES, 64), 0, s, groups, out_point, n, partials, points,
-                     pinf);
+  hipLaunchKernelGGL(k_reduce, GRID(n, 64), 0, s, groups, out_point, n, partials, points, pinf);
 }
 
 // The sums of the per-item checks of aggregated groups: only for the items of
 // groups whose combined check failed (valid[gate] == 0, written by the group
 // pairing launch), or the jobs list[0 .. *count) selected as such; one sum on
 // GATED_LANES lanes (up to (L + 2) / 4 comb partials for an item of L
-// messages).
-constexpr uint32_t GATED_LANES = 8;
+// messages: a long serial chain on one lane, in a launch of few sums).
+constexpr uint32_t GATED_LANES = 4;
 __global__ __launch_bounds__(64) void k_reduce_gated(const GatedReduce* __restrict__ jobs, uint32_t n,
                                                     const uint32_t* __restrict__ partials,
                                                     uint32_t* __restrict__ points,
