@@ -112,6 +112,7 @@ __global__ __launch_bounds__(256) void k_hash_stage(const DomainJob* __restrict_
                                                    const uint8_t* __restrict__ pk_bytes,
                                                    const uint8_t* __restrict__ gen_comp,
                                                    uint32_t* __restrict__ out) {
+  wave_priority<PRIO_EARLY>();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_dom) {
     domain_job(dom[i], blob, pk_bytes, gen_comp, out);
@@ -420,6 +421,7 @@ __global__ __launch_bounds__(128) void k_comb_msm(const CombChunk* __restrict__ 
                                                  const uint8_t* __restrict__ valid,
                                                  const uint32_t* __restrict__ list,
                                                  const uint32_t* __restrict__ count) {
+  wave_priority<PRIO_MID>();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (list ? *count : n)) return;
   const uint32_t ci = list ? list[i] : i;

@@ -120,11 +120,12 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v) {
 }
 
 // ------------------------------------------------------------------ histogram
-// The Pippenger stages raise their waves' issue priority: the prime-order
-// subgroup checks of the MSM's points run beside them (a launch of long,
-// VALU-bound lanes on another stream), and at equal priority they took the
-// issue slots the bucket accumulation needs (0.38 -> 1.3 ms).
-DEV void pip_priority() { __builtin_amdgcn_s_setprio(3); }
+// The Pippenger stages raise their waves' issue priority above the product
+// kernel's: the prime-order subgroup checks of the MSM's points run beside
+// them (a launch of long, VALU-bound lanes on another stream), and at equal
+// priority they took the issue slots the bucket accumulation needs
+// (0.38 -> 1.3 ms).
+DEV void pip_priority() { wave_priority<PRIO_PIP>(); }
 
 __global__ __launch_bounds__(256) void k_pip_count(PipShape sh, const PipTerm* __restrict__ terms,
                                                   const uint8_t* __restrict__ pinf,

@@ -64,6 +64,7 @@ __global__ __launch_bounds__(64, 2) void k_decode_g1(const DecodeG1Job* __restri
                                                  uint8_t* __restrict__ pinf,
                                                  uint8_t* __restrict__ status,
                                                  uint32_t check_subgroup) {
+  wave_priority<PRIO_EARLY>();
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   DecodeG1Job j = jobs[i];
@@ -97,6 +98,7 @@ __global__ __launch_bounds__(256) void k_decode_fr(const DecodeFrJob* __restrict
                                                   const uint8_t* __restrict__ blob,
                                                   uint32_t* __restrict__ scalars,
                                                   uint8_t* __restrict__ status) {
+  wave_priority<PRIO_EARLY>();
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   DecodeFrJob j = jobs[i];
@@ -130,6 +132,7 @@ void launch_decode_fr(hipStream_t s, const DecodeFrJob* jobs, uint32_t n, const 
 __global__ __launch_bounds__(256) void k_term_fill(const TermFill* __restrict__ jobs, uint32_t n,
                                                   uint32_t* __restrict__ scalars,
                                                   const uint32_t* __restrict__ sum_pairs) {
+  wave_priority<PRIO_EARLY>();
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   TermFill j = jobs[i];
@@ -411,6 +414,7 @@ __global__ __launch_bounds__(64, 2) void k_var_mul(const VarJob* __restrict__ jo
                                                const uint8_t* __restrict__ valid,
                                                const uint32_t* __restrict__ sum_list,
                                                const uint32_t* __restrict__ sum_count) {
+  wave_priority<PRIO_EARLY>();
   extern __shared__ uint32_t dg[];  // 10 * VAR_TERMS_MAX * 64 words when n_sums > 0
   const uint32_t ns = (n_sums + 63) & ~63u;
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -518,6 +522,7 @@ __global__ __launch_bounds__(64) void k_reduce(const ReduceGroup* __restrict__ g
                                               const uint32_t* __restrict__ partials,
                                               uint32_t* __restrict__ points,
                                               uint8_t* __restrict__ pinf) {
+  wave_priority<PRIO_MID>();
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   ReduceGroup g = groups[i];
@@ -974,6 +979,7 @@ __global__ __launch_bounds__(64) void k_pv_challenge(const PvChallengeJob* __res
                                                     const uint8_t* __restrict__ pinf,
                                                     const uint32_t* __restrict__ scalars,
                                                     uint8_t* __restrict__ gate) {
+  wave_priority<PRIO_MID>();
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   PvChallengeJob j = jobs[i];
