@@ -747,6 +747,10 @@ def main():
     ap.add_argument("--fork", action="store_true",
                     help="staged batches fork their decode/product branch onto a second stream "
                          "(lower latency per batch, two hardware queues each)")
+    ap.add_argument("--agg-group", type=int, default=0, metavar="N",
+                    help="aggregation group size of the timed sets (2..8; 0: the tuning table's)")
+    ap.add_argument("--agg-super", type=int, default=0, metavar="M",
+                    help="groups per supergroup check of the timed sets (2..8; 0: the tuning table's)")
     ap.add_argument("--comb-chunk", type=int, default=0, metavar="N",
                     help="fixed-base comb terms per lane of the timed sets (0: the tuning table's)")
     ap.add_argument("--agg-straus", type=int, default=0, metavar="N",
@@ -838,9 +842,13 @@ def main():
                                      (INFLIGHT_SETS if combined else INFLIGHT)))
     single, agg = not args.fork, not args.no_aggregate
     straus = args.agg_straus if agg else 0
+    # the supergroup level in use: the flag's, else the table's when the
+    # group size is left to the table too
+    sup_used = args.agg_super or ((tuned or {}).get("agg_super", 0) if not args.agg_group else 0)
     if combined:
-        sets = [make_set(g, vreqs, pvreqs, True, aggregate=agg, straus=straus, comb_chunk=args.comb_chunk)
-                for _ in range(nset)]
+        grp = (args.agg_group or True) if agg else False
+        sets = [make_set(g, vreqs, pvreqs, True, aggregate=grp, straus=straus, comb_chunk=args.comb_chunk,
+                         supergroup=args.agg_super) for _ in range(nset)]
     else:
         sets = [[g.StagedBatch(g.BATCH_VERIFY, vreqs, single_stream=single, aggregate=agg, straus=straus),
                  g.StagedBatch(g.BATCH_PROOF_VERIFY, pvreqs, single_stream=single, aggregate=agg)]
@@ -932,9 +940,9 @@ def main():
                       "streams_per_batch": 1 if single else 2,
                       "pairing_checks": ("per-key groups of %d as random linear combinations%s, "
                                          "per-item fallback for failing groups"
-                                         % (tuned["agg_group"] if tuned else 8,
-                                            (", checked %d groups at a time first" % tuned["agg_super"])
-                                            if tuned and tuned.get("agg_super") else "")) if agg else "per item",
+                                         % (args.agg_group or (tuned["agg_group"] if tuned else 8),
+                                            (", checked %d groups at a time first" % sup_used)
+                                            if sup_used else "")) if agg else "per item",
                       "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                       "msm_terms": msm_terms},
            "roofline": roofline,
