@@ -396,11 +396,12 @@ DEV void var_sum_chain(G1J& r, const VarSum* __restrict__ jp, const uint32_t* __
 // Lanes [0, ns): the multi-term jobs (ns = n_sums rounded up to whole waves,
 // so no wave mixes them with single products); a dual job forms its second
 // sum over the same tables, a gated job returns at once when valid[gate] is
-// set.  Lanes [ns, ns + n): one
-// product per job (workspace ws + t * VAR_WS_WORDS).  Lanes after those, up
-// to n_sub more: the prime-order subgroup check of the points decoded by jobs
-// sub[] without it, run beside the products; a failure marks the item's
-// status (the products of such a point are computed but never used).
+// set, and with sum_list only the jobs sum_list[0 .. *sum_count) run.  Lanes
+// [ns, ns + n): one product per job (workspace ws + t * VAR_WS_WORDS).
+// Lanes after those, up to n_sub more: the prime-order subgroup check of the
+// points decoded by jobs sub[] without it, run beside the products; a
+// failure marks the item's status (the products of such a point are
+// computed but never used).
 __global__ __launch_bounds__(64, 2) void k_var_mul(const VarJob* __restrict__ jobs, uint32_t n,
                                                const uint32_t* __restrict__ points,
                                                const uint8_t* __restrict__ pinf,
