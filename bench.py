@@ -323,6 +323,7 @@ def serving_run(g, rate=20000.0, warm_s=5.0, steady_s=30.0, seed=b"cubbs-hip/ser
         rc, res, lat, _ = srv.run_open_loop(pool, order, arrivals)
         wall = time.perf_counter() - t0
         st = srv.stats()
+        qs = srv.queue_stats()
     finally:
         srv.close()
     assert rc == 0, "serving run failed with status %d" % rc
@@ -337,6 +338,8 @@ def serving_run(g, rate=20000.0, warm_s=5.0, steady_s=30.0, seed=b"cubbs-hip/ser
             "p50_ms": round(pct(0.50), 2), "p99_ms": round(pct(0.99), 2), "max_ms": round(steady[-1] / 1e6, 2),
             "achieved_rps": round(n / wall, 1), "verify_proofverify_rps": round(vps / steady_s, 1),
             "batches": st["batches"], "avg_batch": round(st["requests"] / max(1, st["batches"]), 1),
+            "max_batch": qs["max_batch"],
+            "avg_queue_wait_ms": round(qs["total_wait_ns"] / max(1, st["requests"]) / 1e6, 3),
             "host_plan_ms_per_batch": st["host_plan_ms_per_batch"], "batch_ms_per_batch": st["batch_ms_per_batch"],
             "host_cpus": len(os.sched_getaffinity(0)),
             "sub_batches": "HIP graph replay (per-slot graph updated per batch)" if graphs else "kernel by kernel",

@@ -793,8 +793,7 @@ static void plan_verify_grouped(Plan& p, const CuBbsBbsGpuVerifyItem* items,
     for (size_t c0 = 0; c0 < runs.size(); c0 += super_max) {
       const size_t c1 = std::min(runs.size(), c0 + super_max);
       const bool sup = c1 - c0 >= 2;
-      const uint32_t first = runs[c0].begin, total = runs[c1 - 1].end - first;
-      const uint32_t pk = pk_slot[order[first]];
+      const uint32_t pk = pk_slot[order[runs[c0].begin]];
       uint32_t sslot = 0;
       if (sup) sslot = slot_base + p.n_agg_slots++;
       // the A sides: per item (straus == 0) the products c_i A_i and

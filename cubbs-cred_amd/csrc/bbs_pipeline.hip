@@ -503,8 +503,7 @@ DEV void coop_sum(G1J& acc, bool live, uint32_t begin, uint32_t count,
       g1_add_in(acc, p);
     }
   }
-#pragma unroll
-  for (uint32_t h = RL / 2; h >= 1; h /= 2) {
+  for (uint32_t h = RL / 2; h > 0; h >>= 1) {
     store_g1j(xs + threadIdx.x * G1J_WORDS, acc);
     __syncthreads();
     if (live && q < h) {

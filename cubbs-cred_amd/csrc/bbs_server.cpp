@@ -212,8 +212,15 @@ struct Server {
     return cfg.hbm_budget_bytes > res ? (size_t)((cfg.hbm_budget_bytes - res) / cfg.slots) : 0;
   }
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_done, cv_space;
   std::deque<Req*> pending;
+  // one free slot at a time collects a batch (and waits out the coalescing
+  // window), so that slots do not take each other's partial batches
+  bool collecting = false;
+  // batcher statistics (under mu): summed queue wait of dispatched requests,
+  // the largest and the last batch, batches per size (index min(size, cap))
+  uint64_t wait_ns = 0, max_batch_seen = 0, last_batch = 0;
+  std::vector<uint64_t> size_hist;
   std::unordered_map<int64_t, std::unique_ptr<Req>> reqs;
   int64_t next_ticket = 0;
   bool stop = false;
@@ -396,8 +403,18 @@ void Server::worker(unsigned slot) {
     std::vector<Req*> take;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+      cv_work.wait(lk, [&] { return stop || (!pending.empty() && !collecting); });
       if (pending.empty() && stop) return;
+      if (pending.empty()) continue;
+      // a partial batch waits up to coalesce_us for more requests (the
+      // reference batcher's interval: a window for a partial batch, not a
+      // throttle; a full batch goes at once)
+      if (cfg.coalesce_us && pending.size() < cfg.max_batch && !stop) {
+        collecting = true;
+        const auto deadline = Clock::now() + std::chrono::microseconds(cfg.coalesce_us);
+        cv_work.wait_until(lk, deadline, [&] { return stop || pending.size() >= cfg.max_batch; });
+        collecting = false;
+      }
       size_t bytes = 0;
       const size_t share = slot_share();
       while (!pending.empty() && take.size() < cfg.max_batch) {
@@ -407,7 +424,18 @@ void Server::worker(unsigned slot) {
         take.push_back(r);
         pending.pop_front();
       }
+      if (!take.empty()) {
+        const uint64_t t = now_ns();
+        for (const Req* r : take) wait_ns += t > r->t_arrive ? t - r->t_arrive : 0;
+        max_batch_seen = std::max<uint64_t>(max_batch_seen, take.size());
+        last_batch = take.size();
+        size_hist[std::min<size_t>(take.size(), size_hist.size() - 1)]++;
+      }
     }
+    // the queue has room again, and another free slot may collect the rest
+    cv_space.notify_all();
+    cv_work.notify_all();
+    if (take.empty()) continue;
     // A batch that fails for want of memory (device or host) or on a host
     // error fails alone: its requests get code 3 and the slot goes on.  Any
     // other device error leaves the device in an unknown state: the server
@@ -465,13 +493,18 @@ int64_t submit_at(Server* s, const CubbsHipRequest* q, uint64_t t_arrive) {
   // the device
   if (r->est > s->cfg.hbm_budget_bytes) return -(int64_t)CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   r->t_arrive = t_arrive;
-  std::lock_guard<std::mutex> lk(s->mu);
+  std::unique_lock<std::mutex> lk(s->mu);
+  // a full queue blocks the submitter until a slot takes requests (the
+  // reference batcher's bounded channel)
+  if (s->cfg.queue_capacity)
+    s->cv_space.wait(lk, [&] { return s->stop || s->pending.size() < s->cfg.queue_capacity; });
   if (s->stop) return -(int64_t)CUBBS_BBS_GPU_ERR_INTERNAL;
   r->ticket = s->next_ticket++;
   Req* raw = r.get();
   s->reqs.emplace(raw->ticket, std::move(r));
   s->pending.push_back(raw);
-  s->cv_work.notify_one();
+  // every worker: a collecting slot waits for its batch to fill
+  s->cv_work.notify_all();
   return raw->ticket;
 }
 
@@ -536,7 +569,12 @@ void* cubbs_hip_server_create(const CubbsHipServerConfig* cfg, int32_t* status_o
       d.flags = cfg->flags;
     }
     d.slots = std::min<uint32_t>(d.slots, 16);
+    if (cfg) {
+      d.coalesce_us = std::min<uint32_t>(cfg->coalesce_us, 1000000);  // at most 1 s
+      d.queue_capacity = cfg->queue_capacity;
+    }
     s->cfg = d;
+    s->size_hist.assign(std::min<size_t>(d.max_batch, 1u << 16) + 1, 0);
     for (unsigned i = 0; i < d.slots; i++) s->workers.emplace_back([s, i] { s->worker(i); });
     set(CUBBS_BBS_GPU_SUCCESS);
     return s;
@@ -619,6 +657,23 @@ int32_t cubbs_hip_server_stats(void* srv, uint64_t* batches, uint64_t* requests,
   return s->fault;
 }
 
+int32_t cubbs_hip_server_queue_stats(void* srv, uint64_t* queued, uint64_t* total_wait_ns,
+                                     uint64_t* max_batch, uint64_t* last_batch, uint64_t* hist,
+                                     size_t hist_len) {
+  if (!srv || (hist && !hist_len)) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+  auto* s = static_cast<Server*>(srv);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (queued) *queued = s->pending.size();
+  if (total_wait_ns) *total_wait_ns = s->wait_ns;
+  if (max_batch) *max_batch = s->max_batch_seen;
+  if (last_batch) *last_batch = s->last_batch;
+  if (hist) {
+    std::fill(hist, hist + hist_len, 0);
+    for (size_t k = 0; k < s->size_hist.size(); k++) hist[std::min(k, hist_len - 1)] += s->size_hist[k];
+  }
+  return s->fault;
+}
+
 int32_t cubbs_hip_server_timing(void* srv, double* plan_ms_per_batch, double* batch_ms_per_batch) {
   if (!srv) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
   auto* s = static_cast<Server*>(srv);
@@ -636,6 +691,7 @@ void cubbs_hip_server_destroy(void* srv) {
     s->stop = true;
   }
   s->cv_work.notify_all();
+  s->cv_space.notify_all();
   for (auto& t : s->workers) t.join();
   delete s;
 }

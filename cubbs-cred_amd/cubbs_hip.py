@@ -82,6 +82,7 @@ EXT_SYMBOLS = [
     "cubbs_hip_server_wait",
     "cubbs_hip_server_run_open_loop",
     "cubbs_hip_server_stats",
+    "cubbs_hip_server_queue_stats",
     "cubbs_hip_server_destroy",
 ]
 
@@ -206,6 +207,7 @@ class T2PrepareItem(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
                 ("hbm_budget_bytes", ctypes.c_uint64), ("flags", ctypes.c_uint32),
+                ("coalesce_us", ctypes.c_uint32), ("queue_capacity", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32)]
 
 
@@ -264,6 +266,7 @@ def lib():
                                                         ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
                                                         ctypes.c_void_p, ctypes.c_void_p]
         _lib.cubbs_hip_server_stats.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+        _lib.cubbs_hip_server_queue_stats.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 5 + [ctypes.c_size_t]
         _lib.cubbs_hip_server_destroy.argtypes = [ctypes.c_void_p]
         _lib.cubbs_hip_server_destroy.restype = None
     return _lib
@@ -760,9 +763,13 @@ class Server:
     verify_batch / proof_verify_batch / sign_batch / proof_gen_batch and seed a
     32-byte ProofGen blinding seed or None."""
 
-    def __init__(self, slots=0, max_batch=0, hbm_budget=0, graphs=False, aggregate=False):
+    def __init__(self, slots=0, max_batch=0, hbm_budget=0, graphs=False, aggregate=False, coalesce_us=0,
+                 queue_capacity=0):
+        """coalesce_us: the window a partial batch waits to fill; queue_capacity:
+        pending requests at most (submit blocks while full); 0 = none."""
         cfg = ServerConfig(slots, max_batch, hbm_budget,
-                           (SERVER_GRAPHS if graphs else 0) | (SERVER_AGGREGATE if aggregate else 0), 0)
+                           (SERVER_GRAPHS if graphs else 0) | (SERVER_AGGREGATE if aggregate else 0),
+                           coalesce_us, queue_capacity, 0)
         st = ctypes.c_int32(0)
         self.h = lib().cubbs_hip_server_create(ctypes.byref(cfg), ctypes.byref(st))
         if not self.h:
@@ -840,6 +847,22 @@ class Server:
         lib().cubbs_hip_server_timing(self.h, ctypes.byref(pm), ctypes.byref(bm))
         d["host_plan_ms_per_batch"] = round(pm.value, 3)
         d["batch_ms_per_batch"] = round(bm.value, 3)
+        return d
+
+    def queue_stats(self, hist_len=0):
+        """Batcher statistics: requests queued now, summed queue wait (ns) of the
+        dispatched requests, the largest and the last batch, and (hist_len > 0)
+        batches per size, the last bucket holding every larger batch."""
+        v = [ctypes.c_uint64() for _ in range(4)]
+        hist = (ctypes.c_uint64 * max(1, hist_len))()
+        rc = lib().cubbs_hip_server_queue_stats(self.h, *[ctypes.byref(x) for x in v],
+                                                hist if hist_len else None, ctypes.c_size_t(hist_len))
+        if rc < 0 or rc == ERR_INVALID_INPUT:
+            raise RuntimeError("queue_stats failed %d" % rc)
+        d = dict(zip(["queued", "total_wait_ns", "max_batch", "last_batch"], [x.value for x in v]))
+        if hist_len:
+            d["hist"] = list(hist)
+        d["fault"] = rc
         return d
 
     def close(self):

@@ -237,6 +237,15 @@ typedef struct CubbsHipServerConfig {
   uint32_t max_batch;
   uint64_t hbm_budget_bytes;
   uint32_t flags;  /* CUBBS_HIP_SERVER_* below */
+  /* The request batcher of the reference's W3C service
+   * (crates/cubbs-w3c-service/src/batcher.rs:22-36, 222-300), in the loop:
+   * coalesce_us is the window a free slot waits for a partial batch (fewer
+   * than max_batch pending) to fill before it takes it, one slot collecting
+   * at a time (0: take what is pending at once); queue_capacity bounds the
+   * pending requests, and a submit blocks while the queue is full (0:
+   * unbounded). */
+  uint32_t coalesce_us;
+  uint32_t queue_capacity;
   uint32_t reserved;
 } CubbsHipServerConfig;
 /* Each sub-batch's launch sequence is captured into a HIP graph, and the
@@ -285,6 +294,15 @@ int32_t cubbs_hip_server_stats(void *server, uint64_t *batches, uint64_t *reques
 /* Averages per batch since creation: host time planning and packing the
  * sub-batches, and the whole batch from its first plan to its last result. */
 int32_t cubbs_hip_server_timing(void *server, double *plan_ms_per_batch, double *batch_ms_per_batch);
+/* Batcher statistics since creation (the reference batcher's snapshot,
+ * crates/cubbs-w3c-service/src/batcher.rs:195-220): requests queued now, the
+ * summed wait of every dispatched request from its submission to its batch's
+ * dispatch, the largest and the last batch, and hist[k] = batches of k
+ * requests for k < hist_len (larger batches counted in hist[hist_len - 1]);
+ * hist may be NULL.  Returns the server's fault code (0 = healthy). */
+int32_t cubbs_hip_server_queue_stats(void *server, uint64_t *queued, uint64_t *total_wait_ns,
+                                     uint64_t *max_batch, uint64_t *last_batch, uint64_t *hist,
+                                     size_t hist_len);
 /* Drains queued requests, joins the workers and frees the server. */
 void cubbs_hip_server_destroy(void *server);
 

@@ -192,3 +192,31 @@ def test_server_open_loop_rejects_unordered_arrivals(gpu):
         assert rc == 0 and res == [(c, v) for _, c, v, _ in alone]
     finally:
         srv.close()
+
+
+def test_server_batcher_window_queue_and_stats(gpu):
+    """The request batcher of the reference's W3C service in the loop: a burst
+    submitted one by one against a queue of capacity 8 (a submit blocks while
+    it is full) and a 3 ms coalescing window for partial batches.  Every
+    result equals the request run alone, and the batcher statistics add up:
+    the size histogram covers every batch and request, no batch exceeds
+    max_batch, nothing stays queued, and the requests waited."""
+    g = gpu
+    pool = _pool(g, b"server-batcher", n_verify=16, n_pv=8, n_sign=6, n_pg=4)
+    alone = [_alone(g, *p) for p in pool]
+    srv = g.Server(slots=2, max_batch=6, coalesce_us=3000, queue_capacity=8)
+    try:
+        tickets = [srv.submit(k, p, s) for k, p, s in pool]
+        for t, a in zip(tickets, alone):
+            code, value, out, _ = srv.wait(t)
+            assert (code, value, out) == a[1:]
+        st = srv.stats()
+        q = srv.queue_stats(hist_len=8)
+        assert st["fault"] == 0 and q["fault"] == 0 and q["queued"] == 0
+        assert sum(q["hist"]) == st["batches"]
+        assert sum(k * c for k, c in enumerate(q["hist"])) == st["requests"] == len(pool)
+        assert q["hist"][0] == 0 and q["hist"][7] == 0 and 1 <= q["max_batch"] <= 6
+        assert q["total_wait_ns"] > 0
+        assert st["batches"] < len(pool)  # the window gathered requests into batches
+    finally:
+        srv.close()
