@@ -697,6 +697,54 @@ def tune_pipeline(g, vreqs, expect_v, pvreqs, graph=True):
             "classes": classes, "table": lines}
 
 
+TUNE_STRAUS = (0, 1)
+
+
+def tune_straus(g, vreqs, expect_v, pvreqs, table, graph=True):
+    """Refines a pipeline table: for each size class, at its best shape, times
+    the Verify groups' sums as per-item products (straus 0) and as product
+    jobs of one signature (straus 1, one table per point for both sums) and
+    keeps the faster.  Returns the updated table."""
+    lines = []
+    for c in table["classes"]:
+        nv, npv, best = c["verify_items"], c["proof_verify_items"], dict(c["best"])
+        vr, pr, ev = vreqs[:nv], pvreqs[:npv], expect_v[:nv]
+        n = best["sets_in_flight"]
+        timed = {}
+        for st in TUNE_STRAUS:
+            sets = [make_set(g, vr, pr, best["verify_set"], aggregate=best["agg_group"],
+                             supergroup=best["agg_super"], comb_chunk=best["comb_chunk"], straus=st)
+                    for _ in range(n)]
+            for one in sets:
+                for b in one:
+                    b.use_graph(graph)
+                    b.launch()
+            expect = []
+            for one in sets:
+                for b in one:
+                    b.sync()
+                expect.append(check_set(one, ev))
+            run_in_flight(sets, n, expect)  # warm-up
+            ms = min(run_in_flight(sets, TUNE_STEPS, expect) for _ in range(5)) * 1e3 / TUNE_STEPS
+            timed[st] = round(ms, 4)
+            c["candidates"].append(dict(best, agg_straus=st, ms_per_step=timed[st]))
+            log("tune %d+%d straus=%d: %.3f ms/step" % (nv, npv, st, ms))
+            for one in sets:
+                for b in one:
+                    b.free()
+        st = min(timed, key=timed.get)
+        c["best"] = dict(best, agg_straus=st, ms_per_step=timed[st])
+        for kind, items in (("verify", nv), ("proof_verify", npv)):
+            lines.append("pipeline kind=%s log2_items=%d chunk=%d group=%d super=%d straus=%d" %
+                         (kind, round(math.log2(items)), best["comb_chunk"], best["agg_group"],
+                          best["agg_super"], st if kind == "verify" else 0))
+    table["table"] = lines
+    table["description"] += ("; then, at each class's best shape, the Verify groups' sums as per-item "
+                             "products (agg_straus 0) or product jobs of one signature (1)")
+    table["command"] += " && python bench.py --tune-straus cubbs-cred_amd/pipeline_tuning.json"
+    return table
+
+
 def load_pipeline_tuning(g):
     """Installs the committed pipeline table in the library and returns the
     choice for the headline's size class (None without a table)."""
@@ -757,10 +805,14 @@ def main():
     ap.add_argument("--comb-chunk", type=int, default=0, metavar="N",
                     help="fixed-base comb terms per lane of the timed sets (0: the tuning table's)")
     ap.add_argument("--agg-straus", type=int, default=0, metavar="N",
-                    help="signatures per product job of an aggregated Verify group's sums (1..4; 0: two products per item)")
+                    help="signatures per product job of an aggregated Verify group's sums (1..4; 0: the "
+                         "tuning table's, else two products per item)")
     ap.add_argument("--inflight", type=int, default=0,
                     help="staged batch sets in flight in the timed loop (1 = launch, wait, repeat; "
                          "default: the pipeline tuning table's choice, else %d)" % INFLIGHT)
+    ap.add_argument("--tune-straus", metavar="PATH", default="",
+                    help="time per-item products against one-signature product jobs at the committed "
+                         "table's best shapes and write the refined table to PATH")
     ap.add_argument("--tune-pipeline", metavar="PATH", default="",
                     help="measure the Verify/ProofVerify pipeline shapes (aggregation group, comb "
                          "terms per lane, sets in flight) per size class and write the table to PATH")
@@ -802,7 +854,8 @@ def main():
             sys.stdout.write(p.stdout.decode())
             sys.exit(p.returncode)
         out = json.loads(p.stdout.decode().strip().splitlines()[-1])
-        if not args.no_serving and not args.profile_only and not args.msm_only and not args.tune_pipeline:
+        if not args.no_serving and not args.profile_only and not args.msm_only and not args.tune_pipeline \
+                and not args.tune_straus:
             t = time.time()
             out["latency"] = serving_child(args)
             log("serving run %.1fs: %s" % (time.time() - t, out["latency"]))
@@ -832,6 +885,14 @@ def main():
     t0 = time.time()
     vreqs, expect_v, pvreqs, kps, abi_ops, _ = build_workload(g)
     log("workload built in %.1fs" % (time.time() - t0))
+    if args.tune_straus:
+        with open(PIPELINE_TUNING) as f:
+            table = tune_straus(g, vreqs, expect_v, pvreqs, json.load(f), graph=not args.no_graph)
+        with open(args.tune_straus, "w") as f:
+            json.dump(table, f, indent=1)
+            f.write("\n")
+        print(json.dumps({"pipeline_tuning": [c["best"] for c in table["classes"]]}), flush=True)
+        return
     if args.tune_pipeline:
         table = tune_pipeline(g, vreqs, expect_v, pvreqs, graph=not args.no_graph)
         with open(args.tune_pipeline, "w") as f:
@@ -845,6 +906,9 @@ def main():
                                      (INFLIGHT_SETS if combined else INFLIGHT)))
     single, agg = not args.fork, not args.no_aggregate
     straus = args.agg_straus if agg else 0
+    # the product jobs in use: the flag's, else the table's when the group
+    # size is left to the table too
+    straus_used = (straus or ((tuned or {}).get("agg_straus", 0) if not args.agg_group else 0)) if agg else 0
     # the supergroup level in use: the flag's, else the table's when the
     # group size is left to the table too
     sup_used = args.agg_super or ((tuned or {}).get("agg_super", 0) if not args.agg_group else 0)
@@ -933,7 +997,7 @@ def main():
                       "verify_items": N_VERIFY, "proof_verify_items": N_PROOFS,
                       "launch": "kernel by kernel" if args.no_graph else "HIP graph replay per staged batch",
                       "batch_sets_in_flight": nset,
-                      "agg_straus": straus,
+                      "agg_straus": straus_used,
                       "comb_chunk": args.comb_chunk or (tuned or {}).get("comb_chunk", 4),
                       "staging": ("verify set: the Verify and the ProofVerify batch staged as one batch, "
                                   "every stage launched once for both" if combined

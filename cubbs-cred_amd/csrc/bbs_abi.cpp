@@ -505,7 +505,7 @@ struct PrepareScope {
     uint32_t group = (flags >> 8) & 15u;
     const uint32_t chunk = (flags >> 12) & 127u;
     uint32_t super = (flags >> 20) & 15u;
-    const uint32_t straus = (flags >> 24) & 15u;
+    uint32_t straus = (flags >> 24) & 15u;
     const bool agg = (flags & CUBBS_HIP_BATCH_AGGREGATE) != 0;
     if ((flags & ~(uint32_t)(CUBBS_HIP_BATCH_SINGLE_STREAM | CUBBS_HIP_BATCH_AGGREGATE |
                              CUBBS_HIP_BATCH_AGG_GROUP(15) | CUBBS_HIP_BATCH_COMB_CHUNK(127) |
@@ -516,10 +516,11 @@ struct PrepareScope {
     if (agg) {
       // the tuned shape of the batch's size class where the flags leave it
       // open: groups of 8 and no supergroups without a table entry
-      uint32_t c = 0, gsz = 0, sup = 0;
-      const bool tuned = tuned_pipeline(kd, items, c, gsz, sup);
+      uint32_t c = 0, gsz = 0, sup = 0, st = 0;
+      const bool tuned = tuned_pipeline(kd, items, c, gsz, sup, &st);
       if (!group) group = tuned && gsz ? gsz : 8u;
       if (!super && !(flags & CUBBS_HIP_BATCH_AGG_GROUP(15))) super = tuned ? sup : 0u;
+      if (!straus && !(flags & CUBBS_HIP_BATCH_AGG_GROUP(15))) straus = tuned ? st : 0u;
     }
     if (chunk) tune_force().comb_chunk = chunk;
     set_thread_single_stream((flags & CUBBS_HIP_BATCH_SINGLE_STREAM) != 0);

@@ -873,10 +873,11 @@ static void plan_verify_grouped(Plan& p, const CuBbsBbsGpuVerifyItem* items,
         supers.push_back(PairJob{sslot, la, lb, pk, PAIR_NEG_G2, 0, 0});
       }
       // the items' own checks, for when their group check fails: their
-      // products (the per-item ones above, or a gated dual product of c A
-      // and (-c e) A), and one gated sum per check point over comb chunks of
-      // at least 4 terms (on this critical path a longer chunk costs less
-      // than the serial additions of the sum it saves)
+      // products (the per-item ones above, also with jobs of one item each,
+      // else a gated dual product of c A and (-c e) A), and one gated sum
+      // per check point over comb chunks of at least 4 terms (on this
+      // critical path a longer chunk costs less than the serial additions of
+      // the sum it saves)
       const uint32_t fch = std::max(CH, 4u);
       for (size_t r = c0; r < c1; r++) {
         const Run& run = runs[r];
@@ -885,7 +886,7 @@ static void plan_verify_grouped(Plan& p, const CuBbsBbsGpuVerifyItem* items,
           const uint32_t ft = own_terms(k);
           const uint32_t g = gslot[r - c0];
           uint32_t px = xb[r - c0] + (m - run.begin), py = vb[r - c0] + (m - run.begin);
-          if (straus) {
+          if (straus > 1) {
             px = p.partials(1);
             py = p.partials(1);
             p.var_dual(&pa[k], &se[k], &sc[k], 1, py, px, g);

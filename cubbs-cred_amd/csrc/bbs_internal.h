@@ -365,11 +365,12 @@ void launch_g2_mul_gen(hipStream_t s, const uint8_t* sk, uint32_t n, uint8_t* pk
 bool tuned_pip(uint32_t terms_per_msm, uint32_t& c, uint32_t& S);
 uint32_t tuned_comb_chunk(uint64_t batch_terms);
 // Whole-operation pipelines (kind 0 = Verify, 1 = ProofVerify), by items per
-// batch: comb terms per lane, aggregation group size and groups per
-// supergroup (0: none), from the pipeline lines of the table (measured by
-// running sets of such batches in flight).
+// batch: comb terms per lane, aggregation group size, groups per
+// supergroup (0: none) and signatures per product job of a Verify group
+// (0: two products per item), from the pipeline lines of the table (measured
+// by running sets of such batches in flight).
 bool tuned_pipeline(int kind, uint64_t items, uint32_t& comb_chunk, uint32_t& agg_group,
-                    uint32_t& agg_super);
+                    uint32_t& agg_super, uint32_t* agg_straus = nullptr);
 // Shapes forced on the calling thread while the tuner times a candidate (or
 // by cubbs_hip_test_force_shapes); 0 = not forced.
 struct TuneForce {

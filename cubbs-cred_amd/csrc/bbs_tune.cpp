@@ -40,6 +40,7 @@ struct Table {
   uint8_t pipe_chunk[2][CLASSES] = {};
   uint8_t pipe_group[2][CLASSES] = {};
   uint8_t pipe_super[2][CLASSES] = {};
+  uint8_t pipe_straus[2][CLASSES] = {};
 };
 Table& table() {
   static Table t;
@@ -128,7 +129,7 @@ uint32_t tuned_comb_chunk(uint64_t terms) {
 }
 
 bool tuned_pipeline(int kind, uint64_t items, uint32_t& comb_chunk, uint32_t& agg_group,
-                    uint32_t& agg_super) {
+                    uint32_t& agg_super, uint32_t* agg_straus) {
   if (kind < 0 || kind > 1) return false;
   Table& t = table();
   std::lock_guard<std::mutex> g(t.mu);
@@ -137,6 +138,7 @@ bool tuned_pipeline(int kind, uint64_t items, uint32_t& comb_chunk, uint32_t& ag
   comb_chunk = t.pipe_chunk[kind][k];
   agg_group = t.pipe_group[kind][k];
   agg_super = t.pipe_super[kind][k];
+  if (agg_straus) *agg_straus = t.pipe_straus[kind][k];
   return true;
 }
 
@@ -266,6 +268,7 @@ int32_t cubbs_hip_tune_load(const char* text) {
     std::memset(t.pipe_chunk, 0, sizeof t.pipe_chunk);
     std::memset(t.pipe_group, 0, sizeof t.pipe_group);
     std::memset(t.pipe_super, 0, sizeof t.pipe_super);
+    std::memset(t.pipe_straus, 0, sizeof t.pipe_straus);
     return CUBBS_BBS_GPU_SUCCESS;
   }
   const char* p = text;
@@ -283,16 +286,21 @@ int32_t cubbs_hip_tune_load(const char* text) {
       t.comb[k] = (uint8_t)a;
     } else if (std::sscanf(ln.c_str(), "pipeline kind=%15s log2_items=%u chunk=%u group=%u", kind, &k, &a,
                            &b) == 4) {
-      // optional: " super=M" (groups per supergroup, 0 or 2..8)
-      unsigned sup = 0;
+      // optional: " super=M" (groups per supergroup, 0 or 2..8) and
+      // " straus=S" (signatures per product job of a Verify group, 0..4)
+      unsigned sup = 0, st = 0;
       const char* sp = std::strstr(ln.c_str(), " super=");
       if (sp && std::sscanf(sp, " super=%u", &sup) != 1) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
+      const char* ss = std::strstr(ln.c_str(), " straus=");
+      if (ss && std::sscanf(ss, " straus=%u", &st) != 1) return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
       const int kd = std::strcmp(kind, "verify") == 0 ? 0 : std::strcmp(kind, "proof_verify") == 0 ? 1 : -1;
-      if (kd < 0 || k >= CLASSES || a < 1 || a > 64 || b < 2 || b > 8 || sup == 1 || sup > 8)
+      if (kd < 0 || k >= CLASSES || a < 1 || a > 64 || b < 2 || b > 8 || sup == 1 || sup > 8 ||
+          st > VAR_TERMS_MAX)
         return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
       t.pipe_chunk[kd][k] = (uint8_t)a;
       t.pipe_group[kd][k] = (uint8_t)b;
       t.pipe_super[kd][k] = (uint8_t)sup;
+      t.pipe_straus[kd][k] = (uint8_t)st;
     } else if (!ln.empty()) {
       return CUBBS_BBS_GPU_ERR_INVALID_INPUT;
     }

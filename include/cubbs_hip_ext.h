@@ -107,9 +107,12 @@ void *cubbs_hip_batch_prepare(uint32_t abi_version, int kind, const void *items,
  * Verify items): a group's sums sum c_i A_i and sum (-c_i e_i) A_i are formed
  * by product jobs over up to n of its signatures' A points each (one table
  * of multiples per point for both sums, one chain of doublings per sum for
- * all n points) instead of two products per item; the items' own products
- * then run only for groups whose check fails.  Fewer field operations per
- * item, a longer product launch.  Without it: two products per item. */
+ * all n points) instead of two products per item; with n >= 2 the items'
+ * own products then run only for groups whose check fails (with n = 1 the
+ * jobs' sums are the items' own products).  Fewer field operations per
+ * item, a longer product launch.  Without it, a batch that leaves the group
+ * size open too takes the table's choice for its size class, else two
+ * products per item. */
 #define CUBBS_HIP_BATCH_AGGREGATE 2u
 #define CUBBS_HIP_BATCH_AGG_GROUP(n) (((uint32_t)(n) & 15u) << 8)
 #define CUBBS_HIP_BATCH_COMB_CHUNK(n) (((uint32_t)(n) & 127u) << 12)
@@ -317,11 +320,13 @@ int32_t cubbs_hip_autotune(uint32_t max_log2_terms, char *report, size_t report_
 /* Installs a table in the report format (e.g. saved from an earlier run);
  * an empty string clears the table.  A malformed line fails the call.  The
  * table also takes whole-operation pipeline lines,
- * "pipeline kind=verify|proof_verify log2_items=K chunk=N group=G [super=M]",
- * measured by the caller with sets of such batches in flight: batches of
- * about 2^K items then plan their comb with N terms per lane and, when
- * aggregated without an explicit group, groups of G (2 <= G <= 8) and
- * supergroups of M groups (0: none, else 2 <= M <= 8). */
+ * "pipeline kind=verify|proof_verify log2_items=K chunk=N group=G [super=M]
+ * [straus=S]", measured by the caller with sets of such batches in flight:
+ * batches of about 2^K items then plan their comb with N terms per lane and,
+ * when aggregated without an explicit group, groups of G (2 <= G <= 8),
+ * supergroups of M groups (0: none, else 2 <= M <= 8) and, for Verify items,
+ * the groups' sums in product jobs of S signatures (as
+ * CUBBS_HIP_BATCH_AGG_STRAUS(S); 0: two products per item). */
 int32_t cubbs_hip_tune_load(const char *table);
 
 /* Launch shapes forced on the calling thread for the calls it makes next

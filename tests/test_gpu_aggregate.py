@@ -144,7 +144,7 @@ def test_aggregated_staged_batch_graph_replays(gpu, oracle):
         b.free()
 
 
-@pytest.mark.parametrize("straus", [0, 3, 4])
+@pytest.mark.parametrize("straus", [0, 1, 3, 4])
 def test_supergroup_levels_gate_groups_and_items(gpu, oracle, straus):
     """Supergroups of up to 4 groups of 8 per key, replayed from a graph:
     the per-item results equal the per-item batch's, a supergroup that holds
@@ -186,10 +186,13 @@ def test_supergroup_levels_gate_groups_and_items(gpu, oracle, straus):
                 assert w["pair_checks_run"] == 2 + 4 + 8
                 if straus:
                     # groups of 8, 8, 4, 8, 8, 4 and 9 items in jobs of at
-                    # most `straus` signatures, and the failing group's 8
-                    # items' own (gated) dual products
+                    # most `straus` signatures, and (straus > 1) the failing
+                    # group's 8 items' own gated dual products; jobs of one
+                    # item are the items' own products
                     jobs = sum(-(-n // straus) for n in (8, 8, 4, 8, 8, 4, 9))
-                    assert w["straus_short_sums"] == jobs + 8 and w["straus_short_terms"] == 49 + 8
+                    gated = 8 if straus > 1 else 0
+                    assert w["straus_short_sums"] == jobs + gated
+                    assert w["straus_short_terms"] == 49 + gated
                     assert w["glv_products"] == w["agg_products"] == 0
                 else:
                     assert w["straus_short_sums"] == 0 and w["glv_products"] == w["agg_products"] == 49

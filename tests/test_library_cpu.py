@@ -171,6 +171,11 @@ def test_pipeline_tuning_lines_parse_and_reject():
     assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=4 group=8 super=1\n") == 1
     assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=4 group=8 super=9\n") == 1
     assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=4 group=8 super=x\n") == 1
+    # signatures per product job of a Verify group: 0..4
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=2 group=8 super=0 straus=1\n") == 0
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=2 group=8 straus=4\n") == 0
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=2 group=8 straus=5\n") == 1
+    assert lib.cubbs_hip_tune_load(b"pipeline kind=verify log2_items=13 chunk=2 group=8 straus=y\n") == 1
     assert lib.cubbs_hip_tune_load(b"") == 0
 
 
