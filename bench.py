@@ -739,9 +739,13 @@ def tune_straus(g, vreqs, expect_v, pvreqs, table, graph=True):
                          (kind, round(math.log2(items)), best["comb_chunk"], best["agg_group"],
                           best["agg_super"], st if kind == "verify" else 0))
     table["table"] = lines
-    table["description"] += ("; then, at each class's best shape, the Verify groups' sums as per-item "
-                             "products (agg_straus 0) or product jobs of one signature (1)")
-    table["command"] += " && python bench.py --tune-straus cubbs-cred_amd/pipeline_tuning.json"
+    note = ("; then, at each class's best shape, the Verify groups' sums as per-item products "
+            "(agg_straus 0) or product jobs of one signature (1)")
+    if note not in table["description"]:
+        table["description"] += note
+    step = " && python bench.py --tune-straus cubbs-cred_amd/pipeline_tuning.json"
+    if step not in table["command"]:
+        table["command"] += step
     return table
 
 
