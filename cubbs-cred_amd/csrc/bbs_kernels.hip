@@ -118,8 +118,8 @@ __global__ __launch_bounds__(256) void k_hash_stage(const DomainJob* __restrict_
     domain_job(dom[i], blob, pk_bytes, gen_comp, out);
   } else if (i < n_dom + n_msg) {
     const MsgHashJob j = msg[i - n_dom];
-    Fr sc;
-    h2s_blocks(sc, reinterpret_cast<const uint32_t*>(blob + j.off), j.blocks, zpad, tail, tail_blocks);
+    const Fr sc = h2s_blocks(reinterpret_cast<const uint32_t*>(blob + j.off), j.blocks, zpad, tail,
+                             tail_blocks);
     store_fr(out + (size_t)j.out_index * FR_WORDS, sc);
   }
 }

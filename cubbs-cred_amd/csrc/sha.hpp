@@ -20,7 +20,9 @@ __constant__ static const uint32_t SHA256_K[64] = {
 
 DEV uint32_t rotr32(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
 
-DEVN void sha256_compress(uint32_t h[8], uint32_t w[16]) {
+// The 64 rounds, inlined where the state and the block are locals indexed by
+// constants only, so both stay in VGPRs.
+DEV void sha256_rounds(uint32_t h[8], uint32_t w[16]) {
   uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
 #pragma unroll
   for (int i = 0; i < 64; i++) {
@@ -51,37 +53,9 @@ DEVN void sha256_compress(uint32_t h[8], uint32_t w[16]) {
   h[0] += a; h[1] += b; h[2] += c; h[3] += d;
   h[4] += e; h[5] += f; h[6] += g; h[7] += hh;
 }
-
-// By-value form of the compression for callers that keep their state in
-// registers (no addressable arrays, so nothing is placed in scratch).
-struct ShaState {
-  uint32_t h[8];
-};
-struct ShaBlock {
-  uint32_t w[16];
-};
-DEVN ShaState sha256_block(ShaState st, ShaBlock b) {
-  sha256_compress(st.h, b.w);
-  return st;
-}
-DEV ShaState sha_iv() {
-  return ShaState{{0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a, 0x510e527f, 0x9b05688c, 0x1f83d9ab,
-                   0x5be0cd19}};
-}
-// 16 words of a block laid out as big-endian SHA words in 32-bit units (16-byte aligned).
-DEV ShaBlock sha_load_block(const uint32_t* p) {
-  ShaBlock b;
-  const uint4* q = reinterpret_cast<const uint4*>(p);
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const uint4 v = q[k];
-    b.w[4 * k] = v.x;
-    b.w[4 * k + 1] = v.y;
-    b.w[4 * k + 2] = v.z;
-    b.w[4 * k + 3] = v.w;
-  }
-  return b;
-}
+// One out-of-line copy for the byte-stream hasher below, whose state lives
+// in memory across its helpers.
+DEVN void sha256_compress(uint32_t h[8], uint32_t w[16]) { sha256_rounds(h, w); }
 
 struct Sha256 {
   uint32_t h[8];
@@ -273,23 +247,6 @@ DEVN void xmd48_from_b0(uint32_t okm[12], const uint32_t b0[8], const uint8_t* d
   for (int k = 0; k < 4; k++) okm[8 + k] = b2[k];
 }
 
-// b_i = H(in || I2OSP(i, 1) || DST') for the 32-byte `in`, from the constant
-// words of that input (`tail`, nb blocks of 16 words laid out by the planner:
-// words 8.. of block 0 and the following blocks; words 0..7 of block 0 are
-// `in`).
-DEV ShaState xmd_bi_tail(const uint32_t in[8], const uint32_t* tail, uint32_t nb) {
-  ShaState h = sha_iv();
-  for (uint32_t b = 0; b < nb; b++) {
-    ShaBlock w = sha_load_block(tail + 16 * b);
-    if (b == 0) {
-#pragma unroll
-      for (int k = 0; k < 8; k++) w.w[k] = in[k];
-    }
-    h = sha256_block(h, w);
-  }
-  return h;
-}
-
 // OS2IP(48 bytes as 12 big-endian words) mod r -> canonical scalar limbs.
 DEVN void scalar_from_okm48(Fr& out, const uint32_t okm[12]) {
   uint32_t v[16];
@@ -303,27 +260,65 @@ DEVN void scalar_from_okm48(Fr& out, const uint32_t okm[12]) {
 // hash_to_scalar of a message laid out by the planner (crates/cubbs-bbs-
 // protocol/src/utils/util.rs:104-133): the input of b_0 after its all-zero
 // Z_pad block, i.e. msg || I2OSP(48, 2) || 0x00 || DST' and the SHA-256
-// padding, as `blocks` blocks of big-endian words; `zpad` is the state after
-// the Z_pad block; `tail` the constant words of the b_1 and b_2 inputs
-// (tail_blocks blocks each).  Every lane of a wave with the same block count
-// runs the same compressions: no byte assembly, no divergent compression.
-DEVN void h2s_blocks(Fr& out, const uint32_t* words, uint32_t blocks, const uint32_t* zpad,
-                     const uint32_t* tail, uint32_t tail_blocks) {
-  ShaState h;
+// padding, as `blocks` blocks of big-endian words (16-byte aligned); `zpad`
+// is the state after the Z_pad block; `tail` the constant words of the b_1
+// and b_2 inputs (tail_blocks blocks each, b_1's first, words 0..7 of each
+// one's first block left for b_0 and b_0 ^ b_1).  All blocks run through one
+// loop with one inlined compression, so the state, the block and b_0, b_1
+// stay in registers; every lane of a wave with the same block count runs
+// the same compressions.
+DEV Fr h2s_blocks(const uint32_t* words, uint32_t blocks, const uint32_t* zpad,
+                  const uint32_t* tail, uint32_t tail_blocks) {
+  uint32_t h[8], b0[8], b1[8];
 #pragma unroll
-  for (int k = 0; k < 8; k++) h.h[k] = zpad[k];
-  for (uint32_t b = 0; b < blocks; b++) h = sha256_block(h, sha_load_block(words + 16 * b));
-  const ShaState b1 = xmd_bi_tail(h.h, tail, tail_blocks);
-  uint32_t x[8];
+  for (int k = 0; k < 8; k++) {
+    h[k] = zpad[k];
+    b0[k] = 0;
+    b1[k] = 0;
+  }
+  const uint32_t n1 = blocks + tail_blocks, n = n1 + tail_blocks;
+#pragma unroll 1
+  for (uint32_t b = 0; b < n; b++) {
+    const uint32_t* src = b < blocks ? words + 16 * (size_t)b : tail + 16 * (size_t)(b - blocks);
+    const uint4* q = reinterpret_cast<const uint4*>(src);
+    uint32_t w[16];
 #pragma unroll
-  for (int k = 0; k < 8; k++) x[k] = h.h[k] ^ b1.h[k];
-  const ShaState b2 = xmd_bi_tail(x, tail + 16 * tail_blocks, tail_blocks);
-  uint32_t okm[12];
+    for (int k = 0; k < 4; k++) {
+      const uint4 v = q[k];
+      w[4 * k] = v.x;
+      w[4 * k + 1] = v.y;
+      w[4 * k + 2] = v.z;
+      w[4 * k + 3] = v.w;
+    }
+    if (b == blocks || b == n1) {
+      // first block of b_1 = H(b_0 || 1 || DST') or b_2 = H((b_0 ^ b_1) || 2 || DST')
+      const bool second = b == n1;
 #pragma unroll
-  for (int k = 0; k < 8; k++) okm[k] = b1.h[k];
+      for (int k = 0; k < 8; k++) {
+        if (second) {
+          b1[k] = h[k];
+          w[k] = b0[k] ^ h[k];
+        } else {
+          b0[k] = h[k];
+          w[k] = h[k];
+        }
+      }
+      h[0] = 0x6a09e667; h[1] = 0xbb67ae85; h[2] = 0x3c6ef372; h[3] = 0xa54ff53a;
+      h[4] = 0x510e527f; h[5] = 0x9b05688c; h[6] = 0x1f83d9ab; h[7] = 0x5be0cd19;
+    }
+    sha256_rounds(h, w);
+  }
+  // okm = b_1 || b_2[0..16] as 12 big-endian words; OS2IP mod r
+  uint32_t v[16];
 #pragma unroll
-  for (int k = 0; k < 4; k++) okm[8 + k] = b2.h[k];
-  scalar_from_okm48(out, okm);
+  for (int k = 0; k < 8; k++) v[11 - k] = b1[k];
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[3 - k] = h[k];
+#pragma unroll
+  for (int k = 12; k < 16; k++) v[k] = 0;
+  Fr out;
+  fr_reduce_wide(out, v);
+  return out;
 }
 
 // hash_to_scalar(msg, dst) for a message in global memory (crates/cubbs-bbs-
