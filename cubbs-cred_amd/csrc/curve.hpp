@@ -469,7 +469,15 @@ DEV void g1_add_in(G1J& p, const G1J& q) {
   fp_sub(t5, t5, t4);       // S2 - S1
   if (fp_is_zero(t3)) {
     if (fp_is_zero(t5)) {
-      G1J a = p, r;  // locals only, as in g1_madd_in
+      // locals only, as in g1_madd_in; z is normalized here so that `a` is
+      // not a plain copy, which the compiler would replace by p itself and
+      // so keep the caller's accumulator in scratch memory on every addition
+      G1J a, r;
+      Fp zero;
+      fp_zero(zero);
+      a.x = p.x;
+      a.y = p.y;
+      fp_add(a.z, p.z, zero);
       pt_dbl(r, a);
       p = r;
     } else {
@@ -607,8 +615,15 @@ DEV void g1z_madd(G1Z& p, const G1A& q) {
   fp_sub(R, R, p.y);      // R = S2 - Y1
   if (fp_is_zero(P)) {
     if (fp_is_zero(R)) {
+      // a new point for the out-of-line doubling (y normalized here, so it is
+      // not a plain copy the compiler would replace by q): passing q itself
+      // would take the address of the caller's point, which then goes to
+      // scratch memory on every addition, not only on this path
+      G1A qa;
+      qa.x = q.x;
+      fp_add(qa.y, q.y, z);
       G1Z d;
-      g1z_dbl_affine(d, q);
+      g1z_dbl_affine(d, qa);
       p = d;
     } else {
       g1z_set_inf(p);
