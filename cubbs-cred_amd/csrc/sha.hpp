@@ -1,10 +1,16 @@
 // SHA-256 (FIPS 180-4) and RFC 9380 expand_message_xmd on the device.
 //
-// One lane hashes one input.  The 64-byte block lives in 16 VGPRs; bytes are
-// appended through a 32-bit shift register and committed to the block with a
-// select (no dynamically indexed register arrays, so nothing spills to
-// scratch).  Inputs are described as a sequence of byte sources (global
-// memory, constant strings, values already in registers).
+// One lane hashes one input.  Two forms:
+//  * block-aligned inputs laid out by the planner (h2s_blocks, the message
+//    hashes): one loop over all blocks with the compression inlined once, so
+//    the state and the block stay in VGPRs and nothing goes to scratch;
+//  * a byte stream (Sha256: domains, transcripts, generator seeds): bytes are
+//    appended through a 32-bit shift register and committed to the block with
+//    a select (no dynamically indexed register arrays).  Its helpers and the
+//    compression are out of line and take the state by reference, so that
+//    state lives in the lane's scratch memory; these inputs are few.
+// Inputs are described as a sequence of byte sources (global memory, constant
+// strings, values already in registers).
 #pragma once
 #include "field.hpp"
 
